@@ -1,0 +1,217 @@
+#!/usr/bin/env python3
+"""Benchmark: masked grouped reduction over a 1,000,000 x 36,601 fp16 count
+matrix (64 row groups with Dirichlet(0.5) sizes, row mask Bernoulli(0.8),
+3,000-feature mask), plus the same workload's conversions (12 log2 nnz
+buckets + rebuild, CSR -> Sliced-ELL slice 32, CSR -> CSC).
+
+One step = one masked grouped {sum, sumsq, nnz} reduction call on the
+HBM-resident CSR (feature map, label sort, partition, reduce, FP64 combine,
+mean/var).  value = algorithmic bytes of the step / step time.
+
+Prints ONE JSON line.  Single GPU only.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+PEAK_HBM_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def parse_args():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--rows", type=int, default=1_000_000)
+    ap.add_argument("--cols", type=int, default=36_601)
+    ap.add_argument("--groups", type=int, default=64)
+    ap.add_argument("--features", type=int, default=3000)
+    ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--cpu-sample-rows", type=int, default=400_000)
+    ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--skip-ops", action="store_true")
+    ap.add_argument("--reduce-only", action="store_true", help="only the reduce steps (for profiling)")
+    return ap.parse_args()
+
+
+def read_traffic(path: Path):
+    """Per-launch HBM bytes of the reduce kernel from a committed rocprofv3
+    PMC summary (line: 'hbm_bytes_per_launch: N'), else None."""
+    if not path.exists():
+        return None
+    for line in path.read_text().splitlines():
+        if line.startswith("hbm_bytes_per_launch:"):
+            try:
+                return float(line.split(":", 1)[1].strip())
+            except ValueError:
+                return None
+    return None
+
+
+def main() -> int:
+    args = parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(json.dumps({"error": "single-GPU build: bench.py measures one MI355X; --gpus must be 1",
+                          "n_gpus_requested": args.gpus}))
+        return 2
+
+    import numpy as np
+    import torch
+
+    from cellshard_amd import abi as A
+    from cellshard_amd.device import to_dev_u32, to_dev_u8
+    from cellshard_amd.ops import (ConvertWorkspace, LabelReduce, TransposeScratch, build_major_nnz_bucket_plan,
+                                   compressed_transpose, log2_bucket_offsets, rebuild_compressed_major_order,
+                                   sliced_ell_from_compressed, synth_csr)
+
+    torch.cuda.set_device(0)
+    dev = torch.device("cuda", 0)
+    R, Cc, G, FK = args.rows, args.cols, args.groups, args.features
+    props = torch.cuda.get_device_properties(0)
+    log(f"device {props.name} CUs={props.multi_processor_count}")
+
+    # ---------------- synthetic inputs (config 2 generator) ----------------
+    t0 = time.time()
+    ws = ConvertWorkspace()
+    src = synth_csr(R, Cc, median=2000.0, sigma=0.6, seed=args.seed, value_type=A.VALUE_F16, ws=ws)
+    torch.cuda.synchronize()
+    gen_s = time.time() - t0
+    nnz = src.nnz
+    log(f"generated CSR {R}x{Cc} nnz={nnz} in {gen_s:.1f}s")
+    rng = np.random.default_rng(args.seed)
+    props_g = rng.dirichlet(np.full(G, 0.5))
+    labels = rng.choice(G, size=R, p=props_g).astype(np.uint32)
+    row_keep = (rng.random(R) < 0.8).astype(np.uint8)
+    feature_keep = np.zeros(Cc, np.uint8)
+    feature_keep[rng.choice(Cc, FK, replace=False)] = 1
+    d_labels, d_rk, d_fk = to_dev_u32(labels, dev), to_dev_u8(row_keep, dev), to_dev_u8(feature_keep, dev)
+    ptr_h = src.major_ptr.cpu().numpy().view(np.uint32).astype(np.int64)
+    row_nnz = np.diff(ptr_h)
+    kept = (row_keep == 1) & (labels < G)
+    kept_nnz = int(row_nnz[kept].sum())
+    kept_rows = int(kept.sum())
+
+    # algorithmic bytes (CSR input): kept entries (2 B value + 4 B index),
+    # row pointers, per-row keep byte + label, feature mask, outputs
+    out_bytes = G * FK * (8 + 8 + 4 + 8 + 8) + G * 4
+    b_alg_step = kept_nnz * 6 + (R + 1) * 4 + R * (1 + 4) + Cc * 4 + out_bytes
+    # dominant kernel: kept entries + per kept row (row id + two pointers)
+    b_alg_kernel = kept_nnz * 6 + kept_rows * 12
+
+    # ---------------- reduce steps ----------------
+    lr = LabelReduce(stream=torch.cuda.current_stream().cuda_stream)
+    for _ in range(args.warmup):
+        lr.run(src, d_labels, G, d_rk, d_fk)
+    torch.cuda.synchronize()
+    kernel_ms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        lr.run(src, d_labels, G, d_rk, d_fk)
+        kernel_ms.append(lr.last_kernel_ms())
+    torch.cuda.synchronize()
+    step_ms = (time.perf_counter() - t0) * 1e3 / max(args.steps, 1)
+    res = lr.run(src, d_labels, G, d_rk, d_fk)
+    got = LabelReduce.fetch(res)
+    k_ms = float(np.mean(kernel_ms)) if kernel_ms else float("nan")
+    value = b_alg_step / (step_ms * 1e-3) / 1e9
+    achieved = b_alg_kernel / (k_ms * 1e-3) / 1e9
+    log(f"reduce: step {step_ms:.3f} ms ({value:.0f} GB/s), kernel {k_ms:.3f} ms ({achieved:.0f} GB/s)")
+
+    # ---------------- conversions of the same workload ----------------
+    ops = {}
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        s = time.perf_counter()
+        out = fn()
+        torch.cuda.synchronize()
+        return out, (time.perf_counter() - s) * 1e3
+
+    if not args.skip_ops and not args.reduce_only:
+        csr_bytes = (R + 1) * 4 + nnz * 6
+        plan, ms = timed(lambda: build_major_nnz_bucket_plan(src, 12))
+        l2, ms2 = timed(lambda: log2_bucket_offsets(plan["major_nnz_sorted"], 12, 5))
+        ops["bucket_plan_12_log2"] = {"ms": round(ms + ms2, 3), "gb_s": round(((R + 1) * 4 + 2 * R * 4) / ((ms + ms2) * 1e-3) / 1e9, 1),
+                                      "bucket_sizes": np.diff(l2.cpu().numpy().view(np.uint32).astype(np.int64)).tolist()}
+        (reb, inv), ms = timed(lambda: rebuild_compressed_major_order(src, plan["major_order"]))
+        ops["bucket_rebuild"] = {"ms": round(ms, 3), "gb_s": round(2 * csr_bytes / (ms * 1e-3) / 1e9, 1)}
+        del inv
+        sell, ms = timed(lambda: sliced_ell_from_compressed(reb, 32, ws))
+        ops["csr_to_sell_s32_sorted"] = {"ms": round(ms, 3), "gb_s": round((csr_bytes + (2 * sell.slice_count + 1) * 4 +
+                                                                            sell.total_slots * 6) / (ms * 1e-3) / 1e9, 1),
+                                         "slots_per_nnz": round(sell.total_slots / nnz, 3)}
+        del sell, reb
+        torch.cuda.empty_cache()
+        scratch = TransposeScratch()
+        compressed_transpose(src, scratch)  # warm
+        csc, ms = timed(lambda: compressed_transpose(src, scratch))
+        ops["csr_to_csc"] = {"ms": round(ms, 3), "gb_s": round((csr_bytes + (Cc + 1) * 4 + nnz * 6) / (ms * 1e-3) / 1e9, 1)}
+        del csc, scratch
+        torch.cuda.empty_cache()
+        log(f"ops: {json.dumps(ops)}")
+
+    # ---------------- CPU baseline (oracle, bounded sample) ----------------
+    cpu = None
+    if not args.skip_cpu and not args.reduce_only:
+        from oracle import oracle as O
+
+        S = min(args.cpu_sample_rows, R)
+        end = int(ptr_h[S])
+        p = ptr_h[: S + 1].astype(np.uint32)
+        idx = src.minor_idx[:end].cpu().numpy().view(np.uint32)
+        val = src.val[:end].cpu().numpy()
+        t0 = time.perf_counter()
+        O.label_reduce_csr(S, Cc, p, idx, val, A.VALUE_F16, labels[:S], row_keep[:S], feature_keep, G)
+        cpu_s = time.perf_counter() - t0
+        k = kept[:S]
+        b_cpu = int(row_nnz[:S][k].sum()) * 6 + (S + 1) * 4 + S * 5 + Cc * 4 + out_bytes
+        cpu = {"value": round(b_cpu / cpu_s / 1e9, 3), "unit": "GB/s", "cores": 1, "kind": "port",
+               "sample": f"first {S} of {R} rows ({end} stored entries), same masks/labels; {cpu_s:.2f} s single-threaded C oracle"}
+        log(f"cpu baseline: {cpu}")
+
+    traffic = read_traffic(ROOT / "profiles" / "label_reduce_csr_1Mx36601_pmc.txt")
+    line = {
+        "metric": "masked group-reduce HBM GB/s (% of peak)",
+        "value": round(value, 1),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(step_ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (GPU generator: LogNormal row nnz, Zipf(1.1) columns over a seeded permutation, zero-truncated NB2 counts)",
+        "config": {"workload": "masked_grouped_sum_sumsq_nnz_csr_1000000x36601_fp16_64groups", "rows": R, "cols": Cc,
+                   "nnz": nnz, "value_storage": "f16", "groups": G, "group_sizes": "Dirichlet(0.5)", "row_keep_p": 0.8,
+                   "kept_rows": kept_rows, "kept_nnz": kept_nnz, "kept_features": int(got["kept_features"]),
+                   "algorithmic_bytes_per_step": b_alg_step, "pct_of_peak": round(100.0 * value / PEAK_HBM_GBS, 2)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
+                     "kernel": "label_reduce_rows_kernel", "kernel_ms": round(k_ms, 4), "algorithmic_bytes": b_alg_kernel},
+        "cpu_baseline": cpu,
+        "ops": ops,
+        "generate_s": round(gen_s, 2),
+    }
+    print(json.dumps(line), flush=True)
+    lr.close()
+    ws.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,134 @@
+"""Loader for libcellshard_amd.so and the C-ABI prototypes.
+
+The library is required: there is no CPU fallback for any entry point.  A
+missing or unloadable library raises LibraryMissing at first use.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+from . import abi as A
+
+LIB_PATH = Path(__file__).resolve().parent / "libcellshard_amd.so"
+
+P = C.POINTER
+vp = C.c_void_p
+u32 = C.c_uint32
+u64 = C.c_uint64
+i32 = C.c_int
+f32 = C.c_float
+sz = C.c_size_t
+
+# name -> (restype, [argtypes])
+PROTOTYPES = {
+    "csamd_last_error": (C.c_char_p, []),
+    "csamd_abi_version": (i32, []),
+    "csamd_device_count": (i32, []),
+    # mask_groups
+    "csamd_group_reduce_init": (None, [P(A.sparse_group_reduce_workspace)]),
+    "csamd_group_reduce_clear": (None, [P(A.sparse_group_reduce_workspace)]),
+    "csamd_group_reduce_setup": (i32, [P(A.sparse_group_reduce_workspace), i32, vp]),
+    "csamd_group_reduce_reserve": (i32, [P(A.sparse_group_reduce_workspace), u32, u32, u32, u32]),
+    "csamd_upload_feature_group_masks": (i32, [P(A.sparse_group_reduce_workspace), u32, vp]),
+    "csamd_compute_sparse_group_reduce_blocked_ell": (i32, [P(A.blocked_ell_view), P(A.sparse_group_reduce_workspace),
+        P(A.group_mask_config_view), P(A.row_feature_mask_view), P(A.sparse_group_filter_params), P(A.sparse_group_reduce_result)]),
+    "csamd_compute_sparse_group_reduce_sliced_ell": (i32, [P(A.sliced_ell_view), P(A.sparse_group_reduce_workspace),
+        P(A.group_mask_config_view), P(A.row_feature_mask_view), P(A.sparse_group_filter_params), P(A.sparse_group_reduce_result)]),
+    "csamd_compute_sparse_group_reduce_compressed": (i32, [P(A.compressed_view), P(A.sparse_group_reduce_workspace),
+        P(A.group_mask_config_view), P(A.row_feature_mask_view), P(A.sparse_group_filter_params), P(A.sparse_group_reduce_result)]),
+    "csamd_compute_sparse_group_reduce_compressed_typed": (i32, [P(A.compressed_view), u32, P(A.sparse_group_reduce_workspace),
+        P(A.group_mask_config_view), P(A.row_feature_mask_view), P(A.sparse_group_filter_params), P(A.sparse_group_reduce_result)]),
+    "csamd_compute_sparse_group_reduce_blocked_ell_typed": (i32, [P(A.blocked_ell_view), u32, P(A.sparse_group_reduce_workspace),
+        P(A.group_mask_config_view), P(A.row_feature_mask_view), P(A.sparse_group_filter_params), P(A.sparse_group_reduce_result)]),
+    "csamd_compute_sparse_group_reduce_sliced_ell_typed": (i32, [P(A.sliced_ell_view), u32, P(A.sparse_group_reduce_workspace),
+        P(A.group_mask_config_view), P(A.row_feature_mask_view), P(A.sparse_group_filter_params), P(A.sparse_group_reduce_result)]),
+    "csamd_fleet_init": (None, [P(A.sparse_group_reduce_fleet_workspace)]),
+    "csamd_fleet_clear": (None, [P(A.sparse_group_reduce_fleet_workspace)]),
+    "csamd_setup_fleet": (i32, [P(A.sparse_group_reduce_fleet_workspace), P(A.sparse_group_reduce_fleet_config)]),
+    "csamd_compute_sparse_group_reduce_fleet_blocked_ell": (i32, [P(A.blocked_ell_view), P(A.sparse_group_reduce_fleet_workspace),
+        P(A.group_mask_config_view), P(A.row_feature_mask_view), P(A.sparse_group_filter_params), P(A.sparse_group_reduce_fleet_result)]),
+    "csamd_compute_sparse_group_reduce_fleet_sliced_ell": (i32, [P(A.sliced_ell_view), P(A.sparse_group_reduce_fleet_workspace),
+        P(A.group_mask_config_view), P(A.row_feature_mask_view), P(A.sparse_group_filter_params), P(A.sparse_group_reduce_fleet_result)]),
+    "csamd_predict_masked_sparse_reoptimization": (i32, [P(A.auto_reoptimize_prediction_input), P(A.auto_reoptimize_prediction)]),
+    "csamd_reoptimize_init": (None, [P(A.masked_sparse_reoptimize_workspace)]),
+    "csamd_reoptimize_clear": (None, [P(A.masked_sparse_reoptimize_workspace)]),
+    "csamd_reoptimize_setup": (i32, [P(A.masked_sparse_reoptimize_workspace), i32, vp]),
+    "csamd_manual_reoptimize_masked_sparse": (i32, [P(A.blocked_ell_view), P(A.row_feature_mask_view),
+        P(A.masked_sparse_reoptimize_config), P(A.masked_sparse_reoptimize_workspace), P(A.masked_sparse_reoptimize_result)]),
+    # label reduce
+    "csamd_label_reduce_init": (None, [P(A.label_reduce_workspace)]),
+    "csamd_label_reduce_clear": (None, [P(A.label_reduce_workspace)]),
+    "csamd_label_reduce_setup": (i32, [P(A.label_reduce_workspace), i32, vp]),
+    "csamd_label_reduce_compressed": (i32, [P(A.compressed_view), P(A.label_reduce_query), P(A.label_reduce_workspace), P(A.label_reduce_result)]),
+    "csamd_label_reduce_sliced_ell": (i32, [P(A.sliced_ell_view), P(A.label_reduce_query), P(A.label_reduce_workspace), P(A.label_reduce_result)]),
+    "csamd_label_reduce_blocked_ell": (i32, [P(A.blocked_ell_view), P(A.label_reduce_query), P(A.label_reduce_workspace), P(A.label_reduce_result)]),
+    "csamd_label_reduce_last_kernel_ms": (f32, [P(A.label_reduce_workspace)]),
+    # convert
+    "csamd_convert_init": (None, [P(A.convert_workspace)]),
+    "csamd_convert_clear": (None, [P(A.convert_workspace)]),
+    "csamd_convert_setup": (i32, [P(A.convert_workspace), i32, vp]),
+    "csamd_blocked_ell_shape": (i32, [P(A.compressed_view), u32, P(A.convert_workspace), P(u32), P(u64), P(C.c_double)]),
+    "csamd_choose_blocked_ell_block_size": (i32, [P(A.compressed_view), P(u32), u32, u32, P(A.convert_workspace), P(A.blocked_ell_tune_result)]),
+    "csamd_blocked_ell_from_compressed": (i32, [P(A.compressed_view), u32, P(A.blocked_ell_view), P(A.convert_workspace)]),
+    "csamd_sliced_ell_layout": (i32, [P(A.compressed_view), u32, vp, vp, vp, P(A.convert_workspace), P(u32), P(u64)]),
+    "csamd_sliced_ell_layout_from_blocked_ell": (i32, [P(A.blocked_ell_view), u32, u32, vp, vp, vp, P(A.convert_workspace), P(u32), P(u64)]),
+    "csamd_choose_sliced_ell_slice_rows": (i32, [P(A.compressed_view), P(u32), u32, P(A.convert_workspace), P(A.sliced_ell_tune_result)]),
+    "csamd_sliced_ell_from_compressed": (i32, [P(A.compressed_view), u32, P(A.sliced_ell_view), P(A.convert_workspace)]),
+    "csamd_sliced_ell_from_blocked_ell": (i32, [P(A.blocked_ell_view), u32, P(A.sliced_ell_view), P(A.convert_workspace)]),
+    "csamd_compressed_transpose_scratch_bytes": (sz, [u32, u32, u32]),
+    "csamd_build_compressed_transpose_raw": (i32, [u32, u32, u32, vp, vp, vp, vp, vp, vp, vp, vp, sz, vp]),
+    "csamd_build_compressed_transpose_raw_typed": (i32, [u32, u32, u32, vp, vp, vp, u32, vp, vp, vp, vp, sz, vp]),
+    # bucket
+    "csamd_clamp_bucket_count": (u32, [u32, u32]),
+    "csamd_major_nnz_bucket_sort_scratch_bytes": (i32, [u32, P(sz)]),
+    "csamd_major_nnz_bucket_scan_scratch_bytes": (i32, [u32, P(sz)]),
+    "csamd_build_major_nnz_bucket_plan_raw": (i32, [vp, u32, vp, vp, vp, vp, vp, u32, vp, sz, vp]),
+    "csamd_fill_log2_bucket_offsets_raw": (i32, [vp, u32, u32, u32, vp, vp]),
+    "csamd_rebuild_compressed_major_order_raw": (i32, [vp, vp, vp, u32, u32, vp, vp, vp, vp, vp, vp, sz, vp]),
+    "csamd_blocked_ell_bucket_plan_raw": (i32, [vp, u32, u32, vp, vp, vp, P(A.convert_workspace)]),
+    # synth / scan
+    "csamd_synth_row_nnz": (i32, [P(A.synth_params), vp, vp]),
+    "csamd_exclusive_scan_u32": (i32, [vp, u32, vp, P(A.convert_workspace), P(u64)]),
+    "csamd_synth_fill": (i32, [P(A.synth_params), vp, vp, vp, vp]),
+    "csamd_synth_log1p_normalize": (i32, [u32, u32, vp, vp, vp, f32, u32, vp, vp, P(A.convert_workspace)]),
+}
+
+
+class LibraryMissing(RuntimeError):
+    pass
+
+
+class CellShardError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Load the HIP library (raises LibraryMissing if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise LibraryMissing(
+            f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`"
+        )
+    lib = C.CDLL(str(LIB_PATH))
+    for name, (res, args) in PROTOTYPES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def last_error() -> str:
+    msg = load().csamd_last_error()
+    return msg.decode() if msg else ""
+
+
+def check(rc: int, what: str) -> None:
+    if rc != 1:
+        raise CellShardError(f"{what} failed: {last_error()}")

@@ -1,0 +1,198 @@
+"""ctypes mirror of include/cellshard_amd/cellshard_amd.h.
+
+Every struct is declared with the C field order; structs the C header marks
+aligned(16) get tail padding so sizeof matches (tests/test_abi.py checks each
+size and field offset against the C compiler's view).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+VALUE_F16 = 3
+VALUE_F32 = 4
+VALUE_F64 = 5
+VALUE_BF16 = 6
+VALUE_FP8_E4M3 = 7
+VALUE_FP8_E5M2 = 8
+
+COMPRESSED_BY_ROW = 0
+COMPRESSED_BY_COL = 1
+INVALID_COL = 0xFFFFFFFF
+MAX_MASK_GROUPS = 32
+MAX_LABEL_GROUPS = 4096
+
+LAYOUT_UNKNOWN = 0
+LAYOUT_BLOCKED_ELL = 1
+LAYOUT_SLICED_ELL = 2
+LAYOUT_COMPRESSED = 3
+
+VALUE_BYTES = {VALUE_F16: 2, VALUE_BF16: 2, VALUE_F32: 4, VALUE_F64: 8, VALUE_FP8_E4M3: 1, VALUE_FP8_E5M2: 1}
+
+c_u32 = C.c_uint32
+c_u32_p = C.POINTER(C.c_uint32)
+c_u8_p = C.POINTER(C.c_uint8)
+c_void_p = C.c_void_p
+c_size_t = C.c_size_t
+
+
+def _aligned16(name: str, fields):
+    """Build a ctypes Structure whose size is rounded up to 16 bytes."""
+    base = type(name + "_unpadded", (C.Structure,), {"_fields_": fields})
+    size = C.sizeof(base)
+    pad = (-size) % 16
+    if pad:
+        fields = list(fields) + [("_tail_pad", C.c_uint8 * pad)]
+    return type(name, (C.Structure,), {"_fields_": fields})
+
+
+compressed_view = _aligned16("compressed_view", [
+    ("rows", c_u32), ("cols", c_u32), ("nnz", c_u32), ("axis", c_u32),
+    ("majorPtr", c_void_p), ("minorIdx", c_void_p), ("val", c_void_p),
+])
+
+blocked_ell_view = _aligned16("blocked_ell_view", [
+    ("rows", c_u32), ("cols", c_u32), ("nnz", c_u32), ("block_size", c_u32), ("ell_cols", c_u32),
+    ("blockColIdx", c_void_p), ("val", c_void_p),
+])
+
+sliced_ell_view = _aligned16("sliced_ell_view", [
+    ("rows", c_u32), ("cols", c_u32), ("nnz", c_u32), ("slice_count", c_u32), ("slice_rows", c_u32),
+    ("slice_row_offsets", c_void_p), ("slice_widths", c_void_p), ("slice_slot_offsets", c_void_p),
+    ("col_idx", c_void_p), ("val", c_void_p),
+])
+
+group_mask_config_view = _aligned16("group_mask_config_view", [
+    ("group_count", c_u32), ("group_names", c_void_p), ("feature_group_masks", c_void_p),
+])
+
+row_feature_mask_view = _aligned16("row_feature_mask_view", [
+    ("row_keep", c_void_p), ("feature_keep", c_void_p), ("row_remap", c_void_p), ("feature_remap", c_void_p),
+])
+
+sparse_group_filter_params = _aligned16("sparse_group_filter_params", [
+    ("min_total", C.c_float), ("min_detected_features", c_u32), ("max_group_fraction", C.c_float),
+    ("fraction_group_index", c_u32),
+])
+
+sparse_group_reduce_result = _aligned16("sparse_group_reduce_result", [
+    ("rows", c_u32), ("group_count", c_u32),
+    ("row_totals", c_void_p), ("detected_features", c_void_p), ("max_values", c_void_p), ("row_keep", c_void_p),
+    ("group_counts", c_void_p), ("group_percentages", c_void_p),
+])
+
+sparse_group_reduce_workspace = _aligned16("sparse_group_reduce_workspace", [
+    ("device", C.c_int), ("stream", c_void_p), ("owns_stream", C.c_int),
+    ("rows_capacity", c_u32), ("cols_capacity", c_u32), ("group_capacity", c_u32), ("values_capacity", c_u32),
+    ("row_block", c_void_p), ("feature_block", c_void_p), ("scratch_block", c_void_p), ("scratch_bytes", c_size_t),
+    ("row_totals", c_void_p), ("detected_features", c_void_p), ("max_values", c_void_p), ("row_keep", c_void_p),
+    ("group_counts", c_void_p), ("group_percentages", c_void_p), ("feature_group_masks", c_void_p),
+])
+
+sparse_group_reduce_fleet_workspace = _aligned16("sparse_group_reduce_fleet_workspace", [
+    ("slot_count", c_u32), ("device_ids", c_void_p), ("slots", c_void_p),
+    ("devices", C.POINTER(sparse_group_reduce_workspace)), ("results", C.POINTER(sparse_group_reduce_result)),
+])
+
+sparse_group_reduce_fleet_result = _aligned16("sparse_group_reduce_fleet_result", [
+    ("slot_count", c_u32), ("leader_index", c_u32), ("slot_results", C.POINTER(sparse_group_reduce_result)),
+])
+
+sparse_group_reduce_fleet_config = _aligned16("sparse_group_reduce_fleet_config", [
+    ("device_ids", c_void_p), ("device_count", c_u32), ("enable_peer_access", c_u32), ("stream_flags", c_u32),
+    ("ranked_nccl", c_void_p),
+])
+
+masked_sparse_reoptimize_config = _aligned16("masked_sparse_reoptimize_config", [
+    ("output_rows", c_u32), ("output_cols", c_u32), ("requested_bucket_count", c_u32), ("output_block_size", c_u32),
+    ("requested_layout", c_u32),
+])
+
+masked_sparse_reoptimize_metadata = _aligned16("masked_sparse_reoptimize_metadata", [
+    ("kept_rows", c_u32), ("kept_features", c_u32), ("live_nnz", c_u32),
+    ("bytes_before", C.c_uint64), ("bytes_after", C.c_uint64), ("output_layout", c_u32),
+])
+
+masked_sparse_reoptimize_result = _aligned16("masked_sparse_reoptimize_result", [
+    ("compressed", compressed_view), ("blocked_ell", blocked_ell_view), ("metadata", masked_sparse_reoptimize_metadata),
+])
+
+masked_sparse_reoptimize_workspace = _aligned16("masked_sparse_reoptimize_workspace", [
+    ("device", C.c_int), ("stream", c_void_p), ("owns_stream", C.c_int), ("private_workspace", c_void_p),
+])
+
+auto_reoptimize_prediction_input = _aligned16("auto_reoptimize_prediction_input", [
+    ("density_change", C.c_float), ("row_drop_ratio", C.c_float), ("feature_drop_ratio", C.c_float),
+    ("layout_padding_change", C.c_float), ("estimated_rebuild_cost_ms", C.c_float), ("expected_reuse_count", c_u32),
+])
+
+auto_reoptimize_prediction = _aligned16("auto_reoptimize_prediction", [
+    ("should_reoptimize", c_u32), ("estimated_net_gain_ms", C.c_float),
+])
+
+label_reduce_query = _aligned16("label_reduce_query", [
+    ("group_count", c_u32), ("value_type", c_u32), ("row_labels", c_void_p), ("row_keep", c_void_p),
+    ("feature_keep", c_void_p), ("feature_scale", c_void_p), ("compute_moments", c_u32), ("reserved", c_u32),
+])
+
+label_reduce_result = _aligned16("label_reduce_result", [
+    ("group_count", c_u32), ("kept_features", c_u32), ("kept_rows", c_u32), ("reserved", c_u32),
+    ("sum", c_void_p), ("sumsq", c_void_p), ("nnz", c_void_p), ("group_rows", c_void_p),
+    ("mean", c_void_p), ("var", c_void_p), ("kept_feature_ids", c_void_p),
+])
+
+label_reduce_workspace = _aligned16("label_reduce_workspace", [
+    ("device", C.c_int), ("stream", c_void_p), ("owns_stream", C.c_int), ("private_workspace", c_void_p),
+])
+
+convert_workspace = _aligned16("convert_workspace", [
+    ("device", C.c_int), ("stream", c_void_p), ("owns_stream", C.c_int), ("scratch", c_void_p), ("scratch_bytes", c_size_t),
+])
+
+blocked_ell_tune_result = _aligned16("blocked_ell_tune_result", [
+    ("block_size", c_u32), ("fill_ratio", C.c_double), ("padded_bytes", c_size_t),
+])
+
+sliced_ell_tune_result = _aligned16("sliced_ell_tune_result", [
+    ("slice_rows", c_u32), ("slice_count", c_u32), ("padded_slots", c_size_t), ("padded_bytes", c_size_t),
+    ("storage_ratio", C.c_double), ("slice_penalty", C.c_double), ("score", C.c_double),
+])
+
+major_nnz_bucket_plan_view = _aligned16("major_nnz_bucket_plan_view", [
+    ("major_dim", c_u32), ("bucket_count", c_u32), ("major_order", c_void_p), ("major_nnz_sorted", c_void_p),
+    ("bucket_offsets", c_void_p), ("inverse_major_order", c_void_p),
+])
+
+synth_params = _aligned16("synth_params", [
+    ("rows", c_u32), ("cols", c_u32), ("value_type", c_u32), ("seed", c_u32),
+    ("row_nnz_median", C.c_float), ("row_nnz_sigma", C.c_float), ("nb_alpha", C.c_float), ("reserved", c_u32),
+    ("d_col_inv_weight", c_void_p), ("d_col_mean", c_void_p),
+])
+
+# Structs whose layout tests/test_abi.py compares against the C header.
+ALL_STRUCTS = {
+    "csamd_compressed_view": compressed_view,
+    "csamd_blocked_ell_view": blocked_ell_view,
+    "csamd_sliced_ell_view": sliced_ell_view,
+    "csamd_group_mask_config_view": group_mask_config_view,
+    "csamd_row_feature_mask_view": row_feature_mask_view,
+    "csamd_sparse_group_filter_params": sparse_group_filter_params,
+    "csamd_sparse_group_reduce_result": sparse_group_reduce_result,
+    "csamd_sparse_group_reduce_workspace": sparse_group_reduce_workspace,
+    "csamd_sparse_group_reduce_fleet_workspace": sparse_group_reduce_fleet_workspace,
+    "csamd_sparse_group_reduce_fleet_result": sparse_group_reduce_fleet_result,
+    "csamd_sparse_group_reduce_fleet_config": sparse_group_reduce_fleet_config,
+    "csamd_masked_sparse_reoptimize_config": masked_sparse_reoptimize_config,
+    "csamd_masked_sparse_reoptimize_metadata": masked_sparse_reoptimize_metadata,
+    "csamd_masked_sparse_reoptimize_result": masked_sparse_reoptimize_result,
+    "csamd_masked_sparse_reoptimize_workspace": masked_sparse_reoptimize_workspace,
+    "csamd_auto_reoptimize_prediction_input": auto_reoptimize_prediction_input,
+    "csamd_auto_reoptimize_prediction": auto_reoptimize_prediction,
+    "csamd_label_reduce_query": label_reduce_query,
+    "csamd_label_reduce_result": label_reduce_result,
+    "csamd_label_reduce_workspace": label_reduce_workspace,
+    "csamd_convert_workspace": convert_workspace,
+    "csamd_blocked_ell_tune_result": blocked_ell_tune_result,
+    "csamd_sliced_ell_tune_result": sliced_ell_tune_result,
+    "csamd_major_nnz_bucket_plan_view": major_nnz_bucket_plan_view,
+    "csamd_synth_params": synth_params,
+}

@@ -1,0 +1,703 @@
+// CSR -> Blocked-ELL (shape, tuner, emit), CSR/CSC -> Sliced-ELL (layout,
+// tuner, emit), Blocked-ELL -> Sliced-ELL.
+//
+// Layout rules are the reference's (src/convert/blocked_ell_from_compressed.cuh,
+// src/convert/sliced_ell_from_compressed.cuh, src/convert/sliced_ell_from_blocked_ell.cuh):
+//   Blocked-ELL: per row block, the unique column blocks touched by any stored
+//   entry, ascending, fill the first slots; W = max over row blocks; unused
+//   slots hold 0xffffffff; value (r, c) sits at r*ell_cols + slot*B + c%B;
+//   nnz = count of nonzero stored values (a duplicate (r, c) keeps the last).
+//   Sliced-ELL: uniform slices of S = clamp(S, 1, rows) rows; width = max row
+//   nnz in the slice; row slots follow CSR order; padding col 0xffffffff, 0.
+//
+// gfx950 mapping: one workgroup per row block with an LDS bitset over column
+// blocks (popcount ranks give the ascending slot of every block column);
+// one wave64 per row for Sliced-ELL emits (coalesced copy + padding).
+#include "common.hh"
+
+namespace csamd {
+namespace {
+
+constexpr unsigned int CV_THREADS = 256u;
+constexpr unsigned int BELL_MAX_BITSET_WORDS = 8192u;  // 262,144 column blocks (32 KB LDS)
+
+__global__ void __launch_bounds__(CV_THREADS) bell_shape_kernel(csamd_compressed_view src, unsigned int bs, unsigned int row_blocks,
+                                                                unsigned int nwords, unsigned int *__restrict__ counts,
+                                                                unsigned int *__restrict__ max_width,
+                                                                unsigned long long *__restrict__ total) {
+    extern __shared__ unsigned int bits[];
+    __shared__ unsigned int red[CV_THREADS / 64u];
+    for (unsigned int rb = blockIdx.x; rb < row_blocks; rb += gridDim.x) {
+        for (unsigned int w = threadIdx.x; w < nwords; w += CV_THREADS) bits[w] = 0u;
+        __syncthreads();
+        const unsigned int r0 = rb * bs;
+        const unsigned int r1 = r0 + bs < src.rows ? r0 + bs : src.rows;
+        const unsigned int b = src.majorPtr[r0], e = src.majorPtr[r1];
+        for (unsigned int i = b + threadIdx.x; i < e; i += CV_THREADS) {
+            const unsigned int c = src.minorIdx[i];
+            if (c < src.cols) {
+                const unsigned int bc = c / bs;
+                atomicOr(&bits[bc >> 5], 1u << (bc & 31u));
+            }
+        }
+        __syncthreads();
+        unsigned int u = 0u;
+        for (unsigned int w = threadIdx.x; w < nwords; w += CV_THREADS) u += (unsigned int) __popc(bits[w]);
+        u = wave_sum_u32(u);
+        if (lane_id() == 0u) red[threadIdx.x >> 6] = u;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            unsigned int t = 0u;
+            for (unsigned int k = 0; k < CV_THREADS / 64u; ++k) t += red[k];
+            counts[rb] = t;
+            atomicMax(max_width, t);
+            atomicAdd(total, (unsigned long long) t);
+        }
+        __syncthreads();
+    }
+}
+
+template<typename S>
+__device__ __forceinline__ bool stored_nonzero(S raw);
+template<> __device__ __forceinline__ bool stored_nonzero<uint16_t>(uint16_t raw) { return (raw & 0x7fffu) != 0u; }
+template<> __device__ __forceinline__ bool stored_nonzero<float>(float raw) { return raw != 0.0f; }
+template<> __device__ __forceinline__ bool stored_nonzero<uint8_t>(uint8_t raw) { return (raw & 0x7fu) != 0u; }
+
+// One workgroup per row block: slots, blockColIdx, zero fill, value scatter, nnz.
+template<typename S>
+__global__ void __launch_bounds__(CV_THREADS) bell_emit_kernel(csamd_compressed_view src, const S *__restrict__ sval,
+                                                               unsigned int bs, unsigned int row_blocks, unsigned int width,
+                                                               unsigned int nwords, unsigned int ell_cols,
+                                                               unsigned int *__restrict__ block_col_idx, S *__restrict__ dval,
+                                                               unsigned long long *__restrict__ nnz_total) {
+    extern __shared__ unsigned int lds[];
+    unsigned int *bits = lds;
+    unsigned int *prefix = lds + nwords;
+    __shared__ unsigned int red[CV_THREADS / 64u];
+    __shared__ unsigned int unsorted_flag;
+    for (unsigned int rb = blockIdx.x; rb < row_blocks; rb += gridDim.x) {
+        for (unsigned int w = threadIdx.x; w < nwords; w += CV_THREADS) bits[w] = 0u;
+        if (threadIdx.x == 0) unsorted_flag = 0u;
+        __syncthreads();
+        const unsigned int r0 = rb * bs;
+        const unsigned int r1 = r0 + bs < src.rows ? r0 + bs : src.rows;
+        const unsigned int b = src.majorPtr[r0], e = src.majorPtr[r1];
+        for (unsigned int i = b + threadIdx.x; i < e; i += CV_THREADS) {
+            const unsigned int c = src.minorIdx[i];
+            if (c < src.cols) {
+                const unsigned int bc = c / bs;
+                atomicOr(&bits[bc >> 5], 1u << (bc & 31u));
+            }
+        }
+        // rows with non-increasing columns (possible duplicates) take the serial path
+        for (unsigned int r = r0; r < r1; ++r) {
+            const unsigned int rb0 = src.majorPtr[r], re0 = src.majorPtr[r + 1u];
+            for (unsigned int i = rb0 + threadIdx.x; i + 1u < re0; i += CV_THREADS) {
+                if (src.minorIdx[i] >= src.minorIdx[i + 1u]) unsorted_flag = 1u;
+            }
+        }
+        __syncthreads();
+        // exclusive prefix of popcounts over the bitset words (serial chunks of 256)
+        {
+            unsigned int carry = 0u;
+            for (unsigned int w0 = 0; w0 < nwords; w0 += CV_THREADS) {
+                const unsigned int w = w0 + threadIdx.x;
+                const unsigned int pc = w < nwords ? (unsigned int) __popc(bits[w]) : 0u;
+                unsigned int tot = 0u;
+                const unsigned int ex = block_exclusive_scan_u32(pc, red, &tot);
+                if (w < nwords) prefix[w] = carry + ex;
+                carry += tot;
+            }
+        }
+        __syncthreads();
+        // block column ids in ascending order, then invalid padding
+        const unsigned long long slot_base = (unsigned long long) rb * width;
+        for (unsigned int w = threadIdx.x; w < nwords; w += CV_THREADS) {
+            unsigned int m = bits[w];
+            unsigned int s = prefix[w];
+            while (m != 0u) {
+                const unsigned int bit = (unsigned int) __builtin_ctz(m);
+                block_col_idx[slot_base + s] = w * 32u + bit;
+                ++s;
+                m &= m - 1u;
+            }
+        }
+        {
+            const unsigned int unique = nwords != 0u ? prefix[nwords - 1u] + (unsigned int) __popc(bits[nwords - 1u]) : 0u;
+            for (unsigned int s = unique + threadIdx.x; s < width; s += CV_THREADS) block_col_idx[slot_base + s] = CSAMD_INVALID_COL;
+        }
+        // values were zero-filled by a preceding kernel
+        unsigned int nz = 0u;
+        if (unsorted_flag == 0u) {
+            for (unsigned int i = b + threadIdx.x; i < e; i += CV_THREADS) {
+                const unsigned int c = src.minorIdx[i];
+                if (c >= src.cols) continue;
+                // row of entry i: search in [r0, r1)
+                unsigned int lo = r0, hi = r1 - 1u;
+                while (lo < hi) {
+                    const unsigned int mid = (lo + hi + 1u) >> 1;
+                    if (src.majorPtr[mid] <= i) lo = mid;
+                    else hi = mid - 1u;
+                }
+                const unsigned int bc = c / bs;
+                const unsigned int wbits = bits[bc >> 5];
+                const unsigned int slot = prefix[bc >> 5] + (unsigned int) __popc(wbits & ((1u << (bc & 31u)) - 1u));
+                const S v = sval[i];
+                dval[(unsigned long long) lo * ell_cols + (unsigned long long) slot * bs + (c - bc * bs)] = v;
+                nz += stored_nonzero<S>(v) ? 1u : 0u;
+            }
+        } else if (threadIdx.x == 0) {
+            // serial CSR-order replay: last write wins, nnz tracks transitions
+            long long count = 0;
+            for (unsigned int r = r0; r < r1; ++r) {
+                for (unsigned int i = src.majorPtr[r]; i < src.majorPtr[r + 1u]; ++i) {
+                    const unsigned int c = src.minorIdx[i];
+                    if (c >= src.cols) continue;
+                    const unsigned int bc = c / bs;
+                    const unsigned int wbits = bits[bc >> 5];
+                    const unsigned int slot = prefix[bc >> 5] + (unsigned int) __popc(wbits & ((1u << (bc & 31u)) - 1u));
+                    S *dst = dval + (unsigned long long) r * ell_cols + (unsigned long long) slot * bs + (c - bc * bs);
+                    const bool prev = stored_nonzero<S>(*dst);
+                    const bool next = stored_nonzero<S>(sval[i]);
+                    if (!prev && next) ++count;
+                    else if (prev && !next) --count;
+                    *dst = sval[i];
+                }
+            }
+            nz = (unsigned int) count;
+        }
+        nz = wave_sum_u32(nz);
+        if (lane_id() == 0u && nz != 0u) atomicAdd(nnz_total, (unsigned long long) nz);
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Sliced-ELL.
+// ---------------------------------------------------------------------------
+__global__ void csr_row_nnz_kernel(const unsigned int *__restrict__ ptr, unsigned int rows, unsigned int *__restrict__ row_nnz) {
+    for (unsigned int r = blockIdx.x * blockDim.x + threadIdx.x; r < rows; r += gridDim.x * blockDim.x) row_nnz[r] = ptr[r + 1u] - ptr[r];
+}
+
+__global__ void csc_row_nnz_kernel(const unsigned int *__restrict__ minor, unsigned int nnz, unsigned int rows,
+                                   unsigned int *__restrict__ row_nnz, unsigned int *__restrict__ bad) {
+    for (unsigned long long i = blockIdx.x * (unsigned long long) blockDim.x + threadIdx.x; i < nnz;
+         i += (unsigned long long) gridDim.x * blockDim.x) {
+        const unsigned int r = minor[i];
+        if (r < rows) atomicAdd(row_nnz + r, 1u);
+        else *bad = 1u;
+    }
+}
+
+// Blocked-ELL row nnz: nonzero values at valid columns (sliced_ell_from_blocked_ell.cuh:36-62).
+template<typename S>
+__global__ void bell_row_nnz_kernel(csamd_blocked_ell_view src, unsigned int *__restrict__ row_nnz) {
+    const unsigned int wave = threadIdx.x >> 6;
+    const unsigned int nw = blockDim.x >> 6;
+    const unsigned int bs = src.block_size;
+    const unsigned int width = bs != 0u ? src.ell_cols / bs : 0u;
+    const S *val = (const S *) src.val;
+    for (unsigned int row = blockIdx.x * nw + wave; row < src.rows; row += gridDim.x * nw) {
+        const unsigned int rb = row / bs;
+        unsigned int count = 0u;
+        for (unsigned int e = lane_id(); e < width * bs; e += 64u) {
+            const unsigned int slot = e / bs;
+            const unsigned int bc = src.blockColIdx[(unsigned long long) rb * width + slot];
+            if (bc == CSAMD_INVALID_COL) continue;
+            const unsigned int col = bc * bs + (e - slot * bs);
+            if (col >= src.cols) continue;
+            if (stored_nonzero<S>(val[(unsigned long long) row * src.ell_cols + e])) ++count;
+        }
+        count = wave_sum_u32(count);
+        if (lane_id() == 0u) row_nnz[row] = count;
+    }
+}
+
+// Per slice: width = max row nnz, slot count = rows_in_slice * width.
+__global__ void sell_slices_kernel(const unsigned int *__restrict__ row_nnz, unsigned int rows, unsigned int S,
+                                   unsigned int slice_count, unsigned int *__restrict__ offsets, unsigned int *__restrict__ widths,
+                                   unsigned int *__restrict__ slot_counts) {
+    for (unsigned int s = blockIdx.x * blockDim.x + threadIdx.x; s < slice_count; s += gridDim.x * blockDim.x) {
+        const unsigned int r0 = s * S;
+        const unsigned int r1 = r0 + S < rows ? r0 + S : rows;
+        unsigned int m = 0u;
+        for (unsigned int r = r0; r < r1; ++r) m = row_nnz[r] > m ? row_nnz[r] : m;
+        widths[s] = m;
+        offsets[s] = r0;
+        const unsigned long long sc = (unsigned long long) (r1 - r0) * m;
+        slot_counts[s] = sc > 0xffffffffull ? 0xffffffffu : (unsigned int) sc;
+        if (s == slice_count - 1u) offsets[slice_count] = rows;
+    }
+}
+
+// Padded slot total for a candidate slice height (evaluate_sliced_candidate_).
+__global__ void sell_padded_kernel(const unsigned int *__restrict__ row_nnz, unsigned int rows, unsigned int S,
+                                   unsigned int slice_count, unsigned long long *__restrict__ total) {
+    unsigned long long acc = 0ull;
+    for (unsigned int s = blockIdx.x * blockDim.x + threadIdx.x; s < slice_count; s += gridDim.x * blockDim.x) {
+        const unsigned int r0 = s * S;
+        const unsigned int r1 = r0 + S < rows ? r0 + S : rows;
+        unsigned int m = 0u;
+        for (unsigned int r = r0; r < r1; ++r) m = row_nnz[r] > m ? row_nnz[r] : m;
+        acc += (unsigned long long) (r1 - r0) * m;
+    }
+    acc = wave_sum_u64(acc);
+    if (lane_id() == 0u && acc != 0ull) atomicAdd(total, acc);
+}
+
+__device__ __forceinline__ unsigned int sell_slice_of(const csamd_sliced_ell_view &d, unsigned int row) {
+    unsigned int slice;
+    if (d.slice_rows != 0u) slice = row / d.slice_rows;
+    else {
+        unsigned int lo = 0u, hi = d.slice_count - 1u;
+        while (lo < hi) {
+            const unsigned int mid = (lo + hi + 1u) >> 1;
+            if (d.slice_row_offsets[mid] <= row) lo = mid;
+            else hi = mid - 1u;
+        }
+        slice = lo;
+    }
+    return slice;
+}
+
+// One wave per row: copy the CSR segment, pad the rest of the row's slots.
+template<typename S>
+__global__ void sell_emit_csr_kernel(csamd_compressed_view src, const S *__restrict__ sval, csamd_sliced_ell_view dst,
+                                     S *__restrict__ dval) {
+    const unsigned int wave = threadIdx.x >> 6;
+    const unsigned int nw = blockDim.x >> 6;
+    for (unsigned int row = blockIdx.x * nw + wave; row < src.rows; row += gridDim.x * nw) {
+        const unsigned int slice = sell_slice_of(dst, row);
+        if (slice >= dst.slice_count) continue;
+        const unsigned int r0 = dst.slice_row_offsets[slice];
+        const unsigned int width = dst.slice_widths[slice];
+        const unsigned long long base = (unsigned long long) dst.slice_slot_offsets[slice] + (unsigned long long) (row - r0) * width;
+        const unsigned int b = src.majorPtr[row], e = src.majorPtr[row + 1u];
+        const unsigned int n = e - b < width ? e - b : width;
+        for (unsigned int k = lane_id(); k < width; k += 64u) {
+            if (k < n) {
+                dst.col_idx[base + k] = src.minorIdx[b + k];
+                dval[base + k] = sval[b + k];
+            } else {
+                dst.col_idx[base + k] = CSAMD_INVALID_COL;
+                dval[base + k] = (S) 0;
+            }
+        }
+    }
+}
+
+// One wave per row: compact nonzero valid entries in block-slot order.
+template<typename S>
+__global__ void sell_emit_bell_kernel(csamd_blocked_ell_view src, csamd_sliced_ell_view dst, S *__restrict__ dval) {
+    const unsigned int wave = threadIdx.x >> 6;
+    const unsigned int nw = blockDim.x >> 6;
+    const unsigned int bs = src.block_size;
+    const unsigned int width_blocks = bs != 0u ? src.ell_cols / bs : 0u;
+    const S *sval = (const S *) src.val;
+    for (unsigned int row = blockIdx.x * nw + wave; row < src.rows; row += gridDim.x * nw) {
+        const unsigned int slice = sell_slice_of(dst, row);
+        if (slice >= dst.slice_count) continue;
+        const unsigned int r0 = dst.slice_row_offsets[slice];
+        const unsigned int width = dst.slice_widths[slice];
+        const unsigned long long base = (unsigned long long) dst.slice_slot_offsets[slice] + (unsigned long long) (row - r0) * width;
+        const unsigned int rb = row / bs;
+        const unsigned int total = width_blocks * bs;
+        unsigned int written = 0u;
+        for (unsigned int e0 = 0u; e0 < total; e0 += 64u) {
+            const unsigned int e = e0 + lane_id();
+            bool keep = false;
+            unsigned int col = 0u;
+            S v = (S) 0;
+            if (e < total) {
+                const unsigned int slot = e / bs;
+                const unsigned int bc = src.blockColIdx[(unsigned long long) rb * width_blocks + slot];
+                if (bc != CSAMD_INVALID_COL) {
+                    col = bc * bs + (e - slot * bs);
+                    v = sval[(unsigned long long) row * src.ell_cols + e];
+                    keep = col < src.cols && stored_nonzero<S>(v);
+                }
+            }
+            const unsigned long long bal = __ballot(keep);
+            const unsigned int pos = written + (unsigned int) __popcll(bal & lanemask_lt());
+            if (keep && pos < width) {
+                dst.col_idx[base + pos] = col;
+                dval[base + pos] = v;
+            }
+            written += (unsigned int) __popcll(bal);
+        }
+        for (unsigned int k = written + lane_id(); k < width; k += 64u) {
+            dst.col_idx[base + k] = CSAMD_INVALID_COL;
+            dval[base + k] = (S) 0;
+        }
+    }
+}
+
+unsigned int rows_grid(unsigned int rows, unsigned int waves_per_block) {
+    return grid_for((rows + waves_per_block - 1u) / waves_per_block, 1u, (unsigned int) device_cu_count() * 32u);
+}
+
+int layout_from_row_nnz(const unsigned int *d_row_nnz, unsigned int rows, unsigned int slice_rows, unsigned int *d_offsets,
+                        unsigned int *d_widths, unsigned int *d_slot_offsets, csamd_convert_workspace *ws, void *scratch_after,
+                        unsigned int *out_slice_count, unsigned long long *out_total) {
+    hipStream_t s = (hipStream_t) ws->stream;
+    const unsigned int S = rows == 0u ? 0u : (slice_rows < 1u ? 1u : (slice_rows > rows ? rows : slice_rows));
+    const unsigned int count = rows == 0u ? 0u : (rows + S - 1u) / S;
+    if (out_slice_count != nullptr) *out_slice_count = count;
+    if (count == 0u) {
+        const unsigned int zero = 0u;
+        if (d_offsets != nullptr) CSAMD_HIP(hipMemcpyAsync(d_offsets, &zero, 4u, hipMemcpyHostToDevice, s), "H2D offsets");
+        CSAMD_HIP(hipStreamSynchronize(s), "sync");
+        if (out_total != nullptr) *out_total = 0ull;
+        return 1;
+    }
+    char *p = (char *) scratch_after;
+    unsigned int *slot_counts = (unsigned int *) p;
+    p += align_up((size_t) count * 4u, 256u);
+    unsigned long long *slot_off64 = (unsigned long long *) p;
+    p += align_up((size_t) (count + 1u) * 8u, 256u);
+    void *scan_tmp = p;
+    sell_slices_kernel<<<grid_for(count, 256u, 8192u), 256, 0, s>>>(d_row_nnz, rows, S, count, d_offsets, d_widths, slot_counts);
+    CSAMD_HIP(hipGetLastError(), "sell_slices_kernel");
+    if (!exclusive_scan_u32_to_u64(slot_counts, count, slot_off64, scan_tmp, exclusive_scan_scratch_bytes(count), s)) return 0;
+    unsigned long long total = 0ull;
+    CSAMD_HIP(hipMemcpyAsync(&total, slot_off64 + count, 8u, hipMemcpyDeviceToHost, s), "D2H total slots");
+    CSAMD_HIP(hipStreamSynchronize(s), "sell layout sync");
+    CSAMD_REQUIRE(total <= 0xffffffffull, "sliced_ell layout: %llu slots exceed the u32 slot offsets", total);
+    if (!exclusive_scan_u32(slot_counts, count, d_slot_offsets, nullptr, scan_tmp, exclusive_scan_scratch_bytes(count), s, false))
+        return 0;
+    CSAMD_HIP(hipStreamSynchronize(s), "sell layout sync 2");
+    if (out_total != nullptr) *out_total = total;
+    return 1;
+}
+
+size_t layout_scratch_bytes(unsigned int rows, unsigned int slice_rows) {
+    const unsigned int S = rows == 0u ? 1u : (slice_rows < 1u ? 1u : (slice_rows > rows ? rows : slice_rows));
+    const unsigned int count = rows == 0u ? 1u : (rows + S - 1u) / S;
+    return align_up((size_t) rows * 4u, 256u) + align_up((size_t) count * 4u, 256u) + align_up((size_t) (count + 1u) * 8u, 256u) +
+           exclusive_scan_scratch_bytes(count) + 4096u;
+}
+
+template<template<typename> class K, typename... Args>
+int bits_dispatch(unsigned int vt, Args... args) {
+    const int bytes = value_type_bytes(vt);
+    switch (bytes) {
+        case 1: return K<uint8_t>::run(args...);
+        case 2: return K<uint16_t>::run(args...);
+        case 4: return K<float>::run(args...);
+        default: set_error("unsupported value type %u", vt); return 0;
+    }
+}
+
+} // namespace
+} // namespace csamd
+
+using namespace csamd;
+
+namespace {
+
+template<typename S> struct bell_emit_launch {
+    static int run(const csamd_compressed_view *src, unsigned int row_blocks, unsigned int width, unsigned int nwords,
+                   csamd_blocked_ell_view *dst, unsigned long long *d_nnz, hipStream_t s) {
+        const size_t lds = (size_t) 2u * nwords * sizeof(unsigned int);
+        auto k = bell_emit_kernel<S>;
+        CSAMD_HIP(hipFuncSetAttribute((const void *) k, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds), "set lds");
+        k<<<grid_for(row_blocks, 1u, (unsigned int) device_cu_count() * 8u), CV_THREADS, lds, s>>>(
+            *src, (const S *) src->val, dst->block_size, row_blocks, width, nwords, dst->ell_cols, dst->blockColIdx, (S *) dst->val, d_nnz);
+        return hip_ok(hipGetLastError(), "bell_emit_kernel");
+    }
+};
+
+template<typename S> struct sell_emit_csr_launch {
+    static int run(const csamd_compressed_view *src, csamd_sliced_ell_view *dst, hipStream_t s) {
+        sell_emit_csr_kernel<S><<<rows_grid(src->rows, 4u), 256, 0, s>>>(*src, (const S *) src->val, *dst, (S *) dst->val);
+        return hip_ok(hipGetLastError(), "sell_emit_csr_kernel");
+    }
+};
+
+template<typename S> struct sell_emit_bell_launch {
+    static int run(const csamd_blocked_ell_view *src, csamd_sliced_ell_view *dst, hipStream_t s) {
+        sell_emit_bell_kernel<S><<<rows_grid(src->rows, 4u), 256, 0, s>>>(*src, *dst, (S *) dst->val);
+        return hip_ok(hipGetLastError(), "sell_emit_bell_kernel");
+    }
+};
+
+template<typename S> struct bell_row_nnz_launch {
+    static int run(const csamd_blocked_ell_view *src, unsigned int *row_nnz, hipStream_t s) {
+        bell_row_nnz_kernel<S><<<rows_grid(src->rows, 4u), 256, 0, s>>>(*src, row_nnz);
+        return hip_ok(hipGetLastError(), "bell_row_nnz_kernel");
+    }
+};
+
+int check_csr(const csamd_compressed_view *src, const char *who) {
+    CSAMD_REQUIRE(src != nullptr, "%s: null source", who);
+    CSAMD_REQUIRE(src->axis == CSAMD_COMPRESSED_BY_ROW, "%s: source axis must be row", who);
+    CSAMD_REQUIRE(src->rows == 0u || src->majorPtr != nullptr, "%s: null majorPtr", who);
+    return 1;
+}
+
+int bell_shape_impl(const csamd_compressed_view *src, unsigned int bs, csamd_convert_workspace *ws, unsigned int *out_w,
+                    unsigned long long *out_total) {
+    hipStream_t s = (hipStream_t) ws->stream;
+    const unsigned int row_blocks = (src->rows + bs - 1u) / bs;
+    const unsigned int col_blocks = (src->cols + bs - 1u) / bs;
+    const unsigned int nwords = (col_blocks + 31u) / 32u;
+    CSAMD_REQUIRE(nwords <= BELL_MAX_BITSET_WORDS, "blocked_ell: %u column blocks exceed the LDS bitset", col_blocks);
+    char *scratch = (char *) convert_scratch(ws, align_up((size_t) row_blocks * 4u, 256u) + 256u);
+    if (scratch == nullptr) return 0;
+    unsigned int *counts = (unsigned int *) (scratch + 256u);
+    unsigned int *d_w = (unsigned int *) scratch;
+    unsigned long long *d_total = (unsigned long long *) (scratch + 64u);
+    CSAMD_HIP(hipMemsetAsync(scratch, 0, 128u, s), "memset shape");
+    if (row_blocks != 0u) {
+        auto k = bell_shape_kernel;
+        const size_t lds = (size_t) nwords * 4u;
+        CSAMD_HIP(hipFuncSetAttribute((const void *) k, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds), "set lds");
+        k<<<grid_for(row_blocks, 1u, (unsigned int) device_cu_count() * 8u), CV_THREADS, lds, s>>>(*src, bs, row_blocks, nwords, counts, d_w,
+                                                                                               d_total);
+        CSAMD_HIP(hipGetLastError(), "bell_shape_kernel");
+    }
+    unsigned int w = 0u;
+    unsigned long long total = 0ull;
+    CSAMD_HIP(hipMemcpyAsync(&w, d_w, 4u, hipMemcpyDeviceToHost, s), "D2H width");
+    CSAMD_HIP(hipMemcpyAsync(&total, d_total, 8u, hipMemcpyDeviceToHost, s), "D2H total");
+    CSAMD_HIP(hipStreamSynchronize(s), "shape sync");
+    *out_w = w;
+    *out_total = total;
+    return 1;
+}
+
+} // namespace
+
+extern "C" {
+
+int csamd_blocked_ell_shape(const csamd_compressed_view *src, unsigned int block_size, csamd_convert_workspace *ws,
+                            unsigned int *out_ell_width, unsigned long long *out_total_blocks, double *out_fill_ratio) {
+    if (!check_csr(src, "blocked_ell_shape")) return 0;
+    CSAMD_REQUIRE(ws != nullptr && out_ell_width != nullptr, "blocked_ell_shape: null argument");
+    CSAMD_REQUIRE(block_size != 0u, "blocked_ell_shape: block_size is 0");
+    if (!ensure_device(ws->device >= 0 ? ws->device : 0)) return 0;
+    *out_ell_width = 0u;
+    if (out_total_blocks != nullptr) *out_total_blocks = 0ull;
+    if (out_fill_ratio != nullptr) *out_fill_ratio = 0.0;
+    if (src->rows == 0u || src->cols == 0u) return 1;
+    unsigned int w = 0u;
+    unsigned long long total = 0ull;
+    if (!bell_shape_impl(src, block_size, ws, &w, &total)) return 0;
+    const unsigned int row_blocks = (src->rows + block_size - 1u) / block_size;
+    *out_ell_width = w;
+    if (out_total_blocks != nullptr) *out_total_blocks = total;
+    if (out_fill_ratio != nullptr)
+        *out_fill_ratio = (row_blocks == 0u || w == 0u) ? 1.0 : (double) total / ((double) row_blocks * (double) w);
+    return 1;
+}
+
+int csamd_choose_blocked_ell_block_size(const csamd_compressed_view *src, const unsigned int *candidates,
+                                        unsigned int candidate_count, unsigned int value_bytes, csamd_convert_workspace *ws,
+                                        csamd_blocked_ell_tune_result *out) {
+    if (!check_csr(src, "choose_blocked_ell_block_size")) return 0;
+    CSAMD_REQUIRE(out != nullptr && candidates != nullptr && candidate_count != 0u && ws != nullptr,
+                  "choose_blocked_ell_block_size: bad arguments");
+    if (value_bytes == 0u) value_bytes = 2u;
+    csamd_blocked_ell_tune_result best{0u, 0.0, 0u};
+    for (unsigned int i = 0; i < candidate_count; ++i) {
+        const unsigned int bs = candidates[i];
+        if (bs == 0u) continue;
+        unsigned int w = 0u;
+        unsigned long long total = 0ull;
+        double fill = 0.0;
+        if (!csamd_blocked_ell_shape(src, bs, ws, &w, &total, &fill)) return 0;
+        if (src->rows == 0u || src->cols == 0u) fill = 0.0;
+        const size_t padded = (size_t) src->rows * (size_t) w * (size_t) bs * (size_t) value_bytes;
+        if (best.block_size == 0u || padded + 1u < best.padded_bytes ||
+            (padded == best.padded_bytes && fill > best.fill_ratio + 1.0e-9) ||
+            (padded == best.padded_bytes && fill + 1.0e-9 >= best.fill_ratio && bs > best.block_size)) {
+            best.block_size = bs;
+            best.fill_ratio = fill;
+            best.padded_bytes = padded;
+        }
+    }
+    CSAMD_REQUIRE(best.block_size != 0u, "choose_blocked_ell_block_size: no usable candidate");
+    *out = best;
+    return 1;
+}
+
+int csamd_blocked_ell_from_compressed(const csamd_compressed_view *src, unsigned int value_type, csamd_blocked_ell_view *dst,
+                                      csamd_convert_workspace *ws) {
+    if (!check_csr(src, "blocked_ell_from_compressed")) return 0;
+    CSAMD_REQUIRE(dst != nullptr && ws != nullptr, "blocked_ell_from_compressed: null argument");
+    CSAMD_REQUIRE(dst->block_size != 0u, "blocked_ell_from_compressed: block_size is 0");
+    CSAMD_REQUIRE(dst->ell_cols % dst->block_size == 0u, "blocked_ell_from_compressed: ell_cols not a multiple of block_size");
+    CSAMD_REQUIRE(value_type_bytes(value_type) != 0 && value_type != CSAMD_VALUE_F64, "blocked_ell_from_compressed: bad value type");
+    if (!ensure_device(ws->device >= 0 ? ws->device : 0)) return 0;
+    hipStream_t s = (hipStream_t) ws->stream;
+    const unsigned int bs = dst->block_size;
+    const unsigned int row_blocks = (src->rows + bs - 1u) / bs;
+    const unsigned int width = dst->ell_cols / bs;
+    const unsigned int col_blocks = (src->cols + bs - 1u) / bs;
+    const unsigned int nwords = (col_blocks + 31u) / 32u;
+    CSAMD_REQUIRE(nwords <= BELL_MAX_BITSET_WORDS, "blocked_ell: %u column blocks exceed the LDS bitset", col_blocks);
+    dst->rows = src->rows;
+    dst->cols = src->cols;
+    if (src->rows == 0u) {
+        dst->nnz = 0u;
+        return 1;
+    }
+    // the requested width must cover every row block
+    unsigned int need_w = 0u;
+    unsigned long long total = 0ull;
+    if (!bell_shape_impl(src, bs, ws, &need_w, &total)) return 0;
+    CSAMD_REQUIRE(need_w <= width, "blocked_ell_from_compressed: ell width %u < required %u", width, need_w);
+    CSAMD_REQUIRE(width == 0u || (dst->blockColIdx != nullptr && dst->val != nullptr),
+                  "blocked_ell_from_compressed: destination arrays missing");
+    unsigned long long *d_nnz = (unsigned long long *) convert_scratch(ws, 256u);
+    if (d_nnz == nullptr) return 0;
+    CSAMD_HIP(hipMemsetAsync(d_nnz, 0, 8u, s), "memset nnz");
+    if (width != 0u) {
+        if (!zero_bytes(dst->val, (size_t) dst->rows * dst->ell_cols * (size_t) value_type_bytes(value_type), s)) return 0;
+        if (!bits_dispatch<bell_emit_launch>(value_type, src, row_blocks, width, nwords, dst, d_nnz, s)) return 0;
+    }
+    unsigned long long nnz = 0ull;
+    CSAMD_HIP(hipMemcpyAsync(&nnz, d_nnz, 8u, hipMemcpyDeviceToHost, s), "D2H bell nnz");
+    CSAMD_HIP(hipStreamSynchronize(s), "bell emit sync");
+    CSAMD_REQUIRE(nnz <= 0xffffffffull, "blocked_ell_from_compressed: nnz overflow");
+    dst->nnz = (unsigned int) nnz;
+    return 1;
+}
+
+int csamd_sliced_ell_layout(const csamd_compressed_view *src, unsigned int slice_rows, unsigned int *d_slice_row_offsets,
+                            unsigned int *d_slice_widths, unsigned int *d_slice_slot_offsets, csamd_convert_workspace *ws,
+                            unsigned int *out_slice_count, unsigned long long *out_total_slots) {
+    CSAMD_REQUIRE(src != nullptr && ws != nullptr, "sliced_ell_layout: null argument");
+    CSAMD_REQUIRE(slice_rows != 0u, "sliced_ell_layout: slice_rows is 0");
+    if (!ensure_device(ws->device >= 0 ? ws->device : 0)) return 0;
+    hipStream_t s = (hipStream_t) ws->stream;
+    const unsigned int rows = src->rows;
+    char *scratch = (char *) convert_scratch(ws, layout_scratch_bytes(rows, slice_rows) + 256u);
+    if (scratch == nullptr) return 0;
+    unsigned int *bad = (unsigned int *) scratch;
+    unsigned int *row_nnz = (unsigned int *) (scratch + 256u);
+    void *after = scratch + 256u + align_up((size_t) rows * 4u, 256u);
+    if (rows != 0u) {
+        if (src->axis == CSAMD_COMPRESSED_BY_ROW) {
+            csr_row_nnz_kernel<<<grid_for(rows, 256u, 8192u), 256, 0, s>>>(src->majorPtr, rows, row_nnz);
+            CSAMD_HIP(hipGetLastError(), "csr_row_nnz_kernel");
+        } else {
+            CSAMD_HIP(hipMemsetAsync(scratch, 0, 256u + (size_t) rows * 4u, s), "memset row nnz");
+            if (src->nnz != 0u) {
+                csc_row_nnz_kernel<<<grid_for(src->nnz, 256u, 8192u), 256, 0, s>>>(src->minorIdx, src->nnz, rows, row_nnz, bad);
+                CSAMD_HIP(hipGetLastError(), "csc_row_nnz_kernel");
+            }
+            unsigned int hbad = 0u;
+            CSAMD_HIP(hipMemcpyAsync(&hbad, bad, 4u, hipMemcpyDeviceToHost, s), "D2H bad");
+            CSAMD_HIP(hipStreamSynchronize(s), "sync");
+            CSAMD_REQUIRE(hbad == 0u, "sliced_ell_layout: CSC row index out of range");
+        }
+    }
+    return layout_from_row_nnz(row_nnz, rows, slice_rows, d_slice_row_offsets, d_slice_widths, d_slice_slot_offsets, ws, after,
+                               out_slice_count, out_total_slots);
+}
+
+int csamd_sliced_ell_layout_from_blocked_ell(const csamd_blocked_ell_view *src, unsigned int value_type, unsigned int slice_rows,
+                                             unsigned int *d_slice_row_offsets, unsigned int *d_slice_widths,
+                                             unsigned int *d_slice_slot_offsets, csamd_convert_workspace *ws,
+                                             unsigned int *out_slice_count, unsigned long long *out_total_slots) {
+    CSAMD_REQUIRE(src != nullptr && ws != nullptr, "sliced_ell_layout_from_blocked_ell: null argument");
+    CSAMD_REQUIRE(slice_rows != 0u && src->block_size != 0u, "sliced_ell_layout_from_blocked_ell: zero slice_rows/block_size");
+    if (!ensure_device(ws->device >= 0 ? ws->device : 0)) return 0;
+    hipStream_t s = (hipStream_t) ws->stream;
+    const unsigned int rows = src->rows;
+    char *scratch = (char *) convert_scratch(ws, layout_scratch_bytes(rows, slice_rows) + 256u);
+    if (scratch == nullptr) return 0;
+    unsigned int *row_nnz = (unsigned int *) (scratch + 256u);
+    void *after = scratch + 256u + align_up((size_t) rows * 4u, 256u);
+    if (rows != 0u) {
+        if (!bits_dispatch<bell_row_nnz_launch>(value_type, src, row_nnz, s)) return 0;
+    }
+    return layout_from_row_nnz(row_nnz, rows, slice_rows, d_slice_row_offsets, d_slice_widths, d_slice_slot_offsets, ws, after,
+                               out_slice_count, out_total_slots);
+}
+
+int csamd_choose_sliced_ell_slice_rows(const csamd_compressed_view *src, const unsigned int *candidates,
+                                       unsigned int candidate_count, csamd_convert_workspace *ws,
+                                       csamd_sliced_ell_tune_result *out) {
+    if (!check_csr(src, "choose_sliced_ell_slice_rows")) return 0;
+    CSAMD_REQUIRE(candidates != nullptr && candidate_count != 0u && out != nullptr && ws != nullptr,
+                  "choose_sliced_ell_slice_rows: bad arguments");
+    if (!ensure_device(ws->device >= 0 ? ws->device : 0)) return 0;
+    hipStream_t s = (hipStream_t) ws->stream;
+    const unsigned int rows = src->rows;
+    const unsigned long long nnz = src->nnz;
+    char *scratch = (char *) convert_scratch(ws, align_up((size_t) rows * 4u, 256u) + 512u);
+    if (scratch == nullptr) return 0;
+    unsigned long long *d_total = (unsigned long long *) scratch;
+    unsigned int *row_nnz = (unsigned int *) (scratch + 256u);
+    if (rows != 0u) {
+        csr_row_nnz_kernel<<<grid_for(rows, 256u, 8192u), 256, 0, s>>>(src->majorPtr, rows, row_nnz);
+        CSAMD_HIP(hipGetLastError(), "csr_row_nnz_kernel");
+    }
+    csamd_sliced_ell_tune_result best;
+    std::memset(&best, 0, sizeof(best));
+    best.score = __builtin_inf();
+    for (unsigned int i = 0; i < candidate_count; ++i) {
+        if (candidates[i] == 0u) continue;
+        csamd_sliced_ell_tune_result r;
+        std::memset(&r, 0, sizeof(r));
+        if (rows == 0u) {
+            r.storage_ratio = 1.0;
+            r.score = 1.0;
+        } else {
+            const unsigned int S = candidates[i] > rows ? rows : candidates[i];
+            const unsigned int count = (rows + S - 1u) / S;
+            CSAMD_HIP(hipMemsetAsync(d_total, 0, 8u, s), "memset");
+            sell_padded_kernel<<<grid_for(count, 256u, 4096u), 256, 0, s>>>(row_nnz, rows, S, count, d_total);
+            CSAMD_HIP(hipGetLastError(), "sell_padded_kernel");
+            unsigned long long padded = 0ull;
+            CSAMD_HIP(hipMemcpyAsync(&padded, d_total, 8u, hipMemcpyDeviceToHost, s), "D2H padded");
+            CSAMD_HIP(hipStreamSynchronize(s), "sync");
+            r.slice_rows = S;
+            r.slice_count = count;
+            r.padded_slots = padded;
+            r.padded_bytes = padded * 6u + ((size_t) count * 2u + 1u) * 4u;
+            r.storage_ratio = nnz == 0ull ? 1.0 : (double) padded / (double) nnz;
+            r.slice_penalty = 0.25 * ((double) count / (double) rows);
+            r.score = r.storage_ratio + r.slice_penalty;
+        }
+        if (best.slice_rows == 0u || r.score + 1.0e-12 < best.score ||
+            (r.score <= best.score + 1.0e-12 && r.padded_bytes < best.padded_bytes) ||
+            (r.score <= best.score + 1.0e-12 && r.padded_bytes == best.padded_bytes && r.slice_rows > best.slice_rows)) {
+            best = r;
+        }
+    }
+    CSAMD_REQUIRE(!(best.slice_rows == 0u && rows != 0u), "choose_sliced_ell_slice_rows: no usable candidate");
+    *out = best;
+    return 1;
+}
+
+int csamd_sliced_ell_from_compressed(const csamd_compressed_view *src, unsigned int value_type, csamd_sliced_ell_view *dst,
+                                     csamd_convert_workspace *ws) {
+    CSAMD_REQUIRE(src != nullptr && dst != nullptr && ws != nullptr, "sliced_ell_from_compressed: null argument");
+    CSAMD_REQUIRE(value_type_bytes(value_type) != 0 && value_type != CSAMD_VALUE_F64, "sliced_ell_from_compressed: bad value type");
+    CSAMD_REQUIRE(src->axis == CSAMD_COMPRESSED_BY_ROW,
+                  "sliced_ell_from_compressed: transpose a CSC source to CSR first (ascending columns per row)");
+    if (!ensure_device(ws->device >= 0 ? ws->device : 0)) return 0;
+    dst->rows = src->rows;
+    dst->cols = src->cols;
+    dst->nnz = src->nnz;
+    if (src->rows == 0u || dst->slice_count == 0u) return 1;
+    return bits_dispatch<sell_emit_csr_launch>(value_type, src, dst, (hipStream_t) ws->stream);
+}
+
+int csamd_sliced_ell_from_blocked_ell(const csamd_blocked_ell_view *src, unsigned int value_type, csamd_sliced_ell_view *dst,
+                                      csamd_convert_workspace *ws) {
+    CSAMD_REQUIRE(src != nullptr && dst != nullptr && ws != nullptr, "sliced_ell_from_blocked_ell: null argument");
+    CSAMD_REQUIRE(src->block_size != 0u, "sliced_ell_from_blocked_ell: block_size is 0");
+    CSAMD_REQUIRE(value_type_bytes(value_type) != 0 && value_type != CSAMD_VALUE_F64, "sliced_ell_from_blocked_ell: bad value type");
+    if (!ensure_device(ws->device >= 0 ? ws->device : 0)) return 0;
+    dst->rows = src->rows;
+    dst->cols = src->cols;
+    dst->nnz = src->nnz;
+    if (src->rows == 0u || dst->slice_count == 0u) return 1;
+    return bits_dispatch<sell_emit_bell_launch>(value_type, src, dst, (hipStream_t) ws->stream);
+}
+
+} // extern "C"

@@ -1,0 +1,846 @@
+// Reference-ABI masked grouped row reductions and masked Blocked-ELL -> CSR
+// compaction.  Semantics follow src/runtime/mask_groups.cu (CellShard):
+//   per row r: masked-out row -> zeros, keep = 0 (emit_empty_row, :81-105);
+//   otherwise over stored (c, v) with c < cols, v != 0, feature_keep[c]:
+//   sum += v, detected += 1, max = max(0, v...), group[b] += v for each set
+//   bit b < G of feature_group_masks[c]; keep = sum >= min_total &&
+//   detected >= min_detected && fraction <= max_fraction (:171-193).
+// Work mapping is gfx950-native: one wave64 per row, lane-strided coalesced
+// streams, per-wave group bins in LDS, wave64 butterfly reductions.
+#include "common.hh"
+
+#include <cstdlib>
+
+namespace csamd {
+namespace {
+
+constexpr unsigned int MG_THREADS = 256u;
+constexpr unsigned int MG_WAVES = MG_THREADS / 64u;
+
+struct reduce_outputs {
+    float *row_totals;
+    float *max_values;
+    unsigned int *detected_features;
+    unsigned char *row_keep;
+    float *group_counts;
+    float *group_percentages;
+};
+
+__device__ __forceinline__ bool row_participates(const csamd_row_feature_mask_view &m, unsigned int row) {
+    return m.row_keep == nullptr || m.row_keep[row] != 0u;
+}
+__device__ __forceinline__ bool feature_participates(const csamd_row_feature_mask_view &m, unsigned int col) {
+    return m.feature_keep == nullptr || m.feature_keep[col] != 0u;
+}
+
+__device__ __forceinline__ void lds_fence() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup"); }
+
+// Per-element accumulation shared by all three layouts.
+struct row_acc {
+    float sum;
+    float vmax;
+    unsigned int detected;
+};
+
+__device__ __forceinline__ void accumulate(row_acc &a, unsigned int col, float v, unsigned int cols,
+                                           const csamd_row_feature_mask_view &masks, const uint32_t *__restrict__ fgm,
+                                           unsigned int group_count, float *wave_groups) {
+    if (col < cols && v != 0.0f && feature_participates(masks, col)) {
+        a.sum += v;
+        if (fgm != nullptr && group_count != 0u) {
+            uint32_t m = fgm[col];
+            while (m != 0u) {
+                const unsigned int g = (unsigned int) __builtin_ctz(m);
+                if (g < group_count) atomicAdd(wave_groups + g, v);
+                m &= m - 1u;
+            }
+        }
+        a.vmax = fmaxf(a.vmax, v);
+        a.detected += 1u;
+    }
+}
+
+__device__ __forceinline__ void emit_empty_row(unsigned int row, unsigned int group_count, const reduce_outputs &o) {
+    const unsigned int lane = lane_id();
+    if (lane == 0u) {
+        o.row_totals[row] = 0.0f;
+        o.detected_features[row] = 0u;
+        o.max_values[row] = 0.0f;
+        o.row_keep[row] = 0u;
+    }
+    if (o.group_counts != nullptr && o.group_percentages != nullptr) {
+        for (unsigned int g = lane; g < group_count; g += 64u) {
+            const unsigned long long off = (unsigned long long) row * group_count + g;
+            o.group_counts[off] = 0.0f;
+            o.group_percentages[off] = 0.0f;
+        }
+    }
+}
+
+__device__ __forceinline__ void finish_row(unsigned int row, row_acc a, unsigned int group_count,
+                                           const csamd_sparse_group_filter_params &filter, const float *wave_groups,
+                                           const reduce_outputs &o) {
+    const unsigned int lane = lane_id();
+    const float row_sum = wave_sum_f32(a.sum);
+    const float vmax = wave_max_f32(a.vmax);
+    const unsigned int detected = wave_sum_u32(a.detected);
+    lds_fence();
+    if (lane == 0u) {
+        o.row_totals[row] = row_sum;
+        o.max_values[row] = vmax;
+        o.detected_features[row] = detected;
+        const float gsel = filter.fraction_group_index < group_count ? wave_groups[filter.fraction_group_index] : 0.0f;
+        const float frac = row_sum > 0.0f ? gsel / row_sum : 0.0f;
+        o.row_keep[row] = (unsigned char) (row_sum >= filter.min_total && detected >= filter.min_detected_features &&
+                                           frac <= filter.max_group_fraction);
+    }
+    if (o.group_counts != nullptr && o.group_percentages != nullptr && lane < group_count) {
+        const float gs = wave_groups[lane];
+        const unsigned long long off = (unsigned long long) row * group_count + lane;
+        o.group_counts[off] = gs;
+        o.group_percentages[off] = row_sum > 0.0f ? 100.0f * gs / row_sum : 0.0f;
+    }
+    lds_fence();
+}
+
+template<int VT>
+__global__ void __launch_bounds__(MG_THREADS) group_reduce_compressed_kernel(csamd_compressed_view src, const uint32_t *__restrict__ fgm,
+                                                                             csamd_row_feature_mask_view masks, unsigned int group_count,
+                                                                             csamd_sparse_group_filter_params filter, reduce_outputs o) {
+    using S = typename value_traits<VT>::storage;
+    __shared__ float groups[MG_WAVES][CSAMD_MAX_MASK_GROUPS];
+    const unsigned int wave = threadIdx.x >> 6;
+    const unsigned int lane = lane_id();
+    float *wg = groups[wave];
+    const S *__restrict__ val = (const S *) src.val;
+    for (unsigned int row = blockIdx.x * MG_WAVES + wave; row < src.rows; row += gridDim.x * MG_WAVES) {
+        if (lane < CSAMD_MAX_MASK_GROUPS) wg[lane] = 0.0f;
+        lds_fence();
+        if (!row_participates(masks, row)) {
+            emit_empty_row(row, group_count, o);
+            continue;
+        }
+        const unsigned int begin = src.majorPtr[row];
+        const unsigned int end = src.majorPtr[row + 1u];
+        row_acc a{0.0f, 0.0f, 0u};
+        for (unsigned int i = begin + lane; i < end; i += 64u) {
+            accumulate(a, src.minorIdx[i], value_traits<VT>::decode(val[i]), src.cols, masks, fgm, group_count, wg);
+        }
+        finish_row(row, a, group_count, filter, wg, o);
+    }
+}
+
+__device__ __forceinline__ unsigned int find_slice_of_row(const csamd_sliced_ell_view &s, unsigned int row) {
+    unsigned int slice;
+    if (s.slice_rows == 32u) {
+        slice = row >> 5;
+    } else if (s.slice_rows != 0u) {
+        slice = row / s.slice_rows;
+    } else {
+        // offsets[slice] <= row < offsets[slice+1]; binary search (the
+        // reference walks linearly, same answer).
+        unsigned int lo = 0u, hi = s.slice_count - 1u;
+        while (lo < hi) {
+            const unsigned int mid = (lo + hi + 1u) >> 1;
+            if (s.slice_row_offsets[mid] <= row) lo = mid;
+            else hi = mid - 1u;
+        }
+        slice = lo;
+    }
+    if (slice >= s.slice_count) slice = s.slice_count - 1u;
+    return slice;
+}
+
+template<int VT>
+__global__ void __launch_bounds__(MG_THREADS) group_reduce_sliced_kernel(csamd_sliced_ell_view src, const uint32_t *__restrict__ fgm,
+                                                                         csamd_row_feature_mask_view masks, unsigned int group_count,
+                                                                         csamd_sparse_group_filter_params filter, reduce_outputs o) {
+    using S = typename value_traits<VT>::storage;
+    __shared__ float groups[MG_WAVES][CSAMD_MAX_MASK_GROUPS];
+    const unsigned int wave = threadIdx.x >> 6;
+    const unsigned int lane = lane_id();
+    float *wg = groups[wave];
+    const S *__restrict__ val = (const S *) src.val;
+    for (unsigned int row = blockIdx.x * MG_WAVES + wave; row < src.rows; row += gridDim.x * MG_WAVES) {
+        if (lane < CSAMD_MAX_MASK_GROUPS) wg[lane] = 0.0f;
+        lds_fence();
+        if (!row_participates(masks, row)) {
+            emit_empty_row(row, group_count, o);
+            continue;
+        }
+        unsigned int width = 0u;
+        unsigned long long slot_base = 0ull;
+        if (src.slice_count != 0u) {
+            const unsigned int slice = find_slice_of_row(src, row);
+            const unsigned int row_begin = src.slice_row_offsets[slice];
+            width = src.slice_widths[slice];
+            slot_base = (unsigned long long) src.slice_slot_offsets[slice] + (unsigned long long) (row - row_begin) * width;
+        }
+        row_acc a{0.0f, 0.0f, 0u};
+        for (unsigned int slot = lane; slot < width; slot += 64u) {
+            accumulate(a, src.col_idx[slot_base + slot], value_traits<VT>::decode(val[slot_base + slot]), src.cols, masks,
+                       fgm, group_count, wg);
+        }
+        finish_row(row, a, group_count, filter, wg, o);
+    }
+}
+
+template<int VT>
+__global__ void __launch_bounds__(MG_THREADS) group_reduce_blocked_kernel(csamd_blocked_ell_view src, const uint32_t *__restrict__ fgm,
+                                                                          csamd_row_feature_mask_view masks, unsigned int group_count,
+                                                                          csamd_sparse_group_filter_params filter, reduce_outputs o) {
+    using S = typename value_traits<VT>::storage;
+    __shared__ float groups[MG_WAVES][CSAMD_MAX_MASK_GROUPS];
+    const unsigned int wave = threadIdx.x >> 6;
+    const unsigned int lane = lane_id();
+    float *wg = groups[wave];
+    const S *__restrict__ val = (const S *) src.val;
+    const unsigned int bs = src.block_size;
+    const unsigned int width = bs != 0u ? src.ell_cols / bs : 0u;
+    for (unsigned int row = blockIdx.x * MG_WAVES + wave; row < src.rows; row += gridDim.x * MG_WAVES) {
+        if (lane < CSAMD_MAX_MASK_GROUPS) wg[lane] = 0.0f;
+        lds_fence();
+        if (!row_participates(masks, row)) {
+            emit_empty_row(row, group_count, o);
+            continue;
+        }
+        const unsigned int rb = bs != 0u ? row / bs : 0u;
+        const unsigned long long row_base = (unsigned long long) row * src.ell_cols;
+        row_acc a{0.0f, 0.0f, 0u};
+        for (unsigned int e = lane; e < src.ell_cols; e += 64u) {
+            const unsigned int slot = bs != 0u ? e / bs : 0u;
+            const unsigned int lib = bs != 0u ? e % bs : 0u;
+            const unsigned int bc = width != 0u ? src.blockColIdx[(unsigned long long) rb * width + slot] : CSAMD_INVALID_COL;
+            const unsigned int gene = bc != CSAMD_INVALID_COL ? bc * bs + lib : src.cols;
+            accumulate(a, gene, value_traits<VT>::decode(val[row_base + e]), src.cols, masks, fgm, group_count, wg);
+        }
+        finish_row(row, a, group_count, filter, wg, o);
+    }
+}
+
+unsigned int reduce_grid(unsigned int rows) {
+    unsigned int blocks = (rows + MG_WAVES - 1u) / MG_WAVES;
+    const unsigned int cap = (unsigned int) device_cu_count() * 16u;
+    if (blocks < 1u) blocks = 1u;
+    if (blocks > cap) blocks = cap;
+    return blocks;
+}
+
+void bind_result(csamd_sparse_group_reduce_workspace *ws, unsigned int rows, unsigned int group_count,
+                 csamd_sparse_group_reduce_result *out) {
+    if (out == nullptr) return;
+    out->rows = rows;
+    out->group_count = group_count;
+    out->row_totals = ws->row_totals;
+    out->detected_features = ws->detected_features;
+    out->max_values = ws->max_values;
+    out->row_keep = ws->row_keep;
+    out->group_counts = ws->group_counts;
+    out->group_percentages = ws->group_percentages;
+}
+
+reduce_outputs outputs_of(const csamd_sparse_group_reduce_workspace *ws, unsigned int group_count) {
+    reduce_outputs o;
+    o.row_totals = ws->row_totals;
+    o.max_values = ws->max_values;
+    o.detected_features = ws->detected_features;
+    o.row_keep = ws->row_keep;
+    o.group_counts = group_count != 0u ? ws->group_counts : nullptr;
+    o.group_percentages = group_count != 0u ? ws->group_percentages : nullptr;
+    return o;
+}
+
+unsigned int sat_u32(unsigned long long v) { return v > 0xffffffffull ? 0xffffffffu : (unsigned int) v; }
+
+struct reduce_prelude {
+    unsigned int group_count;
+    csamd_row_feature_mask_view masks;
+    const uint32_t *fgm;
+};
+
+int prepare_reduce(const void *src, csamd_sparse_group_reduce_workspace *ws, const csamd_group_mask_config_view *groups,
+                   const csamd_row_feature_mask_view *masks, const csamd_sparse_group_filter_params *filter,
+                   unsigned int rows, unsigned int cols, unsigned long long values, unsigned int value_type,
+                   reduce_prelude *p) {
+    CSAMD_REQUIRE(src != nullptr && ws != nullptr && filter != nullptr, "compute_sparse_group_reduce: null argument");
+    CSAMD_REQUIRE(value_type_decodable(value_type), "compute_sparse_group_reduce: unsupported value type %u", value_type);
+    p->group_count = groups != nullptr ? groups->group_count : 0u;
+    CSAMD_REQUIRE(p->group_count <= CSAMD_MAX_MASK_GROUPS, "compute_sparse_group_reduce: group_count %u > %u", p->group_count,
+                  CSAMD_MAX_MASK_GROUPS);
+    if (!csamd_group_reduce_reserve(ws, rows, cols, sat_u32(values), p->group_count)) return 0;
+    if (masks != nullptr) p->masks = *masks;
+    else std::memset(&p->masks, 0, sizeof(p->masks));
+    p->fgm = groups != nullptr && groups->feature_group_masks != nullptr ? groups->feature_group_masks : ws->feature_group_masks;
+    return 1;
+}
+
+template<template<int> class Launch, typename View>
+int dispatch_value_type(unsigned int vt, const View &v, unsigned int grid, hipStream_t stream, const reduce_prelude &p,
+                        const csamd_sparse_group_filter_params &f, const reduce_outputs &o) {
+    switch (vt) {
+        case CSAMD_VALUE_F16: Launch<CSAMD_VALUE_F16>::run(v, grid, stream, p, f, o); break;
+        case CSAMD_VALUE_BF16: Launch<CSAMD_VALUE_BF16>::run(v, grid, stream, p, f, o); break;
+        case CSAMD_VALUE_F32: Launch<CSAMD_VALUE_F32>::run(v, grid, stream, p, f, o); break;
+        case CSAMD_VALUE_FP8_E4M3: Launch<CSAMD_VALUE_FP8_E4M3>::run(v, grid, stream, p, f, o); break;
+        default: set_error("unsupported value type %u", vt); return 0;
+    }
+    return hip_ok(hipGetLastError(), "group reduce kernel launch");
+}
+
+template<int VT> struct launch_compressed {
+    static void run(const csamd_compressed_view &v, unsigned int grid, hipStream_t s, const reduce_prelude &p,
+                    const csamd_sparse_group_filter_params &f, const reduce_outputs &o) {
+        group_reduce_compressed_kernel<VT><<<grid, MG_THREADS, 0, s>>>(v, p.fgm, p.masks, p.group_count, f, o);
+    }
+};
+template<int VT> struct launch_sliced {
+    static void run(const csamd_sliced_ell_view &v, unsigned int grid, hipStream_t s, const reduce_prelude &p,
+                    const csamd_sparse_group_filter_params &f, const reduce_outputs &o) {
+        group_reduce_sliced_kernel<VT><<<grid, MG_THREADS, 0, s>>>(v, p.fgm, p.masks, p.group_count, f, o);
+    }
+};
+template<int VT> struct launch_blocked {
+    static void run(const csamd_blocked_ell_view &v, unsigned int grid, hipStream_t s, const reduce_prelude &p,
+                    const csamd_sparse_group_filter_params &f, const reduce_outputs &o) {
+        group_reduce_blocked_kernel<VT><<<grid, MG_THREADS, 0, s>>>(v, p.fgm, p.masks, p.group_count, f, o);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Masked Blocked-ELL -> CSR (manual_reoptimize_masked_sparse, mask_groups.cu:960-1034).
+// ---------------------------------------------------------------------------
+struct reopt_private {
+    void *compressed_block;
+    size_t compressed_bytes;
+    void *scratch_block;
+    size_t scratch_bytes;
+};
+
+__device__ __forceinline__ bool reopt_keep(const csamd_blocked_ell_view &src, const csamd_row_feature_mask_view &masks,
+                                           unsigned int rb, unsigned int width, unsigned int e, unsigned int row,
+                                           unsigned int output_cols, unsigned int *out_col, uint16_t *bits) {
+    const unsigned int bs = src.block_size;
+    const unsigned int slot = bs != 0u ? e / bs : 0u;
+    const unsigned int lib = bs != 0u ? e % bs : 0u;
+    const unsigned int bc = width != 0u ? src.blockColIdx[(unsigned long long) rb * width + slot] : CSAMD_INVALID_COL;
+    const unsigned int col = bc != CSAMD_INVALID_COL ? bc * bs + lib : src.cols;
+    const uint16_t h = ((const uint16_t *) src.val)[(unsigned long long) row * src.ell_cols + e];
+    const float v = f16_bits_to_float(h);
+    if (col < src.cols && v != 0.0f && feature_participates(masks, col)) {
+        const unsigned int oc = masks.feature_remap != nullptr ? masks.feature_remap[col] : col;
+        if (oc < output_cols) {
+            *out_col = oc;
+            *bits = h;
+            return true;
+        }
+    }
+    return false;
+}
+
+__global__ void __launch_bounds__(MG_THREADS) reopt_count_kernel(csamd_blocked_ell_view src, csamd_row_feature_mask_view masks,
+                                                                 unsigned int output_rows, unsigned int output_cols,
+                                                                 unsigned int *major_ptr) {
+    const unsigned int wave = threadIdx.x >> 6;
+    const unsigned int bs = src.block_size;
+    const unsigned int width = bs != 0u ? src.ell_cols / bs : 0u;
+    for (unsigned int row = blockIdx.x * MG_WAVES + wave; row < src.rows; row += gridDim.x * MG_WAVES) {
+        if (!row_participates(masks, row)) continue;
+        const unsigned int out_row = masks.row_remap != nullptr ? masks.row_remap[row] : row;
+        if (out_row >= output_rows) continue;
+        const unsigned int rb = bs != 0u ? row / bs : 0u;
+        unsigned int count = 0u;
+        for (unsigned int e = lane_id(); e < src.ell_cols; e += 64u) {
+            unsigned int oc;
+            uint16_t bits;
+            count += reopt_keep(src, masks, rb, width, e, row, output_cols, &oc, &bits) ? 1u : 0u;
+        }
+        count = wave_sum_u32(count);
+        if (lane_id() == 0u) major_ptr[out_row + 1u] = count;
+    }
+}
+
+// In-place exclusive scan of major_ptr[1..rows] into major_ptr[0..rows]
+// (single block, loops with a carry).
+__global__ void __launch_bounds__(1024) reopt_scan_kernel(unsigned int rows, unsigned int *major_ptr) {
+    __shared__ unsigned int smem[16];
+    __shared__ unsigned int carry;
+    if (threadIdx.x == 0) carry = 0u;
+    __syncthreads();
+    for (unsigned int start = 0; start < rows; start += blockDim.x) {
+        const unsigned int i = start + threadIdx.x;
+        const unsigned int v = i < rows ? major_ptr[i + 1u] : 0u;
+        unsigned int total = 0u;
+        const unsigned int ex = block_exclusive_scan_u32(v, smem, &total);
+        const unsigned int c = carry;
+        __syncthreads();
+        if (i < rows) major_ptr[i] = c + ex;
+        __syncthreads();
+        if (threadIdx.x == 0) carry = c + total;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) major_ptr[rows] = carry;
+}
+
+__global__ void __launch_bounds__(MG_THREADS) reopt_scatter_kernel(csamd_blocked_ell_view src, csamd_row_feature_mask_view masks,
+                                                                   unsigned int output_rows, unsigned int output_cols,
+                                                                   const unsigned int *__restrict__ major_ptr,
+                                                                   unsigned int *__restrict__ row_write,
+                                                                   unsigned int *__restrict__ minor_idx, uint16_t *__restrict__ val) {
+    const unsigned int wave = threadIdx.x >> 6;
+    const unsigned int bs = src.block_size;
+    const unsigned int width = bs != 0u ? src.ell_cols / bs : 0u;
+    for (unsigned int row = blockIdx.x * MG_WAVES + wave; row < src.rows; row += gridDim.x * MG_WAVES) {
+        if (!row_participates(masks, row)) continue;
+        const unsigned int out_row = masks.row_remap != nullptr ? masks.row_remap[row] : row;
+        if (out_row >= output_rows) continue;
+        const unsigned int rb = bs != 0u ? row / bs : 0u;
+        const unsigned int ell_round = (src.ell_cols + 63u) & ~63u;
+        for (unsigned int e0 = 0u; e0 < ell_round; e0 += 64u) {
+            const unsigned int e = e0 + lane_id();
+            unsigned int oc = 0u;
+            uint16_t bits = 0u;
+            const bool keep = e < src.ell_cols && reopt_keep(src, masks, rb, width, e, row, output_cols, &oc, &bits);
+            const unsigned long long bal = __ballot(keep);
+            if (bal == 0ull) continue;
+            unsigned int base = 0u;
+            if (lane_id() == 0u) base = atomicAdd(row_write + out_row, (unsigned int) __popcll(bal));
+            base = __shfl(base, 0, 64);
+            if (keep) {
+                const unsigned int w = major_ptr[out_row] + base + (unsigned int) __popcll(bal & lanemask_lt());
+                minor_idx[w] = oc;
+                val[w] = bits;
+            }
+        }
+    }
+}
+
+__global__ void fill_u32_kernel(unsigned int *p, unsigned long long n, unsigned int v) {
+    for (unsigned long long i = blockIdx.x * (unsigned long long) blockDim.x + threadIdx.x; i < n;
+         i += (unsigned long long) gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+int fill_u32(unsigned int *p, unsigned long long n, unsigned int v, hipStream_t s) {
+    if (n == 0ull) return 1;
+    fill_u32_kernel<<<grid_for(n, 256u, 4096u), 256, 0, s>>>(p, n, v);
+    return hip_ok(hipGetLastError(), "fill_u32_kernel");
+}
+
+} // namespace
+} // namespace csamd
+
+using namespace csamd;
+
+extern "C" {
+
+void csamd_group_reduce_init(csamd_sparse_group_reduce_workspace *ws) {
+    if (ws == nullptr) return;
+    std::memset(ws, 0, sizeof(*ws));
+    ws->device = -1;
+}
+
+void csamd_group_reduce_clear(csamd_sparse_group_reduce_workspace *ws) {
+    if (ws == nullptr) return;
+    if (ws->device >= 0) (void) hipSetDevice(ws->device);
+    if (ws->owns_stream != 0 && ws->stream != nullptr) (void) hipStreamDestroy((hipStream_t) ws->stream);
+    if (ws->scratch_block != nullptr) (void) hipFree(ws->scratch_block);
+    if (ws->feature_block != nullptr) (void) hipFree(ws->feature_block);
+    if (ws->row_block != nullptr) (void) hipFree(ws->row_block);
+    csamd_group_reduce_init(ws);
+}
+
+int csamd_group_reduce_setup(csamd_sparse_group_reduce_workspace *ws, int device, csamd_stream_t stream) {
+    CSAMD_REQUIRE(ws != nullptr, "setup: null workspace");
+    csamd_group_reduce_clear(ws);
+    CSAMD_HIP(hipSetDevice(device), "hipSetDevice sparse group setup");
+    ws->device = device;
+    if (stream == nullptr) {
+        hipStream_t s = nullptr;
+        CSAMD_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreateWithFlags sparse group");
+        ws->stream = (csamd_stream_t) s;
+        ws->owns_stream = 1;
+    } else {
+        ws->stream = stream;
+        ws->owns_stream = 0;
+    }
+    return 1;
+}
+
+int csamd_group_reduce_reserve(csamd_sparse_group_reduce_workspace *ws, unsigned int rows, unsigned int cols,
+                               unsigned int values, unsigned int group_count) {
+    CSAMD_REQUIRE(ws != nullptr, "reserve: null workspace");
+    CSAMD_REQUIRE(group_count <= CSAMD_MAX_MASK_GROUPS, "reserve: group_count %u > %u", group_count, CSAMD_MAX_MASK_GROUPS);
+    CSAMD_HIP(hipSetDevice(ws->device >= 0 ? ws->device : 0), "hipSetDevice sparse group reserve");
+    if (rows > ws->rows_capacity || group_count > ws->group_capacity) {
+        const unsigned int ar = rows > ws->rows_capacity ? rows : ws->rows_capacity;
+        const unsigned int ag = group_count > ws->group_capacity ? group_count : ws->group_capacity;
+        const size_t n = (size_t) ar, ng = (size_t) ar * ag;
+        size_t off_tot = 0, off_max, off_det, off_keep, off_gc, off_gp, bytes;
+        off_max = align_up(off_tot + n * sizeof(float), 16u);
+        off_det = align_up(off_max + n * sizeof(float), 16u);
+        off_keep = align_up(off_det + n * sizeof(unsigned int), 16u);
+        off_gc = align_up(off_keep + n, 16u);
+        off_gp = align_up(off_gc + ng * sizeof(float), 16u);
+        bytes = off_gp + ng * sizeof(float);
+        if (ws->row_block != nullptr) (void) hipFree(ws->row_block);
+        ws->row_block = nullptr;
+        ws->rows_capacity = 0u;
+        ws->group_capacity = 0u;
+        if (bytes != 0u) CSAMD_HIP(hipMalloc(&ws->row_block, bytes), "hipMalloc sparse group row block");
+        char *b = (char *) ws->row_block;
+        ws->row_totals = (float *) (b + off_tot);
+        ws->max_values = (float *) (b + off_max);
+        ws->detected_features = (unsigned int *) (b + off_det);
+        ws->row_keep = (unsigned char *) (b + off_keep);
+        ws->group_counts = (float *) (b + off_gc);
+        ws->group_percentages = (float *) (b + off_gp);
+        ws->rows_capacity = ar;
+        ws->group_capacity = ag;
+    }
+    if (cols > ws->cols_capacity) {
+        if (ws->feature_block != nullptr) (void) hipFree(ws->feature_block);
+        ws->feature_block = nullptr;
+        ws->feature_group_masks = nullptr;
+        ws->cols_capacity = 0u;
+        CSAMD_HIP(hipMalloc(&ws->feature_block, (size_t) cols * sizeof(uint32_t)), "hipMalloc sparse group feature block");
+        ws->feature_group_masks = (uint32_t *) ws->feature_block;
+        ws->cols_capacity = cols;
+    }
+    if (values > ws->values_capacity) ws->values_capacity = values;
+    return 1;
+}
+
+int csamd_upload_feature_group_masks(csamd_sparse_group_reduce_workspace *ws, unsigned int cols, const uint32_t *host_masks) {
+    CSAMD_REQUIRE(ws != nullptr, "upload_feature_group_masks: null workspace");
+    if (!csamd_group_reduce_reserve(ws, ws->rows_capacity, cols, ws->values_capacity, ws->group_capacity)) return 0;
+    if (cols == 0u) return 1;
+    hipStream_t s = (hipStream_t) ws->stream;
+    if (host_masks == nullptr) return fill_u32(ws->feature_group_masks, cols, 0u, s);
+    CSAMD_HIP(hipMemcpyAsync(ws->feature_group_masks, host_masks, (size_t) cols * sizeof(uint32_t), hipMemcpyHostToDevice, s),
+              "hipMemcpyAsync feature group masks");
+    return 1;
+}
+
+int csamd_compute_sparse_group_reduce_compressed_typed(const csamd_compressed_view *src, unsigned int value_type,
+                                                       csamd_sparse_group_reduce_workspace *ws,
+                                                       const csamd_group_mask_config_view *groups,
+                                                       const csamd_row_feature_mask_view *masks,
+                                                       const csamd_sparse_group_filter_params *filter,
+                                                       csamd_sparse_group_reduce_result *out) {
+    reduce_prelude p;
+    CSAMD_REQUIRE(src != nullptr, "compute_sparse_group_reduce_compressed: null src");
+    CSAMD_REQUIRE(src->axis == CSAMD_COMPRESSED_BY_ROW, "compute_sparse_group_reduce_compressed: axis must be row");
+    if (!prepare_reduce(src, ws, groups, masks, filter, src->rows, src->cols, src->nnz, value_type, &p)) return 0;
+    if (src->rows != 0u) {
+        if (!dispatch_value_type<launch_compressed>(value_type, *src, reduce_grid(src->rows), (hipStream_t) ws->stream, p, *filter,
+                                                    outputs_of(ws, p.group_count)))
+            return 0;
+    }
+    bind_result(ws, src->rows, p.group_count, out);
+    return 1;
+}
+
+int csamd_compute_sparse_group_reduce_compressed(const csamd_compressed_view *src, csamd_sparse_group_reduce_workspace *ws,
+                                                 const csamd_group_mask_config_view *groups,
+                                                 const csamd_row_feature_mask_view *masks,
+                                                 const csamd_sparse_group_filter_params *filter,
+                                                 csamd_sparse_group_reduce_result *out) {
+    return csamd_compute_sparse_group_reduce_compressed_typed(src, CSAMD_VALUE_F16, ws, groups, masks, filter, out);
+}
+
+int csamd_compute_sparse_group_reduce_sliced_ell_typed(const csamd_sliced_ell_view *src, unsigned int value_type,
+                                                       csamd_sparse_group_reduce_workspace *ws,
+                                                       const csamd_group_mask_config_view *groups,
+                                                       const csamd_row_feature_mask_view *masks,
+                                                       const csamd_sparse_group_filter_params *filter,
+                                                       csamd_sparse_group_reduce_result *out) {
+    reduce_prelude p;
+    CSAMD_REQUIRE(src != nullptr, "compute_sparse_group_reduce(sliced_ell): null src");
+    if (!prepare_reduce(src, ws, groups, masks, filter, src->rows, src->cols, src->nnz, value_type, &p)) return 0;
+    CSAMD_REQUIRE(src->slice_count == 0u || (src->slice_row_offsets != nullptr && src->slice_widths != nullptr &&
+                                             src->slice_slot_offsets != nullptr),
+                  "compute_sparse_group_reduce(sliced_ell): slice arrays missing");
+    if (src->rows != 0u) {
+        if (!dispatch_value_type<launch_sliced>(value_type, *src, reduce_grid(src->rows), (hipStream_t) ws->stream, p, *filter,
+                                                outputs_of(ws, p.group_count)))
+            return 0;
+    }
+    bind_result(ws, src->rows, p.group_count, out);
+    return 1;
+}
+
+int csamd_compute_sparse_group_reduce_sliced_ell(const csamd_sliced_ell_view *src, csamd_sparse_group_reduce_workspace *ws,
+                                                 const csamd_group_mask_config_view *groups,
+                                                 const csamd_row_feature_mask_view *masks,
+                                                 const csamd_sparse_group_filter_params *filter,
+                                                 csamd_sparse_group_reduce_result *out) {
+    return csamd_compute_sparse_group_reduce_sliced_ell_typed(src, CSAMD_VALUE_F16, ws, groups, masks, filter, out);
+}
+
+int csamd_compute_sparse_group_reduce_blocked_ell_typed(const csamd_blocked_ell_view *src, unsigned int value_type,
+                                                        csamd_sparse_group_reduce_workspace *ws,
+                                                        const csamd_group_mask_config_view *groups,
+                                                        const csamd_row_feature_mask_view *masks,
+                                                        const csamd_sparse_group_filter_params *filter,
+                                                        csamd_sparse_group_reduce_result *out) {
+    reduce_prelude p;
+    CSAMD_REQUIRE(src != nullptr, "compute_sparse_group_reduce(blocked_ell): null src");
+    if (!prepare_reduce(src, ws, groups, masks, filter, src->rows, src->cols,
+                        (unsigned long long) src->rows * src->ell_cols, value_type, &p))
+        return 0;
+    if (src->rows != 0u) {
+        if (!dispatch_value_type<launch_blocked>(value_type, *src, reduce_grid(src->rows), (hipStream_t) ws->stream, p, *filter,
+                                                 outputs_of(ws, p.group_count)))
+            return 0;
+    }
+    bind_result(ws, src->rows, p.group_count, out);
+    return 1;
+}
+
+int csamd_compute_sparse_group_reduce_blocked_ell(const csamd_blocked_ell_view *src, csamd_sparse_group_reduce_workspace *ws,
+                                                  const csamd_group_mask_config_view *groups,
+                                                  const csamd_row_feature_mask_view *masks,
+                                                  const csamd_sparse_group_filter_params *filter,
+                                                  csamd_sparse_group_reduce_result *out) {
+    return csamd_compute_sparse_group_reduce_blocked_ell_typed(src, CSAMD_VALUE_F16, ws, groups, masks, filter, out);
+}
+
+// ---------------------------------------------------------------------------
+// Fleet (one GPU's shard set).
+// ---------------------------------------------------------------------------
+void csamd_fleet_init(csamd_sparse_group_reduce_fleet_workspace *fleet) {
+    if (fleet == nullptr) return;
+    std::memset(fleet, 0, sizeof(*fleet));
+}
+
+void csamd_fleet_clear(csamd_sparse_group_reduce_fleet_workspace *fleet) {
+    if (fleet == nullptr) return;
+    if (fleet->devices != nullptr)
+        for (unsigned int i = 0; i < fleet->slot_count; ++i) csamd_group_reduce_clear(fleet->devices + i);
+    std::free(fleet->devices);
+    std::free(fleet->results);
+    std::free(fleet->slots);
+    std::free(fleet->device_ids);
+    csamd_fleet_init(fleet);
+}
+
+int csamd_setup_fleet(csamd_sparse_group_reduce_fleet_workspace *fleet, const csamd_sparse_group_reduce_fleet_config *config) {
+    CSAMD_REQUIRE(fleet != nullptr, "setup_fleet: null fleet");
+    csamd_fleet_clear(fleet);
+    CSAMD_REQUIRE(config == nullptr || config->ranked_nccl == nullptr,
+                  "setup_fleet: ranked communicators are not available on a one-GPU build");
+    CSAMD_REQUIRE(config == nullptr || config->device_count == 0u || config->device_ids != nullptr,
+                  "setup_fleet: device_ids missing");
+    const int visible = csamd_device_count();
+    CSAMD_REQUIRE(visible > 0, "setup_fleet: no HIP device");
+    const unsigned int count = config != nullptr && config->device_count != 0u ? config->device_count : (unsigned int) visible;
+    fleet->device_ids = (int *) std::calloc(count, sizeof(int));
+    fleet->slots = (unsigned int *) std::calloc(count, sizeof(unsigned int));
+    fleet->devices = (csamd_sparse_group_reduce_workspace *) std::calloc(count, sizeof(csamd_sparse_group_reduce_workspace));
+    fleet->results = (csamd_sparse_group_reduce_result *) std::calloc(count, sizeof(csamd_sparse_group_reduce_result));
+    if (fleet->device_ids == nullptr || fleet->slots == nullptr || fleet->devices == nullptr || fleet->results == nullptr) {
+        csamd_fleet_clear(fleet);
+        set_error("setup_fleet: host allocation failed");
+        return 0;
+    }
+    fleet->slot_count = count;
+    for (unsigned int i = 0; i < count; ++i) csamd_group_reduce_init(fleet->devices + i);
+    for (unsigned int i = 0; i < count; ++i) {
+        const int dev = config != nullptr && config->device_count != 0u ? config->device_ids[i] : (int) i;
+        if (dev < 0 || dev >= visible) {
+            csamd_fleet_clear(fleet);
+            set_error("setup_fleet: device %d not visible", dev);
+            return 0;
+        }
+        fleet->device_ids[i] = dev;
+        fleet->slots[i] = (unsigned int) dev;
+        if (!csamd_group_reduce_setup(fleet->devices + i, dev, nullptr)) {
+            csamd_fleet_clear(fleet);
+            return 0;
+        }
+    }
+    return 1;
+}
+
+int csamd_compute_sparse_group_reduce_fleet_blocked_ell(const csamd_blocked_ell_view *src_by_slot,
+                                                        csamd_sparse_group_reduce_fleet_workspace *fleet,
+                                                        const csamd_group_mask_config_view *groups,
+                                                        const csamd_row_feature_mask_view *masks,
+                                                        const csamd_sparse_group_filter_params *filter,
+                                                        csamd_sparse_group_reduce_fleet_result *out) {
+    CSAMD_REQUIRE(src_by_slot != nullptr && fleet != nullptr && filter != nullptr && fleet->slot_count != 0u &&
+                      fleet->devices != nullptr,
+                  "compute_sparse_group_reduce_fleet: bad arguments");
+    const unsigned int cols = src_by_slot[0].cols;
+    for (unsigned int i = 0; i < fleet->slot_count; ++i) {
+        CSAMD_REQUIRE(src_by_slot[i].cols == cols, "compute_sparse_group_reduce_fleet: column count differs across slots");
+        if (!csamd_compute_sparse_group_reduce_blocked_ell(src_by_slot + i, fleet->devices + i, groups, masks, filter,
+                                                          fleet->results + i))
+            return 0;
+    }
+    if (out != nullptr) {
+        out->slot_count = fleet->slot_count;
+        out->leader_index = 0u;
+        out->slot_results = fleet->results;
+    }
+    return 1;
+}
+
+int csamd_compute_sparse_group_reduce_fleet_sliced_ell(const csamd_sliced_ell_view *src_by_slot,
+                                                       csamd_sparse_group_reduce_fleet_workspace *fleet,
+                                                       const csamd_group_mask_config_view *groups,
+                                                       const csamd_row_feature_mask_view *masks,
+                                                       const csamd_sparse_group_filter_params *filter,
+                                                       csamd_sparse_group_reduce_fleet_result *out) {
+    CSAMD_REQUIRE(src_by_slot != nullptr && fleet != nullptr && filter != nullptr && fleet->slot_count != 0u &&
+                      fleet->devices != nullptr,
+                  "compute_sparse_group_reduce_fleet: bad arguments");
+    const unsigned int cols = src_by_slot[0].cols;
+    for (unsigned int i = 0; i < fleet->slot_count; ++i) {
+        CSAMD_REQUIRE(src_by_slot[i].cols == cols, "compute_sparse_group_reduce_fleet: column count differs across slots");
+        if (!csamd_compute_sparse_group_reduce_sliced_ell(src_by_slot + i, fleet->devices + i, groups, masks, filter,
+                                                         fleet->results + i))
+            return 0;
+    }
+    if (out != nullptr) {
+        out->slot_count = fleet->slot_count;
+        out->leader_index = 0u;
+        out->slot_results = fleet->results;
+    }
+    return 1;
+}
+
+// ---------------------------------------------------------------------------
+// Masked reoptimize.
+// ---------------------------------------------------------------------------
+int csamd_predict_masked_sparse_reoptimization(const csamd_auto_reoptimize_prediction_input *input,
+                                               csamd_auto_reoptimize_prediction *out) {
+    CSAMD_REQUIRE(input != nullptr && out != nullptr, "predict_masked_sparse_reoptimization: null argument");
+    out->should_reoptimize = 0u;
+    out->estimated_net_gain_ms = 0.0f;
+    return 1;
+}
+
+void csamd_reoptimize_init(csamd_masked_sparse_reoptimize_workspace *ws) {
+    if (ws == nullptr) return;
+    std::memset(ws, 0, sizeof(*ws));
+    ws->device = -1;
+}
+
+void csamd_reoptimize_clear(csamd_masked_sparse_reoptimize_workspace *ws) {
+    if (ws == nullptr) return;
+    if (ws->device >= 0) (void) hipSetDevice(ws->device);
+    if (ws->owns_stream != 0 && ws->stream != nullptr) (void) hipStreamDestroy((hipStream_t) ws->stream);
+    reopt_private *st = (reopt_private *) ws->private_workspace;
+    if (st != nullptr) {
+        if (st->compressed_block != nullptr) (void) hipFree(st->compressed_block);
+        if (st->scratch_block != nullptr) (void) hipFree(st->scratch_block);
+        std::free(st);
+    }
+    csamd_reoptimize_init(ws);
+}
+
+int csamd_reoptimize_setup(csamd_masked_sparse_reoptimize_workspace *ws, int device, csamd_stream_t stream) {
+    CSAMD_REQUIRE(ws != nullptr, "reoptimize setup: null workspace");
+    csamd_reoptimize_clear(ws);
+    CSAMD_HIP(hipSetDevice(device), "hipSetDevice masked reoptimize setup");
+    ws->device = device;
+    if (stream == nullptr) {
+        hipStream_t s = nullptr;
+        CSAMD_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreateWithFlags masked reoptimize");
+        ws->stream = (csamd_stream_t) s;
+        ws->owns_stream = 1;
+    } else {
+        ws->stream = stream;
+        ws->owns_stream = 0;
+    }
+    ws->private_workspace = std::calloc(1u, sizeof(reopt_private));
+    CSAMD_REQUIRE(ws->private_workspace != nullptr, "reoptimize setup: host allocation failed");
+    return 1;
+}
+
+int csamd_manual_reoptimize_masked_sparse(const csamd_blocked_ell_view *src, const csamd_row_feature_mask_view *masks,
+                                          const csamd_masked_sparse_reoptimize_config *config,
+                                          csamd_masked_sparse_reoptimize_workspace *ws,
+                                          csamd_masked_sparse_reoptimize_result *out) {
+    CSAMD_REQUIRE(src != nullptr && config != nullptr && ws != nullptr && out != nullptr,
+                  "manual_reoptimize_masked_sparse: null argument");
+    CSAMD_HIP(hipSetDevice(ws->device >= 0 ? ws->device : 0), "hipSetDevice masked reoptimize");
+    if (ws->private_workspace == nullptr) {
+        ws->private_workspace = std::calloc(1u, sizeof(reopt_private));
+        CSAMD_REQUIRE(ws->private_workspace != nullptr, "reoptimize: host allocation failed");
+    }
+    reopt_private *st = (reopt_private *) ws->private_workspace;
+    hipStream_t s = (hipStream_t) ws->stream;
+    csamd_row_feature_mask_view m;
+    if (masks != nullptr) m = *masks;
+    else std::memset(&m, 0, sizeof(m));
+    const unsigned int output_rows = config->output_rows != 0u ? config->output_rows : src->rows;
+    const unsigned int output_cols = config->output_cols != 0u ? config->output_cols : src->cols;
+    // The reference sizes this as a u32 product (mask_groups.cu:971); here it
+    // is computed in 64 bits and rejected if the CSR view cannot index it.
+    const unsigned long long max_nnz = (unsigned long long) src->rows * src->ell_cols;
+    CSAMD_REQUIRE(max_nnz <= 0xffffffffull, "manual_reoptimize: rows*ell_cols = %llu exceeds u32", max_nnz);
+    const size_t major_bytes = (size_t) (output_rows + 1ull) * sizeof(unsigned int);
+    const size_t minor_off = align_up(major_bytes, 16u);
+    const size_t val_off = align_up(minor_off + max_nnz * sizeof(unsigned int), 16u);
+    const size_t bytes = val_off + max_nnz * sizeof(uint16_t);
+    if (bytes > st->compressed_bytes) {
+        if (st->compressed_block != nullptr) (void) hipFree(st->compressed_block);
+        st->compressed_block = nullptr;
+        st->compressed_bytes = 0u;
+        CSAMD_HIP(hipMalloc(&st->compressed_block, bytes), "hipMalloc masked compressed output");
+        st->compressed_bytes = bytes;
+    }
+    const size_t scratch_needed = (size_t) (output_rows == 0u ? 1u : output_rows) * sizeof(unsigned int);
+    if (scratch_needed > st->scratch_bytes) {
+        if (st->scratch_block != nullptr) (void) hipFree(st->scratch_block);
+        st->scratch_block = nullptr;
+        st->scratch_bytes = 0u;
+        CSAMD_HIP(hipMalloc(&st->scratch_block, scratch_needed), "hipMalloc masked reoptimize scratch");
+        st->scratch_bytes = scratch_needed;
+    }
+    char *base = (char *) st->compressed_block;
+    csamd_compressed_view c;
+    c.rows = output_rows;
+    c.cols = output_cols;
+    c.nnz = (unsigned int) max_nnz;
+    c.axis = CSAMD_COMPRESSED_BY_ROW;
+    c.majorPtr = (unsigned int *) base;
+    c.minorIdx = (unsigned int *) (base + minor_off);
+    c.val = (csamd_half *) (base + val_off);
+    unsigned int *row_write = (unsigned int *) st->scratch_block;
+
+    if (!fill_u32(c.majorPtr, (unsigned long long) output_rows + 1ull, 0u, s)) return 0;
+    if (!fill_u32(row_write, output_rows, 0u, s)) return 0;
+    const unsigned int grid = reduce_grid(src->rows);
+    if (src->rows != 0u) {
+        reopt_count_kernel<<<grid, MG_THREADS, 0, s>>>(*src, m, output_rows, output_cols, c.majorPtr);
+        CSAMD_HIP(hipGetLastError(), "reopt_count_kernel");
+    }
+    reopt_scan_kernel<<<1, 1024, 0, s>>>(output_rows, c.majorPtr);
+    CSAMD_HIP(hipGetLastError(), "reopt_scan_kernel");
+    if (src->rows != 0u) {
+        reopt_scatter_kernel<<<grid, MG_THREADS, 0, s>>>(*src, m, output_rows, output_cols, c.majorPtr, row_write, c.minorIdx,
+                                                         (uint16_t *) c.val);
+        CSAMD_HIP(hipGetLastError(), "reopt_scatter_kernel");
+    }
+    unsigned int host_nnz = 0u;
+    CSAMD_HIP(hipMemcpyAsync(&host_nnz, c.majorPtr + output_rows, sizeof(unsigned int), hipMemcpyDeviceToHost, s),
+              "hipMemcpyAsync masked nnz");
+    CSAMD_HIP(hipStreamSynchronize(s), "hipStreamSynchronize masked reoptimize");
+    c.nnz = host_nnz;
+    out->compressed = c;
+    std::memset(&out->blocked_ell, 0, sizeof(out->blocked_ell));
+    out->metadata.kept_rows = output_rows;
+    out->metadata.kept_features = output_cols;
+    out->metadata.live_nnz = host_nnz;
+    const unsigned long long rbc = src->block_size != 0u ? (src->rows + src->block_size - 1ull) / src->block_size : 0ull;
+    const unsigned long long w = src->block_size != 0u ? src->ell_cols / src->block_size : 0ull;
+    out->metadata.bytes_before = (unsigned long long) src->rows * src->ell_cols * 2ull + rbc * w * 4ull;
+    out->metadata.bytes_after = (unsigned long long) (output_rows + 1ull) * 4ull + (unsigned long long) host_nnz * 6ull;
+    out->metadata.output_layout = CSAMD_LAYOUT_COMPRESSED;
+    return 1;
+}
+
+} // extern "C"

@@ -1,0 +1,414 @@
+"""Device operations over the C ABI: conversion, transpose, bucketing, the
+row-label grouped reduction and the synthetic generator.
+
+Each function calls exactly one family of C-ABI entry points; nothing here
+computes results on the host.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import abi as A
+from ._lib import check, load
+from .device import (DeviceBlockedELL, DeviceCSR, DeviceSlicedELL, default_device, ptr, to_dev_f32, torch_bits_dtype,
+                     value_bytes)
+
+
+def current_stream_handle() -> int:
+    return torch.cuda.current_stream().cuda_stream
+
+
+def _empty_u32(n: int, device=None) -> torch.Tensor:
+    return torch.empty(max(int(n), 0), dtype=torch.int32, device=device or default_device())
+
+
+class ConvertWorkspace:
+    """Owns a csamd_convert_workspace bound to torch's current stream."""
+
+    def __init__(self, device: int | None = None, stream: int | None = None):
+        lib = load()
+        self.ws = A.convert_workspace()
+        lib.csamd_convert_init(C.byref(self.ws))
+        dev = torch.cuda.current_device() if device is None else device
+        check(lib.csamd_convert_setup(C.byref(self.ws), dev, stream if stream is not None else current_stream_handle()),
+              "convert setup")
+
+    def close(self):
+        if self.ws is not None:
+            load().csamd_convert_clear(C.byref(self.ws))
+            self.ws = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------------------
+# Blocked-ELL
+# ---------------------------------------------------------------------------
+def blocked_ell_shape(csr: DeviceCSR, block_size: int, ws: ConvertWorkspace):
+    lib = load()
+    w = C.c_uint32(0)
+    total = C.c_uint64(0)
+    fill = C.c_double(0.0)
+    v = csr.view()
+    check(lib.csamd_blocked_ell_shape(C.byref(v), block_size, C.byref(ws.ws), C.byref(w), C.byref(total), C.byref(fill)),
+          "blocked_ell_shape")
+    return int(w.value), int(total.value), float(fill.value)
+
+
+def choose_blocked_ell_block_size(csr: DeviceCSR, candidates, ws: ConvertWorkspace, value_bytes_: int | None = None):
+    lib = load()
+    cands = (C.c_uint32 * len(candidates))(*candidates)
+    out = A.blocked_ell_tune_result()
+    v = csr.view()
+    vb = value_bytes_ if value_bytes_ is not None else value_bytes(csr.value_type)
+    check(lib.csamd_choose_blocked_ell_block_size(C.byref(v), cands, len(candidates), vb, C.byref(ws.ws), C.byref(out)),
+          "choose_blocked_ell_block_size")
+    return int(out.block_size), float(out.fill_ratio), int(out.padded_bytes)
+
+
+def blocked_ell_from_compressed(csr: DeviceCSR, block_size: int, ws: ConvertWorkspace) -> DeviceBlockedELL:
+    width, _, _ = blocked_ell_shape(csr, block_size, ws)
+    rb = (csr.rows + block_size - 1) // block_size
+    dev = csr.minor_idx.device
+    bci = torch.empty(rb * width, dtype=torch.int32, device=dev)
+    val = torch.empty(csr.rows * width * block_size, dtype=torch_bits_dtype(csr.value_type), device=dev)
+    out = DeviceBlockedELL(csr.rows, csr.cols, 0, block_size, width * block_size, bci, val, csr.value_type)
+    v = out.view()
+    src = csr.view()
+    check(load().csamd_blocked_ell_from_compressed(C.byref(src), csr.value_type, C.byref(v), C.byref(ws.ws)),
+          "blocked_ell_from_compressed")
+    out.nnz = int(v.nnz)
+    return out
+
+
+# ---------------------------------------------------------------------------
+# Sliced-ELL
+# ---------------------------------------------------------------------------
+def _sell_alloc(rows: int, slice_rows: int, dev):
+    s = 0 if rows == 0 else min(max(slice_rows, 1), rows)
+    count = 0 if rows == 0 else (rows + s - 1) // s
+    return s, count, _empty_u32(count + 1, dev), _empty_u32(count, dev), _empty_u32(count, dev)
+
+
+def sliced_ell_layout(csr: DeviceCSR, slice_rows: int, ws: ConvertWorkspace):
+    dev = csr.minor_idx.device
+    s, count, offs, widths, slots = _sell_alloc(csr.rows, slice_rows, dev)
+    c_count = C.c_uint32(0)
+    total = C.c_uint64(0)
+    v = csr.view()
+    check(load().csamd_sliced_ell_layout(C.byref(v), slice_rows, ptr(offs), ptr(widths), ptr(slots), C.byref(ws.ws),
+                                         C.byref(c_count), C.byref(total)), "sliced_ell_layout")
+    return s, int(c_count.value), offs, widths, slots, int(total.value)
+
+
+def sliced_ell_from_compressed(csr: DeviceCSR, slice_rows: int, ws: ConvertWorkspace) -> DeviceSlicedELL:
+    s, count, offs, widths, slots, total = sliced_ell_layout(csr, slice_rows, ws)
+    dev = csr.minor_idx.device
+    out = DeviceSlicedELL(csr.rows, csr.cols, csr.nnz, count, s, offs, widths, slots,
+                          torch.empty(total, dtype=torch.int32, device=dev),
+                          torch.empty(total, dtype=torch_bits_dtype(csr.value_type), device=dev), csr.value_type)
+    v = out.view()
+    src = csr.view()
+    check(load().csamd_sliced_ell_from_compressed(C.byref(src), csr.value_type, C.byref(v), C.byref(ws.ws)),
+          "sliced_ell_from_compressed")
+    return out
+
+
+def choose_sliced_ell_slice_rows(csr: DeviceCSR, candidates, ws: ConvertWorkspace):
+    cands = (C.c_uint32 * len(candidates))(*candidates)
+    out = A.sliced_ell_tune_result()
+    v = csr.view()
+    check(load().csamd_choose_sliced_ell_slice_rows(C.byref(v), cands, len(candidates), C.byref(ws.ws), C.byref(out)),
+          "choose_sliced_ell_slice_rows")
+    return {k: getattr(out, k) for k in ("slice_rows", "slice_count", "padded_slots", "padded_bytes", "storage_ratio",
+                                         "slice_penalty", "score")}
+
+
+def sliced_ell_from_blocked_ell(bell: DeviceBlockedELL, slice_rows: int, ws: ConvertWorkspace) -> DeviceSlicedELL:
+    dev = bell.val.device
+    s, count, offs, widths, slots = _sell_alloc(bell.rows, slice_rows, dev)
+    c_count = C.c_uint32(0)
+    total = C.c_uint64(0)
+    bv = bell.view()
+    check(load().csamd_sliced_ell_layout_from_blocked_ell(C.byref(bv), bell.value_type, slice_rows, ptr(offs), ptr(widths),
+                                                          ptr(slots), C.byref(ws.ws), C.byref(c_count), C.byref(total)),
+          "sliced_ell_layout_from_blocked_ell")
+    t = int(total.value)
+    out = DeviceSlicedELL(bell.rows, bell.cols, bell.nnz, count, s, offs, widths, slots,
+                          torch.empty(t, dtype=torch.int32, device=dev),
+                          torch.empty(t, dtype=torch_bits_dtype(bell.value_type), device=dev), bell.value_type)
+    v = out.view()
+    check(load().csamd_sliced_ell_from_blocked_ell(C.byref(bv), bell.value_type, C.byref(v), C.byref(ws.ws)),
+          "sliced_ell_from_blocked_ell")
+    return out
+
+
+# ---------------------------------------------------------------------------
+# Transpose
+# ---------------------------------------------------------------------------
+class TransposeScratch:
+    def __init__(self):
+        self.buf = None
+
+    def get(self, nbytes: int, dev) -> torch.Tensor:
+        if self.buf is None or self.buf.numel() < nbytes:
+            self.buf = torch.empty(max(nbytes, 256), dtype=torch.uint8, device=dev)
+        return self.buf
+
+
+def compressed_transpose(csr: DeviceCSR, scratch: TransposeScratch | None = None, stream: int | None = None) -> DeviceCSR:
+    """CSR -> CSC (or CSC -> CSR); ascending source index inside each segment."""
+    lib = load()
+    c_dim = csr.major_dim
+    u_dim = csr.rows if csr.axis == A.COMPRESSED_BY_COL else csr.cols
+    dev = csr.minor_idx.device
+    nbytes = int(lib.csamd_compressed_transpose_scratch_bytes(c_dim, u_dim, csr.nnz))
+    scratch = scratch or TransposeScratch()
+    buf = scratch.get(nbytes, dev)
+    out_ptr = _empty_u32(u_dim + 1, dev)
+    out_idx = _empty_u32(csr.nnz, dev)
+    out_val = torch.empty(csr.nnz, dtype=csr.val.dtype, device=dev)
+    check(lib.csamd_build_compressed_transpose_raw_typed(c_dim, u_dim, csr.nnz, ptr(csr.major_ptr), ptr(csr.minor_idx),
+                                                         ptr(csr.val), csr.value_type, ptr(out_ptr), ptr(out_idx), ptr(out_val),
+                                                         ptr(buf), buf.numel(),
+                                                         stream if stream is not None else current_stream_handle()),
+          "compressed_transpose")
+    return DeviceCSR(csr.rows, csr.cols, out_ptr, out_idx, out_val, csr.value_type, 1 - csr.axis)
+
+
+# ---------------------------------------------------------------------------
+# Bucketing
+# ---------------------------------------------------------------------------
+def build_major_nnz_bucket_plan(csr: DeviceCSR, requested_bucket_count: int, stream: int | None = None):
+    lib = load()
+    n = csr.major_dim
+    dev = csr.minor_idx.device
+    sort_bytes = C.c_size_t(0)
+    check(lib.csamd_major_nnz_bucket_sort_scratch_bytes(n, C.byref(sort_bytes)), "sort scratch bytes")
+    sort_tmp = torch.empty(max(int(sort_bytes.value), 256), dtype=torch.uint8, device=dev)
+    nnz = _empty_u32(n, dev)
+    nnz_sorted = _empty_u32(n, dev)
+    order_in = _empty_u32(n, dev)
+    order = _empty_u32(n, dev)
+    b = int(lib.csamd_clamp_bucket_count(n, requested_bucket_count))
+    offsets = _empty_u32(b + 1, dev)
+    check(lib.csamd_build_major_nnz_bucket_plan_raw(ptr(csr.major_ptr), n, ptr(nnz), ptr(nnz_sorted), ptr(order_in),
+                                                    ptr(order), ptr(offsets), requested_bucket_count, ptr(sort_tmp),
+                                                    sort_tmp.numel(), stream if stream is not None else current_stream_handle()),
+          "build_major_nnz_bucket_plan")
+    return {"major_nnz": nnz, "major_nnz_sorted": nnz_sorted, "major_order": order, "bucket_offsets": offsets,
+            "bucket_count": b}
+
+
+def log2_bucket_offsets(major_nnz_sorted: torch.Tensor, bucket_count: int = 12, min_log2: int = 5,
+                        stream: int | None = None) -> torch.Tensor:
+    offsets = _empty_u32(bucket_count + 1, major_nnz_sorted.device)
+    check(load().csamd_fill_log2_bucket_offsets_raw(ptr(major_nnz_sorted), major_nnz_sorted.numel(), bucket_count, min_log2,
+                                                    ptr(offsets), stream if stream is not None else current_stream_handle()),
+          "fill_log2_bucket_offsets")
+    return offsets
+
+
+def rebuild_compressed_major_order(csr: DeviceCSR, order: torch.Tensor, stream: int | None = None):
+    lib = load()
+    n = csr.major_dim
+    dev = csr.minor_idx.device
+    scan_bytes = C.c_size_t(0)
+    check(lib.csamd_major_nnz_bucket_scan_scratch_bytes(n, C.byref(scan_bytes)), "scan scratch bytes")
+    scan_tmp = torch.empty(max(int(scan_bytes.value), 256), dtype=torch.uint8, device=dev)
+    dptr = _empty_u32(n + 1, dev)
+    didx = _empty_u32(csr.nnz, dev)
+    dval = torch.empty(csr.nnz, dtype=csr.val.dtype, device=dev)
+    inv = _empty_u32(n, dev)
+    check(lib.csamd_rebuild_compressed_major_order_raw(ptr(csr.major_ptr), ptr(csr.minor_idx), ptr(csr.val), csr.value_type, n,
+                                                       ptr(order), ptr(dptr), ptr(didx), ptr(dval), ptr(inv), ptr(scan_tmp),
+                                                       scan_tmp.numel(), stream if stream is not None else current_stream_handle()),
+          "rebuild_compressed_major_order")
+    return DeviceCSR(csr.rows, csr.cols, dptr, didx, dval, csr.value_type, csr.axis), inv
+
+
+def blocked_ell_bucket_plan(bell: DeviceBlockedELL, ws: ConvertWorkspace):
+    dev = bell.val.device
+    n = bell.row_blocks
+    order = _empty_u32(n, dev)
+    widths = _empty_u32(n, dev)
+    keys = torch.empty(max(n, 1), dtype=torch.int64, device=dev)
+    check(load().csamd_blocked_ell_bucket_plan_raw(ptr(bell.block_col_idx), n, bell.width, ptr(order), ptr(widths), ptr(keys),
+                                                   C.byref(ws.ws)), "blocked_ell_bucket_plan")
+    return order, widths, keys[:n]
+
+
+# ---------------------------------------------------------------------------
+# Row-label grouped reduction
+# ---------------------------------------------------------------------------
+class LabelReduce:
+    """Workspace-owning runner of csamd_label_reduce_* (results alias the
+    workspace until the next call, like the reference's reduce results)."""
+
+    def __init__(self, device: int | None = None, stream: int | None = None):
+        lib = load()
+        self.ws = A.label_reduce_workspace()
+        lib.csamd_label_reduce_init(C.byref(self.ws))
+        dev = torch.cuda.current_device() if device is None else device
+        check(lib.csamd_label_reduce_setup(C.byref(self.ws), dev, stream if stream is not None else current_stream_handle()),
+              "label reduce setup")
+
+    def close(self):
+        if self.ws is not None:
+            load().csamd_label_reduce_clear(C.byref(self.ws))
+            self.ws = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, src, labels: torch.Tensor, group_count: int, row_keep: torch.Tensor | None = None,
+            feature_keep: torch.Tensor | None = None, feature_scale: torch.Tensor | None = None, moments: bool = True):
+        q = A.label_reduce_query()
+        q.group_count = group_count
+        q.value_type = src.value_type
+        q.row_labels = ptr(labels)
+        q.row_keep = ptr(row_keep)
+        q.feature_keep = ptr(feature_keep)
+        q.feature_scale = ptr(feature_scale)
+        q.compute_moments = 1 if moments else 0
+        res = A.label_reduce_result()
+        v = src.view()
+        lib = load()
+        if isinstance(src, DeviceCSR):
+            rc = lib.csamd_label_reduce_compressed(C.byref(v), C.byref(q), C.byref(self.ws), C.byref(res))
+        elif isinstance(src, DeviceSlicedELL):
+            rc = lib.csamd_label_reduce_sliced_ell(C.byref(v), C.byref(q), C.byref(self.ws), C.byref(res))
+        elif isinstance(src, DeviceBlockedELL):
+            rc = lib.csamd_label_reduce_blocked_ell(C.byref(v), C.byref(q), C.byref(self.ws), C.byref(res))
+        else:
+            raise TypeError(type(src))
+        check(rc, "label_reduce")
+        return res
+
+    def last_kernel_ms(self) -> float:
+        return float(load().csamd_label_reduce_last_kernel_ms(C.byref(self.ws)))
+
+    @staticmethod
+    def fetch(res, with_moments: bool = True) -> dict:
+        """Copy a result to host numpy arrays (synchronizes)."""
+        G, F = int(res.group_count), int(res.kept_features)
+        dl = device_to_numpy
+        out = {
+            "group_count": G,
+            "kept_features": F,
+            "kept_rows": int(res.kept_rows),
+            "sum": dl(res.sum, G * F, np.float64).reshape(G, F),
+            "sumsq": dl(res.sumsq, G * F, np.float64).reshape(G, F),
+            "nnz": dl(res.nnz, G * F, np.uint32).reshape(G, F),
+            "group_rows": dl(res.group_rows, G, np.uint32),
+            "kept_feature_ids": dl(res.kept_feature_ids, F, np.uint32),
+        }
+        if with_moments and res.mean:
+            out["mean"] = dl(res.mean, G * F, np.float64).reshape(G, F)
+            out["var"] = dl(res.var, G * F, np.float64).reshape(G, F)
+        return out
+
+
+def _copy_d2h(host_u8: torch.Tensor, dev_ptr: int, nbytes: int) -> None:
+    """Copy nbytes from a raw device pointer into a host uint8 tensor."""
+    hip = _hip()
+    rc = hip.hipMemcpy(C.c_void_p(host_u8.data_ptr()), C.c_void_p(dev_ptr), C.c_size_t(nbytes), 2)  # D2H
+    if rc != 0:
+        raise RuntimeError(f"hipMemcpy D2H failed ({rc})")
+
+
+_HIP = None
+
+
+def _hip():
+    global _HIP
+    if _HIP is None:
+        _HIP = C.CDLL("libamdhip64.so")
+        _HIP.hipMemcpy.restype = C.c_int
+        _HIP.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    return _HIP
+
+
+def device_to_numpy(dev_ptr: int, count: int, dtype) -> np.ndarray:
+    """Synchronous copy of `count` elements from a raw device pointer."""
+    nbytes = int(count) * np.dtype(dtype).itemsize
+    if nbytes == 0 or not dev_ptr:
+        return np.zeros(int(count), dtype=dtype)
+    torch.cuda.synchronize()
+    t = torch.empty(nbytes, dtype=torch.uint8)
+    _copy_d2h(t, dev_ptr, nbytes)
+    return t.numpy().view(dtype).copy()
+
+
+# ---------------------------------------------------------------------------
+# Synthetic generator
+# ---------------------------------------------------------------------------
+def column_tables(cols: int, seed: int, zipf_s: float = 1.1, mu_sigma: float = 1.5):
+    """Per-column Zipf weights over a seeded permutation and NB means.
+    Returns (inverse weights, means) as float32 numpy arrays."""
+    rng = np.random.default_rng(seed)
+    rank = rng.permutation(cols).astype(np.float64)   # rank of column c
+    w = (rank + 1.0) ** (-zipf_s)
+    w = w / w.sum()
+    mu = rng.lognormal(mean=0.0, sigma=mu_sigma, size=cols)
+    return (1.0 / w).astype(np.float32), mu.astype(np.float32)
+
+
+def synth_csr(rows: int, cols: int, median: float, sigma: float, seed: int, value_type: int = A.VALUE_F16,
+              nb_alpha: float = 0.5, zipf_s: float = 1.1, mu_sigma: float = 1.5, ws: ConvertWorkspace | None = None,
+              device=None) -> DeviceCSR:
+    """Generate a count CSR on the device (see csamd_synth_params)."""
+    lib = load()
+    dev = device or default_device()
+    inv_w, mu = column_tables(cols, seed, zipf_s, mu_sigma)
+    d_inv_w = to_dev_f32(inv_w, dev)
+    d_mu = to_dev_f32(mu, dev)
+    p = A.synth_params()
+    p.rows, p.cols, p.value_type, p.seed = rows, cols, value_type, seed
+    p.row_nnz_median, p.row_nnz_sigma, p.nb_alpha = median, sigma, nb_alpha
+    p.d_col_inv_weight, p.d_col_mean = ptr(d_inv_w), ptr(d_mu)
+    own_ws = ws is None
+    ws = ws or ConvertWorkspace()
+    stream = ws.ws.stream
+    row_nnz = _empty_u32(rows, dev)
+    check(lib.csamd_synth_row_nnz(C.byref(p), ptr(row_nnz), stream), "synth_row_nnz")
+    d_ptr = _empty_u32(rows + 1, dev)
+    total = C.c_uint64(0)
+    check(lib.csamd_exclusive_scan_u32(ptr(row_nnz), rows, ptr(d_ptr), C.byref(ws.ws), C.byref(total)), "synth scan")
+    nnz = int(total.value)
+    idx = _empty_u32(nnz, dev)
+    val = torch.empty(nnz, dtype=torch_bits_dtype(value_type), device=dev)
+    check(lib.csamd_synth_fill(C.byref(p), ptr(d_ptr), ptr(idx), ptr(val), stream), "synth_fill")
+    torch.cuda.synchronize()
+    if own_ws:
+        ws.close()
+    del d_inv_w, d_mu, row_nnz
+    return DeviceCSR(rows, cols, d_ptr, idx, val, value_type, A.COMPRESSED_BY_ROW)
+
+
+def log1p_normalize(counts: DeviceCSR, value_type: int, scale: float = 1e4, ws: ConvertWorkspace | None = None):
+    """log1p(count / row_total * scale) stored as value_type; for FP8 e4m3 also
+    returns the per-feature absmax scale (decoded value * scale[c])."""
+    if counts.value_type != A.VALUE_F32:
+        raise ValueError("log1p_normalize expects f32 counts")
+    lib = load()
+    dev = counts.minor_idx.device
+    own = ws is None
+    ws = ws or ConvertWorkspace()
+    out = torch.empty(counts.nnz, dtype=torch_bits_dtype(value_type), device=dev)
+    fscale = torch.empty(counts.cols, dtype=torch.float32, device=dev) if value_type == A.VALUE_FP8_E4M3 else None
+    check(lib.csamd_synth_log1p_normalize(counts.rows, counts.cols, ptr(counts.major_ptr), ptr(counts.minor_idx), ptr(counts.val),
+                                          scale, value_type, ptr(out), ptr(fscale), C.byref(ws.ws)), "log1p_normalize")
+    if own:
+        ws.close()
+    return DeviceCSR(counts.rows, counts.cols, counts.major_ptr, counts.minor_idx, out, value_type, counts.axis), fscale

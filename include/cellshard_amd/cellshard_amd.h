@@ -1,0 +1,580 @@
+/*
+ * cellshard_amd C ABI.
+ *
+ * MI355X (gfx950) implementation of CellShard's sparse execution hot path:
+ *   - masked grouped reductions (runtime/mask_groups.cuh call signatures),
+ *   - the row-label grouped {sum, sumsq, nnz} reduction with mean/variance,
+ *   - CSR -> Blocked-ELL / Sliced-ELL conversion, Blocked-ELL -> Sliced-ELL,
+ *     CSR -> CSC transpose,
+ *   - major-axis nnz bucketing (stable nnz order, equal-count and log2 cuts).
+ *
+ * Every entry point is plain C: POD structs, device pointers, sizes, and a
+ * HIP stream handle.  Return convention follows the reference: 1 = ok, 0 =
+ * failure; the failure text is available from csamd_last_error() (thread
+ * local).  Nothing here throws.
+ *
+ * All struct layouts below are field-for-field the reference layouts
+ * (alignas(16), LP64), with cudaStream_t -> hipStream_t and __half -> the same
+ * 2-byte storage, so a caller's existing structs can be passed through.
+ * Each declaration names the reference interface it replaces (file:line,
+ * paths relative to the CellShard source tree).
+ */
+#ifndef CELLSHARD_AMD_H
+#define CELLSHARD_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CSAMD_ALIGN16 __attribute__((aligned(16)))
+
+/* Same handle type as hipStream_t (hip_runtime_api.h: typedef struct ihipStream_t *hipStream_t). */
+typedef struct ihipStream_t *csamd_stream_t;
+
+/* IEEE binary16 storage, bit-identical to __half. */
+typedef uint16_t csamd_half;
+
+/* Value type codes: include/CellShard/core/real.cuh:44-51. */
+enum {
+    CSAMD_VALUE_F16 = 3,
+    CSAMD_VALUE_F32 = 4,
+    CSAMD_VALUE_F64 = 5,
+    CSAMD_VALUE_BF16 = 6,
+    CSAMD_VALUE_FP8_E4M3 = 7,
+    CSAMD_VALUE_FP8_E5M2 = 8
+};
+
+enum {
+    CSAMD_COMPRESSED_BY_ROW = 0, /* formats/compressed.cuh:11-14 */
+    CSAMD_COMPRESSED_BY_COL = 1,
+    CSAMD_INVALID_COL = 0xffffffffu /* formats/blocked_ell.cuh:11-14, sliced_ell.cuh:12-15 */
+};
+
+#define CSAMD_MAX_MASK_GROUPS 32u /* runtime/mask_groups.cuh:14 */
+#define CSAMD_MAX_LABEL_GROUPS 4096u
+
+/* ---------------------------------------------------------------------------
+ * Device views: runtime/device/sharded_device.cuh:67-98.
+ * The value pointer is declared as csamd_half*; the *_typed entry points
+ * reinterpret it according to an explicit value type code.
+ * ------------------------------------------------------------------------- */
+typedef struct CSAMD_ALIGN16 csamd_compressed_view {
+    unsigned int rows;
+    unsigned int cols;
+    unsigned int nnz;
+    unsigned int axis;
+    unsigned int *majorPtr;
+    unsigned int *minorIdx;
+    csamd_half *val;
+} csamd_compressed_view;
+
+typedef struct CSAMD_ALIGN16 csamd_blocked_ell_view {
+    unsigned int rows;
+    unsigned int cols;
+    unsigned int nnz;
+    unsigned int block_size;
+    unsigned int ell_cols;
+    unsigned int *blockColIdx;
+    csamd_half *val;
+} csamd_blocked_ell_view;
+
+typedef struct CSAMD_ALIGN16 csamd_sliced_ell_view {
+    unsigned int rows;
+    unsigned int cols;
+    unsigned int nnz;
+    unsigned int slice_count;
+    unsigned int slice_rows;
+    unsigned int *slice_row_offsets;
+    unsigned int *slice_widths;
+    unsigned int *slice_slot_offsets;
+    unsigned int *col_idx;
+    csamd_half *val;
+} csamd_sliced_ell_view;
+
+/* ---------------------------------------------------------------------------
+ * mask_groups types: runtime/mask_groups.cuh:16-154.
+ * ------------------------------------------------------------------------- */
+typedef enum csamd_sparse_mask_layout {
+    CSAMD_LAYOUT_UNKNOWN = 0,
+    CSAMD_LAYOUT_BLOCKED_ELL = 1,
+    CSAMD_LAYOUT_SLICED_ELL = 2,
+    CSAMD_LAYOUT_COMPRESSED = 3
+} csamd_sparse_mask_layout;
+
+typedef struct CSAMD_ALIGN16 csamd_group_mask_config_view {
+    unsigned int group_count;
+    const char *const *group_names;
+    const uint32_t *feature_group_masks;
+} csamd_group_mask_config_view;
+
+typedef struct CSAMD_ALIGN16 csamd_row_feature_mask_view {
+    const unsigned char *row_keep;
+    const unsigned char *feature_keep;
+    const unsigned int *row_remap;
+    const unsigned int *feature_remap;
+} csamd_row_feature_mask_view;
+
+typedef struct CSAMD_ALIGN16 csamd_sparse_group_filter_params {
+    float min_total;
+    unsigned int min_detected_features;
+    float max_group_fraction;
+    unsigned int fraction_group_index;
+} csamd_sparse_group_filter_params;
+
+typedef struct CSAMD_ALIGN16 csamd_sparse_group_reduce_result {
+    unsigned int rows;
+    unsigned int group_count;
+    float *row_totals;
+    unsigned int *detected_features;
+    float *max_values;
+    unsigned char *row_keep;
+    float *group_counts;
+    float *group_percentages;
+} csamd_sparse_group_reduce_result;
+
+typedef struct CSAMD_ALIGN16 csamd_sparse_group_reduce_workspace {
+    int device;
+    csamd_stream_t stream;
+    int owns_stream;
+
+    unsigned int rows_capacity;
+    unsigned int cols_capacity;
+    unsigned int group_capacity;
+    unsigned int values_capacity;
+
+    void *row_block;
+    void *feature_block;
+    void *scratch_block;
+    size_t scratch_bytes;
+
+    float *row_totals;
+    unsigned int *detected_features;
+    float *max_values;
+    unsigned char *row_keep;
+    float *group_counts;
+    float *group_percentages;
+    uint32_t *feature_group_masks;
+} csamd_sparse_group_reduce_workspace;
+
+/* One-GPU fleet: the reference fleet (mask_groups.cuh:78-86) holds a
+ * distributed::local_context over every visible GPU; on this build the shard
+ * set of one GPU is one slot, so the context collapses to the slot table. */
+typedef struct CSAMD_ALIGN16 csamd_sparse_group_reduce_fleet_workspace {
+    unsigned int slot_count;
+    int *device_ids;
+    unsigned int *slots;
+    csamd_sparse_group_reduce_workspace *devices;
+    csamd_sparse_group_reduce_result *results;
+} csamd_sparse_group_reduce_fleet_workspace;
+
+typedef struct CSAMD_ALIGN16 csamd_sparse_group_reduce_fleet_result {
+    unsigned int slot_count;
+    unsigned int leader_index;
+    csamd_sparse_group_reduce_result *slot_results;
+} csamd_sparse_group_reduce_fleet_result;
+
+typedef struct CSAMD_ALIGN16 csamd_sparse_group_reduce_fleet_config {
+    const int *device_ids;
+    unsigned int device_count;
+    unsigned int enable_peer_access;
+    unsigned int stream_flags;
+    const void *ranked_nccl; /* must be NULL: no collectives on one GPU */
+} csamd_sparse_group_reduce_fleet_config;
+
+typedef struct CSAMD_ALIGN16 csamd_masked_sparse_reoptimize_config {
+    unsigned int output_rows;
+    unsigned int output_cols;
+    unsigned int requested_bucket_count;
+    unsigned int output_block_size;
+    csamd_sparse_mask_layout requested_layout;
+} csamd_masked_sparse_reoptimize_config;
+
+typedef struct CSAMD_ALIGN16 csamd_masked_sparse_reoptimize_metadata {
+    unsigned int kept_rows;
+    unsigned int kept_features;
+    unsigned int live_nnz;
+    unsigned long long bytes_before;
+    unsigned long long bytes_after;
+    csamd_sparse_mask_layout output_layout;
+} csamd_masked_sparse_reoptimize_metadata;
+
+typedef struct CSAMD_ALIGN16 csamd_masked_sparse_reoptimize_result {
+    csamd_compressed_view compressed;
+    csamd_blocked_ell_view blocked_ell;
+    csamd_masked_sparse_reoptimize_metadata metadata;
+} csamd_masked_sparse_reoptimize_result;
+
+typedef struct CSAMD_ALIGN16 csamd_masked_sparse_reoptimize_workspace {
+    int device;
+    csamd_stream_t stream;
+    int owns_stream;
+    void *private_workspace;
+} csamd_masked_sparse_reoptimize_workspace;
+
+typedef struct CSAMD_ALIGN16 csamd_auto_reoptimize_prediction_input {
+    float density_change;
+    float row_drop_ratio;
+    float feature_drop_ratio;
+    float layout_padding_change;
+    float estimated_rebuild_cost_ms;
+    unsigned int expected_reuse_count;
+} csamd_auto_reoptimize_prediction_input;
+
+typedef struct CSAMD_ALIGN16 csamd_auto_reoptimize_prediction {
+    unsigned int should_reoptimize;
+    float estimated_net_gain_ms;
+} csamd_auto_reoptimize_prediction;
+
+/* ---------------------------------------------------------------------------
+ * Errors and version.
+ * ------------------------------------------------------------------------- */
+/* Last failure message of the calling thread ("" when none).  Replaces the
+ * reference's stderr-only reporting (mask_groups.cu:44-48). */
+const char *csamd_last_error(void);
+int csamd_abi_version(void);
+/* Number of visible HIP devices (0 on a host without a GPU; never fails). */
+int csamd_device_count(void);
+
+/* ---------------------------------------------------------------------------
+ * mask_groups lifecycle and reductions (reference ABI).
+ * ------------------------------------------------------------------------- */
+/* runtime/mask_groups.cuh:198-202 (init), src/runtime/mask_groups.cu:534-538 */
+void csamd_group_reduce_init(csamd_sparse_group_reduce_workspace *workspace);
+/* mask_groups.cuh:202 (clear), mask_groups.cu:555-563 */
+void csamd_group_reduce_clear(csamd_sparse_group_reduce_workspace *workspace);
+/* mask_groups.cuh:206 (setup), mask_groups.cu:604-618 */
+int csamd_group_reduce_setup(csamd_sparse_group_reduce_workspace *workspace, int device, csamd_stream_t stream);
+/* mask_groups.cuh:211-215 (reserve), mask_groups.cu:723-788 */
+int csamd_group_reduce_reserve(csamd_sparse_group_reduce_workspace *workspace,
+                               unsigned int rows, unsigned int cols, unsigned int values, unsigned int group_count);
+/* mask_groups.cuh:217-219, mask_groups.cu:790-809 */
+int csamd_upload_feature_group_masks(csamd_sparse_group_reduce_workspace *workspace,
+                                     unsigned int cols, const uint32_t *host_masks);
+/* mask_groups.cuh:221-226 (BELL overload), mask_groups.cu:811-845 */
+int csamd_compute_sparse_group_reduce_blocked_ell(const csamd_blocked_ell_view *src,
+                                                  csamd_sparse_group_reduce_workspace *workspace,
+                                                  const csamd_group_mask_config_view *groups,
+                                                  const csamd_row_feature_mask_view *masks,
+                                                  const csamd_sparse_group_filter_params *filter,
+                                                  csamd_sparse_group_reduce_result *out);
+/* mask_groups.cuh:228-233 (SELL overload), mask_groups.cu:847-881 */
+int csamd_compute_sparse_group_reduce_sliced_ell(const csamd_sliced_ell_view *src,
+                                                 csamd_sparse_group_reduce_workspace *workspace,
+                                                 const csamd_group_mask_config_view *groups,
+                                                 const csamd_row_feature_mask_view *masks,
+                                                 const csamd_sparse_group_filter_params *filter,
+                                                 csamd_sparse_group_reduce_result *out);
+/* mask_groups.cuh:235-240 (compressed fallback), mask_groups.cu:883-918 */
+int csamd_compute_sparse_group_reduce_compressed(const csamd_compressed_view *src,
+                                                 csamd_sparse_group_reduce_workspace *workspace,
+                                                 const csamd_group_mask_config_view *groups,
+                                                 const csamd_row_feature_mask_view *masks,
+                                                 const csamd_sparse_group_filter_params *filter,
+                                                 csamd_sparse_group_reduce_result *out);
+/* Same three reductions with an explicit value type for src->val
+ * (f16, bf16, f32, fp8-e4m3).  Extension: the reference reads __half only. */
+int csamd_compute_sparse_group_reduce_compressed_typed(const csamd_compressed_view *src, unsigned int value_type,
+                                                       csamd_sparse_group_reduce_workspace *workspace,
+                                                       const csamd_group_mask_config_view *groups,
+                                                       const csamd_row_feature_mask_view *masks,
+                                                       const csamd_sparse_group_filter_params *filter,
+                                                       csamd_sparse_group_reduce_result *out);
+int csamd_compute_sparse_group_reduce_blocked_ell_typed(const csamd_blocked_ell_view *src, unsigned int value_type,
+                                                        csamd_sparse_group_reduce_workspace *workspace,
+                                                        const csamd_group_mask_config_view *groups,
+                                                        const csamd_row_feature_mask_view *masks,
+                                                        const csamd_sparse_group_filter_params *filter,
+                                                        csamd_sparse_group_reduce_result *out);
+int csamd_compute_sparse_group_reduce_sliced_ell_typed(const csamd_sliced_ell_view *src, unsigned int value_type,
+                                                       csamd_sparse_group_reduce_workspace *workspace,
+                                                       const csamd_group_mask_config_view *groups,
+                                                       const csamd_row_feature_mask_view *masks,
+                                                       const csamd_sparse_group_filter_params *filter,
+                                                       csamd_sparse_group_reduce_result *out);
+
+/* Fleet, collapsed to the shard set of one GPU.
+ * mask_groups.cuh:199,203,207-208 and mask_groups.cu:540-547,565-588,620-705 */
+void csamd_fleet_init(csamd_sparse_group_reduce_fleet_workspace *fleet);
+void csamd_fleet_clear(csamd_sparse_group_reduce_fleet_workspace *fleet);
+int csamd_setup_fleet(csamd_sparse_group_reduce_fleet_workspace *fleet,
+                      const csamd_sparse_group_reduce_fleet_config *config);
+/* mask_groups.cuh:242-254, mask_groups.cu:920-958 */
+int csamd_compute_sparse_group_reduce_fleet_blocked_ell(const csamd_blocked_ell_view *src_by_slot,
+                                                        csamd_sparse_group_reduce_fleet_workspace *fleet,
+                                                        const csamd_group_mask_config_view *groups,
+                                                        const csamd_row_feature_mask_view *masks,
+                                                        const csamd_sparse_group_filter_params *filter,
+                                                        csamd_sparse_group_reduce_fleet_result *out);
+int csamd_compute_sparse_group_reduce_fleet_sliced_ell(const csamd_sliced_ell_view *src_by_slot,
+                                                       csamd_sparse_group_reduce_fleet_workspace *fleet,
+                                                       const csamd_group_mask_config_view *groups,
+                                                       const csamd_row_feature_mask_view *masks,
+                                                       const csamd_sparse_group_filter_params *filter,
+                                                       csamd_sparse_group_reduce_fleet_result *out);
+
+/* Masked Blocked-ELL -> CSR compaction.
+ * mask_groups.cuh:195-196,200,204,209,256-260 and mask_groups.cu:526-532,548-552,590-602,706-721,960-1034 */
+int csamd_predict_masked_sparse_reoptimization(const csamd_auto_reoptimize_prediction_input *input,
+                                               csamd_auto_reoptimize_prediction *out);
+void csamd_reoptimize_init(csamd_masked_sparse_reoptimize_workspace *workspace);
+void csamd_reoptimize_clear(csamd_masked_sparse_reoptimize_workspace *workspace);
+int csamd_reoptimize_setup(csamd_masked_sparse_reoptimize_workspace *workspace, int device, csamd_stream_t stream);
+int csamd_manual_reoptimize_masked_sparse(const csamd_blocked_ell_view *src,
+                                          const csamd_row_feature_mask_view *masks,
+                                          const csamd_masked_sparse_reoptimize_config *config,
+                                          csamd_masked_sparse_reoptimize_workspace *workspace,
+                                          csamd_masked_sparse_reoptimize_result *out);
+
+/* ---------------------------------------------------------------------------
+ * Row-label grouped reduction (project extension; no reference counterpart:
+ * the reference reduces per row with <= 32 feature-group bitmasks).
+ *
+ * For every kept row r (row_keep[r] != 0 and row_labels[r] < group_count) and
+ * every kept feature c (feature_keep[c] != 0), with f = rank of c among the
+ * kept features in ascending column order:
+ *   sum[g][f]   += v(r,c)          g = row_labels[r]
+ *   sumsq[g][f] += v(r,c)^2
+ *   nnz[g][f]   += (v(r,c) != 0)
+ * Stored zeros and invalid/out-of-range columns contribute nothing.  Partials
+ * are combined in FP64; mean = sum/n_g and var = sumsq/n_g - mean^2 (ddof 0),
+ * n_g = number of kept rows with label g.
+ * ------------------------------------------------------------------------- */
+typedef struct CSAMD_ALIGN16 csamd_label_reduce_query {
+    unsigned int group_count;          /* G, 1..CSAMD_MAX_LABEL_GROUPS */
+    unsigned int value_type;           /* CSAMD_VALUE_* of src->val */
+    const unsigned int *row_labels;    /* device [rows]; label >= G drops the row */
+    const unsigned char *row_keep;     /* device [rows] or NULL (all rows) */
+    const unsigned char *feature_keep; /* device [cols] or NULL (all features) */
+    const float *feature_scale;        /* device [cols] or NULL; multiplies decoded values (FP8 absmax scale) */
+    unsigned int compute_moments;      /* 1: also fill mean/var */
+    unsigned int reserved;
+} csamd_label_reduce_query;
+
+typedef struct CSAMD_ALIGN16 csamd_label_reduce_result {
+    unsigned int group_count;
+    unsigned int kept_features;        /* F: width of every [G x F] output */
+    unsigned int kept_rows;
+    unsigned int reserved;
+    double *sum;                       /* device [G x F] */
+    double *sumsq;                     /* device [G x F] */
+    unsigned int *nnz;                 /* device [G x F] */
+    unsigned int *group_rows;          /* device [G] */
+    double *mean;                      /* device [G x F] (compute_moments) */
+    double *var;                       /* device [G x F] (compute_moments) */
+    unsigned int *kept_feature_ids;    /* device [F]: column of output feature f */
+} csamd_label_reduce_result;
+
+typedef struct CSAMD_ALIGN16 csamd_label_reduce_workspace {
+    int device;
+    csamd_stream_t stream;
+    int owns_stream;
+    void *private_workspace;
+} csamd_label_reduce_workspace;
+
+void csamd_label_reduce_init(csamd_label_reduce_workspace *workspace);
+void csamd_label_reduce_clear(csamd_label_reduce_workspace *workspace);
+int csamd_label_reduce_setup(csamd_label_reduce_workspace *workspace, int device, csamd_stream_t stream);
+int csamd_label_reduce_compressed(const csamd_compressed_view *src, const csamd_label_reduce_query *query,
+                                  csamd_label_reduce_workspace *workspace, csamd_label_reduce_result *out);
+int csamd_label_reduce_sliced_ell(const csamd_sliced_ell_view *src, const csamd_label_reduce_query *query,
+                                  csamd_label_reduce_workspace *workspace, csamd_label_reduce_result *out);
+int csamd_label_reduce_blocked_ell(const csamd_blocked_ell_view *src, const csamd_label_reduce_query *query,
+                                   csamd_label_reduce_workspace *workspace, csamd_label_reduce_result *out);
+/* Mean of the last call's main reduce kernel duration on the workspace
+ * stream, in milliseconds (HIP events around that launch).  0 if none. */
+float csamd_label_reduce_last_kernel_ms(const csamd_label_reduce_workspace *workspace);
+
+/* ---------------------------------------------------------------------------
+ * Conversion workspace (grow-only scratch + stream), shared by converters,
+ * transpose and bucketing entry points that need temporary device memory.
+ * ------------------------------------------------------------------------- */
+typedef struct CSAMD_ALIGN16 csamd_convert_workspace {
+    int device;
+    csamd_stream_t stream;
+    int owns_stream;
+    void *scratch;
+    size_t scratch_bytes;
+} csamd_convert_workspace;
+
+void csamd_convert_init(csamd_convert_workspace *ws);
+void csamd_convert_clear(csamd_convert_workspace *ws);
+int csamd_convert_setup(csamd_convert_workspace *ws, int device, csamd_stream_t stream);
+
+/* convert/blocked_ell_from_compressed.cuh:16-20 */
+typedef struct CSAMD_ALIGN16 csamd_blocked_ell_tune_result {
+    unsigned int block_size;
+    double fill_ratio;
+    size_t padded_bytes;
+} csamd_blocked_ell_tune_result;
+
+/* convert/sliced_ell_from_blocked_ell.cuh:18-26 */
+typedef struct CSAMD_ALIGN16 csamd_sliced_ell_tune_result {
+    unsigned int slice_rows;
+    unsigned int slice_count;
+    size_t padded_slots;
+    size_t padded_bytes;
+    double storage_ratio;
+    double slice_penalty;
+    double score;
+} csamd_sliced_ell_tune_result;
+
+/* Blocked-ELL shape of a device CSR (row axis): W = max unique column blocks
+ * per row block, total = sum over row blocks.  Synchronous.
+ * convert/blocked_ell_from_compressed.cuh:32-91 (count_blocked_ell_shape_) */
+int csamd_blocked_ell_shape(const csamd_compressed_view *src, unsigned int block_size,
+                            csamd_convert_workspace *ws, unsigned int *out_ell_width,
+                            unsigned long long *out_total_blocks, double *out_fill_ratio);
+/* convert/blocked_ell_from_compressed.cuh:157-190; padded bytes priced at
+ * value_bytes per stored value (2 in the reference). */
+int csamd_choose_blocked_ell_block_size(const csamd_compressed_view *src, const unsigned int *candidates,
+                                        unsigned int candidate_count, unsigned int value_bytes,
+                                        csamd_convert_workspace *ws, csamd_blocked_ell_tune_result *out);
+/* convert/blocked_ell_from_compressed.cuh:258-347.  dst->block_size and
+ * dst->ell_cols (= W * block_size from csamd_blocked_ell_shape) must be set,
+ * dst->blockColIdx [row_blocks x W] and dst->val [rows x ell_cols] allocated
+ * by the caller.  Writes both, sets dst->nnz (count of nonzero stored
+ * values; duplicate (row, col) keeps the last value).  Synchronous. */
+int csamd_blocked_ell_from_compressed(const csamd_compressed_view *src, unsigned int value_type,
+                                      csamd_blocked_ell_view *dst, csamd_convert_workspace *ws);
+
+/* Uniform Sliced-ELL layout of a device CSR/CSC (axis honoured):
+ * slice_row_offsets [S+1], slice_widths [S], slice_slot_offsets [S]
+ * (device, caller-allocated, S = ceil(rows / clamp(slice_rows,1,rows))).
+ * Returns the total slot count (fails if it does not fit u32, as the
+ * reference's slot offsets are u32).  Synchronous.
+ * convert/sliced_ell_from_blocked_ell.cuh:111-142 and
+ * runtime/device/sharded_device.cuh:609-620 (slot offsets) */
+int csamd_sliced_ell_layout(const csamd_compressed_view *src, unsigned int slice_rows,
+                            unsigned int *d_slice_row_offsets, unsigned int *d_slice_widths,
+                            unsigned int *d_slice_slot_offsets, csamd_convert_workspace *ws,
+                            unsigned int *out_slice_count, unsigned long long *out_total_slots);
+/* Same layout for a device Blocked-ELL source (row nnz = nonzero stored
+ * values at valid columns).  convert/sliced_ell_from_blocked_ell.cuh:36-62 */
+int csamd_sliced_ell_layout_from_blocked_ell(const csamd_blocked_ell_view *src, unsigned int value_type,
+                                             unsigned int slice_rows, unsigned int *d_slice_row_offsets,
+                                             unsigned int *d_slice_widths, unsigned int *d_slice_slot_offsets,
+                                             csamd_convert_workspace *ws, unsigned int *out_slice_count,
+                                             unsigned long long *out_total_slots);
+/* convert/sliced_ell_from_blocked_ell.cuh:196-221 (choose_sliced_ell_slice_rows)
+ * over the row nnz of a device CSR (sliced_ell_from_compressed.cuh:86-94). */
+int csamd_choose_sliced_ell_slice_rows(const csamd_compressed_view *src, const unsigned int *candidates,
+                                       unsigned int candidate_count, csamd_convert_workspace *ws,
+                                       csamd_sliced_ell_tune_result *out);
+/* convert/sliced_ell_from_compressed.cuh:96-164 (CSR and CSC sources).
+ * dst layout arrays must hold the csamd_sliced_ell_layout result; col_idx
+ * and val [total_slots] caller-allocated.  Padding is written as
+ * CSAMD_INVALID_COL / 0.  Sets dst->nnz = src->nnz.  Asynchronous. */
+int csamd_sliced_ell_from_compressed(const csamd_compressed_view *src, unsigned int value_type,
+                                     csamd_sliced_ell_view *dst, csamd_convert_workspace *ws);
+/* convert/sliced_ell_from_blocked_ell.cuh:233-277.  Asynchronous. */
+int csamd_sliced_ell_from_blocked_ell(const csamd_blocked_ell_view *src, unsigned int value_type,
+                                      csamd_sliced_ell_view *dst, csamd_convert_workspace *ws);
+
+/* CSR -> CSC transpose (or CSC -> CSR): out_ptr[uDim+1], out_idx, out_val.
+ * Canonical order: ascending compressed-axis index inside every output
+ * segment (what cuSPARSE csr2cscEx2 and scipy produce).
+ * convert/compressed_transpose_raw.cuh:14-122 */
+size_t csamd_compressed_transpose_scratch_bytes(unsigned int cDim, unsigned int uDim, unsigned int nnz);
+int csamd_build_compressed_transpose_raw(unsigned int cDim, unsigned int uDim, unsigned int nnz,
+                                         const unsigned int *d_cAxPtr, const unsigned int *d_uAxIdx,
+                                         const csamd_half *d_val, unsigned int *d_out_uAxPtr,
+                                         unsigned int *d_heads, unsigned int *d_out_cAxIdx,
+                                         csamd_half *d_out_val, void *d_scan_tmp, size_t scan_bytes,
+                                         csamd_stream_t stream);
+int csamd_build_compressed_transpose_raw_typed(unsigned int cDim, unsigned int uDim, unsigned int nnz,
+                                               const unsigned int *d_cAxPtr, const unsigned int *d_uAxIdx,
+                                               const void *d_val, unsigned int value_type,
+                                               unsigned int *d_out_uAxPtr, unsigned int *d_out_cAxIdx,
+                                               void *d_out_val, void *d_scan_tmp, size_t scan_bytes,
+                                               csamd_stream_t stream);
+
+/* Major-axis nnz bucketing.  bucket/major_nnz_raw.cuh:12-19 (plan view) */
+typedef struct CSAMD_ALIGN16 csamd_major_nnz_bucket_plan_view {
+    unsigned int major_dim;
+    unsigned int bucket_count;
+    const unsigned int *major_order;
+    const unsigned int *major_nnz_sorted;
+    const unsigned int *bucket_offsets;
+    const unsigned int *inverse_major_order;
+} csamd_major_nnz_bucket_plan_view;
+
+/* bucket/major_nnz_raw.cuh:27-32 */
+unsigned int csamd_clamp_bucket_count(unsigned int major_dim, unsigned int requested);
+/* bucket/major_nnz_raw.cuh:34-48 / 50-62 */
+int csamd_major_nnz_bucket_sort_scratch_bytes(unsigned int major_dim, size_t *out_bytes);
+int csamd_major_nnz_bucket_scan_scratch_bytes(unsigned int major_dim, size_t *out_bytes);
+/* bucket/major_nnz_raw.cuh:64-120: nnz per major, stable ascending order,
+ * equal-count offsets [bucket_count + 1]. */
+int csamd_build_major_nnz_bucket_plan_raw(const unsigned int *d_major_ptr, unsigned int major_dim,
+                                          unsigned int *d_major_nnz, unsigned int *d_major_nnz_sorted,
+                                          unsigned int *d_major_order_in, unsigned int *d_major_order_out,
+                                          unsigned int *d_bucket_offsets, unsigned int requested_bucket_count,
+                                          void *d_sort_tmp, size_t sort_tmp_bytes, csamd_stream_t stream);
+/* log2 nnz buckets over an ascending major_nnz_sorted: bucket of n is
+ * clamp(bit_width(n) - min_log2 - 1 + 1, 0, bucket_count - 1) with
+ * bucket 0 = n < 2^(min_log2); offsets [bucket_count + 1].  Project rule. */
+int csamd_fill_log2_bucket_offsets_raw(const unsigned int *d_major_nnz_sorted, unsigned int major_dim,
+                                       unsigned int bucket_count, unsigned int min_log2,
+                                       unsigned int *d_bucket_offsets, csamd_stream_t stream);
+/* bucket/major_nnz_raw.cuh:188-250 */
+int csamd_rebuild_compressed_major_order_raw(const unsigned int *d_src_major_ptr, const unsigned int *d_src_minor_idx,
+                                             const void *d_src_val, unsigned int value_type, unsigned int major_dim,
+                                             const unsigned int *d_major_order, unsigned int *d_dst_major_ptr,
+                                             unsigned int *d_dst_minor_idx, void *d_dst_val,
+                                             unsigned int *d_inverse_major_order, void *d_scan_tmp,
+                                             size_t scan_tmp_bytes, csamd_stream_t stream);
+
+/* Blocked-ELL row-block bucket plan: key = (valid slots << 32) | hash of the
+ * valid block columns, stable ascending order.  Writes order and the sorted
+ * widths (device, [row_block_count]).  bucket/operators/blocked_ell_bucket.cuh:77-96,149-210 */
+int csamd_blocked_ell_bucket_plan_raw(const unsigned int *d_block_col_idx, unsigned int row_block_count,
+                                      unsigned int width_blocks, unsigned int *d_order_out,
+                                      unsigned int *d_width_sorted, unsigned long long *d_keys_sorted,
+                                      csamd_convert_workspace *ws);
+
+/* ---------------------------------------------------------------------------
+ * Synthetic CSR generator for measurement and tests (not a reference path).
+ * Rows: nnz ~ round(LogNormal(ln median, sigma)) capped at cols; columns
+ * drawn without replacement with weights w_c (Efraimidis-Spirakis keys
+ * log(u)/w_c, top k, emitted ascending); values ~ zero-truncated NB2(mean
+ * col_mean[c], alpha) counts, stored as value_type (f16 or f32).
+ * Deterministic in seed.
+ * ------------------------------------------------------------------------- */
+typedef struct CSAMD_ALIGN16 csamd_synth_params {
+    unsigned int rows;
+    unsigned int cols;
+    unsigned int value_type;
+    unsigned int seed;
+    float row_nnz_median;
+    float row_nnz_sigma;
+    float nb_alpha;
+    unsigned int reserved;
+    const float *d_col_inv_weight; /* [cols]: 1 / w_c */
+    const float *d_col_mean;       /* [cols] */
+} csamd_synth_params;
+
+/* Pass 1: row nnz into d_row_nnz [rows]. */
+int csamd_synth_row_nnz(const csamd_synth_params *p, unsigned int *d_row_nnz, csamd_stream_t stream);
+/* Exclusive scan of d_counts [n] into d_ptr [n+1] (u32; fails with an error
+ * if the total overflows u32).  Synchronous; returns the total. */
+int csamd_exclusive_scan_u32(const unsigned int *d_counts, unsigned int n, unsigned int *d_ptr,
+                             csamd_convert_workspace *ws, unsigned long long *out_total);
+/* Pass 2: fill minor indices and values of every row given d_ptr [rows+1]. */
+int csamd_synth_fill(const csamd_synth_params *p, const unsigned int *d_ptr, unsigned int *d_idx,
+                     void *d_val, csamd_stream_t stream);
+/* log1p(count / row_total * scale) transform of fp32 counts -> value_type
+ * (bf16 or fp8 e4m3 with per-feature absmax scale written to d_feature_scale
+ * [cols] when value_type is fp8; f32/f16/bf16 otherwise). */
+int csamd_synth_log1p_normalize(unsigned int rows, unsigned int cols, const unsigned int *d_ptr,
+                                const unsigned int *d_idx, const float *d_counts, float scale,
+                                unsigned int value_type, void *d_val_out, float *d_feature_scale,
+                                csamd_convert_workspace *ws);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CELLSHARD_AMD_H */

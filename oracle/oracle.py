@@ -1,0 +1,387 @@
+"""numpy front end of the CPU oracle (oracle.c) and of the reference CPU
+converter built under oracle/_ref.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, always as the checker / CPU baseline.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "liboracle.so"
+REF_PATH = HERE / "_ref" / "libref_convert.so"
+
+VT_F16, VT_F32, VT_BF16, VT_FP8 = 3, 4, 6, 7
+VB = {VT_F16: 2, VT_F32: 4, VT_BF16: 2, VT_FP8: 1}
+INVALID = 0xFFFFFFFF
+
+_lib = None
+_ref = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} missing; run __graft_entry__.build()")
+        _lib = C.CDLL(str(LIB_PATH))
+        _lib.oracle_half_to_float.restype = C.c_float
+        _lib.oracle_half_to_float.argtypes = [C.c_uint16]
+        _lib.oracle_fp8_e4m3_to_float.restype = C.c_float
+        _lib.oracle_fp8_e4m3_to_float.argtypes = [C.c_uint8]
+        _lib.oracle_kept_features.restype = C.c_uint32
+        _lib.oracle_log2_bucket.restype = C.c_uint32
+        _lib.oracle_log2_bucket.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32]
+    return _lib
+
+
+def ref_available() -> bool:
+    return REF_PATH.exists()
+
+
+def ref():
+    global _ref
+    if _ref is None:
+        _ref = C.CDLL(str(REF_PATH))
+    return _ref
+
+
+def _p(a):
+    return None if a is None else C.c_void_p(a.ctypes.data)
+
+
+def _u32(a):
+    return None if a is None else np.ascontiguousarray(np.asarray(a, dtype=np.uint32))
+
+
+def _u8(a):
+    return None if a is None else np.ascontiguousarray(np.asarray(a, dtype=np.uint8))
+
+
+def _f(x):
+    return C.c_float(float(x))
+
+
+def _ok(rc, what):
+    if rc != 1:
+        raise RuntimeError(f"oracle {what} failed")
+
+
+# ---------------------------------------------------------------------------
+def _reduce_outputs(rows, G):
+    return dict(row_totals=np.zeros(rows, np.float64), detected=np.zeros(rows, np.uint32),
+                max_values=np.zeros(rows, np.float32), keep=np.zeros(rows, np.uint8),
+                group_counts=np.zeros(rows * max(G, 1), np.float64), group_pct=np.zeros(rows * max(G, 1), np.float32))
+
+
+def _filter_args(filt):
+    return [_f(filt[0]), C.c_uint32(int(filt[1])), _f(filt[2]), C.c_uint32(int(filt[3]))]
+
+
+def _finish(o, rows, G):
+    o["group_counts"] = o["group_counts"][: rows * G].reshape(rows, G)
+    o["group_pct"] = o["group_pct"][: rows * G].reshape(rows, G)
+    return o
+
+
+def group_reduce_csr(rows, cols, ptr, idx, val, vt, row_keep, feature_keep, fgm, G, filt):
+    ptr, idx, row_keep, feature_keep, fgm = _u32(ptr), _u32(idx), _u8(row_keep), _u8(feature_keep), _u32(fgm)
+    val = np.ascontiguousarray(val)
+    o = _reduce_outputs(rows, G)
+    _ok(lib().oracle_group_reduce_csr(C.c_uint32(rows), C.c_uint32(cols), _p(ptr), _p(idx), _p(val), C.c_uint32(vt),
+                                      _p(row_keep), _p(feature_keep), _p(fgm), C.c_uint32(G), *_filter_args(filt),
+                                      *[_p(o[k]) for k in ("row_totals", "detected", "max_values", "keep", "group_counts",
+                                                           "group_pct")]), "group_reduce_csr")
+    return _finish(o, rows, G)
+
+
+def group_reduce_sell(rows, cols, slice_rows, offsets, widths, slot_offsets, col_idx, val, vt, row_keep, feature_keep, fgm, G,
+                      filt):
+    offsets, widths, slot_offsets, col_idx = _u32(offsets), _u32(widths), _u32(slot_offsets), _u32(col_idx)
+    row_keep, feature_keep, fgm = _u8(row_keep), _u8(feature_keep), _u32(fgm)
+    val = np.ascontiguousarray(val)
+    o = _reduce_outputs(rows, G)
+    _ok(lib().oracle_group_reduce_sell(C.c_uint32(rows), C.c_uint32(cols), C.c_uint32(len(offsets) - 1), C.c_uint32(slice_rows),
+                                       _p(offsets), _p(widths), _p(slot_offsets), _p(col_idx), _p(val), C.c_uint32(vt),
+                                       _p(row_keep), _p(feature_keep), _p(fgm), C.c_uint32(G), *_filter_args(filt),
+                                       *[_p(o[k]) for k in ("row_totals", "detected", "max_values", "keep", "group_counts",
+                                                            "group_pct")]), "group_reduce_sell")
+    return _finish(o, rows, G)
+
+
+def group_reduce_bell(rows, cols, block_size, ell_cols, bci, val, vt, row_keep, feature_keep, fgm, G, filt):
+    bci, row_keep, feature_keep, fgm = _u32(bci), _u8(row_keep), _u8(feature_keep), _u32(fgm)
+    val = np.ascontiguousarray(val)
+    o = _reduce_outputs(rows, G)
+    _ok(lib().oracle_group_reduce_bell(C.c_uint32(rows), C.c_uint32(cols), C.c_uint32(block_size), C.c_uint32(ell_cols), _p(bci),
+                                       _p(val), C.c_uint32(vt), _p(row_keep), _p(feature_keep), _p(fgm), C.c_uint32(G),
+                                       *_filter_args(filt),
+                                       *[_p(o[k]) for k in ("row_totals", "detected", "max_values", "keep", "group_counts",
+                                                            "group_pct")]), "group_reduce_bell")
+    return _finish(o, rows, G)
+
+
+# ---------------------------------------------------------------------------
+def kept_features(cols, feature_keep):
+    fk = _u8(feature_keep)
+    ids = np.zeros(max(cols, 1), np.uint32)
+    F = lib().oracle_kept_features(C.c_uint32(cols), _p(fk), _p(ids))
+    return ids[:F]
+
+
+def _label_outputs(G, F):
+    return (np.zeros(G * max(F, 1), np.float64), np.zeros(G * max(F, 1), np.float64), np.zeros(G * max(F, 1), np.uint32),
+            np.zeros(G, np.uint32))
+
+
+def _label_finish(G, F, s, q, n, gr, ids):
+    return dict(sum=s[: G * F].reshape(G, F), sumsq=q[: G * F].reshape(G, F), nnz=n[: G * F].reshape(G, F), group_rows=gr,
+                kept_feature_ids=ids)
+
+
+def label_reduce_csr(rows, cols, ptr, idx, val, vt, labels, row_keep, feature_keep, G, scale=None):
+    ids = kept_features(cols, feature_keep)
+    F = len(ids)
+    s, q, n, gr = _label_outputs(G, F)
+    ptr, idx, labels, row_keep, feature_keep = _u32(ptr), _u32(idx), _u32(labels), _u8(row_keep), _u8(feature_keep)
+    scale = None if scale is None else np.ascontiguousarray(scale, dtype=np.float32)
+    val = np.ascontiguousarray(val)
+    _ok(lib().oracle_label_reduce_csr(C.c_uint32(rows), C.c_uint32(cols), _p(ptr), _p(idx), _p(val), C.c_uint32(vt), _p(scale),
+                                      _p(labels), _p(row_keep), _p(feature_keep), C.c_uint32(G), _p(s), _p(q), _p(n), _p(gr)),
+        "label_reduce_csr")
+    return _label_finish(G, F, s, q, n, gr, ids)
+
+
+def label_reduce_sell(rows, cols, slice_rows, offsets, widths, slot_offsets, col_idx, val, vt, labels, row_keep, feature_keep, G,
+                      scale=None):
+    ids = kept_features(cols, feature_keep)
+    F = len(ids)
+    s, q, n, gr = _label_outputs(G, F)
+    offsets, widths, slot_offsets, col_idx = _u32(offsets), _u32(widths), _u32(slot_offsets), _u32(col_idx)
+    labels, row_keep, feature_keep = _u32(labels), _u8(row_keep), _u8(feature_keep)
+    scale = None if scale is None else np.ascontiguousarray(scale, dtype=np.float32)
+    val = np.ascontiguousarray(val)
+    _ok(lib().oracle_label_reduce_sell(C.c_uint32(rows), C.c_uint32(cols), C.c_uint32(len(offsets) - 1), C.c_uint32(slice_rows),
+                                       _p(offsets), _p(widths), _p(slot_offsets), _p(col_idx), _p(val), C.c_uint32(vt), _p(scale),
+                                       _p(labels), _p(row_keep), _p(feature_keep), C.c_uint32(G), _p(s), _p(q), _p(n), _p(gr)),
+        "label_reduce_sell")
+    return _label_finish(G, F, s, q, n, gr, ids)
+
+
+def label_reduce_bell(rows, cols, block_size, ell_cols, bci, val, vt, labels, row_keep, feature_keep, G, scale=None):
+    ids = kept_features(cols, feature_keep)
+    F = len(ids)
+    s, q, n, gr = _label_outputs(G, F)
+    bci, labels, row_keep, feature_keep = _u32(bci), _u32(labels), _u8(row_keep), _u8(feature_keep)
+    scale = None if scale is None else np.ascontiguousarray(scale, dtype=np.float32)
+    val = np.ascontiguousarray(val)
+    _ok(lib().oracle_label_reduce_bell(C.c_uint32(rows), C.c_uint32(cols), C.c_uint32(block_size), C.c_uint32(ell_cols), _p(bci),
+                                       _p(val), C.c_uint32(vt), _p(scale), _p(labels), _p(row_keep), _p(feature_keep),
+                                       C.c_uint32(G), _p(s), _p(q), _p(n), _p(gr)), "label_reduce_bell")
+    return _label_finish(G, F, s, q, n, gr, ids)
+
+
+# ---------------------------------------------------------------------------
+def blocked_ell_shape(rows, cols, ptr, idx, bs):
+    ptr, idx = _u32(ptr), _u32(idx)
+    w, t, f = C.c_uint32(), C.c_uint64(), C.c_double()
+    _ok(lib().oracle_blocked_ell_shape(C.c_uint32(rows), C.c_uint32(cols), _p(ptr), _p(idx), C.c_uint32(bs), C.byref(w),
+                                       C.byref(t), C.byref(f)), "blocked_ell_shape")
+    return w.value, t.value, f.value
+
+
+def choose_blocked_ell_block_size(rows, cols, ptr, idx, candidates, value_bytes=2):
+    ptr, idx, cands = _u32(ptr), _u32(idx), _u32(candidates)
+    bs, fill, pad = C.c_uint32(), C.c_double(), C.c_uint64()
+    _ok(lib().oracle_choose_blocked_ell_block_size(C.c_uint32(rows), C.c_uint32(cols), _p(ptr), _p(idx), _p(cands),
+                                                   C.c_uint32(len(cands)), C.c_uint32(value_bytes), C.byref(bs), C.byref(fill),
+                                                   C.byref(pad)), "choose_blocked_ell_block_size")
+    return bs.value, fill.value, pad.value
+
+
+def blocked_ell_from_compressed(rows, cols, ptr, idx, val, bs):
+    """Returns (block_col_idx, val_bits, ell_cols, nnz)."""
+    w, _, _ = blocked_ell_shape(rows, cols, ptr, idx, bs)
+    ptr, idx = _u32(ptr), _u32(idx)
+    val = np.ascontiguousarray(val)
+    vb = val.dtype.itemsize
+    rb = (rows + bs - 1) // bs
+    bci = np.zeros(max(rb * w, 1), np.uint32)
+    out = np.zeros(max(rows * w * bs, 1), val.dtype)
+    nnz = C.c_uint32()
+    _ok(lib().oracle_blocked_ell_from_compressed(C.c_uint32(rows), C.c_uint32(cols), _p(ptr), _p(idx), _p(val), C.c_uint32(vb),
+                                                 C.c_uint32(bs), C.c_uint32(w), _p(bci), _p(out), C.byref(nnz)),
+        "blocked_ell_from_compressed")
+    return bci[: rb * w], out[: rows * w * bs], w * bs, nnz.value
+
+
+def compressed_row_nnz(rows, cols, axis, ptr, idx):
+    ptr, idx = _u32(ptr), _u32(idx)
+    out = np.zeros(max(rows, 1), np.uint32)
+    _ok(lib().oracle_compressed_row_nnz(C.c_uint32(rows), C.c_uint32(cols), C.c_uint32(axis), _p(ptr), _p(idx), _p(out)),
+        "compressed_row_nnz")
+    return out[:rows]
+
+
+def sliced_ell_layout(row_nnz, rows, slice_rows):
+    rn = _u32(row_nnz) if rows else np.zeros(1, np.uint32)
+    s = min(max(slice_rows, 1), rows) if rows else 0
+    count = (rows + s - 1) // s if rows else 0
+    offs = np.zeros(count + 1, np.uint32)
+    widths = np.zeros(max(count, 1), np.uint32)
+    slots = np.zeros(max(count, 1), np.uint32)
+    c, t = C.c_uint32(), C.c_uint64()
+    rc = lib().oracle_sliced_ell_layout(_p(rn), C.c_uint32(rows), C.c_uint32(slice_rows), _p(offs), _p(widths), _p(slots),
+                                        C.byref(c), C.byref(t))
+    if rc != 1:
+        raise OverflowError("sliced_ell layout: total slots exceed u32")
+    return s, offs, widths[:count], slots[:count], t.value
+
+
+def sliced_ell_from_compressed(rows, cols, axis, ptr, idx, val, slice_rows):
+    rn = compressed_row_nnz(rows, cols, axis, ptr, idx)
+    s, offs, widths, slots, total = sliced_ell_layout(rn, rows, slice_rows)
+    ptr, idx = _u32(ptr), _u32(idx)
+    val = np.ascontiguousarray(val)
+    col = np.zeros(max(total, 1), np.uint32)
+    out = np.zeros(max(total, 1), val.dtype)
+    _ok(lib().oracle_sliced_ell_from_compressed(C.c_uint32(rows), C.c_uint32(cols), C.c_uint32(axis), _p(ptr), _p(idx), _p(val),
+                                                C.c_uint32(val.dtype.itemsize), C.c_uint32(len(offs) - 1), _p(offs), _p(widths),
+                                                _p(slots), C.c_uint64(total), _p(col), _p(out)), "sliced_ell_from_compressed")
+    return dict(slice_rows=s, offsets=offs, widths=widths, slot_offsets=slots, col_idx=col[:total], val=out[:total])
+
+
+def blocked_ell_row_nnz(rows, cols, bs, ell_cols, bci, val):
+    bci = _u32(bci)
+    val = np.ascontiguousarray(val)
+    out = np.zeros(max(rows, 1), np.uint32)
+    _ok(lib().oracle_blocked_ell_row_nnz(C.c_uint32(rows), C.c_uint32(cols), C.c_uint32(bs), C.c_uint32(ell_cols), _p(bci),
+                                         _p(val), C.c_uint32(val.dtype.itemsize), _p(out)), "blocked_ell_row_nnz")
+    return out[:rows]
+
+
+def sliced_ell_from_blocked_ell(rows, cols, bs, ell_cols, bci, val, slice_rows):
+    rn = blocked_ell_row_nnz(rows, cols, bs, ell_cols, bci, val)
+    s, offs, widths, slots, total = sliced_ell_layout(rn, rows, slice_rows)
+    bci = _u32(bci)
+    val = np.ascontiguousarray(val)
+    col = np.zeros(max(total, 1), np.uint32)
+    out = np.zeros(max(total, 1), val.dtype)
+    _ok(lib().oracle_sliced_ell_from_blocked_ell(C.c_uint32(rows), C.c_uint32(cols), C.c_uint32(bs), C.c_uint32(ell_cols), _p(bci),
+                                                 _p(val), C.c_uint32(val.dtype.itemsize), C.c_uint32(len(offs) - 1), _p(offs),
+                                                 _p(widths), _p(slots), C.c_uint64(total), _p(col), _p(out)),
+        "sliced_ell_from_blocked_ell")
+    return dict(slice_rows=s, offsets=offs, widths=widths, slot_offsets=slots, col_idx=col[:total], val=out[:total])
+
+
+def choose_sliced_ell_slice_rows(row_nnz, rows, nnz, candidates):
+    rn, cands = _u32(row_nnz) if rows else np.zeros(1, np.uint32), _u32(candidates)
+    s, c, p, sc = C.c_uint32(), C.c_uint32(), C.c_uint64(), C.c_double()
+    _ok(lib().oracle_choose_sliced_ell_slice_rows(_p(rn), C.c_uint32(rows), C.c_uint32(nnz), _p(cands), C.c_uint32(len(cands)),
+                                                  C.byref(s), C.byref(c), C.byref(p), C.byref(sc)), "choose_sliced_ell_slice_rows")
+    return dict(slice_rows=s.value, slice_count=c.value, padded_slots=p.value, score=sc.value)
+
+
+# ---------------------------------------------------------------------------
+def transpose(cdim, udim, ptr, idx, val):
+    ptr, idx = _u32(ptr), _u32(idx)
+    val = np.ascontiguousarray(val)
+    nnz = len(idx)
+    optr = np.zeros(udim + 1, np.uint32)
+    oidx = np.zeros(max(nnz, 1), np.uint32)
+    oval = np.zeros(max(nnz, 1), val.dtype)
+    _ok(lib().oracle_transpose(C.c_uint32(cdim), C.c_uint32(udim), _p(ptr), _p(idx), _p(val), C.c_uint32(val.dtype.itemsize),
+                               _p(optr), _p(oidx), _p(oval)), "transpose")
+    return optr, oidx[:nnz], oval[:nnz]
+
+
+def bucket_plan(ptr, n, requested):
+    ptr = _u32(ptr)
+    nnz, srt, order = (np.zeros(max(n, 1), np.uint32) for _ in range(3))
+    offs = np.zeros(max(n, 1) + 2, np.uint32)
+    b = C.c_uint32()
+    _ok(lib().oracle_bucket_plan(_p(ptr), C.c_uint32(n), C.c_uint32(requested), _p(nnz), _p(srt), _p(order), _p(offs), C.byref(b)),
+        "bucket_plan")
+    return dict(major_nnz=nnz[:n], major_nnz_sorted=srt[:n], major_order=order[:n], bucket_offsets=offs[: b.value + 1],
+                bucket_count=b.value)
+
+
+def log2_offsets(nnz_sorted, B=12, min_log2=5):
+    s = _u32(nnz_sorted)
+    offs = np.zeros(B + 1, np.uint32)
+    _ok(lib().oracle_log2_offsets(_p(s) if len(s) else None, C.c_uint32(len(s)), C.c_uint32(B), C.c_uint32(min_log2), _p(offs)),
+        "log2_offsets")
+    return offs
+
+
+def log2_bucket(v, B=12, min_log2=5):
+    return lib().oracle_log2_bucket(int(v), B, min_log2)
+
+
+def rebuild(ptr, idx, val, order):
+    ptr, idx, order = _u32(ptr), _u32(idx), _u32(order)
+    val = np.ascontiguousarray(val)
+    n = len(order)
+    dptr = np.zeros(n + 1, np.uint32)
+    didx = np.zeros(max(len(idx), 1), np.uint32)
+    dval = np.zeros(max(len(idx), 1), val.dtype)
+    inv = np.zeros(max(n, 1), np.uint32)
+    _ok(lib().oracle_rebuild(_p(ptr), _p(idx), _p(val), C.c_uint32(val.dtype.itemsize), C.c_uint32(n), _p(order), _p(dptr),
+                             _p(didx), _p(dval), _p(inv)), "rebuild")
+    return dptr, didx[: len(idx)], dval[: len(idx)], inv[:n]
+
+
+def bell_bucket_keys(row_blocks, width, bci):
+    bci = _u32(bci) if len(bci) else np.zeros(1, np.uint32)
+    keys = np.zeros(max(row_blocks, 1), np.uint64)
+    lib().oracle_bell_bucket_keys(C.c_uint32(row_blocks), C.c_uint32(width), _p(bci), _p(keys))
+    return keys[:row_blocks]
+
+
+def reoptimize(rows, cols, bs, ell_cols, bci, val16, row_keep, feature_keep, row_remap, feature_remap, out_rows, out_cols):
+    bci, row_keep, feature_keep = _u32(bci), _u8(row_keep), _u8(feature_keep)
+    row_remap, feature_remap = _u32(row_remap), _u32(feature_remap)
+    val16 = np.ascontiguousarray(val16, dtype=np.uint16)
+    maxn = rows * ell_cols
+    mp = np.zeros(out_rows + 1, np.uint32)
+    mi = np.zeros(max(maxn, 1), np.uint32)
+    mv = np.zeros(max(maxn, 1), np.uint16)
+    nnz = C.c_uint32()
+    _ok(lib().oracle_reoptimize(C.c_uint32(rows), C.c_uint32(cols), C.c_uint32(bs), C.c_uint32(ell_cols), _p(bci), _p(val16),
+                                _p(row_keep), _p(feature_keep), _p(row_remap), _p(feature_remap), C.c_uint32(out_rows),
+                                C.c_uint32(out_cols), _p(mp), _p(mi), _p(mv), C.byref(nnz)), "reoptimize")
+    return mp, mi[: nnz.value], mv[: nnz.value], nnz.value
+
+
+# ---------------------------------------------------------------------------
+# Reference CPU converter (oracle/_ref), fp16 values.
+def ref_choose_blocked_ell_block_size(rows, cols, ptr, idx, val16, candidates):
+    ptr, idx, cands = _u32(ptr), _u32(idx), _u32(candidates)
+    val16 = np.ascontiguousarray(val16).view(np.uint16)
+    bs, fill, pad = C.c_uint32(), C.c_double(), C.c_ulonglong()
+    _ok(ref().ref_choose_blocked_ell_block_size(C.c_uint32(rows), C.c_uint32(cols), C.c_uint32(len(idx)), _p(ptr), _p(idx),
+                                                _p(val16), _p(cands), C.c_uint32(len(cands)), C.byref(bs), C.byref(fill),
+                                                C.byref(pad)), "ref choose")
+    return bs.value, fill.value, pad.value
+
+
+def ref_blocked_ell_from_compressed(rows, cols, ptr, idx, val16, bs):
+    ptr, idx = _u32(ptr), _u32(idx)
+    val16 = np.ascontiguousarray(val16).view(np.uint16)
+    ell, nnz = C.c_uint32(), C.c_uint32()
+    r = ref()
+    _ok(r.ref_blocked_ell_from_compressed(C.c_uint32(rows), C.c_uint32(cols), C.c_uint32(len(idx)), _p(ptr), _p(idx), _p(val16),
+                                          C.c_uint32(bs), C.byref(ell), C.byref(nnz), None, C.c_ulonglong(0), None,
+                                          C.c_ulonglong(0)), "ref convert (size)")
+    w = ell.value // bs
+    rb = (rows + bs - 1) // bs
+    bci = np.zeros(max(rb * w, 1), np.uint32)
+    out = np.zeros(max(rows * ell.value, 1), np.uint16)
+    _ok(r.ref_blocked_ell_from_compressed(C.c_uint32(rows), C.c_uint32(cols), C.c_uint32(len(idx)), _p(ptr), _p(idx), _p(val16),
+                                          C.c_uint32(bs), C.byref(ell), C.byref(nnz), _p(bci), C.c_ulonglong(len(bci)), _p(out),
+                                          C.c_ulonglong(len(out))), "ref convert")
+    return bci[: rb * w], out[: rows * ell.value], ell.value, nnz.value

@@ -1,0 +1,198 @@
+"""mask_groups (reference ABI) on the GPU: the reference test's fixtures
+(tests/mask_groups_runtime_test.cu, F3-corrected) through the same call
+sequence, then random parity against the oracle."""
+import ctypes as C
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+import torch
+
+from cellshard_amd import abi as A
+from cellshard_amd import runtime as csr_rt
+from cellshard_amd.device import DeviceBlockedELL, DeviceCSR, DeviceSlicedELL, ptr, to_dev_u32, to_dev_u8
+from cellshard_amd.ops import device_to_numpy
+from helpers import BF16, F16, F32, FP8, encode, f16_bits, random_csr
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+GOLD = Path(__file__).resolve().parent / "golden"
+
+
+def fetch(ws, res):
+    torch.cuda.synchronize()
+    rows, G = res.rows, res.group_count
+    out = dict(
+        row_totals=device_to_numpy(res.row_totals, rows, np.float32),
+        detected=device_to_numpy(res.detected_features, rows, np.uint32),
+        max_values=device_to_numpy(res.max_values, rows, np.float32),
+        keep=device_to_numpy(res.row_keep, rows, np.uint8),
+    )
+    if G:
+        out["group_counts"] = device_to_numpy(res.group_counts, rows * G, np.float32).reshape(rows, G)
+        out["group_pct"] = device_to_numpy(res.group_percentages, rows * G, np.float32).reshape(rows, G)
+    return out
+
+
+@pytest.fixture()
+def ws(gpu):
+    w = A.sparse_group_reduce_workspace()
+    csr_rt.init(w)
+    assert csr_rt.setup(w, 0, torch.cuda.current_stream().cuda_stream) == 1
+    yield w
+    csr_rt.clear(w)
+
+
+def test_reference_fixture_sequence(ws):
+    """Port of tests/mask_groups_runtime_test.cu:52-198."""
+    mg = json.loads((GOLD / "mask_groups_fixture.json").read_text())
+    e = mg["expected"]
+    tol = e["tolerance_abs"]
+    rows, cols = mg["rows"], mg["cols"]
+    row_keep = to_dev_u8(mg["row_keep"])
+    feature_keep = to_dev_u8(mg["feature_keep"])
+    assert csr_rt.upload_feature_group_masks(ws, cols, mg["group_masks"]) == 1
+    groups = A.group_mask_config_view(group_count=mg["group_count"], feature_group_masks=ws.feature_group_masks)
+    masks = A.row_feature_mask_view(row_keep=ptr(row_keep), feature_keep=ptr(feature_keep))
+    filt = A.sparse_group_filter_params(*mg["filter"])
+    res = A.sparse_group_reduce_result()
+
+    b = mg["blocked_ell"]
+    bell = DeviceBlockedELL.from_numpy(rows, cols, b["nnz"], b["block_size"], b["ell_cols"], b["block_col_idx"],
+                                       f16_bits(b["values"]))
+    assert csr_rt.compute_sparse_group_reduce(bell.view(), ws, groups, masks, filt, res) == 1
+    o = fetch(ws, res)
+    be = e["blocked_ell_masked"]
+    for k, v in be["row_totals"].items():
+        assert abs(o["row_totals"][int(k)] - v) <= tol
+    for k, v in be["detected"].items():
+        assert o["detected"][int(k)] == v
+    for k, v in be["keep"].items():
+        assert o["keep"][int(k)] == v
+    for k, v in be["group_counts"].items():
+        assert abs(o["group_counts"].ravel()[int(k)] - v) <= tol
+    for k, v in be["group_pct"].items():
+        assert abs(o["group_pct"].ravel()[int(k)] - v) <= tol
+
+    no_groups = A.group_mask_config_view(group_count=0)
+    assert csr_rt.compute_sparse_group_reduce(bell.view(), ws, no_groups, None, filt, res) == 1
+    o = fetch(ws, res)
+    assert res.group_count == 0 and abs(o["row_totals"][0] - 15.0) <= tol and o["detected"][0] == 3
+
+    c = mg["compressed"]
+    csr = DeviceCSR.from_numpy(rows, cols, c["ptr"], c["idx"], f16_bits(c["values"]))
+    assert csr_rt.compute_sparse_group_reduce_compressed_fallback(csr.view(), ws, groups, masks, filt, res) == 1
+    o = fetch(ws, res)
+    assert abs(o["row_totals"][2] - 7.0) <= tol and abs(o["group_counts"].ravel()[8] - 2.0) <= tol
+
+    s = mg["sliced_ell"]
+    sell = DeviceSlicedELL.from_numpy(rows, cols, s["nnz"], s["slice_rows"], s["slice_row_offsets"], s["slice_widths"],
+                                      s["slice_slot_offsets"], s["col_idx"], f16_bits(s["values"]))
+    assert csr_rt.compute_sparse_group_reduce(sell.view(), ws, groups, masks, filt, res) == 1
+    o = fetch(ws, res)
+    assert abs(o["row_totals"][0] - 8.0) <= tol and abs(o["row_totals"][3] - 18.0) <= tol
+
+    r = mg["reoptimize"]
+    rws = A.masked_sparse_reoptimize_workspace()
+    csr_rt.init(rws)
+    assert csr_rt.setup(rws, 0) == 1
+    row_remap = to_dev_u32(r["row_remap"])
+    feature_remap = to_dev_u32(r["feature_remap"])
+    rmasks = A.row_feature_mask_view(row_keep=ptr(row_keep), feature_keep=ptr(feature_keep), row_remap=ptr(row_remap),
+                                     feature_remap=ptr(feature_remap))
+    cfg = A.masked_sparse_reoptimize_config(**r["config"])
+    out = A.masked_sparse_reoptimize_result()
+    assert csr_rt.manual_reoptimize_masked_sparse(bell.view(), rmasks, cfg, rws, out) == 1
+    md = out.metadata
+    assert (md.kept_rows, md.kept_features, md.live_nnz, md.output_layout) == (3, 4, 7, A.LAYOUT_COMPRESSED)
+    # the compacted CSR equals the oracle's
+    mp, mi, mv, nnz = O.reoptimize(rows, cols, b["block_size"], b["ell_cols"], b["block_col_idx"],
+                                   f16_bits(b["values"]).view(np.uint16), mg["row_keep"], mg["feature_keep"], r["row_remap"],
+                                   r["feature_remap"], 3, 4)
+    assert np.array_equal(device_to_numpy(out.compressed.majorPtr, 4, np.uint32), mp)
+    assert np.array_equal(device_to_numpy(out.compressed.minorIdx, nnz, np.uint32), mi)
+    assert np.array_equal(device_to_numpy(out.compressed.val, nnz, np.uint16), mv)
+    csr_rt.clear(rws)
+
+
+def test_error_paths(ws):
+    filt = A.sparse_group_filter_params(0.0, 0, 1.0, 0)
+    res = A.sparse_group_reduce_result()
+    csr = DeviceCSR.from_numpy(2, 3, [0, 1, 2], [0, 2], f16_bits([1, 2]))
+    big = A.group_mask_config_view(group_count=33)
+    assert csr_rt.compute_sparse_group_reduce_compressed_fallback(csr.view(), ws, big, None, filt, res) == 0
+    csc = DeviceCSR.from_numpy(2, 3, [0, 1, 1, 2], [0, 1], f16_bits([1, 2]), axis=A.COMPRESSED_BY_COL)
+    assert csr_rt.compute_sparse_group_reduce_compressed_fallback(csc.view(), ws, None, None, filt, res) == 0
+    assert csr_rt.compute_sparse_group_reduce_compressed_fallback(csr.view(), ws, None, None, None, res) == 0
+
+
+def _compare(o, ref, G, vt):
+    # exact for integer-valued inputs well below 2^24
+    assert np.array_equal(o["detected"], ref["detected"])
+    assert np.array_equal(o["keep"], ref["keep"])
+    np.testing.assert_allclose(o["row_totals"], ref["row_totals"], rtol=1e-6, atol=0)
+    np.testing.assert_array_equal(o["max_values"], ref["max_values"])
+    if G:
+        np.testing.assert_allclose(o["group_counts"], ref["group_counts"], rtol=1e-6, atol=0)
+        np.testing.assert_allclose(o["group_pct"], ref["group_pct"], rtol=1e-5, atol=1e-5)
+
+
+@pytest.mark.parametrize("vt", [F16, F32, BF16, FP8])
+@pytest.mark.parametrize("layout", ["csr", "sell", "bell"])
+def test_random_parity(ws, vt, layout):
+    rows, cols, G = 300, 211, 32 if vt == F16 else 9
+    ptr_, idx, val = random_csr(rows, cols, 0.08, seed=31 + vt, zeros=0.03, empty_rows=0.05, fp8_safe=(vt == FP8))
+    bits = encode(val, vt)
+    rng = np.random.default_rng(vt)
+    rk = (rng.random(rows) < 0.9).astype(np.uint8)
+    fk = (rng.random(cols) < 0.8).astype(np.uint8)
+    fgm = rng.integers(0, 2 ** 32, size=cols, dtype=np.uint64).astype(np.uint32)
+    filt_t = [10.0, 3, 0.4, 2]
+    d_rk, d_fk, d_fgm = to_dev_u8(rk), to_dev_u8(fk), to_dev_u32(fgm)
+    groups = A.group_mask_config_view(group_count=G, feature_group_masks=ptr(d_fgm))
+    masks = A.row_feature_mask_view(row_keep=ptr(d_rk), feature_keep=ptr(d_fk))
+    filt = A.sparse_group_filter_params(*filt_t)
+    res = A.sparse_group_reduce_result()
+    if layout == "csr":
+        src = DeviceCSR.from_numpy(rows, cols, ptr_, idx, bits, vt)
+        assert csr_rt.compute_sparse_group_reduce_compressed_fallback(src.view(), ws, groups, masks, filt, res, vt) == 1
+        ref = O.group_reduce_csr(rows, cols, ptr_, idx, bits, vt, rk, fk, fgm, G, filt_t)
+    elif layout == "sell":
+        s = O.sliced_ell_from_compressed(rows, cols, 0, ptr_, idx, bits, 32)
+        src = DeviceSlicedELL.from_numpy(rows, cols, len(idx), s["slice_rows"], s["offsets"], s["widths"], s["slot_offsets"],
+                                         s["col_idx"], s["val"], vt)
+        assert csr_rt.compute_sparse_group_reduce(src.view(), ws, groups, masks, filt, res, vt) == 1
+        ref = O.group_reduce_sell(rows, cols, s["slice_rows"], s["offsets"], s["widths"], s["slot_offsets"], s["col_idx"],
+                                  s["val"], vt, rk, fk, fgm, G, filt_t)
+    else:
+        bci, bv, ell, nnz = O.blocked_ell_from_compressed(rows, cols, ptr_, idx, bits, 8)
+        src = DeviceBlockedELL.from_numpy(rows, cols, nnz, 8, ell, bci, bv, vt)
+        assert csr_rt.compute_sparse_group_reduce(src.view(), ws, groups, masks, filt, res, vt) == 1
+        ref = O.group_reduce_bell(rows, cols, 8, ell, bci, bv, vt, rk, fk, fgm, G, filt_t)
+    _compare(fetch(ws, res), ref, G, vt)
+
+
+def test_fleet_single_gpu(gpu):
+    fleet = A.sparse_group_reduce_fleet_workspace()
+    csr_rt.init(fleet)
+    cfg = A.sparse_group_reduce_fleet_config()
+    ids = (C.c_int * 1)(0)
+    cfg.device_ids = C.cast(ids, C.c_void_p)
+    cfg.device_count = 1
+    assert csr_rt.setup_fleet(fleet, cfg) == 1
+    rows, cols = 64, 40
+    ptr_, idx, val = random_csr(rows, cols, 0.2, seed=3)
+    bits = f16_bits(val)
+    bci, bv, ell, nnz = O.blocked_ell_from_compressed(rows, cols, ptr_, idx, bits, 4)
+    src = DeviceBlockedELL.from_numpy(rows, cols, nnz, 4, ell, bci, bv)
+    out = A.sparse_group_reduce_fleet_result()
+    filt = A.sparse_group_filter_params(0.0, 0, 1.0, 0)
+    assert csr_rt.compute_sparse_group_reduce_fleet([src.view()], fleet, None, None, filt, out) == 1
+    assert out.slot_count == 1
+    torch.cuda.synchronize()
+    r = out.slot_results[0]
+    got = device_to_numpy(r.row_totals, rows, np.float32)
+    ref = O.group_reduce_bell(rows, cols, 4, ell, bci, bv, F16, None, None, None, 0, [0, 0, 1, 0])
+    np.testing.assert_allclose(got, ref["row_totals"], rtol=1e-6)
+    csr_rt.clear(fleet)

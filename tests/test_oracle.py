@@ -1,0 +1,377 @@
+"""Pin the CPU oracle (oracle/oracle.c) before it is trusted as the checker.
+
+Each oracle function is checked against something independent of its own
+formula: the reference's fixtures, the reference's compiled CPU converter
+(golden vectors), dense numpy recomputation, scipy, or closed forms.
+"""
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from helpers import BF16, F16, F32, FP8, decode, dense_of, encode, f16_bits, random_csr
+from oracle import oracle as O
+
+GOLD = Path(__file__).resolve().parent / "golden"
+
+
+@pytest.fixture(scope="module")
+def mg():
+    return json.loads((GOLD / "mask_groups_fixture.json").read_text())
+
+
+def _check_asserted(got, expected, tol):
+    for k, v in expected.items():
+        assert abs(float(got[int(k)]) - float(v)) <= tol, (k, got[int(k)], v)
+
+
+def test_decoders_match_numpy():
+    h = np.arange(0, 65536, 97, dtype=np.uint32).astype(np.uint16)
+    for b in h:
+        ref = np.array([b], np.uint16).view(np.float16).astype(np.float32)[0]
+        got = O.lib().oracle_half_to_float(int(b))
+        assert (np.isnan(ref) and np.isnan(got)) or ref == got
+    # e4m3fn: closed form (1 + m/8) * 2^(e-7), subnormal m * 2^-9, 0x7f/0xff NaN
+    for b in range(256):
+        got = O.lib().oracle_fp8_e4m3_to_float(b)
+        ref = decode(np.array([b], np.uint8), FP8)[0]
+        assert (np.isnan(ref) and np.isnan(got)) or ref == got
+    assert O.lib().oracle_fp8_e4m3_to_float(0x7E) == 448.0
+
+
+def test_group_reduce_reference_fixture_blocked(mg):
+    b = mg["blocked_ell"]
+    e = mg["expected"]["blocked_ell_masked"]
+    tol = mg["expected"]["tolerance_abs"]
+    o = O.group_reduce_bell(mg["rows"], mg["cols"], b["block_size"], b["ell_cols"], b["block_col_idx"], f16_bits(b["values"]), F16,
+                            mg["row_keep"], mg["feature_keep"], mg["group_masks"], mg["group_count"], mg["filter"])
+    _check_asserted(o["row_totals"], e["row_totals"], tol)
+    _check_asserted(o["detected"], e["detected"], 0)
+    _check_asserted(o["keep"], e["keep"], 0)
+    _check_asserted(o["group_counts"].ravel(), e["group_counts"], tol)
+    _check_asserted(o["group_pct"].ravel(), e["group_pct"], tol)
+    u = mg["expected"]["blocked_ell_unmasked_no_groups"]
+    o2 = O.group_reduce_bell(mg["rows"], mg["cols"], b["block_size"], b["ell_cols"], b["block_col_idx"], f16_bits(b["values"]), F16,
+                             None, None, None, 0, mg["filter"])
+    _check_asserted(o2["row_totals"], u["row_totals"], tol)
+    _check_asserted(o2["detected"], u["detected"], 0)
+
+
+def test_group_reduce_reference_fixture_compressed_and_sliced(mg):
+    c = mg["compressed"]
+    tol = mg["expected"]["tolerance_abs"]
+    o = O.group_reduce_csr(mg["rows"], mg["cols"], c["ptr"], c["idx"], f16_bits(c["values"]), F16, mg["row_keep"],
+                           mg["feature_keep"], mg["group_masks"], mg["group_count"], mg["filter"])
+    _check_asserted(o["row_totals"], mg["expected"]["compressed_masked"]["row_totals"], tol)
+    _check_asserted(o["group_counts"].ravel(), mg["expected"]["compressed_masked"]["group_counts"], tol)
+    s = mg["sliced_ell"]
+    o = O.group_reduce_sell(mg["rows"], mg["cols"], s["slice_rows"], s["slice_row_offsets"], s["slice_widths"],
+                            s["slice_slot_offsets"], s["col_idx"], f16_bits(s["values"]), F16, mg["row_keep"], mg["feature_keep"],
+                            mg["group_masks"], mg["group_count"], mg["filter"])
+    _check_asserted(o["row_totals"], mg["expected"]["sliced_ell_masked"]["row_totals"], tol)
+
+
+@pytest.mark.parametrize("vt", [F16, F32, BF16, FP8])
+def test_group_reduce_matches_dense_numpy(vt):
+    rows, cols, G = 60, 45, 7
+    ptr, idx, val = random_csr(rows, cols, 0.2, seed=11, zeros=0.05, fp8_safe=(vt == FP8))
+    rng = np.random.default_rng(3)
+    rk = (rng.random(rows) < 0.8).astype(np.uint8)
+    fk = (rng.random(cols) < 0.7).astype(np.uint8)
+    fgm = rng.integers(0, 1 << G, size=cols).astype(np.uint32)
+    bits = encode(val, vt)
+    o = O.group_reduce_csr(rows, cols, ptr, idx, bits, vt, rk, fk, fgm, G, [5.0, 2, 0.5, 1])
+    d = dense_of(rows, cols, ptr, idx, decode(bits, vt)) * fk[None, :] * rk[:, None]
+    np.testing.assert_allclose(o["row_totals"], d.sum(1), rtol=1e-12)
+    assert np.array_equal(o["detected"], (d != 0).sum(1))
+    for g in range(G):
+        cols_g = ((fgm >> g) & 1).astype(bool)
+        np.testing.assert_allclose(o["group_counts"][:, g], d[:, cols_g].sum(1), rtol=1e-12)
+    np.testing.assert_allclose(o["max_values"], np.maximum(d.max(1), 0), rtol=0)
+    s = d.sum(1).astype(np.float32)
+    frac = np.where(s > 0, d[:, ((fgm >> 1) & 1).astype(bool)].sum(1).astype(np.float32) / np.where(s > 0, s, 1), 0)
+    keep = (s >= 5.0) & ((d != 0).sum(1) >= 2) & (frac <= 0.5) & (rk == 1)
+    assert np.array_equal(o["keep"].astype(bool), keep)
+
+
+@pytest.mark.parametrize("vt", [F16, BF16, FP8, F32])
+def test_label_reduce_matches_np_add_at(vt):
+    rows, cols, G = 80, 50, 5
+    ptr, idx, val = random_csr(rows, cols, 0.25, seed=21, zeros=0.05, fp8_safe=(vt == FP8))
+    rng = np.random.default_rng(4)
+    labels = rng.integers(0, G + 1, size=rows).astype(np.uint32)  # label G drops the row
+    rk = (rng.random(rows) < 0.85).astype(np.uint8)
+    fk = (rng.random(cols) < 0.6).astype(np.uint8)
+    scale = rng.uniform(0.5, 2.0, cols).astype(np.float32) if vt == FP8 else None
+    bits = encode(val, vt)
+    o = O.label_reduce_csr(rows, cols, ptr, idx, bits, vt, labels, rk, fk, G, scale)
+    kept = np.flatnonzero(fk)
+    v = decode(bits, vt).astype(np.float32)
+    if scale is not None:
+        v = (v * scale[idx]).astype(np.float32)
+    v = v.astype(np.float64)
+    rows_of = np.repeat(np.arange(rows), np.diff(ptr))
+    ok = (rk[rows_of] == 1) & (labels[rows_of] < G) & (fk[idx] == 1) & (v != 0)
+    rank = np.full(cols, -1)
+    rank[kept] = np.arange(len(kept))
+    S = np.zeros((G, len(kept)))
+    Q = np.zeros((G, len(kept)))
+    N = np.zeros((G, len(kept)), np.int64)
+    np.add.at(S, (labels[rows_of][ok], rank[idx][ok]), v[ok])
+    np.add.at(Q, (labels[rows_of][ok], rank[idx][ok]), v[ok] ** 2)
+    np.add.at(N, (labels[rows_of][ok], rank[idx][ok]), 1)
+    np.testing.assert_allclose(o["sum"], S, rtol=1e-12)
+    np.testing.assert_allclose(o["sumsq"], Q, rtol=1e-12)
+    assert np.array_equal(o["nnz"], N)
+    assert np.array_equal(o["kept_feature_ids"], kept)
+    assert np.array_equal(o["group_rows"], np.bincount(labels[(rk == 1) & (labels < G)], minlength=G + 1)[:G])
+
+
+def test_label_reduce_layouts_agree():
+    rows, cols, G = 50, 40, 3
+    ptr, idx, val = random_csr(rows, cols, 0.3, seed=5)
+    bits = f16_bits(val)
+    labels = (np.arange(rows) % G).astype(np.uint32)
+    fk = (np.arange(cols) % 3 != 0).astype(np.uint8)
+    a = O.label_reduce_csr(rows, cols, ptr, idx, bits, F16, labels, None, fk, G)
+    s = O.sliced_ell_from_compressed(rows, cols, 0, ptr, idx, bits, 8)
+    b = O.label_reduce_sell(rows, cols, s["slice_rows"], s["offsets"], s["widths"], s["slot_offsets"], s["col_idx"], s["val"], F16,
+                            labels, None, fk, G)
+    bci, bv, ell, _ = O.blocked_ell_from_compressed(rows, cols, ptr, idx, bits, 4)
+    c = O.label_reduce_bell(rows, cols, 4, ell, bci, bv, F16, labels, None, fk, G)
+    for k in ("sum", "sumsq", "nnz"):
+        assert np.array_equal(a[k], b[k]) and np.array_equal(a[k], c[k]), k
+
+
+@pytest.fixture(scope="module")
+def bell_gold():
+    return np.load(GOLD / "bell_ref_vectors.npz")
+
+
+def _cases(g):
+    return sorted({k.split("/")[0] for k in g.files})
+
+
+def test_blocked_ell_matches_reference_converter(bell_gold):
+    g = bell_gold
+    for name in _cases(g):
+        rows, cols = (int(x) for x in g[f"{name}/meta"])
+        ptr, idx, val = g[f"{name}/ptr"], g[f"{name}/idx"], g[f"{name}/val"]
+        for key in [k for k in g.files if k.startswith(f"{name}/b") and k.endswith("/bci")]:
+            bs = int(key.split("/")[1][1:])
+            bci, v, ell, nnz = O.blocked_ell_from_compressed(rows, cols, ptr, idx, val, bs)
+            ref_ell, ref_nnz = (int(x) for x in g[f"{name}/b{bs}/shape"])
+            assert ell == ref_ell and nnz == ref_nnz, (name, bs)
+            assert np.array_equal(bci, g[f"{name}/b{bs}/bci"]), (name, bs)
+            assert np.array_equal(v.view(np.int16), g[f"{name}/b{bs}/val"]), (name, bs)
+
+
+def test_blocked_ell_tuner_matches_reference(bell_gold):
+    g = bell_gold
+    for name in _cases(g):
+        rows, cols = (int(x) for x in g[f"{name}/meta"])
+        for k in range(3):
+            if f"{name}/choose{k}/cands" not in g.files:
+                continue
+            cands = g[f"{name}/choose{k}/cands"]
+            bs, fill, pad = O.choose_blocked_ell_block_size(rows, cols, g[f"{name}/ptr"], g[f"{name}/idx"], cands, 2)
+            rbs, rpad = (int(x) for x in g[f"{name}/choose{k}/result"])
+            assert (bs, pad) == (rbs, rpad), (name, k)
+            assert abs(fill - float(g[f"{name}/choose{k}/fill"][0])) < 1e-12
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="reference CPU converter not built (no reference tree)")
+def test_blocked_ell_matches_reference_converter_live():
+    for seed in range(6):
+        rows, cols = 20 + seed * 7, 30 + seed * 11
+        ptr, idx, val = random_csr(rows, cols, 0.15, seed=100 + seed, sorted_rows=seed % 2 == 0, zeros=0.05,
+                                   duplicates=0.05 * (seed % 2))
+        bits = f16_bits(val)
+        for bs in (1, 2, 5, 8, 16):
+            a = O.blocked_ell_from_compressed(rows, cols, ptr, idx, bits, bs)
+            r = O.ref_blocked_ell_from_compressed(rows, cols, ptr, idx, bits, bs)
+            assert a[2:] == r[2:]
+            assert np.array_equal(a[0], r[0]) and np.array_equal(a[1].view(np.uint16), r[1])
+
+
+def test_blocked_ell_dense_round_trip():
+    rows, cols, bs = 30, 40, 4
+    ptr, idx, val = random_csr(rows, cols, 0.2, seed=8, zeros=0.1)
+    bits = f16_bits(val)
+    bci, v, ell, nnz = O.blocked_ell_from_compressed(rows, cols, ptr, idx, bits, bs)
+    w = ell // bs
+    d = np.zeros((rows, cols))
+    vals = v.view(np.float16).astype(np.float64).reshape(rows, ell)
+    for r in range(rows):
+        for s in range(w):
+            b = bci[(r // bs) * w + s]
+            if b == O.INVALID:
+                continue
+            for lane in range(bs):
+                if b * bs + lane < cols:
+                    d[r, b * bs + lane] = vals[r, s * bs + lane]
+    ref = dense_of(rows, cols, ptr, idx, val)
+    assert np.array_equal(d, ref)
+    assert nnz == int((ref != 0).sum())
+
+
+def test_sliced_ell_layout_closed_form_and_round_trip():
+    rows, cols = 45, 30
+    ptr, idx, val = random_csr(rows, cols, 0.2, seed=9, empty_rows=0.2)
+    bits = f16_bits(val)
+    rn = np.diff(ptr)
+    for S in (1, 4, 7, 32, 100):
+        s = O.sliced_ell_from_compressed(rows, cols, 0, ptr, idx, bits, S)
+        Se = min(S, rows)
+        starts = np.arange(0, rows, Se)
+        assert np.array_equal(s["offsets"], np.append(starts, rows))
+        widths = np.array([rn[a:a + Se].max() for a in starts])
+        assert np.array_equal(s["widths"], widths)
+        heights = np.diff(np.append(starts, rows))
+        assert np.array_equal(s["slot_offsets"], np.concatenate([[0], np.cumsum(heights * widths)[:-1]]))
+        d = np.zeros((rows, cols))
+        vv = s["val"].view(np.float16).astype(np.float64)
+        for k, a in enumerate(starts):
+            for r in range(a, min(a + Se, rows)):
+                base = s["slot_offsets"][k] + (r - a) * widths[k]
+                row_cols = s["col_idx"][base:base + widths[k]]
+                n = rn[r]
+                assert np.array_equal(row_cols[:n], idx[ptr[r]:ptr[r + 1]])
+                assert np.all(row_cols[n:] == O.INVALID) and np.all(vv[base + n:base + widths[k]] == 0)
+                d[r, row_cols[:n]] = vv[base:base + n]
+        assert np.array_equal(d, dense_of(rows, cols, ptr, idx, val))
+
+
+def test_sliced_ell_from_csc_equals_from_csr():
+    rows, cols = 33, 28
+    ptr, idx, val = random_csr(rows, cols, 0.2, seed=10)
+    bits = f16_bits(val)
+    m = sp.csr_matrix((np.arange(1, len(idx) + 1), idx, ptr), shape=(rows, cols)).tocsc()
+    cbits = bits[m.data - 1]
+    a = O.sliced_ell_from_compressed(rows, cols, 0, ptr, idx, bits, 8)
+    b = O.sliced_ell_from_compressed(rows, cols, 1, m.indptr, m.indices, cbits, 8)
+    for k in a:
+        assert np.array_equal(np.asarray(a[k]), np.asarray(b[k])), k
+
+
+def test_sliced_ell_from_blocked_ell_compacts_nonzeros():
+    rows, cols, bs = 26, 31, 4
+    ptr, idx, val = random_csr(rows, cols, 0.2, seed=12, zeros=0.1)
+    bits = f16_bits(val)
+    bci, bv, ell, nnz = O.blocked_ell_from_compressed(rows, cols, ptr, idx, bits, bs)
+    s = O.sliced_ell_from_blocked_ell(rows, cols, bs, ell, bci, bv, 8)
+    # equals the SELL of the CSR with explicit zeros removed
+    keep = val != 0
+    rows_of = np.repeat(np.arange(rows), np.diff(ptr))
+    ptr2 = np.concatenate([[0], np.cumsum(np.bincount(rows_of[keep], minlength=rows))]).astype(np.uint32)
+    t = O.sliced_ell_from_compressed(rows, cols, 0, ptr2, idx[keep], bits[keep], 8)
+    for k in t:
+        assert np.array_equal(np.asarray(s[k]), np.asarray(t[k])), k
+
+
+def test_sliced_ell_tuner_closed_form():
+    rn = np.array([5, 0, 3, 9, 1, 1, 2, 8, 4, 4], np.uint32)
+    out = O.choose_sliced_ell_slice_rows(rn, len(rn), int(rn.sum()), [2, 4, 8])
+
+    def score(S):
+        pad = sum(int(rn[a:a + S].max()) * len(rn[a:a + S]) for a in range(0, len(rn), S))
+        cnt = -(-len(rn) // S)
+        return pad / rn.sum() + 0.25 * cnt / len(rn), pad
+    best = min((score(S)[0], S) for S in (2, 4, 8))
+    assert out["slice_rows"] == best[1]
+    assert abs(out["score"] - best[0]) < 1e-12
+
+
+def test_transpose_matches_scipy():
+    for seed, (rows, cols) in enumerate([(40, 30), (1, 7), (25, 1), (60, 200)]):
+        ptr, idx, val = random_csr(rows, cols, 0.2, seed=seed, empty_rows=0.1)
+        bits = f16_bits(val)
+        optr, oidx, oval = O.transpose(rows, cols, ptr, idx, bits)
+        m = sp.csr_matrix((np.arange(1, len(idx) + 1), idx, ptr), shape=(rows, cols)).tocsc()
+        m.sort_indices()
+        assert np.array_equal(optr, m.indptr) and np.array_equal(oidx, m.indices)
+        assert np.array_equal(oval, bits[m.data - 1])
+
+
+def test_bucket_plan_matches_stable_argsort():
+    rng = np.random.default_rng(7)
+    for n, req in [(1000, 12), (17, 5), (5, 9), (0, 3), (1, 1)]:
+        rn = rng.integers(0, 40, size=n).astype(np.uint32)
+        ptr = np.concatenate([[0], np.cumsum(rn)]).astype(np.uint32)
+        p = O.bucket_plan(ptr, n, req)
+        order = np.argsort(rn, kind="stable")
+        assert np.array_equal(p["major_order"], order) and np.array_equal(p["major_nnz_sorted"], rn[order])
+        b = 1 if n == 0 else min(max(req, 1), n)
+        assert p["bucket_count"] == b
+        if n:
+            sizes = np.diff(p["bucket_offsets"].astype(np.int64))
+            assert sizes.sum() == n and sizes.max() - sizes.min() <= 1 and np.all(np.diff(sizes) <= 0)
+
+
+def test_log2_buckets_closed_form():
+    vals = np.array([0, 1, 31, 32, 33, 63, 64, 1000, 32767, 32768, 40000, 1 << 20], np.uint32)
+    expect = [0, 0, 0, 1, 1, 1, 2, 5, 10, 11, 11, 11]
+    assert [O.log2_bucket(v) for v in vals] == expect
+    offs = O.log2_offsets(np.sort(vals))
+    assert np.array_equal(np.diff(offs), np.bincount(expect, minlength=12))
+
+
+def test_rebuild_invariants():
+    ptr, idx, val = random_csr(30, 20, 0.3, seed=13, empty_rows=0.2)
+    bits = f16_bits(val)
+    order = np.argsort(np.diff(ptr), kind="stable").astype(np.uint32)
+    dptr, didx, dval, inv = O.rebuild(ptr, idx, bits, order)
+    assert np.array_equal(inv[order], np.arange(30))
+    for d, s in enumerate(order):
+        assert np.array_equal(didx[dptr[d]:dptr[d + 1]], idx[ptr[s]:ptr[s + 1]])
+        assert np.array_equal(dval[dptr[d]:dptr[d + 1]], bits[ptr[s]:ptr[s + 1]])
+
+
+def test_bell_bucket_keys_closed_form():
+    rng = np.random.default_rng(3)
+    rb, w = 9, 6
+    bci = rng.integers(0, 50, size=rb * w).astype(np.uint32)
+    bci[rng.random(rb * w) < 0.3] = O.INVALID
+    keys = O.bell_bucket_keys(rb, w, bci)
+    for r in range(rb):
+        h, valid = 2166136261, 0
+        for c in bci[r * w:(r + 1) * w]:
+            if c == O.INVALID:
+                continue
+            valid += 1
+            h = (h ^ ((int(c) + 0x9E3779B9 + (h << 6) + (h >> 2)) & 0xFFFFFFFF)) & 0xFFFFFFFF
+            h = (h * 16777619) & 0xFFFFFFFF
+        assert int(keys[r]) == (valid << 32) | h
+
+
+def test_reoptimize_reference_fixture(mg):
+    b = mg["blocked_ell"]
+    r = mg["reoptimize"]
+    cfg = r["config"]
+    mp, mi, mv, nnz = O.reoptimize(mg["rows"], mg["cols"], b["block_size"], b["ell_cols"], b["block_col_idx"],
+                                   f16_bits(b["values"]).view(np.uint16), mg["row_keep"], mg["feature_keep"], r["row_remap"],
+                                   r["feature_remap"], cfg["output_rows"], cfg["output_cols"])
+    assert nnz == r["expected"]["live_nnz"]
+
+
+def test_reoptimize_export_fixture():
+    e = json.loads((GOLD / "export_bell_fixture.json").read_text())
+    mp, mi, mv, nnz = O.reoptimize(e["rows"], e["cols"], e["block_size"], e["ell_cols"], e["block_col_idx"],
+                                   f16_bits(e["values"]).view(np.uint16), None, None, None, None, e["rows"], e["cols"])
+    assert list(mp) == e["expected_csr"]["indptr"]
+    assert list(mi) == e["expected_csr"]["indices"]
+    assert list(mv.view(np.float16).astype(float)) == e["expected_csr"]["data"]
+
+
+def test_planted_mistake_is_caught():
+    """A dropped term in the oracle (simulated by zeroing one entry) must fail
+    the dense comparison used above."""
+    rows, cols = 10, 10
+    ptr, idx, val = random_csr(rows, cols, 0.4, seed=1)
+    bits = f16_bits(val)
+    o = O.group_reduce_csr(rows, cols, ptr, idx, bits, F16, None, None, None, 0, [0, 0, 1, 0])
+    d = dense_of(rows, cols, ptr, idx, val)
+    d[0, idx[ptr[0]]] = 0
+    assert not np.allclose(o["row_totals"], d.sum(1))
