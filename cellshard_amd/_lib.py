@@ -25,6 +25,7 @@ PROTOTYPES = {
     "csamd_last_error": (C.c_char_p, []),
     "csamd_abi_version": (i32, []),
     "csamd_device_count": (i32, []),
+    "csamd_set_device": (i32, [i32]),
     # mask_groups
     "csamd_group_reduce_init": (None, [P(A.sparse_group_reduce_workspace)]),
     "csamd_group_reduce_clear": (None, [P(A.sparse_group_reduce_workspace)]),
@@ -87,6 +88,32 @@ PROTOTYPES = {
     "csamd_fill_log2_bucket_offsets_raw": (i32, [vp, u32, u32, u32, vp, vp]),
     "csamd_rebuild_compressed_major_order_raw": (i32, [vp, vp, vp, u32, u32, vp, vp, vp, vp, vp, vp, sz, vp]),
     "csamd_blocked_ell_bucket_plan_raw": (i32, [vp, u32, u32, vp, vp, vp, P(A.convert_workspace)]),
+    # host-struct converters
+    "csamd_host_compressed_clear": (None, [P(A.host_compressed)]),
+    "csamd_host_blocked_ell_clear": (None, [P(A.host_blocked_ell)]),
+    "csamd_host_sliced_ell_clear": (None, [P(A.host_sliced_ell)]),
+    "csamd_host_choose_blocked_ell_block_size": (i32, [P(A.host_compressed), P(u32), u32, P(A.blocked_ell_tune_result)]),
+    "csamd_host_blocked_ell_from_compressed": (i32, [P(A.host_compressed), u32, P(A.host_blocked_ell)]),
+    "csamd_host_blocked_ell_from_compressed_auto": (i32, [P(A.host_compressed), P(u32), u32, P(A.host_blocked_ell),
+                                                          P(A.blocked_ell_tune_result)]),
+    "csamd_host_choose_sliced_ell_slice_rows": (i32, [P(A.host_compressed), P(u32), u32, P(A.sliced_ell_tune_result)]),
+    "csamd_host_sliced_ell_from_compressed": (i32, [P(A.host_compressed), u32, P(A.host_sliced_ell)]),
+    "csamd_host_sliced_ell_from_compressed_auto": (i32, [P(A.host_compressed), P(u32), u32, P(A.host_sliced_ell),
+                                                         P(A.sliced_ell_tune_result)]),
+    "csamd_host_sliced_ell_from_blocked_ell": (i32, [P(A.host_blocked_ell), u32, P(A.host_sliced_ell)]),
+    # staging
+    "csamd_allocation_cache_init": (None, [P(A.allocation_cache)]),
+    "csamd_allocation_cache_clear": (None, [P(A.allocation_cache)]),
+    "csamd_take_cached_allocation": (vp, [P(A.allocation_cache), i32, sz, P(sz)]),
+    "csamd_cache_allocation": (i32, [P(A.allocation_cache), i32, vp, sz]),
+    "csamd_upload_compressed": (i32, [P(A.host_compressed), P(A.partition_record), P(A.compressed_view), P(A.allocation_cache), vp]),
+    "csamd_upload_blocked_ell": (i32, [P(A.host_blocked_ell), P(A.partition_record), P(A.blocked_ell_view), P(A.allocation_cache), vp]),
+    "csamd_upload_sliced_ell": (i32, [P(A.host_sliced_ell), P(A.partition_record), P(A.sliced_ell_view), P(A.allocation_cache), vp]),
+    "csamd_release_record": (i32, [P(A.partition_record), P(A.allocation_cache)]),
+    "csamd_upload_blocked_ell_shard": (i32, [P(P(A.host_blocked_ell)), u32, P(A.partition_record), P(A.blocked_ell_view),
+                                             P(A.allocation_cache), vp]),
+    "csamd_upload_compressed_shard": (i32, [P(P(A.host_compressed)), u32, P(A.partition_record), P(A.compressed_view),
+                                            P(A.allocation_cache), vp]),
     # synth / scan
     "csamd_synth_row_nnz": (i32, [P(A.synth_params), vp, vp]),
     "csamd_exclusive_scan_u32": (i32, [vp, u32, vp, P(A.convert_workspace), P(u64)]),

@@ -168,6 +168,34 @@ synth_params = _aligned16("synth_params", [
     ("d_col_inv_weight", c_void_p), ("d_col_mean", c_void_p),
 ])
 
+host_compressed = _aligned16("host_compressed", [
+    ("rows", c_u32), ("cols", c_u32), ("nnz", c_u32), ("axis", c_u32), ("flags", c_u32),
+    ("storage", c_void_p), ("majorPtr", c_void_p), ("minorIdx", c_void_p), ("val", c_void_p),
+])
+
+host_blocked_ell = _aligned16("host_blocked_ell", [
+    ("rows", c_u32), ("cols", c_u32), ("nnz", c_u32), ("block_size", c_u32), ("ell_cols", c_u32), ("flags", c_u32),
+    ("storage", c_void_p), ("blockColIdx", c_void_p), ("val", c_void_p),
+])
+
+host_sliced_ell = _aligned16("host_sliced_ell", [
+    ("rows", c_u32), ("cols", c_u32), ("nnz", c_u32), ("slice_count", c_u32), ("flags", c_u32),
+    ("storage", c_void_p), ("slice_row_offsets", c_void_p), ("slice_widths", c_void_p), ("col_idx", c_void_p),
+    ("val", c_void_p),
+])
+
+partition_record = _aligned16("partition_record", [
+    ("allocation", c_void_p), ("allocation_bytes", c_size_t), ("storage", c_void_p), ("view", c_void_p),
+    ("a0", c_void_p), ("a1", c_void_p), ("a2", c_void_p), ("a3", c_void_p),
+    ("group_begin", C.c_ulong), ("group_end", C.c_ulong), ("device_id", C.c_int),
+])
+
+cached_allocation = _aligned16("cached_allocation", [("ptr", c_void_p), ("bytes", c_size_t), ("device_id", C.c_int)])
+
+allocation_cache = _aligned16("allocation_cache", [
+    ("cache_count", C.c_ulong), ("cache_capacity", C.c_ulong), ("cache", C.POINTER(cached_allocation)),
+])
+
 # Structs whose layout tests/test_abi.py compares against the C header.
 ALL_STRUCTS = {
     "csamd_compressed_view": compressed_view,
@@ -195,4 +223,10 @@ ALL_STRUCTS = {
     "csamd_sliced_ell_tune_result": sliced_ell_tune_result,
     "csamd_major_nnz_bucket_plan_view": major_nnz_bucket_plan_view,
     "csamd_synth_params": synth_params,
+    "csamd_host_compressed": host_compressed,
+    "csamd_host_blocked_ell": host_blocked_ell,
+    "csamd_host_sliced_ell": host_sliced_ell,
+    "csamd_partition_record": partition_record,
+    "csamd_cached_allocation": cached_allocation,
+    "csamd_allocation_cache": allocation_cache,
 }

@@ -29,6 +29,8 @@ HIP_SOURCES = [
     "transpose.hip",
     "bucket.hip",
     "synth.hip",
+    "staging.hip",
+    "host_convert.hip",
 ]
 
 HIPCC_FLAGS = [

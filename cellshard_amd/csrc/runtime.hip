@@ -364,6 +364,8 @@ int csamd_device_count(void) {
     return n;
 }
 
+int csamd_set_device(int device) { return csamd::hip_ok(hipSetDevice(device), "hipSetDevice"); }
+
 void csamd_convert_init(csamd_convert_workspace *ws) {
     if (ws == nullptr) return;
     std::memset(ws, 0, sizeof(*ws));

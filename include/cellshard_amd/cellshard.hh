@@ -1,0 +1,507 @@
+// C++ drop-in layer over the cellshard_amd C ABI, with the reference's
+// namespaces, type names and function signatures for the sparse hot path
+// (cudaStream_t -> hipStream_t).  Header-only; link libcellshard_amd.so.
+//
+//   cellshard::sparse    compressed / blocked_ell / sliced_ell host structs
+//                        (formats/{compressed,blocked_ell,sliced_ell}.cuh)
+//   cellshard::device    compressed_view / blocked_ell_view / sliced_ell_view,
+//                        partition_record, upload/release, allocation cache
+//                        (runtime/device/sharded_device.cuh)
+//   cellshard::runtime   mask_groups (runtime/mask_groups.cuh)
+//   cellshard::convert   Blocked-ELL / Sliced-ELL converters and tuners,
+//                        build_compressed_transpose_raw (src/convert/)
+//   cellshard::bucket    major-nnz bucket plan / rebuild (src/bucket/major_nnz_raw.cuh)
+//   cellshard::sharded   partition / shard metadata (runtime/layout/sharded.cuh)
+#pragma once
+
+#if defined(__HIPCC__)
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+#ifndef CSAMD_HALF_T
+#define CSAMD_HALF_T __half
+#endif
+#endif
+
+#include "cellshard_amd.h"
+
+#include <cstddef>
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+
+#if !defined(__HIPCC__)
+#if defined(__has_include) && __has_include(<hip/hip_runtime_api.h>) && defined(__HIP_PLATFORM_AMD__)
+#include <hip/hip_runtime_api.h>
+#else
+typedef struct ihipStream_t *hipStream_t;
+typedef int hipError_t;
+#endif
+#endif
+
+namespace cellshard {
+
+namespace types {
+using u32 = std::uint32_t;
+using i32 = std::int32_t;
+using u64 = std::uint64_t;
+using dim_t = u32;
+using nnz_t = u32;
+using idx_t = u32;
+using ptr_t = u32;
+} // namespace types
+
+namespace real {
+using storage_t = csamd_half;
+using compute_t = float;
+using accum_t = float;
+enum { value_f16 = CSAMD_VALUE_F16, value_f32 = CSAMD_VALUE_F32, value_f64 = CSAMD_VALUE_F64,
+       value_bf16 = CSAMD_VALUE_BF16, value_fp8_e4m3 = CSAMD_VALUE_FP8_E4M3, value_fp8_e5m2 = CSAMD_VALUE_FP8_E5M2 };
+} // namespace real
+
+// Binary search over offsets[i] <= row < offsets[i+1] (core/offset_span.cuh:12).
+template<typename OffsetT>
+inline OffsetT find_offset_span(OffsetT row, const OffsetT *offsets, OffsetT count) {
+    OffsetT lo = 0, hi = count;
+    while (lo < hi) {
+        const OffsetT mid = lo + ((hi - lo) >> 1);
+        if (row < offsets[mid]) hi = mid;
+        else if (row >= offsets[mid + 1]) lo = mid + 1;
+        else return mid;
+    }
+    return count;
+}
+
+// ---------------------------------------------------------------------------
+namespace sparse {
+
+enum { compressed_by_row = CSAMD_COMPRESSED_BY_ROW, compressed_by_col = CSAMD_COMPRESSED_BY_COL };
+enum { compressed_host_registered = 1u, blocked_ell_host_registered = 1u, sliced_ell_host_registered = 1u };
+enum : std::uint32_t { blocked_ell_invalid_col = CSAMD_INVALID_COL, sliced_ell_invalid_col = CSAMD_INVALID_COL };
+
+using compressed = csamd_host_compressed;
+using blocked_ell = csamd_host_blocked_ell;
+using sliced_ell = csamd_host_sliced_ell;
+
+inline types::dim_t major_dim(const compressed *m) { return m->axis == compressed_by_col ? m->cols : m->rows; }
+inline types::dim_t minor_dim(const compressed *m) { return m->axis == compressed_by_col ? m->rows : m->cols; }
+
+inline void init(compressed *m, types::dim_t rows = 0, types::dim_t cols = 0, types::nnz_t nnz = 0,
+                 types::u32 axis = compressed_by_row) {
+    std::memset(m, 0, sizeof(*m));
+    m->rows = rows;
+    m->cols = cols;
+    m->nnz = nnz;
+    m->axis = axis;
+}
+inline void init(blocked_ell *m, types::dim_t rows = 0, types::dim_t cols = 0, types::nnz_t nnz = 0, types::u32 block_size = 0,
+                 types::u32 ell_cols = 0) {
+    std::memset(m, 0, sizeof(*m));
+    m->rows = rows;
+    m->cols = cols;
+    m->nnz = nnz;
+    m->block_size = block_size;
+    m->ell_cols = ell_cols;
+}
+inline void init(sliced_ell *m, types::dim_t rows = 0, types::dim_t cols = 0, types::nnz_t nnz = 0) {
+    std::memset(m, 0, sizeof(*m));
+    m->rows = rows;
+    m->cols = cols;
+    m->nnz = nnz;
+}
+
+inline void clear(compressed *m) { csamd_host_compressed_clear(m); }
+inline void clear(blocked_ell *m) { csamd_host_blocked_ell_clear(m); }
+inline void clear(sliced_ell *m) { csamd_host_sliced_ell_clear(m); }
+
+inline std::size_t align_up_(std::size_t v, std::size_t a) { return (v + a - 1u) / a * a; }
+
+// formats/compressed.cuh:122-153 (one packed block: ptr | idx | val)
+inline int allocate(compressed *m) {
+    const std::size_t ptr_count = (std::size_t) major_dim(m) + 1u;
+    const std::size_t minor_off = align_up_(ptr_count * 4u, 4u);
+    const std::size_t val_off = align_up_(minor_off + (std::size_t) m->nnz * 4u, 2u);
+    const std::size_t total = val_off + (std::size_t) m->nnz * 2u;
+    const types::dim_t rows = m->rows, cols = m->cols, nnz = m->nnz;
+    const types::u32 axis = m->axis;
+    clear(m);
+    m->rows = rows;
+    m->cols = cols;
+    m->nnz = nnz;
+    m->axis = axis;
+    if (total == 0u) return 1;
+    m->storage = std::malloc(total);
+    if (m->storage == nullptr) return 0;
+    m->majorPtr = (types::ptr_t *) m->storage;
+    m->minorIdx = nnz != 0u ? (types::idx_t *) ((char *) m->storage + minor_off) : nullptr;
+    m->val = nnz != 0u ? (real::storage_t *) ((char *) m->storage + val_off) : nullptr;
+    return 1;
+}
+
+inline types::dim_t row_block_count(const blocked_ell *m) { return m->block_size == 0u ? 0u : (m->rows + m->block_size - 1u) / m->block_size; }
+inline types::dim_t col_block_count(const blocked_ell *m) { return m->block_size == 0u ? 0u : (m->cols + m->block_size - 1u) / m->block_size; }
+inline types::u32 ell_width_blocks(const blocked_ell *m) {
+    return (m->block_size == 0u || m->ell_cols == 0u) ? 0u : m->ell_cols / m->block_size;
+}
+inline unsigned long pack_blocked_ell_aux(types::u32 block_size, unsigned long ell_width) {
+    return (ell_width << 16u) | (unsigned long) block_size;
+}
+
+// formats/blocked_ell.cuh:134-158
+inline int allocate(blocked_ell *m) {
+    const std::size_t idx_bytes = (std::size_t) row_block_count(m) * ell_width_blocks(m) * 4u;
+    const std::size_t val_off = align_up_(idx_bytes, 2u);
+    const std::size_t total = val_off + (std::size_t) m->rows * m->ell_cols * 2u;
+    blocked_ell keep = *m;
+    keep.storage = nullptr;
+    clear(m);
+    *m = keep;
+    m->flags = 0u;
+    m->blockColIdx = nullptr;
+    m->val = nullptr;
+    if (total == 0u) return 1;
+    m->storage = std::malloc(total);
+    if (m->storage == nullptr) return 0;
+    m->blockColIdx = idx_bytes != 0u ? (types::idx_t *) m->storage : nullptr;
+    m->val = m->rows != 0u && m->ell_cols != 0u ? (real::storage_t *) ((char *) m->storage + val_off) : nullptr;
+    return 1;
+}
+
+inline types::u32 total_slots(const sliced_ell *m) {
+    types::u32 t = 0u;
+    if (m == nullptr || m->slice_row_offsets == nullptr || m->slice_widths == nullptr) return 0u;
+    for (types::u32 s = 0; s < m->slice_count; ++s) t += (m->slice_row_offsets[s + 1u] - m->slice_row_offsets[s]) * m->slice_widths[s];
+    return t;
+}
+inline unsigned long pack_sliced_ell_aux(types::u32 slice_count, types::u32 slots) {
+    return ((unsigned long) slots << 32u) | (unsigned long) slice_count;
+}
+inline types::u32 uniform_slice_rows(const sliced_ell *m) {
+    if (m == nullptr || m->slice_count == 0u || m->slice_row_offsets == nullptr) return 0u;
+    if (m->slice_count == 1u) return m->rows;
+    const types::u32 step = m->slice_row_offsets[1] - m->slice_row_offsets[0];
+    if (step == 0u) return 0u;
+    for (types::u32 s = 1; s + 1u < m->slice_count; ++s)
+        if (m->slice_row_offsets[s + 1u] - m->slice_row_offsets[s] != step) return 0u;
+    if (m->slice_row_offsets[m->slice_count] - m->slice_row_offsets[m->slice_count - 1u] > step) return 0u;
+    return step;
+}
+
+// formats/sliced_ell.cuh:165-224
+inline int allocate(sliced_ell *m, types::u32 slice_count, const types::u32 *offsets, const types::u32 *widths) {
+    if (m == nullptr || offsets == nullptr || widths == nullptr) return 0;
+    if (slice_count != 0u && offsets[slice_count] != m->rows) return 0;
+    std::size_t total = 0u;
+    for (types::u32 s = 0; s < slice_count; ++s) total += (std::size_t) (offsets[s + 1u] - offsets[s]) * widths[s];
+    const std::size_t w_off = align_up_((std::size_t) (slice_count + 1u) * 4u, 4u);
+    const std::size_t col_off = align_up_(w_off + (std::size_t) slice_count * 4u, 4u);
+    const std::size_t val_off = align_up_(col_off + total * 4u, 2u);
+    const std::size_t bytes = val_off + total * 2u;
+    const types::dim_t rows = m->rows, cols = m->cols, nnz = m->nnz;
+    clear(m);
+    m->rows = rows;
+    m->cols = cols;
+    m->nnz = nnz;
+    m->slice_count = slice_count;
+    if (bytes == 0u) return 1;
+    m->storage = std::malloc(bytes);
+    if (m->storage == nullptr) {
+        m->slice_count = 0u;
+        return 0;
+    }
+    char *b = (char *) m->storage;
+    m->slice_row_offsets = (types::u32 *) b;
+    m->slice_widths = (types::u32 *) (b + w_off);
+    m->col_idx = (types::idx_t *) (b + col_off);
+    m->val = (real::storage_t *) (b + val_off);
+    std::memcpy(m->slice_row_offsets, offsets, (std::size_t) (slice_count + 1u) * 4u);
+    if (slice_count != 0u) std::memcpy(m->slice_widths, widths, (std::size_t) slice_count * 4u);
+    for (std::size_t k = 0; k < total; ++k) m->col_idx[k] = sliced_ell_invalid_col;
+    if (total != 0u) std::memset((void *) m->val, 0, total * 2u);
+    return 1;
+}
+
+} // namespace sparse
+
+// ---------------------------------------------------------------------------
+namespace device {
+
+using compressed_view = csamd_compressed_view;
+using blocked_ell_view = csamd_blocked_ell_view;
+using sliced_ell_view = csamd_sliced_ell_view;
+
+template<typename MatrixT>
+struct partition_record : csamd_partition_record {};
+
+using cached_allocation = csamd_cached_allocation;
+
+template<typename MatrixT>
+inline void zero_record(partition_record<MatrixT> *r) {
+    std::memset(r, 0, sizeof(*r));
+    r->device_id = -1;
+}
+
+// runtime/device/sharded_device.cuh:343-755: one packed device allocation,
+// H2D copies of the arrays and of the view descriptor.  `cache` may be null.
+inline hipError_t upload(const sparse::compressed *src, partition_record<sparse::compressed> *r, csamd_allocation_cache *cache = nullptr) {
+    return csamd_upload_compressed(src, r, nullptr, cache, nullptr) ? (hipError_t) 0 : (hipError_t) 1;
+}
+inline hipError_t upload_async(const sparse::compressed *src, partition_record<sparse::compressed> *r, hipStream_t s,
+                               csamd_allocation_cache *cache = nullptr) {
+    return csamd_upload_compressed(src, r, nullptr, cache, (csamd_stream_t) s) ? (hipError_t) 0 : (hipError_t) 1;
+}
+inline hipError_t upload(const sparse::blocked_ell *src, partition_record<sparse::blocked_ell> *r, csamd_allocation_cache *cache = nullptr) {
+    return csamd_upload_blocked_ell(src, r, nullptr, cache, nullptr) ? (hipError_t) 0 : (hipError_t) 1;
+}
+inline hipError_t upload_async(const sparse::blocked_ell *src, partition_record<sparse::blocked_ell> *r, hipStream_t s,
+                               csamd_allocation_cache *cache = nullptr) {
+    return csamd_upload_blocked_ell(src, r, nullptr, cache, (csamd_stream_t) s) ? (hipError_t) 0 : (hipError_t) 1;
+}
+inline hipError_t upload(const sparse::sliced_ell *src, partition_record<sparse::sliced_ell> *r, csamd_allocation_cache *cache = nullptr) {
+    return csamd_upload_sliced_ell(src, r, nullptr, cache, nullptr) ? (hipError_t) 0 : (hipError_t) 1;
+}
+inline hipError_t upload_async(const sparse::sliced_ell *src, partition_record<sparse::sliced_ell> *r, hipStream_t s,
+                               csamd_allocation_cache *cache = nullptr) {
+    return csamd_upload_sliced_ell(src, r, nullptr, cache, (csamd_stream_t) s) ? (hipError_t) 0 : (hipError_t) 1;
+}
+template<typename MatrixT>
+inline hipError_t release(partition_record<MatrixT> *r, csamd_allocation_cache *cache = nullptr) {
+    return csamd_release_record(r, cache) ? (hipError_t) 0 : (hipError_t) 1;
+}
+
+} // namespace device
+
+// ---------------------------------------------------------------------------
+namespace runtime {
+
+static constexpr unsigned int CELLSHARD_MAX_MASK_GROUPS = CSAMD_MAX_MASK_GROUPS;
+
+using sparse_mask_layout = csamd_sparse_mask_layout;
+static constexpr sparse_mask_layout sparse_mask_layout_unknown = CSAMD_LAYOUT_UNKNOWN;
+static constexpr sparse_mask_layout sparse_mask_layout_blocked_ell = CSAMD_LAYOUT_BLOCKED_ELL;
+static constexpr sparse_mask_layout sparse_mask_layout_sliced_ell = CSAMD_LAYOUT_SLICED_ELL;
+static constexpr sparse_mask_layout sparse_mask_layout_compressed = CSAMD_LAYOUT_COMPRESSED;
+
+using group_mask_config_view = csamd_group_mask_config_view;
+using row_feature_mask_view = csamd_row_feature_mask_view;
+using sparse_group_filter_params = csamd_sparse_group_filter_params;
+using sparse_group_reduce_result = csamd_sparse_group_reduce_result;
+using sparse_group_reduce_workspace = csamd_sparse_group_reduce_workspace;
+using sparse_group_reduce_fleet_result = csamd_sparse_group_reduce_fleet_result;
+using sparse_group_reduce_fleet_workspace = csamd_sparse_group_reduce_fleet_workspace;
+using sparse_group_reduce_fleet_config = csamd_sparse_group_reduce_fleet_config;
+using masked_sparse_reoptimize_config = csamd_masked_sparse_reoptimize_config;
+using masked_sparse_reoptimize_metadata = csamd_masked_sparse_reoptimize_metadata;
+using masked_sparse_reoptimize_result = csamd_masked_sparse_reoptimize_result;
+using masked_sparse_reoptimize_workspace = csamd_masked_sparse_reoptimize_workspace;
+using auto_reoptimize_prediction_input = csamd_auto_reoptimize_prediction_input;
+using auto_reoptimize_prediction = csamd_auto_reoptimize_prediction;
+
+inline int predict_masked_sparse_reoptimization(const auto_reoptimize_prediction_input *in, auto_reoptimize_prediction *out) {
+    return csamd_predict_masked_sparse_reoptimization(in, out);
+}
+
+inline void init(sparse_group_reduce_workspace *w) { csamd_group_reduce_init(w); }
+inline void init(sparse_group_reduce_fleet_workspace *f) { csamd_fleet_init(f); }
+inline void init(masked_sparse_reoptimize_workspace *w) { csamd_reoptimize_init(w); }
+inline void clear(sparse_group_reduce_workspace *w) { csamd_group_reduce_clear(w); }
+inline void clear(sparse_group_reduce_fleet_workspace *f) { csamd_fleet_clear(f); }
+inline void clear(masked_sparse_reoptimize_workspace *w) { csamd_reoptimize_clear(w); }
+
+inline int setup(sparse_group_reduce_workspace *w, int device, hipStream_t stream = (hipStream_t) 0) {
+    return csamd_group_reduce_setup(w, device, (csamd_stream_t) stream);
+}
+inline int setup_fleet(sparse_group_reduce_fleet_workspace *f, const sparse_group_reduce_fleet_config *config = nullptr) {
+    return csamd_setup_fleet(f, config);
+}
+inline int setup(masked_sparse_reoptimize_workspace *w, int device, hipStream_t stream = (hipStream_t) 0) {
+    return csamd_reoptimize_setup(w, device, (csamd_stream_t) stream);
+}
+inline int reserve(sparse_group_reduce_workspace *w, unsigned int rows, unsigned int cols, unsigned int values, unsigned int groups) {
+    return csamd_group_reduce_reserve(w, rows, cols, values, groups);
+}
+inline int upload_feature_group_masks(sparse_group_reduce_workspace *w, unsigned int cols, const std::uint32_t *host_masks) {
+    return csamd_upload_feature_group_masks(w, cols, host_masks);
+}
+inline int compute_sparse_group_reduce(const device::blocked_ell_view *src, sparse_group_reduce_workspace *w,
+                                       const group_mask_config_view *groups, const row_feature_mask_view *masks,
+                                       const sparse_group_filter_params *filter, sparse_group_reduce_result *out) {
+    return csamd_compute_sparse_group_reduce_blocked_ell(src, w, groups, masks, filter, out);
+}
+inline int compute_sparse_group_reduce(const device::sliced_ell_view *src, sparse_group_reduce_workspace *w,
+                                       const group_mask_config_view *groups, const row_feature_mask_view *masks,
+                                       const sparse_group_filter_params *filter, sparse_group_reduce_result *out) {
+    return csamd_compute_sparse_group_reduce_sliced_ell(src, w, groups, masks, filter, out);
+}
+inline int compute_sparse_group_reduce_compressed_fallback(const device::compressed_view *src, sparse_group_reduce_workspace *w,
+                                                           const group_mask_config_view *groups, const row_feature_mask_view *masks,
+                                                           const sparse_group_filter_params *filter, sparse_group_reduce_result *out) {
+    return csamd_compute_sparse_group_reduce_compressed(src, w, groups, masks, filter, out);
+}
+inline int compute_sparse_group_reduce_fleet(const device::blocked_ell_view *src_by_slot, sparse_group_reduce_fleet_workspace *f,
+                                             const group_mask_config_view *groups, const row_feature_mask_view *masks,
+                                             const sparse_group_filter_params *filter, sparse_group_reduce_fleet_result *out) {
+    return csamd_compute_sparse_group_reduce_fleet_blocked_ell(src_by_slot, f, groups, masks, filter, out);
+}
+inline int compute_sparse_group_reduce_fleet(const device::sliced_ell_view *src_by_slot, sparse_group_reduce_fleet_workspace *f,
+                                             const group_mask_config_view *groups, const row_feature_mask_view *masks,
+                                             const sparse_group_filter_params *filter, sparse_group_reduce_fleet_result *out) {
+    return csamd_compute_sparse_group_reduce_fleet_sliced_ell(src_by_slot, f, groups, masks, filter, out);
+}
+inline int manual_reoptimize_masked_sparse(const device::blocked_ell_view *src, const row_feature_mask_view *masks,
+                                           const masked_sparse_reoptimize_config *config, masked_sparse_reoptimize_workspace *w,
+                                           masked_sparse_reoptimize_result *out) {
+    return csamd_manual_reoptimize_masked_sparse(src, masks, config, w, out);
+}
+
+} // namespace runtime
+
+// ---------------------------------------------------------------------------
+namespace convert {
+
+using blocked_ell_tune_result = csamd_blocked_ell_tune_result;
+using sliced_ell_tune_result = csamd_sliced_ell_tune_result;
+
+inline int choose_blocked_ell_block_size(const sparse::compressed *src, const unsigned int *candidates, unsigned int count,
+                                         blocked_ell_tune_result *out) {
+    return csamd_host_choose_blocked_ell_block_size(src, candidates, count, out);
+}
+inline int blocked_ell_from_compressed(const sparse::compressed *src, unsigned int block_size, sparse::blocked_ell *dst) {
+    return csamd_host_blocked_ell_from_compressed(src, block_size, dst);
+}
+inline int blocked_ell_from_compressed_auto(const sparse::compressed *src, const unsigned int *candidates, unsigned int count,
+                                            sparse::blocked_ell *dst, blocked_ell_tune_result *picked = nullptr) {
+    return csamd_host_blocked_ell_from_compressed_auto(src, candidates, count, dst, picked);
+}
+inline int choose_sliced_ell_slice_rows_from_compressed(const sparse::compressed *src, const unsigned int *candidates,
+                                                        unsigned int count, sliced_ell_tune_result *out) {
+    return csamd_host_choose_sliced_ell_slice_rows(src, candidates, count, out);
+}
+inline int sliced_ell_from_compressed(const sparse::compressed *src, unsigned int slice_rows, sparse::sliced_ell *dst) {
+    return csamd_host_sliced_ell_from_compressed(src, slice_rows, dst);
+}
+inline int sliced_ell_from_compressed_cuda(const sparse::compressed *src, unsigned int slice_rows, sparse::sliced_ell *dst,
+                                           int device = 0, hipStream_t stream = (hipStream_t) 0) {
+    (void) stream;
+    if (!csamd_set_device(device)) return 0;
+    return csamd_host_sliced_ell_from_compressed(src, slice_rows, dst);
+}
+inline int sliced_ell_from_compressed_auto(const sparse::compressed *src, const unsigned int *candidates, unsigned int count,
+                                           sparse::sliced_ell *dst, sliced_ell_tune_result *picked = nullptr) {
+    return csamd_host_sliced_ell_from_compressed_auto(src, candidates, count, dst, picked);
+}
+inline int sliced_ell_from_blocked_ell(const sparse::blocked_ell *src, unsigned int slice_rows, sparse::sliced_ell *dst) {
+    return csamd_host_sliced_ell_from_blocked_ell(src, slice_rows, dst);
+}
+inline int sliced_ell_from_blocked_ell_cuda(const sparse::blocked_ell *src, unsigned int slice_rows, sparse::sliced_ell *dst,
+                                            int device = 0, hipStream_t stream = (hipStream_t) 0) {
+    (void) stream;
+    if (!csamd_set_device(device)) return 0;
+    return csamd_host_sliced_ell_from_blocked_ell(src, slice_rows, dst);
+}
+
+// src/convert/compressed_transpose_raw.cuh:14-27 (same parameters; d_heads is
+// unused, d_scan_tmp must hold csamd_compressed_transpose_scratch_bytes()).
+static inline int build_compressed_transpose_raw(types::dim_t cDim, types::dim_t uDim, types::nnz_t nnz, const types::ptr_t *d_cAxPtr,
+                                                 const types::idx_t *d_uAxIdx, const real::storage_t *d_val, types::ptr_t *d_out_uAxPtr,
+                                                 types::ptr_t *d_heads, types::idx_t *d_out_cAxIdx, real::storage_t *d_out_val,
+                                                 void *d_scan_tmp, std::size_t scan_bytes, hipStream_t stream) {
+    return csamd_build_compressed_transpose_raw(cDim, uDim, nnz, d_cAxPtr, d_uAxIdx, d_val, d_out_uAxPtr, d_heads, d_out_cAxIdx,
+                                                d_out_val, d_scan_tmp, scan_bytes, (csamd_stream_t) stream);
+}
+static inline int build_transpose_cs_from_cs_raw(types::dim_t cDim, types::dim_t uDim, types::nnz_t nnz, const types::ptr_t *a,
+                                                 const types::idx_t *b, const real::storage_t *c, types::ptr_t *d, types::ptr_t *e,
+                                                 types::idx_t *f, real::storage_t *g, void *h, std::size_t i, hipStream_t s) {
+    return build_compressed_transpose_raw(cDim, uDim, nnz, a, b, c, d, e, f, g, h, i, s);
+}
+
+} // namespace convert
+
+// ---------------------------------------------------------------------------
+namespace bucket {
+
+using major_nnz_bucket_plan_view = csamd_major_nnz_bucket_plan_view;
+
+static inline types::idx_t clamp_bucket_count(types::dim_t major_dim, types::idx_t requested) {
+    return csamd_clamp_bucket_count(major_dim, requested);
+}
+static inline int major_nnz_bucket_sort_scratch_bytes(types::dim_t major_dim, std::size_t *out) {
+    return csamd_major_nnz_bucket_sort_scratch_bytes(major_dim, out);
+}
+static inline int major_nnz_bucket_scan_scratch_bytes(types::dim_t major_dim, std::size_t *out) {
+    return csamd_major_nnz_bucket_scan_scratch_bytes(major_dim, out);
+}
+static inline int build_major_nnz_bucket_plan_raw(const types::ptr_t *d_major_ptr, types::dim_t major_dim, types::idx_t *d_major_nnz,
+                                                  types::idx_t *d_major_nnz_sorted, types::idx_t *d_major_order_in,
+                                                  types::idx_t *d_major_order_out, types::idx_t *d_bucket_offsets,
+                                                  types::idx_t requested_bucket_count, void *d_sort_tmp, std::size_t sort_tmp_bytes,
+                                                  hipStream_t stream) {
+    return csamd_build_major_nnz_bucket_plan_raw(d_major_ptr, major_dim, d_major_nnz, d_major_nnz_sorted, d_major_order_in,
+                                                 d_major_order_out, d_bucket_offsets, requested_bucket_count, d_sort_tmp,
+                                                 sort_tmp_bytes, (csamd_stream_t) stream);
+}
+static inline int rebuild_compressed_major_order_raw(const types::ptr_t *d_src_major_ptr, const types::idx_t *d_src_minor_idx,
+                                                     const real::storage_t *d_src_val, types::dim_t major_dim,
+                                                     const types::idx_t *d_major_order, types::ptr_t *d_dst_major_ptr,
+                                                     types::idx_t *d_dst_minor_idx, real::storage_t *d_dst_val,
+                                                     types::idx_t *d_inverse_major_order, void *d_scan_tmp,
+                                                     std::size_t scan_tmp_bytes, hipStream_t stream) {
+    return csamd_rebuild_compressed_major_order_raw(d_src_major_ptr, d_src_minor_idx, d_src_val, CSAMD_VALUE_F16, major_dim,
+                                                    d_major_order, d_dst_major_ptr, d_dst_minor_idx, d_dst_val,
+                                                    d_inverse_major_order, d_scan_tmp, scan_tmp_bytes, (csamd_stream_t) stream);
+}
+
+} // namespace bucket
+
+// ---------------------------------------------------------------------------
+// sharded<MatrixT>: partition / shard metadata (runtime/layout/sharded.cuh:30-252).
+template<typename MatrixT>
+struct alignas(16) sharded {
+    unsigned long rows;
+    unsigned long cols;
+    unsigned long nnz;
+    unsigned long num_partitions;
+    unsigned long partition_capacity;
+    MatrixT **parts;
+    unsigned long *partition_offsets;
+    unsigned long *partition_rows;
+    unsigned long *partition_nnz;
+    unsigned long *partition_aux;
+    unsigned long num_shards;
+    unsigned long shard_capacity;
+    unsigned long *shard_offsets;
+    unsigned long *shard_parts;
+};
+
+template<typename MatrixT>
+inline void init(sharded<MatrixT> *m) {
+    std::memset(m, 0, sizeof(*m));
+}
+
+template<typename MatrixT>
+inline unsigned long find_partition(const sharded<MatrixT> *m, unsigned long row) {
+    if (m->partition_offsets == nullptr || m->num_partitions == 0) return m->num_partitions;
+    return find_offset_span(row, m->partition_offsets, m->num_partitions);
+}
+template<typename MatrixT>
+inline unsigned long find_shard(const sharded<MatrixT> *m, unsigned long row) {
+    if (m->shard_offsets == nullptr || m->num_shards == 0) return m->num_shards;
+    return find_offset_span(row, m->shard_offsets, m->num_shards);
+}
+template<typename MatrixT>
+inline unsigned long first_row_in_partition(const sharded<MatrixT> *m, unsigned long p) {
+    return (p >= m->num_partitions || m->partition_offsets == nullptr) ? m->rows : m->partition_offsets[p];
+}
+template<typename MatrixT>
+inline unsigned long last_row_in_partition(const sharded<MatrixT> *m, unsigned long p) {
+    return (p >= m->num_partitions || m->partition_offsets == nullptr) ? m->rows : m->partition_offsets[p + 1];
+}
+template<typename MatrixT>
+inline unsigned long first_partition_in_shard(const sharded<MatrixT> *m, unsigned long s) {
+    return (s >= m->num_shards || m->shard_parts == nullptr) ? m->num_partitions : m->shard_parts[s];
+}
+template<typename MatrixT>
+inline unsigned long last_partition_in_shard(const sharded<MatrixT> *m, unsigned long s) {
+    return (s >= m->num_shards || m->shard_parts == nullptr) ? m->num_partitions : m->shard_parts[s + 1];
+}
+
+} // namespace cellshard

@@ -34,8 +34,14 @@ extern "C" {
 /* Same handle type as hipStream_t (hip_runtime_api.h: typedef struct ihipStream_t *hipStream_t). */
 typedef struct ihipStream_t *csamd_stream_t;
 
-/* IEEE binary16 storage, bit-identical to __half. */
+/* IEEE binary16 storage, bit-identical to __half.  A C++/HIP includer may
+ * define CSAMD_HALF_T (e.g. to __half) before including this header; the ABI
+ * is unchanged (the member is a pointer). */
+#ifdef CSAMD_HALF_T
+typedef CSAMD_HALF_T csamd_half;
+#else
 typedef uint16_t csamd_half;
+#endif
 
 /* Value type codes: include/CellShard/core/real.cuh:44-51. */
 enum {
@@ -237,6 +243,8 @@ const char *csamd_last_error(void);
 int csamd_abi_version(void);
 /* Number of visible HIP devices (0 on a host without a GPU; never fails). */
 int csamd_device_count(void);
+/* hipSetDevice with the 1/0 convention. */
+int csamd_set_device(int device);
 
 /* ---------------------------------------------------------------------------
  * mask_groups lifecycle and reductions (reference ABI).
@@ -534,6 +542,142 @@ int csamd_blocked_ell_bucket_plan_raw(const unsigned int *d_block_col_idx, unsig
                                       unsigned int width_blocks, unsigned int *d_order_out,
                                       unsigned int *d_width_sorted, unsigned long long *d_keys_sorted,
                                       csamd_convert_workspace *ws);
+
+/* ---------------------------------------------------------------------------
+ * Host matrix structs (reference layouts) and the host-struct converters.
+ * formats/compressed.cuh:24-37, formats/blocked_ell.cuh:15-26,
+ * formats/sliced_ell.cuh:16-28.  Output structs are (re)allocated with one
+ * packed malloc block (`storage`), exactly as sparse::allocate does, so a
+ * caller's sparse::clear()/free() releases them.  flags bit 0 marks memory
+ * pinned with hipHostRegister (released with hipHostUnregister).
+ * ------------------------------------------------------------------------- */
+typedef struct CSAMD_ALIGN16 csamd_host_compressed {
+    unsigned int rows;
+    unsigned int cols;
+    unsigned int nnz;
+    unsigned int axis;
+    unsigned int flags;
+    void *storage;
+    unsigned int *majorPtr;
+    unsigned int *minorIdx;
+    csamd_half *val;
+} csamd_host_compressed;
+
+typedef struct CSAMD_ALIGN16 csamd_host_blocked_ell {
+    unsigned int rows;
+    unsigned int cols;
+    unsigned int nnz;
+    unsigned int block_size;
+    unsigned int ell_cols;
+    unsigned int flags;
+    void *storage;
+    unsigned int *blockColIdx;
+    csamd_half *val;
+} csamd_host_blocked_ell;
+
+typedef struct CSAMD_ALIGN16 csamd_host_sliced_ell {
+    unsigned int rows;
+    unsigned int cols;
+    unsigned int nnz;
+    unsigned int slice_count;
+    unsigned int flags;
+    void *storage;
+    unsigned int *slice_row_offsets;
+    unsigned int *slice_widths;
+    unsigned int *col_idx;
+    csamd_half *val;
+} csamd_host_sliced_ell;
+
+/* Release host storage (formats/*.cuh clear()); resets metadata. */
+void csamd_host_compressed_clear(csamd_host_compressed *m);
+void csamd_host_blocked_ell_clear(csamd_host_blocked_ell *m);
+void csamd_host_sliced_ell_clear(csamd_host_sliced_ell *m);
+
+/* convert/blocked_ell_from_compressed.cuh:157-190 */
+int csamd_host_choose_blocked_ell_block_size(const csamd_host_compressed *src, const unsigned int *candidates,
+                                             unsigned int candidate_count, csamd_blocked_ell_tune_result *out);
+/* convert/blocked_ell_from_compressed.cuh:258-347 */
+int csamd_host_blocked_ell_from_compressed(const csamd_host_compressed *src, unsigned int block_size,
+                                           csamd_host_blocked_ell *dst);
+/* convert/blocked_ell_from_compressed.cuh:449-461 */
+int csamd_host_blocked_ell_from_compressed_auto(const csamd_host_compressed *src, const unsigned int *candidates,
+                                                unsigned int candidate_count, csamd_host_blocked_ell *dst,
+                                                csamd_blocked_ell_tune_result *picked);
+/* convert/sliced_ell_from_compressed.cuh:86-94 */
+int csamd_host_choose_sliced_ell_slice_rows(const csamd_host_compressed *src, const unsigned int *candidates,
+                                            unsigned int candidate_count, csamd_sliced_ell_tune_result *out);
+/* convert/sliced_ell_from_compressed.cuh:96-164 and :166-254 (CSR or CSC source) */
+int csamd_host_sliced_ell_from_compressed(const csamd_host_compressed *src, unsigned int slice_rows,
+                                          csamd_host_sliced_ell *dst);
+/* convert/sliced_ell_from_compressed.cuh:256-284 */
+int csamd_host_sliced_ell_from_compressed_auto(const csamd_host_compressed *src, const unsigned int *candidates,
+                                               unsigned int candidate_count, csamd_host_sliced_ell *dst,
+                                               csamd_sliced_ell_tune_result *picked);
+/* convert/sliced_ell_from_blocked_ell.cuh:233-364 */
+int csamd_host_sliced_ell_from_blocked_ell(const csamd_host_blocked_ell *src, unsigned int slice_rows,
+                                           csamd_host_sliced_ell *dst);
+
+/* ---------------------------------------------------------------------------
+ * Device staging.  runtime/device/sharded_device.cuh:22-50 (records, cache),
+ * :199-243 (allocation cache), :343-755 (upload / upload_async), :958-990
+ * (release), :1336-1432 (packed shard upload).
+ * One packed device allocation per partition (or per shard): the device copy
+ * of the view sits at offset 0, the arrays follow 256-byte aligned.
+ * ------------------------------------------------------------------------- */
+typedef struct CSAMD_ALIGN16 csamd_partition_record {
+    void *allocation;
+    size_t allocation_bytes;
+    void *storage;
+    void *view;          /* device copy of the view */
+    void *a0;
+    void *a1;
+    void *a2;
+    void *a3;
+    unsigned long group_begin;
+    unsigned long group_end;
+    int device_id;
+} csamd_partition_record;
+
+/* Per-process device allocation cache, 16 entries per device (best fit). */
+typedef struct CSAMD_ALIGN16 csamd_cached_allocation {
+    void *ptr;
+    size_t bytes;
+    int device_id;
+} csamd_cached_allocation;
+
+typedef struct CSAMD_ALIGN16 csamd_allocation_cache {
+    unsigned long cache_count;
+    unsigned long cache_capacity;
+    csamd_cached_allocation *cache;
+} csamd_allocation_cache;
+
+void csamd_allocation_cache_init(csamd_allocation_cache *c);
+void csamd_allocation_cache_clear(csamd_allocation_cache *c);
+/* sharded_device.cuh:199-220: smallest cached block >= min_bytes on device, or NULL */
+void *csamd_take_cached_allocation(csamd_allocation_cache *c, int device_id, size_t min_bytes, size_t *out_bytes);
+/* sharded_device.cuh:222-243: keep a block (frees it when the cache is full) */
+int csamd_cache_allocation(csamd_allocation_cache *c, int device_id, void *ptr, size_t bytes);
+
+/* Upload one host partition (stream NULL = synchronous).  cache may be NULL.
+ * out_view receives the host copy of the device view. */
+int csamd_upload_compressed(const csamd_host_compressed *src, csamd_partition_record *record,
+                            csamd_compressed_view *out_view, csamd_allocation_cache *cache, csamd_stream_t stream);
+int csamd_upload_blocked_ell(const csamd_host_blocked_ell *src, csamd_partition_record *record,
+                             csamd_blocked_ell_view *out_view, csamd_allocation_cache *cache, csamd_stream_t stream);
+/* Computes slice_slot_offsets and slice_rows (uniform height or 0) like
+ * sharded_device.cuh:609-620. */
+int csamd_upload_sliced_ell(const csamd_host_sliced_ell *src, csamd_partition_record *record,
+                            csamd_sliced_ell_view *out_view, csamd_allocation_cache *cache, csamd_stream_t stream);
+/* sharded_device.cuh:958-990: return the allocation to the cache (or free). */
+int csamd_release_record(csamd_partition_record *record, csamd_allocation_cache *cache);
+/* Packed shard upload: `count` Blocked-ELL partitions into ONE allocation,
+ * out_views[count] host copies (sharded_device.cuh:1336-1432). */
+int csamd_upload_blocked_ell_shard(const csamd_host_blocked_ell *const *parts, unsigned int count,
+                                   csamd_partition_record *record, csamd_blocked_ell_view *out_views,
+                                   csamd_allocation_cache *cache, csamd_stream_t stream);
+int csamd_upload_compressed_shard(const csamd_host_compressed *const *parts, unsigned int count,
+                                  csamd_partition_record *record, csamd_compressed_view *out_views,
+                                  csamd_allocation_cache *cache, csamd_stream_t stream);
 
 /* ---------------------------------------------------------------------------
  * Synthetic CSR generator for measurement and tests (not a reference path).
