@@ -43,6 +43,8 @@ def parse_args():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-ops", action="store_true")
     ap.add_argument("--reduce-only", action="store_true", help="only the reduce steps (for profiling)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run a plain 2 GiB device copy (reference point for HBM byte counters)")
     return ap.parse_args()
 
 
@@ -82,6 +84,17 @@ def main() -> int:
     R, Cc, G, FK = args.rows, args.cols, args.groups, args.features
     props = torch.cuda.get_device_properties(0)
     log(f"device {props.name} CUs={props.multi_processor_count}")
+
+    if args.calibrate_copy:
+        # plain copy of known size: 2 GiB read + 2 GiB written per launch
+        a_ = torch.empty(1 << 31, dtype=torch.uint8, device=dev).fill_(1)
+        b_ = torch.empty_like(a_)
+        for _ in range(2):
+            b_.copy_(a_)
+        torch.cuda.synchronize()
+        del a_, b_
+        torch.cuda.empty_cache()
+        log("copy calibration: 2 x (2 GiB read + 2 GiB write)")
 
     # ---------------- synthetic inputs (config 2 generator) ----------------
     t0 = time.time()

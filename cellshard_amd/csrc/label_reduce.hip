@@ -166,14 +166,17 @@ struct sell_geom {
     }
 };
 
+// Per sorted kept row: its entry range (both layouts index entries with u32)
+// and its work weight for the partition.
 template<typename Geom>
 __global__ void row_work_kernel(Geom geom, const unsigned int *__restrict__ sorted_rows, unsigned int n,
-                                unsigned int *__restrict__ work) {
+                                unsigned int *__restrict__ work, uint2 *__restrict__ ranges) {
     for (unsigned int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         unsigned long long b, e;
         geom.range(sorted_rows[i], &b, &e);
         const unsigned long long len = e - b;
-        work[i] = (unsigned int) (len > 0xffffffffull ? 0xffffffffull : len) + 1u;  // +1: per-row overhead
+        work[i] = (unsigned int) (len > 0xfffffff0ull ? 0xfffffff0ull : len) + 8u;  // +8: per-row overhead
+        if (ranges != nullptr) ranges[i] = make_uint2((unsigned int) b, (unsigned int) e);
     }
 }
 
@@ -246,104 +249,153 @@ __device__ __forceinline__ void consume(unsigned int c, typename value_traits<VT
     }
 }
 
-// Vector unit: 4 consecutive entries.
+// Vector unit: 4 consecutive entries, loaded with non-temporal vector loads.
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+
 template<int VT> struct unit_loader;
 template<> struct unit_loader<CSAMD_VALUE_F16> {
     using S = uint16_t;
     static __device__ __forceinline__ void load(const S *v, unsigned long long e, S out[4]) {
-        const uint2 w = *(const uint2 *) (v + e);
+        const u32x2 w = __builtin_nontemporal_load((const u32x2 *) (v + e));
         out[0] = (S) (w.x & 0xffffu); out[1] = (S) (w.x >> 16); out[2] = (S) (w.y & 0xffffu); out[3] = (S) (w.y >> 16);
     }
-    static constexpr unsigned int align = 8u;
 };
 template<> struct unit_loader<CSAMD_VALUE_BF16> {
     using S = uint16_t;
     static __device__ __forceinline__ void load(const S *v, unsigned long long e, S out[4]) {
-        const uint2 w = *(const uint2 *) (v + e);
+        const u32x2 w = __builtin_nontemporal_load((const u32x2 *) (v + e));
         out[0] = (S) (w.x & 0xffffu); out[1] = (S) (w.x >> 16); out[2] = (S) (w.y & 0xffffu); out[3] = (S) (w.y >> 16);
     }
-    static constexpr unsigned int align = 8u;
 };
 template<> struct unit_loader<CSAMD_VALUE_F32> {
     using S = float;
     static __device__ __forceinline__ void load(const S *v, unsigned long long e, S out[4]) {
-        const float4 w = *(const float4 *) (v + e);
-        out[0] = w.x; out[1] = w.y; out[2] = w.z; out[3] = w.w;
+        const u32x4 w = __builtin_nontemporal_load((const u32x4 *) (v + e));
+        out[0] = __uint_as_float(w.x); out[1] = __uint_as_float(w.y); out[2] = __uint_as_float(w.z); out[3] = __uint_as_float(w.w);
     }
-    static constexpr unsigned int align = 16u;
 };
 template<> struct unit_loader<CSAMD_VALUE_FP8_E4M3> {
     using S = uint8_t;
     static __device__ __forceinline__ void load(const S *v, unsigned long long e, S out[4]) {
-        const unsigned int w = *(const unsigned int *) (v + e);
+        const unsigned int w = __builtin_nontemporal_load((const unsigned int *) (v + e));
         out[0] = (S) (w & 0xffu); out[1] = (S) ((w >> 8) & 0xffu); out[2] = (S) ((w >> 16) & 0xffu); out[3] = (S) (w >> 24);
     }
-    static constexpr unsigned int align = 4u;
 };
 
-// Stream the entries [b, e) of one row (one wave).  `limit` = number of
-// entries in the arrays (vector units never read past it).
+constexpr unsigned int LR_UNITS = 2u;               // 4-entry units per lane per chunk
+constexpr unsigned int LR_CHUNK_UNITS = 64u * LR_UNITS;
+
+template<int VT> struct chunk_regs {
+    unsigned int c[4 * LR_UNITS];
+    typename value_traits<VT>::storage v[4 * LR_UNITS];
+};
+
+// Load one chunk (LR_UNITS units per lane) of the entry range [b, e); entries
+// outside [b, e) get column CSAMD_INVALID_COL.  Units below full_end are
+// entirely inside the arrays and use vector loads.
 template<int VT, bool VEC>
-__device__ __forceinline__ void stream_row(const unsigned int *__restrict__ idx, const typename value_traits<VT>::storage *__restrict__ val,
-                                           unsigned long long b, unsigned long long e, unsigned long long limit,
-                                           const uint2 *__restrict__ lpairs, const reduce_args &a, const acc_lds &acc) {
-    using S = typename value_traits<VT>::storage;
+__device__ __forceinline__ void load_chunk(const unsigned int *__restrict__ idx, const typename value_traits<VT>::storage *__restrict__ val,
+                                           unsigned long long u, unsigned long long u_end, unsigned long long full_end, unsigned int b,
+                                           unsigned int e, chunk_regs<VT> &r) {
     const unsigned int lane = lane_id();
-    if (!VEC) {
-        for (unsigned long long i = b + lane; i < e; i += 64ull) consume<VT>(idx[i], val[i], lpairs, a, acc);
-        return;
-    }
-    if (b >= e) return;
-    const unsigned long long u0 = b >> 2;
-    const unsigned long long u_end = (e + 3ull) >> 2;       // units touching [b, e)
-    const unsigned long long full_end = limit >> 2;          // units fully inside the arrays
-    for (unsigned long long u = u0 + lane; u < u_end; u += 128ull) {
-        const unsigned long long ua = u, ub = u + 64ull;
-        uint4 ca = make_uint4(CSAMD_INVALID_COL, CSAMD_INVALID_COL, CSAMD_INVALID_COL, CSAMD_INVALID_COL);
-        uint4 cb = ca;
-        S va[4] = {0, 0, 0, 0}, vb[4] = {0, 0, 0, 0};
-        if (ua < full_end) {
-            ca = *(const uint4 *) (idx + ua * 4ull);
-            unit_loader<VT>::load(val, ua * 4ull, va);
-        } else {
-            unsigned int *cp = (unsigned int *) &ca;
-            for (unsigned int k = 0; k < 4u; ++k) {
-                const unsigned long long i = ua * 4ull + k;
-                if (i < e) { cp[k] = idx[i]; va[k] = val[i]; }
-            }
-        }
-        const bool has_b = ub < u_end;
-        if (has_b) {
-            if (ub < full_end) {
-                cb = *(const uint4 *) (idx + ub * 4ull);
-                unit_loader<VT>::load(val, ub * 4ull, vb);
-            } else {
-                unsigned int *cp = (unsigned int *) &cb;
-                for (unsigned int k = 0; k < 4u; ++k) {
-                    const unsigned long long i = ub * 4ull + k;
-                    if (i < e) { cp[k] = idx[i]; vb[k] = val[i]; }
-                }
-            }
-        }
-        const unsigned int *pa = (const unsigned int *) &ca;
+#pragma unroll
+    for (unsigned int j = 0; j < LR_UNITS; ++j) {
+        const unsigned long long uu = u + lane + 64ull * j;
+        unsigned int *cp = r.c + 4 * j;
+        auto *vp = r.v + 4 * j;
+#pragma unroll
         for (unsigned int k = 0; k < 4u; ++k) {
-            const unsigned long long i = ua * 4ull + k;
-            if (i >= b && i < e) consume<VT>(pa[k], va[k], lpairs, a, acc);
+            cp[k] = CSAMD_INVALID_COL;
+            vp[k] = 0;
         }
-        if (has_b) {
-            const unsigned int *pb = (const unsigned int *) &cb;
+        if (uu >= u_end) continue;
+        const unsigned long long i0 = uu * 4ull;
+        if (VEC && uu < full_end) {
+            const u32x4 w = __builtin_nontemporal_load((const u32x4 *) (idx + i0));
+            cp[0] = w.x; cp[1] = w.y; cp[2] = w.z; cp[3] = w.w;
+            unit_loader<VT>::load(val, i0, vp);
+#pragma unroll
+            for (unsigned int k = 0; k < 4u; ++k)
+                if (i0 + k < b || i0 + k >= e) cp[k] = CSAMD_INVALID_COL;
+        } else {
+#pragma unroll
             for (unsigned int k = 0; k < 4u; ++k) {
-                const unsigned long long i = ub * 4ull + k;
-                if (i >= b && i < e) consume<VT>(pb[k], vb[k], lpairs, a, acc);
+                const unsigned long long i = i0 + k;
+                if (i >= b && i < e) {
+                    cp[k] = idx[i];
+                    vp[k] = val[i];
+                }
             }
         }
     }
 }
 
-template<typename Geom, int VT, bool VEC>
-__global__ void label_reduce_rows_kernel(Geom geom, const unsigned int *__restrict__ idx,
-                                         const typename value_traits<VT>::storage *__restrict__ val,
-                                         unsigned long long limit, reduce_args a) {
+__device__ __forceinline__ void row_units(uint2 rng, unsigned long long *u, unsigned long long *ue) {
+    *u = rng.x >> 2;
+    *ue = rng.x == rng.y ? *u : ((unsigned long long) rng.y + 3ull) >> 2;
+}
+
+// Rows of one label segment, one wave per row in turn (i = s0 + wave,
+// stride nwaves).  The next chunk (possibly of the next row) is loaded
+// before the current one is consumed, and each row's range is fetched one
+// row ahead, so two chunks and one range are in flight per wave.
+template<int VT, bool VEC>
+__device__ __forceinline__ void stream_rows(const unsigned int *__restrict__ idx, const typename value_traits<VT>::storage *__restrict__ val,
+                                            const uint2 *__restrict__ ranges, unsigned int s0, unsigned int s1, unsigned int wave,
+                                            unsigned int nwaves, unsigned long long full_end, const uint2 *__restrict__ lpairs,
+                                            const reduce_args &a, const acc_lds &acc) {
+    unsigned int i = s0 + wave;
+    if (i >= s1) return;
+    const uint2 zero = make_uint2(0u, 0u);
+    uint2 rng = ranges[i];
+    unsigned int inext = i + nwaves;
+    uint2 nxt = inext < s1 ? ranges[inext] : zero;
+    unsigned long long u, ue;
+    row_units(rng, &u, &ue);
+    while (u >= ue) {  // skip empty rows
+        i = inext;
+        if (i >= s1) return;
+        rng = nxt;
+        inext += nwaves;
+        nxt = inext < s1 ? ranges[inext] : zero;
+        row_units(rng, &u, &ue);
+    }
+    chunk_regs<VT> A, B;
+    load_chunk<VT, VEC>(idx, val, u, ue, full_end, rng.x, rng.y, A);
+    while (true) {
+        unsigned int i2 = i;
+        uint2 r2 = rng;
+        unsigned long long u2 = u + LR_CHUNK_UNITS, ue2 = ue;
+        bool more = true;
+        if (u2 >= ue2) {
+            do {
+                i2 = inext;
+                if (i2 >= s1) {
+                    more = false;
+                    break;
+                }
+                r2 = nxt;
+                inext += nwaves;
+                nxt = inext < s1 ? ranges[inext] : zero;
+                row_units(r2, &u2, &ue2);
+            } while (u2 >= ue2);
+        }
+        if (more) load_chunk<VT, VEC>(idx, val, u2, ue2, full_end, r2.x, r2.y, B);
+#pragma unroll
+        for (unsigned int k = 0; k < 4u * LR_UNITS; ++k) consume<VT>(A.c[k], A.v[k], lpairs, a, acc);
+        if (!more) break;
+        A = B;
+        i = i2;
+        rng = r2;
+        u = u2;
+        ue = ue2;
+    }
+}
+
+template<int VT, bool VEC>
+__global__ void label_reduce_rows_kernel(const unsigned int *__restrict__ idx, const typename value_traits<VT>::storage *__restrict__ val,
+                                         unsigned long long limit, const uint2 *__restrict__ ranges, reduce_args a) {
     extern __shared__ __align__(16) unsigned char lds[];
     const unsigned int nthreads = blockDim.x;
     const unsigned int nwaves = nthreads >> 6;
@@ -353,14 +405,13 @@ __global__ void label_reduce_rows_kernel(Geom geom, const unsigned int *__restri
     double *lsumsq = lsum + a.ftile;
     unsigned int *lcnt = (unsigned int *) (lsumsq + a.ftile);
     const acc_lds acc{lsum, lsumsq, lcnt};
+    const unsigned long long full_end = limit >> 2;
 
     for (unsigned int w = threadIdx.x; w < a.nwords; w += nthreads) lpairs[w] = a.fpairs[w];
 
-    const unsigned int W = gridDim.x;
     const unsigned int lo = a.bounds[blockIdx.x];
     const unsigned int hi = a.bounds[blockIdx.x + 1u];
     if (lo >= hi) return;  // uniform across the block
-    // first label touching lo
     unsigned int g;
     {
         unsigned int l = 0u, h = a.G;
@@ -383,12 +434,7 @@ __global__ void label_reduce_rows_kernel(Geom geom, const unsigned int *__restri
             lcnt[f] = 0u;
         }
         __syncthreads();
-        for (unsigned int i = s0 + wave; i < s1; i += nwaves) {
-            const unsigned int row = a.sorted_rows[i];
-            unsigned long long b, e;
-            geom.range(row, &b, &e);
-            stream_row<VT, VEC>(idx, val, b, e, limit, lpairs, a, acc);
-        }
+        stream_rows<VT, VEC>(idx, val, ranges, s0, s1, wave, nwaves, full_end, lpairs, a, acc);
         __syncthreads();
         const unsigned long long slot = (unsigned long long) (blockIdx.x + g) * a.ftile;
         for (unsigned int f = threadIdx.x; f < a.f_count; f += nthreads) {
@@ -398,7 +444,6 @@ __global__ void label_reduce_rows_kernel(Geom geom, const unsigned int *__restri
         }
         __syncthreads();
     }
-    (void) W;
 }
 
 // Blocked-ELL rows: entries decoded through blockColIdx (scalar lane stride).
@@ -559,28 +604,18 @@ struct layout_in {
 
 template<int VT>
 int launch_rows(const layout_in &L, unsigned int grid, unsigned int threads, size_t lds, hipStream_t s, const reduce_args &a,
-                bool vec) {
+                bool vec, const uint2 *ranges) {
     using S = typename value_traits<VT>::storage;
     const S *val = (const S *) L.val;
-    if (L.kind == 0) {
+    if (L.kind != 2) {
         if (vec) {
-            auto k = label_reduce_rows_kernel<csr_geom, VT, true>;
+            auto k = label_reduce_rows_kernel<VT, true>;
             CSAMD_HIP(hipFuncSetAttribute((const void *) k, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds), "set lds");
-            k<<<grid, threads, lds, s>>>(L.csr, L.idx, val, L.limit, a);
+            k<<<grid, threads, lds, s>>>(L.idx, val, L.limit, ranges, a);
         } else {
-            auto k = label_reduce_rows_kernel<csr_geom, VT, false>;
+            auto k = label_reduce_rows_kernel<VT, false>;
             CSAMD_HIP(hipFuncSetAttribute((const void *) k, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds), "set lds");
-            k<<<grid, threads, lds, s>>>(L.csr, L.idx, val, L.limit, a);
-        }
-    } else if (L.kind == 1) {
-        if (vec) {
-            auto k = label_reduce_rows_kernel<sell_geom, VT, true>;
-            CSAMD_HIP(hipFuncSetAttribute((const void *) k, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds), "set lds");
-            k<<<grid, threads, lds, s>>>(L.sell, L.idx, val, L.limit, a);
-        } else {
-            auto k = label_reduce_rows_kernel<sell_geom, VT, false>;
-            CSAMD_HIP(hipFuncSetAttribute((const void *) k, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds), "set lds");
-            k<<<grid, threads, lds, s>>>(L.sell, L.idx, val, L.limit, a);
+            k<<<grid, threads, lds, s>>>(L.idx, val, L.limit, ranges, a);
         }
     } else {
         auto k = label_reduce_blocked_kernel<VT>;
@@ -687,18 +722,19 @@ int run_label_reduce(const layout_in &L, unsigned int value_type, const csamd_la
 
     // 4. work bounds
     const size_t work_scan_bytes = exclusive_scan_scratch_bytes(K);
-    if (!ensure_buf(P->work, (size_t) (K + 1u) * sizeof(unsigned int) + (size_t) (K + 1u) * sizeof(unsigned long long) + 512u +
-                                 work_scan_bytes,
-                    "hipMalloc work"))
-        return 0;
+    const size_t work_bytes = align_up((size_t) (K + 1u) * sizeof(unsigned int), 256u);
+    const size_t prefix_bytes = align_up((size_t) (K + 1u) * sizeof(unsigned long long), 256u);
+    const size_t ranges_bytes = align_up((size_t) (K + 1u) * sizeof(uint2), 256u);
+    if (!ensure_buf(P->work, work_bytes + prefix_bytes + ranges_bytes + work_scan_bytes, "hipMalloc work")) return 0;
     unsigned int *work = (unsigned int *) P->work.p;
-    unsigned long long *wprefix64 = (unsigned long long *) ((char *) P->work.p + align_up((size_t) (K + 1u) * sizeof(unsigned int), 256u));
-    void *wscan = (char *) wprefix64 + align_up((size_t) (K + 1u) * sizeof(unsigned long long), 256u);
+    unsigned long long *wprefix64 = (unsigned long long *) ((char *) P->work.p + work_bytes);
+    uint2 *ranges = (uint2 *) ((char *) P->work.p + work_bytes + prefix_bytes);
+    void *wscan = (char *) P->work.p + work_bytes + prefix_bytes + ranges_bytes;
     if (!ensure_buf(P->bounds, (size_t) (W + 1u) * sizeof(unsigned int), "hipMalloc bounds")) return 0;
     unsigned int *bounds = (unsigned int *) P->bounds.p;
     if (K != 0u) {
-        if (L.kind == 0) row_work_kernel<csr_geom><<<grid_for(K, 256u, 8192u), 256, 0, s>>>(L.csr, ids_out, K, work);
-        else if (L.kind == 1) row_work_kernel<sell_geom><<<grid_for(K, 256u, 8192u), 256, 0, s>>>(L.sell, ids_out, K, work);
+        if (L.kind == 0) row_work_kernel<csr_geom><<<grid_for(K, 256u, 8192u), 256, 0, s>>>(L.csr, ids_out, K, work, ranges);
+        else if (L.kind == 1) row_work_kernel<sell_geom><<<grid_for(K, 256u, 8192u), 256, 0, s>>>(L.sell, ids_out, K, work, ranges);
         else blocked_work_kernel<<<grid_for(K, 256u, 8192u), 256, 0, s>>>(L.bell.ell_cols, K, work);
         CSAMD_HIP(hipGetLastError(), "row_work_kernel");
     }
@@ -755,10 +791,10 @@ int run_label_reduce(const layout_in &L, unsigned int value_type, const csamd_la
             if (first) CSAMD_HIP(hipEventRecord(P->ev0, s), "hipEventRecord");
             int ok = 0;
             switch (value_type) {
-                case CSAMD_VALUE_F16: ok = launch_rows<CSAMD_VALUE_F16>(L, W, threads, lds, s, a, aligned); break;
-                case CSAMD_VALUE_BF16: ok = launch_rows<CSAMD_VALUE_BF16>(L, W, threads, lds, s, a, aligned); break;
-                case CSAMD_VALUE_F32: ok = launch_rows<CSAMD_VALUE_F32>(L, W, threads, lds, s, a, aligned); break;
-                case CSAMD_VALUE_FP8_E4M3: ok = launch_rows<CSAMD_VALUE_FP8_E4M3>(L, W, threads, lds, s, a, aligned); break;
+                case CSAMD_VALUE_F16: ok = launch_rows<CSAMD_VALUE_F16>(L, W, threads, lds, s, a, aligned, ranges); break;
+                case CSAMD_VALUE_BF16: ok = launch_rows<CSAMD_VALUE_BF16>(L, W, threads, lds, s, a, aligned, ranges); break;
+                case CSAMD_VALUE_F32: ok = launch_rows<CSAMD_VALUE_F32>(L, W, threads, lds, s, a, aligned, ranges); break;
+                case CSAMD_VALUE_FP8_E4M3: ok = launch_rows<CSAMD_VALUE_FP8_E4M3>(L, W, threads, lds, s, a, aligned, ranges); break;
             }
             if (!ok) return 0;
             if (first) {

@@ -101,7 +101,7 @@ def test_sliced_ell_tuner(ws):
 
 
 def test_sliced_from_blocked_parity(ws):
-    rows, cols, bs = 333, 400
+    rows, cols = 333, 400
     ptr, idx, val = random_csr(rows, cols, 0.05, seed=4, zeros=0.1)
     bits = f16_bits(val)
     src = DeviceCSR.from_numpy(rows, cols, ptr, idx, bits)
