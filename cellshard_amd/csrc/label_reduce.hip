@@ -296,12 +296,12 @@ template<int VT> struct chunk_regs {
 // entirely inside the arrays and use vector loads.
 template<int VT, bool VEC>
 __device__ __forceinline__ void load_chunk(const unsigned int *__restrict__ idx, const typename value_traits<VT>::storage *__restrict__ val,
-                                           unsigned long long u, unsigned long long u_end, unsigned long long full_end, unsigned int b,
+                                           unsigned int u, unsigned int u_end, unsigned int full_end, unsigned int b,
                                            unsigned int e, chunk_regs<VT> &r) {
     const unsigned int lane = lane_id();
 #pragma unroll
     for (unsigned int j = 0; j < LR_UNITS; ++j) {
-        const unsigned long long uu = u + lane + 64ull * j;
+        const unsigned int uu = u + lane + 64u * j;
         unsigned int *cp = r.c + 4 * j;
         auto *vp = r.v + 4 * j;
 #pragma unroll
@@ -310,7 +310,7 @@ __device__ __forceinline__ void load_chunk(const unsigned int *__restrict__ idx,
             vp[k] = 0;
         }
         if (uu >= u_end) continue;
-        const unsigned long long i0 = uu * 4ull;
+        const unsigned int i0 = uu * 4u;
         if (VEC && uu < full_end) {
             const u32x4 w = __builtin_nontemporal_load((const u32x4 *) (idx + i0));
             cp[0] = w.x; cp[1] = w.y; cp[2] = w.z; cp[3] = w.w;
@@ -321,7 +321,7 @@ __device__ __forceinline__ void load_chunk(const unsigned int *__restrict__ idx,
         } else {
 #pragma unroll
             for (unsigned int k = 0; k < 4u; ++k) {
-                const unsigned long long i = i0 + k;
+                const unsigned int i = i0 + k;
                 if (i >= b && i < e) {
                     cp[k] = idx[i];
                     vp[k] = val[i];
@@ -331,9 +331,39 @@ __device__ __forceinline__ void load_chunk(const unsigned int *__restrict__ idx,
     }
 }
 
-__device__ __forceinline__ void row_units(uint2 rng, unsigned long long *u, unsigned long long *ue) {
+// Consume a batch: all rank lookups first (independent LDS reads), then the
+// accumulator atomics, so the lookups are not ordered behind the atomics.
+// lpairs[nwords] is a {0, 0} sentinel, so out-of-range columns (including
+// padding) resolve to "not kept" without a branch.
+template<int VT, unsigned int N>
+__device__ __forceinline__ void consume_batch(const unsigned int *c, const typename value_traits<VT>::storage *raw,
+                                              const uint2 *__restrict__ lpairs, const reduce_args &a, const acc_lds &acc) {
+    uint2 pm[N];
+#pragma unroll
+    for (unsigned int k = 0; k < N; ++k) {
+        const unsigned int w = c[k] >> 5;
+        pm[k] = lpairs[w < a.nwords ? w : a.nwords];
+    }
+#pragma unroll
+    for (unsigned int k = 0; k < N; ++k) {
+        const unsigned int bit = c[k] & 31u;
+        if (((pm[k].x >> bit) & 1u) == 0u) continue;
+        const unsigned int f = pm[k].y + (unsigned int) __popc(pm[k].x & ((1u << bit) - 1u)) - a.f0;
+        if (f >= a.f_count) continue;
+        float v = value_traits<VT>::decode(raw[k]);
+        if (a.scale != nullptr) v *= a.scale[c[k]];
+        if (v != 0.0f) {
+            const double d = (double) v;
+            atomicAdd(acc.sum + f, d);
+            atomicAdd(acc.sumsq + f, d * d);
+            atomicAdd(acc.cnt + f, 1u);
+        }
+    }
+}
+
+__device__ __forceinline__ void row_units(uint2 rng, unsigned int *u, unsigned int *ue) {
     *u = rng.x >> 2;
-    *ue = rng.x == rng.y ? *u : ((unsigned long long) rng.y + 3ull) >> 2;
+    *ue = rng.x == rng.y ? *u : (unsigned int) (((unsigned long long) rng.y + 3ull) >> 2);
 }
 
 // Rows of one label segment, one wave per row in turn (i = s0 + wave,
@@ -343,7 +373,7 @@ __device__ __forceinline__ void row_units(uint2 rng, unsigned long long *u, unsi
 template<int VT, bool VEC>
 __device__ __forceinline__ void stream_rows(const unsigned int *__restrict__ idx, const typename value_traits<VT>::storage *__restrict__ val,
                                             const uint2 *__restrict__ ranges, unsigned int s0, unsigned int s1, unsigned int wave,
-                                            unsigned int nwaves, unsigned long long full_end, const uint2 *__restrict__ lpairs,
+                                            unsigned int nwaves, unsigned int full_end, const uint2 *__restrict__ lpairs,
                                             const reduce_args &a, const acc_lds &acc) {
     unsigned int i = s0 + wave;
     if (i >= s1) return;
@@ -351,7 +381,7 @@ __device__ __forceinline__ void stream_rows(const unsigned int *__restrict__ idx
     uint2 rng = ranges[i];
     unsigned int inext = i + nwaves;
     uint2 nxt = inext < s1 ? ranges[inext] : zero;
-    unsigned long long u, ue;
+    unsigned int u, ue;
     row_units(rng, &u, &ue);
     while (u >= ue) {  // skip empty rows
         i = inext;
@@ -366,7 +396,7 @@ __device__ __forceinline__ void stream_rows(const unsigned int *__restrict__ idx
     while (true) {
         unsigned int i2 = i;
         uint2 r2 = rng;
-        unsigned long long u2 = u + LR_CHUNK_UNITS, ue2 = ue;
+        unsigned int u2 = u + LR_CHUNK_UNITS, ue2 = ue;
         bool more = true;
         if (u2 >= ue2) {
             do {
@@ -382,8 +412,7 @@ __device__ __forceinline__ void stream_rows(const unsigned int *__restrict__ idx
             } while (u2 >= ue2);
         }
         if (more) load_chunk<VT, VEC>(idx, val, u2, ue2, full_end, r2.x, r2.y, B);
-#pragma unroll
-        for (unsigned int k = 0; k < 4u * LR_UNITS; ++k) consume<VT>(A.c[k], A.v[k], lpairs, a, acc);
+        consume_batch<VT, 4u * LR_UNITS>(A.c, A.v, lpairs, a, acc);
         if (!more) break;
         A = B;
         i = i2;
@@ -401,13 +430,13 @@ __global__ void label_reduce_rows_kernel(const unsigned int *__restrict__ idx, c
     const unsigned int nwaves = nthreads >> 6;
     const unsigned int wave = threadIdx.x >> 6;
     uint2 *lpairs = (uint2 *) lds;
-    double *lsum = (double *) (lds + align_up((size_t) a.nwords * sizeof(uint2), 16u));
+    double *lsum = (double *) (lds + align_up((size_t) (a.nwords + 1u) * sizeof(uint2), 16u));
     double *lsumsq = lsum + a.ftile;
     unsigned int *lcnt = (unsigned int *) (lsumsq + a.ftile);
     const acc_lds acc{lsum, lsumsq, lcnt};
-    const unsigned long long full_end = limit >> 2;
+    const unsigned int full_end = (unsigned int) (limit >> 2);
 
-    for (unsigned int w = threadIdx.x; w < a.nwords; w += nthreads) lpairs[w] = a.fpairs[w];
+    for (unsigned int w = threadIdx.x; w <= a.nwords; w += nthreads) lpairs[w] = w < a.nwords ? a.fpairs[w] : make_uint2(0u, 0u);
 
     const unsigned int lo = a.bounds[blockIdx.x];
     const unsigned int hi = a.bounds[blockIdx.x + 1u];
@@ -455,7 +484,7 @@ __global__ void label_reduce_blocked_kernel(csamd_blocked_ell_view src, reduce_a
     const unsigned int nwaves = nthreads >> 6;
     const unsigned int wave = threadIdx.x >> 6;
     uint2 *lpairs = (uint2 *) lds;
-    double *lsum = (double *) (lds + align_up((size_t) a.nwords * sizeof(uint2), 16u));
+    double *lsum = (double *) (lds + align_up((size_t) (a.nwords + 1u) * sizeof(uint2), 16u));
     double *lsumsq = lsum + a.ftile;
     unsigned int *lcnt = (unsigned int *) (lsumsq + a.ftile);
     const acc_lds acc{lsum, lsumsq, lcnt};
@@ -698,7 +727,7 @@ int run_label_reduce(const layout_in &L, unsigned int value_type, const csamd_la
     CSAMD_HIP(hipStreamSynchronize(s), "label reduce sync (sizes)");
 
     // 3. launch shape
-    const size_t fixed_lds = align_up((size_t) nwords * sizeof(uint2), 16u);
+    const size_t fixed_lds = align_up((size_t) (nwords + 1u) * sizeof(uint2), 16u);
     const int max_lds = max_lds_per_block();
     CSAMD_REQUIRE(fixed_lds + 64u * LR_ACC_BYTES <= (size_t) max_lds,
                   "label_reduce: %u columns need %zu bytes of LDS for the feature map", cols, fixed_lds);
@@ -706,7 +735,10 @@ int run_label_reduce(const layout_in &L, unsigned int value_type, const csamd_la
     unsigned int threads, per_cu, ftile;
     const size_t need_all = fixed_lds + (size_t) (F == 0u ? 1u : F) * LR_ACC_BYTES;
     if (need_all <= (size_t) max_lds / 2u) {
-        threads = 512u;
+        // tuning override for sweeps: CSAMD_LR_THREADS in {256, 512, 768, 1024}
+        const char *env = std::getenv("CSAMD_LR_THREADS");
+        const unsigned int t = env != nullptr ? (unsigned int) std::atoi(env) : 512u;
+        threads = (t >= 256u && t <= 1024u && t % 64u == 0u) ? t : 512u;
         per_cu = 2u;
         ftile = F == 0u ? 1u : F;
     } else {
