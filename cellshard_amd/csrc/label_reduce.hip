@@ -218,7 +218,6 @@ struct reduce_args {
     const unsigned int *gstart;   // [G+1]
     const unsigned int *bounds;   // [W+1]
     const uint2 *fpairs;          // [nwords]
-    const float *scale;           // [cols] or null
     double *psum;                 // [(W+G) x ftile]
     double *psumsq;
     unsigned int *pcnt;
@@ -239,8 +238,7 @@ __device__ __forceinline__ void consume(unsigned int c, typename value_traits<VT
     if (((pm.x >> bit) & 1u) == 0u) return;
     const unsigned int f = pm.y + (unsigned int) __popc(pm.x & ((1u << bit) - 1u)) - a.f0;
     if (f >= a.f_count) return;
-    float v = value_traits<VT>::decode(raw);
-    if (a.scale != nullptr) v *= a.scale[c];
+    const float v = value_traits<VT>::decode(raw);
     if (v != 0.0f) {
         const double d = (double) v;
         atomicAdd(acc.sum + f, d);
@@ -249,37 +247,71 @@ __device__ __forceinline__ void consume(unsigned int c, typename value_traits<VT
     }
 }
 
-// Vector unit: 4 consecutive entries, loaded with non-temporal vector loads.
+// A unit is 4 consecutive entries, loaded with one non-temporal vector load
+// of indices and one of values.  Loaded words stay packed in registers until
+// they are consumed: any instruction that reads them at load time would make
+// the wave wait for the load right there and void the prefetch.
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 
-template<int VT> struct unit_loader;
-template<> struct unit_loader<CSAMD_VALUE_F16> {
-    using S = uint16_t;
-    static __device__ __forceinline__ void load(const S *v, unsigned long long e, S out[4]) {
-        const u32x2 w = __builtin_nontemporal_load((const u32x2 *) (v + e));
-        out[0] = (S) (w.x & 0xffffu); out[1] = (S) (w.x >> 16); out[2] = (S) (w.y & 0xffffu); out[3] = (S) (w.y >> 16);
+template<int VT> struct unit_vals;
+template<> struct unit_vals<CSAMD_VALUE_F16> {
+    using raw = u32x2;
+    static __device__ __forceinline__ raw vload(const uint16_t *v, unsigned int i0) {
+        return __builtin_nontemporal_load((const u32x2 *) (v + i0));
+    }
+    static __device__ __forceinline__ raw sload(const uint16_t *v, unsigned int i0, unsigned int last) {
+        raw r;
+        r.x = (unsigned int) v[min(i0, last)] | ((unsigned int) v[min(i0 + 1u, last)] << 16);
+        r.y = (unsigned int) v[min(i0 + 2u, last)] | ((unsigned int) v[min(i0 + 3u, last)] << 16);
+        return r;
+    }
+    static __device__ __forceinline__ float get(const raw &r, unsigned int k) {
+        const unsigned int w = k < 2u ? r.x : r.y;
+        return f16_bits_to_float((uint16_t) ((k & 1u) ? (w >> 16) : (w & 0xffffu)));
     }
 };
-template<> struct unit_loader<CSAMD_VALUE_BF16> {
-    using S = uint16_t;
-    static __device__ __forceinline__ void load(const S *v, unsigned long long e, S out[4]) {
-        const u32x2 w = __builtin_nontemporal_load((const u32x2 *) (v + e));
-        out[0] = (S) (w.x & 0xffffu); out[1] = (S) (w.x >> 16); out[2] = (S) (w.y & 0xffffu); out[3] = (S) (w.y >> 16);
+template<> struct unit_vals<CSAMD_VALUE_BF16> {
+    using raw = u32x2;
+    static __device__ __forceinline__ raw vload(const uint16_t *v, unsigned int i0) {
+        return __builtin_nontemporal_load((const u32x2 *) (v + i0));
+    }
+    static __device__ __forceinline__ raw sload(const uint16_t *v, unsigned int i0, unsigned int last) {
+        return unit_vals<CSAMD_VALUE_F16>::sload(v, i0, last);
+    }
+    static __device__ __forceinline__ float get(const raw &r, unsigned int k) {
+        const unsigned int w = k < 2u ? r.x : r.y;
+        return __uint_as_float((k & 1u) ? (w & 0xffff0000u) : (w << 16));
     }
 };
-template<> struct unit_loader<CSAMD_VALUE_F32> {
-    using S = float;
-    static __device__ __forceinline__ void load(const S *v, unsigned long long e, S out[4]) {
-        const u32x4 w = __builtin_nontemporal_load((const u32x4 *) (v + e));
-        out[0] = __uint_as_float(w.x); out[1] = __uint_as_float(w.y); out[2] = __uint_as_float(w.z); out[3] = __uint_as_float(w.w);
+template<> struct unit_vals<CSAMD_VALUE_F32> {
+    using raw = u32x4;
+    static __device__ __forceinline__ raw vload(const float *v, unsigned int i0) {
+        return __builtin_nontemporal_load((const u32x4 *) (v + i0));
+    }
+    static __device__ __forceinline__ raw sload(const float *v, unsigned int i0, unsigned int last) {
+        raw r;
+        r.x = __float_as_uint(v[min(i0, last)]);
+        r.y = __float_as_uint(v[min(i0 + 1u, last)]);
+        r.z = __float_as_uint(v[min(i0 + 2u, last)]);
+        r.w = __float_as_uint(v[min(i0 + 3u, last)]);
+        return r;
+    }
+    static __device__ __forceinline__ float get(const raw &r, unsigned int k) {
+        return __uint_as_float(k == 0u ? r.x : k == 1u ? r.y : k == 2u ? r.z : r.w);
     }
 };
-template<> struct unit_loader<CSAMD_VALUE_FP8_E4M3> {
-    using S = uint8_t;
-    static __device__ __forceinline__ void load(const S *v, unsigned long long e, S out[4]) {
-        const unsigned int w = __builtin_nontemporal_load((const unsigned int *) (v + e));
-        out[0] = (S) (w & 0xffu); out[1] = (S) ((w >> 8) & 0xffu); out[2] = (S) ((w >> 16) & 0xffu); out[3] = (S) (w >> 24);
+template<> struct unit_vals<CSAMD_VALUE_FP8_E4M3> {
+    using raw = unsigned int;
+    static __device__ __forceinline__ raw vload(const uint8_t *v, unsigned int i0) {
+        return __builtin_nontemporal_load((const unsigned int *) (v + i0));
+    }
+    static __device__ __forceinline__ raw sload(const uint8_t *v, unsigned int i0, unsigned int last) {
+        return (unsigned int) v[min(i0, last)] | ((unsigned int) v[min(i0 + 1u, last)] << 8) |
+               ((unsigned int) v[min(i0 + 2u, last)] << 16) | ((unsigned int) v[min(i0 + 3u, last)] << 24);
+    }
+    static __device__ __forceinline__ float get(const raw &r, unsigned int k) {
+        return fp8_e4m3_to_float((uint8_t) (r >> (8u * k)));
     }
 };
 
@@ -287,72 +319,86 @@ constexpr unsigned int LR_UNITS = 2u;               // 4-entry units per lane pe
 constexpr unsigned int LR_CHUNK_UNITS = 64u * LR_UNITS;
 
 template<int VT> struct chunk_regs {
-    unsigned int c[4 * LR_UNITS];
-    typename value_traits<VT>::storage v[4 * LR_UNITS];
+    u32x4 c[LR_UNITS];
+    typename unit_vals<VT>::raw v[LR_UNITS];
 };
 
-// Load one chunk (LR_UNITS units per lane) of the entry range [b, e); entries
-// outside [b, e) get column CSAMD_INVALID_COL.  Units below full_end are
-// entirely inside the arrays and use vector loads.
+// Issue the loads of one chunk: unit u + lane + 64 j of the current row.
+// Lanes past the row's last unit u_end - 1 re-read that unit (same cache
+// lines, masked when consumed), so every lane loads and no lane branches.
+// The whole chunk uses vector loads when all its units lie below full_end
+// (the count of complete 4-entry units of the arrays); the chunk holding the
+// arrays' partial last unit reads single entries clamped to index last.
 template<int VT, bool VEC>
 __device__ __forceinline__ void load_chunk(const unsigned int *__restrict__ idx, const typename value_traits<VT>::storage *__restrict__ val,
-                                           unsigned int u, unsigned int u_end, unsigned int full_end, unsigned int b,
-                                           unsigned int e, chunk_regs<VT> &r) {
+                                           unsigned int u, unsigned int u_end, unsigned int full_end, unsigned int last,
+                                           chunk_regs<VT> &r) {
     const unsigned int lane = lane_id();
+    const unsigned int top = min(u + LR_CHUNK_UNITS - 1u, u_end - 1u);  // wave-uniform
+    if (VEC && top < full_end) {
 #pragma unroll
-    for (unsigned int j = 0; j < LR_UNITS; ++j) {
-        const unsigned int uu = u + lane + 64u * j;
-        unsigned int *cp = r.c + 4 * j;
-        auto *vp = r.v + 4 * j;
-#pragma unroll
-        for (unsigned int k = 0; k < 4u; ++k) {
-            cp[k] = CSAMD_INVALID_COL;
-            vp[k] = 0;
+        for (unsigned int j = 0; j < LR_UNITS; ++j) {
+            const unsigned int uu = min(u + lane + 64u * j, u_end - 1u);
+            r.c[j] = __builtin_nontemporal_load((const u32x4 *) (idx + uu * 4u));
+            r.v[j] = unit_vals<VT>::vload(val, uu * 4u);
         }
-        if (uu >= u_end) continue;
-        const unsigned int i0 = uu * 4u;
-        if (VEC && uu < full_end) {
-            const u32x4 w = __builtin_nontemporal_load((const u32x4 *) (idx + i0));
-            cp[0] = w.x; cp[1] = w.y; cp[2] = w.z; cp[3] = w.w;
-            unit_loader<VT>::load(val, i0, vp);
+    } else {
 #pragma unroll
-            for (unsigned int k = 0; k < 4u; ++k)
-                if (i0 + k < b || i0 + k >= e) cp[k] = CSAMD_INVALID_COL;
-        } else {
-#pragma unroll
-            for (unsigned int k = 0; k < 4u; ++k) {
-                const unsigned int i = i0 + k;
-                if (i >= b && i < e) {
-                    cp[k] = idx[i];
-                    vp[k] = val[i];
-                }
-            }
+        for (unsigned int j = 0; j < LR_UNITS; ++j) {
+            const unsigned int i0 = min(u + lane + 64u * j, u_end - 1u) * 4u;
+            r.c[j].x = idx[min(i0, last)];
+            r.c[j].y = idx[min(i0 + 1u, last)];
+            r.c[j].z = idx[min(i0 + 2u, last)];
+            r.c[j].w = idx[min(i0 + 3u, last)];
+            r.v[j] = unit_vals<VT>::sload(val, i0, last);
         }
     }
 }
 
-// Consume a batch: all rank lookups first (independent LDS reads), then the
-// accumulator atomics, so the lookups are not ordered behind the atomics.
-// lpairs[nwords] is a {0, 0} sentinel, so out-of-range columns (including
-// padding) resolve to "not kept" without a branch.
-template<int VT, unsigned int N>
-__device__ __forceinline__ void consume_batch(const unsigned int *c, const typename value_traits<VT>::storage *raw,
+__device__ __forceinline__ unsigned int lane_of(const u32x4 &w, unsigned int k) {
+    return k == 0u ? w.x : k == 1u ? w.y : k == 2u ? w.z : w.w;
+}
+
+// Consume one chunk of the row whose entries are [b, e), starting at unit u.
+// All bitmap lookups are issued first (independent LDS reads); keep tests,
+// ranks and values are computed without branches; one branch per entry
+// guards the three accumulator atomics.  Nothing here reads global memory,
+// so the prefetched chunk stays in flight (a feature scale is applied by the
+// combine).  lpairs[nwords] is a {0, 0}
+// sentinel, so columns >= cols (and CSAMD_INVALID_COL) are "not kept".
+// Entries outside [b, e) exist only in a row's first and last chunk; only
+// those chunks (a wave-uniform test) pay for the range mask.
+template<int VT>
+__device__ __forceinline__ void consume_chunk(const chunk_regs<VT> &r, unsigned int u, unsigned int b, unsigned int e,
                                               const uint2 *__restrict__ lpairs, const reduce_args &a, const acc_lds &acc) {
+    constexpr unsigned int N = 4u * LR_UNITS;
+    unsigned int c[N];
+#pragma unroll
+    for (unsigned int j = 0; j < LR_UNITS; ++j)
+#pragma unroll
+        for (unsigned int k = 0; k < 4u; ++k) c[4u * j + k] = lane_of(r.c[j], k);
+    const unsigned long long first = 4ull * u;
+    if (first < b || first + 4ull * LR_CHUNK_UNITS > e) {
+        const unsigned int lane = lane_id();
+        const unsigned int len = e - b;
+#pragma unroll
+        for (unsigned int j = 0; j < LR_UNITS; ++j)
+#pragma unroll
+            for (unsigned int k = 0; k < 4u; ++k) {
+                const unsigned int i = (u + lane + 64u * j) * 4u + k;
+                if (i - b >= len) c[4u * j + k] = CSAMD_INVALID_COL;
+            }
+    }
     uint2 pm[N];
 #pragma unroll
-    for (unsigned int k = 0; k < N; ++k) {
-        const unsigned int w = c[k] >> 5;
-        pm[k] = lpairs[w < a.nwords ? w : a.nwords];
-    }
+    for (unsigned int k = 0; k < N; ++k) pm[k] = lpairs[min(c[k] >> 5, a.nwords)];
 #pragma unroll
     for (unsigned int k = 0; k < N; ++k) {
         const unsigned int bit = c[k] & 31u;
-        if (((pm[k].x >> bit) & 1u) == 0u) continue;
         const unsigned int f = pm[k].y + (unsigned int) __popc(pm[k].x & ((1u << bit) - 1u)) - a.f0;
-        if (f >= a.f_count) continue;
-        float v = value_traits<VT>::decode(raw[k]);
-        if (a.scale != nullptr) v *= a.scale[c[k]];
-        if (v != 0.0f) {
+        const float v = unit_vals<VT>::get(r.v[k >> 2], k & 3u);
+        const bool kept = ((pm[k].x >> bit) & 1u) != 0u && f < a.f_count && v != 0.0f;
+        if (kept) {
             const double d = (double) v;
             atomicAdd(acc.sum + f, d);
             atomicAdd(acc.sumsq + f, d * d);
@@ -373,8 +419,8 @@ __device__ __forceinline__ void row_units(uint2 rng, unsigned int *u, unsigned i
 template<int VT, bool VEC>
 __device__ __forceinline__ void stream_rows(const unsigned int *__restrict__ idx, const typename value_traits<VT>::storage *__restrict__ val,
                                             const uint2 *__restrict__ ranges, unsigned int s0, unsigned int s1, unsigned int wave,
-                                            unsigned int nwaves, unsigned int full_end, const uint2 *__restrict__ lpairs,
-                                            const reduce_args &a, const acc_lds &acc) {
+                                            unsigned int nwaves, unsigned int full_end, unsigned int last,
+                                            const uint2 *__restrict__ lpairs, const reduce_args &a, const acc_lds &acc) {
     unsigned int i = s0 + wave;
     if (i >= s1) return;
     const uint2 zero = make_uint2(0u, 0u);
@@ -392,7 +438,7 @@ __device__ __forceinline__ void stream_rows(const unsigned int *__restrict__ idx
         row_units(rng, &u, &ue);
     }
     chunk_regs<VT> A, B;
-    load_chunk<VT, VEC>(idx, val, u, ue, full_end, rng.x, rng.y, A);
+    load_chunk<VT, VEC>(idx, val, u, ue, full_end, last, A);
     while (true) {
         unsigned int i2 = i;
         uint2 r2 = rng;
@@ -411,8 +457,8 @@ __device__ __forceinline__ void stream_rows(const unsigned int *__restrict__ idx
                 row_units(r2, &u2, &ue2);
             } while (u2 >= ue2);
         }
-        if (more) load_chunk<VT, VEC>(idx, val, u2, ue2, full_end, r2.x, r2.y, B);
-        consume_batch<VT, 4u * LR_UNITS>(A.c, A.v, lpairs, a, acc);
+        if (more) load_chunk<VT, VEC>(idx, val, u2, ue2, full_end, last, B);
+        consume_chunk<VT>(A, u, rng.x, rng.y, lpairs, a, acc);
         if (!more) break;
         A = B;
         i = i2;
@@ -435,6 +481,7 @@ __global__ void label_reduce_rows_kernel(const unsigned int *__restrict__ idx, c
     unsigned int *lcnt = (unsigned int *) (lsumsq + a.ftile);
     const acc_lds acc{lsum, lsumsq, lcnt};
     const unsigned int full_end = (unsigned int) (limit >> 2);
+    const unsigned int last = limit != 0ull ? (unsigned int) (limit - 1ull) : 0u;
 
     for (unsigned int w = threadIdx.x; w <= a.nwords; w += nthreads) lpairs[w] = w < a.nwords ? a.fpairs[w] : make_uint2(0u, 0u);
 
@@ -463,7 +510,7 @@ __global__ void label_reduce_rows_kernel(const unsigned int *__restrict__ idx, c
             lcnt[f] = 0u;
         }
         __syncthreads();
-        stream_rows<VT, VEC>(idx, val, ranges, s0, s1, wave, nwaves, full_end, lpairs, a, acc);
+        stream_rows<VT, VEC>(idx, val, ranges, s0, s1, wave, nwaves, full_end, last, lpairs, a, acc);
         __syncthreads();
         const unsigned long long slot = (unsigned long long) (blockIdx.x + g) * a.ftile;
         for (unsigned int f = threadIdx.x; f < a.f_count; f += nthreads) {
@@ -540,11 +587,14 @@ __global__ void label_reduce_blocked_kernel(csamd_blocked_ell_view src, reduce_a
     }
 }
 
-// Combine: out[g][f0 + f] = sum over touching workgroups w (ascending) of slot (w + g).
+// Combine: out[g][f0 + f] = sum over touching workgroups w (ascending) of
+// slot (w + g); then the feature's scale s (if any): s * sum, s^2 * sumsq,
+// and no nonzeros when s == 0.
 __global__ void label_combine_kernel(const double *__restrict__ psum, const double *__restrict__ psumsq,
                                      const unsigned int *__restrict__ pcnt, const unsigned int *__restrict__ gstart,
                                      const unsigned int *__restrict__ bounds, unsigned int W, unsigned int G, unsigned int ftile,
-                                     unsigned int f0, unsigned int f_count, unsigned int F, double *__restrict__ out_sum,
+                                     unsigned int f0, unsigned int f_count, unsigned int F, const float *__restrict__ scale,
+                                     const unsigned int *__restrict__ fids, double *__restrict__ out_sum,
                                      double *__restrict__ out_sumsq, unsigned int *__restrict__ out_cnt) {
     const unsigned long long total = (unsigned long long) G * f_count;
     for (unsigned long long t = blockIdx.x * (unsigned long long) blockDim.x + threadIdx.x; t < total;
@@ -571,6 +621,12 @@ __global__ void label_combine_kernel(const double *__restrict__ psum, const doub
                 q += psumsq[slot];
                 n += pcnt[slot];
             }
+        }
+        if (scale != nullptr) {
+            const double sc = (double) scale[fids[f0 + f]];
+            s *= sc;
+            q *= sc * sc;
+            if (sc == 0.0) n = 0u;
         }
         const unsigned long long o = (unsigned long long) g * F + f0 + f;
         out_sum[o] = s;
@@ -737,8 +793,8 @@ int run_label_reduce(const layout_in &L, unsigned int value_type, const csamd_la
     if (need_all <= (size_t) max_lds / 2u) {
         // tuning override for sweeps: CSAMD_LR_THREADS in {256, 512, 768, 1024}
         const char *env = std::getenv("CSAMD_LR_THREADS");
-        const unsigned int t = env != nullptr ? (unsigned int) std::atoi(env) : 512u;
-        threads = (t >= 256u && t <= 1024u && t % 64u == 0u) ? t : 512u;
+        const unsigned int t = env != nullptr ? (unsigned int) std::atoi(env) : 768u;
+        threads = (t >= 256u && t <= 1024u && t % 64u == 0u) ? t : 768u;
         per_cu = 2u;
         ftile = F == 0u ? 1u : F;
     } else {
@@ -803,7 +859,6 @@ int run_label_reduce(const layout_in &L, unsigned int value_type, const csamd_la
     a.gstart = gstart;
     a.bounds = bounds;
     a.fpairs = (const uint2 *) P->fpairs.p;
-    a.scale = q->feature_scale;
     a.psum = (double *) (pb + p_sum);
     a.psumsq = (double *) (pb + p_sumsq);
     a.pcnt = (unsigned int *) (pb + p_cnt);
@@ -836,7 +891,8 @@ int run_label_reduce(const layout_in &L, unsigned int value_type, const csamd_la
         }
         if (a.f_count != 0u) {
             label_combine_kernel<<<grid_for((unsigned long long) G * a.f_count, 256u, 8192u), 256, 0, s>>>(
-                a.psum, a.psumsq, a.pcnt, gstart, bounds, W, G, ftile, f0, a.f_count, F, out_sum, out_sumsq, out_cnt);
+                a.psum, a.psumsq, a.pcnt, gstart, bounds, W, G, ftile, f0, a.f_count, F, q->feature_scale, fids, out_sum,
+                out_sumsq, out_cnt);
             CSAMD_HIP(hipGetLastError(), "label_combine_kernel");
         }
         first = false;

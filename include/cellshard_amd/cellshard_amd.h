@@ -348,7 +348,9 @@ int csamd_manual_reoptimize_masked_sparse(const csamd_blocked_ell_view *src,
  *   nnz[g][f]   += (v(r,c) != 0)
  * Stored zeros and invalid/out-of-range columns contribute nothing.  Partials
  * are combined in FP64; mean = sum/n_g and var = sumsq/n_g - mean^2 (ddof 0),
- * n_g = number of kept rows with label g.
+ * n_g = number of kept rows with label g.  With feature_scale, v(r,c) is the
+ * decoded value times scale[c] in FP64; the scale is per feature, so it is
+ * applied once per (g, f) after the sums (s*sum, s^2*sumsq, nnz 0 if s == 0).
  * ------------------------------------------------------------------------- */
 typedef struct CSAMD_ALIGN16 csamd_label_reduce_query {
     unsigned int group_count;          /* G, 1..CSAMD_MAX_LABEL_GROUPS */
@@ -356,7 +358,7 @@ typedef struct CSAMD_ALIGN16 csamd_label_reduce_query {
     const unsigned int *row_labels;    /* device [rows]; label >= G drops the row */
     const unsigned char *row_keep;     /* device [rows] or NULL (all rows) */
     const unsigned char *feature_keep; /* device [cols] or NULL (all features) */
-    const float *feature_scale;        /* device [cols] or NULL; multiplies decoded values (FP8 absmax scale) */
+    const float *feature_scale;        /* device [cols] or NULL; multiplies decoded values in FP64 (FP8 absmax scale) */
     unsigned int compute_moments;      /* 1: also fill mean/var */
     unsigned int reserved;
 } csamd_label_reduce_query;
@@ -588,7 +590,7 @@ typedef struct CSAMD_ALIGN16 csamd_host_sliced_ell {
     csamd_half *val;
 } csamd_host_sliced_ell;
 
-/* Release host storage (formats/*.cuh clear()); resets metadata. */
+/* Release host storage (the formats/ headers' clear()); resets metadata. */
 void csamd_host_compressed_clear(csamd_host_compressed *m);
 void csamd_host_blocked_ell_clear(csamd_host_blocked_ell *m);
 void csamd_host_sliced_ell_clear(csamd_host_sliced_ell *m);

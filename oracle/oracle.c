@@ -290,9 +290,10 @@ static void label_entry(const label_ctx *L, uint32_t g, uint32_t c, float raw) {
     if (c >= L->cols) return;
     const uint32_t f = L->rank[c];
     if (f == INVALID_COL) return;
-    const float v = L->scale != NULL ? raw * L->scale[c] : raw;
-    if (v != 0.0f) {
-        const double d = (double) v;
+    /* the per-feature scale multiplies in FP64, so an entry counts iff both
+     * the stored value and its feature's scale are nonzero */
+    const double d = L->scale != NULL ? (double) raw * (double) L->scale[c] : (double) raw;
+    if (d != 0.0) {
         const size_t o = (size_t) g * L->F + f;
         L->sum[o] += d;
         L->sumsq[o] += d * d;

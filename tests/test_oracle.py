@@ -108,10 +108,9 @@ def test_label_reduce_matches_np_add_at(vt):
     bits = encode(val, vt)
     o = O.label_reduce_csr(rows, cols, ptr, idx, bits, vt, labels, rk, fk, G, scale)
     kept = np.flatnonzero(fk)
-    v = decode(bits, vt).astype(np.float32)
+    v = decode(bits, vt).astype(np.float32).astype(np.float64)
     if scale is not None:
-        v = (v * scale[idx]).astype(np.float32)
-    v = v.astype(np.float64)
+        v = v * scale[idx].astype(np.float64)
     rows_of = np.repeat(np.arange(rows), np.diff(ptr))
     ok = (rk[rows_of] == 1) & (labels[rows_of] < G) & (fk[idx] == 1) & (v != 0)
     rank = np.full(cols, -1)
