@@ -29,7 +29,6 @@ namespace csamd {
 namespace {
 
 constexpr unsigned int LR_ACC_BYTES = 20u;   // f64 sum + f64 sumsq + u32 nnz per kept feature
-constexpr unsigned int LR_MAX_WAVES = 16u;
 
 struct dev_buf {
     void *p;
@@ -59,6 +58,7 @@ struct label_private {
     dev_buf work;        // u32 per sorted row + u64 prefix
     dev_buf bounds;      // [W+1]
     dev_buf partial;     // (W+G) x Ftile x {f64, f64, u32}
+    dev_buf tail;        // tail records
     dev_buf out;         // outputs
     hipEvent_t ev0, ev1;
     int events_ready;
@@ -136,6 +136,16 @@ struct csr_geom {
         *b = ptr[row];
         *e = ptr[row + 1u];
     }
+    // Row holding entry i: the last row r with ptr[r] <= i (rows if none).
+    __device__ __forceinline__ unsigned int row_of(unsigned long long i, unsigned int rows) const {
+        unsigned int lo = 0u, hi = rows;  // first r with ptr[r + 1] > i
+        while (lo < hi) {
+            const unsigned int mid = lo + ((hi - lo) >> 1);
+            if ((unsigned long long) ptr[mid + 1u] <= i) lo = mid + 1u;
+            else hi = mid;
+        }
+        return lo;
+    }
 };
 
 struct sell_geom {
@@ -164,19 +174,82 @@ struct sell_geom {
         *b = base;
         *e = base + w;
     }
+    // Row holding slot i (rows if none): its slice, then its row in the slice.
+    __device__ __forceinline__ unsigned int row_of(unsigned long long i, unsigned int rows) const {
+        if (s.slice_count == 0u) return rows;
+        unsigned int lo = 0u, hi = s.slice_count - 1u;  // last slice with slot offset <= i
+        while (lo < hi) {
+            const unsigned int mid = (lo + hi + 1u) >> 1;
+            if ((unsigned long long) s.slice_slot_offsets[mid] <= i) lo = mid;
+            else hi = mid - 1u;
+        }
+        const unsigned int w = s.slice_widths[lo];
+        if (w == 0u) return rows;
+        const unsigned long long r = (unsigned long long) s.slice_row_offsets[lo] + (i - s.slice_slot_offsets[lo]) / w;
+        return r < rows ? (unsigned int) r : rows;
+    }
 };
 
 // Per sorted kept row: its entry range (both layouts index entries with u32)
-// and its work weight for the partition.
+// and its work weight for the partition.  Ranges are clipped at entry clip:
+// the vector-load kernel streams only whole 4-entry units, and the at most 3
+// entries past the last whole unit are tail records (tail_records_kernel).
 template<typename Geom>
 __global__ void row_work_kernel(Geom geom, const unsigned int *__restrict__ sorted_rows, unsigned int n,
-                                unsigned int *__restrict__ work, uint2 *__restrict__ ranges) {
+                                unsigned long long clip, unsigned int *__restrict__ work, uint2 *__restrict__ ranges) {
     for (unsigned int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         unsigned long long b, e;
         geom.range(sorted_rows[i], &b, &e);
         const unsigned long long len = e - b;
         work[i] = (unsigned int) (len > 0xfffffff0ull ? 0xfffffff0ull : len) + 8u;  // +8: per-row overhead
-        if (ranges != nullptr) ranges[i] = make_uint2((unsigned int) b, (unsigned int) e);
+        if (ranges != nullptr) ranges[i] = make_uint2((unsigned int) (b < clip ? b : clip), (unsigned int) (e < clip ? e : clip));
+    }
+}
+
+// One stored entry past the last whole 4-entry unit, resolved for the
+// combine: label g (>= G: contributes nothing), global kept rank f, value.
+struct tail_record {
+    unsigned int g;
+    unsigned int f;
+    double v;
+};
+constexpr unsigned int LR_MAX_TAIL = 3u;
+
+__device__ __forceinline__ float decode_any(const void *val, unsigned long long i, unsigned int vt) {
+    switch (vt) {
+        case CSAMD_VALUE_F16: return f16_bits_to_float(((const uint16_t *) val)[i]);
+        case CSAMD_VALUE_BF16: return bf16_bits_to_float(((const uint16_t *) val)[i]);
+        case CSAMD_VALUE_F32: return ((const float *) val)[i];
+        default: return fp8_e4m3_to_float(((const uint8_t *) val)[i]);
+    }
+}
+
+// Entries [clip, limit) (at most 3): find each one's row, apply the row and
+// feature masks and write a tail record.  One thread; runs before the main
+// kernel and is read by the combine.
+template<typename Geom>
+__global__ void tail_records_kernel(Geom geom, unsigned int rows, const unsigned int *__restrict__ idx, const void *val,
+                                    unsigned int vt, unsigned long long clip, unsigned long long limit, unsigned int cols,
+                                    const unsigned int *__restrict__ labels, const unsigned char *__restrict__ row_keep,
+                                    unsigned int G, const uint2 *__restrict__ fpairs, tail_record *__restrict__ out) {
+    if (blockIdx.x != 0u || threadIdx.x != 0u) return;
+    for (unsigned int t = 0; t < LR_MAX_TAIL; ++t) {
+        tail_record rec{G, CSAMD_INVALID_COL, 0.0};
+        const unsigned long long i = clip + t;
+        if (i < limit) {
+            const unsigned int r = geom.row_of(i, rows);
+            const unsigned int c = idx[i];
+            if (r < rows && (row_keep == nullptr || row_keep[r] != 0u) && labels[r] < G && c < cols) {
+                const uint2 pm = fpairs[c >> 5];
+                const unsigned int bit = c & 31u;
+                if ((pm.x >> bit) & 1u) {
+                    rec.g = labels[r];
+                    rec.f = pm.y + (unsigned int) __popc(pm.x & ((1u << bit) - 1u));
+                    rec.v = (double) decode_any(val, i, vt);
+                }
+            }
+        }
+        out[t] = rec;
     }
 }
 
@@ -315,7 +388,7 @@ template<> struct unit_vals<CSAMD_VALUE_FP8_E4M3> {
     }
 };
 
-constexpr unsigned int LR_UNITS = 2u;               // 4-entry units per lane per chunk
+constexpr unsigned int LR_UNITS = 4u;               // 4-entry units per lane per chunk (even)
 constexpr unsigned int LR_CHUNK_UNITS = 64u * LR_UNITS;
 
 template<int VT> struct chunk_regs {
@@ -326,16 +399,16 @@ template<int VT> struct chunk_regs {
 // Issue the loads of one chunk: unit u + lane + 64 j of the current row.
 // Lanes past the row's last unit u_end - 1 re-read that unit (same cache
 // lines, masked when consumed), so every lane loads and no lane branches.
-// The whole chunk uses vector loads when all its units lie below full_end
-// (the count of complete 4-entry units of the arrays); the chunk holding the
-// arrays' partial last unit reads single entries clamped to index last.
+// VEC: row ranges end at or before the last whole 4-entry unit of the
+// arrays, so every unit is one 16-byte index load and one value load, and
+// the loop carries a fixed number of loads per chunk (which lets the
+// compiler wait for the current chunk only).  Otherwise (unaligned arrays)
+// single entries are read, clamped to index last.
 template<int VT, bool VEC>
 __device__ __forceinline__ void load_chunk(const unsigned int *__restrict__ idx, const typename value_traits<VT>::storage *__restrict__ val,
-                                           unsigned int u, unsigned int u_end, unsigned int full_end, unsigned int last,
-                                           chunk_regs<VT> &r) {
+                                           unsigned int u, unsigned int u_end, unsigned int last, chunk_regs<VT> &r) {
     const unsigned int lane = lane_id();
-    const unsigned int top = min(u + LR_CHUNK_UNITS - 1u, u_end - 1u);  // wave-uniform
-    if (VEC && top < full_end) {
+    if (VEC) {
 #pragma unroll
         for (unsigned int j = 0; j < LR_UNITS; ++j) {
             const unsigned int uu = min(u + lane + 64u * j, u_end - 1u);
@@ -359,50 +432,52 @@ __device__ __forceinline__ unsigned int lane_of(const u32x4 &w, unsigned int k) 
     return k == 0u ? w.x : k == 1u ? w.y : k == 2u ? w.z : w.w;
 }
 
-// Consume one chunk of the row whose entries are [b, e), starting at unit u.
-// All bitmap lookups are issued first (independent LDS reads); keep tests,
-// ranks and values are computed without branches; one branch per entry
-// guards the three accumulator atomics.  Nothing here reads global memory,
-// so the prefetched chunk stays in flight (a feature scale is applied by the
-// combine).  lpairs[nwords] is a {0, 0}
-// sentinel, so columns >= cols (and CSAMD_INVALID_COL) are "not kept".
-// Entries outside [b, e) exist only in a row's first and last chunk; only
-// those chunks (a wave-uniform test) pay for the range mask.
+// Consume one chunk of the row whose entries are [b, e), starting at unit u,
+// two units (8 entries per lane) at a time.  Per pair, the 8 bitmap lookups
+// are issued first
+// (independent LDS reads); keep tests, ranks and values are computed
+// without branches; one branch per entry guards the three accumulator
+// atomics.  Nothing here reads global memory, so the prefetched chunk stays
+// in flight (a feature scale is applied by the combine).  lpairs[nwords] is
+// a {0, 0} sentinel, so columns >= cols (and CSAMD_INVALID_COL) are "not
+// kept".  Entries outside [b, e) exist only in a row's first and last
+// chunk; only those chunks (a wave-uniform test) pay for the range mask.
+// Every loaded unit is consumed: a load whose registers are never read is
+// waited for wherever the registers are next written.
 template<int VT>
 __device__ __forceinline__ void consume_chunk(const chunk_regs<VT> &r, unsigned int u, unsigned int b, unsigned int e,
                                               const uint2 *__restrict__ lpairs, const reduce_args &a, const acc_lds &acc) {
-    constexpr unsigned int N = 4u * LR_UNITS;
-    unsigned int c[N];
-#pragma unroll
-    for (unsigned int j = 0; j < LR_UNITS; ++j)
-#pragma unroll
-        for (unsigned int k = 0; k < 4u; ++k) c[4u * j + k] = lane_of(r.c[j], k);
     const unsigned long long first = 4ull * u;
-    if (first < b || first + 4ull * LR_CHUNK_UNITS > e) {
-        const unsigned int lane = lane_id();
-        const unsigned int len = e - b;
+    const bool edge = first < b || first + 4ull * LR_CHUNK_UNITS > e;
+    const unsigned int lane = lane_id();
+    const unsigned int len = e - b;
 #pragma unroll
-        for (unsigned int j = 0; j < LR_UNITS; ++j)
+    for (unsigned int p = 0; p < LR_UNITS / 2u; ++p) {
+        unsigned int c[8];
 #pragma unroll
-            for (unsigned int k = 0; k < 4u; ++k) {
-                const unsigned int i = (u + lane + 64u * j) * 4u + k;
-                if (i - b >= len) c[4u * j + k] = CSAMD_INVALID_COL;
+        for (unsigned int k = 0; k < 8u; ++k) c[k] = lane_of(r.c[2u * p + (k >> 2)], k & 3u);
+        if (edge) {
+#pragma unroll
+            for (unsigned int k = 0; k < 8u; ++k) {
+                const unsigned int i = (u + lane + 64u * (2u * p + (k >> 2))) * 4u + (k & 3u);
+                if (i - b >= len) c[k] = CSAMD_INVALID_COL;
             }
-    }
-    uint2 pm[N];
+        }
+        uint2 pm[8];
 #pragma unroll
-    for (unsigned int k = 0; k < N; ++k) pm[k] = lpairs[min(c[k] >> 5, a.nwords)];
+        for (unsigned int k = 0; k < 8u; ++k) pm[k] = lpairs[min(c[k] >> 5, a.nwords)];
 #pragma unroll
-    for (unsigned int k = 0; k < N; ++k) {
-        const unsigned int bit = c[k] & 31u;
-        const unsigned int f = pm[k].y + (unsigned int) __popc(pm[k].x & ((1u << bit) - 1u)) - a.f0;
-        const float v = unit_vals<VT>::get(r.v[k >> 2], k & 3u);
-        const bool kept = ((pm[k].x >> bit) & 1u) != 0u && f < a.f_count && v != 0.0f;
-        if (kept) {
-            const double d = (double) v;
-            atomicAdd(acc.sum + f, d);
-            atomicAdd(acc.sumsq + f, d * d);
-            atomicAdd(acc.cnt + f, 1u);
+        for (unsigned int k = 0; k < 8u; ++k) {
+            const unsigned int bit = c[k] & 31u;
+            const unsigned int f = pm[k].y + (unsigned int) __popc(pm[k].x & ((1u << bit) - 1u)) - a.f0;
+            const float v = unit_vals<VT>::get(r.v[2u * p + (k >> 2)], k & 3u);
+            const bool kept = ((pm[k].x >> bit) & 1u) != 0u && f < a.f_count && v != 0.0f;
+            if (kept) {
+                const double d = (double) v;
+                atomicAdd(acc.sum + f, d);
+                atomicAdd(acc.sumsq + f, d * d);
+                atomicAdd(acc.cnt + f, 1u);
+            }
         }
     }
 }
@@ -412,59 +487,85 @@ __device__ __forceinline__ void row_units(uint2 rng, unsigned int *u, unsigned i
     *ue = rng.x == rng.y ? *u : (unsigned int) (((unsigned long long) rng.y + 3ull) >> 2);
 }
 
-// Rows of one label segment, one wave per row in turn (i = s0 + wave,
-// stride nwaves).  The next chunk (possibly of the next row) is loaded
-// before the current one is consumed, and each row's range is fetched one
-// row ahead, so two chunks and one range are in flight per wave.
+// Next row of the label segment for this wave: one LDS atomic per row, so a
+// wave that runs faster (wave age decides issue priority) takes more rows
+// and all waves of the workgroup finish the segment together.
+__device__ __forceinline__ unsigned int grab_row(unsigned int *next) {
+    unsigned int v = 0u;
+    if (lane_id() == 0u) v = atomicAdd(next, 1u);
+    return __builtin_amdgcn_readfirstlane(v);
+}
+
+// Position of a chunk in the wave's row stream (wave-uniform values).
+struct row_cursor {
+    uint2 rng;        // entry range [x, y) of the row
+    unsigned int u;   // first unit of the chunk
+    unsigned int ue;  // end unit of the row
+};
+
+// Rows of one label segment [.., s1), taken one at a time from the shared
+// row counter.  The chunk stream runs through two fixed register buffers in
+// turn (A, B, A, ...): the next chunk, possibly of the next row, is loaded
+// before the current one is consumed, with no register copies between
+// them, and the next row is taken and its range fetched one row ahead.
+struct row_stream {
+    const uint2 *ranges;
+    unsigned int *next;
+    unsigned int s1;
+    unsigned int inext;  // row taken ahead (>= s1: none left)
+    uint2 nxt;           // its range
+
+    __device__ __forceinline__ void take() {
+        inext = grab_row(next);
+        nxt = inext < s1 ? ranges[inext] : make_uint2(0u, 0u);
+    }
+    // Move to the first non-empty row taken ahead; false when none is left.
+    __device__ __forceinline__ bool next_row(row_cursor &c) {
+        do {
+            if (inext >= s1) return false;
+            c.rng = nxt;
+            take();
+            row_units(c.rng, &c.u, &c.ue);
+        } while (c.u >= c.ue);
+        return true;
+    }
+    __device__ __forceinline__ bool advance(row_cursor &c) {
+        const unsigned int u2 = c.u + LR_CHUNK_UNITS;
+        if (u2 < c.ue) {
+            c.u = u2;
+            return true;
+        }
+        return next_row(c);
+    }
+};
+
 template<int VT, bool VEC>
 __device__ __forceinline__ void stream_rows(const unsigned int *__restrict__ idx, const typename value_traits<VT>::storage *__restrict__ val,
-                                            const uint2 *__restrict__ ranges, unsigned int s0, unsigned int s1, unsigned int wave,
-                                            unsigned int nwaves, unsigned int full_end, unsigned int last,
-                                            const uint2 *__restrict__ lpairs, const reduce_args &a, const acc_lds &acc) {
-    unsigned int i = s0 + wave;
-    if (i >= s1) return;
-    const uint2 zero = make_uint2(0u, 0u);
-    uint2 rng = ranges[i];
-    unsigned int inext = i + nwaves;
-    uint2 nxt = inext < s1 ? ranges[inext] : zero;
-    unsigned int u, ue;
-    row_units(rng, &u, &ue);
-    while (u >= ue) {  // skip empty rows
-        i = inext;
-        if (i >= s1) return;
-        rng = nxt;
-        inext += nwaves;
-        nxt = inext < s1 ? ranges[inext] : zero;
-        row_units(rng, &u, &ue);
-    }
+                                            const uint2 *__restrict__ ranges, unsigned int *next, unsigned int s1,
+                                            unsigned int last, const uint2 *__restrict__ lpairs,
+                                            const reduce_args &a, const acc_lds &acc) {
+    row_stream st{ranges, next, s1, 0u, make_uint2(0u, 0u)};
+    st.take();
+    row_cursor ca, cb;
+    if (!st.next_row(ca)) return;
+    // Loads are issued unconditionally (after the last chunk the wave
+    // re-loads it, in bounds and unused): a load under a branch makes the
+    // compiler merge old and new buffer contents, which waits for the load.
     chunk_regs<VT> A, B;
-    load_chunk<VT, VEC>(idx, val, u, ue, full_end, last, A);
+    load_chunk<VT, VEC>(idx, val, ca.u, ca.ue, last, A);
     while (true) {
-        unsigned int i2 = i;
-        uint2 r2 = rng;
-        unsigned int u2 = u + LR_CHUNK_UNITS, ue2 = ue;
-        bool more = true;
-        if (u2 >= ue2) {
-            do {
-                i2 = inext;
-                if (i2 >= s1) {
-                    more = false;
-                    break;
-                }
-                r2 = nxt;
-                inext += nwaves;
-                nxt = inext < s1 ? ranges[inext] : zero;
-                row_units(r2, &u2, &ue2);
-            } while (u2 >= ue2);
-        }
-        if (more) load_chunk<VT, VEC>(idx, val, u2, ue2, full_end, last, B);
-        consume_chunk<VT>(A, u, rng.x, rng.y, lpairs, a, acc);
+        cb = ca;
+        bool more = st.advance(cb);
+        if (!more) cb = ca;
+        load_chunk<VT, VEC>(idx, val, cb.u, cb.ue, last, B);
+        consume_chunk<VT>(A, ca.u, ca.rng.x, ca.rng.y, lpairs, a, acc);
         if (!more) break;
-        A = B;
-        i = i2;
-        rng = r2;
-        u = u2;
-        ue = ue2;
+        ca = cb;
+        more = st.advance(ca);
+        if (!more) ca = cb;
+        load_chunk<VT, VEC>(idx, val, ca.u, ca.ue, last, A);
+        consume_chunk<VT>(B, cb.u, cb.rng.x, cb.rng.y, lpairs, a, acc);
+        if (!more) break;
     }
 }
 
@@ -472,15 +573,13 @@ template<int VT, bool VEC>
 __global__ void label_reduce_rows_kernel(const unsigned int *__restrict__ idx, const typename value_traits<VT>::storage *__restrict__ val,
                                          unsigned long long limit, const uint2 *__restrict__ ranges, reduce_args a) {
     extern __shared__ __align__(16) unsigned char lds[];
+    __shared__ unsigned int next_row;
     const unsigned int nthreads = blockDim.x;
-    const unsigned int nwaves = nthreads >> 6;
-    const unsigned int wave = threadIdx.x >> 6;
     uint2 *lpairs = (uint2 *) lds;
     double *lsum = (double *) (lds + align_up((size_t) (a.nwords + 1u) * sizeof(uint2), 16u));
     double *lsumsq = lsum + a.ftile;
     unsigned int *lcnt = (unsigned int *) (lsumsq + a.ftile);
     const acc_lds acc{lsum, lsumsq, lcnt};
-    const unsigned int full_end = (unsigned int) (limit >> 2);
     const unsigned int last = limit != 0ull ? (unsigned int) (limit - 1ull) : 0u;
 
     for (unsigned int w = threadIdx.x; w <= a.nwords; w += nthreads) lpairs[w] = w < a.nwords ? a.fpairs[w] : make_uint2(0u, 0u);
@@ -509,8 +608,9 @@ __global__ void label_reduce_rows_kernel(const unsigned int *__restrict__ idx, c
             lsumsq[f] = 0.0;
             lcnt[f] = 0u;
         }
+        if (threadIdx.x == 0u) next_row = s0;
         __syncthreads();
-        stream_rows<VT, VEC>(idx, val, ranges, s0, s1, wave, nwaves, full_end, last, lpairs, a, acc);
+        stream_rows<VT, VEC>(idx, val, ranges, &next_row, s1, last, lpairs, a, acc);
         __syncthreads();
         const unsigned long long slot = (unsigned long long) (blockIdx.x + g) * a.ftile;
         for (unsigned int f = threadIdx.x; f < a.f_count; f += nthreads) {
@@ -594,8 +694,9 @@ __global__ void label_combine_kernel(const double *__restrict__ psum, const doub
                                      const unsigned int *__restrict__ pcnt, const unsigned int *__restrict__ gstart,
                                      const unsigned int *__restrict__ bounds, unsigned int W, unsigned int G, unsigned int ftile,
                                      unsigned int f0, unsigned int f_count, unsigned int F, const float *__restrict__ scale,
-                                     const unsigned int *__restrict__ fids, double *__restrict__ out_sum,
-                                     double *__restrict__ out_sumsq, unsigned int *__restrict__ out_cnt) {
+                                     const unsigned int *__restrict__ fids, const tail_record *__restrict__ tail,
+                                     double *__restrict__ out_sum, double *__restrict__ out_sumsq,
+                                     unsigned int *__restrict__ out_cnt) {
     const unsigned long long total = (unsigned long long) G * f_count;
     for (unsigned long long t = blockIdx.x * (unsigned long long) blockDim.x + threadIdx.x; t < total;
          t += (unsigned long long) gridDim.x * blockDim.x) {
@@ -621,6 +722,15 @@ __global__ void label_combine_kernel(const double *__restrict__ psum, const doub
                 q += psumsq[slot];
                 n += pcnt[slot];
             }
+        }
+        if (tail != nullptr) {
+#pragma unroll
+            for (unsigned int t = 0; t < LR_MAX_TAIL; ++t)
+                if (tail[t].g == g && tail[t].f == f0 + f && tail[t].v != 0.0) {
+                    s += tail[t].v;
+                    q += tail[t].v * tail[t].v;
+                    n += 1u;
+                }
         }
         if (scale != nullptr) {
             const double sc = (double) scale[fids[f0 + f]];
@@ -788,24 +898,31 @@ int run_label_reduce(const layout_in &L, unsigned int value_type, const csamd_la
     CSAMD_REQUIRE(fixed_lds + 64u * LR_ACC_BYTES <= (size_t) max_lds,
                   "label_reduce: %u columns need %zu bytes of LDS for the feature map", cols, fixed_lds);
     const unsigned int cu = (unsigned int) device_cu_count();
-    unsigned int threads, per_cu, ftile;
+    // One 1024-thread workgroup per CU: its waves share the CU's row range
+    // through an LDS counter.  (Two workgroups per CU run unevenly: the
+    // older one gets issue priority and the younger one finishes late at
+    // half occupancy.)  CSAMD_LR_THREADS / CSAMD_LR_PER_CU / CSAMD_LR_SPLIT
+    // override the shape for tuning sweeps.
+    const size_t lds_avail = (size_t) max_lds - 16u;  // static LDS: the row counter
+    auto env_u = [](const char *name, unsigned int dflt) {
+        const char *e = std::getenv(name);
+        return e != nullptr ? (unsigned int) std::atoi(e) : dflt;
+    };
+    unsigned int threads = env_u("CSAMD_LR_THREADS", 1024u);
+    if (threads < 256u || threads > 1024u || threads % 64u != 0u) threads = 1024u;
+    unsigned int per_cu = env_u("CSAMD_LR_PER_CU", 1u);
     const size_t need_all = fixed_lds + (size_t) (F == 0u ? 1u : F) * LR_ACC_BYTES;
-    if (need_all <= (size_t) max_lds / 2u) {
-        // tuning override for sweeps: CSAMD_LR_THREADS in {256, 512, 768, 1024}
-        const char *env = std::getenv("CSAMD_LR_THREADS");
-        const unsigned int t = env != nullptr ? (unsigned int) std::atoi(env) : 768u;
-        threads = (t >= 256u && t <= 1024u && t % 64u == 0u) ? t : 768u;
-        per_cu = 2u;
-        ftile = F == 0u ? 1u : F;
-    } else {
-        threads = 1024u;
-        per_cu = 1u;
-        ftile = (unsigned int) (((size_t) max_lds - fixed_lds) / LR_ACC_BYTES);
+    if (per_cu != 2u || need_all > lds_avail / 2u) per_cu = 1u;
+    unsigned int ftile = F == 0u ? 1u : F;
+    if (need_all > lds_avail) {
+        ftile = (unsigned int) ((lds_avail - fixed_lds) / LR_ACC_BYTES);
         ftile &= ~63u;
         if (ftile > F) ftile = F;
         if (ftile == 0u) ftile = 64u;
     }
-    const unsigned int W = cu * per_cu;
+    unsigned int split = env_u("CSAMD_LR_SPLIT", 1u);
+    if (split < 1u || split > 16u) split = 1u;
+    const unsigned int W = cu * per_cu * split;
     const size_t lds = fixed_lds + (size_t) ftile * LR_ACC_BYTES;
 
     // 4. work bounds
@@ -820,9 +937,31 @@ int run_label_reduce(const layout_in &L, unsigned int value_type, const csamd_la
     void *wscan = (char *) P->work.p + work_bytes + prefix_bytes + ranges_bytes;
     if (!ensure_buf(P->bounds, (size_t) (W + 1u) * sizeof(unsigned int), "hipMalloc bounds")) return 0;
     unsigned int *bounds = (unsigned int *) P->bounds.p;
+    // Vector loads need 16-byte aligned indices and unit-aligned values; the
+    // vector kernel streams whole 4-entry units only (ranges clipped at
+    // clip) and the entries past clip become tail records.
+    const bool aligned = L.kind != 2 && ((uintptr_t) L.idx % 16u == 0u) &&
+                         ((uintptr_t) L.val % (uintptr_t) (value_type == CSAMD_VALUE_F32 ? 16u
+                                                           : value_type == CSAMD_VALUE_FP8_E4M3 ? 4u : 8u) == 0u);
+    const unsigned long long clip = aligned ? (L.limit & ~3ull) : ~0ull;
+    const tail_record *tail = nullptr;
+    if (aligned && clip < L.limit) {
+        if (!ensure_buf(P->tail, LR_MAX_TAIL * sizeof(tail_record), "hipMalloc tail records")) return 0;
+        tail_record *t = (tail_record *) P->tail.p;
+        if (L.kind == 0)
+            tail_records_kernel<csr_geom><<<1, 64, 0, s>>>(L.csr, rows, L.idx, L.val, value_type, clip, L.limit, cols, q->row_labels,
+                                                           q->row_keep, G, (const uint2 *) P->fpairs.p, t);
+        else
+            tail_records_kernel<sell_geom><<<1, 64, 0, s>>>(L.sell, rows, L.idx, L.val, value_type, clip, L.limit, cols,
+                                                            q->row_labels, q->row_keep, G, (const uint2 *) P->fpairs.p, t);
+        CSAMD_HIP(hipGetLastError(), "tail_records_kernel");
+        tail = t;
+    }
     if (K != 0u) {
-        if (L.kind == 0) row_work_kernel<csr_geom><<<grid_for(K, 256u, 8192u), 256, 0, s>>>(L.csr, ids_out, K, work, ranges);
-        else if (L.kind == 1) row_work_kernel<sell_geom><<<grid_for(K, 256u, 8192u), 256, 0, s>>>(L.sell, ids_out, K, work, ranges);
+        if (L.kind == 0)
+            row_work_kernel<csr_geom><<<grid_for(K, 256u, 8192u), 256, 0, s>>>(L.csr, ids_out, K, clip, work, ranges);
+        else if (L.kind == 1)
+            row_work_kernel<sell_geom><<<grid_for(K, 256u, 8192u), 256, 0, s>>>(L.sell, ids_out, K, clip, work, ranges);
         else blocked_work_kernel<<<grid_for(K, 256u, 8192u), 256, 0, s>>>(L.bell.ell_cols, K, work);
         CSAMD_HIP(hipGetLastError(), "row_work_kernel");
     }
@@ -867,9 +1006,6 @@ int run_label_reduce(const layout_in &L, unsigned int value_type, const csamd_la
     a.nwords = nwords;
     a.ftile = ftile;
     P->last_valid = 0;
-    const bool aligned = L.kind != 2 && ((uintptr_t) L.idx % 16u == 0u) &&
-                         ((uintptr_t) L.val % (uintptr_t) (value_type == CSAMD_VALUE_F32 ? 16u
-                                                           : value_type == CSAMD_VALUE_FP8_E4M3 ? 4u : 8u) == 0u);
     bool first = true;
     for (unsigned int f0 = 0u; f0 < F || (F == 0u && first); f0 += ftile) {
         a.f0 = f0;
@@ -891,7 +1027,7 @@ int run_label_reduce(const layout_in &L, unsigned int value_type, const csamd_la
         }
         if (a.f_count != 0u) {
             label_combine_kernel<<<grid_for((unsigned long long) G * a.f_count, 256u, 8192u), 256, 0, s>>>(
-                a.psum, a.psumsq, a.pcnt, gstart, bounds, W, G, ftile, f0, a.f_count, F, q->feature_scale, fids, out_sum,
+                a.psum, a.psumsq, a.pcnt, gstart, bounds, W, G, ftile, f0, a.f_count, F, q->feature_scale, fids, tail, out_sum,
                 out_sumsq, out_cnt);
             CSAMD_HIP(hipGetLastError(), "label_combine_kernel");
         }
@@ -935,7 +1071,7 @@ void csamd_label_reduce_clear(csamd_label_reduce_workspace *ws) {
     label_private *P = (label_private *) ws->private_workspace;
     if (P != nullptr) {
         dev_buf *bufs[] = {&P->fpairs, &P->fcounts, &P->fids, &P->keys, &P->rows, &P->sort_tmp, &P->gstart,
-                           &P->work, &P->bounds, &P->partial, &P->out};
+                           &P->work, &P->bounds, &P->partial, &P->out, &P->tail};
         for (dev_buf *b : bufs)
             if (b->p != nullptr) (void) hipFree(b->p);
         if (P->events_ready) {

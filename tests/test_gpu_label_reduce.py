@@ -147,3 +147,45 @@ def test_synthetic_counts_parity(lr):
     for k in ("sum", "sumsq", "nnz"):
         assert np.array_equal(got[k], got2[k])
     ws.close()
+
+
+@pytest.mark.parametrize("extra", [0, 1, 2, 3])
+def test_tail_entries(lr, extra):
+    """nnz % 4 == extra: the entries past the last whole 4-entry unit are
+    resolved as tail records; put them in a kept row on kept features, for
+    CSR and for the SELL built from it, and for matrices with fewer than 4
+    entries."""
+    rows, cols, G = 41, 64, 3
+    rng = np.random.default_rng(100 + extra)
+    per_row = rng.integers(0, 9, size=rows)
+    per_row[-1] = 4 + extra
+    total = int(per_row.sum())
+    per_row[-2] += (extra - total) % 4  # nnz % 4 == extra
+    ptr = np.concatenate([[0], np.cumsum(per_row)]).astype(np.uint32)
+    idx = np.concatenate([np.sort(rng.choice(cols, n, replace=False)) for n in per_row]).astype(np.uint32)
+    val = rng.integers(1, 50, size=len(idx)).astype(np.float64)
+    bits = encode(val, F16)
+    assert len(idx) % 4 == extra
+    labels = rng.integers(0, G, size=rows).astype(np.uint32)
+    rk = np.ones(rows, np.uint8)
+    fk = np.ones(cols, np.uint8)
+    fk[::5] = 0
+    src = DeviceCSR.from_numpy(rows, cols, ptr, idx, bits)
+    ref = O.label_reduce_csr(rows, cols, ptr, idx, bits, F16, labels, rk, fk, G)
+    check(lr_fetch(lr, src, labels, G, rk, fk), ref)
+    s = O.sliced_ell_from_compressed(rows, cols, 0, ptr, idx, bits, 8)
+    sell = DeviceSlicedELL.from_numpy(rows, cols, len(idx), s["slice_rows"], s["offsets"], s["widths"], s["slot_offsets"],
+                                      s["col_idx"], s["val"])
+    check(lr_fetch(lr, sell, labels, G, rk, fk), ref)
+    # a matrix with only `extra` entries: everything is tail
+    if extra:
+        ptr2 = np.zeros(rows + 1, np.uint32)
+        ptr2[-1] = extra
+        idx2 = idx[-extra:].copy()
+        bits2 = bits[-extra:].copy()
+        src2 = DeviceCSR.from_numpy(rows, cols, ptr2, idx2, bits2)
+        check(lr_fetch(lr, src2, labels, G, rk, fk), O.label_reduce_csr(rows, cols, ptr2, idx2, bits2, F16, labels, rk, fk, G))
+
+
+def lr_fetch(lr, src, labels, G, rk, fk):
+    return LabelReduce.fetch(lr.run(src, to_dev_u32(labels), G, to_dev_u8(rk), to_dev_u8(fk)))
