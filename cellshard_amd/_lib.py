@@ -65,6 +65,7 @@ PROTOTYPES = {
     "csamd_label_reduce_sliced_ell": (i32, [P(A.sliced_ell_view), P(A.label_reduce_query), P(A.label_reduce_workspace), P(A.label_reduce_result)]),
     "csamd_label_reduce_blocked_ell": (i32, [P(A.blocked_ell_view), P(A.label_reduce_query), P(A.label_reduce_workspace), P(A.label_reduce_result)]),
     "csamd_label_reduce_last_kernel_ms": (f32, [P(A.label_reduce_workspace)]),
+    "csamd_label_reduce_set_timing": (i32, [P(A.label_reduce_workspace), i32]),
     # convert
     "csamd_convert_init": (None, [P(A.convert_workspace)]),
     "csamd_convert_clear": (None, [P(A.convert_workspace)]),

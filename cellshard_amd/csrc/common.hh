@@ -209,6 +209,11 @@ __device__ __forceinline__ unsigned long long block_exclusive_scan_u64(unsigned 
 // ---------------------------------------------------------------------------
 // Device-wide scans (reduce-then-scan; inputs are metadata-sized).
 // ---------------------------------------------------------------------------
+// Tiles of CSAMD_SCAN_TILE consecutive items: per-tile exclusive offsets
+// into d_tile_offsets[0..tiles), total into d_tile_offsets[tiles]
+// (tiles = max(1, ceil(n / CSAMD_SCAN_TILE))).
+#define CSAMD_SCAN_TILE 4096u
+int scan_tile_offsets_u32(const unsigned int *d_in, unsigned int n, unsigned long long *d_tile_offsets, hipStream_t stream);
 // Exclusive scan of u32 counts into u64 offsets out[0..n] (out[n] = total).
 int exclusive_scan_u32_to_u64(const unsigned int *d_in, unsigned int n, unsigned long long *d_out,
                               void *scratch, size_t scratch_bytes, hipStream_t stream);

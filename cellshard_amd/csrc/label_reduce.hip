@@ -49,82 +49,154 @@ int ensure_buf(dev_buf &b, size_t bytes, const char *label) {
 
 struct label_private {
     dev_buf fpairs;      // uint2 {mask, prefix} per 32-column word
-    dev_buf fcounts;     // per-word popcounts / scan tmp
     dev_buf fids;        // kept feature ids
-    dev_buf keys;        // sort keys in/out
-    dev_buf rows;        // sort values in/out
-    dev_buf sort_tmp;
-    dev_buf gstart;      // [G+1]
-    dev_buf work;        // u32 per sorted row + u64 prefix
-    dev_buf bounds;      // [W+1]
-    dev_buf partial;     // (W+G) x Ftile x {f64, f64, u32}
+    dev_buf labels;      // label table [(G + 1) x sort workgroups], gstart [G + 1], F
+    dev_buf rows;        // sorted kept rows [rows]
+    dev_buf work;        // work weights [rows], tile offsets, ranges [rows]
+    dev_buf bounds;      // [W + 1]
+    dev_buf partial;     // (W + G) x ftile x {f64, f64, u32}
     dev_buf tail;        // tail records
     dev_buf out;         // outputs
-    hipEvent_t ev0, ev1;
+    volatile unsigned int *host;  // pinned: [0] kept rows, [1] kept features
+    hipEvent_t ev0, ev1, ev_meta;
     int events_ready;
-    float last_ms;
+    int timing;
     int last_valid;
 };
 
+// One stored entry past the last whole 4-entry unit, resolved for the
+// combine: label g (>= G: contributes nothing), global kept rank f, value.
+struct tail_record {
+    unsigned int g;
+    unsigned int f;
+    double v;
+};
+constexpr unsigned int LR_MAX_TAIL = 3u;
+
+__device__ __forceinline__ float decode_any(const void *val, unsigned long long i, unsigned int vt) {
+    switch (vt) {
+        case CSAMD_VALUE_F16: return f16_bits_to_float(((const uint16_t *) val)[i]);
+        case CSAMD_VALUE_BF16: return bf16_bits_to_float(((const uint16_t *) val)[i]);
+        case CSAMD_VALUE_F32: return ((const float *) val)[i];
+        default: return fp8_e4m3_to_float(((const uint8_t *) val)[i]);
+    }
+}
+
 // ---------------------------------------------------------------------------
-// Feature map.
+// Feature map (one workgroup): per 32-column word {kept mask, kept features
+// before the word}, kept feature ids in column order, F.  Also clears the
+// tail records (label G + 1: contributes nothing) that the scatter fills
+// for kept rows.
 // ---------------------------------------------------------------------------
-__global__ void feature_words_kernel(const unsigned char *__restrict__ feature_keep, unsigned int cols, unsigned int nwords,
-                                     unsigned int *__restrict__ words, unsigned int *__restrict__ counts) {
-    for (unsigned int w = blockIdx.x * blockDim.x + threadIdx.x; w < nwords; w += gridDim.x * blockDim.x) {
+__global__ void __launch_bounds__(1024) feature_map_kernel(const unsigned char *__restrict__ feature_keep, unsigned int cols,
+                                                           unsigned int nwords, uint2 *__restrict__ pairs,
+                                                           unsigned int *__restrict__ fids, unsigned int *__restrict__ f_total,
+                                                           tail_record *__restrict__ tail, unsigned int nlabels) {
+    __shared__ unsigned int smem[16];
+    if (tail != nullptr && threadIdx.x < LR_MAX_TAIL) tail[threadIdx.x] = tail_record{nlabels, CSAMD_INVALID_COL, 0.0};
+    unsigned int carry = 0u;  // same value in every thread
+    for (unsigned int start = 0; start < nwords; start += blockDim.x) {
+        const unsigned int w = start + threadIdx.x;
         unsigned int m = 0u;
-        const unsigned int c0 = w * 32u;
-        for (unsigned int b = 0; b < 32u; ++b) {
-            const unsigned int c = c0 + b;
-            if (c < cols && (feature_keep == nullptr || feature_keep[c] != 0u)) m |= 1u << b;
+        if (w < nwords) {
+            const unsigned int c0 = w * 32u;
+            if (feature_keep == nullptr) {
+                m = cols - c0 >= 32u ? 0xffffffffu : (1u << (cols - c0)) - 1u;
+            } else {
+                unsigned char v[32];  // all loads issued before any is used
+#pragma unroll
+                for (unsigned int b = 0; b < 32u; ++b) v[b] = feature_keep[min(c0 + b, cols - 1u)];
+#pragma unroll
+                for (unsigned int b = 0; b < 32u; ++b) m |= (c0 + b < cols && v[b] != 0u ? 1u : 0u) << b;
+            }
         }
-        words[w] = m;
-        counts[w] = (unsigned int) __popc(m);
-    }
-}
-
-__global__ void feature_pairs_kernel(const unsigned int *__restrict__ words, const unsigned int *__restrict__ prefix,
-                                     unsigned int nwords, uint2 *__restrict__ pairs, unsigned int *__restrict__ fids) {
-    for (unsigned int w = blockIdx.x * blockDim.x + threadIdx.x; w < nwords; w += gridDim.x * blockDim.x) {
-        const unsigned int m = words[w];
-        const unsigned int p = prefix[w];
-        pairs[w] = make_uint2(m, p);
-        unsigned int mm = m, k = 0u;
-        while (mm != 0u) {
-            const unsigned int b = (unsigned int) __builtin_ctz(mm);
-            fids[p + k] = w * 32u + b;
-            ++k;
-            mm &= mm - 1u;
+        unsigned int total = 0u;
+        const unsigned int ex = carry + block_exclusive_scan_u32((unsigned int) __popc(m), smem, &total);
+        if (w < nwords) {
+            pairs[w] = make_uint2(m, ex);
+            unsigned int mm = m, k = 0u;
+            while (mm != 0u) {
+                fids[ex + k++] = w * 32u + (unsigned int) __builtin_ctz(mm);
+                mm &= mm - 1u;
+            }
         }
+        carry += total;
     }
+    if (threadIdx.x == 0) *f_total = carry;
 }
 
 // ---------------------------------------------------------------------------
-// Row keys: label if kept else G (dropped); identity values.
+// Rows grouped by label: a counting sort on key = label (kept rows) or G
+// (dropped).  Workgroup b histograms its rows (the tiles b, b + grid, ...)
+// in LDS and stores the counts to hist[g * grid + b]; one workgroup scans
+// that label-major table into each (label, workgroup) pair's first slot;
+// the scatter re-reads the same tiles and places rows.  No global atomics:
+// device-scope atomics on a few hot label counters serialize.
 // ---------------------------------------------------------------------------
-__global__ void row_keys_kernel(const unsigned int *__restrict__ labels, const unsigned char *__restrict__ row_keep,
-                                unsigned int rows, unsigned int G, unsigned int *__restrict__ keys,
-                                unsigned int *__restrict__ ids) {
-    for (unsigned int r = blockIdx.x * blockDim.x + threadIdx.x; r < rows; r += gridDim.x * blockDim.x) {
-        const unsigned int l = labels[r];
-        const bool keep = (row_keep == nullptr || row_keep[r] != 0u) && l < G;
-        keys[r] = keep ? l : G;
-        ids[r] = r;
-    }
+constexpr unsigned int LR_SORT_THREADS = 1024u;
+constexpr unsigned int LR_SORT_ITEMS = 8u;
+constexpr unsigned int LR_SORT_TILE = LR_SORT_THREADS * LR_SORT_ITEMS;
+
+__device__ __forceinline__ unsigned int row_key(const unsigned int *__restrict__ labels, const unsigned char *__restrict__ row_keep,
+                                                unsigned int r, unsigned int G) {
+    const unsigned int l = labels[r];
+    return ((row_keep == nullptr || row_keep[r] != 0u) && l < G) ? l : G;
 }
 
-// gstart[g] = first sorted position with key >= g, g = 0..G.
-__global__ void group_starts_kernel(const unsigned int *__restrict__ sorted_keys, unsigned int n, unsigned int G,
-                                    unsigned int *__restrict__ gstart) {
-    for (unsigned int g = blockIdx.x * blockDim.x + threadIdx.x; g <= G; g += gridDim.x * blockDim.x) {
-        unsigned int lo = 0u, hi = n;
-        while (lo < hi) {
-            const unsigned int mid = lo + ((hi - lo) >> 1);
-            if (sorted_keys[mid] < g) lo = mid + 1u;
-            else hi = mid;
+// Also clears work[0..work_n): positions past the kept rows keep weight 0
+// for the partition scan.
+__global__ void __launch_bounds__(LR_SORT_THREADS) label_count_kernel(const unsigned int *__restrict__ labels,
+                                                                      const unsigned char *__restrict__ row_keep, unsigned int rows,
+                                                                      unsigned int G, unsigned int *__restrict__ hist_out,
+                                                                      unsigned int *__restrict__ work, unsigned int work_n) {
+    extern __shared__ unsigned int hist[];  // [G + 1]
+    for (unsigned int g = threadIdx.x; g <= G; g += blockDim.x) hist[g] = 0u;
+    __syncthreads();
+    for (unsigned int t0 = blockIdx.x * LR_SORT_TILE; t0 < rows; t0 += gridDim.x * LR_SORT_TILE) {
+        unsigned int key[LR_SORT_ITEMS];
+#pragma unroll
+        for (unsigned int k = 0; k < LR_SORT_ITEMS; ++k) {
+            const unsigned int r = t0 + k * LR_SORT_THREADS + threadIdx.x;
+            key[k] = r < rows ? row_key(labels, row_keep, r, G) : G + 1u;
         }
-        gstart[g] = lo;
+#pragma unroll
+        for (unsigned int k = 0; k < LR_SORT_ITEMS; ++k)
+            if (key[k] <= G) atomicAdd(&hist[key[k]], 1u);
     }
+    const unsigned int stride = gridDim.x * blockDim.x;
+    for (unsigned int i = blockIdx.x * blockDim.x + threadIdx.x; i < work_n; i += stride) work[i] = 0u;
+    __syncthreads();
+    for (unsigned int g = threadIdx.x; g <= G; g += blockDim.x) hist_out[(unsigned long long) g * gridDim.x + blockIdx.x] = hist[g];
+}
+
+// One workgroup: exclusive scan of the label-major table hist[(G + 1) x nb]
+// in place; gstart[g] = hist[g * nb] (gstart[G] = kept rows).
+constexpr unsigned int LR_OFF_ITEMS = 8u;
+
+__global__ void __launch_bounds__(1024) label_offsets_kernel(unsigned int *__restrict__ hist, unsigned int G, unsigned int nb,
+                                                             unsigned int *__restrict__ gstart) {
+    __shared__ unsigned int smem[16];
+    const unsigned long long n = (unsigned long long) (G + 1u) * nb;
+    unsigned int carry = 0u;  // same value in every thread
+    for (unsigned long long start = 0; start < n; start += (unsigned long long) blockDim.x * LR_OFF_ITEMS) {
+        const unsigned long long i0 = start + (unsigned long long) threadIdx.x * LR_OFF_ITEMS;
+        unsigned int v[LR_OFF_ITEMS], own = 0u;
+#pragma unroll
+        for (unsigned int k = 0; k < LR_OFF_ITEMS; ++k) {
+            v[k] = i0 + k < n ? hist[i0 + k] : 0u;
+            own += v[k];
+        }
+        unsigned int total = 0u;
+        unsigned int o = carry + block_exclusive_scan_u32(own, smem, &total);
+#pragma unroll
+        for (unsigned int k = 0; k < LR_OFF_ITEMS; ++k) {
+            if (i0 + k < n) hist[i0 + k] = o;
+            o += v[k];
+        }
+        carry += total;
+    }
+    __syncthreads();  // the block's table stores are visible to the block
+    for (unsigned int g = threadIdx.x; g <= G; g += blockDim.x) gstart[g] = hist[(unsigned long long) g * nb];
 }
 
 // ---------------------------------------------------------------------------
@@ -190,90 +262,128 @@ struct sell_geom {
     }
 };
 
-// Per sorted kept row: its entry range (both layouts index entries with u32)
-// and its work weight for the partition.  Ranges are clipped at entry clip:
-// the vector-load kernel streams only whole 4-entry units, and the at most 3
-// entries past the last whole unit are tail records (tail_records_kernel).
-template<typename Geom>
-__global__ void row_work_kernel(Geom geom, const unsigned int *__restrict__ sorted_rows, unsigned int n,
-                                unsigned long long clip, unsigned int *__restrict__ work, uint2 *__restrict__ ranges) {
-    for (unsigned int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        unsigned long long b, e;
-        geom.range(sorted_rows[i], &b, &e);
-        const unsigned long long len = e - b;
-        work[i] = (unsigned int) (len > 0xfffffff0ull ? 0xfffffff0ull : len) + 8u;  // +8: per-row overhead
-        if (ranges != nullptr) ranges[i] = make_uint2((unsigned int) (b < clip ? b : clip), (unsigned int) (e < clip ? e : clip));
+// Blocked-ELL rows all hold ell_cols slots (ranges are not used).
+struct bell_geom {
+    unsigned int ell_cols;
+    __device__ __forceinline__ void range(unsigned int, unsigned long long *b, unsigned long long *e) const {
+        *b = 0ull;
+        *e = ell_cols;
     }
-}
-
-// One stored entry past the last whole 4-entry unit, resolved for the
-// combine: label g (>= G: contributes nothing), global kept rank f, value.
-struct tail_record {
-    unsigned int g;
-    unsigned int f;
-    double v;
 };
-constexpr unsigned int LR_MAX_TAIL = 3u;
 
-__device__ __forceinline__ float decode_any(const void *val, unsigned long long i, unsigned int vt) {
-    switch (vt) {
-        case CSAMD_VALUE_F16: return f16_bits_to_float(((const uint16_t *) val)[i]);
-        case CSAMD_VALUE_BF16: return bf16_bits_to_float(((const uint16_t *) val)[i]);
-        case CSAMD_VALUE_F32: return ((const float *) val)[i];
-        default: return fp8_e4m3_to_float(((const uint8_t *) val)[i]);
-    }
-}
-
-// Entries [clip, limit) (at most 3): find each one's row, apply the row and
-// feature masks and write a tail record.  One thread; runs before the main
-// kernel and is read by the combine.
+// Scatter kept rows to their label's slots: workgroup b's rows of label g
+// start at the scanned table entry (g, b); within a tile, ranks come from
+// LDS atomics, so the order of rows within a label is not fixed (sums are
+// order independent up to FP64 rounding; counts are exact).  For each placed row
+// it also writes its entry range (both layouts index entries with u32),
+// clipped at entry clip -- the vector-load kernel streams only whole 4-entry
+// units; a kept row holding some of the at most 3 entries past the last
+// whole unit writes them as tail records -- and its work weight for the
+// partition (entries + 8 for the per-row cost).
 template<typename Geom>
-__global__ void tail_records_kernel(Geom geom, unsigned int rows, const unsigned int *__restrict__ idx, const void *val,
-                                    unsigned int vt, unsigned long long clip, unsigned long long limit, unsigned int cols,
-                                    const unsigned int *__restrict__ labels, const unsigned char *__restrict__ row_keep,
-                                    unsigned int G, const uint2 *__restrict__ fpairs, tail_record *__restrict__ out) {
-    if (blockIdx.x != 0u || threadIdx.x != 0u) return;
-    for (unsigned int t = 0; t < LR_MAX_TAIL; ++t) {
-        tail_record rec{G, CSAMD_INVALID_COL, 0.0};
-        const unsigned long long i = clip + t;
-        if (i < limit) {
-            const unsigned int r = geom.row_of(i, rows);
-            const unsigned int c = idx[i];
-            if (r < rows && (row_keep == nullptr || row_keep[r] != 0u) && labels[r] < G && c < cols) {
-                const uint2 pm = fpairs[c >> 5];
-                const unsigned int bit = c & 31u;
-                if ((pm.x >> bit) & 1u) {
-                    rec.g = labels[r];
-                    rec.f = pm.y + (unsigned int) __popc(pm.x & ((1u << bit) - 1u));
-                    rec.v = (double) decode_any(val, i, vt);
+__global__ void __launch_bounds__(LR_SORT_THREADS) label_scatter_kernel(Geom geom, const unsigned int *__restrict__ labels,
+                                                                        const unsigned char *__restrict__ row_keep, unsigned int rows,
+                                                                        unsigned int G, const unsigned int *__restrict__ base_table,
+                                                                        unsigned long long clip, unsigned int *__restrict__ sorted_rows,
+                                                                        uint2 *__restrict__ ranges, unsigned int *__restrict__ work,
+                                                                        tail_record *__restrict__ tail, const unsigned int *__restrict__ idx,
+                                                                        const void *val, unsigned int vt, unsigned int cols,
+                                                                        const uint2 *__restrict__ fpairs) {
+    extern __shared__ unsigned int hist[];  // [G + 1] tile counts, then [G + 1] running bases
+    unsigned int *base = hist + (G + 1u);
+    for (unsigned int g = threadIdx.x; g < G; g += blockDim.x) base[g] = base_table[(unsigned long long) g * gridDim.x + blockIdx.x];
+    for (unsigned int t0 = blockIdx.x * LR_SORT_TILE; t0 < rows; t0 += gridDim.x * LR_SORT_TILE) {
+        for (unsigned int g = threadIdx.x; g <= G; g += blockDim.x) hist[g] = 0u;
+        __syncthreads();
+        unsigned int key[LR_SORT_ITEMS], rank[LR_SORT_ITEMS];
+        unsigned long long rb[LR_SORT_ITEMS], re[LR_SORT_ITEMS];
+#pragma unroll
+        for (unsigned int k = 0; k < LR_SORT_ITEMS; ++k) {
+            const unsigned int r = t0 + k * LR_SORT_THREADS + threadIdx.x;
+            key[k] = r < rows ? row_key(labels, row_keep, r, G) : G;
+            rb[k] = re[k] = 0ull;
+            if (r < rows) geom.range(r, &rb[k], &re[k]);
+        }
+#pragma unroll
+        for (unsigned int k = 0; k < LR_SORT_ITEMS; ++k) rank[k] = key[k] < G ? atomicAdd(&hist[key[k]], 1u) : 0u;
+        __syncthreads();
+        __syncthreads();
+#pragma unroll
+        for (unsigned int k = 0; k < LR_SORT_ITEMS; ++k) {
+            if (key[k] >= G) continue;
+            const unsigned int r = t0 + k * LR_SORT_THREADS + threadIdx.x;
+            const unsigned int pos = base[key[k]] + rank[k];
+            const unsigned long long b = rb[k], e = re[k];
+            const unsigned long long len = e - b;
+            sorted_rows[pos] = r;
+            ranges[pos] = make_uint2((unsigned int) (b < clip ? b : clip), (unsigned int) (e < clip ? e : clip));
+            work[pos] = (unsigned int) (len > 0xfffffff0ull ? 0xfffffff0ull : len) + 8u;
+            if (tail != nullptr && e > clip) {  // this row holds tail entries
+                for (unsigned long long i = b > clip ? b : clip; i < e; ++i) {
+                    const unsigned int c = idx[i];
+                    tail_record rec{G + 1u, CSAMD_INVALID_COL, 0.0};
+                    if (c < cols) {
+                        const uint2 pm = fpairs[c >> 5];
+                        const unsigned int bit = c & 31u;
+                        if ((pm.x >> bit) & 1u) {
+                            rec.g = key[k];
+                            rec.f = pm.y + (unsigned int) __popc(pm.x & ((1u << bit) - 1u));
+                            rec.v = (double) decode_any(val, i, vt);
+                        }
+                    }
+                    tail[i - clip] = rec;
                 }
             }
         }
-        out[t] = rec;
+        __syncthreads();
+        for (unsigned int g = threadIdx.x; g < G; g += blockDim.x) base[g] += hist[g];
+        __syncthreads();
     }
 }
 
-__global__ void blocked_work_kernel(unsigned int ell_cols, unsigned int n, unsigned int *__restrict__ work) {
-    for (unsigned int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) work[i] = ell_cols + 1u;
-}
+// Partition bounds from the work weights: bounds[w] = first sorted position
+// p with prefix(p) >= floor(total * w / W), prefix(p) = sum of work[0..p).
+// Each thread scans its SCAN_ITEMS positions of a CSAMD_SCAN_TILE tile (tile
+// offsets precomputed) and writes the bounds that fall in its positions:
+// position q + 1 takes every w with prefix(q) < target(w) <= prefix(q + 1);
+// position 0 takes every w with target(w) == 0.  Positions at or past the
+// kept rows have weight 0 and take none, except through the total: the
+// last kept row's interval ends at total = target(W).
+constexpr unsigned int LR_BOUNDS_ITEMS = CSAMD_SCAN_TILE / 256u;
 
-// bounds[w] = first i with prefix[i] >= w * total / W; bounds[W] = n.
-__global__ void work_bounds_kernel(const unsigned long long *__restrict__ prefix, unsigned int n, unsigned int W,
-                                   unsigned int *__restrict__ bounds) {
-    const unsigned long long total = prefix[n];
-    for (unsigned int w = blockIdx.x * blockDim.x + threadIdx.x; w <= W; w += gridDim.x * blockDim.x) {
-        if (w == W) {
-            bounds[w] = n;
-            continue;
-        }
-        const unsigned long long target = (total / W) * w + ((total % W) * w) / W;
-        unsigned int lo = 0u, hi = n;
-        while (lo < hi) {
-            const unsigned int mid = lo + ((hi - lo) >> 1);
-            if (prefix[mid] < target) lo = mid + 1u;
-            else hi = mid;
-        }
-        bounds[w] = lo;
+__global__ void __launch_bounds__(256) work_bounds_kernel(const unsigned int *__restrict__ work, unsigned int n,
+                                                         const unsigned long long *__restrict__ tile_offsets, unsigned int tiles,
+                                                         unsigned int W, unsigned int *__restrict__ bounds) {
+    __shared__ unsigned long long smem[4];
+    const unsigned long long total = tile_offsets[tiles];
+    const unsigned long long first = (unsigned long long) blockIdx.x * CSAMD_SCAN_TILE + (unsigned long long) threadIdx.x * LR_BOUNDS_ITEMS;
+    unsigned int v[LR_BOUNDS_ITEMS];
+    unsigned long long own = 0ull;
+#pragma unroll
+    for (unsigned int i = 0; i < LR_BOUNDS_ITEMS; ++i) {
+        v[i] = first + i < n ? work[first + i] : 0u;
+        own += v[i];
+    }
+    unsigned long long blk = 0ull;
+    unsigned long long pre = tile_offsets[blockIdx.x] + block_exclusive_scan_u64(own, smem, &blk);
+    if (total == 0ull) {  // no kept rows: every range is empty
+        if (blockIdx.x == 0u)
+            for (unsigned int w = threadIdx.x; w <= W; w += blockDim.x) bounds[w] = 0u;
+        return;
+    }
+    if (first == 0ull) {
+        const unsigned long long wmax = (W - 1ull) / total;  // target(w) == 0 for w <= wmax
+        for (unsigned long long w = 0; w <= wmax && w <= W; ++w) bounds[w] = 0u;
+    }
+    // w whose target lies in this thread's span (pre, pre + own]: usually none
+    const unsigned long long w0 = ((pre + 1ull) * W + total - 1ull) / total;
+    const unsigned long long w1 = ((pre + own + 1ull) * W - 1ull) / total;
+    for (unsigned long long w = w0; w <= w1 && w <= W; ++w) {
+        const unsigned long long target = total * w / W;
+        unsigned long long hi = pre;
+        unsigned int i = 0u;
+        while (i < LR_BOUNDS_ITEMS && hi + v[i] < target) hi += v[i++];  // first item whose end reaches target
+        bounds[w] = (unsigned int) (first + i + 1ull);
     }
 }
 
@@ -291,16 +401,67 @@ struct reduce_args {
     const unsigned int *gstart;   // [G+1]
     const unsigned int *bounds;   // [W+1]
     const uint2 *fpairs;          // [nwords]
-    double *psum;                 // [(W+G) x ftile]
+    const unsigned int *f_total;  // kept features F (device)
+    double *psum;                 // [tiles x (W+G) x ftile]
     double *psumsq;
     unsigned int *pcnt;
     unsigned int G;
     unsigned int cols;
     unsigned int nwords;
     unsigned int ftile;           // features held in LDS per pass
-    unsigned int f0;              // first feature of this pass
-    unsigned int f_count;         // features in this pass
+    unsigned int f0;              // first feature of this pass (set per pass)
+    unsigned int f_count;         // features in this pass (set per pass)
 };
+
+// The workgroup's sorted-row range [bounds[b], bounds[b + 1]), walked one
+// feature tile (LDS pass) at a time and, within it, one label segment at a
+// time: zero the LDS accumulators, run body(segment start, end, tile args),
+// store the accumulators to partial slot (tile, workgroup + label).  F is
+// read on the device, so the host launches one kernel for all tiles.
+template<typename Body>
+__device__ __forceinline__ void walk_segments(const reduce_args &a, const acc_lds &acc, Body body) {
+    const unsigned int lo = a.bounds[blockIdx.x];
+    const unsigned int hi = a.bounds[blockIdx.x + 1u];
+    if (lo >= hi) return;  // uniform across the block
+    unsigned int g0;
+    {
+        unsigned int l = 0u, h = a.G;
+        while (l < h) {  // largest g with gstart[g] <= lo
+            const unsigned int mid = (l + h + 1u) >> 1;
+            if (a.gstart[mid] <= lo) l = mid;
+            else h = mid - 1u;
+        }
+        g0 = l;
+    }
+    const unsigned int F = *a.f_total;
+    const unsigned long long tile_slots = (unsigned long long) (gridDim.x + a.G) * a.ftile;
+    reduce_args t = a;
+    for (unsigned int tile = 0u; tile * a.ftile < F; ++tile) {
+        t.f0 = tile * a.ftile;
+        t.f_count = F - t.f0 < a.ftile ? F - t.f0 : a.ftile;
+        for (unsigned int g = g0; g < a.G; ++g) {
+            const unsigned int gs = a.gstart[g], ge = a.gstart[g + 1u];
+            const unsigned int s0 = gs > lo ? gs : lo;
+            const unsigned int s1 = ge < hi ? ge : hi;
+            if (s0 >= hi) break;
+            if (s0 >= s1) continue;
+            for (unsigned int f = threadIdx.x; f < t.f_count; f += blockDim.x) {
+                acc.sum[f] = 0.0;
+                acc.sumsq[f] = 0.0;
+                acc.cnt[f] = 0u;
+            }
+            body(s0, s1, (const reduce_args &) t);
+            __syncthreads();
+            const unsigned long long slot = tile * tile_slots + (unsigned long long) (blockIdx.x + g) * a.ftile;
+            for (unsigned int f = threadIdx.x; f < t.f_count; f += blockDim.x) {
+                a.psum[slot + f] = acc.sum[f];
+                a.psumsq[slot + f] = acc.sumsq[f];
+                a.pcnt[slot + f] = acc.cnt[f];
+            }
+            __syncthreads();
+        }
+    }
+}
 
 template<int VT>
 __device__ __forceinline__ void consume(unsigned int c, typename value_traits<VT>::storage raw, const uint2 *__restrict__ lpairs,
@@ -583,43 +744,11 @@ __global__ void label_reduce_rows_kernel(const unsigned int *__restrict__ idx, c
     const unsigned int last = limit != 0ull ? (unsigned int) (limit - 1ull) : 0u;
 
     for (unsigned int w = threadIdx.x; w <= a.nwords; w += nthreads) lpairs[w] = w < a.nwords ? a.fpairs[w] : make_uint2(0u, 0u);
-
-    const unsigned int lo = a.bounds[blockIdx.x];
-    const unsigned int hi = a.bounds[blockIdx.x + 1u];
-    if (lo >= hi) return;  // uniform across the block
-    unsigned int g;
-    {
-        unsigned int l = 0u, h = a.G;
-        while (l < h) {  // largest g with gstart[g] <= lo
-            const unsigned int mid = (l + h + 1u) >> 1;
-            if (a.gstart[mid] <= lo) l = mid;
-            else h = mid - 1u;
-        }
-        g = l;
-    }
-    for (; g < a.G; ++g) {
-        const unsigned int gs = a.gstart[g], ge = a.gstart[g + 1u];
-        const unsigned int s0 = gs > lo ? gs : lo;
-        const unsigned int s1 = ge < hi ? ge : hi;
-        if (s0 >= hi) break;
-        if (s0 >= s1) continue;
-        for (unsigned int f = threadIdx.x; f < a.f_count; f += nthreads) {
-            lsum[f] = 0.0;
-            lsumsq[f] = 0.0;
-            lcnt[f] = 0u;
-        }
+    walk_segments(a, acc, [&](unsigned int s0, unsigned int s1, const reduce_args &t) {
         if (threadIdx.x == 0u) next_row = s0;
-        __syncthreads();
-        stream_rows<VT, VEC>(idx, val, ranges, &next_row, s1, last, lpairs, a, acc);
-        __syncthreads();
-        const unsigned long long slot = (unsigned long long) (blockIdx.x + g) * a.ftile;
-        for (unsigned int f = threadIdx.x; f < a.f_count; f += nthreads) {
-            a.psum[slot + f] = lsum[f];
-            a.psumsq[slot + f] = lsumsq[f];
-            a.pcnt[slot + f] = lcnt[f];
-        }
-        __syncthreads();
-    }
+        __syncthreads();  // also publishes the zeroed accumulators
+        stream_rows<VT, VEC>(idx, val, ranges, &next_row, s1, last, lpairs, t, acc);
+    });
 }
 
 // Blocked-ELL rows: entries decoded through blockColIdx (scalar lane stride).
@@ -640,134 +769,115 @@ __global__ void label_reduce_blocked_kernel(csamd_blocked_ell_view src, reduce_a
     const unsigned int width = bs != 0u ? src.ell_cols / bs : 0u;
 
     for (unsigned int w = threadIdx.x; w < a.nwords; w += nthreads) lpairs[w] = a.fpairs[w];
-    const unsigned int lo = a.bounds[blockIdx.x];
-    const unsigned int hi = a.bounds[blockIdx.x + 1u];
-    if (lo >= hi) return;
-    unsigned int g;
-    {
-        unsigned int l = 0u, h = a.G;
-        while (l < h) {
-            const unsigned int mid = (l + h + 1u) >> 1;
-            if (a.gstart[mid] <= lo) l = mid;
-            else h = mid - 1u;
-        }
-        g = l;
-    }
-    for (; g < a.G; ++g) {
-        const unsigned int gs = a.gstart[g], ge = a.gstart[g + 1u];
-        const unsigned int s0 = gs > lo ? gs : lo;
-        const unsigned int s1 = ge < hi ? ge : hi;
-        if (s0 >= hi) break;
-        if (s0 >= s1) continue;
-        for (unsigned int f = threadIdx.x; f < a.f_count; f += nthreads) {
-            lsum[f] = 0.0;
-            lsumsq[f] = 0.0;
-            lcnt[f] = 0u;
-        }
-        __syncthreads();
+    walk_segments(a, acc, [&](unsigned int s0, unsigned int s1, const reduce_args &t) {
+        __syncthreads();  // publishes the zeroed accumulators
         for (unsigned int i = s0 + wave; i < s1; i += nwaves) {
-            const unsigned int row = a.sorted_rows[i];
+            const unsigned int row = t.sorted_rows[i];
             const unsigned int rb = bs != 0u ? row / bs : 0u;
             const unsigned long long base = (unsigned long long) row * src.ell_cols;
             for (unsigned int e = lane_id(); e < src.ell_cols; e += 64u) {
                 const unsigned int slot = e / bs;
                 const unsigned int bc = src.blockColIdx[(unsigned long long) rb * width + slot];
                 const unsigned int c = bc != CSAMD_INVALID_COL ? bc * bs + (e - slot * bs) : CSAMD_INVALID_COL;
-                consume<VT>(c, val[base + e], lpairs, a, acc);
+                consume<VT>(c, val[base + e], lpairs, t, acc);
             }
         }
-        __syncthreads();
-        const unsigned long long slot = (unsigned long long) (blockIdx.x + g) * a.ftile;
-        for (unsigned int f = threadIdx.x; f < a.f_count; f += nthreads) {
-            a.psum[slot + f] = lsum[f];
-            a.psumsq[slot + f] = lsumsq[f];
-            a.pcnt[slot + f] = lcnt[f];
-        }
-        __syncthreads();
-    }
+    });
 }
 
-// Combine: out[g][f0 + f] = sum over touching workgroups w (ascending) of
-// slot (w + g); then the feature's scale s (if any): s * sum, s^2 * sumsq,
-// and no nonzeros when s == 0.
-__global__ void label_combine_kernel(const double *__restrict__ psum, const double *__restrict__ psumsq,
-                                     const unsigned int *__restrict__ pcnt, const unsigned int *__restrict__ gstart,
-                                     const unsigned int *__restrict__ bounds, unsigned int W, unsigned int G, unsigned int ftile,
-                                     unsigned int f0, unsigned int f_count, unsigned int F, const float *__restrict__ scale,
-                                     const unsigned int *__restrict__ fids, const tail_record *__restrict__ tail,
-                                     double *__restrict__ out_sum, double *__restrict__ out_sumsq,
-                                     unsigned int *__restrict__ out_cnt) {
-    const unsigned long long total = (unsigned long long) G * f_count;
+// Finish: out[g][f] = sum, over the workgroups w whose range holds rows of
+// label g (ascending w), of partial slot (tile of f, w + g); then the tail
+// records; then the feature's scale s (if any): s * sum, s^2 * sumsq, no
+// nonzeros when s == 0; then mean = sum / n_g and var = sumsq / n_g - mean^2
+// (clamped at 0), n_g = kept rows of label g.  F is read on the device;
+// gstart and bounds are staged in LDS for the per-thread search.
+__global__ void __launch_bounds__(256) label_finish_kernel(
+    const double *__restrict__ psum, const double *__restrict__ psumsq, const unsigned int *__restrict__ pcnt,
+    const unsigned int *__restrict__ gstart_g, const unsigned int *__restrict__ bounds_g, unsigned int W, unsigned int G,
+    unsigned int ftile, const unsigned int *__restrict__ f_total, const float *__restrict__ scale,
+    const unsigned int *__restrict__ fids, const tail_record *__restrict__ tail, double *__restrict__ out_sum,
+    double *__restrict__ out_sumsq, unsigned int *__restrict__ out_cnt, double *__restrict__ mean, double *__restrict__ var,
+    unsigned int *__restrict__ group_rows) {
+    extern __shared__ unsigned int meta[];  // gstart[G + 1], bounds[W + 1]
+    unsigned int *gstart = meta;
+    unsigned int *bounds = meta + G + 1u;
+    for (unsigned int i = threadIdx.x; i <= G; i += blockDim.x) gstart[i] = gstart_g[i];
+    for (unsigned int i = threadIdx.x; i <= W; i += blockDim.x) bounds[i] = bounds_g[i];
+    __syncthreads();
+    const unsigned int F = *f_total;
+    const unsigned long long tile_slots = (unsigned long long) (W + G) * ftile;
+    const unsigned long long total = (unsigned long long) G * F;
     for (unsigned long long t = blockIdx.x * (unsigned long long) blockDim.x + threadIdx.x; t < total;
          t += (unsigned long long) gridDim.x * blockDim.x) {
-        const unsigned int g = (unsigned int) (t / f_count);
-        const unsigned int f = (unsigned int) (t - (unsigned long long) g * f_count);
+        const unsigned int g = (unsigned int) (t / F);
+        const unsigned int f = (unsigned int) (t - (unsigned long long) g * F);
+        const unsigned int tile = f / ftile;
+        const unsigned long long fbase = tile * tile_slots + (f - tile * ftile);
         const unsigned int gs = gstart[g], ge = gstart[g + 1u];
-        double s = 0.0, q = 0.0;
+        double sm = 0.0, q = 0.0;
         unsigned int n = 0u;
         if (gs < ge) {
-            // first w with bounds[w+1] > gs
-            unsigned int l = 0u, h = W - 1u;
+            unsigned int l = 0u, h = W - 1u;  // first w with bounds[w + 1] > gs
             while (l < h) {
                 const unsigned int mid = (l + h) >> 1;
                 if (bounds[mid + 1u] > gs) h = mid;
                 else l = mid + 1u;
             }
-            for (unsigned int w = l; w < W && bounds[w] < ge; ++w) {
-                const unsigned int a0 = bounds[w] > gs ? bounds[w] : gs;
-                const unsigned int a1 = bounds[w + 1u] < ge ? bounds[w + 1u] : ge;
-                if (a0 >= a1) continue;
-                const unsigned long long slot = (unsigned long long) (w + g) * ftile + f;
-                s += psum[slot];
-                q += psumsq[slot];
-                n += pcnt[slot];
+            unsigned int r = l;  // one past the last w with bounds[w] < ge
+            while (r < W && bounds[r] < ge) ++r;
+            for (unsigned int w0 = l; w0 < r; w0 += 4u) {
+                double ps[4], pq[4];
+                unsigned int pc[4];
+#pragma unroll
+                for (unsigned int k = 0; k < 4u; ++k) {  // loads first, then sums in w order
+                    const unsigned int w = w0 + k < r ? w0 + k : l;  // clamped: never reads past the slots
+                    const bool used = w0 + k < r && bounds[w] < bounds[w + 1u];
+                    const unsigned long long slot = fbase + (unsigned long long) (w + g) * ftile;
+                    ps[k] = used ? psum[slot] : 0.0;
+                    pq[k] = used ? psumsq[slot] : 0.0;
+                    pc[k] = used ? pcnt[slot] : 0u;
+                }
+#pragma unroll
+                for (unsigned int k = 0; k < 4u; ++k) {
+                    sm += ps[k];
+                    q += pq[k];
+                    n += pc[k];
+                }
             }
         }
         if (tail != nullptr) {
 #pragma unroll
-            for (unsigned int t = 0; t < LR_MAX_TAIL; ++t)
-                if (tail[t].g == g && tail[t].f == f0 + f && tail[t].v != 0.0) {
-                    s += tail[t].v;
-                    q += tail[t].v * tail[t].v;
+            for (unsigned int k = 0; k < LR_MAX_TAIL; ++k)
+                if (tail[k].g == g && tail[k].f == f && tail[k].v != 0.0) {
+                    sm += tail[k].v;
+                    q += tail[k].v * tail[k].v;
                     n += 1u;
                 }
         }
         if (scale != nullptr) {
-            const double sc = (double) scale[fids[f0 + f]];
-            s *= sc;
+            const double sc = (double) scale[fids[f]];
+            sm *= sc;
             q *= sc * sc;
             if (sc == 0.0) n = 0u;
         }
-        const unsigned long long o = (unsigned long long) g * F + f0 + f;
-        out_sum[o] = s;
-        out_sumsq[o] = q;
-        out_cnt[o] = n;
-    }
-}
-
-__global__ void label_moments_kernel(const double *__restrict__ sum, const double *__restrict__ sumsq,
-                                     const unsigned int *__restrict__ gstart, unsigned int G, unsigned int F,
-                                     double *__restrict__ mean, double *__restrict__ var, unsigned int *__restrict__ group_rows) {
-    const unsigned long long total = (unsigned long long) G * F;
-    for (unsigned long long t = blockIdx.x * (unsigned long long) blockDim.x + threadIdx.x; t < total;
-         t += (unsigned long long) gridDim.x * blockDim.x) {
-        const unsigned int g = (unsigned int) (t / F);
-        const unsigned int n = gstart[g + 1u] - gstart[g];
+        out_sum[t] = sm;
+        out_sumsq[t] = q;
+        out_cnt[t] = n;
         if (mean != nullptr) {
-            if (n == 0u) {
+            const unsigned int rows_g = ge - gs;
+            if (rows_g == 0u) {
                 mean[t] = 0.0;
                 var[t] = 0.0;
             } else {
-                const double m = sum[t] / (double) n;
-                double v = sumsq[t] / (double) n - m * m;
-                if (v < 0.0) v = 0.0;
+                const double m = sm / (double) rows_g;
+                const double v = q / (double) rows_g - m * m;
                 mean[t] = m;
-                var[t] = v;
+                var[t] = v < 0.0 ? 0.0 : v;
             }
         }
     }
-    for (unsigned int g = blockIdx.x * blockDim.x + threadIdx.x; g < G; g += gridDim.x * blockDim.x)
-        group_rows[g] = gstart[g + 1u] - gstart[g];
+    if (blockIdx.x == 0u)
+        for (unsigned int g = threadIdx.x; g < G; g += blockDim.x) group_rows[g] = gstart[g + 1u] - gstart[g];
 }
 
 // ---------------------------------------------------------------------------
@@ -778,12 +888,6 @@ label_private *private_of(csamd_label_reduce_workspace *ws) {
         ws->private_workspace = std::calloc(1u, sizeof(label_private));
     }
     return (label_private *) ws->private_workspace;
-}
-
-unsigned int bits_for(unsigned int v) {
-    unsigned int b = 0u;
-    while (b < 32u && (v >> b) != 0u) ++b;
-    return b;
 }
 
 struct layout_in {
@@ -834,6 +938,8 @@ int run_label_reduce(const layout_in &L, unsigned int value_type, const csamd_la
                   q->group_count);
     CSAMD_REQUIRE(value_type_decodable(value_type), "label_reduce: unsupported value type %u", value_type);
     CSAMD_REQUIRE(L.rows == 0u || q->row_labels != nullptr, "label_reduce: row_labels is null");
+    CSAMD_REQUIRE(L.kind != 2 || (unsigned long long) L.rows * (L.bell.ell_cols + 8ull) < (1ull << 48),
+                  "label_reduce: blocked-ELL has too many slots");
     CSAMD_HIP(hipSetDevice(ws->device >= 0 ? ws->device : 0), "hipSetDevice label reduce");
     label_private *P = private_of(ws);
     CSAMD_REQUIRE(P != nullptr, "label_reduce: host allocation failed");
@@ -842,101 +948,66 @@ int run_label_reduce(const layout_in &L, unsigned int value_type, const csamd_la
     const unsigned int rows = L.rows, cols = L.cols;
     const unsigned int nwords = (cols + 31u) / 32u;
 
-    // 1. feature map
-    const size_t scan_n_bytes = exclusive_scan_scratch_bytes(nwords);
-    if (!ensure_buf(P->fpairs, (size_t) (nwords + 1u) * sizeof(uint2), "hipMalloc fpairs")) return 0;
-    if (!ensure_buf(P->fcounts, (size_t) (2u * (nwords + 1u)) * sizeof(unsigned int) + 256u + scan_n_bytes, "hipMalloc fcounts"))
-        return 0;
-    unsigned int *words = (unsigned int *) P->fcounts.p;
-    unsigned int *wprefix = words + (nwords + 1u);
-    void *scan_tmp = (char *) P->fcounts.p + align_up((size_t) (2u * (nwords + 1u)) * sizeof(unsigned int), 256u);
-    unsigned int *counts = nullptr;
-    {
-        // per-word popcounts live past the kept-feature ids (at most cols of them)
-        if (!ensure_buf(P->fids, (size_t) (nwords + 1u) * sizeof(unsigned int) + (size_t) cols * sizeof(unsigned int) + 64u,
-                        "hipMalloc fids"))
-            return 0;
-        counts = (unsigned int *) P->fids.p + cols + 16u;
-        if (nwords != 0u) {
-            feature_words_kernel<<<grid_for(nwords, 256u, 4096u), 256, 0, s>>>(q->feature_keep, cols, nwords, words, counts);
-            CSAMD_HIP(hipGetLastError(), "feature_words_kernel");
-        }
+    if (!P->events_ready) {
+        CSAMD_HIP(hipEventCreate(&P->ev0), "hipEventCreate");
+        CSAMD_HIP(hipEventCreate(&P->ev1), "hipEventCreate");
+        CSAMD_HIP(hipEventCreateWithFlags(&P->ev_meta, hipEventDisableTiming), "hipEventCreate");
+        P->events_ready = 1;
     }
-    if (!exclusive_scan_u32(counts, nwords, wprefix, nullptr, scan_tmp, scan_n_bytes, s)) return 0;
-    unsigned int *fids = (unsigned int *) P->fids.p;
-    if (nwords != 0u) {
-        feature_pairs_kernel<<<grid_for(nwords, 256u, 4096u), 256, 0, s>>>(words, wprefix, nwords, (uint2 *) P->fpairs.p, fids);
-        CSAMD_HIP(hipGetLastError(), "feature_pairs_kernel");
+    if (P->host == nullptr) {
+        void *h = nullptr;
+        CSAMD_HIP(hipHostMalloc(&h, 64u, hipHostMallocDefault), "hipHostMalloc label reduce");
+        std::memset(h, 0, 64u);
+        P->host = (volatile unsigned int *) h;
     }
-    unsigned int F = 0u;
-    CSAMD_HIP(hipMemcpyAsync(&F, wprefix + nwords, sizeof(unsigned int), hipMemcpyDeviceToHost, s), "D2H kept features");
 
-    // 2. rows sorted by label
-    const size_t sort_bytes = radix_sort_pairs_u32_scratch_bytes(rows);
-    if (!ensure_buf(P->keys, (size_t) 2u * rows * sizeof(unsigned int) + 64u, "hipMalloc keys")) return 0;
-    if (!ensure_buf(P->rows, (size_t) 2u * rows * sizeof(unsigned int) + 64u, "hipMalloc rows")) return 0;
-    if (!ensure_buf(P->sort_tmp, sort_bytes, "hipMalloc sort tmp")) return 0;
-    if (!ensure_buf(P->gstart, (size_t) (G + 1u) * sizeof(unsigned int), "hipMalloc gstart")) return 0;
-    unsigned int *keys_in = (unsigned int *) P->keys.p, *keys_out = keys_in + rows;
-    unsigned int *ids_in = (unsigned int *) P->rows.p, *ids_out = ids_in + rows;
-    unsigned int *gstart = (unsigned int *) P->gstart.p;
-    if (rows != 0u) {
-        row_keys_kernel<<<grid_for(rows, 256u, 8192u), 256, 0, s>>>(q->row_labels, q->row_keep, rows, G, keys_in, ids_in);
-        CSAMD_HIP(hipGetLastError(), "row_keys_kernel");
-        if (!radix_sort_pairs_u32(keys_in, keys_out, ids_in, ids_out, rows, bits_for(G), P->sort_tmp.p, P->sort_tmp.bytes, s))
-            return 0;
-    }
-    group_starts_kernel<<<grid_for(G + 1u, 256u, 64u), 256, 0, s>>>(keys_out, rows, G, gstart);
-    CSAMD_HIP(hipGetLastError(), "group_starts_kernel");
-    unsigned int K = 0u;
-    CSAMD_HIP(hipMemcpyAsync(&K, gstart + G, sizeof(unsigned int), hipMemcpyDeviceToHost, s), "D2H kept rows");
-    CSAMD_HIP(hipStreamSynchronize(s), "label reduce sync (sizes)");
-
-    // 3. launch shape
+    // Launch shape.  One 1024-thread workgroup per CU: its waves share the
+    // CU's row range through an LDS counter.  (Two workgroups per CU run
+    // unevenly: the older one gets issue priority and the younger one
+    // finishes late at half occupancy.)  CSAMD_LR_THREADS / CSAMD_LR_PER_CU /
+    // CSAMD_LR_SPLIT override the shape for tuning sweeps.
     const size_t fixed_lds = align_up((size_t) (nwords + 1u) * sizeof(uint2), 16u);
     const int max_lds = max_lds_per_block();
-    CSAMD_REQUIRE(fixed_lds + 64u * LR_ACC_BYTES <= (size_t) max_lds,
-                  "label_reduce: %u columns need %zu bytes of LDS for the feature map", cols, fixed_lds);
-    const unsigned int cu = (unsigned int) device_cu_count();
-    // One 1024-thread workgroup per CU: its waves share the CU's row range
-    // through an LDS counter.  (Two workgroups per CU run unevenly: the
-    // older one gets issue priority and the younger one finishes late at
-    // half occupancy.)  CSAMD_LR_THREADS / CSAMD_LR_PER_CU / CSAMD_LR_SPLIT
-    // override the shape for tuning sweeps.
     const size_t lds_avail = (size_t) max_lds - 16u;  // static LDS: the row counter
+    CSAMD_REQUIRE(fixed_lds + 64u * LR_ACC_BYTES <= lds_avail,
+                  "label_reduce: %u columns need %zu bytes of LDS for the feature map", cols, fixed_lds);
     auto env_u = [](const char *name, unsigned int dflt) {
         const char *e = std::getenv(name);
         return e != nullptr ? (unsigned int) std::atoi(e) : dflt;
     };
+    const unsigned int cu = (unsigned int) device_cu_count();
     unsigned int threads = env_u("CSAMD_LR_THREADS", 1024u);
     if (threads < 256u || threads > 1024u || threads % 64u != 0u) threads = 1024u;
     unsigned int per_cu = env_u("CSAMD_LR_PER_CU", 1u);
-    const size_t need_all = fixed_lds + (size_t) (F == 0u ? 1u : F) * LR_ACC_BYTES;
-    if (per_cu != 2u || need_all > lds_avail / 2u) per_cu = 1u;
-    unsigned int ftile = F == 0u ? 1u : F;
-    if (need_all > lds_avail) {
-        ftile = (unsigned int) ((lds_avail - fixed_lds) / LR_ACC_BYTES);
-        ftile &= ~63u;
-        if (ftile > F) ftile = F;
-        if (ftile == 0u) ftile = 64u;
-    }
     unsigned int split = env_u("CSAMD_LR_SPLIT", 1u);
     if (split < 1u || split > 16u) split = 1u;
-    const unsigned int W = cu * per_cu * split;
-    const size_t lds = fixed_lds + (size_t) ftile * LR_ACC_BYTES;
 
-    // 4. work bounds
-    const size_t work_scan_bytes = exclusive_scan_scratch_bytes(K);
-    const size_t work_bytes = align_up((size_t) (K + 1u) * sizeof(unsigned int), 256u);
-    const size_t prefix_bytes = align_up((size_t) (K + 1u) * sizeof(unsigned long long), 256u);
-    const size_t ranges_bytes = align_up((size_t) (K + 1u) * sizeof(uint2), 256u);
-    if (!ensure_buf(P->work, work_bytes + prefix_bytes + ranges_bytes + work_scan_bytes, "hipMalloc work")) return 0;
+    // Buffers sized by what the host knows (rows, cols, G); the kept counts
+    // stay on the device until the feature count is needed.
+    const unsigned int tiles = rows == 0u ? 1u : (rows + CSAMD_SCAN_TILE - 1u) / CSAMD_SCAN_TILE;
+    unsigned int sort_cap = 2u * cu;  // label table (G + 1) x workgroups stays small enough for one scanning workgroup
+    if (sort_cap > 262144u / (G + 1u)) sort_cap = 262144u / (G + 1u);
+    if (sort_cap < 8u) sort_cap = 8u;
+    const unsigned int sort_grid = grid_for(rows == 0u ? 1u : rows, LR_SORT_TILE, sort_cap);
+    const size_t table_n = (size_t) (G + 1u) * sort_grid;
+    if (!ensure_buf(P->fpairs, (size_t) (nwords + 1u) * sizeof(uint2), "hipMalloc fpairs")) return 0;
+    if (!ensure_buf(P->fids, (size_t) (cols + 1u) * sizeof(unsigned int), "hipMalloc fids")) return 0;
+    if (!ensure_buf(P->labels, (table_n + (G + 1u) + 1u) * sizeof(unsigned int), "hipMalloc label table")) return 0;
+    if (!ensure_buf(P->rows, (size_t) (rows + 1u) * sizeof(unsigned int), "hipMalloc sorted rows")) return 0;
+    const size_t work_bytes = align_up((size_t) (rows + 1u) * sizeof(unsigned int), 256u);
+    const size_t toff_bytes = align_up((size_t) (tiles + 1u) * sizeof(unsigned long long), 256u);
+    if (!ensure_buf(P->work, work_bytes + toff_bytes + (size_t) (rows + 1u) * sizeof(uint2), "hipMalloc work")) return 0;
+    if (!ensure_buf(P->tail, LR_MAX_TAIL * sizeof(tail_record), "hipMalloc tail records")) return 0;
+    unsigned int *table = (unsigned int *) P->labels.p;  // [(G + 1) x sort_grid]
+    unsigned int *gstart = table + table_n;               // [G + 1]
+    unsigned int *f_dev = gstart + (G + 1u);              // [1]
+    unsigned int *sorted = (unsigned int *) P->rows.p;
     unsigned int *work = (unsigned int *) P->work.p;
-    unsigned long long *wprefix64 = (unsigned long long *) ((char *) P->work.p + work_bytes);
-    uint2 *ranges = (uint2 *) ((char *) P->work.p + work_bytes + prefix_bytes);
-    void *wscan = (char *) P->work.p + work_bytes + prefix_bytes + ranges_bytes;
-    if (!ensure_buf(P->bounds, (size_t) (W + 1u) * sizeof(unsigned int), "hipMalloc bounds")) return 0;
-    unsigned int *bounds = (unsigned int *) P->bounds.p;
+    unsigned long long *toff = (unsigned long long *) ((char *) P->work.p + work_bytes);
+    uint2 *ranges = (uint2 *) ((char *) P->work.p + work_bytes + toff_bytes);
+    unsigned int *fids = (unsigned int *) P->fids.p;
+    uint2 *fpairs = (uint2 *) P->fpairs.p;
+
     // Vector loads need 16-byte aligned indices and unit-aligned values; the
     // vector kernel streams whole 4-entry units only (ranges clipped at
     // clip) and the entries past clip become tail records.
@@ -944,33 +1015,60 @@ int run_label_reduce(const layout_in &L, unsigned int value_type, const csamd_la
                          ((uintptr_t) L.val % (uintptr_t) (value_type == CSAMD_VALUE_F32 ? 16u
                                                            : value_type == CSAMD_VALUE_FP8_E4M3 ? 4u : 8u) == 0u);
     const unsigned long long clip = aligned ? (L.limit & ~3ull) : ~0ull;
-    const tail_record *tail = nullptr;
-    if (aligned && clip < L.limit) {
-        if (!ensure_buf(P->tail, LR_MAX_TAIL * sizeof(tail_record), "hipMalloc tail records")) return 0;
-        tail_record *t = (tail_record *) P->tail.p;
+    tail_record *tail = aligned && clip < L.limit ? (tail_record *) P->tail.p : nullptr;
+
+    // 1. feature map
+    feature_map_kernel<<<1, 1024, 0, s>>>(q->feature_keep, cols, nwords, fpairs, fids, f_dev, tail, G + 1u);
+    CSAMD_HIP(hipGetLastError(), "feature_map_kernel");
+
+    // 2. kept rows grouped by label (counting sort), with ranges and work
+    const size_t hist_lds = (size_t) (G + 1u) * sizeof(unsigned int);
+    label_count_kernel<<<sort_grid, LR_SORT_THREADS, hist_lds, s>>>(q->row_labels, q->row_keep, rows, G, table, work, rows + 1u);
+    CSAMD_HIP(hipGetLastError(), "label_count_kernel");
+    label_offsets_kernel<<<1, 1024, 0, s>>>(table, G, sort_grid, gstart);
+    CSAMD_HIP(hipGetLastError(), "label_offsets_kernel");
+    // gstart[G] (kept rows) and F sit next to each other: one copy to pinned
+    // memory, waited for at the end of the call (long done by then)
+    CSAMD_HIP(hipMemcpyAsync((void *) P->host, gstart + G, 2u * sizeof(unsigned int), hipMemcpyDeviceToHost, s), "D2H counts");
+    CSAMD_HIP(hipEventRecord(P->ev_meta, s), "hipEventRecord");
+    if (rows != 0u) {
         if (L.kind == 0)
-            tail_records_kernel<csr_geom><<<1, 64, 0, s>>>(L.csr, rows, L.idx, L.val, value_type, clip, L.limit, cols, q->row_labels,
-                                                           q->row_keep, G, (const uint2 *) P->fpairs.p, t);
-        else
-            tail_records_kernel<sell_geom><<<1, 64, 0, s>>>(L.sell, rows, L.idx, L.val, value_type, clip, L.limit, cols,
-                                                            q->row_labels, q->row_keep, G, (const uint2 *) P->fpairs.p, t);
-        CSAMD_HIP(hipGetLastError(), "tail_records_kernel");
-        tail = t;
-    }
-    if (K != 0u) {
-        if (L.kind == 0)
-            row_work_kernel<csr_geom><<<grid_for(K, 256u, 8192u), 256, 0, s>>>(L.csr, ids_out, K, clip, work, ranges);
+            label_scatter_kernel<csr_geom><<<sort_grid, LR_SORT_THREADS, 2u * hist_lds, s>>>(
+                L.csr, q->row_labels, q->row_keep, rows, G, table, clip, sorted, ranges, work, tail, L.idx, L.val, value_type,
+                cols, fpairs);
         else if (L.kind == 1)
-            row_work_kernel<sell_geom><<<grid_for(K, 256u, 8192u), 256, 0, s>>>(L.sell, ids_out, K, clip, work, ranges);
-        else blocked_work_kernel<<<grid_for(K, 256u, 8192u), 256, 0, s>>>(L.bell.ell_cols, K, work);
-        CSAMD_HIP(hipGetLastError(), "row_work_kernel");
+            label_scatter_kernel<sell_geom><<<sort_grid, LR_SORT_THREADS, 2u * hist_lds, s>>>(
+                L.sell, q->row_labels, q->row_keep, rows, G, table, clip, sorted, ranges, work, tail, L.idx, L.val, value_type,
+                cols, fpairs);
+        else
+            label_scatter_kernel<bell_geom><<<sort_grid, LR_SORT_THREADS, 2u * hist_lds, s>>>(
+                bell_geom{L.bell.ell_cols}, q->row_labels, q->row_keep, rows, G, table, clip, sorted, ranges, work, nullptr,
+                nullptr, nullptr, value_type, cols, fpairs);
+        CSAMD_HIP(hipGetLastError(), "label_scatter_kernel");
     }
-    if (!exclusive_scan_u32_to_u64(work, K, wprefix64, wscan, work_scan_bytes, s)) return 0;
-    work_bounds_kernel<<<grid_for(W + 1u, 256u, 64u), 256, 0, s>>>(wprefix64, K, W, bounds);
+
+    // 3. launch shape: the LDS feature tile is sized for all columns (at most
+    // F of them are kept; the device reads F), so no host wait is needed here
+    const unsigned int fmax = cols == 0u ? 1u : cols;
+    if (per_cu != 2u || fixed_lds + 64u * LR_ACC_BYTES > lds_avail / 2u) per_cu = 1u;
+    const size_t lds_budget = per_cu == 2u ? lds_avail / 2u : lds_avail;
+    unsigned int ftile = (unsigned int) ((lds_budget - fixed_lds) / LR_ACC_BYTES);
+    if (ftile >= fmax) ftile = fmax;
+    else ftile &= ~63u;
+    const unsigned int ntiles = (fmax + ftile - 1u) / ftile;
+    const unsigned int W = cu * per_cu * split;
+    const size_t lds = fixed_lds + (size_t) ftile * LR_ACC_BYTES;
+    if (!ensure_buf(P->bounds, (size_t) (W + 1u) * sizeof(unsigned int), "hipMalloc bounds")) return 0;
+    unsigned int *bounds = (unsigned int *) P->bounds.p;
+
+    // 4. partition bounds from the work weights (positions past the kept
+    // rows weigh 0)
+    if (!scan_tile_offsets_u32(work, rows, toff, s)) return 0;
+    work_bounds_kernel<<<tiles, 256, 0, s>>>(work, rows, toff, tiles, W, bounds);
     CSAMD_HIP(hipGetLastError(), "work_bounds_kernel");
 
-    // 5. outputs
-    const size_t GF = (size_t) G * F;
+    // 5. outputs [G x F] (room for F = cols) and partial slots
+    const size_t GF = (size_t) G * fmax;
     const size_t o_sum = 0, o_sumsq = align_up(o_sum + GF * 8u, 256u), o_mean = align_up(o_sumsq + GF * 8u, 256u),
                  o_var = align_up(o_mean + GF * 8u, 256u), o_cnt = align_up(o_var + GF * 8u, 256u),
                  o_grows = align_up(o_cnt + GF * 4u, 256u), o_end = align_up(o_grows + (size_t) G * 4u, 256u);
@@ -979,25 +1077,19 @@ int run_label_reduce(const layout_in &L, unsigned int value_type, const csamd_la
     double *out_sum = (double *) (ob + o_sum), *out_sumsq = (double *) (ob + o_sumsq);
     double *out_mean = (double *) (ob + o_mean), *out_var = (double *) (ob + o_var);
     unsigned int *out_cnt = (unsigned int *) (ob + o_cnt), *out_grows = (unsigned int *) (ob + o_grows);
-
-    const size_t slots = (size_t) W + G;
-    const size_t p_sum = 0, p_sumsq = align_up(slots * ftile * 8u, 256u), p_cnt = align_up(p_sumsq + slots * ftile * 8u, 256u),
-                 p_end = p_cnt + slots * ftile * 4u;
+    const size_t slots = (size_t) ntiles * ((size_t) W + G) * ftile;
+    const size_t p_sum = 0, p_sumsq = align_up(slots * 8u, 256u), p_cnt = align_up(p_sumsq + slots * 8u, 256u),
+                 p_end = p_cnt + slots * 4u;
     if (!ensure_buf(P->partial, p_end, "hipMalloc label partials")) return 0;
     char *pb = (char *) P->partial.p;
 
-    if (!P->events_ready) {
-        CSAMD_HIP(hipEventCreate(&P->ev0), "hipEventCreate");
-        CSAMD_HIP(hipEventCreate(&P->ev1), "hipEventCreate");
-        P->events_ready = 1;
-    }
-
-    // 6. main passes over feature tiles
+    // 6. one reduce launch for all feature tiles, then the finish
     reduce_args a;
-    a.sorted_rows = ids_out;
+    a.sorted_rows = sorted;
     a.gstart = gstart;
     a.bounds = bounds;
-    a.fpairs = (const uint2 *) P->fpairs.p;
+    a.fpairs = fpairs;
+    a.f_total = f_dev;
     a.psum = (double *) (pb + p_sum);
     a.psumsq = (double *) (pb + p_sumsq);
     a.pcnt = (unsigned int *) (pb + p_cnt);
@@ -1005,39 +1097,34 @@ int run_label_reduce(const layout_in &L, unsigned int value_type, const csamd_la
     a.cols = cols;
     a.nwords = nwords;
     a.ftile = ftile;
+    a.f0 = 0u;
+    a.f_count = 0u;
     P->last_valid = 0;
-    bool first = true;
-    for (unsigned int f0 = 0u; f0 < F || (F == 0u && first); f0 += ftile) {
-        a.f0 = f0;
-        a.f_count = F == 0u ? 0u : (F - f0 < ftile ? F - f0 : ftile);
-        if (K != 0u && a.f_count != 0u) {
-            if (first) CSAMD_HIP(hipEventRecord(P->ev0, s), "hipEventRecord");
-            int ok = 0;
-            switch (value_type) {
-                case CSAMD_VALUE_F16: ok = launch_rows<CSAMD_VALUE_F16>(L, W, threads, lds, s, a, aligned, ranges); break;
-                case CSAMD_VALUE_BF16: ok = launch_rows<CSAMD_VALUE_BF16>(L, W, threads, lds, s, a, aligned, ranges); break;
-                case CSAMD_VALUE_F32: ok = launch_rows<CSAMD_VALUE_F32>(L, W, threads, lds, s, a, aligned, ranges); break;
-                case CSAMD_VALUE_FP8_E4M3: ok = launch_rows<CSAMD_VALUE_FP8_E4M3>(L, W, threads, lds, s, a, aligned, ranges); break;
-            }
-            if (!ok) return 0;
-            if (first) {
-                CSAMD_HIP(hipEventRecord(P->ev1, s), "hipEventRecord");
-                P->last_valid = 1;
-            }
+    if (rows != 0u && cols != 0u) {
+        const bool timed = P->timing != 0;
+        if (timed) CSAMD_HIP(hipEventRecord(P->ev0, s), "hipEventRecord");
+        int ok = 0;
+        switch (value_type) {
+            case CSAMD_VALUE_F16: ok = launch_rows<CSAMD_VALUE_F16>(L, W, threads, lds, s, a, aligned, ranges); break;
+            case CSAMD_VALUE_BF16: ok = launch_rows<CSAMD_VALUE_BF16>(L, W, threads, lds, s, a, aligned, ranges); break;
+            case CSAMD_VALUE_F32: ok = launch_rows<CSAMD_VALUE_F32>(L, W, threads, lds, s, a, aligned, ranges); break;
+            case CSAMD_VALUE_FP8_E4M3: ok = launch_rows<CSAMD_VALUE_FP8_E4M3>(L, W, threads, lds, s, a, aligned, ranges); break;
         }
-        if (a.f_count != 0u) {
-            label_combine_kernel<<<grid_for((unsigned long long) G * a.f_count, 256u, 8192u), 256, 0, s>>>(
-                a.psum, a.psumsq, a.pcnt, gstart, bounds, W, G, ftile, f0, a.f_count, F, q->feature_scale, fids, tail, out_sum,
-                out_sumsq, out_cnt);
-            CSAMD_HIP(hipGetLastError(), "label_combine_kernel");
+        if (!ok) return 0;
+        if (timed) {
+            CSAMD_HIP(hipEventRecord(P->ev1, s), "hipEventRecord");
+            P->last_valid = 1;
         }
-        first = false;
-        if (F == 0u) break;
     }
-    label_moments_kernel<<<grid_for(GF == 0u ? (unsigned long long) G : GF, 256u, 8192u), 256, 0, s>>>(
-        out_sum, out_sumsq, gstart, G, F, q->compute_moments ? out_mean : nullptr, out_var, out_grows);
-    CSAMD_HIP(hipGetLastError(), "label_moments_kernel");
+    const size_t meta_lds = (size_t) (G + 1u + W + 1u) * sizeof(unsigned int);
+    label_finish_kernel<<<grid_for((unsigned long long) G * fmax, 256u, 4u * cu), 256, meta_lds, s>>>(
+        a.psum, a.psumsq, a.pcnt, gstart, bounds, W, G, ftile, f_dev, q->feature_scale, fids, rows != 0u ? tail : nullptr,
+        out_sum, out_sumsq, out_cnt, q->compute_moments ? out_mean : nullptr, out_var, out_grows);
+    CSAMD_HIP(hipGetLastError(), "label_finish_kernel");
 
+    // kept rows and features, copied right after the label offsets
+    CSAMD_HIP(hipEventSynchronize(P->ev_meta), "label reduce sync (counts)");
+    const unsigned int K = P->host[0], F = P->host[1];
     out->group_count = G;
     out->kept_features = F;
     out->kept_rows = K;
@@ -1070,13 +1157,17 @@ void csamd_label_reduce_clear(csamd_label_reduce_workspace *ws) {
     if (ws->device >= 0) (void) hipSetDevice(ws->device);
     label_private *P = (label_private *) ws->private_workspace;
     if (P != nullptr) {
-        dev_buf *bufs[] = {&P->fpairs, &P->fcounts, &P->fids, &P->keys, &P->rows, &P->sort_tmp, &P->gstart,
-                           &P->work, &P->bounds, &P->partial, &P->out, &P->tail};
+        dev_buf *bufs[] = {&P->fpairs, &P->fids, &P->labels, &P->rows, &P->work, &P->bounds, &P->partial, &P->tail, &P->out};
         for (dev_buf *b : bufs)
             if (b->p != nullptr) (void) hipFree(b->p);
+        if (P->host != nullptr) {
+            (void) hipStreamSynchronize((hipStream_t) ws->stream);  // kernels may still write it
+            (void) hipHostFree((void *) P->host);
+        }
         if (P->events_ready) {
             (void) hipEventDestroy(P->ev0);
             (void) hipEventDestroy(P->ev1);
+            (void) hipEventDestroy(P->ev_meta);
         }
         std::free(P);
     }
@@ -1155,6 +1246,15 @@ int csamd_label_reduce_blocked_ell(const csamd_blocked_ell_view *src, const csam
     L.cols = src->cols;
     L.bell = *src;
     return run_label_reduce(L, query->value_type, query, ws, out);
+}
+
+int csamd_label_reduce_set_timing(csamd_label_reduce_workspace *ws, int enable) {
+    CSAMD_REQUIRE(ws != nullptr, "label_reduce set_timing: null workspace");
+    label_private *P = private_of(ws);
+    CSAMD_REQUIRE(P != nullptr, "label_reduce set_timing: host allocation failed");
+    P->timing = enable != 0;
+    if (!P->timing) P->last_valid = 0;
+    return 1;
 }
 
 float csamd_label_reduce_last_kernel_ms(const csamd_label_reduce_workspace *ws) {

@@ -67,16 +67,24 @@ namespace {
 constexpr unsigned int SCAN_THREADS = 256u;
 constexpr unsigned int SCAN_ITEMS = 16u;
 constexpr unsigned int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
+static_assert(SCAN_TILE == CSAMD_SCAN_TILE, "scan tile size");
+
+// Each thread owns SCAN_ITEMS consecutive items of the tile.
+__device__ __forceinline__ void load_items(const unsigned int *__restrict__ in, unsigned long long n, unsigned long long first,
+                                           unsigned int v[SCAN_ITEMS]) {
+#pragma unroll
+    for (unsigned int i = 0; i < SCAN_ITEMS; ++i) v[i] = first + i < n ? in[first + i] : 0u;
+}
 
 __global__ void __launch_bounds__(SCAN_THREADS) scan_tile_sums_kernel(const unsigned int *__restrict__ in, unsigned int n,
                                                                       unsigned long long *__restrict__ tile_sums) {
     __shared__ unsigned long long red[SCAN_THREADS / 64u];
-    const unsigned long long base = (unsigned long long) blockIdx.x * SCAN_TILE;
+    const unsigned long long first = (unsigned long long) blockIdx.x * SCAN_TILE + (unsigned long long) threadIdx.x * SCAN_ITEMS;
+    unsigned int v[SCAN_ITEMS];
+    load_items(in, n, first, v);
     unsigned long long s = 0ull;
-    for (unsigned int i = 0; i < SCAN_ITEMS; ++i) {
-        const unsigned long long idx = base + (unsigned long long) i * SCAN_THREADS + threadIdx.x;
-        if (idx < n) s += in[idx];
-    }
+#pragma unroll
+    for (unsigned int i = 0; i < SCAN_ITEMS; ++i) s += v[i];
     s = wave_sum_u64(s);
     if (lane_id() == 0u) red[threadIdx.x >> 6] = s;
     __syncthreads();
@@ -90,19 +98,14 @@ __global__ void __launch_bounds__(SCAN_THREADS) scan_tile_sums_kernel(const unsi
 // Single block: exclusive scan of tile sums in place, total into tile_sums[count].
 __global__ void __launch_bounds__(1024) scan_tile_offsets_kernel(unsigned long long *__restrict__ tile_sums, unsigned int count) {
     __shared__ unsigned long long smem[16];
-    __shared__ unsigned long long carry;
-    if (threadIdx.x == 0) carry = 0ull;
-    __syncthreads();
+    unsigned long long carry = 0ull;  // same value in every thread
     for (unsigned int start = 0; start < count; start += blockDim.x) {
         const unsigned int i = start + threadIdx.x;
         const unsigned long long v = i < count ? tile_sums[i] : 0ull;
         unsigned long long total = 0ull;
         const unsigned long long ex = block_exclusive_scan_u64(v, smem, &total);
-        const unsigned long long c = carry;
-        if (i < count) tile_sums[i] = c + ex;
-        __syncthreads();
-        if (threadIdx.x == 0) carry = c + total;
-        __syncthreads();
+        if (i < count) tile_sums[i] = carry + ex;
+        carry += total;
     }
     if (threadIdx.x == 0) tile_sums[count] = carry;
 }
@@ -113,20 +116,24 @@ __global__ void __launch_bounds__(SCAN_THREADS) scan_apply_kernel(const unsigned
                                                                   unsigned int tile_count, OutT *__restrict__ out,
                                                                   unsigned int *__restrict__ overflow, int write_total) {
     __shared__ unsigned long long smem[SCAN_THREADS / 64u];
-    const unsigned long long base = (unsigned long long) blockIdx.x * SCAN_TILE;
-    unsigned long long carry = tile_offsets[blockIdx.x];
+    const unsigned long long first = (unsigned long long) blockIdx.x * SCAN_TILE + (unsigned long long) threadIdx.x * SCAN_ITEMS;
+    unsigned int v[SCAN_ITEMS];
+    load_items(in, n, first, v);
+    unsigned long long own = 0ull;
+#pragma unroll
+    for (unsigned int i = 0; i < SCAN_ITEMS; ++i) own += v[i];
+    unsigned long long total = 0ull;
+    unsigned long long o = tile_offsets[blockIdx.x] + block_exclusive_scan_u64(own, smem, &total);
+    bool over = false;
+#pragma unroll
     for (unsigned int i = 0; i < SCAN_ITEMS; ++i) {
-        const unsigned long long idx = base + (unsigned long long) i * SCAN_THREADS + threadIdx.x;
-        const unsigned long long v = idx < n ? in[idx] : 0ull;
-        unsigned long long total = 0ull;
-        const unsigned long long ex = block_exclusive_scan_u64(v, smem, &total);
-        const unsigned long long o = carry + ex;
-        if (idx < n) {
-            out[idx] = (OutT) o;
-            if (overflow != nullptr && o > 0xffffffffull) *overflow = 1u;
+        if (first + i < n) {
+            out[first + i] = (OutT) o;
+            over |= o > 0xffffffffull;
         }
-        carry += total;
+        o += v[i];
     }
+    if (over && overflow != nullptr) *overflow = 1u;
     if (write_total && blockIdx.x == tile_count - 1u && threadIdx.x == 0) {
         const unsigned long long t = tile_offsets[tile_count];
         out[n] = (OutT) t;
@@ -153,6 +160,15 @@ static int exclusive_scan_impl(const unsigned int *d_in, unsigned int n, OutT *d
     CSAMD_HIP(hipGetLastError(), "scan_tile_offsets_kernel");
     scan_apply_kernel<OutT><<<tiles, SCAN_THREADS, 0, stream>>>(d_in, n, tile_sums, tiles, d_out, d_overflow, write_total ? 1 : 0);
     CSAMD_HIP(hipGetLastError(), "scan_apply_kernel");
+    return 1;
+}
+
+int scan_tile_offsets_u32(const unsigned int *d_in, unsigned int n, unsigned long long *d_tile_offsets, hipStream_t stream) {
+    const unsigned int tiles = n == 0u ? 1u : (n + SCAN_TILE - 1u) / SCAN_TILE;
+    scan_tile_sums_kernel<<<tiles, SCAN_THREADS, 0, stream>>>(d_in, n, d_tile_offsets);
+    CSAMD_HIP(hipGetLastError(), "scan_tile_sums_kernel");
+    scan_tile_offsets_kernel<<<1, 1024, 0, stream>>>(d_tile_offsets, tiles);
+    CSAMD_HIP(hipGetLastError(), "scan_tile_offsets_kernel");
     return 1;
 }
 

@@ -295,6 +295,10 @@ class LabelReduce:
         check(rc, "label_reduce")
         return res
 
+    def set_timing(self, enable: bool) -> None:
+        """Record HIP events around the main reduce kernel on later calls."""
+        check(load().csamd_label_reduce_set_timing(C.byref(self.ws), 1 if enable else 0), "label reduce timing")
+
     def last_kernel_ms(self) -> float:
         return float(load().csamd_label_reduce_last_kernel_ms(C.byref(self.ws)))
 

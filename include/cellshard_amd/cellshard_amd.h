@@ -393,8 +393,12 @@ int csamd_label_reduce_sliced_ell(const csamd_sliced_ell_view *src, const csamd_
                                   csamd_label_reduce_workspace *workspace, csamd_label_reduce_result *out);
 int csamd_label_reduce_blocked_ell(const csamd_blocked_ell_view *src, const csamd_label_reduce_query *query,
                                    csamd_label_reduce_workspace *workspace, csamd_label_reduce_result *out);
-/* Mean of the last call's main reduce kernel duration on the workspace
- * stream, in milliseconds (HIP events around that launch).  0 if none. */
+/* Kernel timing (project extension): with enable != 0, later calls record
+ * HIP events around the main reduce kernel (first feature tile) on the
+ * workspace stream; off by default, since each event record costs a few
+ * microseconds of stream time.  csamd_label_reduce_last_kernel_ms returns
+ * the last timed call's kernel duration in milliseconds, 0 if none. */
+int csamd_label_reduce_set_timing(csamd_label_reduce_workspace *workspace, int enable);
 float csamd_label_reduce_last_kernel_ms(const csamd_label_reduce_workspace *workspace);
 
 /* ---------------------------------------------------------------------------
