@@ -83,49 +83,6 @@ __device__ __forceinline__ float decode_any(const void *val, unsigned long long 
 }
 
 // ---------------------------------------------------------------------------
-// Feature map (one workgroup): per 32-column word {kept mask, kept features
-// before the word}, kept feature ids in column order, F.  Also clears the
-// tail records (label G + 1: contributes nothing) that the scatter fills
-// for kept rows.
-// ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(1024) feature_map_kernel(const unsigned char *__restrict__ feature_keep, unsigned int cols,
-                                                           unsigned int nwords, uint2 *__restrict__ pairs,
-                                                           unsigned int *__restrict__ fids, unsigned int *__restrict__ f_total,
-                                                           tail_record *__restrict__ tail, unsigned int nlabels) {
-    __shared__ unsigned int smem[16];
-    if (tail != nullptr && threadIdx.x < LR_MAX_TAIL) tail[threadIdx.x] = tail_record{nlabels, CSAMD_INVALID_COL, 0.0};
-    unsigned int carry = 0u;  // same value in every thread
-    for (unsigned int start = 0; start < nwords; start += blockDim.x) {
-        const unsigned int w = start + threadIdx.x;
-        unsigned int m = 0u;
-        if (w < nwords) {
-            const unsigned int c0 = w * 32u;
-            if (feature_keep == nullptr) {
-                m = cols - c0 >= 32u ? 0xffffffffu : (1u << (cols - c0)) - 1u;
-            } else {
-                unsigned char v[32];  // all loads issued before any is used
-#pragma unroll
-                for (unsigned int b = 0; b < 32u; ++b) v[b] = feature_keep[min(c0 + b, cols - 1u)];
-#pragma unroll
-                for (unsigned int b = 0; b < 32u; ++b) m |= (c0 + b < cols && v[b] != 0u ? 1u : 0u) << b;
-            }
-        }
-        unsigned int total = 0u;
-        const unsigned int ex = carry + block_exclusive_scan_u32((unsigned int) __popc(m), smem, &total);
-        if (w < nwords) {
-            pairs[w] = make_uint2(m, ex);
-            unsigned int mm = m, k = 0u;
-            while (mm != 0u) {
-                fids[ex + k++] = w * 32u + (unsigned int) __builtin_ctz(mm);
-                mm &= mm - 1u;
-            }
-        }
-        carry += total;
-    }
-    if (threadIdx.x == 0) *f_total = carry;
-}
-
-// ---------------------------------------------------------------------------
 // Rows grouped by label: a counting sort on key = label (kept rows) or G
 // (dropped).  Workgroup b histograms its rows (the tiles b, b + grid, ...)
 // in LDS and stores the counts to hist[g * grid + b]; one workgroup scans
@@ -143,14 +100,44 @@ __device__ __forceinline__ unsigned int row_key(const unsigned int *__restrict__
     return ((row_keep == nullptr || row_keep[r] != 0u) && l < G) ? l : G;
 }
 
-// Also clears work[0..work_n): positions past the kept rows keep weight 0
-// for the partition scan.
+// Also: clears work[0..work_n) (positions past the kept rows keep weight 0
+// for the partition scan); builds the kept-feature mask of each 32-column
+// word; clears the tail records (label G + 1: contributes nothing) that the
+// scatter fills for kept rows.
 __global__ void __launch_bounds__(LR_SORT_THREADS) label_count_kernel(const unsigned int *__restrict__ labels,
                                                                       const unsigned char *__restrict__ row_keep, unsigned int rows,
                                                                       unsigned int G, unsigned int *__restrict__ hist_out,
-                                                                      unsigned int *__restrict__ work, unsigned int work_n) {
+                                                                      unsigned int *__restrict__ work, unsigned int work_n,
+                                                                      const unsigned char *__restrict__ feature_keep, unsigned int cols,
+                                                                      unsigned int nwords, unsigned int *__restrict__ word_mask,
+                                                                      tail_record *__restrict__ tail) {
     extern __shared__ unsigned int hist[];  // [G + 1]
     for (unsigned int g = threadIdx.x; g <= G; g += blockDim.x) hist[g] = 0u;
+    if (tail != nullptr && blockIdx.x == 0u && threadIdx.x < LR_MAX_TAIL)
+        tail[threadIdx.x] = tail_record{G + 1u, CSAMD_INVALID_COL, 0.0};
+    // Feature words, 64 per wave: pass j reads 64 consecutive columns (one
+    // coalesced load), a ballot turns them into words 2j and 2j + 1, kept
+    // by lanes 2j and 2j + 1.
+    const unsigned int lane = lane_id();
+    const unsigned int nwave = (gridDim.x * blockDim.x) >> 6;
+    for (unsigned int gw = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; gw * 64u < nwords; gw += nwave) {
+        const unsigned int w0 = gw * 64u;
+        unsigned char v[32];
+#pragma unroll
+        for (unsigned int j = 0; j < 32u; ++j) {
+            const unsigned int c = (w0 + 2u * j) * 32u + lane;
+            v[j] = feature_keep == nullptr ? (unsigned char) 1u : feature_keep[min(c, cols - 1u)];
+        }
+        unsigned int m = 0u;
+#pragma unroll
+        for (unsigned int j = 0; j < 32u; ++j) {
+            const unsigned int c = (w0 + 2u * j) * 32u + lane;
+            const unsigned long long b = __ballot(c < cols && v[j] != 0u);
+            if (lane == 2u * j) m = (unsigned int) b;
+            if (lane == 2u * j + 1u) m = (unsigned int) (b >> 32);
+        }
+        if (w0 + lane < nwords) word_mask[w0 + lane] = m;
+    }
     __syncthreads();
     for (unsigned int t0 = blockIdx.x * LR_SORT_TILE; t0 < rows; t0 += gridDim.x * LR_SORT_TILE) {
         unsigned int key[LR_SORT_ITEMS];
@@ -170,12 +157,34 @@ __global__ void __launch_bounds__(LR_SORT_THREADS) label_count_kernel(const unsi
 }
 
 // One workgroup: exclusive scan of the label-major table hist[(G + 1) x nb]
-// in place; gstart[g] = hist[g * nb] (gstart[G] = kept rows).
+// in place; gstart[g] = hist[g * nb] (gstart[G] = kept rows).  Then the
+// feature map: per word {kept mask, kept features before the word}, kept
+// feature ids in column order, F.
 constexpr unsigned int LR_OFF_ITEMS = 8u;
 
 __global__ void __launch_bounds__(1024) label_offsets_kernel(unsigned int *__restrict__ hist, unsigned int G, unsigned int nb,
-                                                             unsigned int *__restrict__ gstart) {
+                                                             unsigned int *__restrict__ gstart,
+                                                             const unsigned int *__restrict__ word_mask, unsigned int nwords,
+                                                             uint2 *__restrict__ pairs, unsigned int *__restrict__ fids,
+                                                             unsigned int *__restrict__ f_total) {
     __shared__ unsigned int smem[16];
+    unsigned int fcarry = 0u;  // same value in every thread
+    for (unsigned int start = 0; start < nwords; start += blockDim.x) {
+        const unsigned int w = start + threadIdx.x;
+        const unsigned int m = w < nwords ? word_mask[w] : 0u;
+        unsigned int total = 0u;
+        const unsigned int ex = fcarry + block_exclusive_scan_u32((unsigned int) __popc(m), smem, &total);
+        if (w < nwords) {
+            pairs[w] = make_uint2(m, ex);
+            unsigned int mm = m, k = 0u;
+            while (mm != 0u) {
+                fids[ex + k++] = w * 32u + (unsigned int) __builtin_ctz(mm);
+                mm &= mm - 1u;
+            }
+        }
+        fcarry += total;
+    }
+    if (threadIdx.x == 0) *f_total = fcarry;
     const unsigned long long n = (unsigned long long) (G + 1u) * nb;
     unsigned int carry = 0u;  // same value in every thread
     for (unsigned long long start = 0; start < n; start += (unsigned long long) blockDim.x * LR_OFF_ITEMS) {
@@ -991,7 +1000,7 @@ int run_label_reduce(const layout_in &L, unsigned int value_type, const csamd_la
     const unsigned int sort_grid = grid_for(rows == 0u ? 1u : rows, LR_SORT_TILE, sort_cap);
     const size_t table_n = (size_t) (G + 1u) * sort_grid;
     if (!ensure_buf(P->fpairs, (size_t) (nwords + 1u) * sizeof(uint2), "hipMalloc fpairs")) return 0;
-    if (!ensure_buf(P->fids, (size_t) (cols + 1u) * sizeof(unsigned int), "hipMalloc fids")) return 0;
+    if (!ensure_buf(P->fids, ((size_t) cols + 16u + nwords + 1u) * sizeof(unsigned int), "hipMalloc fids")) return 0;
     if (!ensure_buf(P->labels, (table_n + (G + 1u) + 1u) * sizeof(unsigned int), "hipMalloc label table")) return 0;
     if (!ensure_buf(P->rows, (size_t) (rows + 1u) * sizeof(unsigned int), "hipMalloc sorted rows")) return 0;
     const size_t work_bytes = align_up((size_t) (rows + 1u) * sizeof(unsigned int), 256u);
@@ -1017,15 +1026,13 @@ int run_label_reduce(const layout_in &L, unsigned int value_type, const csamd_la
     const unsigned long long clip = aligned ? (L.limit & ~3ull) : ~0ull;
     tail_record *tail = aligned && clip < L.limit ? (tail_record *) P->tail.p : nullptr;
 
-    // 1. feature map
-    feature_map_kernel<<<1, 1024, 0, s>>>(q->feature_keep, cols, nwords, fpairs, fids, f_dev, tail, G + 1u);
-    CSAMD_HIP(hipGetLastError(), "feature_map_kernel");
-
-    // 2. kept rows grouped by label (counting sort), with ranges and work
+    // 1. label histograms and feature words; 2. label offsets and feature map
     const size_t hist_lds = (size_t) (G + 1u) * sizeof(unsigned int);
-    label_count_kernel<<<sort_grid, LR_SORT_THREADS, hist_lds, s>>>(q->row_labels, q->row_keep, rows, G, table, work, rows + 1u);
+    unsigned int *word_mask = (unsigned int *) P->fids.p + (cols + 16u);  // past the kept-feature ids
+    label_count_kernel<<<sort_grid, LR_SORT_THREADS, hist_lds, s>>>(q->row_labels, q->row_keep, rows, G, table, work, rows + 1u,
+                                                                    q->feature_keep, cols, nwords, word_mask, tail);
     CSAMD_HIP(hipGetLastError(), "label_count_kernel");
-    label_offsets_kernel<<<1, 1024, 0, s>>>(table, G, sort_grid, gstart);
+    label_offsets_kernel<<<1, 1024, 0, s>>>(table, G, sort_grid, gstart, word_mask, nwords, fpairs, fids, f_dev);
     CSAMD_HIP(hipGetLastError(), "label_offsets_kernel");
     // gstart[G] (kept rows) and F sit next to each other: one copy to pinned
     // memory, waited for at the end of the call (long done by then)
