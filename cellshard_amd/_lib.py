@@ -66,6 +66,17 @@ PROTOTYPES = {
     "csamd_label_reduce_blocked_ell": (i32, [P(A.blocked_ell_view), P(A.label_reduce_query), P(A.label_reduce_workspace), P(A.label_reduce_result)]),
     "csamd_label_reduce_last_kernel_ms": (f32, [P(A.label_reduce_workspace)]),
     "csamd_label_reduce_set_timing": (i32, [P(A.label_reduce_workspace), i32]),
+    "csamd_serve_init": (None, [P(A.serve_workspace)]),
+    "csamd_serve_clear": (None, [P(A.serve_workspace)]),
+    "csamd_serve_setup": (i32, [P(A.serve_workspace), P(A.serve_config)]),
+    "csamd_serve_add_shard": (i32, [P(A.serve_workspace), P(A.compressed_view), u32, P(u32)]),
+    "csamd_serve_submit": (i32, [P(A.serve_workspace), P(A.serve_query), P(u64)]),
+    "csamd_serve_pump": (i32, [P(A.serve_workspace), P(u32)]),
+    "csamd_serve_drain": (i32, [P(A.serve_workspace)]),
+    "csamd_serve_get_result": (i32, [P(A.serve_workspace), u64, P(A.serve_result)]),
+    "csamd_serve_release": (i32, [P(A.serve_workspace), u64]),
+    "csamd_serve_replay": (i32, [P(A.serve_workspace), P(A.serve_query), P(C.c_double), u32, P(u64), P(C.c_double), P(C.c_double)]),
+    "csamd_serve_materialize": (i32, [u32, u32, u32, u32, u32, vp, vp, vp]),
     # convert
     "csamd_convert_init": (None, [P(A.convert_workspace)]),
     "csamd_convert_clear": (None, [P(A.convert_workspace)]),

@@ -196,6 +196,22 @@ allocation_cache = _aligned16("allocation_cache", [
     ("cache_count", C.c_ulong), ("cache_capacity", C.c_ulong), ("cache", C.POINTER(cached_allocation)),
 ])
 
+serve_query = _aligned16("serve_query", [
+    ("shard", c_u32), ("op", c_u32), ("group_count", c_u32), ("feature_count", c_u32),
+    ("features", c_void_p), ("row_labels", c_void_p), ("row_keep", c_void_p),
+    ("label_seed", c_u32), ("mask_seed", c_u32), ("keep_thr24", c_u32), ("reserved", c_u32),
+])
+serve_result = _aligned16("serve_result", [
+    ("status", C.c_int), ("op", c_u32), ("group_count", c_u32), ("feature_count", c_u32),
+    ("out", c_void_p), ("latency_us", C.c_double),
+])
+serve_config = _aligned16("serve_config", [
+    ("device", C.c_int), ("max_slots", c_u32), ("max_batch_queries", c_u32), ("max_batches_in_flight", c_u32),
+    ("slot_bytes", C.c_uint64), ("budget_bytes", C.c_uint64),
+])
+serve_workspace = _aligned16("serve_workspace", [("device", C.c_int), ("state", c_void_p)])
+SERVE_SUM, SERVE_SUMSQ, SERVE_NNZ = 0, 1, 2
+
 # Structs whose layout tests/test_abi.py compares against the C header.
 ALL_STRUCTS = {
     "csamd_compressed_view": compressed_view,
@@ -229,4 +245,8 @@ ALL_STRUCTS = {
     "csamd_partition_record": partition_record,
     "csamd_cached_allocation": cached_allocation,
     "csamd_allocation_cache": allocation_cache,
+    "csamd_serve_query": serve_query,
+    "csamd_serve_result": serve_result,
+    "csamd_serve_config": serve_config,
+    "csamd_serve_workspace": serve_workspace,
 }

@@ -31,6 +31,7 @@ HIP_SOURCES = [
     "synth.hip",
     "staging.hip",
     "host_convert.hip",
+    "serve.hip",
 ]
 
 HIPCC_FLAGS = [

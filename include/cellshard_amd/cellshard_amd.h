@@ -402,6 +402,104 @@ int csamd_label_reduce_set_timing(csamd_label_reduce_workspace *workspace, int e
 float csamd_label_reduce_last_kernel_ms(const csamd_label_reduce_workspace *workspace);
 
 /* ---------------------------------------------------------------------------
+ * Serving loop (project extension; the reference runtime is one call per
+ * query, mask_groups.cuh:159-224, and has no request loop).
+ *
+ * Shards stay resident in HBM as column-major (CSC) device views registered
+ * with the server; their bytes count against the server's device-memory
+ * budget, and registration fails past it.  A query names a shard, a feature
+ * subset (ascending distinct columns), a group count G and one op:
+ *   out[j][g] = sum over kept rows r with label(r) = g of
+ *               v(r, features[j])      (CSAMD_SERVE_SUM,   f64)
+ *               v(r, features[j])^2    (CSAMD_SERVE_SUMSQ, f64)
+ *               [v(r, features[j]) != 0] (CSAMD_SERVE_NNZ,  u32)
+ * over stored nonzero entries; output is feature-major [k x G].  Row labels
+ * and the row mask come from device arrays [rows] when given, else from the
+ * seeds: label(r) = (lowbias32(label_seed ^ (r * 0x9E3779B9)) * G) >> 32 and
+ * keep(r) = (lowbias32(mask_seed ^ (r * 0x85EBCA6B)) >> 8) < keep_thr24
+ * (keep_thr24 = fraction * 2^24, 2^24 keeps every row).
+ *
+ * Queries are admitted into output slots (slot_bytes each, counted against
+ * the budget) as slots free up; admitted queries are launched together, one
+ * workgroup per (query, feature) item taken from a shared counter, with up
+ * to max_batches_in_flight batches running on separate streams, so queries
+ * that arrive while others run overlap with them on the device.  Each
+ * query's result does not depend on what it was batched with (per-(query,
+ * feature) LDS accumulation in entry order; exact for integer values).
+ * ------------------------------------------------------------------------- */
+#define CSAMD_SERVE_SUM 0u
+#define CSAMD_SERVE_SUMSQ 1u
+#define CSAMD_SERVE_NNZ 2u
+
+typedef struct CSAMD_ALIGN16 csamd_serve_query {
+    unsigned int shard;                /* id from csamd_serve_add_shard */
+    unsigned int op;                   /* CSAMD_SERVE_* */
+    unsigned int group_count;          /* G, 1..CSAMD_SERVE_MAX_GROUPS */
+    unsigned int feature_count;        /* k >= 1 */
+    const unsigned int *features;      /* device [k], ascending, < cols */
+    const unsigned int *row_labels;    /* device [rows] or NULL (seeded labels) */
+    const unsigned char *row_keep;     /* device [rows] or NULL (seeded mask) */
+    unsigned int label_seed;
+    unsigned int mask_seed;
+    unsigned int keep_thr24;           /* seeded mask threshold, 0..2^24 */
+    unsigned int reserved;
+} csamd_serve_query;
+#define CSAMD_SERVE_MAX_GROUPS 4096u
+
+typedef struct CSAMD_ALIGN16 csamd_serve_result {
+    int status;                        /* 0 queued, 1 running, 2 done, 3 released, -1 unknown ticket */
+    unsigned int op;
+    unsigned int group_count;
+    unsigned int feature_count;
+    const void *out;                   /* device [k x G] (f64, or u32 for NNZ); valid until release */
+    double latency_us;                 /* submit (or replay arrival) to completion seen by the host */
+} csamd_serve_result;
+
+typedef struct CSAMD_ALIGN16 csamd_serve_config {
+    int device;
+    unsigned int max_slots;            /* output slots (queries admitted at once) */
+    unsigned int max_batch_queries;    /* queries per launch */
+    unsigned int max_batches_in_flight;/* 1..4 launches running at once (one stream each) */
+    unsigned long long slot_bytes;     /* output bytes per slot (k x G x 8 must fit) */
+    unsigned long long budget_bytes;   /* device-memory budget: shards + slots */
+} csamd_serve_config;
+
+typedef struct CSAMD_ALIGN16 csamd_serve_workspace {
+    int device;
+    void *state;
+} csamd_serve_workspace;
+
+void csamd_serve_init(csamd_serve_workspace *workspace);
+void csamd_serve_clear(csamd_serve_workspace *workspace);
+int csamd_serve_setup(csamd_serve_workspace *workspace, const csamd_serve_config *config);
+/* Register a resident CSC shard (axis CSAMD_COMPRESSED_BY_COL, values of
+ * value_type; the caller keeps the device arrays alive).  Fails when it
+ * would exceed the budget. */
+int csamd_serve_add_shard(csamd_serve_workspace *workspace, const csamd_compressed_view *csc, unsigned int value_type,
+                          unsigned int *shard_id);
+/* Queue a query; *ticket identifies it.  Does not launch. */
+int csamd_serve_submit(csamd_serve_workspace *workspace, const csamd_serve_query *query, unsigned long long *ticket);
+/* Retire finished launches and launch queued queries that fit; never
+ * blocks.  *completed (may be NULL) = queries finished by this call. */
+int csamd_serve_pump(csamd_serve_workspace *workspace, unsigned int *completed);
+/* Block until every submitted query is done. */
+int csamd_serve_drain(csamd_serve_workspace *workspace);
+int csamd_serve_get_result(csamd_serve_workspace *workspace, unsigned long long ticket, csamd_serve_result *out);
+/* Free a finished query's output slot. */
+int csamd_serve_release(csamd_serve_workspace *workspace, unsigned long long ticket);
+/* Replay a schedule: query i is submitted when arrival_us[i] microseconds
+ * have passed since the start (ascending), the loop pumps until all are
+ * done, and latency_us[i] = completion seen - arrival; *wall_us = start to
+ * last completion.  With tickets, results stay held (release them); with
+ * tickets NULL each slot is released as soon as its query finishes. */
+int csamd_serve_replay(csamd_serve_workspace *workspace, const csamd_serve_query *queries, const double *arrival_us,
+                       unsigned int n, unsigned long long *tickets, double *latency_us, double *wall_us);
+/* The seeded row labels / row mask of a query as device arrays (labels
+ * [rows] u32, keep [rows] u8; either may be NULL).  Asynchronous. */
+int csamd_serve_materialize(unsigned int rows, unsigned int group_count, unsigned int label_seed, unsigned int mask_seed,
+                            unsigned int keep_thr24, unsigned int *d_labels, unsigned char *d_keep, csamd_stream_t stream);
+
+/* ---------------------------------------------------------------------------
  * Conversion workspace (grow-only scratch + stream), shared by converters,
  * transpose and bucketing entry points that need temporary device memory.
  * ------------------------------------------------------------------------- */

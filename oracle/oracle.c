@@ -866,3 +866,55 @@ int oracle_reoptimize(uint32_t rows, uint32_t cols, uint32_t bs, uint32_t ell_co
     free(wr);
     return 1;
 }
+
+/* ------------------------------------------------------------------------
+ * Serving query (project extension, no reference counterpart).  Restated on
+ * the row-major CSR, so it does not share the column layout the device
+ * path streams.  Seeded labels / mask: lowbias32 hash (Wellons), labels by
+ * multiply-high range reduction, mask by the top 24 hash bits.
+ * ------------------------------------------------------------------------ */
+uint32_t oracle_serve_hash(uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+
+uint32_t oracle_serve_label(uint32_t seed, uint32_t r, uint32_t G) {
+    return (uint32_t) (((uint64_t) oracle_serve_hash(seed ^ (r * 0x9E3779B9u)) * G) >> 32);
+}
+
+int oracle_serve_keep(uint32_t seed, uint32_t r, uint32_t thr24) {
+    return (oracle_serve_hash(seed ^ (r * 0x85EBCA6Bu)) >> 8) < thr24;
+}
+
+/* out is [k x G]: f64 for op 0 (sum) / 1 (sumsq), u32 for op 2 (nnz). */
+int oracle_serve_query(uint32_t rows, uint32_t cols, const uint32_t *ptr, const uint32_t *idx, const void *val, uint32_t vt,
+                       const uint32_t *features, uint32_t k, uint32_t G, uint32_t op, const uint32_t *labels,
+                       const uint8_t *keep, uint32_t label_seed, uint32_t mask_seed, uint32_t thr24, void *out) {
+    uint32_t *slot = (uint32_t *) malloc(((size_t) cols + 1u) * sizeof(uint32_t));
+    if (slot == NULL) return 0;
+    for (uint32_t c = 0; c < cols; ++c) slot[c] = INVALID_COL;
+    for (uint32_t j = 0; j < k; ++j) slot[features[j]] = j;
+    if (op == 2u) memset(out, 0, (size_t) k * G * sizeof(uint32_t));
+    else memset(out, 0, (size_t) k * G * sizeof(double));
+    for (uint32_t r = 0; r < rows; ++r) {
+        const int kept = keep != NULL ? keep[r] != 0u : oracle_serve_keep(mask_seed, r, thr24);
+        if (!kept) continue;
+        const uint32_t g = labels != NULL ? labels[r] : oracle_serve_label(label_seed, r, G);
+        if (g >= G) continue;
+        for (uint32_t e = ptr[r]; e < ptr[r + 1u]; ++e) {
+            const uint32_t c = idx[e];
+            if (c >= cols || slot[c] == INVALID_COL) continue;
+            const float v = decode(vt, val, e);
+            if (v == 0.0f) continue;
+            const size_t o = (size_t) slot[c] * G + g;
+            if (op == 2u) ((uint32_t *) out)[o] += 1u;
+            else ((double *) out)[o] += op == 1u ? (double) v * (double) v : (double) v;
+        }
+    }
+    free(slot);
+    return 1;
+}

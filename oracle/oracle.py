@@ -385,3 +385,34 @@ def ref_blocked_ell_from_compressed(rows, cols, ptr, idx, val16, bs):
                                           C.c_uint32(bs), C.byref(ell), C.byref(nnz), _p(bci), C.c_ulonglong(len(bci)), _p(out),
                                           C.c_ulonglong(len(out))), "ref convert")
     return bci[: rb * w], out[: rows * ell.value], ell.value, nnz.value
+
+
+def serve_label(seed: int, rows: int, G: int) -> np.ndarray:
+    L = lib()
+    L.oracle_serve_label.restype = C.c_uint32
+    L.oracle_serve_label.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32]
+    return np.array([L.oracle_serve_label(seed, r, G) for r in range(rows)], dtype=np.uint32)
+
+
+def serve_keep(seed: int, rows: int, thr24: int) -> np.ndarray:
+    L = lib()
+    L.oracle_serve_keep.restype = C.c_int
+    L.oracle_serve_keep.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32]
+    return np.array([L.oracle_serve_keep(seed, r, thr24) for r in range(rows)], dtype=np.uint8)
+
+
+def serve_query(rows, cols, ptr, idx, val, vt, features, G, op, labels=None, keep=None, label_seed=0, mask_seed=0,
+                thr24=1 << 24):
+    """Feature-subset grouped query on the CSR; returns [k x G] (f64, or u32 for op 2)."""
+    ptr = np.ascontiguousarray(ptr, dtype=np.uint32)
+    idx = np.ascontiguousarray(idx, dtype=np.uint32)
+    val = np.ascontiguousarray(val)
+    features = np.ascontiguousarray(features, dtype=np.uint32)
+    k = len(features)
+    out = np.zeros((k, G), dtype=np.uint32 if op == 2 else np.float64)
+    labels = None if labels is None else np.ascontiguousarray(labels, dtype=np.uint32)
+    keep = None if keep is None else np.ascontiguousarray(keep, dtype=np.uint8)
+    _ok(lib().oracle_serve_query(C.c_uint32(rows), C.c_uint32(cols), _p(ptr), _p(idx), _p(val), C.c_uint32(vt), _p(features),
+                                 C.c_uint32(k), C.c_uint32(G), C.c_uint32(op), _p(labels), _p(keep), C.c_uint32(label_seed),
+                                 C.c_uint32(mask_seed), C.c_uint32(thr24), _p(out)), "serve_query")
+    return out

@@ -374,3 +374,40 @@ def test_planted_mistake_is_caught():
     d = dense_of(rows, cols, ptr, idx, val)
     d[0, idx[ptr[0]]] = 0
     assert not np.allclose(o["row_totals"], d.sum(1))
+
+
+@pytest.mark.parametrize("op", [0, 1, 2])
+def test_serve_query_matches_dense(op):
+    """Feature-subset grouped query restated on the CSR vs a dense numpy
+    recomputation (explicit labels / mask; labels >= G drop the row)."""
+    rows, cols, G = 120, 60, 7
+    ptr, idx, val = random_csr(rows, cols, 0.2, seed=31, zeros=0.05)
+    bits = encode(val, F16)
+    rng = np.random.default_rng(8)
+    labels = rng.integers(0, G + 2, size=rows).astype(np.uint32)
+    keep = (rng.random(rows) < 0.7).astype(np.uint8)
+    feats = np.sort(rng.choice(cols, 17, replace=False)).astype(np.uint32)
+    got = O.serve_query(rows, cols, ptr, idx, bits, F16, feats, G, op, labels=labels, keep=keep)
+    dense = dense_of(rows, cols, ptr, idx, decode(bits, F16).astype(np.float64))
+    ref = np.zeros((len(feats), G))
+    for r in range(rows):
+        if keep[r] and labels[r] < G:
+            x = dense[r, feats]
+            ref[:, labels[r]] += (x != 0) if op == 2 else (x * x if op == 1 else x)
+    if op == 2:
+        assert got.dtype == np.uint32 and np.array_equal(got, ref.astype(np.uint32))
+    else:
+        np.testing.assert_allclose(got, ref, rtol=1e-12)
+
+
+def test_serve_seeded_labels_and_mask_are_uniform():
+    """Seeded labels cover [0, G) evenly and the mask keeps the requested
+    fraction (a wrong shift or range reduction skews both)."""
+    n = 40000
+    lab = O.serve_label(77, n, 10)
+    counts = np.bincount(lab, minlength=10)
+    assert lab.max() < 10 and counts.min() > 0.9 * n / 10 and counts.max() < 1.1 * n / 10
+    for frac in (0.05, 0.5, 1.0):
+        k = O.serve_keep(5, n, int(frac * (1 << 24) + 0.5))
+        assert abs(k.mean() - frac) < 0.01
+    assert O.serve_keep(5, 100, 0).sum() == 0
