@@ -45,7 +45,147 @@ def parse_args():
     ap.add_argument("--reduce-only", action="store_true", help="only the reduce steps (for profiling)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 2 GiB device copy (reference point for HBM byte counters)")
+    ap.add_argument("--skip-serve", action="store_true", help="skip the serving-loop measurement")
+    ap.add_argument("--serve-only", action="store_true", help="only the serving-loop measurement")
+    ap.add_argument("--serve-shards", type=int, default=64)
+    ap.add_argument("--serve-queries", type=int, default=4000)
+    ap.add_argument("--serve-rate", type=float, default=2000.0, help="offered Poisson arrival rate (queries/s)")
     return ap.parse_args()
+
+
+def pct(a, q):
+    import numpy as np
+    return float(np.percentile(np.asarray(a), q)) if len(a) else float("nan")
+
+
+def run_serving(args, log):
+    """Serving loop: 64 resident shards (rows U[50k, 200k] x 36,601, fp16
+    counts from the 1,500-median generator), Poisson arrivals; each query
+    takes a random shard, row-mask density U[0.05, 1], a log-uniform
+    [50, 5,000] feature subset, log-uniform [2, 512] groups and one op.
+    Batched loop vs one launch per query (same kernel), and vs one
+    row-scan reduce call per query on the shard's CSR."""
+    import numpy as np
+    import torch
+
+    from cellshard_amd import abi as A
+    from cellshard_amd.device import to_dev_u32
+    from cellshard_amd.ops import ConvertWorkspace, LabelReduce, TransposeScratch, compressed_transpose, synth_csr
+    from cellshard_amd.serve import Server, make_query, materialize
+
+    dev = torch.device("cuda", 0)
+    cols = 36_601
+    rng = np.random.default_rng(5)
+    n_sh = args.serve_shards
+    shard_rows = rng.integers(50_000, 200_001, size=n_sh)
+    ws = ConvertWorkspace()
+    scratch = TransposeScratch()
+    csr, csc = [], []
+    t0 = time.time()
+    for s in range(n_sh):
+        a = synth_csr(int(shard_rows[s]), cols, median=1500.0, sigma=0.5, seed=1000 + s, value_type=A.VALUE_F16, ws=ws)
+        csr.append(a)
+        csc.append(compressed_transpose(a, scratch))
+    torch.cuda.synchronize()
+    del scratch
+    torch.cuda.empty_cache()
+    nnz_total = sum(a.nnz for a in csr)
+    csc_bytes = sum(c.format_bytes() for c in csc)
+    log(f"serve: {n_sh} shards, {int(shard_rows.sum())} rows, nnz {nnz_total}, CSC {csc_bytes / 1e9:.1f} GB, "
+        f"generated in {time.time() - t0:.1f}s")
+
+    # queries
+    nq = args.serve_queries
+    q_shard = rng.integers(0, n_sh, size=nq)
+    q_dens = rng.uniform(0.05, 1.0, size=nq)
+    q_k = np.exp(rng.uniform(np.log(50), np.log(5000), size=nq)).astype(np.int64)
+    q_G = np.exp(rng.uniform(np.log(2), np.log(512), size=nq)).astype(np.int64)
+    q_op = rng.integers(0, 3, size=nq)
+    q_seed = rng.integers(0, 1 << 32, size=(nq, 2), dtype=np.uint64)
+    feats = [np.sort(rng.choice(cols, size=int(k), replace=False)).astype(np.uint32) for k in q_k]
+    offs = np.concatenate([[0], np.cumsum(q_k)])
+    d_feats = to_dev_u32(np.concatenate(feats), dev)
+    structs = [make_query(int(q_shard[i]), int(q_op[i]), int(q_G[i]), d_feats[int(offs[i]):int(offs[i + 1])],
+                          label_seed=int(q_seed[i, 0]), mask_seed=int(q_seed[i, 1]), keep_fraction=float(q_dens[i]))
+               for i in range(nq)]
+    arrivals = np.cumsum(rng.exponential(1e6 / args.serve_rate, size=nq))
+    torch.cuda.synchronize()
+    slot_bytes = 5000 * 512 * 8
+    budget = 200 << 30
+
+    def server(batch, inflight, slots):
+        srv = Server(max_slots=slots, max_batch_queries=batch, max_batches_in_flight=inflight, slot_bytes=slot_bytes,
+                     budget_bytes=budget)
+        for c in csc:
+            srv.add_shard(c)
+        return srv
+
+    # algorithmic bytes of a query: its columns' entries (4 B row + 2 B value) and the feature list
+    col_len = [np.diff(c.major_ptr.cpu().numpy().view(np.uint32).astype(np.int64)) for c in csc]
+    q_bytes = np.array([col_len[int(q_shard[i])][feats[i]].sum() * 6 + 4 * len(feats[i]) for i in range(nq)], dtype=np.float64)
+    out = {"workload": f"serve_{n_sh}_csc_shards_rows50k-200k_x36601_fp16_poisson{int(args.serve_rate)}",
+           "shards": n_sh, "rows": int(shard_rows.sum()), "nnz": int(nnz_total), "resident_csc_bytes": int(csc_bytes),
+           "resident_csr_bytes": int(sum(a.format_bytes() for a in csr)), "budget_bytes": budget,
+           "queries": nq, "offered_qps": args.serve_rate, "query_bytes_mean": int(q_bytes.mean())}
+
+    # batched loop
+    srv = server(64, 2, 256)
+    srv.replay(structs[:200], np.zeros(200), hold=False)  # warm
+    _, lat, wall = srv.replay(structs, arrivals, hold=False)
+    _, lat_s, wall_s = srv.replay(structs, np.zeros(nq), hold=False)
+    out["batched"] = {"qps_at_offered": round(nq / (wall * 1e-6), 1), "p50_ms": round(pct(lat, 50) / 1e3, 4),
+                      "p99_ms": round(pct(lat, 99) / 1e3, 4), "saturated_qps": round(nq / (wall_s * 1e-6), 1),
+                      "saturated_gb_s": round(q_bytes.sum() / (wall_s * 1e-6) / 1e9, 1),
+                      "saturated_p99_ms": round(pct(lat_s, 99) / 1e3, 3), "max_batch_queries": 64, "launches_in_flight": 2}
+    # identity: a sample of queries alone (one launch each) vs in the mix
+    sample = list(range(0, nq, max(1, nq // 24)))[:24]
+    tk_mix, _, _ = srv.replay([structs[i] for i in sample], np.zeros(len(sample)), hold=True)
+    mix = [srv.result(t)["out"] for t in tk_mix]
+    for t in tk_mix:
+        srv.release(t)
+    srv.close()
+
+    # plain: one launch per query, one at a time (same kernel)
+    srv1 = server(1, 1, 4)
+    srv1.replay(structs[:50], np.zeros(50), hold=False)
+    _, lat1, wall1 = srv1.replay(structs, arrivals, hold=False)
+    _, lat1s, wall1s = srv1.replay(structs, np.zeros(nq), hold=False)
+    alone = []
+    for i in sample:
+        t = srv1.submit(structs[i])
+        srv1.drain()
+        alone.append(srv1.result(t)["out"])
+        srv1.release(t)
+    srv1.close()
+    same = all(np.array_equal(a, b) for a, b in zip(alone, mix))
+    out["plain"] = {"qps_at_offered": round(nq / (wall1 * 1e-6), 1), "p50_ms": round(pct(lat1, 50) / 1e3, 4),
+                    "p99_ms": round(pct(lat1, 99) / 1e3, 4), "saturated_qps": round(nq / (wall1s * 1e-6), 1),
+                    "saturated_gb_s": round(q_bytes.sum() / (wall1s * 1e-6) / 1e9, 1)}
+    out["identity_alone_vs_mixed"] = {"queries": len(sample), "identical": bool(same)}
+    out["speedup_saturated_qps_vs_plain"] = round(out["batched"]["saturated_qps"] / out["plain"]["saturated_qps"], 2)
+
+    # row scan: one label-reduce call per query over the whole shard CSR
+    lr = LabelReduce(stream=torch.cuda.current_stream().cuda_stream)
+    nrs = min(nq, 200)
+    fk = torch.zeros(cols, dtype=torch.uint8, device=dev)
+    times = []
+    for i in range(min(nq, nrs + 5)):
+        s = int(q_shard[i])
+        t1 = time.perf_counter()
+        labels, keep = materialize(int(shard_rows[s]), int(q_G[i]), int(q_seed[i, 0]), int(q_seed[i, 1]), float(q_dens[i]), dev)
+        fk.zero_()
+        fk[d_feats[int(offs[i]):int(offs[i + 1])].long()] = 1
+        lr.run(csr[s], labels, int(q_G[i]), keep, fk, moments=False)
+        torch.cuda.synchronize()
+        if i >= min(5, nq - 1):
+            times.append(time.perf_counter() - t1)
+    lr.close()
+    out["plain_row_scan"] = {"saturated_qps": round(len(times) / sum(times), 1), "mean_ms": round(1e3 * float(np.mean(times)), 3),
+                             "queries": len(times)}
+    out["speedup_saturated_qps_vs_row_scan"] = round(out["batched"]["saturated_qps"] / out["plain_row_scan"]["saturated_qps"], 1)
+    ws.close()
+    log(f"serve: {json.dumps(out)}")
+    return out
 
 
 def read_traffic(path: Path):
@@ -72,6 +212,11 @@ def main() -> int:
 
     import numpy as np
     import torch
+
+    if args.serve_only:
+        torch.cuda.set_device(0)
+        print(json.dumps({"serving": run_serving(args, log)}), flush=True)
+        return 0
 
     from cellshard_amd import abi as A
     from cellshard_amd.device import to_dev_u32, to_dev_u8
@@ -202,6 +347,14 @@ def main() -> int:
         log(f"cpu baseline: {cpu}")
 
     traffic = read_traffic(ROOT / "profiles" / "label_reduce_csr_1Mx36601_pmc.txt")
+    kept_features = int(got["kept_features"])
+    lr.close()
+    ws.close()
+    del src, lr, res, got
+    torch.cuda.empty_cache()
+    serving = None
+    if not args.skip_serve and not args.reduce_only:
+        serving = run_serving(args, log)
     line = {
         "metric": "masked group-reduce HBM GB/s (% of peak)",
         "value": round(value, 1),
@@ -217,18 +370,17 @@ def main() -> int:
         "data": "synthetic (GPU generator: LogNormal row nnz, Zipf(1.1) columns over a seeded permutation, zero-truncated NB2 counts)",
         "config": {"workload": "masked_grouped_sum_sumsq_nnz_csr_1000000x36601_fp16_64groups", "rows": R, "cols": Cc,
                    "nnz": nnz, "value_storage": "f16", "groups": G, "group_sizes": "Dirichlet(0.5)", "row_keep_p": 0.8,
-                   "kept_rows": kept_rows, "kept_nnz": kept_nnz, "kept_features": int(got["kept_features"]),
+                   "kept_rows": kept_rows, "kept_nnz": kept_nnz, "kept_features": kept_features,
                    "algorithmic_bytes_per_step": b_alg_step, "pct_of_peak": round(100.0 * value / PEAK_HBM_GBS, 2)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
                      "kernel": "label_reduce_rows_kernel", "kernel_ms": round(k_ms, 4), "algorithmic_bytes": b_alg_kernel},
         "cpu_baseline": cpu,
         "ops": ops,
+        "serving": serving,
         "generate_s": round(gen_s, 2),
     }
     print(json.dumps(line), flush=True)
-    lr.close()
-    ws.close()
     return 0
 
 
