@@ -46,6 +46,8 @@ def parse_args():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 2 GiB device copy (reference point for HBM byte counters)")
     ap.add_argument("--skip-serve", action="store_true", help="skip the serving-loop measurement")
+    ap.add_argument("--skip-lowp", action="store_true", help="skip the BF16 / FP8 reduced-precision measurement")
+    ap.add_argument("--lowp-only", action="store_true", help="only the BF16 / FP8 reduced-precision measurement")
     ap.add_argument("--serve-only", action="store_true", help="only the serving-loop measurement")
     ap.add_argument("--serve-shards", type=int, default=64)
     ap.add_argument("--serve-queries", type=int, default=4000)
@@ -188,6 +190,97 @@ def run_serving(args, log):
     return out
 
 
+def run_reduced_precision(args, log):
+    """500,000 x 36,601 log1p(count / row_total x 1e4) values (1,500-median
+    generator) stored as BF16 and as FP8 e4m3 with per-feature absmax scale;
+    masked grouped mean/variance over 128 labels (row keep 0.8, 3,000
+    features).  Per dtype: step and kernel GB/s, and per-group relative error
+    of sum / sumsq / mean / var vs the FP64 oracle on the same stored values
+    (a row sample), plus the storage error vs unquantized FP32 values."""
+    import numpy as np
+    import torch
+
+    from cellshard_amd import abi as A
+    from cellshard_amd.device import to_dev_u32, to_dev_u8
+    from cellshard_amd.ops import ConvertWorkspace, LabelReduce, log1p_normalize, synth_csr
+    from oracle import oracle as O
+
+    dev = torch.device("cuda", 0)
+    R, Cc, G, FK = 500_000, 36_601, 128, 3000
+    ws = ConvertWorkspace()
+    counts = synth_csr(R, Cc, median=1500.0, sigma=0.5, seed=4, value_type=A.VALUE_F32, ws=ws)
+    rng = np.random.default_rng(4)
+    labels = rng.integers(0, G, size=R).astype(np.uint32)
+    rk = (rng.random(R) < 0.8).astype(np.uint8)
+    fk = np.zeros(Cc, np.uint8)
+    fk[rng.choice(Cc, FK, replace=False)] = 1
+    d_l, d_rk, d_fk = to_dev_u32(labels, dev), to_dev_u8(rk, dev), to_dev_u8(fk, dev)
+    ptr_h = counts.major_ptr.cpu().numpy().view(np.uint32).astype(np.int64)
+    kept = (rk == 1)
+    kept_nnz = int(np.diff(ptr_h)[kept].sum())
+    S = 60_000  # error sample: the first S rows
+    end = int(ptr_h[S])
+    idx_s = counts.minor_idx[:end].cpu().numpy().view(np.uint32)
+    p_s = ptr_h[: S + 1].astype(np.uint32)
+    f32v, _ = log1p_normalize(counts, A.VALUE_F32, ws=ws)
+    exact = O.label_reduce_csr(S, Cc, p_s, idx_s, f32v.val[:end].cpu().numpy(), A.VALUE_F32, labels[:S], rk[:S], fk, G)
+    del f32v
+    out = {"workload": "masked_grouped_mean_var_csr_500000x36601_log1p_128groups", "rows": R, "cols": Cc, "nnz": counts.nnz,
+           "groups": G, "kept_features": FK, "row_keep_p": 0.8, "error_sample_rows": S}
+    lr = LabelReduce(stream=torch.cuda.current_stream().cuda_stream)
+
+    def rel(a, b):
+        m = b != 0
+        e = np.zeros_like(b)
+        e[m] = np.abs(a[m] - b[m]) / np.abs(b[m])
+        e[~m] = np.abs(a[~m])
+        return e
+
+    for name, vt, vb in (("bf16", A.VALUE_BF16, 2), ("fp8_e4m3", A.VALUE_FP8_E4M3, 1)):
+        src, scale = log1p_normalize(counts, vt, ws=ws)
+        for _ in range(3):
+            lr.run(src, d_l, G, d_rk, d_fk, scale)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            lr.run(src, d_l, G, d_rk, d_fk, scale)
+        torch.cuda.synchronize()
+        step_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+        lr.set_timing(True)
+        km = []
+        for _ in range(args.steps):
+            lr.run(src, d_l, G, d_rk, d_fk, scale)
+            km.append(lr.last_kernel_ms())
+        lr.set_timing(False)
+        k_ms = float(np.mean(km))
+        b_step = kept_nnz * (4 + vb) + (R + 1) * 4 + R * 5 + Cc * 4 + G * FK * 36 + (Cc * 4 if scale is not None else 0)
+        b_kernel = kept_nnz * (4 + vb) + int(kept.sum()) * 12
+        # errors on the row sample: same stored values (FP64 oracle) and vs unquantized FP32 values
+        sub = type(src)(S, Cc, src.major_ptr[: S + 1], src.minor_idx[:end], src.val[:end], vt, src.axis)
+        got = LabelReduce.fetch(lr.run(sub, d_l[:S], G, d_rk[:S], d_fk, scale))
+        sc = scale.cpu().numpy() if scale is not None else None
+        ref = O.label_reduce_csr(S, Cc, p_s, idx_s, src.val[:end].cpu().numpy(), vt, labels[:S], rk[:S], fk, G, sc)
+        assert np.array_equal(got["nnz"], ref["nnz"]) and np.array_equal(got["group_rows"], ref["group_rows"])
+        n = np.maximum(ref["group_rows"], 1)[:, None].astype(np.float64)
+        r_mean, r_var = ref["sum"] / n, np.maximum(ref["sumsq"] / n - (ref["sum"] / n) ** 2, 0)
+        e = {k: rel(got[k], v).max(axis=1) for k, v in (("sum", ref["sum"]), ("sumsq", ref["sumsq"]), ("mean", r_mean), ("var", r_var))}
+        qm = rel(ref["sum"], exact["sum"]).max(axis=1)
+        out[name] = {"value_bytes": vb, "ms_per_step": round(step_ms, 4), "step_gb_s": round(b_step / (step_ms * 1e-3) / 1e9, 1),
+                     "kernel_ms": round(k_ms, 4), "kernel_gb_s": round(b_kernel / (k_ms * 1e-3) / 1e9, 1),
+                     "kernel_frac_of_peak": round(b_kernel / (k_ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
+                     "nnz_counts_exact": True,
+                     "rel_err_vs_fp64_oracle": {k: {"max_over_groups": float(v.max()), "median_over_groups": float(np.median(v))}
+                                                for k, v in e.items()},
+                     "storage_rel_err_sum_vs_fp32_values": {"max_over_groups": float(qm.max()),
+                                                            "median_over_groups": float(np.median(qm))}}
+        del src, scale, sub
+        torch.cuda.empty_cache()
+    lr.close()
+    ws.close()
+    log(f"reduced precision: {json.dumps(out)}")
+    return out
+
+
 def read_traffic(path: Path):
     """Per-launch HBM bytes of the reduce kernel from a committed rocprofv3
     PMC summary (line: 'hbm_bytes_per_launch: N'), else None."""
@@ -216,6 +309,10 @@ def main() -> int:
     if args.serve_only:
         torch.cuda.set_device(0)
         print(json.dumps({"serving": run_serving(args, log)}), flush=True)
+        return 0
+    if args.lowp_only:
+        torch.cuda.set_device(0)
+        print(json.dumps({"reduced_precision": run_reduced_precision(args, log)}), flush=True)
         return 0
 
     from cellshard_amd import abi as A
@@ -352,6 +449,9 @@ def main() -> int:
     ws.close()
     del src, lr, res, got
     torch.cuda.empty_cache()
+    lowp = None
+    if not args.skip_lowp and not args.reduce_only:
+        lowp = run_reduced_precision(args, log)
     serving = None
     if not args.skip_serve and not args.reduce_only:
         serving = run_serving(args, log)
@@ -377,6 +477,7 @@ def main() -> int:
                      "kernel": "label_reduce_rows_kernel", "kernel_ms": round(k_ms, 4), "algorithmic_bytes": b_alg_kernel},
         "cpu_baseline": cpu,
         "ops": ops,
+        "reduced_precision": lowp,
         "serving": serving,
         "generate_s": round(gen_s, 2),
     }

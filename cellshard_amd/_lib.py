@@ -77,6 +77,7 @@ PROTOTYPES = {
     "csamd_serve_release": (i32, [P(A.serve_workspace), u64]),
     "csamd_serve_replay": (i32, [P(A.serve_workspace), P(A.serve_query), P(C.c_double), u32, P(u64), P(C.c_double), P(C.c_double)]),
     "csamd_serve_materialize": (i32, [u32, u32, u32, u32, u32, vp, vp, vp]),
+    "csamd_decode_values": (i32, [u32, vp, u64, vp, vp]),
     # convert
     "csamd_convert_init": (None, [P(A.convert_workspace)]),
     "csamd_convert_clear": (None, [P(A.convert_workspace)]),

@@ -553,9 +553,7 @@ template<> struct unit_vals<CSAMD_VALUE_FP8_E4M3> {
         return (unsigned int) v[min(i0, last)] | ((unsigned int) v[min(i0 + 1u, last)] << 8) |
                ((unsigned int) v[min(i0 + 2u, last)] << 16) | ((unsigned int) v[min(i0 + 3u, last)] << 24);
     }
-    static __device__ __forceinline__ float get(const raw &r, unsigned int k) {
-        return fp8_e4m3_to_float((uint8_t) (r >> (8u * k)));
-    }
+    static __device__ __forceinline__ float get(const raw &r, unsigned int k) { return fp8_e4m3_decode_hw(r, (int) k); }
 };
 
 constexpr unsigned int LR_UNITS = 4u;               // 4-entry units per lane per chunk (even)

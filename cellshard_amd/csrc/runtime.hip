@@ -363,6 +363,18 @@ int radix_sort_pairs_u64(const unsigned long long *d_keys_in, unsigned long long
 } // namespace csamd
 
 // ---------------------------------------------------------------------------
+namespace csamd {
+namespace {
+template<int VT>
+__global__ void decode_values_kernel(const typename value_traits<VT>::storage *__restrict__ in, unsigned long long n,
+                                     float *__restrict__ out) {
+    for (unsigned long long i = blockIdx.x * (unsigned long long) blockDim.x + threadIdx.x; i < n;
+         i += (unsigned long long) gridDim.x * blockDim.x)
+        out[i] = value_traits<VT>::decode(in[i]);
+}
+} // namespace
+} // namespace csamd
+
 // C ABI: errors, device count, convert workspace, scan.
 // ---------------------------------------------------------------------------
 extern "C" {
@@ -436,6 +448,21 @@ int csamd_exclusive_scan_u32(const unsigned int *d_counts, unsigned int n, unsig
     CSAMD_HIP(hipStreamSynchronize(stream), "scan sync 2");
     if (out_total != nullptr) *out_total = total;
     return 1;
+}
+
+int csamd_decode_values(unsigned int value_type, const void *d_values, unsigned long long n, float *d_out, csamd_stream_t stream) {
+    CSAMD_REQUIRE(csamd::value_type_decodable(value_type), "decode_values: unsupported value type %u", value_type);
+    if (n == 0ull) return 1;
+    CSAMD_REQUIRE(d_values != nullptr && d_out != nullptr, "decode_values: null argument");
+    const unsigned int grid = csamd::grid_for(n, 256u, 8192u);
+    hipStream_t s = (hipStream_t) stream;
+    switch (value_type) {
+        case CSAMD_VALUE_F16: csamd::decode_values_kernel<CSAMD_VALUE_F16><<<grid, 256, 0, s>>>((const uint16_t *) d_values, n, d_out); break;
+        case CSAMD_VALUE_BF16: csamd::decode_values_kernel<CSAMD_VALUE_BF16><<<grid, 256, 0, s>>>((const uint16_t *) d_values, n, d_out); break;
+        case CSAMD_VALUE_F32: csamd::decode_values_kernel<CSAMD_VALUE_F32><<<grid, 256, 0, s>>>((const float *) d_values, n, d_out); break;
+        default: csamd::decode_values_kernel<CSAMD_VALUE_FP8_E4M3><<<grid, 256, 0, s>>>((const uint8_t *) d_values, n, d_out); break;
+    }
+    return csamd::hip_ok(hipGetLastError(), "decode_values_kernel");
 }
 
 } // extern "C"

@@ -808,6 +808,9 @@ typedef struct CSAMD_ALIGN16 csamd_synth_params {
 int csamd_synth_row_nnz(const csamd_synth_params *p, unsigned int *d_row_nnz, csamd_stream_t stream);
 /* Exclusive scan of d_counts [n] into d_ptr [n+1] (u32; fails with an error
  * if the total overflows u32).  Synchronous; returns the total. */
+/* Decode n stored values of value_type (f16, bf16, f32, fp8 e4m3) to fp32
+ * with the same device decoders the kernels use.  Asynchronous. */
+int csamd_decode_values(unsigned int value_type, const void *d_values, unsigned long long n, float *d_out, csamd_stream_t stream);
 int csamd_exclusive_scan_u32(const unsigned int *d_counts, unsigned int n, unsigned int *d_ptr,
                              csamd_convert_workspace *ws, unsigned long long *out_total);
 /* Pass 2: fill minor indices and values of every row given d_ptr [rows+1]. */
