@@ -52,6 +52,9 @@ def parse_args():
     ap.add_argument("--serve-shards", type=int, default=64)
     ap.add_argument("--serve-queries", type=int, default=4000)
     ap.add_argument("--serve-rate", type=float, default=2000.0, help="offered Poisson arrival rate (queries/s)")
+    ap.add_argument("--skip-batched", action="store_true", help="skip the batched uneven-partition measurement")
+    ap.add_argument("--batched-only", action="store_true", help="only the batched uneven-partition measurement")
+    ap.add_argument("--c3-partitions", type=int, default=256)
     return ap.parse_args()
 
 
@@ -295,6 +298,146 @@ def read_traffic(path: Path):
     return None
 
 
+def run_batched_partitions(args, log):
+    """Batched uneven partitions: 256 CSR partitions (rows U[512, 65,536],
+    median row length U[300, 5,000], LogNormal sigma 0.7, 36,601 columns,
+    fp16 counts).  The layout tuner (Blocked-ELL {8,16,32}, Sliced-ELL
+    {32,64}) over every partition in one launch; partitions whose smallest
+    layout is Sliced-ELL are converted; then the masked grouped reduce over
+    32 groups (Dirichlet(0.5) sizes, row mask 0.8, 3,000 features) of all
+    partitions in one reduce launch: eager calls, replays of the captured
+    HIP graph, and one call per partition for comparison."""
+    import numpy as np
+    import torch
+
+    from cellshard_amd import abi as A
+    from cellshard_amd.device import to_dev_u32, to_dev_u8
+    from cellshard_amd.ops import (ConvertWorkspace, LabelReduce, LabelReduceGraph, sliced_ell_from_compressed, synth_csr,
+                                   tune_partitions)
+
+    dev = torch.device("cuda", 0)
+    cols, G, FK, n = 36_601, 32, 3000, args.c3_partitions
+    rng = np.random.default_rng(33)
+    rows_p = rng.integers(512, 65_537, size=n)
+    med_p = rng.uniform(300.0, 5000.0, size=n)
+    ws = ConvertWorkspace()
+    t0 = time.time()
+    parts = [synth_csr(int(rows_p[i]), cols, median=float(med_p[i]), sigma=0.7, seed=3000 + i, value_type=A.VALUE_F16, ws=ws)
+             for i in range(n)]
+    torch.cuda.synchronize()
+    R = int(rows_p.sum())
+    nnz = sum(p.nnz for p in parts)
+    csr_bytes = sum(p.format_bytes() for p in parts)
+    log(f"batched: {n} partitions, {R} rows, nnz {nnz}, CSR {csr_bytes / 1e9:.1f} GB, generated in {time.time() - t0:.1f}s")
+
+    row_len = np.concatenate([np.diff(p.major_ptr.cpu().numpy().view(np.uint32).astype(np.int64)) for p in parts])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    tune = tune_partitions(parts, ws)
+    tune_ms = (time.perf_counter() - t0) * 1e3
+    layouts = {k: sum(1 for t in tune if t["layout"] == k) for k in ("compressed", "sliced_ell", "blocked_ell")}
+    chosen_bytes = sum(min(t["csr_bytes"], t["sell_bytes"], t["bell_bytes"]) for t in tune)
+    log(f"batched: tuner {tune_ms:.1f} ms, layouts {layouts}")
+    conv_ms = 0.0
+    for i, t in enumerate(tune):
+        if t["layout"] == "sliced_ell":
+            torch.cuda.synchronize()
+            s0 = time.perf_counter()
+            parts[i] = sliced_ell_from_compressed(parts[i], t["sell_slice_rows"], ws)
+            torch.cuda.synchronize()
+            conv_ms += (time.perf_counter() - s0) * 1e3
+    # Blocked-ELL partitions are reduced from their CSR (the batched reduce
+    # streams compressed and sliced rows); none are chosen on this workload
+    torch.cuda.empty_cache()
+
+    props_g = rng.dirichlet(np.full(G, 0.5))
+    labels = rng.choice(G, size=R, p=props_g).astype(np.uint32)
+    row_keep = (rng.random(R) < 0.8).astype(np.uint8)
+    feature_keep = np.zeros(cols, np.uint8)
+    feature_keep[rng.choice(cols, FK, replace=False)] = 1
+    d_labels, d_rk, d_fk = to_dev_u32(labels, dev), to_dev_u8(row_keep, dev), to_dev_u8(feature_keep, dev)
+    kept = (row_keep == 1) & (labels < G)
+    kept_nnz = int(row_len[kept].sum())
+    out_bytes = G * FK * (8 + 8 + 4 + 8 + 8) + G * 4
+    b_alg = kept_nnz * 6 + (R + n) * 4 + R * 5 + cols * 4 + out_bytes
+
+    stream = torch.cuda.current_stream().cuda_stream
+    lr = LabelReduce(stream=stream)
+    table = LabelReduce.partition_table(parts)
+    for _ in range(args.warmup):
+        lr.run_batched(parts, d_labels, G, d_rk, d_fk, table=table)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        lr.run_batched(parts, d_labels, G, d_rk, d_fk, table=table)
+    torch.cuda.synchronize()
+    eager_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    eager = LabelReduce.fetch(lr.run_batched(parts, d_labels, G, d_rk, d_fk, table=table))
+    lr.set_timing(True)
+    kms = []
+    for _ in range(args.steps):
+        lr.run_batched(parts, d_labels, G, d_rk, d_fk, table=table)
+        kms.append(lr.last_kernel_ms())
+    lr.set_timing(False)
+    k_ms = float(np.mean(kms))
+
+    graph = LabelReduceGraph(parts, d_labels, G, d_rk, d_fk)
+    for _ in range(args.warmup):
+        graph.launch(stream)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        graph.launch(stream)
+    torch.cuda.synchronize()
+    graph_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    replay = LabelReduce.fetch(graph.result())
+    # counts are exact; FP64 sums are order-independent only while they stay
+    # below 2^53 (sum of squares of large counts can pass it), so those are
+    # compared by relative difference
+    counts_same = all(np.array_equal(np.asarray(eager[k]), np.asarray(replay[k]))
+                      for k in ("nnz", "group_rows", "kept_feature_ids", "kept_rows", "kept_features"))
+    rel = max(float(np.max(np.abs(eager[k] - replay[k]) / np.maximum(np.abs(eager[k]), 1e-300), initial=0.0))
+              for k in ("sum", "sumsq"))
+    big = float(np.max(eager["sumsq"], initial=0.0))
+    graph.close()
+
+    # one reduce call per partition (the reference's per-partition launches)
+    offs = np.concatenate([[0], np.cumsum(rows_p)])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i, p in enumerate(parts):
+        lr.run(p, d_labels[int(offs[i]):int(offs[i + 1])], G, d_rk[int(offs[i]):int(offs[i + 1])], d_fk)
+    torch.cuda.synchronize()
+    per_ms = (time.perf_counter() - t0) * 1e3
+    lr.close()
+    ws.close()
+    kept_rows = int(kept.sum())
+    group_ok = bool(np.array_equal(eager["group_rows"], np.bincount(labels[kept], minlength=G)[:G]))
+    res = {
+        "workload": f"batched_{n}_csr_partitions_rows512-65536_x36601_fp16_32groups",
+        "partitions": n, "rows": R, "nnz": nnz, "csr_bytes": csr_bytes, "kept_rows": kept_rows, "kept_nnz": kept_nnz,
+        "tuner": {"ms": round(tune_ms, 2), "layouts": layouts, "chosen_bytes": chosen_bytes,
+                  "bell_block_sizes": {str(b): sum(1 for t in tune if t["bell_block_size"] == b) for b in (8, 16, 32)},
+                  "sell_slice_rows": {str(s_): sum(1 for t in tune if t["sell_slice_rows"] == s_) for s_ in (32, 64)},
+                  "median_bytes_ratio_sell_to_csr": round(float(np.median([t["sell_bytes"] / t["csr_bytes"] for t in tune])), 3),
+                  "median_bytes_ratio_bell_to_csr": round(float(np.median([t["bell_bytes"] / t["csr_bytes"] for t in tune])), 3),
+                  "sell_conversion_ms": round(conv_ms, 2)},
+        "algorithmic_bytes_per_step": b_alg,
+        "eager": {"ms_per_step": round(eager_ms, 4), "gb_s": round(b_alg / (eager_ms * 1e-3) / 1e9, 1),
+                  "kernel_ms": round(k_ms, 4),
+                  "kernel_gb_s": round((kept_nnz * 6 + kept_rows * 12) / (k_ms * 1e-3) / 1e9, 1)},
+        "graph": {"ms_per_step": round(graph_ms, 4), "gb_s": round(b_alg / (graph_ms * 1e-3) / 1e9, 1),
+                  "counts_identical_to_eager": bool(counts_same), "max_rel_diff_sums_vs_eager": rel,
+                  "max_sumsq": big, "max_sumsq_over_2p53": round(big / 2.0 ** 53, 3)},
+        "per_partition_calls": {"ms_per_pass": round(per_ms, 3), "gb_s": round(b_alg / (per_ms * 1e-3) / 1e9, 1)},
+        "group_rows_match": group_ok,
+    }
+    log(f"batched: {json.dumps(res)}")
+    del parts, d_labels, d_rk, d_fk
+    torch.cuda.empty_cache()
+    return res
+
+
 def main() -> int:
     args = parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -309,6 +452,10 @@ def main() -> int:
     if args.serve_only:
         torch.cuda.set_device(0)
         print(json.dumps({"serving": run_serving(args, log)}), flush=True)
+        return 0
+    if args.batched_only:
+        torch.cuda.set_device(0)
+        print(json.dumps({"batched_partitions": run_batched_partitions(args, log)}), flush=True)
         return 0
     if args.lowp_only:
         torch.cuda.set_device(0)
@@ -452,6 +599,9 @@ def main() -> int:
     lowp = None
     if not args.skip_lowp and not args.reduce_only:
         lowp = run_reduced_precision(args, log)
+    batched = None
+    if not args.skip_batched and not args.reduce_only:
+        batched = run_batched_partitions(args, log)
     serving = None
     if not args.skip_serve and not args.reduce_only:
         serving = run_serving(args, log)
@@ -478,6 +628,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "ops": ops,
         "reduced_precision": lowp,
+        "batched_partitions": batched,
         "serving": serving,
         "generate_s": round(gen_s, 2),
     }

@@ -65,6 +65,12 @@ PROTOTYPES = {
     "csamd_label_reduce_sliced_ell": (i32, [P(A.sliced_ell_view), P(A.label_reduce_query), P(A.label_reduce_workspace), P(A.label_reduce_result)]),
     "csamd_label_reduce_blocked_ell": (i32, [P(A.blocked_ell_view), P(A.label_reduce_query), P(A.label_reduce_workspace), P(A.label_reduce_result)]),
     "csamd_label_reduce_last_kernel_ms": (f32, [P(A.label_reduce_workspace)]),
+    "csamd_label_reduce_batched": (i32, [P(A.label_reduce_partition), u32, P(A.label_reduce_query), P(A.label_reduce_workspace), P(A.label_reduce_result)]),
+    "csamd_label_reduce_graph_capture": (i32, [P(A.label_reduce_partition), u32, P(A.label_reduce_query), i32, P(A.label_reduce_graph)]),
+    "csamd_label_reduce_graph_launch": (i32, [P(A.label_reduce_graph), vp]),
+    "csamd_label_reduce_graph_result": (i32, [P(A.label_reduce_graph), P(A.label_reduce_result)]),
+    "csamd_label_reduce_graph_destroy": (None, [P(A.label_reduce_graph)]),
+    "csamd_tune_partitions": (i32, [P(A.compressed_view), u32, u32, P(A.convert_workspace), P(A.partition_tune_result)]),
     "csamd_label_reduce_set_timing": (i32, [P(A.label_reduce_workspace), i32]),
     "csamd_serve_init": (None, [P(A.serve_workspace)]),
     "csamd_serve_clear": (None, [P(A.serve_workspace)]),

@@ -212,6 +212,17 @@ serve_config = _aligned16("serve_config", [
 serve_workspace = _aligned16("serve_workspace", [("device", C.c_int), ("state", c_void_p)])
 SERVE_SUM, SERVE_SUMSQ, SERVE_NNZ = 0, 1, 2
 
+label_reduce_partition = _aligned16("label_reduce_partition", [
+    ("layout", c_u32), ("reserved", c_u32 * 3), ("compressed", compressed_view), ("sliced_ell", sliced_ell_view),
+])
+label_reduce_graph = _aligned16("label_reduce_graph", [("device", C.c_int), ("state", c_void_p)])
+partition_tune_result = _aligned16("partition_tune_result", [
+    ("layout", c_u32), ("bell_block_size", c_u32), ("bell_ell_width", c_u32), ("sell_slice_rows", c_u32),
+    ("sell_slice_count", c_u32), ("reserved", c_u32), ("sell_padded_slots", C.c_uint64), ("csr_bytes", C.c_uint64),
+    ("bell_bytes", C.c_uint64), ("sell_bytes", C.c_uint64), ("bell_fill_ratio", C.c_double), ("sell_score", C.c_double),
+    ("row_length_log2_hist", c_u32 * 32),
+])
+
 # Structs whose layout tests/test_abi.py compares against the C header.
 ALL_STRUCTS = {
     "csamd_compressed_view": compressed_view,
@@ -234,6 +245,9 @@ ALL_STRUCTS = {
     "csamd_label_reduce_query": label_reduce_query,
     "csamd_label_reduce_result": label_reduce_result,
     "csamd_label_reduce_workspace": label_reduce_workspace,
+    "csamd_label_reduce_partition": label_reduce_partition,
+    "csamd_label_reduce_graph": label_reduce_graph,
+    "csamd_partition_tune_result": partition_tune_result,
     "csamd_convert_workspace": convert_workspace,
     "csamd_blocked_ell_tune_result": blocked_ell_tune_result,
     "csamd_sliced_ell_tune_result": sliced_ell_tune_result,

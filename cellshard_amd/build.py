@@ -32,6 +32,7 @@ HIP_SOURCES = [
     "staging.hip",
     "host_convert.hip",
     "serve.hip",
+    "partition_tune.hip",
 ]
 
 HIPCC_FLAGS = [

@@ -1,5 +1,8 @@
 // Row-label masked grouped reduction: per (label g, kept feature f)
-// {sum, sumsq, nnz} over kept rows, FP64 partials, deterministic combine.
+// {sum, sumsq, nnz} over kept rows, FP64 partials, FP64 combine, mean/var.
+// One call covers one matrix or a batch of partitions (CSR / Sliced-ELL
+// partitions with the same columns; row arrays span the partitions' rows
+// in order), with one reduce launch over all of them.
 //
 // There is no reference kernel for this operation (mask_groups.cu reduces per
 // row over <= 32 feature-group bitmasks); its masking rules are the
@@ -8,27 +11,34 @@
 // (mask_groups.cu:155).
 //
 // Design (gfx950):
-//   1. kept-feature bitmap + per-word prefix (feature -> compact rank), staged
-//      in LDS as {mask, prefix} pairs, one 8-byte LDS read per entry;
-//   2. kept rows stably sorted by label (radix sort of (label, row));
-//   3. the sorted row list split into one contiguous, nnz-balanced range per
-//      persistent workgroup (2 per CU for F <= ~3.3k kept features);
-//   4. each workgroup walks its range label segment by label segment; one
-//      wave64 streams one row at a time with 16-byte index / 8-byte value
-//      vector loads; kept entries land in LDS FP64 accumulators
-//      (ds_add_f64) and u32 counters; at each label change the LDS partial
-//      is written to slot (workgroup + label);
-//   5. a combine kernel sums, per (g, f), the slots of the workgroups that
-//      touched label g in ascending workgroup order (FP64), then optional
-//      mean/variance.
+//   1. label histograms per workgroup (LDS) + kept-feature words (ballots);
+//      one workgroup scans the label table into slot offsets and builds the
+//      feature map {mask, prefix} per 32-column word (feature -> rank);
+//   2. kept rows scattered by label (counting sort) with, per placed row, a
+//      32-byte range record {index array, value array, entry range} of its
+//      partition and a work weight; entries past a partition's last whole
+//      4-entry unit become tail records keyed by the row's sorted position;
+//   3. nnz-balanced contiguous ranges of the sorted rows, one per workgroup
+//      (one 1024-thread workgroup per CU), from a tile scan of the weights;
+//   4. each workgroup walks its range label segment by label segment; its
+//      waves take rows from an LDS counter and stream them with 16-byte index
+//      / 8-byte value non-temporal vector loads through two register buffers;
+//      kept entries land in LDS FP64 accumulators (ds_add_f64) and u32
+//      counters; tail records of the segment are added; the LDS partial is
+//      stored to slot (feature tile, workgroup + label);
+//   5. a finish kernel sums, per (g, f), the slots of the workgroups that
+//      hold label g, applies the feature scale, and derives mean/variance.
 #include "common.hh"
 
 #include <cstdlib>
+#include <vector>
 
 namespace csamd {
 namespace {
 
 constexpr unsigned int LR_ACC_BYTES = 20u;   // f64 sum + f64 sumsq + u32 nnz per kept feature
+constexpr unsigned int LR_MAX_PARTS = 4096u;
+constexpr unsigned int LR_MAX_TAIL = 3u;     // entries past the last whole 4-entry unit, per partition
 
 struct dev_buf {
     void *p;
@@ -49,14 +59,17 @@ int ensure_buf(dev_buf &b, size_t bytes, const char *label) {
 
 struct label_private {
     dev_buf fpairs;      // uint2 {mask, prefix} per 32-column word
-    dev_buf fids;        // kept feature ids
+    dev_buf fids;        // kept feature ids, then per-word masks
     dev_buf labels;      // label table [(G + 1) x sort workgroups], gstart [G + 1], F
     dev_buf rows;        // sorted kept rows [rows]
-    dev_buf work;        // work weights [rows], tile offsets, ranges [rows]
+    dev_buf work;        // work weights [rows], tile offsets, row range records [rows]
     dev_buf bounds;      // [W + 1]
-    dev_buf partial;     // (W + G) x ftile x {f64, f64, u32}
-    dev_buf tail;        // tail records
+    dev_buf partial;     // tiles x (W + G) x ftile x {f64, f64, u32}
+    dev_buf tail;        // tail records [3 x partitions]
+    dev_buf parts;       // partition descriptors
     dev_buf out;         // outputs
+    void *parts_host;    // pinned staging of the descriptors
+    size_t parts_host_bytes;
     volatile unsigned int *host;  // pinned: [0] kept rows, [1] kept features
     hipEvent_t ev0, ev1, ev_meta;
     int events_ready;
@@ -64,21 +77,117 @@ struct label_private {
     int last_valid;
 };
 
-// One stored entry past the last whole 4-entry unit, resolved for the
-// combine: label g (>= G: contributes nothing), global kept rank f, value.
-struct tail_record {
-    unsigned int g;
-    unsigned int f;
-    double v;
+// ---------------------------------------------------------------------------
+// Partitions.
+// ---------------------------------------------------------------------------
+enum : unsigned int { PART_CSR = 0u, PART_SELL = 1u, PART_BELL = 2u };
+
+struct part_desc {
+    const unsigned int *idx;              // CSR minorIdx / SELL col_idx
+    const void *val;
+    const unsigned int *ptr;              // CSR majorPtr
+    const unsigned int *slice_row_offsets;  // SELL
+    const unsigned int *slice_widths;
+    const unsigned int *slice_slot_offsets;
+    unsigned long long limit;             // stored entries (CSR nnz / SELL slots); SELL: set on the device
+    unsigned long long clip;              // vector loads stream entries below clip
+    unsigned int kind;
+    unsigned int rows;
+    unsigned int row0;                    // first row of the partition in the row arrays
+    unsigned int slice_count;
+    unsigned int slice_rows;
+    unsigned int ell_cols;                // BELL (work weights only)
+    unsigned int last;                    // limit - 1 (scalar-load clamp)
+    unsigned int vec;                     // 1: vector loads (clip = whole units), 0: clip = limit
 };
-constexpr unsigned int LR_MAX_TAIL = 3u;
+
+// Entry range [b, e) of local row r, relative to the partition's arrays.
+__device__ __forceinline__ void part_range(const part_desc &P, unsigned int r, unsigned long long *b, unsigned long long *e) {
+    if (P.kind == PART_CSR) {
+        *b = P.ptr[r];
+        *e = P.ptr[r + 1u];
+    } else if (P.kind == PART_SELL) {
+        if (P.slice_count == 0u) {
+            *b = *e = 0ull;
+            return;
+        }
+        unsigned int slice;
+        if (P.slice_rows == 32u) slice = r >> 5;
+        else if (P.slice_rows != 0u) slice = r / P.slice_rows;
+        else {
+            unsigned int lo = 0u, hi = P.slice_count - 1u;
+            while (lo < hi) {
+                const unsigned int mid = (lo + hi + 1u) >> 1;
+                if (P.slice_row_offsets[mid] <= r) lo = mid;
+                else hi = mid - 1u;
+            }
+            slice = lo;
+        }
+        if (slice >= P.slice_count) slice = P.slice_count - 1u;
+        const unsigned int w = P.slice_widths[slice];
+        const unsigned long long base =
+            (unsigned long long) P.slice_slot_offsets[slice] + (unsigned long long) (r - P.slice_row_offsets[slice]) * w;
+        *b = base;
+        *e = base + w;
+    } else {
+        *b = 0ull;
+        *e = P.ell_cols;
+    }
+}
+
+// Up to LR_ARG_PARTS descriptors travel as kernel arguments (no copy).
+constexpr unsigned int LR_ARG_PARTS = 24u;
+struct part_args {
+    part_desc d[LR_ARG_PARTS];
+};
+
+// Descriptor table on the device (from the kernel arguments when inline),
+// with SELL slot totals, clips and clamps computed here (no host round
+// trip).  Idempotent: a replayed graph recomputes the same table.
+__global__ void part_prepare_kernel(part_desc *__restrict__ parts, unsigned int nparts, part_args inl, unsigned int use_inline) {
+    for (unsigned int p = blockIdx.x * blockDim.x + threadIdx.x; p < nparts; p += gridDim.x * blockDim.x) {
+        part_desc P = use_inline ? inl.d[p] : parts[p];
+        if (P.kind == PART_SELL) {
+            unsigned long long total = 0ull;
+            if (P.slice_count != 0u) {
+                const unsigned int s = P.slice_count - 1u;
+                total = (unsigned long long) P.slice_slot_offsets[s] +
+                        (unsigned long long) (P.rows - P.slice_row_offsets[s]) * P.slice_widths[s];
+            }
+            P.limit = total;
+        }
+        P.clip = P.vec ? (P.limit & ~3ull) : ~0ull;
+        P.last = P.limit != 0ull ? (unsigned int) (P.limit - 1ull) : 0u;
+        parts[p] = P;
+    }
+}
+
+// Per placed row: where its entries are.  32 bytes, read with one scalar
+// load per row by the reduce kernel.
+struct row_range {
+    const unsigned int *idx;
+    const void *val;
+    unsigned int b;     // first entry (partition-relative)
+    unsigned int e;     // one past the last entry streamed (clipped)
+    unsigned int last;  // partition's last entry index (scalar-load clamp)
+    unsigned int part;
+};
+
+// An entry past its partition's last whole 4-entry unit, for the reduce
+// kernel to add when it holds sorted position pos (0xffffffff: none).
+struct tail_record {
+    unsigned int pos;
+    unsigned int col;
+    float v;
+    unsigned int pad;
+};
 
 __device__ __forceinline__ float decode_any(const void *val, unsigned long long i, unsigned int vt) {
     switch (vt) {
-        case CSAMD_VALUE_F16: return f16_bits_to_float(((const uint16_t *) val)[i]);
-        case CSAMD_VALUE_BF16: return bf16_bits_to_float(((const uint16_t *) val)[i]);
+        case CSAMD_VALUE_F16: return value_traits<CSAMD_VALUE_F16>::decode(((const uint16_t *) val)[i]);
+        case CSAMD_VALUE_BF16: return value_traits<CSAMD_VALUE_BF16>::decode(((const uint16_t *) val)[i]);
         case CSAMD_VALUE_F32: return ((const float *) val)[i];
-        default: return fp8_e4m3_to_float(((const uint8_t *) val)[i]);
+        default: return value_traits<CSAMD_VALUE_FP8_E4M3>::decode(((const uint8_t *) val)[i]);
     }
 }
 
@@ -102,19 +211,18 @@ __device__ __forceinline__ unsigned int row_key(const unsigned int *__restrict__
 
 // Also: clears work[0..work_n) (positions past the kept rows keep weight 0
 // for the partition scan); builds the kept-feature mask of each 32-column
-// word; clears the tail records (label G + 1: contributes nothing) that the
-// scatter fills for kept rows.
+// word; clears the tail records.
 __global__ void __launch_bounds__(LR_SORT_THREADS) label_count_kernel(const unsigned int *__restrict__ labels,
                                                                       const unsigned char *__restrict__ row_keep, unsigned int rows,
                                                                       unsigned int G, unsigned int *__restrict__ hist_out,
                                                                       unsigned int *__restrict__ work, unsigned int work_n,
                                                                       const unsigned char *__restrict__ feature_keep, unsigned int cols,
                                                                       unsigned int nwords, unsigned int *__restrict__ word_mask,
-                                                                      tail_record *__restrict__ tail) {
+                                                                      tail_record *__restrict__ tail, unsigned int ntail) {
     extern __shared__ unsigned int hist[];  // [G + 1]
     for (unsigned int g = threadIdx.x; g <= G; g += blockDim.x) hist[g] = 0u;
-    if (tail != nullptr && blockIdx.x == 0u && threadIdx.x < LR_MAX_TAIL)
-        tail[threadIdx.x] = tail_record{G + 1u, CSAMD_INVALID_COL, 0.0};
+    for (unsigned int t = blockIdx.x * blockDim.x + threadIdx.x; t < ntail; t += gridDim.x * blockDim.x)
+        tail[t] = tail_record{0xffffffffu, CSAMD_INVALID_COL, 0.0f, 0u};
     // Feature words, 64 per wave: pass j reads 64 consecutive columns (one
     // coalesced load), a ballot turns them into words 2j and 2j + 1, kept
     // by lanes 2j and 2j + 1.
@@ -208,140 +316,70 @@ __global__ void __launch_bounds__(1024) label_offsets_kernel(unsigned int *__res
     for (unsigned int g = threadIdx.x; g <= G; g += blockDim.x) gstart[g] = hist[(unsigned long long) g * nb];
 }
 
-// ---------------------------------------------------------------------------
-// Per-layout row geometry.
-// ---------------------------------------------------------------------------
-struct csr_geom {
-    const unsigned int *ptr;
-    __device__ __forceinline__ void range(unsigned int row, unsigned long long *b, unsigned long long *e) const {
-        *b = ptr[row];
-        *e = ptr[row + 1u];
-    }
-    // Row holding entry i: the last row r with ptr[r] <= i (rows if none).
-    __device__ __forceinline__ unsigned int row_of(unsigned long long i, unsigned int rows) const {
-        unsigned int lo = 0u, hi = rows;  // first r with ptr[r + 1] > i
-        while (lo < hi) {
-            const unsigned int mid = lo + ((hi - lo) >> 1);
-            if ((unsigned long long) ptr[mid + 1u] <= i) lo = mid + 1u;
-            else hi = mid;
-        }
-        return lo;
-    }
-};
-
-struct sell_geom {
-    csamd_sliced_ell_view s;
-    __device__ __forceinline__ void range(unsigned int row, unsigned long long *b, unsigned long long *e) const {
-        if (s.slice_count == 0u) {
-            *b = *e = 0ull;
-            return;
-        }
-        unsigned int slice;
-        if (s.slice_rows == 32u) slice = row >> 5;
-        else if (s.slice_rows != 0u) slice = row / s.slice_rows;
-        else {
-            unsigned int lo = 0u, hi = s.slice_count - 1u;
-            while (lo < hi) {
-                const unsigned int mid = (lo + hi + 1u) >> 1;
-                if (s.slice_row_offsets[mid] <= row) lo = mid;
-                else hi = mid - 1u;
-            }
-            slice = lo;
-        }
-        if (slice >= s.slice_count) slice = s.slice_count - 1u;
-        const unsigned int w = s.slice_widths[slice];
-        const unsigned long long base =
-            (unsigned long long) s.slice_slot_offsets[slice] + (unsigned long long) (row - s.slice_row_offsets[slice]) * w;
-        *b = base;
-        *e = base + w;
-    }
-    // Row holding slot i (rows if none): its slice, then its row in the slice.
-    __device__ __forceinline__ unsigned int row_of(unsigned long long i, unsigned int rows) const {
-        if (s.slice_count == 0u) return rows;
-        unsigned int lo = 0u, hi = s.slice_count - 1u;  // last slice with slot offset <= i
-        while (lo < hi) {
-            const unsigned int mid = (lo + hi + 1u) >> 1;
-            if ((unsigned long long) s.slice_slot_offsets[mid] <= i) lo = mid;
-            else hi = mid - 1u;
-        }
-        const unsigned int w = s.slice_widths[lo];
-        if (w == 0u) return rows;
-        const unsigned long long r = (unsigned long long) s.slice_row_offsets[lo] + (i - s.slice_slot_offsets[lo]) / w;
-        return r < rows ? (unsigned int) r : rows;
-    }
-};
-
-// Blocked-ELL rows all hold ell_cols slots (ranges are not used).
-struct bell_geom {
-    unsigned int ell_cols;
-    __device__ __forceinline__ void range(unsigned int, unsigned long long *b, unsigned long long *e) const {
-        *b = 0ull;
-        *e = ell_cols;
-    }
-};
-
 // Scatter kept rows to their label's slots: workgroup b's rows of label g
 // start at the scanned table entry (g, b); within a tile, ranks come from
 // LDS atomics, so the order of rows within a label is not fixed (sums are
-// order independent up to FP64 rounding; counts are exact).  For each placed row
-// it also writes its entry range (both layouts index entries with u32),
-// clipped at entry clip -- the vector-load kernel streams only whole 4-entry
-// units; a kept row holding some of the at most 3 entries past the last
-// whole unit writes them as tail records -- and its work weight for the
-// partition (entries + 8 for the per-row cost).
-template<typename Geom>
-__global__ void __launch_bounds__(LR_SORT_THREADS) label_scatter_kernel(Geom geom, const unsigned int *__restrict__ labels,
+// order independent up to FP64 rounding; counts are exact).  For each
+// placed row it writes the row's range record (entries clipped at its
+// partition's clip: the vector-load kernel streams only whole 4-entry
+// units), the row's tail records for entries past the clip, and its work
+// weight for the partition (entries + 8 for the per-row cost).
+__global__ void __launch_bounds__(LR_SORT_THREADS) label_scatter_kernel(const part_desc *__restrict__ parts, unsigned int nparts,
+                                                                        const unsigned int *__restrict__ labels,
                                                                         const unsigned char *__restrict__ row_keep, unsigned int rows,
                                                                         unsigned int G, const unsigned int *__restrict__ base_table,
-                                                                        unsigned long long clip, unsigned int *__restrict__ sorted_rows,
-                                                                        uint2 *__restrict__ ranges, unsigned int *__restrict__ work,
-                                                                        tail_record *__restrict__ tail, const unsigned int *__restrict__ idx,
-                                                                        const void *val, unsigned int vt, unsigned int cols,
-                                                                        const uint2 *__restrict__ fpairs) {
-    extern __shared__ unsigned int hist[];  // [G + 1] tile counts, then [G + 1] running bases
+                                                                        unsigned int *__restrict__ sorted_rows, row_range *__restrict__ ranges,
+                                                                        unsigned int *__restrict__ work, tail_record *__restrict__ tail,
+                                                                        unsigned int vt) {
+    extern __shared__ unsigned int hist[];  // [G + 1] tile counts, [G + 1] running bases, [nparts + 1] first rows
     unsigned int *base = hist + (G + 1u);
+    unsigned int *row0 = base + (G + 1u);
     for (unsigned int g = threadIdx.x; g < G; g += blockDim.x) base[g] = base_table[(unsigned long long) g * gridDim.x + blockIdx.x];
+    for (unsigned int p = threadIdx.x; p <= nparts; p += blockDim.x) row0[p] = p < nparts ? parts[p].row0 : rows;
     for (unsigned int t0 = blockIdx.x * LR_SORT_TILE; t0 < rows; t0 += gridDim.x * LR_SORT_TILE) {
         for (unsigned int g = threadIdx.x; g <= G; g += blockDim.x) hist[g] = 0u;
         __syncthreads();
-        unsigned int key[LR_SORT_ITEMS], rank[LR_SORT_ITEMS];
+        unsigned int key[LR_SORT_ITEMS], rank[LR_SORT_ITEMS], part[LR_SORT_ITEMS];
         unsigned long long rb[LR_SORT_ITEMS], re[LR_SORT_ITEMS];
 #pragma unroll
         for (unsigned int k = 0; k < LR_SORT_ITEMS; ++k) {
             const unsigned int r = t0 + k * LR_SORT_THREADS + threadIdx.x;
             key[k] = r < rows ? row_key(labels, row_keep, r, G) : G;
+            unsigned int lo = 0u, hi = nparts - 1u;  // last partition with row0 <= r
+            while (lo < hi) {
+                const unsigned int mid = (lo + hi + 1u) >> 1;
+                if (row0[mid] <= r) lo = mid;
+                else hi = mid - 1u;
+            }
+            part[k] = lo;
             rb[k] = re[k] = 0ull;
-            if (r < rows) geom.range(r, &rb[k], &re[k]);
+            if (key[k] < G) part_range(parts[lo], r - row0[lo], &rb[k], &re[k]);
         }
 #pragma unroll
         for (unsigned int k = 0; k < LR_SORT_ITEMS; ++k) rank[k] = key[k] < G ? atomicAdd(&hist[key[k]], 1u) : 0u;
-        __syncthreads();
         __syncthreads();
 #pragma unroll
         for (unsigned int k = 0; k < LR_SORT_ITEMS; ++k) {
             if (key[k] >= G) continue;
             const unsigned int r = t0 + k * LR_SORT_THREADS + threadIdx.x;
             const unsigned int pos = base[key[k]] + rank[k];
-            const unsigned long long b = rb[k], e = re[k];
+            const part_desc &P = parts[part[k]];
+            const unsigned long long b = rb[k], e = re[k], clip = P.clip;
             const unsigned long long len = e - b;
             sorted_rows[pos] = r;
-            ranges[pos] = make_uint2((unsigned int) (b < clip ? b : clip), (unsigned int) (e < clip ? e : clip));
+            row_range rr;
+            rr.idx = P.idx;
+            rr.val = P.val;
+            rr.b = (unsigned int) (b < clip ? b : clip);
+            rr.e = (unsigned int) (e < clip ? e : clip);
+            rr.last = P.last;
+            rr.part = part[k];
+            ranges[pos] = rr;
             work[pos] = (unsigned int) (len > 0xfffffff0ull ? 0xfffffff0ull : len) + 8u;
-            if (tail != nullptr && e > clip) {  // this row holds tail entries
-                for (unsigned long long i = b > clip ? b : clip; i < e; ++i) {
-                    const unsigned int c = idx[i];
-                    tail_record rec{G + 1u, CSAMD_INVALID_COL, 0.0};
-                    if (c < cols) {
-                        const uint2 pm = fpairs[c >> 5];
-                        const unsigned int bit = c & 31u;
-                        if ((pm.x >> bit) & 1u) {
-                            rec.g = key[k];
-                            rec.f = pm.y + (unsigned int) __popc(pm.x & ((1u << bit) - 1u));
-                            rec.v = (double) decode_any(val, i, vt);
-                        }
-                    }
-                    tail[i - clip] = rec;
-                }
+            if (tail != nullptr && P.kind != PART_BELL && e > clip) {  // entries past the last whole unit
+                for (unsigned long long i = b > clip ? b : clip; i < e; ++i)
+                    tail[part[k] * LR_MAX_TAIL + (unsigned int) (i - clip)] =
+                        tail_record{pos, P.idx[i], decode_any(P.val, i, vt), 0u};
             }
         }
         __syncthreads();
@@ -403,6 +441,7 @@ struct acc_lds {
     double *sum;
     double *sumsq;
     unsigned int *cnt;
+    const uint2 *lpairs;  // [nwords + 1], {0, 0} sentinel last
 };
 
 struct reduce_args {
@@ -411,9 +450,11 @@ struct reduce_args {
     const unsigned int *bounds;   // [W+1]
     const uint2 *fpairs;          // [nwords]
     const unsigned int *f_total;  // kept features F (device)
+    const tail_record *tail;      // [ntail]
     double *psum;                 // [tiles x (W+G) x ftile]
     double *psumsq;
     unsigned int *pcnt;
+    unsigned int ntail;
     unsigned int G;
     unsigned int cols;
     unsigned int nwords;
@@ -422,11 +463,19 @@ struct reduce_args {
     unsigned int f_count;         // features in this pass (set per pass)
 };
 
+__device__ __forceinline__ void acc_add(const acc_lds &acc, unsigned int f, float v) {
+    const double d = (double) v;
+    atomicAdd(acc.sum + f, d);
+    atomicAdd(acc.sumsq + f, d * d);
+    atomicAdd(acc.cnt + f, 1u);
+}
+
 // The workgroup's sorted-row range [bounds[b], bounds[b + 1]), walked one
 // feature tile (LDS pass) at a time and, within it, one label segment at a
 // time: zero the LDS accumulators, run body(segment start, end, tile args),
-// store the accumulators to partial slot (tile, workgroup + label).  F is
-// read on the device, so the host launches one kernel for all tiles.
+// add the segment's tail records, store the accumulators to partial slot
+// (tile, workgroup + label).  F is read on the device, so the host launches
+// one kernel for all tiles.
 template<typename Body>
 __device__ __forceinline__ void walk_segments(const reduce_args &a, const acc_lds &acc, Body body) {
     const unsigned int lo = a.bounds[blockIdx.x];
@@ -461,6 +510,15 @@ __device__ __forceinline__ void walk_segments(const reduce_args &a, const acc_ld
             }
             body(s0, s1, (const reduce_args &) t);
             __syncthreads();
+            for (unsigned int k = threadIdx.x; k < a.ntail; k += blockDim.x) {
+                const tail_record rec = a.tail[k];
+                if (rec.pos < s0 || rec.pos >= s1 || rec.col >= a.cols || rec.v == 0.0f) continue;
+                const uint2 pm = acc.lpairs[rec.col >> 5];
+                const unsigned int bit = rec.col & 31u;
+                const unsigned int f = pm.y + (unsigned int) __popc(pm.x & ((1u << bit) - 1u)) - t.f0;
+                if (((pm.x >> bit) & 1u) != 0u && f < t.f_count) acc_add(acc, f, rec.v);
+            }
+            __syncthreads();
             const unsigned long long slot = tile * tile_slots + (unsigned long long) (blockIdx.x + g) * a.ftile;
             for (unsigned int f = threadIdx.x; f < t.f_count; f += blockDim.x) {
                 a.psum[slot + f] = acc.sum[f];
@@ -473,21 +531,16 @@ __device__ __forceinline__ void walk_segments(const reduce_args &a, const acc_ld
 }
 
 template<int VT>
-__device__ __forceinline__ void consume(unsigned int c, typename value_traits<VT>::storage raw, const uint2 *__restrict__ lpairs,
-                                        const reduce_args &a, const acc_lds &acc) {
+__device__ __forceinline__ void consume(unsigned int c, typename value_traits<VT>::storage raw, const reduce_args &a,
+                                        const acc_lds &acc) {
     if (c >= a.cols) return;
-    const uint2 pm = lpairs[c >> 5];
+    const uint2 pm = acc.lpairs[c >> 5];
     const unsigned int bit = c & 31u;
     if (((pm.x >> bit) & 1u) == 0u) return;
     const unsigned int f = pm.y + (unsigned int) __popc(pm.x & ((1u << bit) - 1u)) - a.f0;
     if (f >= a.f_count) return;
     const float v = value_traits<VT>::decode(raw);
-    if (v != 0.0f) {
-        const double d = (double) v;
-        atomicAdd(acc.sum + f, d);
-        atomicAdd(acc.sumsq + f, d * d);
-        atomicAdd(acc.cnt + f, 1u);
-    }
+    if (v != 0.0f) acc_add(acc, f, v);
 }
 
 // A unit is 4 consecutive entries, loaded with one non-temporal vector load
@@ -567,11 +620,11 @@ template<int VT> struct chunk_regs {
 // Issue the loads of one chunk: unit u + lane + 64 j of the current row.
 // Lanes past the row's last unit u_end - 1 re-read that unit (same cache
 // lines, masked when consumed), so every lane loads and no lane branches.
-// VEC: row ranges end at or before the last whole 4-entry unit of the
-// arrays, so every unit is one 16-byte index load and one value load, and
-// the loop carries a fixed number of loads per chunk (which lets the
-// compiler wait for the current chunk only).  Otherwise (unaligned arrays)
-// single entries are read, clamped to index last.
+// VEC: row ranges end at or before the last whole 4-entry unit of their
+// partition's arrays, so every unit is one 16-byte index load and one value
+// load, and the loop carries a fixed number of loads per chunk (which lets
+// the compiler wait for the current chunk only).  Otherwise (unaligned
+// arrays) single entries are read, clamped to index last.
 template<int VT, bool VEC>
 __device__ __forceinline__ void load_chunk(const unsigned int *__restrict__ idx, const typename value_traits<VT>::storage *__restrict__ val,
                                            unsigned int u, unsigned int u_end, unsigned int last, chunk_regs<VT> &r) {
@@ -602,19 +655,19 @@ __device__ __forceinline__ unsigned int lane_of(const u32x4 &w, unsigned int k) 
 
 // Consume one chunk of the row whose entries are [b, e), starting at unit u,
 // two units (8 entries per lane) at a time.  Per pair, the 8 bitmap lookups
-// are issued first
-// (independent LDS reads); keep tests, ranks and values are computed
-// without branches; one branch per entry guards the three accumulator
-// atomics.  Nothing here reads global memory, so the prefetched chunk stays
-// in flight (a feature scale is applied by the combine).  lpairs[nwords] is
-// a {0, 0} sentinel, so columns >= cols (and CSAMD_INVALID_COL) are "not
-// kept".  Entries outside [b, e) exist only in a row's first and last
-// chunk; only those chunks (a wave-uniform test) pay for the range mask.
-// Every loaded unit is consumed: a load whose registers are never read is
-// waited for wherever the registers are next written.
+// are issued first (independent LDS reads); keep tests, ranks and values are
+// computed without branches; one branch per entry guards the three
+// accumulator atomics.  Nothing here reads global memory, so the prefetched
+// chunk stays in flight (a feature scale is applied by the finish).
+// lpairs[nwords] is a {0, 0} sentinel, so columns >= cols (and
+// CSAMD_INVALID_COL) are "not kept".  Entries outside [b, e) exist only in a
+// row's first and last chunk; only those chunks (a wave-uniform test) pay
+// for the range mask.  Every loaded unit is consumed: a load whose
+// registers are never read is waited for wherever the registers are next
+// written.
 template<int VT>
 __device__ __forceinline__ void consume_chunk(const chunk_regs<VT> &r, unsigned int u, unsigned int b, unsigned int e,
-                                              const uint2 *__restrict__ lpairs, const reduce_args &a, const acc_lds &acc) {
+                                              const reduce_args &a, const acc_lds &acc) {
     const unsigned long long first = 4ull * u;
     const bool edge = first < b || first + 4ull * LR_CHUNK_UNITS > e;
     const unsigned int lane = lane_id();
@@ -633,26 +686,21 @@ __device__ __forceinline__ void consume_chunk(const chunk_regs<VT> &r, unsigned 
         }
         uint2 pm[8];
 #pragma unroll
-        for (unsigned int k = 0; k < 8u; ++k) pm[k] = lpairs[min(c[k] >> 5, a.nwords)];
+        for (unsigned int k = 0; k < 8u; ++k) pm[k] = acc.lpairs[min(c[k] >> 5, a.nwords)];
 #pragma unroll
         for (unsigned int k = 0; k < 8u; ++k) {
             const unsigned int bit = c[k] & 31u;
             const unsigned int f = pm[k].y + (unsigned int) __popc(pm[k].x & ((1u << bit) - 1u)) - a.f0;
             const float v = unit_vals<VT>::get(r.v[2u * p + (k >> 2)], k & 3u);
             const bool kept = ((pm[k].x >> bit) & 1u) != 0u && f < a.f_count && v != 0.0f;
-            if (kept) {
-                const double d = (double) v;
-                atomicAdd(acc.sum + f, d);
-                atomicAdd(acc.sumsq + f, d * d);
-                atomicAdd(acc.cnt + f, 1u);
-            }
+            if (kept) acc_add(acc, f, v);
         }
     }
 }
 
-__device__ __forceinline__ void row_units(uint2 rng, unsigned int *u, unsigned int *ue) {
-    *u = rng.x >> 2;
-    *ue = rng.x == rng.y ? *u : (unsigned int) (((unsigned long long) rng.y + 3ull) >> 2);
+__device__ __forceinline__ void row_units(unsigned int b, unsigned int e, unsigned int *u, unsigned int *ue) {
+    *u = b >> 2;
+    *ue = b == e ? *u : (unsigned int) (((unsigned long long) e + 3ull) >> 2);
 }
 
 // Next row of the label segment for this wave: one LDS atomic per row, so a
@@ -666,7 +714,7 @@ __device__ __forceinline__ unsigned int grab_row(unsigned int *next) {
 
 // Position of a chunk in the wave's row stream (wave-uniform values).
 struct row_cursor {
-    uint2 rng;        // entry range [x, y) of the row
+    row_range rng;    // the row's arrays and entry range [rng.b, rng.e)
     unsigned int u;   // first unit of the chunk
     unsigned int ue;  // end unit of the row
 };
@@ -675,17 +723,18 @@ struct row_cursor {
 // row counter.  The chunk stream runs through two fixed register buffers in
 // turn (A, B, A, ...): the next chunk, possibly of the next row, is loaded
 // before the current one is consumed, with no register copies between
-// them, and the next row is taken and its range fetched one row ahead.
+// them, and the next row is taken and its range record fetched one row
+// ahead.
 struct row_stream {
-    const uint2 *ranges;
+    const row_range *ranges;
     unsigned int *next;
     unsigned int s1;
     unsigned int inext;  // row taken ahead (>= s1: none left)
-    uint2 nxt;           // its range
+    row_range nxt;       // its range record
 
     __device__ __forceinline__ void take() {
         inext = grab_row(next);
-        nxt = inext < s1 ? ranges[inext] : make_uint2(0u, 0u);
+        if (inext < s1) nxt = ranges[inext];
     }
     // Move to the first non-empty row taken ahead; false when none is left.
     __device__ __forceinline__ bool next_row(row_cursor &c) {
@@ -693,7 +742,7 @@ struct row_stream {
             if (inext >= s1) return false;
             c.rng = nxt;
             take();
-            row_units(c.rng, &c.u, &c.ue);
+            row_units(c.rng.b, c.rng.e, &c.u, &c.ue);
         } while (c.u >= c.ue);
         return true;
     }
@@ -708,11 +757,15 @@ struct row_stream {
 };
 
 template<int VT, bool VEC>
-__device__ __forceinline__ void stream_rows(const unsigned int *__restrict__ idx, const typename value_traits<VT>::storage *__restrict__ val,
-                                            const uint2 *__restrict__ ranges, unsigned int *next, unsigned int s1,
-                                            unsigned int last, const uint2 *__restrict__ lpairs,
+__device__ __forceinline__ void load_cursor(const row_cursor &c, chunk_regs<VT> &r) {
+    using S = typename value_traits<VT>::storage;
+    load_chunk<VT, VEC>(c.rng.idx, (const S *) c.rng.val, c.u, c.ue, c.rng.last, r);
+}
+
+template<int VT, bool VEC>
+__device__ __forceinline__ void stream_rows(const row_range *__restrict__ ranges, unsigned int *next, unsigned int s1,
                                             const reduce_args &a, const acc_lds &acc) {
-    row_stream st{ranges, next, s1, 0u, make_uint2(0u, 0u)};
+    row_stream st{ranges, next, s1, 0u, row_range{}};
     st.take();
     row_cursor ca, cb;
     if (!st.next_row(ca)) return;
@@ -720,41 +773,42 @@ __device__ __forceinline__ void stream_rows(const unsigned int *__restrict__ idx
     // re-loads it, in bounds and unused): a load under a branch makes the
     // compiler merge old and new buffer contents, which waits for the load.
     chunk_regs<VT> A, B;
-    load_chunk<VT, VEC>(idx, val, ca.u, ca.ue, last, A);
+    load_cursor<VT, VEC>(ca, A);
     while (true) {
         cb = ca;
         bool more = st.advance(cb);
         if (!more) cb = ca;
-        load_chunk<VT, VEC>(idx, val, cb.u, cb.ue, last, B);
-        consume_chunk<VT>(A, ca.u, ca.rng.x, ca.rng.y, lpairs, a, acc);
+        load_cursor<VT, VEC>(cb, B);
+        consume_chunk<VT>(A, ca.u, ca.rng.b, ca.rng.e, a, acc);
         if (!more) break;
         ca = cb;
         more = st.advance(ca);
         if (!more) ca = cb;
-        load_chunk<VT, VEC>(idx, val, ca.u, ca.ue, last, A);
-        consume_chunk<VT>(B, cb.u, cb.rng.x, cb.rng.y, lpairs, a, acc);
+        load_cursor<VT, VEC>(ca, A);
+        consume_chunk<VT>(B, cb.u, cb.rng.b, cb.rng.e, a, acc);
         if (!more) break;
     }
 }
 
-template<int VT, bool VEC>
-__global__ void label_reduce_rows_kernel(const unsigned int *__restrict__ idx, const typename value_traits<VT>::storage *__restrict__ val,
-                                         unsigned long long limit, const uint2 *__restrict__ ranges, reduce_args a) {
-    extern __shared__ __align__(16) unsigned char lds[];
-    __shared__ unsigned int next_row;
-    const unsigned int nthreads = blockDim.x;
+__device__ __forceinline__ acc_lds lds_layout(unsigned char *lds, const reduce_args &a) {
     uint2 *lpairs = (uint2 *) lds;
     double *lsum = (double *) (lds + align_up((size_t) (a.nwords + 1u) * sizeof(uint2), 16u));
     double *lsumsq = lsum + a.ftile;
     unsigned int *lcnt = (unsigned int *) (lsumsq + a.ftile);
-    const acc_lds acc{lsum, lsumsq, lcnt};
-    const unsigned int last = limit != 0ull ? (unsigned int) (limit - 1ull) : 0u;
+    return acc_lds{lsum, lsumsq, lcnt, lpairs};
+}
 
-    for (unsigned int w = threadIdx.x; w <= a.nwords; w += nthreads) lpairs[w] = w < a.nwords ? a.fpairs[w] : make_uint2(0u, 0u);
+template<int VT, bool VEC>
+__global__ void label_reduce_rows_kernel(const row_range *__restrict__ ranges, reduce_args a) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    __shared__ unsigned int next_row;
+    const acc_lds acc = lds_layout(lds, a);
+    uint2 *lpairs = (uint2 *) acc.lpairs;
+    for (unsigned int w = threadIdx.x; w <= a.nwords; w += blockDim.x) lpairs[w] = w < a.nwords ? a.fpairs[w] : make_uint2(0u, 0u);
     walk_segments(a, acc, [&](unsigned int s0, unsigned int s1, const reduce_args &t) {
         if (threadIdx.x == 0u) next_row = s0;
         __syncthreads();  // also publishes the zeroed accumulators
-        stream_rows<VT, VEC>(idx, val, ranges, &next_row, s1, last, lpairs, t, acc);
+        stream_rows<VT, VEC>(ranges, &next_row, s1, t, acc);
     });
 }
 
@@ -763,19 +817,15 @@ template<int VT>
 __global__ void label_reduce_blocked_kernel(csamd_blocked_ell_view src, reduce_args a) {
     using S = typename value_traits<VT>::storage;
     extern __shared__ __align__(16) unsigned char lds[];
-    const unsigned int nthreads = blockDim.x;
-    const unsigned int nwaves = nthreads >> 6;
+    const unsigned int nwaves = blockDim.x >> 6;
     const unsigned int wave = threadIdx.x >> 6;
-    uint2 *lpairs = (uint2 *) lds;
-    double *lsum = (double *) (lds + align_up((size_t) (a.nwords + 1u) * sizeof(uint2), 16u));
-    double *lsumsq = lsum + a.ftile;
-    unsigned int *lcnt = (unsigned int *) (lsumsq + a.ftile);
-    const acc_lds acc{lsum, lsumsq, lcnt};
+    const acc_lds acc = lds_layout(lds, a);
+    uint2 *lpairs = (uint2 *) acc.lpairs;
     const S *__restrict__ val = (const S *) src.val;
     const unsigned int bs = src.block_size;
     const unsigned int width = bs != 0u ? src.ell_cols / bs : 0u;
 
-    for (unsigned int w = threadIdx.x; w < a.nwords; w += nthreads) lpairs[w] = a.fpairs[w];
+    for (unsigned int w = threadIdx.x; w <= a.nwords; w += blockDim.x) lpairs[w] = w < a.nwords ? a.fpairs[w] : make_uint2(0u, 0u);
     walk_segments(a, acc, [&](unsigned int s0, unsigned int s1, const reduce_args &t) {
         __syncthreads();  // publishes the zeroed accumulators
         for (unsigned int i = s0 + wave; i < s1; i += nwaves) {
@@ -786,25 +836,24 @@ __global__ void label_reduce_blocked_kernel(csamd_blocked_ell_view src, reduce_a
                 const unsigned int slot = e / bs;
                 const unsigned int bc = src.blockColIdx[(unsigned long long) rb * width + slot];
                 const unsigned int c = bc != CSAMD_INVALID_COL ? bc * bs + (e - slot * bs) : CSAMD_INVALID_COL;
-                consume<VT>(c, val[base + e], lpairs, t, acc);
+                consume<VT>(c, val[base + e], t, acc);
             }
         }
     });
 }
 
 // Finish: out[g][f] = sum, over the workgroups w whose range holds rows of
-// label g (ascending w), of partial slot (tile of f, w + g); then the tail
-// records; then the feature's scale s (if any): s * sum, s^2 * sumsq, no
-// nonzeros when s == 0; then mean = sum / n_g and var = sumsq / n_g - mean^2
-// (clamped at 0), n_g = kept rows of label g.  F is read on the device;
-// gstart and bounds are staged in LDS for the per-thread search.
+// label g (ascending w), of partial slot (tile of f, w + g); then the
+// feature's scale s (if any): s * sum, s^2 * sumsq, no nonzeros when s == 0;
+// then mean = sum / n_g and var = sumsq / n_g - mean^2 (clamped at 0),
+// n_g = kept rows of label g.  F is read on the device; gstart and bounds
+// are staged in LDS for the per-thread search.
 __global__ void __launch_bounds__(256) label_finish_kernel(
     const double *__restrict__ psum, const double *__restrict__ psumsq, const unsigned int *__restrict__ pcnt,
     const unsigned int *__restrict__ gstart_g, const unsigned int *__restrict__ bounds_g, unsigned int W, unsigned int G,
     unsigned int ftile, const unsigned int *__restrict__ f_total, const float *__restrict__ scale,
-    const unsigned int *__restrict__ fids, const tail_record *__restrict__ tail, double *__restrict__ out_sum,
-    double *__restrict__ out_sumsq, unsigned int *__restrict__ out_cnt, double *__restrict__ mean, double *__restrict__ var,
-    unsigned int *__restrict__ group_rows) {
+    const unsigned int *__restrict__ fids, double *__restrict__ out_sum, double *__restrict__ out_sumsq,
+    unsigned int *__restrict__ out_cnt, double *__restrict__ mean, double *__restrict__ var, unsigned int *__restrict__ group_rows) {
     extern __shared__ unsigned int meta[];  // gstart[G + 1], bounds[W + 1]
     unsigned int *gstart = meta;
     unsigned int *bounds = meta + G + 1u;
@@ -852,15 +901,6 @@ __global__ void __launch_bounds__(256) label_finish_kernel(
                 }
             }
         }
-        if (tail != nullptr) {
-#pragma unroll
-            for (unsigned int k = 0; k < LR_MAX_TAIL; ++k)
-                if (tail[k].g == g && tail[k].f == f && tail[k].v != 0.0) {
-                    sm += tail[k].v;
-                    q += tail[k].v * tail[k].v;
-                    n += 1u;
-                }
-        }
         if (scale != nullptr) {
             const double sc = (double) scale[fids[f]];
             sm *= sc;
@@ -897,36 +937,35 @@ label_private *private_of(csamd_label_reduce_workspace *ws) {
     return (label_private *) ws->private_workspace;
 }
 
-struct layout_in {
-    int kind;  // 0 CSR, 1 SELL, 2 BELL
-    unsigned int rows, cols;
+struct part_in {
+    unsigned int kind;  // PART_*
+    unsigned int rows;
+    unsigned int cols;
     const unsigned int *idx;
     const void *val;
-    unsigned long long limit;
-    csr_geom csr;
-    sell_geom sell;
+    unsigned long long nnz;  // CSR stored entries
+    const unsigned int *ptr;
+    csamd_sliced_ell_view sell;
     csamd_blocked_ell_view bell;
 };
 
 template<int VT>
-int launch_rows(const layout_in &L, unsigned int grid, unsigned int threads, size_t lds, hipStream_t s, const reduce_args &a,
-                bool vec, const uint2 *ranges) {
-    using S = typename value_traits<VT>::storage;
-    const S *val = (const S *) L.val;
-    if (L.kind != 2) {
+int launch_rows(bool bell, const csamd_blocked_ell_view &bv, unsigned int grid, unsigned int threads, size_t lds, hipStream_t s,
+                const reduce_args &a, bool vec, const row_range *ranges, bool set_attr) {
+    if (!bell) {
         if (vec) {
             auto k = label_reduce_rows_kernel<VT, true>;
-            CSAMD_HIP(hipFuncSetAttribute((const void *) k, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds), "set lds");
-            k<<<grid, threads, lds, s>>>(L.idx, val, L.limit, ranges, a);
+            if (set_attr) CSAMD_HIP(hipFuncSetAttribute((const void *) k, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds), "set lds");
+            else k<<<grid, threads, lds, s>>>(ranges, a);
         } else {
             auto k = label_reduce_rows_kernel<VT, false>;
-            CSAMD_HIP(hipFuncSetAttribute((const void *) k, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds), "set lds");
-            k<<<grid, threads, lds, s>>>(L.idx, val, L.limit, ranges, a);
+            if (set_attr) CSAMD_HIP(hipFuncSetAttribute((const void *) k, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds), "set lds");
+            else k<<<grid, threads, lds, s>>>(ranges, a);
         }
     } else {
         auto k = label_reduce_blocked_kernel<VT>;
-        CSAMD_HIP(hipFuncSetAttribute((const void *) k, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds), "set lds");
-        k<<<grid, threads, lds, s>>>(L.bell, a);
+        if (set_attr) CSAMD_HIP(hipFuncSetAttribute((const void *) k, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds), "set lds");
+        else k<<<grid, threads, lds, s>>>(bv, a);
     }
     return hip_ok(hipGetLastError(), "label reduce kernel");
 }
@@ -938,22 +977,68 @@ int max_lds_per_block() {
     return v;
 }
 
-int run_label_reduce(const layout_in &L, unsigned int value_type, const csamd_label_reduce_query *q,
-                     csamd_label_reduce_workspace *ws, csamd_label_reduce_result *out) {
-    CSAMD_REQUIRE(q != nullptr && ws != nullptr && out != nullptr, "label_reduce: null argument");
+// Everything one call launches, decided and allocated before any launch
+// (so that the launches can be captured into a graph).
+struct lr_plan {
+    unsigned int rows, cols, G, nwords, nparts, sort_grid, tiles, ntail, W, threads, fmax, vt, cu;
+    bool bell, aligned, inline_parts, moments;
+    size_t lds, hist_lds, scatter_lds, meta_lds, parts_bytes;
+    const unsigned int *labels;
+    const unsigned char *row_keep, *feature_keep;
+    const float *scale;
+    unsigned int *table, *gstart, *f_dev, *sorted, *work, *word_mask, *fids, *bounds;
+    unsigned long long *toff;
+    row_range *ranges;
+    uint2 *fpairs;
+    tail_record *tail;
+    part_desc *d_parts;
+    double *out_sum, *out_sumsq, *out_mean, *out_var;
+    unsigned int *out_cnt, *out_grows;
+    reduce_args a;
+    csamd_blocked_ell_view bv;
+    part_args inl;
+};
+
+int plan_label_reduce(const std::vector<part_in> &parts, unsigned int value_type, const csamd_label_reduce_query *q,
+                      label_private *P, lr_plan &L) {
+    CSAMD_REQUIRE(!parts.empty() && parts.size() <= LR_MAX_PARTS, "label_reduce: 1..%u partitions", LR_MAX_PARTS);
     CSAMD_REQUIRE(q->group_count >= 1u && q->group_count <= CSAMD_MAX_LABEL_GROUPS, "label_reduce: group_count %u out of range",
                   q->group_count);
     CSAMD_REQUIRE(value_type_decodable(value_type), "label_reduce: unsupported value type %u", value_type);
-    CSAMD_REQUIRE(L.rows == 0u || q->row_labels != nullptr, "label_reduce: row_labels is null");
-    CSAMD_REQUIRE(L.kind != 2 || (unsigned long long) L.rows * (L.bell.ell_cols + 8ull) < (1ull << 48),
+    std::memset(&L, 0, sizeof(L));
+    const unsigned int nparts = (unsigned int) parts.size();
+    const bool bell = parts[0].kind == PART_BELL;
+    CSAMD_REQUIRE(!bell || nparts == 1u, "label_reduce: a blocked-ELL input is a single partition");
+    const unsigned int cols = parts[0].cols;
+    unsigned long long rows64 = 0ull;
+    bool aligned = !bell;
+    const uintptr_t val_align = value_type == CSAMD_VALUE_F32 ? 16u : value_type == CSAMD_VALUE_FP8_E4M3 ? 4u : 8u;
+    for (const part_in &pi : parts) {
+        CSAMD_REQUIRE(pi.cols == cols, "label_reduce: partitions must have the same column count");
+        rows64 += pi.rows;
+        aligned = aligned && (uintptr_t) pi.idx % 16u == 0u && (uintptr_t) pi.val % val_align == 0u;
+    }
+    CSAMD_REQUIRE(rows64 < 0xffffffffull, "label_reduce: more than 2^32 - 1 rows");
+    CSAMD_REQUIRE(!bell || (unsigned long long) parts[0].rows * (parts[0].bell.ell_cols + 8ull) < (1ull << 48),
                   "label_reduce: blocked-ELL has too many slots");
-    CSAMD_HIP(hipSetDevice(ws->device >= 0 ? ws->device : 0), "hipSetDevice label reduce");
-    label_private *P = private_of(ws);
-    CSAMD_REQUIRE(P != nullptr, "label_reduce: host allocation failed");
-    hipStream_t s = (hipStream_t) ws->stream;
+    const unsigned int rows = (unsigned int) rows64;
+    CSAMD_REQUIRE(rows == 0u || q->row_labels != nullptr, "label_reduce: row_labels is null");
     const unsigned int G = q->group_count;
-    const unsigned int rows = L.rows, cols = L.cols;
     const unsigned int nwords = (cols + 31u) / 32u;
+    L.rows = rows;
+    L.cols = cols;
+    L.G = G;
+    L.nwords = nwords;
+    L.nparts = nparts;
+    L.vt = value_type;
+    L.bell = bell;
+    L.aligned = aligned;
+    L.labels = q->row_labels;
+    L.row_keep = q->row_keep;
+    L.feature_keep = q->feature_keep;
+    L.scale = q->feature_scale;
+    L.moments = q->compute_moments != 0u;
+    if (bell) L.bv = parts[0].bell;
 
     if (!P->events_ready) {
         CSAMD_HIP(hipEventCreate(&P->ev0), "hipEventCreate");
@@ -988,113 +1073,141 @@ int run_label_reduce(const layout_in &L, unsigned int value_type, const csamd_la
     unsigned int per_cu = env_u("CSAMD_LR_PER_CU", 1u);
     unsigned int split = env_u("CSAMD_LR_SPLIT", 1u);
     if (split < 1u || split > 16u) split = 1u;
+    L.cu = cu;
+    L.threads = threads;
 
-    // Buffers sized by what the host knows (rows, cols, G); the kept counts
-    // stay on the device until the feature count is needed.
+    // Buffers sized by what the host knows (rows, cols, G, partitions); the
+    // kept counts stay on the device.
     const unsigned int tiles = rows == 0u ? 1u : (rows + CSAMD_SCAN_TILE - 1u) / CSAMD_SCAN_TILE;
     unsigned int sort_cap = 2u * cu;  // label table (G + 1) x workgroups stays small enough for one scanning workgroup
     if (sort_cap > 262144u / (G + 1u)) sort_cap = 262144u / (G + 1u);
     if (sort_cap < 8u) sort_cap = 8u;
     const unsigned int sort_grid = grid_for(rows == 0u ? 1u : rows, LR_SORT_TILE, sort_cap);
     const size_t table_n = (size_t) (G + 1u) * sort_grid;
+    const unsigned int ntail = aligned ? nparts * LR_MAX_TAIL : 0u;
+    L.tiles = tiles;
+    L.sort_grid = sort_grid;
+    L.ntail = ntail;
+    L.parts_bytes = (size_t) nparts * sizeof(part_desc);
     if (!ensure_buf(P->fpairs, (size_t) (nwords + 1u) * sizeof(uint2), "hipMalloc fpairs")) return 0;
     if (!ensure_buf(P->fids, ((size_t) cols + 16u + nwords + 1u) * sizeof(unsigned int), "hipMalloc fids")) return 0;
     if (!ensure_buf(P->labels, (table_n + (G + 1u) + 1u) * sizeof(unsigned int), "hipMalloc label table")) return 0;
     if (!ensure_buf(P->rows, (size_t) (rows + 1u) * sizeof(unsigned int), "hipMalloc sorted rows")) return 0;
     const size_t work_bytes = align_up((size_t) (rows + 1u) * sizeof(unsigned int), 256u);
     const size_t toff_bytes = align_up((size_t) (tiles + 1u) * sizeof(unsigned long long), 256u);
-    if (!ensure_buf(P->work, work_bytes + toff_bytes + (size_t) (rows + 1u) * sizeof(uint2), "hipMalloc work")) return 0;
-    if (!ensure_buf(P->tail, LR_MAX_TAIL * sizeof(tail_record), "hipMalloc tail records")) return 0;
-    unsigned int *table = (unsigned int *) P->labels.p;  // [(G + 1) x sort_grid]
-    unsigned int *gstart = table + table_n;               // [G + 1]
-    unsigned int *f_dev = gstart + (G + 1u);              // [1]
-    unsigned int *sorted = (unsigned int *) P->rows.p;
-    unsigned int *work = (unsigned int *) P->work.p;
-    unsigned long long *toff = (unsigned long long *) ((char *) P->work.p + work_bytes);
-    uint2 *ranges = (uint2 *) ((char *) P->work.p + work_bytes + toff_bytes);
-    unsigned int *fids = (unsigned int *) P->fids.p;
-    uint2 *fpairs = (uint2 *) P->fpairs.p;
+    if (!ensure_buf(P->work, work_bytes + toff_bytes + (size_t) (rows + 1u) * sizeof(row_range), "hipMalloc work")) return 0;
+    if (!ensure_buf(P->tail, ((size_t) nparts * LR_MAX_TAIL) * sizeof(tail_record), "hipMalloc tail records")) return 0;
+    if (!ensure_buf(P->parts, L.parts_bytes, "hipMalloc label partitions")) return 0;
+    L.table = (unsigned int *) P->labels.p;  // [(G + 1) x sort_grid]
+    L.gstart = L.table + table_n;             // [G + 1]
+    L.f_dev = L.gstart + (G + 1u);            // [1], next to gstart[G]
+    L.sorted = (unsigned int *) P->rows.p;
+    L.work = (unsigned int *) P->work.p;
+    L.toff = (unsigned long long *) ((char *) P->work.p + work_bytes);
+    L.ranges = (row_range *) ((char *) P->work.p + work_bytes + toff_bytes);
+    L.fids = (unsigned int *) P->fids.p;
+    L.word_mask = L.fids + (cols + 16u);  // past the kept-feature ids
+    L.fpairs = (uint2 *) P->fpairs.p;
+    L.tail = ntail != 0u ? (tail_record *) P->tail.p : nullptr;
+    L.d_parts = (part_desc *) P->parts.p;
 
-    // Vector loads need 16-byte aligned indices and unit-aligned values; the
-    // vector kernel streams whole 4-entry units only (ranges clipped at
-    // clip) and the entries past clip become tail records.
-    const bool aligned = L.kind != 2 && ((uintptr_t) L.idx % 16u == 0u) &&
-                         ((uintptr_t) L.val % (uintptr_t) (value_type == CSAMD_VALUE_F32 ? 16u
-                                                           : value_type == CSAMD_VALUE_FP8_E4M3 ? 4u : 8u) == 0u);
-    const unsigned long long clip = aligned ? (L.limit & ~3ull) : ~0ull;
-    tail_record *tail = aligned && clip < L.limit ? (tail_record *) P->tail.p : nullptr;
-
-    // 1. label histograms and feature words; 2. label offsets and feature map
-    const size_t hist_lds = (size_t) (G + 1u) * sizeof(unsigned int);
-    unsigned int *word_mask = (unsigned int *) P->fids.p + (cols + 16u);  // past the kept-feature ids
-    label_count_kernel<<<sort_grid, LR_SORT_THREADS, hist_lds, s>>>(q->row_labels, q->row_keep, rows, G, table, work, rows + 1u,
-                                                                    q->feature_keep, cols, nwords, word_mask, tail);
-    CSAMD_HIP(hipGetLastError(), "label_count_kernel");
-    label_offsets_kernel<<<1, 1024, 0, s>>>(table, G, sort_grid, gstart, word_mask, nwords, fpairs, fids, f_dev);
-    CSAMD_HIP(hipGetLastError(), "label_offsets_kernel");
-    // gstart[G] (kept rows) and F sit next to each other: one copy to pinned
-    // memory, waited for at the end of the call (long done by then)
-    CSAMD_HIP(hipMemcpyAsync((void *) P->host, gstart + G, 2u * sizeof(unsigned int), hipMemcpyDeviceToHost, s), "D2H counts");
-    CSAMD_HIP(hipEventRecord(P->ev_meta, s), "hipEventRecord");
-    if (rows != 0u) {
-        if (L.kind == 0)
-            label_scatter_kernel<csr_geom><<<sort_grid, LR_SORT_THREADS, 2u * hist_lds, s>>>(
-                L.csr, q->row_labels, q->row_keep, rows, G, table, clip, sorted, ranges, work, tail, L.idx, L.val, value_type,
-                cols, fpairs);
-        else if (L.kind == 1)
-            label_scatter_kernel<sell_geom><<<sort_grid, LR_SORT_THREADS, 2u * hist_lds, s>>>(
-                L.sell, q->row_labels, q->row_keep, rows, G, table, clip, sorted, ranges, work, tail, L.idx, L.val, value_type,
-                cols, fpairs);
-        else
-            label_scatter_kernel<bell_geom><<<sort_grid, LR_SORT_THREADS, 2u * hist_lds, s>>>(
-                bell_geom{L.bell.ell_cols}, q->row_labels, q->row_keep, rows, G, table, clip, sorted, ranges, work, nullptr,
-                nullptr, nullptr, value_type, cols, fpairs);
-        CSAMD_HIP(hipGetLastError(), "label_scatter_kernel");
+    // partition descriptors: small tables go to the device as kernel
+    // arguments, larger ones through pinned staging
+    std::vector<part_desc> descs(nparts);
+    {
+        unsigned int row0 = 0u;
+        for (unsigned int p = 0; p < nparts; ++p) {
+            const part_in &pi = parts[p];
+            part_desc &d = descs[p];
+            std::memset(&d, 0, sizeof(d));
+            d.idx = pi.idx;
+            d.val = pi.val;
+            d.ptr = pi.ptr;
+            d.kind = pi.kind;
+            d.rows = pi.rows;
+            d.row0 = row0;
+            d.limit = pi.nnz;
+            d.vec = aligned ? 1u : 0u;
+            if (pi.kind == PART_SELL) {
+                d.idx = pi.sell.col_idx;
+                d.val = pi.sell.val;
+                d.slice_row_offsets = pi.sell.slice_row_offsets;
+                d.slice_widths = pi.sell.slice_widths;
+                d.slice_slot_offsets = pi.sell.slice_slot_offsets;
+                d.slice_count = pi.sell.slice_count;
+                d.slice_rows = pi.sell.slice_rows;
+            } else if (pi.kind == PART_BELL) {
+                d.ell_cols = pi.bell.ell_cols;
+            }
+            row0 += pi.rows;
+        }
+    }
+    L.inline_parts = nparts <= LR_ARG_PARTS;
+    if (L.inline_parts) {
+        for (unsigned int p = 0; p < nparts; ++p) L.inl.d[p] = descs[p];
+    } else {
+        if (P->parts_host_bytes < L.parts_bytes) {
+            if (P->parts_host != nullptr) {
+                CSAMD_HIP(hipEventSynchronize(P->ev_meta), "label reduce sync (descriptor staging)");
+                (void) hipHostFree(P->parts_host);
+            }
+            P->parts_host = nullptr;
+            P->parts_host_bytes = 0u;
+            CSAMD_HIP(hipHostMalloc(&P->parts_host, L.parts_bytes, hipHostMallocDefault), "hipHostMalloc label partitions");
+            P->parts_host_bytes = L.parts_bytes;
+        } else {
+            // the previous call's upload has left the staging once its counts copy is done
+            CSAMD_HIP(hipEventSynchronize(P->ev_meta), "label reduce sync (descriptor staging)");
+        }
+        std::memcpy(P->parts_host, descs.data(), L.parts_bytes);
     }
 
-    // 3. launch shape: the LDS feature tile is sized for all columns (at most
-    // F of them are kept; the device reads F), so no host wait is needed here
+    // the LDS feature tile is sized for all columns (at most F of them are
+    // kept; the device reads F), so no host wait is needed
     const unsigned int fmax = cols == 0u ? 1u : cols;
     if (per_cu != 2u || fixed_lds + 64u * LR_ACC_BYTES > lds_avail / 2u) per_cu = 1u;
     const size_t lds_budget = per_cu == 2u ? lds_avail / 2u : lds_avail;
     unsigned int ftile = (unsigned int) ((lds_budget - fixed_lds) / LR_ACC_BYTES);
     if (ftile >= fmax) ftile = fmax;
     else ftile &= ~63u;
-    const unsigned int ntiles = (fmax + ftile - 1u) / ftile;
+    const unsigned int nftiles = (fmax + ftile - 1u) / ftile;
     const unsigned int W = cu * per_cu * split;
-    const size_t lds = fixed_lds + (size_t) ftile * LR_ACC_BYTES;
+    L.fmax = fmax;
+    L.W = W;
+    L.lds = fixed_lds + (size_t) ftile * LR_ACC_BYTES;
+    L.hist_lds = (size_t) (G + 1u) * sizeof(unsigned int);
+    L.scatter_lds = (2u * (size_t) (G + 1u) + nparts + 1u) * sizeof(unsigned int);
+    L.meta_lds = (size_t) (G + 1u + W + 1u) * sizeof(unsigned int);
     if (!ensure_buf(P->bounds, (size_t) (W + 1u) * sizeof(unsigned int), "hipMalloc bounds")) return 0;
-    unsigned int *bounds = (unsigned int *) P->bounds.p;
+    L.bounds = (unsigned int *) P->bounds.p;
 
-    // 4. partition bounds from the work weights (positions past the kept
-    // rows weigh 0)
-    if (!scan_tile_offsets_u32(work, rows, toff, s)) return 0;
-    work_bounds_kernel<<<tiles, 256, 0, s>>>(work, rows, toff, tiles, W, bounds);
-    CSAMD_HIP(hipGetLastError(), "work_bounds_kernel");
-
-    // 5. outputs [G x F] (room for F = cols) and partial slots
+    // outputs [G x F] (room for F = cols) and partial slots
     const size_t GF = (size_t) G * fmax;
     const size_t o_sum = 0, o_sumsq = align_up(o_sum + GF * 8u, 256u), o_mean = align_up(o_sumsq + GF * 8u, 256u),
                  o_var = align_up(o_mean + GF * 8u, 256u), o_cnt = align_up(o_var + GF * 8u, 256u),
                  o_grows = align_up(o_cnt + GF * 4u, 256u), o_end = align_up(o_grows + (size_t) G * 4u, 256u);
     if (!ensure_buf(P->out, o_end, "hipMalloc label outputs")) return 0;
     char *ob = (char *) P->out.p;
-    double *out_sum = (double *) (ob + o_sum), *out_sumsq = (double *) (ob + o_sumsq);
-    double *out_mean = (double *) (ob + o_mean), *out_var = (double *) (ob + o_var);
-    unsigned int *out_cnt = (unsigned int *) (ob + o_cnt), *out_grows = (unsigned int *) (ob + o_grows);
-    const size_t slots = (size_t) ntiles * ((size_t) W + G) * ftile;
+    L.out_sum = (double *) (ob + o_sum);
+    L.out_sumsq = (double *) (ob + o_sumsq);
+    L.out_mean = (double *) (ob + o_mean);
+    L.out_var = (double *) (ob + o_var);
+    L.out_cnt = (unsigned int *) (ob + o_cnt);
+    L.out_grows = (unsigned int *) (ob + o_grows);
+    const size_t slots = (size_t) nftiles * ((size_t) W + G) * ftile;
     const size_t p_sum = 0, p_sumsq = align_up(slots * 8u, 256u), p_cnt = align_up(p_sumsq + slots * 8u, 256u),
                  p_end = p_cnt + slots * 4u;
     if (!ensure_buf(P->partial, p_end, "hipMalloc label partials")) return 0;
     char *pb = (char *) P->partial.p;
 
-    // 6. one reduce launch for all feature tiles, then the finish
-    reduce_args a;
-    a.sorted_rows = sorted;
-    a.gstart = gstart;
-    a.bounds = bounds;
-    a.fpairs = fpairs;
-    a.f_total = f_dev;
+    reduce_args &a = L.a;
+    a.sorted_rows = L.sorted;
+    a.gstart = L.gstart;
+    a.bounds = L.bounds;
+    a.fpairs = L.fpairs;
+    a.f_total = L.f_dev;
+    a.tail = L.tail;
+    a.ntail = L.tail != nullptr ? ntail : 0u;
     a.psum = (double *) (pb + p_sum);
     a.psumsq = (double *) (pb + p_sumsq);
     a.pcnt = (unsigned int *) (pb + p_cnt);
@@ -1104,16 +1217,64 @@ int run_label_reduce(const layout_in &L, unsigned int value_type, const csamd_la
     a.ftile = ftile;
     a.f0 = 0u;
     a.f_count = 0u;
-    P->last_valid = 0;
+
+    // kernel attributes are set here, outside any stream capture
     if (rows != 0u && cols != 0u) {
-        const bool timed = P->timing != 0;
-        if (timed) CSAMD_HIP(hipEventRecord(P->ev0, s), "hipEventRecord");
         int ok = 0;
         switch (value_type) {
-            case CSAMD_VALUE_F16: ok = launch_rows<CSAMD_VALUE_F16>(L, W, threads, lds, s, a, aligned, ranges); break;
-            case CSAMD_VALUE_BF16: ok = launch_rows<CSAMD_VALUE_BF16>(L, W, threads, lds, s, a, aligned, ranges); break;
-            case CSAMD_VALUE_F32: ok = launch_rows<CSAMD_VALUE_F32>(L, W, threads, lds, s, a, aligned, ranges); break;
-            case CSAMD_VALUE_FP8_E4M3: ok = launch_rows<CSAMD_VALUE_FP8_E4M3>(L, W, threads, lds, s, a, aligned, ranges); break;
+            case CSAMD_VALUE_F16: ok = launch_rows<CSAMD_VALUE_F16>(bell, L.bv, W, threads, L.lds, nullptr, a, aligned, L.ranges, true); break;
+            case CSAMD_VALUE_BF16: ok = launch_rows<CSAMD_VALUE_BF16>(bell, L.bv, W, threads, L.lds, nullptr, a, aligned, L.ranges, true); break;
+            case CSAMD_VALUE_F32: ok = launch_rows<CSAMD_VALUE_F32>(bell, L.bv, W, threads, L.lds, nullptr, a, aligned, L.ranges, true); break;
+            case CSAMD_VALUE_FP8_E4M3: ok = launch_rows<CSAMD_VALUE_FP8_E4M3>(bell, L.bv, W, threads, L.lds, nullptr, a, aligned, L.ranges, true); break;
+        }
+        if (!ok) return 0;
+    }
+    return 1;
+}
+
+// The call's launches, in stream order; no allocation, no host wait, no
+// memset.  upload: copy the staged descriptor table first (large tables;
+// graphs upload theirs once, before capture).  eager: also record the
+// counts copy / timing events.
+int enqueue_label_reduce(const lr_plan &L, label_private *P, hipStream_t s, bool upload, bool eager) {
+    if (!L.inline_parts && upload)
+        CSAMD_HIP(hipMemcpyAsync(L.d_parts, P->parts_host, L.parts_bytes, hipMemcpyHostToDevice, s), "H2D label partitions");
+    part_prepare_kernel<<<grid_for(L.nparts, 256u, 64u), 256, 0, s>>>(L.d_parts, L.nparts, L.inl, L.inline_parts ? 1u : 0u);
+    CSAMD_HIP(hipGetLastError(), "part_prepare_kernel");
+    // 1. label histograms and feature words; 2. label offsets and feature map
+    label_count_kernel<<<L.sort_grid, LR_SORT_THREADS, L.hist_lds, s>>>(L.labels, L.row_keep, L.rows, L.G, L.table, L.work,
+                                                                        L.rows + 1u, L.feature_keep, L.cols, L.nwords, L.word_mask,
+                                                                        L.tail, L.ntail);
+    CSAMD_HIP(hipGetLastError(), "label_count_kernel");
+    label_offsets_kernel<<<1, 1024, 0, s>>>(L.table, L.G, L.sort_grid, L.gstart, L.word_mask, L.nwords, L.fpairs, L.fids, L.f_dev);
+    CSAMD_HIP(hipGetLastError(), "label_offsets_kernel");
+    if (eager) {
+        // gstart[G] (kept rows) and F sit next to each other: one copy to
+        // pinned memory, waited for at the end of the call (long done by then)
+        CSAMD_HIP(hipMemcpyAsync((void *) P->host, L.gstart + L.G, 2u * sizeof(unsigned int), hipMemcpyDeviceToHost, s), "D2H counts");
+        CSAMD_HIP(hipEventRecord(P->ev_meta, s), "hipEventRecord");
+    }
+    // 3. kept rows grouped by label, with range records, tails and work
+    if (L.rows != 0u) {
+        label_scatter_kernel<<<L.sort_grid, LR_SORT_THREADS, L.scatter_lds, s>>>(L.d_parts, L.nparts, L.labels, L.row_keep, L.rows,
+                                                                                 L.G, L.table, L.sorted, L.ranges, L.work, L.tail, L.vt);
+        CSAMD_HIP(hipGetLastError(), "label_scatter_kernel");
+    }
+    // 4. workgroup ranges from the work weights (positions past the kept rows weigh 0)
+    if (!scan_tile_offsets_u32(L.work, L.rows, L.toff, s)) return 0;
+    work_bounds_kernel<<<L.tiles, 256, 0, s>>>(L.work, L.rows, L.toff, L.tiles, L.W, L.bounds);
+    CSAMD_HIP(hipGetLastError(), "work_bounds_kernel");
+    // 5. one reduce launch for all feature tiles and partitions, then the finish
+    if (eager) P->last_valid = 0;
+    if (L.rows != 0u && L.cols != 0u) {
+        const bool timed = eager && P->timing != 0;
+        if (timed) CSAMD_HIP(hipEventRecord(P->ev0, s), "hipEventRecord");
+        int ok = 0;
+        switch (L.vt) {
+            case CSAMD_VALUE_F16: ok = launch_rows<CSAMD_VALUE_F16>(L.bell, L.bv, L.W, L.threads, L.lds, s, L.a, L.aligned, L.ranges, false); break;
+            case CSAMD_VALUE_BF16: ok = launch_rows<CSAMD_VALUE_BF16>(L.bell, L.bv, L.W, L.threads, L.lds, s, L.a, L.aligned, L.ranges, false); break;
+            case CSAMD_VALUE_F32: ok = launch_rows<CSAMD_VALUE_F32>(L.bell, L.bv, L.W, L.threads, L.lds, s, L.a, L.aligned, L.ranges, false); break;
+            case CSAMD_VALUE_FP8_E4M3: ok = launch_rows<CSAMD_VALUE_FP8_E4M3>(L.bell, L.bv, L.W, L.threads, L.lds, s, L.a, L.aligned, L.ranges, false); break;
         }
         if (!ok) return 0;
         if (timed) {
@@ -1121,27 +1282,128 @@ int run_label_reduce(const layout_in &L, unsigned int value_type, const csamd_la
             P->last_valid = 1;
         }
     }
-    const size_t meta_lds = (size_t) (G + 1u + W + 1u) * sizeof(unsigned int);
-    label_finish_kernel<<<grid_for((unsigned long long) G * fmax, 256u, 4u * cu), 256, meta_lds, s>>>(
-        a.psum, a.psumsq, a.pcnt, gstart, bounds, W, G, ftile, f_dev, q->feature_scale, fids, rows != 0u ? tail : nullptr,
-        out_sum, out_sumsq, out_cnt, q->compute_moments ? out_mean : nullptr, out_var, out_grows);
+    label_finish_kernel<<<grid_for((unsigned long long) L.G * L.fmax, 256u, 4u * L.cu), 256, L.meta_lds, s>>>(
+        L.a.psum, L.a.psumsq, L.a.pcnt, L.gstart, L.bounds, L.W, L.G, L.a.ftile, L.f_dev, L.scale, L.fids, L.out_sum, L.out_sumsq,
+        L.out_cnt, L.moments ? L.out_mean : nullptr, L.out_var, L.out_grows);
     CSAMD_HIP(hipGetLastError(), "label_finish_kernel");
-
-    // kept rows and features, copied right after the label offsets
-    CSAMD_HIP(hipEventSynchronize(P->ev_meta), "label reduce sync (counts)");
-    const unsigned int K = P->host[0], F = P->host[1];
-    out->group_count = G;
-    out->kept_features = F;
-    out->kept_rows = K;
-    out->reserved = 0u;
-    out->sum = out_sum;
-    out->sumsq = out_sumsq;
-    out->nnz = out_cnt;
-    out->group_rows = out_grows;
-    out->mean = q->compute_moments ? out_mean : nullptr;
-    out->var = q->compute_moments ? out_var : nullptr;
-    out->kept_feature_ids = fids;
     return 1;
+}
+
+void fill_result(const lr_plan &L, unsigned int kept_rows, unsigned int kept_features, csamd_label_reduce_result *out) {
+    out->group_count = L.G;
+    out->kept_features = kept_features;
+    out->kept_rows = kept_rows;
+    out->reserved = 0u;
+    out->sum = L.out_sum;
+    out->sumsq = L.out_sumsq;
+    out->nnz = L.out_cnt;
+    out->group_rows = L.out_grows;
+    out->mean = L.moments ? L.out_mean : nullptr;
+    out->var = L.moments ? L.out_var : nullptr;
+    out->kept_feature_ids = L.fids;
+}
+
+int run_label_reduce(const std::vector<part_in> &parts, unsigned int value_type, const csamd_label_reduce_query *q,
+                     csamd_label_reduce_workspace *ws, csamd_label_reduce_result *out) {
+    CSAMD_REQUIRE(q != nullptr && ws != nullptr && out != nullptr, "label_reduce: null argument");
+    CSAMD_HIP(hipSetDevice(ws->device >= 0 ? ws->device : 0), "hipSetDevice label reduce");
+    label_private *P = private_of(ws);
+    CSAMD_REQUIRE(P != nullptr, "label_reduce: host allocation failed");
+    hipStream_t s = (hipStream_t) ws->stream;
+    lr_plan *L = (lr_plan *) std::malloc(sizeof(lr_plan));  // ~3 KB of inline descriptors: off the stack
+    CSAMD_REQUIRE(L != nullptr, "label_reduce: host allocation failed");
+    int ok = plan_label_reduce(parts, value_type, q, P, *L) && enqueue_label_reduce(*L, P, s, true, true);
+    if (ok) {
+        // kept rows and features, copied right after the label offsets
+        ok = hip_ok(hipEventSynchronize(P->ev_meta), "label reduce sync (counts)");
+        if (ok) fill_result(*L, P->host[0], P->host[1], out);
+    }
+    std::free(L);
+    return ok;
+}
+
+// A captured call: its own buffers, descriptor table and graph.
+struct lr_graph_state {
+    label_private priv;
+    lr_plan plan;
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+    hipEvent_t done;
+    int launched;
+    int device;
+};
+
+void free_private(label_private *P) {
+    dev_buf *bufs[] = {&P->fpairs, &P->fids, &P->labels, &P->rows, &P->work, &P->bounds,
+                       &P->partial, &P->tail, &P->parts, &P->out};
+    for (dev_buf *b : bufs)
+        if (b->p != nullptr) (void) hipFree(b->p);
+    if (P->host != nullptr) (void) hipHostFree((void *) P->host);
+    if (P->parts_host != nullptr) (void) hipHostFree(P->parts_host);
+    if (P->events_ready) {
+        (void) hipEventDestroy(P->ev0);
+        (void) hipEventDestroy(P->ev1);
+        (void) hipEventDestroy(P->ev_meta);
+    }
+    std::memset(P, 0, sizeof(*P));
+}
+
+std::vector<part_in> batched_parts(const csamd_label_reduce_partition *parts, unsigned int partition_count, int *ok);
+
+part_in csr_part(const csamd_compressed_view *src) {
+    part_in pi;
+    std::memset(&pi, 0, sizeof(pi));
+    pi.kind = PART_CSR;
+    pi.rows = src->rows;
+    pi.cols = src->cols;
+    pi.idx = src->minorIdx;
+    pi.val = src->val;
+    pi.nnz = src->nnz;
+    pi.ptr = src->majorPtr;
+    return pi;
+}
+
+part_in sell_part(const csamd_sliced_ell_view *src) {
+    part_in pi;
+    std::memset(&pi, 0, sizeof(pi));
+    pi.kind = PART_SELL;
+    pi.rows = src->rows;
+    pi.cols = src->cols;
+    pi.idx = src->col_idx;
+    pi.val = src->val;
+    pi.sell = *src;
+    return pi;
+}
+
+std::vector<part_in> batched_parts(const csamd_label_reduce_partition *parts, unsigned int partition_count, int *ok) {
+    std::vector<part_in> v;
+    *ok = 0;
+    if (parts == nullptr || partition_count < 1u || partition_count > LR_MAX_PARTS) {
+        set_error("label_reduce_batched: 1..%u partitions required", LR_MAX_PARTS);
+        return v;
+    }
+    v.reserve(partition_count);
+    for (unsigned int p = 0; p < partition_count; ++p) {
+        const csamd_label_reduce_partition &lp = parts[p];
+        if (lp.layout == CSAMD_LAYOUT_COMPRESSED) {
+            if (lp.compressed.axis != CSAMD_COMPRESSED_BY_ROW) {
+                set_error("label_reduce_batched: partition %u is not row-major", p);
+                return v;
+            }
+            if (lp.compressed.rows != 0u && lp.compressed.majorPtr == nullptr) {
+                set_error("label_reduce_batched: partition %u has no majorPtr", p);
+                return v;
+            }
+            v.push_back(csr_part(&lp.compressed));
+        } else if (lp.layout == CSAMD_LAYOUT_SLICED_ELL) {
+            v.push_back(sell_part(&lp.sliced_ell));
+        } else {
+            set_error("label_reduce_batched: partition %u layout %u (compressed or sliced-ELL only)", p, (unsigned) lp.layout);
+            return v;
+        }
+    }
+    *ok = 1;
+    return v;
 }
 
 } // namespace
@@ -1162,18 +1424,8 @@ void csamd_label_reduce_clear(csamd_label_reduce_workspace *ws) {
     if (ws->device >= 0) (void) hipSetDevice(ws->device);
     label_private *P = (label_private *) ws->private_workspace;
     if (P != nullptr) {
-        dev_buf *bufs[] = {&P->fpairs, &P->fids, &P->labels, &P->rows, &P->work, &P->bounds, &P->partial, &P->tail, &P->out};
-        for (dev_buf *b : bufs)
-            if (b->p != nullptr) (void) hipFree(b->p);
-        if (P->host != nullptr) {
-            (void) hipStreamSynchronize((hipStream_t) ws->stream);  // kernels may still write it
-            (void) hipHostFree((void *) P->host);
-        }
-        if (P->events_ready) {
-            (void) hipEventDestroy(P->ev0);
-            (void) hipEventDestroy(P->ev1);
-            (void) hipEventDestroy(P->ev_meta);
-        }
+        if (ws->stream != nullptr) (void) hipStreamSynchronize((hipStream_t) ws->stream);  // kernels may still use the buffers
+        free_private(P);
         std::free(P);
     }
     if (ws->owns_stream != 0 && ws->stream != nullptr) (void) hipStreamDestroy((hipStream_t) ws->stream);
@@ -1204,53 +1456,107 @@ int csamd_label_reduce_compressed(const csamd_compressed_view *src, const csamd_
     CSAMD_REQUIRE(src != nullptr && query != nullptr, "label_reduce_compressed: null argument");
     CSAMD_REQUIRE(src->axis == CSAMD_COMPRESSED_BY_ROW, "label_reduce_compressed: axis must be row");
     CSAMD_REQUIRE(src->rows == 0u || src->majorPtr != nullptr, "label_reduce_compressed: null majorPtr");
-    layout_in L;
-    std::memset(&L, 0, sizeof(L));
-    L.kind = 0;
-    L.rows = src->rows;
-    L.cols = src->cols;
-    L.idx = src->minorIdx;
-    L.val = src->val;
-    L.limit = src->nnz;
-    L.csr.ptr = src->majorPtr;
-    return run_label_reduce(L, query->value_type, query, ws, out);
+    return run_label_reduce(std::vector<part_in>{csr_part(src)}, query->value_type, query, ws, out);
 }
 
 int csamd_label_reduce_sliced_ell(const csamd_sliced_ell_view *src, const csamd_label_reduce_query *query,
                                   csamd_label_reduce_workspace *ws, csamd_label_reduce_result *out) {
     CSAMD_REQUIRE(src != nullptr && query != nullptr, "label_reduce_sliced_ell: null argument");
-    layout_in L;
-    std::memset(&L, 0, sizeof(L));
-    L.kind = 1;
-    L.rows = src->rows;
-    L.cols = src->cols;
-    L.idx = src->col_idx;
-    L.val = src->val;
-    // total slots: last slot offset + rows in last slice * width, computed on the host
-    unsigned long long total = 0ull;
-    if (src->slice_count != 0u) {
-        unsigned int last_off = 0u, last_w = 0u, last_r0 = 0u;
-        CSAMD_HIP(hipMemcpy(&last_off, src->slice_slot_offsets + (src->slice_count - 1u), 4u, hipMemcpyDeviceToHost), "D2H sell");
-        CSAMD_HIP(hipMemcpy(&last_w, src->slice_widths + (src->slice_count - 1u), 4u, hipMemcpyDeviceToHost), "D2H sell");
-        CSAMD_HIP(hipMemcpy(&last_r0, src->slice_row_offsets + (src->slice_count - 1u), 4u, hipMemcpyDeviceToHost), "D2H sell");
-        total = (unsigned long long) last_off + (unsigned long long) (src->rows - last_r0) * last_w;
-    }
-    L.limit = total;
-    L.sell.s = *src;
-    return run_label_reduce(L, query->value_type, query, ws, out);
+    return run_label_reduce(std::vector<part_in>{sell_part(src)}, query->value_type, query, ws, out);
 }
 
 int csamd_label_reduce_blocked_ell(const csamd_blocked_ell_view *src, const csamd_label_reduce_query *query,
                                    csamd_label_reduce_workspace *ws, csamd_label_reduce_result *out) {
     CSAMD_REQUIRE(src != nullptr && query != nullptr, "label_reduce_blocked_ell: null argument");
     CSAMD_REQUIRE(src->block_size != 0u || src->ell_cols == 0u, "label_reduce_blocked_ell: block_size is 0");
-    layout_in L;
-    std::memset(&L, 0, sizeof(L));
-    L.kind = 2;
-    L.rows = src->rows;
-    L.cols = src->cols;
-    L.bell = *src;
-    return run_label_reduce(L, query->value_type, query, ws, out);
+    part_in pi;
+    std::memset(&pi, 0, sizeof(pi));
+    pi.kind = PART_BELL;
+    pi.rows = src->rows;
+    pi.cols = src->cols;
+    pi.bell = *src;
+    return run_label_reduce(std::vector<part_in>{pi}, query->value_type, query, ws, out);
+}
+
+int csamd_label_reduce_batched(const csamd_label_reduce_partition *parts, unsigned int partition_count,
+                               const csamd_label_reduce_query *query, csamd_label_reduce_workspace *ws,
+                               csamd_label_reduce_result *out) {
+    CSAMD_REQUIRE(query != nullptr, "label_reduce_batched: null query");
+    int ok = 0;
+    std::vector<part_in> v = batched_parts(parts, partition_count, &ok);
+    if (!ok) return 0;
+    return run_label_reduce(v, query->value_type, query, ws, out);
+}
+
+int csamd_label_reduce_graph_capture(const csamd_label_reduce_partition *parts, unsigned int partition_count,
+                                     const csamd_label_reduce_query *query, int device, csamd_label_reduce_graph *graph) {
+    CSAMD_REQUIRE(graph != nullptr && query != nullptr, "label_reduce_graph_capture: null argument");
+    std::memset(graph, 0, sizeof(*graph));
+    int ok = 0;
+    std::vector<part_in> v = batched_parts(parts, partition_count, &ok);
+    if (!ok) return 0;
+    CSAMD_HIP(hipSetDevice(device), "hipSetDevice label reduce graph");
+    lr_graph_state *G = (lr_graph_state *) std::calloc(1u, sizeof(lr_graph_state));
+    CSAMD_REQUIRE(G != nullptr, "label_reduce_graph_capture: host allocation failed");
+    G->device = device;
+    graph->device = device;
+    graph->state = G;
+    hipStream_t cs = nullptr;
+    ok = plan_label_reduce(v, query->value_type, query, &G->priv, G->plan) &&
+         hip_ok(hipEventCreateWithFlags(&G->done, hipEventDisableTiming), "hipEventCreate graph");
+    if (ok && !G->plan.inline_parts)  // the graph's own descriptor table, uploaded once
+        ok = hip_ok(hipMemcpy(G->plan.d_parts, G->priv.parts_host, G->plan.parts_bytes, hipMemcpyHostToDevice), "H2D graph partitions");
+    if (ok) ok = hip_ok(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking), "hipStreamCreate capture");
+    if (ok) ok = hip_ok(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    if (ok) {
+        const int enq = enqueue_label_reduce(G->plan, &G->priv, cs, false, false);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(cs, &g);  // always ends the capture
+        G->graph = g;
+        ok = enq && hip_ok(e, "hipStreamEndCapture");
+    }
+    if (ok) ok = hip_ok(hipGraphInstantiate(&G->exec, G->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+    if (cs != nullptr) (void) hipStreamDestroy(cs);
+    if (!ok) {
+        csamd_label_reduce_graph_destroy(graph);
+        return 0;
+    }
+    return 1;
+}
+
+int csamd_label_reduce_graph_launch(csamd_label_reduce_graph *graph, csamd_stream_t stream) {
+    CSAMD_REQUIRE(graph != nullptr && graph->state != nullptr, "label_reduce_graph_launch: no graph");
+    lr_graph_state *G = (lr_graph_state *) graph->state;
+    CSAMD_HIP(hipSetDevice(G->device), "hipSetDevice label reduce graph");
+    CSAMD_HIP(hipGraphLaunch(G->exec, (hipStream_t) stream), "hipGraphLaunch");
+    CSAMD_HIP(hipEventRecord(G->done, (hipStream_t) stream), "hipEventRecord graph");
+    G->launched = 1;
+    return 1;
+}
+
+int csamd_label_reduce_graph_result(csamd_label_reduce_graph *graph, csamd_label_reduce_result *out) {
+    CSAMD_REQUIRE(graph != nullptr && graph->state != nullptr && out != nullptr, "label_reduce_graph_result: null argument");
+    lr_graph_state *G = (lr_graph_state *) graph->state;
+    CSAMD_REQUIRE(G->launched, "label_reduce_graph_result: the graph has not been launched");
+    CSAMD_HIP(hipSetDevice(G->device), "hipSetDevice label reduce graph");
+    CSAMD_HIP(hipEventSynchronize(G->done), "label reduce graph sync");
+    unsigned int counts[2] = {0u, 0u};
+    CSAMD_HIP(hipMemcpy(counts, G->plan.gstart + G->plan.G, sizeof(counts), hipMemcpyDeviceToHost), "D2H graph counts");
+    fill_result(G->plan, counts[0], counts[1], out);
+    return 1;
+}
+
+void csamd_label_reduce_graph_destroy(csamd_label_reduce_graph *graph) {
+    if (graph == nullptr || graph->state == nullptr) return;
+    lr_graph_state *G = (lr_graph_state *) graph->state;
+    (void) hipSetDevice(G->device);
+    if (G->launched) (void) hipEventSynchronize(G->done);  // replays may still use the buffers
+    if (G->exec != nullptr) (void) hipGraphExecDestroy(G->exec);
+    if (G->graph != nullptr) (void) hipGraphDestroy(G->graph);
+    if (G->done != nullptr) (void) hipEventDestroy(G->done);
+    free_private(&G->priv);
+    std::free(G);
+    graph->state = nullptr;
 }
 
 int csamd_label_reduce_set_timing(csamd_label_reduce_workspace *ws, int enable) {

@@ -153,6 +153,31 @@ def sliced_ell_from_blocked_ell(bell: DeviceBlockedELL, slice_rows: int, ws: Con
 # ---------------------------------------------------------------------------
 # Transpose
 # ---------------------------------------------------------------------------
+LAYOUT_NAMES = {A.LAYOUT_COMPRESSED: "compressed", A.LAYOUT_SLICED_ELL: "sliced_ell", A.LAYOUT_BLOCKED_ELL: "blocked_ell"}
+
+
+def tune_partitions(parts, ws: ConvertWorkspace) -> list[dict]:
+    """Per-partition layout choice for CSR partitions (one launch for all):
+    the reference's Blocked-ELL {8,16,32} and Sliced-ELL {32,64} choices,
+    the bytes of each format, the smallest layout and the log2 row-length
+    histogram."""
+    if not parts:
+        return []
+    vt = parts[0].value_type
+    views = (A.compressed_view * len(parts))(*[p.view() for p in parts])
+    out = (A.partition_tune_result * len(parts))()
+    check(load().csamd_tune_partitions(views, len(parts), vt, C.byref(ws.ws), out), "tune_partitions")
+    res = []
+    for r in out:
+        res.append(dict(layout=LAYOUT_NAMES.get(int(r.layout), "unknown"), bell_block_size=int(r.bell_block_size),
+                        bell_ell_width=int(r.bell_ell_width), bell_fill_ratio=float(r.bell_fill_ratio),
+                        sell_slice_rows=int(r.sell_slice_rows), sell_slice_count=int(r.sell_slice_count),
+                        sell_padded_slots=int(r.sell_padded_slots), sell_score=float(r.sell_score),
+                        csr_bytes=int(r.csr_bytes), bell_bytes=int(r.bell_bytes), sell_bytes=int(r.sell_bytes),
+                        row_length_log2_hist=[int(x) for x in r.row_length_log2_hist]))
+    return res
+
+
 class TransposeScratch:
     def __init__(self):
         self.buf = None
@@ -271,16 +296,21 @@ class LabelReduce:
         except Exception:
             pass
 
-    def run(self, src, labels: torch.Tensor, group_count: int, row_keep: torch.Tensor | None = None,
-            feature_keep: torch.Tensor | None = None, feature_scale: torch.Tensor | None = None, moments: bool = True):
+    @staticmethod
+    def _query(value_type, labels, group_count, row_keep, feature_keep, feature_scale, moments):
         q = A.label_reduce_query()
         q.group_count = group_count
-        q.value_type = src.value_type
+        q.value_type = value_type
         q.row_labels = ptr(labels)
         q.row_keep = ptr(row_keep)
         q.feature_keep = ptr(feature_keep)
         q.feature_scale = ptr(feature_scale)
         q.compute_moments = 1 if moments else 0
+        return q
+
+    def run(self, src, labels: torch.Tensor, group_count: int, row_keep: torch.Tensor | None = None,
+            feature_keep: torch.Tensor | None = None, feature_scale: torch.Tensor | None = None, moments: bool = True):
+        q = self._query(src.value_type, labels, group_count, row_keep, feature_keep, feature_scale, moments)
         res = A.label_reduce_result()
         v = src.view()
         lib = load()
@@ -293,6 +323,38 @@ class LabelReduce:
         else:
             raise TypeError(type(src))
         check(rc, "label_reduce")
+        return res
+
+    @staticmethod
+    def partition_table(parts) -> "C.Array":
+        """csamd_label_reduce_partition array for CSR / Sliced-ELL partitions."""
+        arr = (A.label_reduce_partition * len(parts))()
+        for i, p in enumerate(parts):
+            if isinstance(p, DeviceCSR):
+                arr[i].layout = A.LAYOUT_COMPRESSED
+                arr[i].compressed = p.view()
+            elif isinstance(p, DeviceSlicedELL):
+                arr[i].layout = A.LAYOUT_SLICED_ELL
+                arr[i].sliced_ell = p.view()
+            else:
+                raise TypeError(type(p))
+        return arr
+
+    def run_batched(self, parts, labels: torch.Tensor, group_count: int, row_keep: torch.Tensor | None = None,
+                    feature_keep: torch.Tensor | None = None, feature_scale: torch.Tensor | None = None,
+                    moments: bool = True, table=None):
+        """One reduce over partitions (same columns, same value type); the
+        row arrays cover the partitions' rows in order."""
+        if not parts:
+            raise ValueError("no partitions")
+        vt = parts[0].value_type
+        if any(p.value_type != vt for p in parts):
+            raise ValueError("partitions must share the value type")
+        q = self._query(vt, labels, group_count, row_keep, feature_keep, feature_scale, moments)
+        arr = table if table is not None else self.partition_table(parts)
+        res = A.label_reduce_result()
+        check(load().csamd_label_reduce_batched(arr, len(parts), C.byref(q), C.byref(self.ws), C.byref(res)),
+              "label_reduce_batched")
         return res
 
     def set_timing(self, enable: bool) -> None:
@@ -321,6 +383,49 @@ class LabelReduce:
             out["mean"] = dl(res.mean, G * F, np.float64).reshape(G, F)
             out["var"] = dl(res.var, G * F, np.float64).reshape(G, F)
         return out
+
+
+class LabelReduceGraph:
+    """A batched label reduce captured into a HIP graph (own buffers); each
+    launch() replays it on a stream, result() waits and returns the
+    result struct.  The partitions' and the query's device arrays are bound
+    at capture and must stay alive; replays read their current contents."""
+
+    def __init__(self, parts, labels: torch.Tensor, group_count: int, row_keep: torch.Tensor | None = None,
+                 feature_keep: torch.Tensor | None = None, feature_scale: torch.Tensor | None = None,
+                 moments: bool = True, device: int | None = None):
+        if not parts:
+            raise ValueError("no partitions")
+        vt = parts[0].value_type
+        if any(p.value_type != vt for p in parts):
+            raise ValueError("partitions must share the value type")
+        self._keep = (list(parts), labels, row_keep, feature_keep, feature_scale)
+        q = LabelReduce._query(vt, labels, group_count, row_keep, feature_keep, feature_scale, moments)
+        table = LabelReduce.partition_table(parts)
+        self.g = A.label_reduce_graph()
+        dev = torch.cuda.current_device() if device is None else device
+        check(load().csamd_label_reduce_graph_capture(table, len(parts), C.byref(q), dev, C.byref(self.g)),
+              "label_reduce_graph_capture")
+
+    def launch(self, stream: int | None = None) -> None:
+        check(load().csamd_label_reduce_graph_launch(C.byref(self.g), stream if stream is not None else current_stream_handle()),
+              "label_reduce_graph_launch")
+
+    def result(self):
+        res = A.label_reduce_result()
+        check(load().csamd_label_reduce_graph_result(C.byref(self.g), C.byref(res)), "label_reduce_graph_result")
+        return res
+
+    def close(self):
+        if self.g is not None:
+            load().csamd_label_reduce_graph_destroy(C.byref(self.g))
+            self.g = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def _copy_d2h(host_u8: torch.Tensor, dev_ptr: int, nbytes: int) -> None:

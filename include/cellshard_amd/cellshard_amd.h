@@ -393,6 +393,41 @@ int csamd_label_reduce_sliced_ell(const csamd_sliced_ell_view *src, const csamd_
                                   csamd_label_reduce_workspace *workspace, csamd_label_reduce_result *out);
 int csamd_label_reduce_blocked_ell(const csamd_blocked_ell_view *src, const csamd_label_reduce_query *query,
                                    csamd_label_reduce_workspace *workspace, csamd_label_reduce_result *out);
+/* Batched partitions (project extension; the reference launches once per
+ * partition): partitions with the same column count, each compressed
+ * (row axis) or sliced-ELL, reduced by one launch.  The query's row arrays
+ * (row_labels, row_keep) cover the partitions' rows in partition order;
+ * results are those of the row-wise concatenation of the partitions. */
+typedef struct CSAMD_ALIGN16 csamd_label_reduce_partition {
+    unsigned int layout;               /* CSAMD_LAYOUT_COMPRESSED or CSAMD_LAYOUT_SLICED_ELL */
+    unsigned int reserved[3];
+    csamd_compressed_view compressed;  /* layout == CSAMD_LAYOUT_COMPRESSED */
+    csamd_sliced_ell_view sliced_ell;  /* layout == CSAMD_LAYOUT_SLICED_ELL */
+} csamd_label_reduce_partition;
+
+int csamd_label_reduce_batched(const csamd_label_reduce_partition *partitions, unsigned int partition_count,
+                               const csamd_label_reduce_query *query, csamd_label_reduce_workspace *workspace,
+                               csamd_label_reduce_result *out);
+
+/* Captured calls (project extension; the reference has no graphs): the
+ * batched call's launches (descriptor table, label sort, work ranges,
+ * reduce, finish) captured once into a HIP graph with buffers of its own,
+ * then replayed with one hipGraphLaunch per call.  The views' arrays and the
+ * query's arrays are bound at capture: replays read their current contents.
+ * The graph holds no memset nodes and no host copies; results alias the
+ * graph's buffers until the next launch or destroy. */
+typedef struct CSAMD_ALIGN16 csamd_label_reduce_graph {
+    int device;
+    void *state;
+} csamd_label_reduce_graph;
+
+int csamd_label_reduce_graph_capture(const csamd_label_reduce_partition *partitions, unsigned int partition_count,
+                                     const csamd_label_reduce_query *query, int device, csamd_label_reduce_graph *graph);
+int csamd_label_reduce_graph_launch(csamd_label_reduce_graph *graph, csamd_stream_t stream);
+/* Waits for the last launch, then fills out (kept rows / features included). */
+int csamd_label_reduce_graph_result(csamd_label_reduce_graph *graph, csamd_label_reduce_result *out);
+void csamd_label_reduce_graph_destroy(csamd_label_reduce_graph *graph);
+
 /* Kernel timing (project extension): with enable != 0, later calls record
  * HIP events around the main reduce kernel (first feature tile) on the
  * workspace stream; off by default, since each event record costs a few
@@ -510,6 +545,36 @@ typedef struct CSAMD_ALIGN16 csamd_convert_workspace {
     void *scratch;
     size_t scratch_bytes;
 } csamd_convert_workspace;
+
+/* Layout autotuner over a batch of row-major partitions (project
+ * extension; one launch for all partitions): per partition, the reference's
+ * Blocked-ELL block-size choice among {8, 16, 32}
+ * (src/convert/blocked_ell_from_compressed.cuh:157-190) and Sliced-ELL
+ * slice-height choice among {32, 64}
+ * (src/convert/sliced_ell_from_blocked_ell.cuh:196-221), the format bytes of
+ * each (value bytes from value_type), the layout with the fewest bytes, and
+ * the histogram of row lengths by log2 bucket (bucket 0: empty rows; bucket
+ * b >= 1: lengths in [2^(b-1), 2^b); the last bucket takes the rest).
+ * Partitions share the column count (cols <= 524,288). */
+typedef struct CSAMD_ALIGN16 csamd_partition_tune_result {
+    unsigned int layout;               /* CSAMD_LAYOUT_COMPRESSED / _SLICED_ELL / _BLOCKED_ELL */
+    unsigned int bell_block_size;
+    unsigned int bell_ell_width;       /* column blocks per row block (W) */
+    unsigned int sell_slice_rows;
+    unsigned int sell_slice_count;
+    unsigned int reserved;
+    unsigned long long sell_padded_slots;
+    unsigned long long csr_bytes;      /* (rows + 1) x 4 + nnz x (4 + value bytes) */
+    unsigned long long bell_bytes;     /* row blocks x W x 4 + rows x W x B x value bytes */
+    unsigned long long sell_bytes;     /* (2 slices + 1) x 4 + padded slots x (4 + value bytes) */
+    double bell_fill_ratio;
+    double sell_score;
+    unsigned int row_length_log2_hist[32];
+} csamd_partition_tune_result;
+
+int csamd_tune_partitions(const csamd_compressed_view *partitions, unsigned int partition_count, unsigned int value_type,
+                          csamd_convert_workspace *workspace, csamd_partition_tune_result *out);
+
 
 void csamd_convert_init(csamd_convert_workspace *ws);
 void csamd_convert_clear(csamd_convert_workspace *ws);
