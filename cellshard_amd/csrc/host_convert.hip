@@ -13,7 +13,7 @@
 namespace csamd {
 namespace {
 
-constexpr unsigned int HOST_REGISTERED = 1u;
+constexpr unsigned int HOST_PINNED = 1u;  // storage from hipHostMalloc
 
 struct dev_arrays {
     void *p[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
@@ -192,8 +192,8 @@ extern "C" {
 
 void csamd_host_compressed_clear(csamd_host_compressed *m) {
     if (m == nullptr) return;
-    if ((m->flags & HOST_REGISTERED) != 0u && m->storage != nullptr) (void) hipHostUnregister(m->storage);
-    if (m->storage != nullptr) std::free(m->storage);
+    if ((m->flags & HOST_PINNED) != 0u && m->storage != nullptr) (void) hipHostFree(m->storage);
+    else if (m->storage != nullptr) std::free(m->storage);
     else {
         std::free(m->majorPtr);
         std::free(m->minorIdx);
@@ -204,8 +204,8 @@ void csamd_host_compressed_clear(csamd_host_compressed *m) {
 
 void csamd_host_blocked_ell_clear(csamd_host_blocked_ell *m) {
     if (m == nullptr) return;
-    if ((m->flags & HOST_REGISTERED) != 0u && m->storage != nullptr) (void) hipHostUnregister(m->storage);
-    if (m->storage != nullptr) std::free(m->storage);
+    if ((m->flags & HOST_PINNED) != 0u && m->storage != nullptr) (void) hipHostFree(m->storage);
+    else if (m->storage != nullptr) std::free(m->storage);
     else {
         std::free(m->blockColIdx);
         std::free(m->val);
@@ -215,8 +215,8 @@ void csamd_host_blocked_ell_clear(csamd_host_blocked_ell *m) {
 
 void csamd_host_sliced_ell_clear(csamd_host_sliced_ell *m) {
     if (m == nullptr) return;
-    if ((m->flags & HOST_REGISTERED) != 0u && m->storage != nullptr) (void) hipHostUnregister(m->storage);
-    if (m->storage != nullptr) std::free(m->storage);
+    if ((m->flags & HOST_PINNED) != 0u && m->storage != nullptr) (void) hipHostFree(m->storage);
+    else if (m->storage != nullptr) std::free(m->storage);
     else {
         std::free(m->slice_row_offsets);
         std::free(m->slice_widths);

@@ -717,8 +717,9 @@ int csamd_blocked_ell_bucket_plan_raw(const unsigned int *d_block_col_idx, unsig
  * formats/compressed.cuh:24-37, formats/blocked_ell.cuh:15-26,
  * formats/sliced_ell.cuh:16-28.  Output structs are (re)allocated with one
  * packed malloc block (`storage`), exactly as sparse::allocate does, so a
- * caller's sparse::clear()/free() releases them.  flags bit 0 marks memory
- * pinned with hipHostRegister (released with hipHostUnregister).
+ * caller's sparse::clear()/free() releases them.  flags bit 0 marks storage
+ * allocated pinned with hipHostMalloc (released with hipHostFree); existing
+ * memory is never registered after the fact.
  * ------------------------------------------------------------------------- */
 typedef struct CSAMD_ALIGN16 csamd_host_compressed {
     unsigned int rows;
