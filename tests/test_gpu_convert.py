@@ -114,9 +114,18 @@ def test_sliced_from_blocked_parity(ws):
     assert np.array_equal(col, ref["col_idx"]) and np.array_equal(v, ref["val"])
 
 
-@pytest.mark.parametrize("case", ["sorted", "unsorted", "empty_cols", "wide", "nnz0"])
+@pytest.mark.parametrize("case", ["sorted", "unsorted", "empty_cols", "wide", "nnz0", "dense", "duplicates"])
 def test_transpose_parity(gpu, case):
-    if case == "nnz0":
+    if case == "dense":
+        # > 8192 entries per (256-row tile, 512-column band): chunks staged in
+        # several row batches; partial last tile and band
+        rows, cols = 700, 1100
+        ptr, idx, val = random_csr(rows, cols, 0.85, seed=3, empty_rows=0.05)
+    elif case == "duplicates":  # repeated columns inside rows: the fallback scatter
+        rows, cols = 400, 300
+        ptr, idx, val = random_csr(rows, cols, 0.05, seed=9, duplicates=0.3)
+        val = np.ones_like(val)  # equal values: the order of repeats inside a row is not fixed
+    elif case == "nnz0":
         rows, cols = 10, 6
         ptr, idx, val = np.zeros(11, np.uint32), np.zeros(0, np.uint32), np.zeros(0)
     elif case == "wide":
@@ -137,6 +146,18 @@ def test_transpose_parity(gpu, case):
     assert np.array_equal(bptr, ptr)
     if case != "unsorted":
         assert np.array_equal(bidx, idx) and np.array_equal(bval, bits)
+
+
+@pytest.mark.parametrize("vt", [F32, FP8])
+def test_transpose_value_types(gpu, vt):
+    rows, cols = 1300, 2100
+    ptr, idx, val = random_csr(rows, cols, 0.2, seed=21, fp8_safe=True)
+    bits = encode(val, vt)
+    src = DeviceCSR.from_numpy(rows, cols, ptr, idx, bits, vt)
+    optr, oidx, oval = compressed_transpose(src, TransposeScratch()).to_numpy()
+    rptr, ridx, rval = O.transpose(rows, cols, ptr, idx, bits)
+    assert np.array_equal(optr, rptr) and np.array_equal(oidx, ridx)
+    assert np.array_equal(oval.view(np.uint8), np.ascontiguousarray(rval).view(np.uint8))
 
 
 def test_bucket_plan_parity(gpu):
