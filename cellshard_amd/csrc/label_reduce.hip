@@ -143,8 +143,10 @@ struct part_args {
 
 // Descriptor table on the device (from the kernel arguments when inline),
 // with SELL slot totals, clips and clamps computed here (no host round
-// trip).  Idempotent: a replayed graph recomputes the same table.
-__global__ void part_prepare_kernel(part_desc *__restrict__ parts, unsigned int nparts, part_args inl, unsigned int use_inline) {
+// trip).  Idempotent: a replayed graph recomputes the same table.  Run by
+// the label count kernel's threads (no launch of its own).
+__device__ __forceinline__ void prepare_parts(part_desc *__restrict__ parts, unsigned int nparts, const part_args &inl,
+                                              unsigned int use_inline) {
     for (unsigned int p = blockIdx.x * blockDim.x + threadIdx.x; p < nparts; p += gridDim.x * blockDim.x) {
         part_desc P = use_inline ? inl.d[p] : parts[p];
         if (P.kind == PART_SELL) {
@@ -209,17 +211,21 @@ __device__ __forceinline__ unsigned int row_key(const unsigned int *__restrict__
     return ((row_keep == nullptr || row_keep[r] != 0u) && l < G) ? l : G;
 }
 
-// Also: clears work[0..work_n) (positions past the kept rows keep weight 0
-// for the partition scan); builds the kept-feature mask of each 32-column
-// word; clears the tail records.
+// Also: prepares the partition descriptors; clears work[0..work_n)
+// (positions past the kept rows keep weight 0 for the partition scan);
+// builds the kept-feature mask of each 32-column word; clears the tail
+// records.
 __global__ void __launch_bounds__(LR_SORT_THREADS) label_count_kernel(const unsigned int *__restrict__ labels,
                                                                       const unsigned char *__restrict__ row_keep, unsigned int rows,
                                                                       unsigned int G, unsigned int *__restrict__ hist_out,
                                                                       unsigned int *__restrict__ work, unsigned int work_n,
                                                                       const unsigned char *__restrict__ feature_keep, unsigned int cols,
                                                                       unsigned int nwords, unsigned int *__restrict__ word_mask,
-                                                                      tail_record *__restrict__ tail, unsigned int ntail) {
+                                                                      tail_record *__restrict__ tail, unsigned int ntail,
+                                                                      part_desc *__restrict__ parts, unsigned int nparts,
+                                                                      part_args inl, unsigned int use_inline) {
     extern __shared__ unsigned int hist[];  // [G + 1]
+    prepare_parts(parts, nparts, inl, use_inline);
     for (unsigned int g = threadIdx.x; g <= G; g += blockDim.x) hist[g] = 0u;
     for (unsigned int t = blockIdx.x * blockDim.x + threadIdx.x; t < ntail; t += gridDim.x * blockDim.x)
         tail[t] = tail_record{0xffffffffu, CSAMD_INVALID_COL, 0.0f, 0u};
@@ -1239,12 +1245,11 @@ int plan_label_reduce(const std::vector<part_in> &parts, unsigned int value_type
 int enqueue_label_reduce(const lr_plan &L, label_private *P, hipStream_t s, bool upload, bool eager) {
     if (!L.inline_parts && upload)
         CSAMD_HIP(hipMemcpyAsync(L.d_parts, P->parts_host, L.parts_bytes, hipMemcpyHostToDevice, s), "H2D label partitions");
-    part_prepare_kernel<<<grid_for(L.nparts, 256u, 64u), 256, 0, s>>>(L.d_parts, L.nparts, L.inl, L.inline_parts ? 1u : 0u);
-    CSAMD_HIP(hipGetLastError(), "part_prepare_kernel");
     // 1. label histograms and feature words; 2. label offsets and feature map
     label_count_kernel<<<L.sort_grid, LR_SORT_THREADS, L.hist_lds, s>>>(L.labels, L.row_keep, L.rows, L.G, L.table, L.work,
                                                                         L.rows + 1u, L.feature_keep, L.cols, L.nwords, L.word_mask,
-                                                                        L.tail, L.ntail);
+                                                                        L.tail, L.ntail, L.d_parts, L.nparts, L.inl,
+                                                                        L.inline_parts ? 1u : 0u);
     CSAMD_HIP(hipGetLastError(), "label_count_kernel");
     label_offsets_kernel<<<1, 1024, 0, s>>>(L.table, L.G, L.sort_grid, L.gstart, L.word_mask, L.nwords, L.fpairs, L.fids, L.f_dev);
     CSAMD_HIP(hipGetLastError(), "label_offsets_kernel");
