@@ -485,6 +485,21 @@ def main() -> int:
         torch.cuda.empty_cache()
         log("copy calibration: 2 x (2 GiB read + 2 GiB write)")
 
+    # plain device copy of 2 GiB (read + write), for scale: what a trivial
+    # stream reaches on this box
+    a_ = torch.empty(1 << 31, dtype=torch.uint8, device=dev).fill_(1)
+    b_ = torch.empty_like(a_)
+    b_.copy_(a_)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(5):
+        b_.copy_(a_)
+    torch.cuda.synchronize()
+    copy_gbs = 5 * 2 * (1 << 31) / (time.perf_counter() - t0) / 1e9
+    del a_, b_
+    torch.cuda.empty_cache()
+    log(f"device copy: {copy_gbs:.0f} GB/s (read + write)")
+
     # ---------------- synthetic inputs (config 2 generator) ----------------
     t0 = time.time()
     ws = ConvertWorkspace()
@@ -631,6 +646,7 @@ def main() -> int:
         "batched_partitions": batched,
         "serving": serving,
         "generate_s": round(gen_s, 2),
+        "device_copy_gb_s": round(copy_gbs, 1),
     }
     print(json.dumps(line), flush=True)
     return 0
