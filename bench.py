@@ -55,6 +55,8 @@ def parse_args():
     ap.add_argument("--skip-batched", action="store_true", help="skip the batched uneven-partition measurement")
     ap.add_argument("--batched-only", action="store_true", help="only the batched uneven-partition measurement")
     ap.add_argument("--c3-partitions", type=int, default=256)
+    ap.add_argument("--skip-plain", action="store_true", help="skip the 20,000 x 33,538 plain-path measurement")
+    ap.add_argument("--plain-only", action="store_true", help="only the plain-path measurement")
     return ap.parse_args()
 
 
@@ -298,6 +300,82 @@ def read_traffic(path: Path):
     return None
 
 
+def run_plain_path(args, log):
+    """The plain path at its full size: CSR 20,000 x 33,538 fp32 counts (row
+    length LogNormal(median 1,500, sigma 0.5)); CSR -> Blocked-ELL (block 16),
+    CSR -> CSC, then the masked group row-sum + nnz over 16 uniform labels
+    (row mask 0.9, 2,000 features).  GPU times beside the reference's own CPU
+    Blocked-ELL converter (compiled from the reference sources; fp16 values,
+    the reference structs' type) and the C oracle's reduce, on this host."""
+    import numpy as np
+    import torch
+
+    from cellshard_amd import abi as A
+    from cellshard_amd.device import to_dev_u32, to_dev_u8
+    from cellshard_amd.ops import (ConvertWorkspace, LabelReduce, TransposeScratch, blocked_ell_from_compressed,
+                                   compressed_transpose, synth_csr)
+    from oracle import oracle as O
+
+    dev = torch.device("cuda", 0)
+    R, Cc, G = 20_000, 33_538, 16
+    ws = ConvertWorkspace()
+    src = synth_csr(R, Cc, median=1500.0, sigma=0.5, seed=1, value_type=A.VALUE_F32, ws=ws)
+    ptr, idx, val = src.to_numpy()
+    nnz = src.nnz
+    csr_bytes = (R + 1) * 4 + nnz * 8
+    rng = np.random.default_rng(1)
+    labels = rng.integers(0, G, size=R).astype(np.uint32)
+    rk = (rng.random(R) < 0.9).astype(np.uint8)
+    fk = np.zeros(Cc, np.uint8)
+    fk[rng.choice(Cc, 2000, replace=False)] = 1
+
+    def gpu_ms(fn, reps=5):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            out = fn()
+        torch.cuda.synchronize()
+        return out, (time.perf_counter() - t0) * 1e3 / reps
+
+    bell, bell_ms = gpu_ms(lambda: blocked_ell_from_compressed(src, 16, ws))
+    bell_bytes = bell.format_bytes()
+    del bell
+    torch.cuda.empty_cache()
+    scratch = TransposeScratch()
+    _, tr_ms = gpu_ms(lambda: compressed_transpose(src, scratch))
+    del scratch
+    d_l, d_rk, d_fk = to_dev_u32(labels, dev), to_dev_u8(rk, dev), to_dev_u8(fk, dev)
+    lr = LabelReduce(stream=torch.cuda.current_stream().cuda_stream)
+    _, red_ms = gpu_ms(lambda: lr.run(src, d_l, G, d_rk, d_fk), reps=20)
+    lr.close()
+    kept = (rk == 1)
+    row_len = np.diff(ptr.astype(np.int64))
+    red_bytes = int(row_len[kept].sum()) * 8 + (R + 1) * 4 + R * 5 + Cc * 4 + G * 2000 * 36
+    res = {"workload": "plain_csr_20000x33538_fp32_bell16_transpose_16labels", "rows": R, "cols": Cc, "nnz": nnz,
+           "gpu": {"blocked_ell_16_ms": round(bell_ms, 3), "blocked_ell_16_gb_s": round((csr_bytes + bell_bytes) / (bell_ms * 1e-3) / 1e9, 1),
+                   "blocked_ell_bytes_over_csr": round(bell_bytes / csr_bytes, 2),
+                   "transpose_ms": round(tr_ms, 3), "transpose_gb_s": round(2 * csr_bytes / (tr_ms * 1e-3) / 1e9, 1),
+                   "reduce_ms": round(red_ms, 4), "reduce_gb_s": round(red_bytes / (red_ms * 1e-3) / 1e9, 1)}}
+    if not args.skip_cpu:
+        t0 = time.perf_counter()
+        O.label_reduce_csr(R, Cc, ptr, idx, val, A.VALUE_F32, labels, rk, fk, G)
+        res["cpu_reduce"] = {"s": round(time.perf_counter() - t0, 3), "kind": "port", "cores": 1}
+        if O.ref_available():
+            v16 = val.view(np.float32).astype(np.float16)
+            t0 = time.perf_counter()
+            O.ref_blocked_ell_from_compressed(R, Cc, ptr, idx, v16, 16)
+            cpu_s = time.perf_counter() - t0
+            res["cpu_blocked_ell_16"] = {"s": round(cpu_s, 3), "kind": "reference", "cores": 1,
+                                         "note": "reference convert::blocked_ell_from_compressed (CPU, fp16 values), compiled from its sources",
+                                         "gpu_speedup": round(cpu_s * 1e3 / bell_ms, 1)}
+    ws.close()
+    del src
+    torch.cuda.empty_cache()
+    log(f"plain path: {json.dumps(res)}")
+    return res
+
+
 def run_batched_partitions(args, log):
     """Batched uneven partitions: 256 CSR partitions (rows U[512, 65,536],
     median row length U[300, 5,000], LogNormal sigma 0.7, 36,601 columns,
@@ -453,6 +531,10 @@ def main() -> int:
         torch.cuda.set_device(0)
         print(json.dumps({"serving": run_serving(args, log)}), flush=True)
         return 0
+    if args.plain_only:
+        torch.cuda.set_device(0)
+        print(json.dumps({"plain_path": run_plain_path(args, log)}), flush=True)
+        return 0
     if args.batched_only:
         torch.cuda.set_device(0)
         print(json.dumps({"batched_partitions": run_batched_partitions(args, log)}), flush=True)
@@ -603,6 +685,22 @@ def main() -> int:
         b_cpu = int(row_nnz[:S][k].sum()) * 6 + (S + 1) * 4 + S * 5 + Cc * 4 + out_bytes
         cpu = {"value": round(b_cpu / cpu_s / 1e9, 3), "unit": "GB/s", "cores": 1, "kind": "port",
                "sample": f"first {S} of {R} rows ({end} stored entries), same masks/labels; {cpu_s:.2f} s single-threaded C oracle"}
+        # the same sample on every host core: row chunks reduced by the C
+        # oracle in parallel threads (ctypes releases the GIL), partials summed
+        from concurrent.futures import ThreadPoolExecutor
+        nt = max(1, min(16, os.cpu_count() or 1))
+        cuts = np.linspace(0, S, nt + 1).astype(np.int64)
+
+        def chunk(k):
+            a, b = int(cuts[k]), int(cuts[k + 1])
+            pa = (p[a:b + 1] - p[a]).astype(np.uint32)
+            return O.label_reduce_csr(b - a, Cc, pa, idx[p[a]:p[b]], val[p[a]:p[b]], A.VALUE_F16, labels[a:b], row_keep[a:b],
+                                      feature_keep, G)
+        t0 = time.perf_counter()
+        with ThreadPoolExecutor(nt) as ex:
+            parts_ = list(ex.map(chunk, range(nt)))
+        mt_s = time.perf_counter() - t0
+        cpu["all_cores"] = {"value": round(b_cpu / mt_s / 1e9, 3), "cores": nt, "s": round(mt_s, 3)}
         log(f"cpu baseline: {cpu}")
 
     traffic = read_traffic(ROOT / "profiles" / "label_reduce_csr_1Mx36601_pmc.txt")
@@ -614,6 +712,9 @@ def main() -> int:
     lowp = None
     if not args.skip_lowp and not args.reduce_only:
         lowp = run_reduced_precision(args, log)
+    plain = None
+    if not args.skip_plain and not args.reduce_only:
+        plain = run_plain_path(args, log)
     batched = None
     if not args.skip_batched and not args.reduce_only:
         batched = run_batched_partitions(args, log)
@@ -642,6 +743,7 @@ def main() -> int:
                      "kernel": "label_reduce_rows_kernel", "kernel_ms": round(k_ms, 4), "algorithmic_bytes": b_alg_kernel},
         "cpu_baseline": cpu,
         "ops": ops,
+        "plain_path": plain,
         "reduced_precision": lowp,
         "batched_partitions": batched,
         "serving": serving,
