@@ -71,6 +71,17 @@ PROTOTYPES = {
     "csamd_label_reduce_graph_result": (i32, [P(A.label_reduce_graph), P(A.label_reduce_result)]),
     "csamd_label_reduce_graph_destroy": (None, [P(A.label_reduce_graph)]),
     "csamd_tune_partitions": (i32, [P(A.compressed_view), u32, u32, P(A.convert_workspace), P(A.partition_tune_result)]),
+    "csamd_choose_blocked_ell_block_size_from_coo": (i32, [P(A.host_coo), u32, vp, vp, u32, i32, P(A.blocked_ell_tune_result)]),
+    "csamd_blocked_ell_from_coo": (i32, [P(A.host_coo), u32, vp, u32, P(A.host_blocked_ell), i32]),
+    "csamd_blocked_ell_from_coo_auto": (i32, [P(A.host_coo), u32, vp, vp, u32, P(A.host_blocked_ell), i32, P(A.blocked_ell_tune_result)]),
+    "csamd_compressed_from_coo_scratch_bytes": (sz, [u32]),
+    "csamd_build_bucketed_blocked_ell": (i32, [P(A.blocked_ell_view), u32, u32, P(A.convert_workspace), P(A.bucketed_blocked_ell)]),
+    "csamd_choose_bucket_count_blocked_ell": (i32, [P(A.blocked_ell_view), P(A.convert_workspace), P(u32), P(C.c_uint64)]),
+    "csamd_bucketed_blocked_ell_clear": (None, [P(A.bucketed_blocked_ell)]),
+    "csamd_build_bucketed_sliced_ell": (i32, [P(A.sliced_ell_view), u32, u32, P(A.convert_workspace), P(A.bucketed_sliced_ell)]),
+    "csamd_choose_bucket_count_sliced_ell": (i32, [P(A.sliced_ell_view), P(A.convert_workspace), P(u32), P(C.c_uint64)]),
+    "csamd_bucketed_sliced_ell_clear": (None, [P(A.bucketed_sliced_ell)]),
+    "csamd_compressed_from_coo_raw": (i32, [u32, u32, u32, vp, vp, vp, u32, vp, u32, vp, vp, vp, vp, sz, vp]),
     "csamd_label_reduce_set_timing": (i32, [P(A.label_reduce_workspace), i32]),
     "csamd_serve_init": (None, [P(A.serve_workspace)]),
     "csamd_serve_clear": (None, [P(A.serve_workspace)]),

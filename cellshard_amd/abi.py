@@ -173,6 +173,28 @@ host_compressed = _aligned16("host_compressed", [
     ("storage", c_void_p), ("majorPtr", c_void_p), ("minorIdx", c_void_p), ("val", c_void_p),
 ])
 
+bucketed_blocked_ell = _aligned16("bucketed_blocked_ell", [
+    ("rows", c_u32), ("cols", c_u32), ("nnz", c_u32), ("segment_count", c_u32),
+    ("segments", C.POINTER(blocked_ell_view)), ("segment_row_offsets", C.POINTER(c_u32)),
+    ("exec_to_canonical_rows", c_void_p), ("canonical_to_exec_rows", c_void_p),
+    ("exec_to_canonical_cols", c_void_p), ("canonical_to_exec_cols", c_void_p),
+    ("bytes", C.c_uint64), ("storage", c_void_p),
+])
+
+bucketed_sliced_ell = _aligned16("bucketed_sliced_ell", [
+    ("rows", c_u32), ("cols", c_u32), ("nnz", c_u32), ("segment_count", c_u32),
+    ("segments", C.POINTER(sliced_ell_view)), ("segment_row_offsets", C.POINTER(c_u32)),
+    ("exec_to_canonical_rows", c_void_p), ("canonical_to_exec_rows", c_void_p),
+    ("canonical_slice_count", c_u32), ("reserved", c_u32),
+    ("canonical_slice_row_offsets", c_void_p), ("canonical_slice_widths", c_void_p),
+    ("bytes", C.c_uint64), ("storage", c_void_p),
+])
+
+host_coo = _aligned16("host_coo", [
+    ("rows", c_u32), ("cols", c_u32), ("nnz", c_u32), ("storage", c_void_p), ("rowIdx", c_void_p), ("colIdx", c_void_p),
+    ("val", c_void_p),
+])
+
 host_blocked_ell = _aligned16("host_blocked_ell", [
     ("rows", c_u32), ("cols", c_u32), ("nnz", c_u32), ("block_size", c_u32), ("ell_cols", c_u32), ("flags", c_u32),
     ("storage", c_void_p), ("blockColIdx", c_void_p), ("val", c_void_p),
@@ -248,6 +270,9 @@ ALL_STRUCTS = {
     "csamd_label_reduce_partition": label_reduce_partition,
     "csamd_label_reduce_graph": label_reduce_graph,
     "csamd_partition_tune_result": partition_tune_result,
+    "csamd_host_coo": host_coo,
+    "csamd_bucketed_blocked_ell": bucketed_blocked_ell,
+    "csamd_bucketed_sliced_ell": bucketed_sliced_ell,
     "csamd_convert_workspace": convert_workspace,
     "csamd_blocked_ell_tune_result": blocked_ell_tune_result,
     "csamd_sliced_ell_tune_result": sliced_ell_tune_result,

@@ -33,6 +33,8 @@ HIP_SOURCES = [
     "host_convert.hip",
     "serve.hip",
     "partition_tune.hip",
+    "coo_convert.hip",
+    "execution_layout.hip",
 ]
 
 HIPCC_FLAGS = [

@@ -178,6 +178,137 @@ def tune_partitions(parts, ws: ConvertWorkspace) -> list[dict]:
     return res
 
 
+def _host_coo(rows, cols_src, row_idx, col_idx, val16, keep):
+    arrs = [np.ascontiguousarray(row_idx, np.uint32), np.ascontiguousarray(col_idx, np.uint32),
+            np.ascontiguousarray(val16).view(np.uint16)]
+    keep += arrs
+    h = A.host_coo()
+    h.rows, h.cols, h.nnz = rows, cols_src, len(arrs[0])
+    h.rowIdx, h.colIdx, h.val = (a.ctypes.data if a.size else None for a in arrs)
+    return h
+
+
+def blocked_ell_from_coo(rows, cols_src, row_idx, col_idx, val16, cols, block_size, feature_to_global=None, device=0):
+    """Host COO (fp16 bits) -> host Blocked-ELL through the GPU: returns
+    (block_col_idx u32, val u16 bits, ell_cols, nnz)."""
+    keep = []
+    h = _host_coo(rows, cols_src, row_idx, col_idx, val16, keep)
+    fmap = None if feature_to_global is None else np.ascontiguousarray(feature_to_global, np.uint32)
+    dst = A.host_blocked_ell()
+    lib = load()
+    check(lib.csamd_blocked_ell_from_coo(C.byref(h), cols, fmap.ctypes.data if fmap is not None else None, block_size,
+                                         C.byref(dst), device), "blocked_ell_from_coo")
+    rb = (rows + block_size - 1) // block_size
+    w = dst.ell_cols // block_size
+    bci = np.ctypeslib.as_array(C.cast(dst.blockColIdx, C.POINTER(C.c_uint32)), (rb * w,)).copy() if rb * w else np.zeros(0, np.uint32)
+    val = (np.ctypeslib.as_array(C.cast(dst.val, C.POINTER(C.c_uint16)), (rows * dst.ell_cols,)).copy()
+           if rows * dst.ell_cols else np.zeros(0, np.uint16))
+    ell_cols, nnz = int(dst.ell_cols), int(dst.nnz)
+    lib.csamd_host_blocked_ell_clear(C.byref(dst))
+    return bci, val, ell_cols, nnz
+
+
+def choose_blocked_ell_block_size_from_coo(rows, cols_src, row_idx, col_idx, val16, cols, candidates, feature_to_global=None,
+                                           device=0):
+    keep = []
+    h = _host_coo(rows, cols_src, row_idx, col_idx, val16, keep)
+    fmap = None if feature_to_global is None else np.ascontiguousarray(feature_to_global, np.uint32)
+    cands = (C.c_uint32 * len(candidates))(*candidates)
+    out = A.blocked_ell_tune_result()
+    check(load().csamd_choose_blocked_ell_block_size_from_coo(C.byref(h), cols, fmap.ctypes.data if fmap is not None else None,
+                                                              cands, len(candidates), device, C.byref(out)),
+          "choose_blocked_ell_block_size_from_coo")
+    return int(out.block_size), float(out.fill_ratio), int(out.padded_bytes)
+
+
+def compressed_from_coo(rows, cols, row_idx: torch.Tensor, col_idx: torch.Tensor, val: torch.Tensor, value_type: int,
+                        feature_to_global: torch.Tensor | None = None, stream: int | None = None) -> DeviceCSR:
+    """Device COO -> device CSR (stable by row, then column)."""
+    lib = load()
+    nnz = int(row_idx.numel())
+    dev = row_idx.device
+    scratch = torch.empty(int(lib.csamd_compressed_from_coo_scratch_bytes(nnz)), dtype=torch.uint8, device=dev)
+    out_ptr = _empty_u32(rows + 1, dev)
+    out_idx = _empty_u32(nnz, dev)
+    out_val = torch.empty(nnz, dtype=val.dtype, device=dev)
+    check(lib.csamd_compressed_from_coo_raw(rows, cols, nnz, ptr(row_idx), ptr(col_idx), ptr(val), value_type, ptr(feature_to_global),
+                                            int(feature_to_global.numel()) if feature_to_global is not None else 0, ptr(out_ptr),
+                                            ptr(out_idx), ptr(out_val), ptr(scratch), scratch.numel(),
+                                            stream if stream is not None else current_stream_handle()), "compressed_from_coo")
+    torch.cuda.synchronize()
+    return DeviceCSR(rows, cols, out_ptr, out_idx, out_val, value_type, A.COMPRESSED_BY_ROW)
+
+
+def _bits_dtype(value_type):
+    return {1: np.uint8, 2: np.uint16, 4: np.uint32}[value_bytes(value_type)]
+
+
+def build_bucketed_blocked_ell(bell: DeviceBlockedELL, requested_bucket_count: int, ws: ConvertWorkspace) -> dict:
+    """Bucketed execution partition of a device Blocked-ELL, copied to host
+    numpy: segments [(rows, ell_cols, block_col_idx, val bits)], segment row
+    offsets, exec <-> canonical row maps, the packed-byte estimate."""
+    lib = load()
+    out = A.bucketed_blocked_ell()
+    v = bell.view()
+    check(lib.csamd_build_bucketed_blocked_ell(C.byref(v), bell.value_type, requested_bucket_count, C.byref(ws.ws), C.byref(out)),
+          "build_bucketed_blocked_ell")
+    try:
+        segs = []
+        dt = _bits_dtype(bell.value_type)
+        for k in range(out.segment_count):
+            sv = out.segments[k]
+            rb = (sv.rows + sv.block_size - 1) // sv.block_size
+            w = sv.ell_cols // sv.block_size
+            segs.append(dict(rows=int(sv.rows), ell_cols=int(sv.ell_cols), nnz=int(sv.nnz),
+                             block_col_idx=device_to_numpy(sv.blockColIdx, rb * w, np.uint32),
+                             val=device_to_numpy(sv.val, sv.rows * sv.ell_cols, dt)))
+        return dict(segments=segs, segment_row_offsets=[int(out.segment_row_offsets[k]) for k in range(out.segment_count + 1)],
+                    exec_to_canonical=device_to_numpy(out.exec_to_canonical_rows, out.rows, np.uint32),
+                    canonical_to_exec=device_to_numpy(out.canonical_to_exec_rows, out.rows, np.uint32),
+                    col_maps=(device_to_numpy(out.exec_to_canonical_cols, out.cols, np.uint32),
+                              device_to_numpy(out.canonical_to_exec_cols, out.cols, np.uint32)),
+                    bytes=int(out.bytes))
+    finally:
+        lib.csamd_bucketed_blocked_ell_clear(C.byref(out))
+
+
+def choose_bucket_count_blocked_ell(bell: DeviceBlockedELL, ws: ConvertWorkspace):
+    n, b = C.c_uint32(0), C.c_uint64(0)
+    v = bell.view()
+    check(load().csamd_choose_bucket_count_blocked_ell(C.byref(v), C.byref(ws.ws), C.byref(n), C.byref(b)), "choose_bucket_count")
+    return int(n.value), int(b.value)
+
+
+def build_bucketed_sliced_ell(sell: DeviceSlicedELL, requested_bucket_count: int, ws: ConvertWorkspace) -> dict:
+    lib = load()
+    out = A.bucketed_sliced_ell()
+    v = sell.view()
+    check(lib.csamd_build_bucketed_sliced_ell(C.byref(v), sell.value_type, requested_bucket_count, C.byref(ws.ws), C.byref(out)),
+          "build_bucketed_sliced_ell")
+    try:
+        segs = []
+        dt = _bits_dtype(sell.value_type)
+        for k in range(out.segment_count):
+            sv = out.segments[k]
+            width = int(device_to_numpy(sv.slice_widths, 1, np.uint32)[0]) if sv.slice_count else 0
+            segs.append(dict(rows=int(sv.rows), width=width, nnz=int(sv.nnz), slice_count=int(sv.slice_count),
+                             col_idx=device_to_numpy(sv.col_idx, sv.rows * width, np.uint32),
+                             val=device_to_numpy(sv.val, sv.rows * width, dt)))
+        return dict(segments=segs, segment_row_offsets=[int(out.segment_row_offsets[k]) for k in range(out.segment_count + 1)],
+                    exec_to_canonical=device_to_numpy(out.exec_to_canonical_rows, out.rows, np.uint32),
+                    canonical_to_exec=device_to_numpy(out.canonical_to_exec_rows, out.rows, np.uint32),
+                    bytes=int(out.bytes))
+    finally:
+        lib.csamd_bucketed_sliced_ell_clear(C.byref(out))
+
+
+def choose_bucket_count_sliced_ell(sell: DeviceSlicedELL, ws: ConvertWorkspace):
+    n, b = C.c_uint32(0), C.c_uint64(0)
+    v = sell.view()
+    check(load().csamd_choose_bucket_count_sliced_ell(C.byref(v), C.byref(ws.ws), C.byref(n), C.byref(b)), "choose_bucket_count_sliced")
+    return int(n.value), int(b.value)
+
+
 class TransposeScratch:
     def __init__(self):
         self.buf = None

@@ -575,6 +575,62 @@ typedef struct CSAMD_ALIGN16 csamd_partition_tune_result {
 int csamd_tune_partitions(const csamd_compressed_view *partitions, unsigned int partition_count, unsigned int value_type,
                           csamd_convert_workspace *workspace, csamd_partition_tune_result *out);
 
+/* Bucketed execution partitions (src/io/csh5/execution_builders_part.hh:1035-1596; the
+ * reference builds them on the host from host partitions).  Input: a device view; output:
+ * device segment views (each segment one width; Sliced-ELL segments one slice), host segment
+ * row offsets (in exec rows), device row maps exec <-> canonical, and the reference's packed
+ * byte estimate of the segments (2-byte values).  The library owns the output memory until
+ * the clear call.  Blocked-ELL: api.cuh:316-327; Sliced-ELL: api.cuh:394-406. */
+typedef struct CSAMD_ALIGN16 csamd_bucketed_blocked_ell {
+    unsigned int rows;
+    unsigned int cols;
+    unsigned int nnz;
+    unsigned int segment_count;
+    csamd_blocked_ell_view *segments;      /* host array [segment_count] of device views */
+    unsigned int *segment_row_offsets;     /* host [segment_count + 1] */
+    unsigned int *exec_to_canonical_rows;  /* device [rows] */
+    unsigned int *canonical_to_exec_rows;  /* device [rows] */
+    unsigned int *exec_to_canonical_cols;  /* device [cols] (identity) */
+    unsigned int *canonical_to_exec_cols;  /* device [cols] (identity) */
+    unsigned long long bytes;
+    void *storage;
+} csamd_bucketed_blocked_ell;
+
+typedef struct CSAMD_ALIGN16 csamd_bucketed_sliced_ell {
+    unsigned int rows;
+    unsigned int cols;
+    unsigned int nnz;
+    unsigned int segment_count;
+    csamd_sliced_ell_view *segments;       /* host array [segment_count] of device views */
+    unsigned int *segment_row_offsets;     /* host [segment_count + 1] */
+    unsigned int *exec_to_canonical_rows;  /* device [rows] */
+    unsigned int *canonical_to_exec_rows;  /* device [rows] */
+    unsigned int canonical_slice_count;
+    unsigned int reserved;
+    unsigned int *canonical_slice_row_offsets;  /* device [canonical_slice_count + 1] */
+    unsigned int *canonical_slice_widths;       /* device [canonical_slice_count] */
+    unsigned long long bytes;
+    void *storage;
+} csamd_bucketed_sliced_ell;
+
+/* execution_builders_part.hh:1206-1246 (layout :1035-1098, fill :1100-1204) */
+int csamd_build_bucketed_blocked_ell(const csamd_blocked_ell_view *src, unsigned int value_type,
+                                     unsigned int requested_bucket_count, csamd_convert_workspace *workspace,
+                                     csamd_bucketed_blocked_ell *out);
+/* execution_builders_part.hh:1248-1275 */
+int csamd_choose_bucket_count_blocked_ell(const csamd_blocked_ell_view *src, csamd_convert_workspace *workspace,
+                                          unsigned int *bucket_count, unsigned long long *bucketed_bytes);
+void csamd_bucketed_blocked_ell_clear(csamd_bucketed_blocked_ell *m);
+/* execution_builders_part.hh:1535-1568 (layout DP :1312-1429, fill :1498-1533) */
+int csamd_build_bucketed_sliced_ell(const csamd_sliced_ell_view *src, unsigned int value_type,
+                                    unsigned int requested_bucket_count, csamd_convert_workspace *workspace,
+                                    csamd_bucketed_sliced_ell *out);
+/* execution_builders_part.hh:1570-1596 */
+int csamd_choose_bucket_count_sliced_ell(const csamd_sliced_ell_view *src, csamd_convert_workspace *workspace,
+                                         unsigned int *bucket_count, unsigned long long *bucketed_bytes);
+void csamd_bucketed_sliced_ell_clear(csamd_bucketed_sliced_ell *m);
+
+
 
 void csamd_convert_init(csamd_convert_workspace *ws);
 void csamd_convert_clear(csamd_convert_workspace *ws);
@@ -758,6 +814,17 @@ typedef struct CSAMD_ALIGN16 csamd_host_sliced_ell {
     csamd_half *val;
 } csamd_host_sliced_ell;
 
+/* formats/triplet.cuh:14-25 (sparse::coo): host COO, fp16 values. */
+typedef struct CSAMD_ALIGN16 csamd_host_coo {
+    unsigned int rows;
+    unsigned int cols;
+    unsigned int nnz;
+    void *storage;
+    unsigned int *rowIdx;
+    unsigned int *colIdx;
+    csamd_half *val;
+} csamd_host_coo;
+
 /* Release host storage (the formats/ headers' clear()); resets metadata. */
 void csamd_host_compressed_clear(csamd_host_compressed *m);
 void csamd_host_blocked_ell_clear(csamd_host_blocked_ell *m);
@@ -773,6 +840,30 @@ int csamd_host_blocked_ell_from_compressed(const csamd_host_compressed *src, uns
 int csamd_host_blocked_ell_from_compressed_auto(const csamd_host_compressed *src, const unsigned int *candidates,
                                                 unsigned int candidate_count, csamd_host_blocked_ell *dst,
                                                 csamd_blocked_ell_tune_result *picked);
+/* convert/blocked_ell_from_coo_cuda.cuh:612-661: the tuner on a host COO (columns remapped
+ * through feature_to_global[src->cols] when it is not NULL; padded bytes at 2-byte values). */
+int csamd_choose_blocked_ell_block_size_from_coo(const csamd_host_coo *src, unsigned int cols,
+                                                 const unsigned int *feature_to_global, const unsigned int *candidates,
+                                                 unsigned int candidate_count, int device, csamd_blocked_ell_tune_result *out);
+/* convert/blocked_ell_from_coo_cuda.cuh:663-861: host COO -> host Blocked-ELL (dst reallocated
+ * with malloc).  Repeated (row, column) entries keep their last value in input order; entries
+ * outside rows x cols (after the remap) are rejected. */
+int csamd_blocked_ell_from_coo(const csamd_host_coo *src, unsigned int cols, const unsigned int *feature_to_global,
+                               unsigned int block_size, csamd_host_blocked_ell *dst, int device);
+/* convert/blocked_ell_from_coo_cuda.cuh:863-966 */
+int csamd_blocked_ell_from_coo_auto(const csamd_host_coo *src, unsigned int cols, const unsigned int *feature_to_global,
+                                    const unsigned int *candidates, unsigned int candidate_count, csamd_host_blocked_ell *dst,
+                                    int device, csamd_blocked_ell_tune_result *picked);
+/* Device COO -> device CSR (the sort those converters start with; project extension, the
+ * reference's compressed_from_coo_raw.cuh:97-225 has the same role): stable by (row, column),
+ * optional column remap, any value type; out_ptr[rows + 1], out_idx/out_val[nnz]. */
+size_t csamd_compressed_from_coo_scratch_bytes(unsigned int nnz);
+int csamd_compressed_from_coo_raw(unsigned int rows, unsigned int cols, unsigned int nnz, const unsigned int *d_row_idx,
+                                  const unsigned int *d_col_idx, const void *d_val, unsigned int value_type,
+                                  const unsigned int *d_feature_to_global, unsigned int feature_count, unsigned int *d_out_ptr,
+                                  unsigned int *d_out_idx, void *d_out_val, void *d_scratch, size_t scratch_bytes,
+                                  csamd_stream_t stream);
+
 /* convert/sliced_ell_from_compressed.cuh:86-94 */
 int csamd_host_choose_sliced_ell_slice_rows(const csamd_host_compressed *src, const unsigned int *candidates,
                                             unsigned int candidate_count, csamd_sliced_ell_tune_result *out);

@@ -416,3 +416,125 @@ def serve_query(rows, cols, ptr, idx, val, vt, features, G, op, labels=None, kee
                                  C.c_uint32(k), C.c_uint32(G), C.c_uint32(op), _p(labels), _p(keep), C.c_uint32(label_seed),
                                  C.c_uint32(mask_seed), C.c_uint32(thr24), _p(out)), "serve_query")
     return out
+
+
+# ---------------------------------------------------------------------------
+# Bucketed execution partitions (src/io/csh5/execution_builders_part.hh).
+# Layout rules restated in numpy / small Python loops; byte estimators from
+# include/CellShard/io/pack/packfile.cuh:110-116 and src/io/pack/packfile.cu:
+# 179-188 (16-byte / 13-byte headers, 2-byte values).
+# ---------------------------------------------------------------------------
+INVALID = 0xFFFFFFFF
+
+
+def bell_packed_bytes(rows, ell_cols, bs):
+    return 16 + 4 + 4 + ((rows + bs - 1) // bs) * (ell_cols // bs) * 4 + rows * ell_cols * 2
+
+
+def sell_packed_bytes(slices, slots):
+    return 13 + 4 + (slices + 1) * 4 + slices * 4 + slots * 6
+
+
+def bell_execution_layout(rows, bs, ell_cols, bci, requested):
+    """execution_builders_part.hh:1035-1098 + the byte fallback of :1206-1246.
+    Returns dict(order, begin, width, bytes)."""
+    rb = (rows + bs - 1) // bs
+    W = ell_cols // bs
+    widths = (np.asarray(bci, np.uint32).reshape(rb, W) != INVALID).sum(axis=1).astype(np.int64) if rb * W else np.zeros(rb, np.int64)
+    sortable = rb - 1 if rb and rows - (rb - 1) * bs != bs else rb
+    order = np.arange(rb)
+    order[:sortable] = np.argsort(widths[:sortable], kind="stable")
+
+    def layout(req):
+        if rb == 0:
+            return [0], []
+        nb = max(1, min(req, rb))
+        begin, width = [0], []
+        for b in range(nb):
+            lo, hi = b * rb // nb, (b + 1) * rb // nb
+            width.append(int(widths[order[lo:hi]].max()) if hi > lo else 0)
+            begin.append(hi)
+        if len(width) > 1 and all(w == width[0] for w in width):
+            begin, width = [0, rb], [width[0]]
+        return begin, width
+
+    def nbytes(begin, width):
+        t = 0
+        for s, w in enumerate(width):
+            seg_rows = sum(min(bs, rows - int(q) * bs) for q in order[begin[s]:begin[s + 1]])
+            t += bell_packed_bytes(seg_rows, w * bs, bs)
+        return t
+
+    begin, width = layout(requested)
+    b = nbytes(begin, width)
+    original = bell_packed_bytes(rows, ell_cols, bs)
+    if len(width) > 1 and b >= original:
+        begin, width = layout(1)
+        b = original
+    return dict(order=order, begin=begin, width=width, bytes=b, widths=widths)
+
+
+def bell_choose_bucket_count(rows, bs, ell_cols, bci):
+    """execution_builders_part.hh:1248-1275: fewest bytes over 1..min(8, row
+    blocks) buckets, ties to fewer."""
+    rb = (rows + bs - 1) // bs
+    best = None
+    for b in range(1, max(1, min(8, rb)) + 1):
+        nb = bell_execution_layout(rows, bs, ell_cols, bci, b)["bytes"]
+        if best is None or nb < best[1]:
+            best = (b, nb)
+    return best
+
+
+def bell_fill_reference(rows, bs, ell_cols, bci, val, lay):
+    """Segments filled as execution_builders_part.hh:1155-1204 does: row blocks
+    in layout order, valid slots compacted in slot order, values copied."""
+    W = ell_cols // bs
+    bci = np.asarray(bci, np.uint32).reshape(-1, W) if W else np.zeros(((rows + bs - 1) // bs, 0), np.uint32)
+    val = np.asarray(val).reshape(rows, ell_cols) if rows else np.asarray(val)
+    segs, exec_to_canon = [], []
+    for s, w in enumerate(lay["width"]):
+        rbs = lay["order"][lay["begin"][s]:lay["begin"][s + 1]]
+        seg_rows = sum(min(bs, rows - int(q) * bs) for q in rbs)
+        sb = np.full((len(rbs), w), INVALID, np.uint32)
+        sv = np.zeros((seg_rows, w * bs), val.dtype)
+        r0 = 0
+        for j, q in enumerate(rbs):
+            q = int(q)
+            rin = min(bs, rows - q * bs)
+            slots = np.nonzero(bci[q] != INVALID)[0]
+            sb[j, :len(slots)] = bci[q, slots]
+            for d, t in enumerate(slots):
+                sv[j * bs:j * bs + rin, d * bs:(d + 1) * bs] = val[q * bs:q * bs + rin, t * bs:(t + 1) * bs]
+            exec_to_canon.extend(range(q * bs, q * bs + rin))
+            r0 += rin
+        segs.append((seg_rows, sb.reshape(-1), sv.reshape(-1)))
+    return segs, np.asarray(exec_to_canon, np.uint32)
+
+
+def sell_row_widths(rows, offsets, widths, slot_offsets, col_idx):
+    out = np.zeros(rows, np.int64)
+    for s in range(len(widths)):
+        for r in range(int(offsets[s]), int(offsets[s + 1])):
+            b = int(slot_offsets[s]) + (r - int(offsets[s])) * int(widths[s])
+            out[r] = int((np.asarray(col_idx[b:b + int(widths[s])]) != INVALID).sum())
+    return out
+
+
+def sell_min_split_cost(sorted_widths, k):
+    """Brute-force DP (O(k n^2)): the fewest bytes of exactly k non-empty
+    segments of the sorted widths, a segment (i, j] costing
+    29 + 6 x w[j-1] x (j - i) (one slice, 2-byte values)."""
+    w = [int(x) for x in sorted_widths]
+    n = len(w)
+    F = sell_packed_bytes(1, 0)
+    INF = float("inf")
+    dp = [0] + [INF] * n
+    for _ in range(k):
+        nd = [INF] * (n + 1)
+        for j in range(1, n + 1):
+            for i in range(j):
+                if dp[i] < INF:
+                    nd[j] = min(nd[j], dp[i] + F + 6 * w[j - 1] * (j - i))
+        dp = nd
+    return dp[n]

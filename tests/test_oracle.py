@@ -411,3 +411,33 @@ def test_serve_seeded_labels_and_mask_are_uniform():
         k = O.serve_keep(5, n, int(frac * (1 << 24) + 0.5))
         assert abs(k.mean() - frac) < 0.01
     assert O.serve_keep(5, 100, 0).sum() == 0
+
+
+def test_bell_execution_layout_known_answer():
+    """18 rows, block 4: row blocks of valid widths [3, 1, 2, 1, 3], the last
+    one partial (2 rows) and left in place; stable order [1, 3, 2, 0, 4]; two
+    equal-count buckets [0, 2) and [2, 5) of widths 1 and 3; packed bytes
+    96 + 300 = 396 < 516 for the original, so the split stands."""
+    I = O.INVALID
+    bci = np.array([0, 1, 2, 5, I, I, 1, 4, I, 2, I, I, 0, 3, 6], np.uint32)
+    lay = O.bell_execution_layout(18, 4, 12, bci, 2)
+    assert list(lay["order"]) == [1, 3, 2, 0, 4]
+    assert lay["begin"] == [0, 2, 5] and lay["width"] == [1, 3] and lay["bytes"] == 396
+    assert O.bell_packed_bytes(8, 4, 4) == 96 and O.bell_packed_bytes(10, 12, 4) == 300 and O.bell_packed_bytes(18, 12, 4) == 516
+    # equal widths collapse to one segment; a split that does not shrink the bytes falls back
+    same = np.array([0, 1, I, 2, 3, I, 4, 5, I, 6, 7, I], np.uint32)
+    lay = O.bell_execution_layout(16, 4, 12, same, 3)
+    assert lay["begin"] == [0, 4] and lay["width"] == [2]
+    assert O.bell_choose_bucket_count(16, 4, 12, same)[0] == 1
+
+
+def test_sell_min_split_cost_is_exhaustive_minimum():
+    """The brute-force DP equals the minimum over every split of the sorted
+    widths into exactly k non-empty segments."""
+    from itertools import combinations
+    rng = np.random.default_rng(3)
+    for n, k in [(7, 3), (6, 1), (8, 4), (5, 5)]:
+        w = np.sort(rng.integers(0, 40, n))
+        best = min(sum(O.sell_packed_bytes(1, (b - a) * int(w[b - 1])) for a, b in zip((0,) + cuts, cuts + (n,)))
+                   for cuts in combinations(range(1, n), k - 1))
+        assert O.sell_min_split_cost(w, k) == best
