@@ -81,6 +81,7 @@ enum : std::uint32_t { blocked_ell_invalid_col = CSAMD_INVALID_COL, sliced_ell_i
 using compressed = csamd_host_compressed;
 using blocked_ell = csamd_host_blocked_ell;
 using sliced_ell = csamd_host_sliced_ell;
+using coo = csamd_host_coo;
 
 inline types::dim_t major_dim(const compressed *m) { return m->axis == compressed_by_col ? m->cols : m->rows; }
 inline types::dim_t minor_dim(const compressed *m) { return m->axis == compressed_by_col ? m->rows : m->cols; }
@@ -408,6 +409,32 @@ static inline int build_compressed_transpose_raw(types::dim_t cDim, types::dim_t
     return csamd_build_compressed_transpose_raw(cDim, uDim, nnz, d_cAxPtr, d_uAxIdx, d_val, d_out_uAxPtr, d_heads, d_out_cAxIdx,
                                                 d_out_val, d_scan_tmp, scan_bytes, (csamd_stream_t) stream);
 }
+// src/convert/blocked_ell_from_coo_cuda.cuh:612-966 (the workspace argument
+// is accepted for the signature; the library keeps its own).
+inline int choose_blocked_ell_block_size_from_coo_cuda(const sparse::coo *src, types::dim_t cols, const types::u32 *feature_to_global,
+                                                       const unsigned int *candidates, unsigned int candidate_count, int device,
+                                                       blocked_ell_tune_result *out, void *workspace = nullptr,
+                                                       hipStream_t stream = (hipStream_t) 0) {
+    (void) workspace;
+    (void) stream;
+    return csamd_choose_blocked_ell_block_size_from_coo(src, cols, feature_to_global, candidates, candidate_count, device, out);
+}
+inline int blocked_ell_from_coo_cuda(const sparse::coo *src, types::dim_t cols, const types::u32 *feature_to_global,
+                                     unsigned int block_size, sparse::blocked_ell *dst, int device, void *workspace = nullptr,
+                                     hipStream_t stream = (hipStream_t) 0) {
+    (void) workspace;
+    (void) stream;
+    return csamd_blocked_ell_from_coo(src, cols, feature_to_global, block_size, dst, device);
+}
+inline int blocked_ell_from_coo_cuda_auto(const sparse::coo *src, types::dim_t cols, const types::u32 *feature_to_global,
+                                          const unsigned int *candidates, unsigned int candidate_count, sparse::blocked_ell *dst,
+                                          int device, blocked_ell_tune_result *picked = nullptr, void *workspace = nullptr,
+                                          hipStream_t stream = (hipStream_t) 0) {
+    (void) workspace;
+    (void) stream;
+    return csamd_blocked_ell_from_coo_auto(src, cols, feature_to_global, candidates, candidate_count, dst, device, picked);
+}
+
 static inline int build_transpose_cs_from_cs_raw(types::dim_t cDim, types::dim_t uDim, types::nnz_t nnz, const types::ptr_t *a,
                                                  const types::idx_t *b, const real::storage_t *c, types::ptr_t *d, types::ptr_t *e,
                                                  types::idx_t *f, real::storage_t *g, void *h, std::size_t i, hipStream_t s) {
@@ -415,6 +442,48 @@ static inline int build_transpose_cs_from_cs_raw(types::dim_t cDim, types::dim_t
 }
 
 } // namespace convert
+
+// ---------------------------------------------------------------------------
+// Bucketed execution partitions (src/io/csh5/execution_builders_part.hh:
+// 1206-1275, 1535-1596) over device views; segments stay on the device.
+namespace execution {
+
+using bucketed_blocked_ell_partition = csamd_bucketed_blocked_ell;
+using bucketed_sliced_ell_partition = csamd_bucketed_sliced_ell;
+
+inline int build_bucketed_execution_partition(bucketed_blocked_ell_partition *out, const csamd_blocked_ell_view *part,
+                                              std::uint32_t requested_bucket_count, csamd_convert_workspace *ws,
+                                              std::uint64_t *bucketed_bytes_out = nullptr, unsigned int value_type = CSAMD_VALUE_F16) {
+    if (!csamd_build_bucketed_blocked_ell(part, value_type, requested_bucket_count, ws, out)) return 0;
+    if (bucketed_bytes_out != nullptr) *bucketed_bytes_out = out->bytes;
+    return 1;
+}
+inline int choose_bucket_count_for_part(const csamd_blocked_ell_view *part, csamd_convert_workspace *ws, std::uint32_t *bucket_count_out,
+                                        std::uint64_t *bucketed_bytes_out) {
+    unsigned long long b = 0ull;
+    if (!csamd_choose_bucket_count_blocked_ell(part, ws, bucket_count_out, &b)) return 0;
+    *bucketed_bytes_out = b;
+    return 1;
+}
+inline int build_bucketed_sliced_execution_partition(bucketed_sliced_ell_partition *out, const csamd_sliced_ell_view *part,
+                                                     std::uint32_t requested_bucket_count, csamd_convert_workspace *ws,
+                                                     std::uint64_t *bucketed_bytes_out = nullptr,
+                                                     unsigned int value_type = CSAMD_VALUE_F16) {
+    if (!csamd_build_bucketed_sliced_ell(part, value_type, requested_bucket_count, ws, out)) return 0;
+    if (bucketed_bytes_out != nullptr) *bucketed_bytes_out = out->bytes;
+    return 1;
+}
+inline int choose_bucket_count_for_sliced_part(const csamd_sliced_ell_view *part, csamd_convert_workspace *ws,
+                                               std::uint32_t *bucket_count_out, std::uint64_t *bucketed_bytes_out) {
+    unsigned long long b = 0ull;
+    if (!csamd_choose_bucket_count_sliced_ell(part, ws, bucket_count_out, &b)) return 0;
+    *bucketed_bytes_out = b;
+    return 1;
+}
+inline void clear(bucketed_blocked_ell_partition *p) { csamd_bucketed_blocked_ell_clear(p); }
+inline void clear(bucketed_sliced_ell_partition *p) { csamd_bucketed_sliced_ell_clear(p); }
+
+} // namespace execution
 
 // ---------------------------------------------------------------------------
 namespace bucket {
