@@ -215,8 +215,6 @@ __global__ void __launch_bounds__(TR_SORT_THREADS) tr_band_kernel(unsigned int c
     }
     __syncthreads();
 
-    const unsigned int wave = threadIdx.x >> 6, lane = lane_id();
-    constexpr unsigned int NW = TR_SORT_THREADS / 64u;
     unsigned int rb0 = 0u;
     while (rb0 < nr) {
         // batch: rows [rb0, rb1) with at most cap entries (a row has at most
@@ -234,10 +232,14 @@ __global__ void __launch_bounds__(TR_SORT_THREADS) tr_band_kernel(unsigned int c
         const unsigned int rb1 = ctl[0];
         const unsigned int base = seg_o[rb0];
         const unsigned int n = seg_o[rb1] - base;
-        // source row of every entry of the batch (LDS only)
-        for (unsigned int r = rb0 + wave; r < rb1; r += NW) {
-            const unsigned int o = seg_o[r] - base, len = seg_o[r + 1u] - seg_o[r];
-            for (unsigned int k = lane; k < len; k += 64u) s_in[o + k] = (unsigned short) r;
+        // source row of every entry of the batch (LDS only; one thread per
+        // row, its stores independent of each other)
+        {
+            const unsigned int r = threadIdx.x;  // TR_SORT_THREADS == TR_ROWS
+            if (r >= rb0 && r < rb1) {
+                const unsigned int o = seg_o[r] - base, len = seg_o[r + 1u] - seg_o[r];
+                for (unsigned int k = 0; k < len; ++k) s_in[o + k] = (unsigned short) r;
+            }
         }
         __syncthreads();
         // A: load this thread's entries (all loads issued before any use),
