@@ -32,6 +32,7 @@ constexpr unsigned int PT_THREADS = 256u;
 constexpr unsigned int PT_GROUP_ROWS = 64u;
 constexpr unsigned int PT_MAX_WORDS = 2048u;  // 8 bitmaps x 2048 words = 64 KB of LDS: cols <= 524,288
 constexpr unsigned int PT_HIST = 32u;
+constexpr unsigned int PT_UNROLL = 4u;  // index loads in flight per lane
 
 struct pt_part {
     const unsigned int *ptr;
@@ -75,6 +76,7 @@ __global__ void __launch_bounds__(PT_THREADS) pt_tune_kernel(const pt_part *__re
     extern __shared__ unsigned int bits[];  // [8][nw]
     __shared__ unsigned int red[PT_THREADS / 64u];
     __shared__ unsigned int hist[PT_HIST];
+    __shared__ unsigned int rp[PT_GROUP_ROWS + 1u];  // the group's row pointers
     const unsigned int gi = blockIdx.x;
     unsigned int lo = 0u, hi = nparts - 1u;  // last partition whose first group <= gi
     while (lo < hi) {
@@ -110,16 +112,27 @@ __global__ void __launch_bounds__(PT_THREADS) pt_tune_kernel(const pt_part *__re
             atomicAdd(&A->sell_padded[1], (unsigned long long) nr * m64);
         }
     }
-    // distinct 8-column blocks per 8-row block
+    // distinct 8-column blocks per 8-row block: one wave per row, PT_UNROLL
+    // independent loads in flight per lane
+    if (threadIdx.x <= nr) rp[threadIdx.x] = P.ptr[r0 + threadIdx.x];
+    __syncthreads();
     const unsigned int wave = threadIdx.x >> 6, lane = lane_id();
     for (unsigned int r = wave; r < nr; r += PT_THREADS / 64u) {
         unsigned int *bm = bits + (r >> 3) * nw;
-        const unsigned int b = P.ptr[r0 + r], e = P.ptr[r0 + r + 1u];
-        for (unsigned int i = b + lane; i < e; i += 64u) {
-            const unsigned int c = P.idx[i];
-            if (c < cols) {
-                const unsigned int bc = c >> 3;
-                atomicOr(&bm[bc >> 5], 1u << (bc & 31u));
+        const unsigned int b = rp[r], e = rp[r + 1u];
+        for (unsigned int i0 = b + lane; i0 < e; i0 += 64u * PT_UNROLL) {
+            unsigned int c[PT_UNROLL];
+#pragma unroll
+            for (unsigned int u = 0; u < PT_UNROLL; ++u) {
+                const unsigned int i = i0 + 64u * u;
+                c[u] = i < e ? P.idx[i] : CSAMD_INVALID_COL;
+            }
+#pragma unroll
+            for (unsigned int u = 0; u < PT_UNROLL; ++u) {
+                if (c[u] < cols) {
+                    const unsigned int bc = c[u] >> 3;
+                    atomicOr(&bm[bc >> 5], 1u << (bc & 31u));
+                }
             }
         }
     }

@@ -33,8 +33,13 @@ __device__ __forceinline__ unsigned int hash4(unsigned int seed, unsigned int a,
     return lowbias32(lowbias32(lowbias32(seed ^ (stream * 0x9e3779b9u)) ^ a) + b * 0x85ebca6bu);
 }
 
-// uniform in (0, 1)
+// uniform in (0, 1]: in float, (2^24 - 1) + 0.5 rounds to 2^24, so the
+// largest draw is 1.0 (used where that is harmless: log keys, Box-Muller)
 __device__ __forceinline__ float unit_open(unsigned int h) { return ((float) (h >> 8) + 0.5f) * (1.0f / 16777216.0f); }
+// uniform in (0, 1), exactly: the count sampler inverts a CDF that never
+// reaches 1 in floating point, so a draw of 1.0 would run its search to the
+// underflow of the probability terms (counts of ~1e5, +inf in fp16)
+__device__ __forceinline__ double unit_open_d(unsigned int h) { return ((double) (h >> 8) + 0.5) * (1.0 / 16777216.0); }
 
 __device__ __forceinline__ unsigned int ordered_bits(float f) {
     const unsigned int b = __float_as_uint(f);
@@ -65,7 +70,7 @@ __device__ __forceinline__ unsigned int nb_zero_truncated(unsigned int seed, uns
     const double p = rr / (rr + m);
     const double q = 1.0 - p;
     const double p0 = pow(p, rr);
-    const double u = (double) unit_open(hash4(seed, row, col, 2u));
+    const double u = unit_open_d(hash4(seed, row, col, 2u));
     const double t = p0 + u * (1.0 - p0);
     double pk = p0, cdf = p0;
     unsigned int k = 0u;
