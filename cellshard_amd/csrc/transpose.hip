@@ -31,7 +31,7 @@ namespace {
 
 constexpr unsigned int TR_ROWS = 256u;          // source rows per tile (bits per segment mask)
 constexpr unsigned int TR_MASK_WORDS = TR_ROWS / 32u;
-constexpr unsigned int TR_BAND = 512u;          // output segments per band
+constexpr unsigned int TR_BAND = 256u;          // output segments per band
 constexpr unsigned int TR_MAX_BANDS = 4096u;    // beyond: fallback path (row band table would be too large)
 constexpr unsigned int TR_COUNT_THREADS = 1024u;
 constexpr unsigned int TR_SORT_THREADS = 256u;
@@ -174,7 +174,8 @@ __global__ void __launch_bounds__(TR_SORT_THREADS) tr_band_kernel(unsigned int c
                                                                   const unsigned int *__restrict__ out_ptr,
                                                                   const unsigned int *__restrict__ rbt, unsigned int bands,
                                                                   unsigned int *__restrict__ out_idx, S *__restrict__ out_val) {
-    static_assert(TR_SORT_THREADS == TR_ROWS && 2u * TR_SORT_THREADS == TR_BAND, "one row / two segments per thread");
+    static_assert(TR_SORT_THREADS == TR_ROWS && TR_BAND % TR_SORT_THREADS == 0u, "one row / whole segments per thread");
+    constexpr unsigned int CPT = TR_BAND / TR_SORT_THREADS;  // segments per thread in the run-start scan
     constexpr unsigned int cap = K * TR_SORT_THREADS;
     static_assert(cap >= TR_BAND, "a batch must hold a full row");
     extern __shared__ __align__(16) unsigned char lds[];
@@ -262,18 +263,23 @@ __global__ void __launch_bounds__(TR_SORT_THREADS) tr_band_kernel(unsigned int c
         __syncthreads();
         // B: run lengths -> run starts in the staging
         {
-            const unsigned int cc = threadIdx.x;
-            unsigned int cnt0 = 0u, cnt1 = 0u;
+            unsigned int cnt[CPT], sum = 0u;
 #pragma unroll
-            for (unsigned int w = 0; w < TR_MASK_WORDS; ++w) {
-                cnt0 += (unsigned int) __popc(masks[(2u * cc) * TR_MASK_WORDS + w]);
-                cnt1 += (unsigned int) __popc(masks[(2u * cc + 1u) * TR_MASK_WORDS + w]);
+            for (unsigned int k = 0; k < CPT; ++k) {
+                const unsigned int cc = CPT * threadIdx.x + k;
+                cnt[k] = 0u;
+#pragma unroll
+                for (unsigned int w = 0; w < TR_MASK_WORDS; ++w) cnt[k] += (unsigned int) __popc(masks[cc * TR_MASK_WORDS + w]);
+                sum += cnt[k];
             }
             unsigned int total = 0u;
-            const unsigned int o = block_exclusive_scan_u32(cnt0 + cnt1, scan_smem, &total);
-            lstart[2u * cc] = o;
-            lstart[2u * cc + 1u] = o + cnt0;
-            if (cc == 0u) lstart[TR_BAND] = total;
+            unsigned int o = block_exclusive_scan_u32(sum, scan_smem, &total);
+#pragma unroll
+            for (unsigned int k = 0; k < CPT; ++k) {
+                lstart[CPT * threadIdx.x + k] = o;
+                o += cnt[k];
+            }
+            if (threadIdx.x == 0u) lstart[TR_BAND] = total;
         }
         __syncthreads();
         // C: every entry to its run start + rank among earlier rows
@@ -353,12 +359,12 @@ int launch_scatter(const tr_layout &L, bool sorted, unsigned int cDim, unsigned 
     if (sorted) {
         const unsigned long long blocks = (unsigned long long) L.tiles * L.bands;
         CSAMD_REQUIRE(blocks <= 0x7fffffffull, "transpose: too many chunks");
-        // staging of K x 256 entries per batch (CSAMD_TR_K = 8 / 12 / 24 for sweeps)
+        // staging of K x 256 entries per batch (CSAMD_TR_K = 8 / 12 / 16 for sweeps)
         const char *env = std::getenv("CSAMD_TR_K");
         const unsigned int want = env != nullptr ? (unsigned int) std::atoi(env) : 12u;
-        const unsigned int K = want == 8u ? 8u : want == 24u ? (sizeof(S) >= 4u ? 18u : 24u) : 12u;
+        const unsigned int K = want == 8u ? 8u : want == 16u ? 16u : 12u;
         const size_t lds = band_lds_bytes(K * TR_SORT_THREADS, sizeof(S));
-        auto k = K == 8u ? tr_band_kernel<S, 8u> : K == 12u ? tr_band_kernel<S, 12u> : tr_band_kernel<S, (sizeof(S) >= 4u ? 18u : 24u)>;
+        auto k = K == 8u ? tr_band_kernel<S, 8u> : K == 12u ? tr_band_kernel<S, 12u> : tr_band_kernel<S, 16u>;
         CSAMD_HIP(hipFuncSetAttribute((const void *) k, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds), "set lds");
         k<<<(unsigned int) blocks, TR_SORT_THREADS, lds, s>>>(cDim, uDim, idx, (const S *) val, tile_off, out_ptr, rbt, L.bands,
                                                               out_idx, (S *) out_val);
