@@ -202,7 +202,7 @@ __device__ __forceinline__ float decode_any(const void *val, unsigned long long 
 // device-scope atomics on a few hot label counters serialize.
 // ---------------------------------------------------------------------------
 constexpr unsigned int LR_SORT_THREADS = 1024u;
-constexpr unsigned int LR_SORT_ITEMS = 8u;
+constexpr unsigned int LR_SORT_ITEMS = 2u;
 constexpr unsigned int LR_SORT_TILE = LR_SORT_THREADS * LR_SORT_ITEMS;
 
 __device__ __forceinline__ unsigned int row_key(const unsigned int *__restrict__ labels, const unsigned char *__restrict__ row_keep,
@@ -282,6 +282,7 @@ __global__ void __launch_bounds__(1024) label_offsets_kernel(unsigned int *__res
                                                              uint2 *__restrict__ pairs, unsigned int *__restrict__ fids,
                                                              unsigned int *__restrict__ f_total) {
     __shared__ unsigned int smem[16];
+    __shared__ unsigned int stage[1024u * LR_OFF_ITEMS];
     unsigned int fcarry = 0u;  // same value in every thread
     for (unsigned int start = 0; start < nwords; start += blockDim.x) {
         const unsigned int w = start + threadIdx.x;
@@ -299,23 +300,37 @@ __global__ void __launch_bounds__(1024) label_offsets_kernel(unsigned int *__res
         fcarry += total;
     }
     if (threadIdx.x == 0) *f_total = fcarry;
+    // the table in passes of 1024 x LR_OFF_ITEMS entries, loaded and stored
+    // coalesced through LDS; each thread scans LR_OFF_ITEMS consecutive ones
     const unsigned long long n = (unsigned long long) (G + 1u) * nb;
     unsigned int carry = 0u;  // same value in every thread
-    for (unsigned long long start = 0; start < n; start += (unsigned long long) blockDim.x * LR_OFF_ITEMS) {
-        const unsigned long long i0 = start + (unsigned long long) threadIdx.x * LR_OFF_ITEMS;
+    for (unsigned long long start = 0; start < n; start += 1024ull * LR_OFF_ITEMS) {
+#pragma unroll
+        for (unsigned int k = 0; k < LR_OFF_ITEMS; ++k) {
+            const unsigned long long i = start + k * 1024u + threadIdx.x;
+            stage[k * 1024u + threadIdx.x] = i < n ? hist[i] : 0u;
+        }
+        __syncthreads();
         unsigned int v[LR_OFF_ITEMS], own = 0u;
 #pragma unroll
         for (unsigned int k = 0; k < LR_OFF_ITEMS; ++k) {
-            v[k] = i0 + k < n ? hist[i0 + k] : 0u;
+            v[k] = stage[threadIdx.x * LR_OFF_ITEMS + k];
             own += v[k];
         }
         unsigned int total = 0u;
         unsigned int o = carry + block_exclusive_scan_u32(own, smem, &total);
 #pragma unroll
         for (unsigned int k = 0; k < LR_OFF_ITEMS; ++k) {
-            if (i0 + k < n) hist[i0 + k] = o;
+            stage[threadIdx.x * LR_OFF_ITEMS + k] = o;
             o += v[k];
         }
+        __syncthreads();
+#pragma unroll
+        for (unsigned int k = 0; k < LR_OFF_ITEMS; ++k) {
+            const unsigned long long i = start + k * 1024u + threadIdx.x;
+            if (i < n) hist[i] = stage[k * 1024u + threadIdx.x];
+        }
+        __syncthreads();
         carry += total;
     }
     __syncthreads();  // the block's table stores are visible to the block
