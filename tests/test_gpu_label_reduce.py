@@ -129,7 +129,7 @@ def test_repeatable_and_alone_equals_mixed(lr):
 
 
 def test_synthetic_counts_parity(lr):
-    """Generator output (config-2 style counts) through the reduce vs oracle."""
+    """Generator output (fp16 counts, LogNormal row lengths, Zipf columns) through the reduce vs oracle."""
     ws = ConvertWorkspace()
     src = synth_csr(4000, 5000, median=300, sigma=0.6, seed=12, value_type=F16, ws=ws)
     ptr, idx, bits = src.to_numpy()
