@@ -6,10 +6,11 @@
 // segment the source major index ascends (what csr2cscEx2 and scipy give).
 //
 // gfx950 plan (sources whose rows hold strictly ascending minor indices):
-//   K1 count: one 1024-thread workgroup per tile of TR_ROWS source rows, a
-//      u32 LDS histogram over all output segments (<= 40k fit in LDS), one
-//      wave per row; the same pass records where each TR_BAND-wide column
-//      band starts inside each row, band-major per tile (rbt[t][band][row]);
+//   K1 count: one 1024-thread workgroup per tile of TR_ROWS source rows, one
+//      wave per row, an LDS histogram over all output segments; the same
+//      pass records where each TR_BAND-wide column band starts inside each
+//      row, band-major per tile (rbt[t][band][row]), staged in LDS and
+//      written whole where it fits (about 40k segments);
 //   K2 per segment: exclusive running sums over tiles in place (each
 //      tile's first position inside the segment) and the segment totals;
 //      a scan of the totals gives out_ptr;
@@ -20,6 +21,7 @@
 //      segment's run (its entries from this tile, rows ascending) is written
 //      to its final place with consecutive lanes on consecutive positions.
 //      Chunks larger than the LDS staging go in batches of whole rows.
+//      Chunks are ordered so that each XCD's L2 sees neighbouring chunks.
 // Sources with unsorted or repeated minor indices take a fallback scatter
 // (K4u: LDS cursors, rows in order, entries of one row in any order).
 #include "common.hh"
@@ -66,8 +68,62 @@ tr_layout make_layout(unsigned int cDim, unsigned int uDim) {
     return L;
 }
 
-// K1.  flags[0]: some row's minor indices are not strictly ascending;
+// K1 row walk of the general count kernel: one wave per source row, 64 U
+// entries per pass with U independent loads per lane (all issued before any
+// use); the previous entry's column comes from the neighbour lane.
+// count(c) counts an entry of column c; open(k, i) records that band k of
+// the row starts at entry i (every band 0..bands of a strictly ascending row
+// is opened exactly once; band `bands` is the row end).
+// flags[0]: some row's minor indices are not strictly ascending;
 // flags[1]: a minor index >= uDim.
+template<unsigned int U, typename Count, typename Open>
+__device__ __forceinline__ void tr_walk_row(unsigned int b, unsigned int e, unsigned int uDim, unsigned int bands,
+                                            const unsigned int *__restrict__ idx, unsigned int lane, unsigned int *__restrict__ flags,
+                                            Count count, Open open) {
+    if (b == e) {
+        for (unsigned int k = lane; k <= bands; k += 64u) open(k, b);
+        return;
+    }
+    unsigned int carry = CSAMD_INVALID_COL;  // column of the entry before the pass
+    for (unsigned int i0 = b; i0 < e; i0 += 64u * U) {
+        unsigned int c[U];
+#pragma unroll
+        for (unsigned int u = 0; u < U; ++u) {
+            const unsigned int i = i0 + 64u * u + lane;
+            c[u] = i < e ? idx[i] : CSAMD_INVALID_COL;
+        }
+#pragma unroll
+        for (unsigned int u = 0; u < U; ++u) {
+            const unsigned int i = i0 + 64u * u + lane;
+            const unsigned int up = (unsigned int) __shfl_up((int) c[u], 1);
+            const unsigned int block_prev = u == 0u ? carry : (unsigned int) __shfl((int) c[u == 0u ? 0u : u - 1u], 63);
+            const unsigned int pc = lane == 0u ? block_prev : up;
+            if (i >= e) continue;
+            const unsigned int cc = c[u];
+            if (cc >= uDim) {
+                flags[1] = 1u;
+                continue;
+            }
+            count(cc);
+            // entry i opens every band in (band of entry i - 1, band of cc]
+            const unsigned int band = cc / TR_BAND;
+            int prev = -1;
+            if (i > b) {
+                prev = pc < uDim ? (int) (pc / TR_BAND) : (int) bands;
+                if (pc >= cc) flags[0] = 1u;
+            }
+            for (int k = prev + 1; k <= (int) band; ++k) open((unsigned int) k, i);
+            if (i == e - 1u)
+                for (unsigned int k = band + 1u; k <= bands; ++k) open(k, e);
+        }
+        carry = (unsigned int) __shfl((int) c[U - 1u], 63);
+    }
+}
+
+// K1 (general): counts in an LDS u32 histogram over all segments
+// (use_lds) or straight into the tile's global counts; band starts
+// straight into rbt[t][band][row] (rbt may be null).
+template<unsigned int U>
 __global__ void __launch_bounds__(TR_COUNT_THREADS) tr_count_kernel(unsigned int cDim, unsigned int uDim, const unsigned int *__restrict__ ptr,
                                                                     const unsigned int *__restrict__ idx, unsigned int bands,
                                                                     int use_lds, unsigned int *__restrict__ counts,
@@ -85,55 +141,191 @@ __global__ void __launch_bounds__(TR_COUNT_THREADS) tr_count_kernel(unsigned int
     const unsigned int wave = threadIdx.x >> 6, nw = blockDim.x >> 6, lane = lane_id();
     for (unsigned int r = r0 + wave; r < r1; r += nw) {
         const unsigned int rl = r - r0;
-        const unsigned int b = ptr[r], e = ptr[r + 1u];
-        if (b == e) {
-            if (tile_rbt != nullptr)
-                for (unsigned int k = lane; k <= bands; k += 64u) tile_rbt[(size_t) k * TR_ROWS + rl] = b;
-            continue;
-        }
-        // 256 entries per pass, four independent loads per lane; the
-        // previous entry's column comes from the neighbour lane
-        unsigned int carry = CSAMD_INVALID_COL;  // column of the entry before the pass
-        for (unsigned int i0 = b; i0 < e; i0 += 256u) {
-            unsigned int c[4];
-#pragma unroll
-            for (unsigned int u = 0; u < 4u; ++u) {
-                const unsigned int i = i0 + 64u * u + lane;
-                c[u] = i < e ? idx[i] : CSAMD_INVALID_COL;
-            }
-#pragma unroll
-            for (unsigned int u = 0; u < 4u; ++u) {
-                const unsigned int i = i0 + 64u * u + lane;
-                const unsigned int up = (unsigned int) __shfl_up((int) c[u], 1);
-                const unsigned int block_prev = u == 0u ? carry : (unsigned int) __shfl((int) c[u == 0u ? 0u : u - 1u], 63);
-                const unsigned int pc = lane == 0u ? block_prev : up;
-                if (i >= e) continue;
-                const unsigned int cc = c[u];
-                if (cc >= uDim) {
-                    flags[1] = 1u;
-                    continue;
-                }
-                if (use_lds) atomicAdd(&hist[cc], 1u);
-                else atomicAdd(&tile_counts[cc], 1u);
-                if (tile_rbt == nullptr) continue;
-                // band starts: entry i opens every band in (band of entry i - 1, band of cc]
-                const unsigned int band = cc / TR_BAND;
-                int prev = -1;
-                if (i > b) {
-                    prev = pc < uDim ? (int) (pc / TR_BAND) : (int) bands;
-                    if (pc >= cc) flags[0] = 1u;
-                }
-                for (int k = prev + 1; k <= (int) band; ++k) tile_rbt[(size_t) k * TR_ROWS + rl] = i;
-                if (i == e - 1u)
-                    for (unsigned int k = band + 1u; k <= bands; ++k) tile_rbt[(size_t) k * TR_ROWS + rl] = e;
-            }
-            carry = (unsigned int) __shfl((int) c[3], 63);
-        }
+        tr_walk_row<U>(
+            ptr[r], ptr[r + 1u], uDim, bands, idx, lane, flags,
+            [&](unsigned int c) {
+                if (use_lds) atomicAdd(&hist[c], 1u);
+                else atomicAdd(&tile_counts[c], 1u);
+            },
+            [&](unsigned int k, unsigned int i) {
+                if (tile_rbt != nullptr) tile_rbt[(size_t) k * TR_ROWS + rl] = i;
+            });
     }
     if (use_lds) {
         __syncthreads();
         for (unsigned int c = threadIdx.x; c < uDim; c += blockDim.x) tile_counts[c] = hist[c];
     }
+}
+
+// K1 (packed, uDim < 65536 and the tile's tables fit in LDS): two u16
+// counters per LDS word (a strictly ascending row adds at most 1 per
+// segment, so a tile's count stays <= TR_ROWS), and the band starts as u16
+// offsets from the row start in LDS, written out coalesced at the end.
+// Scattered 4-byte stores straight into rbt reach HBM as partial lines;
+// this way every rbt line is written once and whole.  A tile with a row
+// that is not strictly ascending may overflow a u16 counter: flags[0] is
+// then set and the host counts again with tr_count_kernel.
+//
+// The tile's entries are walked flat, not row by row, in groups of 4
+// consecutive entries (one 16-byte load per lane and group): pass p covers
+// 64 U groups from the tile's first group + 64 U p, wave w takes passes w,
+// w + 16, ..., and the loads of a wave's next pass are in flight while the
+// current one is counted.  Each lane finds its entries' row from the row
+// starts in LDS (binary search once per pass, then a forward walk).  The
+// group holding the tile's first entry may start in the previous tile and
+// the one holding its last may end in the next (those entries are skipped);
+// the array's final partial group, if any, is read one entry at a time.
+template<unsigned int U>
+__global__ void __launch_bounds__(TR_COUNT_THREADS) tr_count_packed_kernel(unsigned int cDim, unsigned int uDim, unsigned int nnz,
+                                                                           const unsigned int *__restrict__ ptr,
+                                                                           const unsigned int *__restrict__ idx, unsigned int bands,
+                                                                           unsigned int *__restrict__ counts, unsigned int *__restrict__ rbt,
+                                                                           unsigned int *__restrict__ flags) {
+    constexpr unsigned int PG = 64u * U;  // groups per pass
+    constexpr unsigned int NW = TR_COUNT_THREADS / 64u;
+    extern __shared__ unsigned int lds_words[];
+    const unsigned int hw = (uDim + 1u) >> 1;
+    unsigned int *hist = lds_words;                                  // [hw] two u16 counters each
+    unsigned int *rp = hist + hw;                                    // [TR_ROWS + 1] row starts (+ pad)
+    unsigned short *off = (unsigned short *) (rp + TR_ROWS + 4u);    // [(bands + 1) * TR_ROWS]
+    const unsigned int t = blockIdx.x;
+    const unsigned int r0 = t * TR_ROWS;
+    const unsigned int nr = cDim - r0 < TR_ROWS ? cDim - r0 : TR_ROWS;
+    const unsigned int tid = threadIdx.x;
+    for (unsigned int w = tid; w < hw; w += blockDim.x) hist[w] = 0u;
+    for (unsigned int r = tid; r <= nr; r += blockDim.x) rp[r] = ptr[r0 + r];
+    __syncthreads();
+    // empty rows: every band starts at offset 0
+    for (unsigned int r = tid; r < nr; r += blockDim.x)
+        if (rp[r] == rp[r + 1u])
+            for (unsigned int k = 0; k <= bands; ++k) off[k * TR_ROWS + r] = 0u;
+    const unsigned int E0 = rp[0], E1 = rp[nr];
+    const unsigned int wave = tid >> 6, lane = lane_id();
+    const unsigned int g0 = E0 >> 2, g1 = (E1 + 3u) >> 2;  // groups [g0, g1) hold the tile
+    const unsigned int ng = E1 > E0 ? g1 - g0 : 0u;
+    const unsigned int gfull = nnz >> 2;                     // groups wholly inside the array
+    const uint4 *idx4 = (const uint4 *) idx;
+
+    // 16-byte loads only (groups past the array's last whole group read that
+    // group; their entries are skipped here and visited one by one at the end)
+    auto load = [&](unsigned int p, uint4 (&c)[U], unsigned int &before) {
+        const unsigned int gp = g0 + p * PG;
+#pragma unroll
+        for (unsigned int u = 0; u < U; ++u) {
+            const unsigned int g = gp + 64u * u + lane;
+            const unsigned int gc = g < g1 ? g : g1 - 1u;
+            c[u] = idx4[gc < gfull ? gc : gfull - 1u];
+        }
+        before = idx[gp > 0u ? gp * 4u - 1u : 0u];
+    };
+    // one entry: i in [E0, E1), its column cc, the previous entry's column pc
+    // (used only when i is not its row's first), row state rs / re
+    auto visit = [&](unsigned int i, unsigned int cc, unsigned int pc, unsigned int row, unsigned int rs, unsigned int re,
+                     bool &bad, bool &unsorted) {
+        bad |= cc >= uDim;
+        const unsigned int cs = cc < uDim ? cc : 0u;  // out of range: reported, not counted right
+        atomicAdd(&hist[cs >> 1], 1u << ((cs & 1u) << 4));
+        // entry i opens every band in (band of entry i - 1, band of cc]
+        const int band = (int) (cs / TR_BAND);
+        const bool inner = i > rs;
+        unsorted |= inner && pc >= cc;
+        const int prev = inner ? (int) ((pc < uDim ? pc : uDim - 1u) / TR_BAND) : -1;
+        unsigned short *orow = off + row;
+        if (band > prev) {
+            const unsigned short o = (unsigned short) (i - rs);
+#pragma unroll 1
+            for (int k = prev + 1; k <= band; ++k) orow[k * TR_ROWS] = o;
+        }
+        if (i == re - 1u) {  // the row's last entry: the bands after it start at the row end
+            const unsigned short o = (unsigned short) (re - rs);
+#pragma unroll 1
+            for (unsigned int k = (unsigned int) band + 1u; k <= bands; ++k) orow[k * TR_ROWS] = o;
+        }
+    };
+    // last row with rp[row] <= i (i in [E0, E1))
+    auto row_of = [&](unsigned int i) -> unsigned int {
+        unsigned int lo = 0u, hi = nr - 1u;
+        while (lo < hi) {
+            const unsigned int mid = (lo + hi + 1u) >> 1;
+            if (rp[mid] <= i) lo = mid;
+            else hi = mid - 1u;
+        }
+        return lo;
+    };
+    const unsigned int Ev = E1 < gfull * 4u ? E1 : gfull * 4u;  // entries read by the group loads: [E0, Ev)
+    auto count = [&](unsigned int p, const uint4 (&c)[U], unsigned int before) {
+        const unsigned int ib = (g0 + p * PG) * 4u;  // first entry of the pass
+        unsigned int i = ib + 4u * lane;
+        unsigned int row = row_of(i < E0 ? E0 : (i < E1 ? i : E1 - 1u));
+        unsigned int rs = rp[row], re = rp[row + 1u];
+        unsigned int carry = before;  // column of the entry before the group row u
+        bool bad = false, unsorted = false;
+#pragma unroll
+        for (unsigned int u = 0; u < U; ++u) {
+            const unsigned int up = (unsigned int) __shfl_up((int) c[u].w, 1);
+            const unsigned int first_prev = lane == 0u ? carry : up;
+            carry = (unsigned int) __shfl((int) c[u].w, 63);
+            const unsigned int col[4] = {c[u].x, c[u].y, c[u].z, c[u].w};
+            const unsigned int base = ib + 256u * u + 4u * lane;
+#pragma unroll
+            for (unsigned int e = 0; e < 4u; ++e) {
+                i = base + e;
+                if (i < E0 || i >= Ev) continue;
+                while (i >= re) {  // forward to the entry's row (empty rows skipped)
+                    ++row;
+                    rs = re;
+                    re = rp[row + 1u];
+                }
+                visit(i, col[e], e == 0u ? first_prev : col[e == 0u ? 0u : e - 1u], row, rs, re, bad, unsorted);
+            }
+        }
+        if (bad) flags[1] = 1u;
+        if (unsorted) flags[0] = 1u;
+    };
+    // two register sets in turn: the next pass loads while this one counts
+    unsigned int p = wave;
+    if (p * PG < ng) {
+        uint4 ca[U], cb[U];
+        unsigned int ba, bb;
+        load(p, ca, ba);
+        for (;;) {
+            unsigned int pn = p + NW;
+            bool more = pn * PG < ng;
+            load(more ? pn : p, cb, bb);
+            count(p, ca, ba);
+            if (!more) break;
+            p = pn;
+            pn = p + NW;
+            more = pn * PG < ng;
+            load(more ? pn : p, ca, ba);
+            count(p, cb, bb);
+            if (!more) break;
+            p = pn;
+        }
+    }
+    // the array's final partial group (at most 3 entries, last tile only)
+    if (tid == 0u && Ev < E1) {
+        bool bad = false, unsorted = false;
+        for (unsigned int i = Ev > E0 ? Ev : E0; i < E1; ++i) {
+            const unsigned int row = row_of(i);
+            visit(i, idx[i], idx[i > 0u ? i - 1u : 0u], row, rp[row], rp[row + 1u], bad, unsorted);
+        }
+        if (bad) flags[1] = 1u;
+        if (unsorted) flags[0] = 1u;
+    }
+    __syncthreads();
+    unsigned int *tile_counts = counts + (size_t) t * uDim;
+    for (unsigned int c = tid; c < uDim; c += blockDim.x) tile_counts[c] = (hist[c >> 1] >> ((c & 1u) << 4)) & 0xffffu;
+    unsigned int *tile_rbt = rbt + (size_t) t * (bands + 1u) * TR_ROWS;
+    const unsigned int n = (bands + 1u) * TR_ROWS;
+    for (unsigned int j = tid; j < n; j += blockDim.x) {
+        const unsigned int rl = j & (TR_ROWS - 1u);
+        if (rl < nr) tile_rbt[j] = rp[rl] + off[j];
+    }
+}
+
+size_t count_packed_lds_bytes(unsigned int uDim, unsigned int bands) {
+    return (size_t) ((uDim + 1u) >> 1) * 4u + (size_t) (TR_ROWS + 4u) * 4u + (size_t) (bands + 1u) * TR_ROWS * 2u;
 }
 
 // K2: counts[t][c] <- sum_{t' < t} counts[t'][c] (in place), coltot[c] = total.
@@ -162,85 +354,109 @@ __global__ void tr_tile_offsets_kernel(unsigned int *__restrict__ counts, unsign
     }
 }
 
-// K4 (sorted sources).  LDS: segment masks [TR_BAND][TR_MASK_WORDS], row
-// band ranges, run starts and cursors, the source row of each staged entry
-// in input order, and the staging of K x 256 entries (source row, segment,
-// value) in output order.  Each thread loads all of its entries of a batch
-// (entry j = thread + 256 k) into registers at once, so one memory latency
-// per batch is exposed, not one per row.
+// K4 (sorted sources): one workgroup per (tile, band) chunk; the chunk's
+// entries are sorted in LDS by output segment, stably by source row, in
+// batches of at most K x 256 entries (whole rows).  Every segment of the
+// band keeps a TR_ROWS-bit mask of the batch rows holding it; an entry's
+// rank in its segment's run is the popcount of the mask below its row;
+// runs are staged in LDS in output order and written with consecutive
+// lanes on consecutive positions.  Each thread loads all of its entries of
+// a batch at once, so one memory latency per batch is exposed, not one per
+// row.
+//
+// Chunk order.  Workgroups go to the 8 XCDs round robin (workgroup i to XCD
+// i mod 8), each XCD with its own L2.  Workgroup i takes logical chunk
+// (i mod 8) * per + i / 8, so every XCD walks one contiguous range of
+// logical chunks in order.  Logical chunks run over groups of `group`
+// consecutive tiles, band-major inside a group: the chunks on one XCD at
+// one time are a few neighbouring bands of a few neighbouring tiles.
+struct tr_chunk_order {
+    unsigned int tiles, bands, per, group;
+};
+
+// Logical chunk L (< tiles * bands) -> (tile, band).
+__device__ __forceinline__ void tr_chunk(const tr_chunk_order &o, unsigned int L, unsigned int &t, unsigned int &band) {
+    const unsigned int full = o.tiles / o.group;  // complete groups
+    const unsigned int gsz = o.group * o.bands;   // chunks per complete group
+    const unsigned int g = L / gsz;
+    if (g < full) {
+        const unsigned int rem = L - g * gsz;
+        band = rem / o.group;
+        t = g * o.group + (rem - band * o.group);
+        return;
+    }
+    const unsigned int last = o.tiles - full * o.group;  // tiles in the final partial group
+    const unsigned int rem = L - full * gsz;
+    band = rem / last;
+    t = full * o.group + (rem - band * last);
+}
+
 template<typename S, unsigned int K>
 __global__ void __launch_bounds__(TR_SORT_THREADS) tr_band_kernel(unsigned int cDim, unsigned int uDim, const unsigned int *__restrict__ idx,
                                                                   const S *__restrict__ val, const unsigned int *__restrict__ tile_off,
                                                                   const unsigned int *__restrict__ out_ptr,
-                                                                  const unsigned int *__restrict__ rbt, unsigned int bands,
+                                                                  const unsigned int *__restrict__ rbt, tr_chunk_order order,
                                                                   unsigned int *__restrict__ out_idx, S *__restrict__ out_val) {
-    static_assert(TR_SORT_THREADS == TR_ROWS && TR_BAND % TR_SORT_THREADS == 0u, "one row / whole segments per thread");
-    constexpr unsigned int CPT = TR_BAND / TR_SORT_THREADS;  // segments per thread in the run-start scan
-    constexpr unsigned int cap = K * TR_SORT_THREADS;
-    static_assert(cap >= TR_BAND, "a batch must hold a full row");
+    static_assert(TR_SORT_THREADS == TR_ROWS && TR_SORT_THREADS == TR_BAND, "one row and one segment per thread");
+    static_assert(TR_ROWS <= 256u && TR_BAND <= 256u, "row and segment packed in 8 bits each");
+    constexpr unsigned int T = TR_SORT_THREADS;
+    constexpr unsigned int cap = K * T;
     extern __shared__ __align__(16) unsigned char lds[];
-    unsigned int *masks = (unsigned int *) lds;                  // [TR_BAND * TR_MASK_WORDS]
-    unsigned int *seg_b = masks + TR_BAND * TR_MASK_WORDS;       // [TR_ROWS]
+    unsigned int *masks = (unsigned int *) lds;                  // [TR_MASK_WORDS][TR_BAND] word-major: lanes on
+                                                                 // different segments hit different banks
+    unsigned int *seg_b = masks + TR_BAND * TR_MASK_WORDS;       // [TR_ROWS] row starts in the band
     unsigned int *seg_o = seg_b + TR_ROWS;                       // [TR_ROWS + 1] chunk offsets of the rows
-    unsigned int *lstart = seg_o + TR_ROWS + 1u;                 // [TR_BAND + 1]
-    unsigned int *gcur = lstart + TR_BAND + 1u;                  // [TR_BAND]
-    unsigned int *ctl = gcur + TR_BAND;                          // [4]
-    unsigned short *s_in = (unsigned short *) (ctl + 4u);        // [cap] source row, input order
-    unsigned short *s_row = s_in + cap;                          // [cap] source row, output order
-    unsigned short *s_seg = s_row + cap;                         // [cap]
-    S *s_val = (S *) align_up((size_t) (s_seg + cap), 16u);      // [cap]
+    unsigned int *lstart = seg_o + TR_ROWS + 1u;                 // [TR_BAND + 1] run starts in the staging
+    unsigned int *gcur = lstart + TR_BAND + 1u;                  // [TR_BAND] output cursors
+    unsigned short *pre = (unsigned short *) (gcur + TR_BAND + 2u);  // [TR_MASK_WORDS][TR_BAND] run start + rows in lower words
+    unsigned char *s_in = (unsigned char *) (pre + TR_MASK_WORDS * TR_BAND);  // [cap] batch row, input order
+    unsigned short *s_rs = (unsigned short *) (s_in + cap);      // [cap] batch row | segment << 8, output order
+    S *s_val = (S *) (s_rs + cap);                               // [cap]
     __shared__ unsigned int scan_smem[16];
 
-    const unsigned int t = blockIdx.x / bands;
-    const unsigned int band = blockIdx.x - t * bands;
+    const unsigned int tid = threadIdx.x;
+    const unsigned int L = (blockIdx.x & 7u) * order.per + (blockIdx.x >> 3);
+    if (L >= order.tiles * order.bands) return;  // the < 8 spare workgroups of the grid
+    unsigned int t, band;
+    tr_chunk(order, L, t, band);
+    const unsigned int bands = order.bands;
     const unsigned int c0 = band * TR_BAND;
     const unsigned int nb = uDim - c0 < TR_BAND ? uDim - c0 : TR_BAND;
     const unsigned int r0 = t * TR_ROWS;
     const unsigned int nr = cDim - r0 < TR_ROWS ? cDim - r0 : TR_ROWS;
     const unsigned int *rb_lo = rbt + ((size_t) t * (bands + 1u) + band) * TR_ROWS;
-    const unsigned int *rb_hi = rb_lo + TR_ROWS;
 
-    for (unsigned int w = threadIdx.x; w < TR_BAND * TR_MASK_WORDS; w += TR_SORT_THREADS) masks[w] = 0u;
-    for (unsigned int c = threadIdx.x; c < nb; c += TR_SORT_THREADS)
-        gcur[c] = out_ptr[c0 + c] + tile_off[(size_t) t * uDim + c0 + c];
-    // row ranges inside the band and their offsets in the chunk
+    for (unsigned int w = tid; w < TR_BAND * TR_MASK_WORDS; w += T) masks[w] = 0u;
+    if (tid < nb) gcur[tid] = out_ptr[c0 + tid] + tile_off[(size_t) t * uDim + c0 + tid];
     {
-        const unsigned int r = threadIdx.x;
-        const unsigned int b = r < nr ? rb_lo[r] : 0u;
-        const unsigned int n = r < nr ? rb_hi[r] - b : 0u;
+        const unsigned int b = tid < nr ? rb_lo[tid] : 0u;
+        const unsigned int n = tid < nr ? rb_lo[tid + TR_ROWS] - b : 0u;
         unsigned int total = 0u;
         const unsigned int o = block_exclusive_scan_u32(n, scan_smem, &total);
-        seg_b[r] = b;
-        seg_o[r] = o;
-        if (r == 0u) seg_o[TR_ROWS] = total;
+        seg_b[tid] = b;
+        seg_o[tid] = o;
+        if (tid == 0u) seg_o[TR_ROWS] = total;
     }
     __syncthreads();
+    // every entry of the chunk is at or after its first row's start, and a
+    // chunk spans less than 4 GiB of the source (rows of one tile, at most
+    // 4096 bands of 256 columns): one uniform base, 32-bit offsets
+    const unsigned int first = seg_b[0];
+    const char *ib = (const char *) (idx + first);
+    const char *vbase = (const char *) (val + first);
 
     unsigned int rb0 = 0u;
     while (rb0 < nr) {
-        // batch: rows [rb0, rb1) with at most cap entries (a row has at most
-        // TR_BAND <= cap entries in the band, so a batch holds >= 1 row)
-        if (threadIdx.x == 0u) {
-            unsigned int lo = rb0 + 1u, hi = nr;  // largest rb1 with seg_o[rb1] - seg_o[rb0] <= cap
-            while (lo < hi) {
-                const unsigned int mid = (lo + hi + 1u) >> 1;
-                if (seg_o[mid] - seg_o[rb0] <= cap) lo = mid;
-                else hi = mid - 1u;
-            }
-            ctl[0] = lo;
-        }
-        __syncthreads();
-        const unsigned int rb1 = ctl[0];
+        // batch: rows [rb0, rb1), the most whole rows within cap entries
+        // (seg_o is nondecreasing, and one row holds at most TR_BAND <= cap
+        // entries of a band, so a batch holds at least one row)
         const unsigned int base = seg_o[rb0];
+        const int fits = tid >= rb0 && tid < nr && seg_o[tid + 1u] - base <= cap;
+        const unsigned int rb1 = rb0 + (unsigned int) __syncthreads_count(fits);
         const unsigned int n = seg_o[rb1] - base;
-        // source row of every entry of the batch (LDS only; one thread per
-        // row, its stores independent of each other)
-        {
-            const unsigned int r = threadIdx.x;  // TR_SORT_THREADS == TR_ROWS
-            if (r >= rb0 && r < rb1) {
-                const unsigned int o = seg_o[r] - base, len = seg_o[r + 1u] - seg_o[r];
-                for (unsigned int k = 0; k < len; ++k) s_in[o + k] = (unsigned short) r;
-            }
+        if (tid >= rb0 && tid < rb1) {
+            const unsigned int o = seg_o[tid] - base, len = seg_o[tid + 1u] - seg_o[tid];
+            for (unsigned int k = 0; k < len; ++k) s_in[o + k] = (unsigned char) (tid - rb0);
         }
         __syncthreads();
         // A: load this thread's entries (all loads issued before any use),
@@ -249,69 +465,63 @@ __global__ void __launch_bounds__(TR_SORT_THREADS) tr_band_kernel(unsigned int c
         S v[K];
 #pragma unroll
         for (unsigned int k = 0; k < K; ++k) {
-            const unsigned int j = threadIdx.x + k * TR_SORT_THREADS;
+            const unsigned int j = tid + k * T;
             const unsigned int jj = j < n ? j : (n != 0u ? n - 1u : 0u);
-            r[k] = n != 0u ? s_in[jj] : rb0;
-            const unsigned int i = seg_b[r[k]] + (base + jj - seg_o[r[k]]);
-            c[k] = n != 0u ? idx[i] : c0;
-            v[k] = n != 0u ? val[i] : S(0);
+            r[k] = n != 0u ? s_in[jj] : 0u;
+            const unsigned int row = rb0 + r[k];
+            const unsigned int li = n != 0u ? seg_b[row] + (base + jj - seg_o[row]) - first : 0u;
+            c[k] = n != 0u ? *(const unsigned int *) (ib + li * 4u) : c0;
+            v[k] = n != 0u ? *(const S *) (vbase + li * (unsigned int) sizeof(S)) : S(0);
         }
 #pragma unroll
         for (unsigned int k = 0; k < K; ++k)
-            if (threadIdx.x + k * TR_SORT_THREADS < n)
-                atomicOr(&masks[(c[k] - c0) * TR_MASK_WORDS + (r[k] >> 5)], 1u << (r[k] & 31u));
+            if (tid + k * T < n) atomicOr(&masks[(r[k] >> 5) * TR_BAND + (c[k] - c0)], 1u << (r[k] & 31u));
         __syncthreads();
-        // B: run lengths -> run starts in the staging
+        // B: run lengths -> run starts in the staging, and each mask word's
+        // first slot (run start + rows in the lower words)
         {
-            unsigned int cnt[CPT], sum = 0u;
+            unsigned int cw[TR_MASK_WORDS], cnt = 0u;
 #pragma unroll
-            for (unsigned int k = 0; k < CPT; ++k) {
-                const unsigned int cc = CPT * threadIdx.x + k;
-                cnt[k] = 0u;
-#pragma unroll
-                for (unsigned int w = 0; w < TR_MASK_WORDS; ++w) cnt[k] += (unsigned int) __popc(masks[cc * TR_MASK_WORDS + w]);
-                sum += cnt[k];
+            for (unsigned int w = 0; w < TR_MASK_WORDS; ++w) {
+                cw[w] = (unsigned int) __popc(masks[w * TR_BAND + tid]);
+                cnt += cw[w];
             }
             unsigned int total = 0u;
-            unsigned int o = block_exclusive_scan_u32(sum, scan_smem, &total);
+            unsigned int o = block_exclusive_scan_u32(cnt, scan_smem, &total);
+            lstart[tid] = o;
+            if (tid == 0u) lstart[TR_BAND] = total;
 #pragma unroll
-            for (unsigned int k = 0; k < CPT; ++k) {
-                lstart[CPT * threadIdx.x + k] = o;
-                o += cnt[k];
+            for (unsigned int w = 0; w < TR_MASK_WORDS; ++w) {
+                pre[w * TR_BAND + tid] = (unsigned short) o;
+                o += cw[w];
             }
-            if (threadIdx.x == 0u) lstart[TR_BAND] = total;
         }
         __syncthreads();
         // C: every entry to its run start + rank among earlier rows
 #pragma unroll
         for (unsigned int k = 0; k < K; ++k) {
-            if (threadIdx.x + k * TR_SORT_THREADS >= n) continue;
+            if (tid + k * T >= n) continue;
             const unsigned int cl = c[k] - c0;
-            const unsigned int *m = masks + cl * TR_MASK_WORDS;
-            const unsigned int wr = r[k] >> 5;
-            unsigned int rank = (unsigned int) __popc(m[wr] & ((1u << (r[k] & 31u)) - 1u));
-            for (unsigned int w = 0; w < wr; ++w) rank += (unsigned int) __popc(m[w]);
-            const unsigned int d = lstart[cl] + rank;
-            s_row[d] = (unsigned short) r[k];
-            s_seg[d] = (unsigned short) cl;
+            const unsigned int wi = (r[k] >> 5) * TR_BAND + cl;
+            const unsigned int d = pre[wi] + (unsigned int) __popc(masks[wi] & ((1u << (r[k] & 31u)) - 1u));
+            s_rs[d] = (unsigned short) (r[k] | (cl << 8));
             s_val[d] = v[k];
         }
         __syncthreads();
         // D: runs to their final places (consecutive staging slots of one run
         // are consecutive output positions)
-        for (unsigned int k = threadIdx.x; k < n; k += TR_SORT_THREADS) {
-            const unsigned int cl = s_seg[k];
+        for (unsigned int k = tid; k < n; k += T) {
+            const unsigned int rs = s_rs[k];
+            const unsigned int cl = rs >> 8;
             const unsigned int pos = gcur[cl] + (k - lstart[cl]);
-            out_idx[pos] = r0 + s_row[k];
+            out_idx[pos] = r0 + rb0 + (rs & 255u);
             out_val[pos] = s_val[k];
         }
         __syncthreads();
         // E: advance the cursors, clear the masks
-        for (unsigned int cc = threadIdx.x; cc < TR_BAND; cc += TR_SORT_THREADS) {
-            if (cc < nb) gcur[cc] += lstart[cc + 1u] - lstart[cc];
+        if (tid < nb) gcur[tid] += lstart[tid + 1u] - lstart[tid];
 #pragma unroll
-            for (unsigned int w = 0; w < TR_MASK_WORDS; ++w) masks[cc * TR_MASK_WORDS + w] = 0u;
-        }
+        for (unsigned int w = 0; w < TR_MASK_WORDS; ++w) masks[w * TR_BAND + tid] = 0u;
         __syncthreads();
         rb0 = rb1;
     }
@@ -348,8 +558,9 @@ __global__ void __launch_bounds__(64) tr_scatter_kernel(unsigned int cDim, unsig
 }
 
 size_t band_lds_bytes(unsigned int cap, size_t vb) {
-    const size_t fixed = ((size_t) TR_BAND * TR_MASK_WORDS + TR_ROWS + (TR_ROWS + 1u) + (TR_BAND + 1u) + TR_BAND + 4u) * 4u;
-    return align_up(fixed + (size_t) cap * 6u, 16u) + (size_t) cap * vb;
+    const size_t words = (size_t) TR_BAND * TR_MASK_WORDS + TR_ROWS + (TR_ROWS + 1u) + (TR_BAND + 1u) + TR_BAND + 2u +
+                         (size_t) TR_MASK_WORDS * TR_BAND / 2u;
+    return align_up(words * 4u + (size_t) cap * 3u + (size_t) cap * vb, 16u);
 }
 
 template<typename S>
@@ -357,16 +568,23 @@ int launch_scatter(const tr_layout &L, bool sorted, unsigned int cDim, unsigned 
                    const unsigned int *idx, const void *val, const unsigned int *tile_off, const unsigned int *out_ptr,
                    const unsigned int *rbt, unsigned int *out_idx, void *out_val, hipStream_t s) {
     if (sorted) {
-        const unsigned long long blocks = (unsigned long long) L.tiles * L.bands;
-        CSAMD_REQUIRE(blocks <= 0x7fffffffull, "transpose: too many chunks");
+        const unsigned long long chunks = (unsigned long long) L.tiles * L.bands;
+        CSAMD_REQUIRE(chunks <= 0xffffffffull, "transpose: too many chunks");
+        const unsigned long long per = (chunks + 7ull) / 8ull;
+        // tiles per group (CSAMD_TR_G for sweeps)
+        const char *genv = std::getenv("CSAMD_TR_G");
+        const unsigned int gw = genv != nullptr ? (unsigned int) std::atoi(genv) : 8u;
+        const tr_chunk_order order{L.tiles, L.bands, (unsigned int) per, gw >= 1u && gw <= 64u ? gw : 8u};
         // staging of K x 256 entries per batch (CSAMD_TR_K = 8 / 12 / 16 for sweeps)
         const char *env = std::getenv("CSAMD_TR_K");
-        const unsigned int want = env != nullptr ? (unsigned int) std::atoi(env) : 12u;
-        const unsigned int K = want == 8u ? 8u : want == 16u ? 16u : 12u;
+        const int kw = env != nullptr ? std::atoi(env) : 16;
+        const unsigned int K = kw == 8 ? 8u : kw == 16 ? 16u : 12u;
         const size_t lds = band_lds_bytes(K * TR_SORT_THREADS, sizeof(S));
-        auto k = K == 8u ? tr_band_kernel<S, 8u> : K == 12u ? tr_band_kernel<S, 12u> : tr_band_kernel<S, 16u>;
+        auto k = K == 8u ? tr_band_kernel<S, 8u> : K == 16u ? tr_band_kernel<S, 16u> : tr_band_kernel<S, 12u>;
         CSAMD_HIP(hipFuncSetAttribute((const void *) k, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds), "set lds");
-        k<<<(unsigned int) blocks, TR_SORT_THREADS, lds, s>>>(cDim, uDim, idx, (const S *) val, tile_off, out_ptr, rbt, L.bands,
+        const unsigned long long blocks = per * 8ull;
+        CSAMD_REQUIRE(blocks <= 0x7fffffffull, "transpose: too many chunks");
+        k<<<(unsigned int) blocks, TR_SORT_THREADS, lds, s>>>(cDim, uDim, idx, (const S *) val, tile_off, out_ptr, rbt, order,
                                                               out_idx, (S *) out_val);
         return hip_ok(hipGetLastError(), "tr_band_kernel");
     }
@@ -402,22 +620,40 @@ int transpose_impl(unsigned int cDim, unsigned int uDim, unsigned int nnz, const
         int dev = 0;
         if (hipGetDevice(&dev) == hipSuccess) (void) hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
     }
-    const int use_lds = (size_t) uDim * 4u <= (size_t) max_lds ? 1 : 0;
-    if (!use_lds && !zero_bytes(counts, (size_t) L.tiles * uDim * 4u, s)) return 0;
-    {
-        const size_t lds = use_lds ? (size_t) uDim * 4u : 0u;
-        auto k = tr_count_kernel;
-        if (use_lds) CSAMD_HIP(hipFuncSetAttribute((const void *) k, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds), "set lds");
-        k<<<L.tiles, TR_COUNT_THREADS, lds, s>>>(cDim, uDim, d_ptr, d_idx, L.bands, use_lds, counts, rbt, flags);
-        CSAMD_HIP(hipGetLastError(), "tr_count_kernel");
-    }
-    tr_tile_offsets_kernel<<<grid_for(uDim, 256u, 4096u), 256, 0, s>>>(counts, L.tiles, uDim, coltot);
-    CSAMD_HIP(hipGetLastError(), "tr_tile_offsets_kernel");
-    if (!exclusive_scan_u32(coltot, uDim, d_out_ptr, nullptr, scan_tmp, exclusive_scan_scratch_bytes(uDim), s)) return 0;
+    // packed kernel: U = 2 / 4 / 8 groups of 4 entries per lane and pass (CSAMD_TR_U for sweeps)
+    const char *uenv = std::getenv("CSAMD_TR_U");
+    const unsigned int uw = uenv != nullptr ? (unsigned int) std::atoi(uenv) : 8u;
+    const size_t packed_lds = count_packed_lds_bytes(uDim, L.bands);
+    const bool packed = L.band_table && uDim < 65536u && nnz >= 4u && nnz < 0xffff0000u && packed_lds <= (size_t) max_lds &&
+                        ((uintptr_t) d_idx & 15u) == 0u;  // 16-byte group loads
+    // counts (and band starts) -> per-tile running offsets -> out_ptr
+    auto count_pass = [&](bool use_packed, bool with_rbt) -> int {
+        if (use_packed) {
+            auto k = uw == 2u ? tr_count_packed_kernel<2u> : uw == 8u ? tr_count_packed_kernel<8u> : tr_count_packed_kernel<4u>;
+            CSAMD_HIP(hipFuncSetAttribute((const void *) k, hipFuncAttributeMaxDynamicSharedMemorySize, (int) packed_lds), "set lds");
+            k<<<L.tiles, TR_COUNT_THREADS, packed_lds, s>>>(cDim, uDim, nnz, d_ptr, d_idx, L.bands, counts, rbt, flags);
+            CSAMD_HIP(hipGetLastError(), "tr_count_packed_kernel");
+        } else {
+            const int use_lds = (size_t) uDim * 4u <= (size_t) max_lds ? 1 : 0;
+            if (!use_lds && !zero_bytes(counts, (size_t) L.tiles * uDim * 4u, s)) return 0;
+            const size_t lds = use_lds ? (size_t) uDim * 4u : 0u;
+            auto k = tr_count_kernel<16u>;
+            if (use_lds) CSAMD_HIP(hipFuncSetAttribute((const void *) k, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds), "set lds");
+            k<<<L.tiles, TR_COUNT_THREADS, lds, s>>>(cDim, uDim, d_ptr, d_idx, L.bands, use_lds, counts, with_rbt ? rbt : nullptr, flags);
+            CSAMD_HIP(hipGetLastError(), "tr_count_kernel");
+        }
+        tr_tile_offsets_kernel<<<grid_for(uDim, 256u, 4096u), 256, 0, s>>>(counts, L.tiles, uDim, coltot);
+        CSAMD_HIP(hipGetLastError(), "tr_tile_offsets_kernel");
+        return exclusive_scan_u32(coltot, uDim, d_out_ptr, nullptr, scan_tmp, exclusive_scan_scratch_bytes(uDim), s);
+    };
+    if (!count_pass(packed, true)) return 0;
     unsigned int hflags[2] = {0u, 0u};
     CSAMD_HIP(hipMemcpyAsync(hflags, flags, 8u, hipMemcpyDeviceToHost, s), "D2H transpose flags");
     CSAMD_HIP(hipStreamSynchronize(s), "transpose sync");
     CSAMD_REQUIRE(hflags[1] == 0u, "transpose: minor index out of range (>= %u)", uDim);
+    // a row that is not strictly ascending can overflow the packed u16
+    // counters: count again in u32 for the fallback scatter
+    if (hflags[0] != 0u && packed && !count_pass(false, false)) return 0;
     const bool sorted = hflags[0] == 0u && L.band_table;
     switch (vb) {
         case 1: return launch_scatter<uint8_t>(L, sorted, cDim, uDim, d_ptr, d_idx, d_val, counts, d_out_ptr, rbt, d_out_idx, d_out_val, s);

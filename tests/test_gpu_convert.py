@@ -114,13 +114,18 @@ def test_sliced_from_blocked_parity(ws):
     assert np.array_equal(col, ref["col_idx"]) and np.array_equal(v, ref["val"])
 
 
-@pytest.mark.parametrize("case", ["sorted", "unsorted", "empty_cols", "wide", "nnz0", "dense", "duplicates"])
+@pytest.mark.parametrize("case", ["sorted", "unsorted", "empty_cols", "wide", "nnz0", "dense", "duplicates", "tall"])
 def test_transpose_parity(gpu, case):
     if case == "dense":
-        # > 8192 entries per (256-row tile, 512-column band): chunks staged in
-        # several row batches; partial last tile and band
+        # more entries per (256-row tile, 256-column band) than one LDS batch
+        # holds: chunks staged in several row batches; partial last tile and band
         rows, cols = 700, 1100
         ptr, idx, val = random_csr(rows, cols, 0.85, seed=3, empty_rows=0.05)
+    elif case == "tall":
+        # 36 tiles x 12 bands: several complete groups of tiles in the chunk
+        # order plus a partial one, spread over all eight XCD ranges
+        rows, cols = 9000, 3000
+        ptr, idx, val = random_csr(rows, cols, 0.02, seed=5, empty_rows=0.05)
     elif case == "duplicates":  # repeated columns inside rows: the fallback scatter
         rows, cols = 400, 300
         ptr, idx, val = random_csr(rows, cols, 0.05, seed=9, duplicates=0.3)
@@ -146,6 +151,24 @@ def test_transpose_parity(gpu, case):
     assert np.array_equal(bptr, ptr)
     if case != "unsorted":
         assert np.array_equal(bidx, idx) and np.array_equal(bval, bits)
+
+
+@pytest.mark.parametrize("tail", [0, 1, 2, 3])
+def test_transpose_group_tails(gpu, tail):
+    # the count pass reads indices in 16-byte groups: tile edges inside a
+    # group, and an array whose last group is partial (nnz % 4 = tail)
+    rows, cols = 777, 700
+    ptr, idx, val = random_csr(rows, cols, 0.03, seed=40 + tail, empty_rows=0.05)
+    cut = (len(idx) - tail) % 4
+    keep = len(idx) - cut
+    ptr = np.minimum(ptr, keep).astype(np.uint32)
+    idx, val = idx[:keep], val[:keep]
+    assert len(idx) % 4 == tail
+    bits = f16_bits(val)
+    src = DeviceCSR.from_numpy(rows, cols, ptr, idx, bits)
+    optr, oidx, oval = compressed_transpose(src, TransposeScratch()).to_numpy()
+    rptr, ridx, rval = O.transpose(rows, cols, ptr, idx, bits)
+    assert np.array_equal(optr, rptr) and np.array_equal(oidx, ridx) and np.array_equal(oval, rval)
 
 
 @pytest.mark.parametrize("vt", [F32, FP8])
