@@ -15,31 +15,62 @@
 // one wave64 per row for Sliced-ELL emits (coalesced copy + padding).
 #include "common.hh"
 
+#include <cstdlib>
+
 namespace csamd {
 namespace {
 
 constexpr unsigned int CV_THREADS = 256u;
 constexpr unsigned int BELL_MAX_BITSET_WORDS = 8192u;  // 262,144 column blocks (32 KB LDS)
+constexpr unsigned int BELL_MAX_BLOCK_SIZE = 8192u;    // rows of a block (their row pointers sit in LDS)
+
+// Marks in `bits` the column blocks touched by rows [r0, r1) (entries
+// [rp[0], rp[r1 - r0]), row pointers in LDS), four loads in flight per
+// thread.  CHECK: also sets *unsorted when some row is not strictly
+// ascending (each thread walks its entries' rows forward).
+template<bool CHECK>
+__device__ __forceinline__ void bell_mark(const csamd_compressed_view &src, unsigned int nrows, unsigned int bs, const unsigned int *rp,
+                                          unsigned int *bits, unsigned int *unsorted) {
+    const unsigned int b = rp[0], e = rp[nrows];
+    unsigned int row = 0u, rend = rp[1];
+    for (unsigned int i0 = b; i0 < e; i0 += 4u * CV_THREADS) {
+        unsigned int c[4], n[4];
+#pragma unroll
+        for (unsigned int u = 0; u < 4u; ++u) {
+            const unsigned int i = i0 + u * CV_THREADS + threadIdx.x;
+            c[u] = i < e ? src.minorIdx[i] : CSAMD_INVALID_COL;
+            if (CHECK) n[u] = i + 1u < e ? src.minorIdx[i + 1u] : CSAMD_INVALID_COL;
+        }
+#pragma unroll
+        for (unsigned int u = 0; u < 4u; ++u) {
+            const unsigned int i = i0 + u * CV_THREADS + threadIdx.x;
+            if (i >= e) continue;
+            if (c[u] < src.cols) {
+                const unsigned int bc = c[u] / bs;
+                atomicOr(&bits[bc >> 5], 1u << (bc & 31u));
+            }
+            if (CHECK) {
+                while (i >= rend) rend = rp[++row + 1u];
+                if (i + 1u < rend && c[u] >= n[u]) *unsorted = 1u;
+            }
+        }
+    }
+}
 
 __global__ void __launch_bounds__(CV_THREADS) bell_shape_kernel(csamd_compressed_view src, unsigned int bs, unsigned int row_blocks,
                                                                 unsigned int nwords, unsigned int *__restrict__ counts,
                                                                 unsigned int *__restrict__ max_width,
                                                                 unsigned long long *__restrict__ total) {
     extern __shared__ unsigned int bits[];
+    unsigned int *rp = bits + nwords;  // [bs + 1] row pointers of the block
     __shared__ unsigned int red[CV_THREADS / 64u];
     for (unsigned int rb = blockIdx.x; rb < row_blocks; rb += gridDim.x) {
         for (unsigned int w = threadIdx.x; w < nwords; w += CV_THREADS) bits[w] = 0u;
-        __syncthreads();
         const unsigned int r0 = rb * bs;
         const unsigned int r1 = r0 + bs < src.rows ? r0 + bs : src.rows;
-        const unsigned int b = src.majorPtr[r0], e = src.majorPtr[r1];
-        for (unsigned int i = b + threadIdx.x; i < e; i += CV_THREADS) {
-            const unsigned int c = src.minorIdx[i];
-            if (c < src.cols) {
-                const unsigned int bc = c / bs;
-                atomicOr(&bits[bc >> 5], 1u << (bc & 31u));
-            }
-        }
+        for (unsigned int r = threadIdx.x; r <= r1 - r0; r += CV_THREADS) rp[r] = src.majorPtr[r0 + r];
+        __syncthreads();
+        bell_mark<false>(src, r1 - r0, bs, rp, bits, nullptr);
         __syncthreads();
         unsigned int u = 0u;
         for (unsigned int w = threadIdx.x; w < nwords; w += CV_THREADS) u += (unsigned int) __popc(bits[w]);
@@ -63,39 +94,33 @@ template<> __device__ __forceinline__ bool stored_nonzero<uint16_t>(uint16_t raw
 template<> __device__ __forceinline__ bool stored_nonzero<float>(float raw) { return raw != 0.0f; }
 template<> __device__ __forceinline__ bool stored_nonzero<uint8_t>(uint8_t raw) { return (raw & 0x7fu) != 0u; }
 
-// One workgroup per row block: slots, blockColIdx, zero fill, value scatter, nnz.
-template<typename S>
-__global__ void __launch_bounds__(CV_THREADS) bell_emit_kernel(csamd_compressed_view src, const S *__restrict__ sval,
-                                                               unsigned int bs, unsigned int row_blocks, unsigned int width,
-                                                               unsigned int nwords, unsigned int ell_cols,
-                                                               unsigned int *__restrict__ block_col_idx, S *__restrict__ dval,
-                                                               unsigned long long *__restrict__ nnz_total) {
+// Blocked-ELL emit, in three launches.
+//
+// bell_blocks_kernel, one workgroup per row block: the set of column blocks
+// the block's rows touch (LDS bitset), the block's slot of every column
+// block (popcount prefix per bitset word), blockColIdx (ascending, then
+// invalid padding), and whether some row is not strictly ascending.  The
+// bitset and its prefix go to a table for the next launches.
+__global__ void __launch_bounds__(CV_THREADS) bell_blocks_kernel(csamd_compressed_view src, unsigned int bs, unsigned int row_blocks,
+                                                                 unsigned int width, unsigned int nwords,
+                                                                 unsigned int *__restrict__ block_col_idx,
+                                                                 unsigned int *__restrict__ tab, unsigned int *__restrict__ unsorted,
+                                                                 unsigned int *__restrict__ need_w) {
     extern __shared__ unsigned int lds[];
     unsigned int *bits = lds;
     unsigned int *prefix = lds + nwords;
+    unsigned int *rp = prefix + nwords;  // [bs + 1] row pointers of the block
     __shared__ unsigned int red[CV_THREADS / 64u];
     __shared__ unsigned int unsorted_flag;
     for (unsigned int rb = blockIdx.x; rb < row_blocks; rb += gridDim.x) {
         for (unsigned int w = threadIdx.x; w < nwords; w += CV_THREADS) bits[w] = 0u;
         if (threadIdx.x == 0) unsorted_flag = 0u;
-        __syncthreads();
         const unsigned int r0 = rb * bs;
         const unsigned int r1 = r0 + bs < src.rows ? r0 + bs : src.rows;
-        const unsigned int b = src.majorPtr[r0], e = src.majorPtr[r1];
-        for (unsigned int i = b + threadIdx.x; i < e; i += CV_THREADS) {
-            const unsigned int c = src.minorIdx[i];
-            if (c < src.cols) {
-                const unsigned int bc = c / bs;
-                atomicOr(&bits[bc >> 5], 1u << (bc & 31u));
-            }
-        }
+        for (unsigned int r = threadIdx.x; r <= r1 - r0; r += CV_THREADS) rp[r] = src.majorPtr[r0 + r];
+        __syncthreads();
         // rows with non-increasing columns (possible duplicates) take the serial path
-        for (unsigned int r = r0; r < r1; ++r) {
-            const unsigned int rb0 = src.majorPtr[r], re0 = src.majorPtr[r + 1u];
-            for (unsigned int i = rb0 + threadIdx.x; i + 1u < re0; i += CV_THREADS) {
-                if (src.minorIdx[i] >= src.minorIdx[i + 1u]) unsorted_flag = 1u;
-            }
-        }
+        bell_mark<true>(src, r1 - r0, bs, rp, bits, &unsorted_flag);
         __syncthreads();
         // exclusive prefix of popcounts over the bitset words (serial chunks of 256)
         {
@@ -110,65 +135,162 @@ __global__ void __launch_bounds__(CV_THREADS) bell_emit_kernel(csamd_compressed_
             }
         }
         __syncthreads();
-        // block column ids in ascending order, then invalid padding
+        // block column ids in ascending order, then invalid padding; a block
+        // needing more than `width` slots is reported (the call fails) and
+        // nothing is written past its slots
         const unsigned long long slot_base = (unsigned long long) rb * width;
+        const unsigned int unique = nwords != 0u ? prefix[nwords - 1u] + (unsigned int) __popc(bits[nwords - 1u]) : 0u;
+        if (unique > width && threadIdx.x == 0) atomicMax(need_w, unique);
         for (unsigned int w = threadIdx.x; w < nwords; w += CV_THREADS) {
             unsigned int m = bits[w];
             unsigned int s = prefix[w];
-            while (m != 0u) {
+            while (m != 0u && s < width) {
                 const unsigned int bit = (unsigned int) __builtin_ctz(m);
                 block_col_idx[slot_base + s] = w * 32u + bit;
                 ++s;
                 m &= m - 1u;
             }
         }
-        {
-            const unsigned int unique = nwords != 0u ? prefix[nwords - 1u] + (unsigned int) __popc(bits[nwords - 1u]) : 0u;
-            for (unsigned int s = unique + threadIdx.x; s < width; s += CV_THREADS) block_col_idx[slot_base + s] = CSAMD_INVALID_COL;
+        for (unsigned int s = unique + threadIdx.x; s < width; s += CV_THREADS) block_col_idx[slot_base + s] = CSAMD_INVALID_COL;
+        unsigned int *t = tab + (size_t) rb * 2u * nwords;
+        for (unsigned int w = threadIdx.x; w < 2u * nwords; w += CV_THREADS) t[w] = lds[w];
+        if (threadIdx.x == 0) unsorted[rb] = unsorted_flag;
+        __syncthreads();
+    }
+}
+
+// Value position of column c (< cols) in its row: slot of its column block
+// times bs plus the lane.
+__device__ __forceinline__ unsigned int bell_pos(const unsigned int *bits, const unsigned int *prefix, unsigned int bs, unsigned int c) {
+    const unsigned int bc = c / bs;
+    const unsigned int slot = prefix[bc >> 5] + (unsigned int) __popc(bits[bc >> 5] & ((1u << (bc & 31u)) - 1u));
+    return slot * bs + (c - bc * bs);
+}
+
+// bell_rows_kernel, one workgroup per row: the value row is written whole
+// (padding included), in segments of SEG_WORDS words staged in LDS
+// (zeros, then the row's entries that fall in the segment), so every output
+// line is written once by one coalesced store.  A row's entries are read
+// once into registers (up to E per thread; longer rows read theirs again
+// for each segment).  Rows of an unsorted row block are only zeroed here
+// (bell_replay_kernel fills them).  Rows that are not whole, aligned 4-byte
+// words (narrow values with an odd ell width) are written value by value.
+template<typename S, unsigned int E, unsigned int SEG_WORDS>
+__global__ void __launch_bounds__(CV_THREADS) bell_rows_kernel(csamd_compressed_view src, const S *__restrict__ sval, unsigned int bs,
+                                                               unsigned int nwords, unsigned int ell_cols,
+                                                               const unsigned int *__restrict__ tab,
+                                                               const unsigned int *__restrict__ unsorted, S *__restrict__ dval,
+                                                               unsigned long long *__restrict__ nnz_total) {
+    extern __shared__ unsigned int lds[];
+    unsigned int *bits = lds;
+    unsigned int *prefix = lds + nwords;
+    unsigned int *seg = lds + ((2u * nwords + 3u) & ~3u);  // [SEG_WORDS], 16-byte aligned
+    S *segv = (S *) seg;
+    constexpr unsigned int PER_WORD = 4u / sizeof(S);
+    constexpr unsigned int SEG = SEG_WORDS * PER_WORD;  // values per segment
+    constexpr unsigned int INV = 0xffffffffu;
+    const bool words = ell_cols % PER_WORD == 0u && ((uintptr_t) dval & 3u) == 0u;
+    const bool quads = ((unsigned long long) ell_cols * sizeof(S)) % 16ull == 0ull && ((uintptr_t) dval & 15u) == 0u;
+    unsigned int nz = 0u;
+    for (unsigned int r = blockIdx.x; r < src.rows; r += gridDim.x) {
+        const unsigned int rb = r / bs;
+        S *orow = dval + (unsigned long long) r * ell_cols;
+        if (unsorted[rb] != 0u) {
+            for (unsigned int k = threadIdx.x; k < ell_cols; k += CV_THREADS) orow[k] = S(0);
+            continue;
         }
-        // values were zero-filled by a preceding kernel
-        unsigned int nz = 0u;
-        if (unsorted_flag == 0u) {
-            for (unsigned int i = b + threadIdx.x; i < e; i += CV_THREADS) {
-                const unsigned int c = src.minorIdx[i];
-                if (c >= src.cols) continue;
-                // row of entry i: search in [r0, r1)
-                unsigned int lo = r0, hi = r1 - 1u;
-                while (lo < hi) {
-                    const unsigned int mid = (lo + hi + 1u) >> 1;
-                    if (src.majorPtr[mid] <= i) lo = mid;
-                    else hi = mid - 1u;
+        const unsigned int *t = tab + (size_t) rb * 2u * nwords;
+        for (unsigned int w = threadIdx.x; w < 2u * nwords; w += CV_THREADS) lds[w] = t[w];
+        const unsigned int b = src.majorPtr[r], e = src.majorPtr[r + 1u];
+        __syncthreads();
+        const bool cached = e - b <= E * CV_THREADS;
+        unsigned int pos[E];
+        S v[E];
+        if (cached) {
+#pragma unroll
+            for (unsigned int k = 0; k < E; ++k) {
+                const unsigned int i = b + threadIdx.x + k * CV_THREADS;
+                pos[k] = INV;
+                v[k] = S(0);
+                if (i < e) {
+                    const unsigned int c = src.minorIdx[i];
+                    if (c < src.cols) {
+                        pos[k] = bell_pos(bits, prefix, bs, c);
+                        v[k] = sval[i];
+                        nz += stored_nonzero<S>(v[k]) ? 1u : 0u;
+                    }
                 }
-                const unsigned int bc = c / bs;
-                const unsigned int wbits = bits[bc >> 5];
-                const unsigned int slot = prefix[bc >> 5] + (unsigned int) __popc(wbits & ((1u << (bc & 31u)) - 1u));
-                const S v = sval[i];
-                dval[(unsigned long long) lo * ell_cols + (unsigned long long) slot * bs + (c - bc * bs)] = v;
-                nz += stored_nonzero<S>(v) ? 1u : 0u;
             }
-        } else if (threadIdx.x == 0) {
-            // serial CSR-order replay: last write wins, nnz tracks transitions
-            long long count = 0;
-            for (unsigned int r = r0; r < r1; ++r) {
-                for (unsigned int i = src.majorPtr[r]; i < src.majorPtr[r + 1u]; ++i) {
+        }
+        for (unsigned int p0 = 0; p0 < ell_cols; p0 += SEG) {
+            const unsigned int ne = ell_cols - p0 < SEG ? ell_cols - p0 : SEG;
+            const unsigned int nw = (ne + PER_WORD - 1u) / PER_WORD;
+            for (unsigned int w = 4u * threadIdx.x; w < nw; w += 4u * CV_THREADS) *(uint4 *) (seg + w) = make_uint4(0u, 0u, 0u, 0u);
+            __syncthreads();
+            if (cached) {
+#pragma unroll
+                for (unsigned int k = 0; k < E; ++k)
+                    if (pos[k] - p0 < ne) segv[pos[k] - p0] = v[k];
+            } else {
+                for (unsigned int i = b + threadIdx.x; i < e; i += CV_THREADS) {
                     const unsigned int c = src.minorIdx[i];
                     if (c >= src.cols) continue;
-                    const unsigned int bc = c / bs;
-                    const unsigned int wbits = bits[bc >> 5];
-                    const unsigned int slot = prefix[bc >> 5] + (unsigned int) __popc(wbits & ((1u << (bc & 31u)) - 1u));
-                    S *dst = dval + (unsigned long long) r * ell_cols + (unsigned long long) slot * bs + (c - bc * bs);
-                    const bool prev = stored_nonzero<S>(*dst);
-                    const bool next = stored_nonzero<S>(sval[i]);
-                    if (!prev && next) ++count;
-                    else if (prev && !next) --count;
-                    *dst = sval[i];
+                    const unsigned int q = bell_pos(bits, prefix, bs, c) - p0;
+                    if (q >= ne) continue;
+                    const S x = sval[i];
+                    segv[q] = x;
+                    nz += stored_nonzero<S>(x) ? 1u : 0u;
                 }
             }
-            nz = (unsigned int) count;
+            __syncthreads();
+            if (quads) {  // nw is a multiple of 4 here; streaming stores
+                typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+                u32x4 *ow = (u32x4 *) (orow + p0);
+                for (unsigned int w = threadIdx.x; 4u * w < nw; w += CV_THREADS)
+                    __builtin_nontemporal_store(*(const u32x4 *) (seg + 4u * w), ow + w);
+            } else if (words) {
+                unsigned int *ow = (unsigned int *) (orow + p0);
+                for (unsigned int w = threadIdx.x; w < nw; w += CV_THREADS) ow[w] = seg[w];
+            } else {
+                for (unsigned int k = threadIdx.x; k < ne; k += CV_THREADS) orow[p0 + k] = segv[k];
+            }
+            __syncthreads();
         }
-        nz = wave_sum_u32(nz);
-        if (lane_id() == 0u && nz != 0u) atomicAdd(nnz_total, (unsigned long long) nz);
-        __syncthreads();
+    }
+    nz = wave_sum_u32(nz);
+    if (lane_id() == 0u && nz != 0u) atomicAdd(nnz_total, (unsigned long long) nz);
+}
+
+// bell_replay_kernel, one thread per unsorted row block (after
+// bell_rows_kernel zeroed its rows): CSR-order replay, the last write of a
+// repeated column wins, nnz tracks zero / nonzero transitions.
+template<typename S>
+__global__ void bell_replay_kernel(csamd_compressed_view src, const S *__restrict__ sval, unsigned int bs, unsigned int row_blocks,
+                                   unsigned int nwords, unsigned int ell_cols, const unsigned int *__restrict__ tab,
+                                   const unsigned int *__restrict__ unsorted, S *__restrict__ dval,
+                                   unsigned long long *__restrict__ nnz_total) {
+    for (unsigned int rb = blockIdx.x * blockDim.x + threadIdx.x; rb < row_blocks; rb += gridDim.x * blockDim.x) {
+        if (unsorted[rb] == 0u) continue;
+        const unsigned int *bits = tab + (size_t) rb * 2u * nwords;
+        const unsigned int *prefix = bits + nwords;
+        const unsigned int r0 = rb * bs;
+        const unsigned int r1 = r0 + bs < src.rows ? r0 + bs : src.rows;
+        long long count = 0;
+        for (unsigned int r = r0; r < r1; ++r) {
+            for (unsigned int i = src.majorPtr[r]; i < src.majorPtr[r + 1u]; ++i) {
+                const unsigned int c = src.minorIdx[i];
+                if (c >= src.cols) continue;
+                const unsigned int q = bell_pos(bits, prefix, bs, c);
+                if (q >= ell_cols) continue;  // width too small: reported by bell_blocks_kernel
+                S *dst = dval + (unsigned long long) r * ell_cols + q;
+                const bool prev = stored_nonzero<S>(*dst);
+                const bool next = stored_nonzero<S>(sval[i]);
+                if (!prev && next) ++count;
+                else if (prev && !next) --count;
+                *dst = sval[i];
+            }
+        }
+        if (count != 0) atomicAdd(nnz_total, (unsigned long long) count);
     }
 }
 
@@ -397,13 +519,31 @@ namespace {
 
 template<typename S> struct bell_emit_launch {
     static int run(const csamd_compressed_view *src, unsigned int row_blocks, unsigned int width, unsigned int nwords,
-                   csamd_blocked_ell_view *dst, unsigned long long *d_nnz, hipStream_t s) {
-        const size_t lds = (size_t) 2u * nwords * sizeof(unsigned int);
-        auto k = bell_emit_kernel<S>;
-        CSAMD_HIP(hipFuncSetAttribute((const void *) k, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds), "set lds");
-        k<<<grid_for(row_blocks, 1u, (unsigned int) device_cu_count() * 8u), CV_THREADS, lds, s>>>(
-            *src, (const S *) src->val, dst->block_size, row_blocks, width, nwords, dst->ell_cols, dst->blockColIdx, (S *) dst->val, d_nnz);
-        return hip_ok(hipGetLastError(), "bell_emit_kernel");
+                   csamd_blocked_ell_view *dst, unsigned int *tab, unsigned int *unsorted, unsigned long long *d_nnz,
+                   unsigned int *need_w, hipStream_t s) {
+        const unsigned int cus = (unsigned int) device_cu_count();
+        {
+            const size_t lds = ((size_t) 2u * nwords + dst->block_size + 1u) * sizeof(unsigned int);
+            auto k = bell_blocks_kernel;
+            CSAMD_HIP(hipFuncSetAttribute((const void *) k, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds), "set lds");
+            k<<<grid_for(row_blocks, 1u, cus * 8u), CV_THREADS, lds, s>>>(*src, dst->block_size, row_blocks, width, nwords,
+                                                                          dst->blockColIdx, tab, unsorted, need_w);
+            CSAMD_HIP(hipGetLastError(), "bell_blocks_kernel");
+        }
+        {
+            // value row segments of 16 or 32 KB (CSAMD_BELL_SEG = 4096 / 8192 words, for sweeps)
+            const char *env = std::getenv("CSAMD_BELL_SEG");
+            const unsigned int segw = env != nullptr && std::atoi(env) == 8192 ? 8192u : 4096u;
+            const size_t lds = ((((size_t) 2u * nwords + 3u) & ~(size_t) 3u) + segw) * sizeof(unsigned int);
+            auto k = segw == 8192u ? bell_rows_kernel<S, 16u, 8192u> : bell_rows_kernel<S, 16u, 4096u>;
+            CSAMD_HIP(hipFuncSetAttribute((const void *) k, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds), "set lds");
+            k<<<grid_for(src->rows, 1u, cus * 16u), CV_THREADS, lds, s>>>(*src, (const S *) src->val, dst->block_size, nwords,
+                                                                          dst->ell_cols, tab, unsorted, (S *) dst->val, d_nnz);
+            CSAMD_HIP(hipGetLastError(), "bell_rows_kernel");
+        }
+        bell_replay_kernel<S><<<grid_for(row_blocks, 64u, 1024u), 64, 0, s>>>(*src, (const S *) src->val, dst->block_size, row_blocks,
+                                                                             nwords, dst->ell_cols, tab, unsorted, (S *) dst->val, d_nnz);
+        return hip_ok(hipGetLastError(), "bell_replay_kernel");
     }
 };
 
@@ -442,6 +582,7 @@ int bell_shape_impl(const csamd_compressed_view *src, unsigned int bs, csamd_con
     const unsigned int col_blocks = (src->cols + bs - 1u) / bs;
     const unsigned int nwords = (col_blocks + 31u) / 32u;
     CSAMD_REQUIRE(nwords <= BELL_MAX_BITSET_WORDS, "blocked_ell: %u column blocks exceed the LDS bitset", col_blocks);
+    CSAMD_REQUIRE(bs <= BELL_MAX_BLOCK_SIZE, "blocked_ell: block size %u exceeds %u", bs, BELL_MAX_BLOCK_SIZE);
     char *scratch = (char *) convert_scratch(ws, align_up((size_t) row_blocks * 4u, 256u) + 256u);
     if (scratch == nullptr) return 0;
     unsigned int *counts = (unsigned int *) (scratch + 256u);
@@ -450,7 +591,7 @@ int bell_shape_impl(const csamd_compressed_view *src, unsigned int bs, csamd_con
     CSAMD_HIP(hipMemsetAsync(scratch, 0, 128u, s), "memset shape");
     if (row_blocks != 0u) {
         auto k = bell_shape_kernel;
-        const size_t lds = (size_t) nwords * 4u;
+        const size_t lds = ((size_t) nwords + bs + 1u) * 4u;
         CSAMD_HIP(hipFuncSetAttribute((const void *) k, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds), "set lds");
         k<<<grid_for(row_blocks, 1u, (unsigned int) device_cu_count() * 8u), CV_THREADS, lds, s>>>(*src, bs, row_blocks, nwords, counts, d_w,
                                                                                                d_total);
@@ -536,29 +677,39 @@ int csamd_blocked_ell_from_compressed(const csamd_compressed_view *src, unsigned
     const unsigned int col_blocks = (src->cols + bs - 1u) / bs;
     const unsigned int nwords = (col_blocks + 31u) / 32u;
     CSAMD_REQUIRE(nwords <= BELL_MAX_BITSET_WORDS, "blocked_ell: %u column blocks exceed the LDS bitset", col_blocks);
+    CSAMD_REQUIRE(bs <= BELL_MAX_BLOCK_SIZE, "blocked_ell: block size %u exceeds %u", bs, BELL_MAX_BLOCK_SIZE);
     dst->rows = src->rows;
     dst->cols = src->cols;
     if (src->rows == 0u) {
         dst->nnz = 0u;
         return 1;
     }
-    // the requested width must cover every row block
-    unsigned int need_w = 0u;
-    unsigned long long total = 0ull;
-    if (!bell_shape_impl(src, bs, ws, &need_w, &total)) return 0;
-    CSAMD_REQUIRE(need_w <= width, "blocked_ell_from_compressed: ell width %u < required %u", width, need_w);
-    CSAMD_REQUIRE(width == 0u || (dst->blockColIdx != nullptr && dst->val != nullptr),
-                  "blocked_ell_from_compressed: destination arrays missing");
-    unsigned long long *d_nnz = (unsigned long long *) convert_scratch(ws, 256u);
-    if (d_nnz == nullptr) return 0;
-    CSAMD_HIP(hipMemsetAsync(d_nnz, 0, 8u, s), "memset nnz");
-    if (width != 0u) {
-        if (!zero_bytes(dst->val, (size_t) dst->rows * dst->ell_cols * (size_t) value_type_bytes(value_type), s)) return 0;
-        if (!bits_dispatch<bell_emit_launch>(value_type, src, row_blocks, width, nwords, dst, d_nnz, s)) return 0;
+    if (width == 0u) {  // no slots: every row block must be empty
+        unsigned int need_w = 0u;
+        unsigned long long total = 0ull;
+        if (!bell_shape_impl(src, bs, ws, &need_w, &total)) return 0;
+        CSAMD_REQUIRE(need_w == 0u, "blocked_ell_from_compressed: ell width 0 < required %u", need_w);
+        dst->nnz = 0u;
+        return 1;
     }
-    unsigned long long nnz = 0ull;
-    CSAMD_HIP(hipMemcpyAsync(&nnz, d_nnz, 8u, hipMemcpyDeviceToHost, s), "D2H bell nnz");
+    CSAMD_REQUIRE(dst->blockColIdx != nullptr && dst->val != nullptr, "blocked_ell_from_compressed: destination arrays missing");
+    // scratch: nnz and the widest row block's need (first 256 bytes), per
+    // row block the bitset and its prefix, unsorted flags
+    const size_t tab_bytes = align_up((size_t) row_blocks * 2u * nwords * 4u, 256u);
+    char *scratch = (char *) convert_scratch(ws, 256u + tab_bytes + align_up((size_t) row_blocks * 4u, 256u));
+    if (scratch == nullptr) return 0;
+    unsigned long long *d_nnz = (unsigned long long *) scratch;
+    unsigned int *d_need = (unsigned int *) (scratch + 64u);
+    unsigned int *tab = (unsigned int *) (scratch + 256u);
+    unsigned int *unsorted = (unsigned int *) (scratch + 256u + tab_bytes);
+    CSAMD_HIP(hipMemsetAsync(scratch, 0, 128u, s), "memset nnz");
+    if (!bits_dispatch<bell_emit_launch>(value_type, src, row_blocks, width, nwords, dst, tab, unsorted, d_nnz, d_need, s)) return 0;
+    unsigned long long hres[9] = {};
+    CSAMD_HIP(hipMemcpyAsync(hres, scratch, 72u, hipMemcpyDeviceToHost, s), "D2H bell nnz");
     CSAMD_HIP(hipStreamSynchronize(s), "bell emit sync");
+    const unsigned long long nnz = hres[0];
+    const unsigned int need = (unsigned int) hres[8];
+    CSAMD_REQUIRE(need <= width, "blocked_ell_from_compressed: ell width %u < required %u", width, need);
     CSAMD_REQUIRE(nnz <= 0xffffffffull, "blocked_ell_from_compressed: nnz overflow");
     dst->nnz = (unsigned int) nnz;
     return 1;

@@ -61,6 +61,40 @@ def test_blocked_ell_random_parity(ws, vt):
         assert np.array_equal(gb, bci) and np.array_equal(gv.view(v.dtype), v)
 
 
+@pytest.mark.parametrize("extra", [-1, 0, 3])
+def test_blocked_ell_given_width(ws, extra):
+    # a caller-chosen ell width: wider than needed pads with invalid slots and
+    # zero values; narrower fails with the width the rows need
+    import ctypes as C
+    from cellshard_amd import CellShardError, load
+    from cellshard_amd.device import DeviceBlockedELL
+    from cellshard_amd.ops import blocked_ell_shape
+    rows, cols, bs = 300, 900, 8
+    ptr, idx, val = random_csr(rows, cols, 0.03, seed=11, empty_rows=0.05)
+    bits = f16_bits(val)
+    src = DeviceCSR.from_numpy(rows, cols, ptr, idx, bits)
+    need, _, _ = blocked_ell_shape(src, bs, ws)
+    width = need + extra
+    rb = (rows + bs - 1) // bs
+    out = DeviceBlockedELL(rows, cols, 0, bs, width * bs, torch.empty(rb * width, dtype=torch.int32, device="cuda"),
+                           torch.empty(rows * width * bs, dtype=torch.int16, device="cuda"), F16)
+    v, sv = out.view(), src.view()
+    ok = load().csamd_blocked_ell_from_compressed(C.byref(sv), F16, C.byref(v), C.byref(ws.ws))
+    if extra < 0:
+        assert ok == 0
+        with pytest.raises(CellShardError, match="required"):
+            from cellshard_amd._lib import check
+            check(ok, "blocked_ell_from_compressed")
+        return
+    assert ok == 1
+    bci, ref_v, ell, nnz = O.blocked_ell_from_compressed(rows, cols, ptr, idx, bits, bs)
+    gb = out.block_col_idx.cpu().numpy().view(np.uint32).reshape(rb, width)
+    gv = out.val.cpu().numpy().view(np.uint16).reshape(rows, width * bs)
+    assert int(v.nnz) == nnz and ell == need * bs
+    assert np.array_equal(gb[:, :need].ravel(), bci) and np.all(gb[:, need:] == 0xFFFFFFFF)
+    assert np.array_equal(gv[:, :need * bs].ravel(), ref_v.view(np.uint16)) and np.all(gv[:, need * bs:] == 0)
+
+
 def test_blocked_ell_unsorted_duplicates(ws):
     rows, cols = 90, 70
     ptr, idx, val = random_csr(rows, cols, 0.1, seed=2, sorted_rows=False, duplicates=0.2, zeros=0.2)
