@@ -312,8 +312,8 @@ def run_plain_path(args, log):
 
     from cellshard_amd import abi as A
     from cellshard_amd.device import to_dev_u32, to_dev_u8
-    from cellshard_amd.ops import (ConvertWorkspace, LabelReduce, TransposeScratch, blocked_ell_from_compressed,
-                                   compressed_transpose, synth_csr)
+    from cellshard_amd.ops import (ConvertWorkspace, FilteredBlockedEllWorkspace, LabelReduce, TransposeScratch,
+                                   blocked_ell_from_compressed, compressed_transpose, filter_blocked_ell_to_compressed, synth_csr)
     from oracle import oracle as O
 
     dev = torch.device("cuda", 0)
@@ -340,7 +340,21 @@ def run_plain_path(args, log):
 
     bell, bell_ms = gpu_ms(lambda: blocked_ell_from_compressed(src, 16, ws))
     bell_bytes = bell.format_bytes()
-    del bell
+    # masked Blocked-ELL -> CSR (the step after the reduce): the kept rows
+    # (p = 0.9) and the 2,000 kept columns, both compacted by remaps
+    rr = np.full(R, 0xFFFFFFFF, np.uint32)
+    rr[rk == 1] = np.arange(int(rk.sum()))
+    cr = np.full(Cc, 0xFFFFFFFF, np.uint32)
+    cr[fk == 1] = np.arange(int(fk.sum()))
+    fws = FilteredBlockedEllWorkspace(stream=torch.cuda.current_stream().cuda_stream)
+    f_args = (bell, int(rk.sum()), int(fk.sum()), to_dev_u8(rk, dev), to_dev_u8(fk, dev), to_dev_u32(rr, dev), to_dev_u32(cr, dev))
+    fo, filt_ms = gpu_ms(lambda: filter_blocked_ell_to_compressed(*f_args, ws=fws))
+    filt_live = int(fo.filtered.nnz)
+    # algorithmic bytes: the kept value rows and the slot table read once, the CSR written
+    # (the count and emit passes each read the kept rows)
+    filt_bytes = int(rk.sum()) * bell.ell_cols * 4 + bell.row_blocks * (bell.ell_cols // 16) * 4 + filt_live * 8 + (int(rk.sum()) + 1) * 4
+    fws.close()
+    del bell, f_args
     torch.cuda.empty_cache()
     scratch = TransposeScratch()
     _, tr_ms = gpu_ms(lambda: compressed_transpose(src, scratch))
@@ -355,6 +369,8 @@ def run_plain_path(args, log):
     res = {"workload": "plain_csr_20000x33538_fp32_bell16_transpose_16labels", "rows": R, "cols": Cc, "nnz": nnz,
            "gpu": {"blocked_ell_16_ms": round(bell_ms, 3), "blocked_ell_16_gb_s": round((csr_bytes + bell_bytes) / (bell_ms * 1e-3) / 1e9, 1),
                    "blocked_ell_bytes_over_csr": round(bell_bytes / csr_bytes, 2),
+                   "filter_bell_to_csr_ms": round(filt_ms, 3), "filter_bell_to_csr_gb_s": round(filt_bytes / (filt_ms * 1e-3) / 1e9, 1),
+                   "filter_live_nnz": filt_live,
                    "transpose_ms": round(tr_ms, 3), "transpose_gb_s": round(2 * csr_bytes / (tr_ms * 1e-3) / 1e9, 1),
                    "reduce_ms": round(red_ms, 4), "reduce_gb_s": round(red_bytes / (red_ms * 1e-3) / 1e9, 1)}}
     if not args.skip_cpu:

@@ -57,6 +57,13 @@ PROTOTYPES = {
     "csamd_reoptimize_setup": (i32, [P(A.masked_sparse_reoptimize_workspace), i32, vp]),
     "csamd_manual_reoptimize_masked_sparse": (i32, [P(A.blocked_ell_view), P(A.row_feature_mask_view),
         P(A.masked_sparse_reoptimize_config), P(A.masked_sparse_reoptimize_workspace), P(A.masked_sparse_reoptimize_result)]),
+    "csamd_filtered_blocked_ell_init": (None, [P(A.filtered_blocked_ell_workspace)]),
+    "csamd_filtered_blocked_ell_clear": (None, [P(A.filtered_blocked_ell_workspace)]),
+    "csamd_filtered_blocked_ell_setup": (i32, [P(A.filtered_blocked_ell_workspace), i32, vp]),
+    "csamd_filter_blocked_ell_to_compressed": (i32, [P(A.blocked_ell_view), u32, u32, u32, vp, vp, vp, vp,
+        P(A.filtered_blocked_ell_workspace), P(A.filtered_blocked_ell_result)]),
+    "csamd_build_bucketed_filtered_blocked_ell_major_view": (i32, [P(A.blocked_ell_view), u32, u32, u32, vp, vp, vp, vp, u32,
+        P(A.filtered_blocked_ell_workspace), P(A.filtered_blocked_ell_result)]),
     # label reduce
     "csamd_label_reduce_init": (None, [P(A.label_reduce_workspace)]),
     "csamd_label_reduce_clear": (None, [P(A.label_reduce_workspace)]),

@@ -162,6 +162,25 @@ major_nnz_bucket_plan_view = _aligned16("major_nnz_bucket_plan_view", [
     ("bucket_offsets", c_void_p), ("inverse_major_order", c_void_p),
 ])
 
+filtered_blocked_ell_stats = _aligned16("filtered_blocked_ell_stats", [
+    ("kept_rows", c_u32), ("output_rows", c_u32), ("output_cols", c_u32), ("live_nnz", c_u32),
+    ("kept_row_slots", C.c_uint64), ("dead_slots", C.c_uint64), ("live_fill_ratio", C.c_double),
+    ("dead_value_bytes", c_size_t),
+])
+
+compressed_major_bucket_result = _aligned16("compressed_major_bucket_result", [
+    ("rebuilt", compressed_view), ("plan", major_nnz_bucket_plan_view),
+])
+
+filtered_blocked_ell_result = _aligned16("filtered_blocked_ell_result", [
+    ("filtered", compressed_view), ("bucketed", compressed_major_bucket_result), ("stats", filtered_blocked_ell_stats),
+    ("has_bucketed", C.c_int),
+])
+
+filtered_blocked_ell_workspace = _aligned16("filtered_blocked_ell_workspace", [
+    ("device", C.c_int), ("stream", c_void_p), ("owns_stream", C.c_int), ("private_workspace", c_void_p),
+])
+
 synth_params = _aligned16("synth_params", [
     ("rows", c_u32), ("cols", c_u32), ("value_type", c_u32), ("seed", c_u32),
     ("row_nnz_median", C.c_float), ("row_nnz_sigma", C.c_float), ("nb_alpha", C.c_float), ("reserved", c_u32),
@@ -277,6 +296,10 @@ ALL_STRUCTS = {
     "csamd_blocked_ell_tune_result": blocked_ell_tune_result,
     "csamd_sliced_ell_tune_result": sliced_ell_tune_result,
     "csamd_major_nnz_bucket_plan_view": major_nnz_bucket_plan_view,
+    "csamd_filtered_blocked_ell_stats": filtered_blocked_ell_stats,
+    "csamd_compressed_major_bucket_result": compressed_major_bucket_result,
+    "csamd_filtered_blocked_ell_result": filtered_blocked_ell_result,
+    "csamd_filtered_blocked_ell_workspace": filtered_blocked_ell_workspace,
     "csamd_synth_params": synth_params,
     "csamd_host_compressed": host_compressed,
     "csamd_host_blocked_ell": host_blocked_ell,

@@ -35,6 +35,7 @@ HIP_SOURCES = [
     "partition_tune.hip",
     "coo_convert.hip",
     "execution_layout.hip",
+    "filter.hip",
 ]
 
 HIPCC_FLAGS = [

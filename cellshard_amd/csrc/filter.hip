@@ -1,0 +1,539 @@
+// Masked Blocked-ELL -> CSR compaction with row / column remaps, and the
+// same followed by a major-nnz bucket plan and rebuild (the step after the
+// grouped reduce: QC keep -> compact -> rebucket).
+//
+// Reference: src/convert/filtered_blocked_ell_to_compressed.cuh:125-415
+// (filtered_blocked_ell_{stats,result,workspace}, filter_blocked_ell_to_compressed,
+// build_bucketed_filtered_blocked_ell_major_view) and
+// src/bucket/operators/major_nnz.cuh:11-213 (build_bucketed_major_view).
+//
+// An entry of source row r at ell position e survives when the row is kept
+// (keep_rows null or nonzero) and maps to out_row = row_remap[r] (or r) <
+// output_rows, its column block is valid, col < cols, the stored value is
+// nonzero, the column is kept, and out_col = col_remap[col] (or col) <
+// output_cols.  Output rows hold their entries in source ell order.  The
+// row pointers are a plain exclusive scan of the per-row counts: row i of
+// the output is [ptr[i], ptr[i + 1]).  (The reference writes each count one
+// place to the right and then runs an exclusive scan, which shifts every
+// row by one more place; the row contents here are what its emit kernel
+// intends.)  Two kept rows mapped to one output row are an error.
+//
+// gfx950 plan: one workgroup per source row, threads across the ell
+// positions (16-byte value groups, four in flight per thread); a count
+// pass, one exclusive scan, then an emit pass that places each thread's
+// entries with a block prefix sum.  Padding (zero values) is rejected
+// before any gather.  Rows that are not whole 16-byte groups take a
+// one-wave-per-row path.
+#include "common.hh"
+
+#include <cstdlib>
+#include <cstring>
+
+namespace csamd {
+namespace {
+
+constexpr unsigned int FL_THREADS = 256u;
+constexpr unsigned int FL_WAVES = FL_THREADS / 64u;
+constexpr unsigned int FL_LDS_SLOTS = 8192u;  // block columns of a row block staged in LDS (32 KB)
+
+template<typename S> __device__ __forceinline__ bool fl_nonzero(S raw);
+template<> __device__ __forceinline__ bool fl_nonzero<uint16_t>(uint16_t raw) { return (raw & 0x7fffu) != 0u; }
+template<> __device__ __forceinline__ bool fl_nonzero<float>(float raw) { return raw != 0.0f; }
+template<> __device__ __forceinline__ bool fl_nonzero<uint8_t>(uint8_t raw) { return (raw & 0x7fu) != 0u; }
+
+struct fl_masks {
+    unsigned int output_rows, output_cols;
+    const unsigned char *keep_rows, *keep_cols;
+    const unsigned int *row_remap, *col_remap;
+    const unsigned int *col_map;  // keep and remap of every column in one lookup (~0u: dropped), or null
+};
+
+// col_map[c] = output column of source column c, or ~0u when it is dropped
+// (not kept, or remapped past output_cols): one gather per surviving value
+// instead of two dependent ones.
+__global__ void fl_col_map_kernel(unsigned int cols, fl_masks m, unsigned int *__restrict__ col_map) {
+    for (unsigned int c = blockIdx.x * blockDim.x + threadIdx.x; c < cols; c += gridDim.x * blockDim.x) {
+        unsigned int o = ~0u;
+        if (m.keep_cols == nullptr || m.keep_cols[c] != 0u) {
+            o = m.col_remap != nullptr ? m.col_remap[c] : c;
+            if (o >= m.output_cols) o = ~0u;
+        }
+        col_map[c] = o;
+    }
+}
+
+// Output row of source row r, or output_rows when the row is dropped.
+__device__ __forceinline__ unsigned int fl_out_row(const fl_masks &m, unsigned int r) {
+    if (m.keep_rows != nullptr && m.keep_rows[r] == 0u) return m.output_rows;
+    const unsigned int o = m.row_remap != nullptr ? m.row_remap[r] : r;
+    return o < m.output_rows ? o : m.output_rows;
+}
+
+// Output column of ell position e of row r (block row rb), or ~0u if the
+// entry is dropped; *v gets its stored value.
+template<typename S>
+__device__ __forceinline__ unsigned int fl_out_col(const csamd_blocked_ell_view &src, const S *val, const fl_masks &m, unsigned int width,
+                                                   unsigned int rb, unsigned int r, unsigned int e, S *v) {
+    const unsigned int bs = src.block_size;
+    const unsigned int slot = e / bs;
+    const unsigned int bc = src.blockColIdx[(unsigned long long) rb * width + slot];
+    if (bc == CSAMD_INVALID_COL) return ~0u;
+    const unsigned int col = bc * bs + (e - slot * bs);
+    if (col >= src.cols) return ~0u;
+    *v = val[(unsigned long long) r * src.ell_cols + e];
+    if (!fl_nonzero<S>(*v)) return ~0u;
+    if (m.keep_cols != nullptr && m.keep_cols[col] == 0u) return ~0u;
+    const unsigned int oc = m.col_remap != nullptr ? m.col_remap[col] : col;
+    return oc < m.output_cols ? oc : ~0u;
+}
+
+// A row's values are read in 16-byte groups (PER values per lane, U
+// groups per lane in flight) when the value rows are whole, 16-byte aligned
+// groups (ell_cols * sizeof(S) % 16 == 0 and an aligned array); otherwise
+// one value per lane at a time.  Groups are in position order: group g of a
+// pass holds positions base + (g * 64 + lane) * PER + k.
+template<typename S> union fl_group {
+    uint4 q;
+    S v[16u / sizeof(S)];
+};
+
+// Surviving entries of group g (bit k: position k of the lane's group);
+// slots: the row block's block columns (LDS or global).
+template<typename S>
+__device__ __forceinline__ unsigned int fl_group_mask(const csamd_blocked_ell_view &src, const fl_masks &m, const unsigned int *slots,
+                                                      unsigned int e, const fl_group<S> &g, unsigned int *oc) {
+    constexpr unsigned int PER = 16u / sizeof(S);
+    unsigned int mask = 0u;
+#pragma unroll
+    for (unsigned int k = 0; k < PER; ++k) {
+        oc[k] = ~0u;
+        if (e + k >= src.ell_cols || !fl_nonzero<S>(g.v[k])) continue;  // most padding stops here, before any gather
+        const unsigned int pos = e + k;
+        const unsigned int slot = pos / src.block_size;
+        const unsigned int bc = slots[slot];
+        if (bc == CSAMD_INVALID_COL) continue;
+        const unsigned int col = bc * src.block_size + (pos - slot * src.block_size);
+        if (col >= src.cols) continue;
+        const unsigned int o = m.col_map[col];
+        if (o == ~0u) continue;
+        oc[k] = o;
+        mask |= 1u << k;
+    }
+    return mask;
+}
+
+// The row block's block columns: staged in LDS when they fit (the caller
+// syncs), else read in place.
+__device__ __forceinline__ const unsigned int *fl_slots(const csamd_blocked_ell_view &src, unsigned int width, unsigned int rb,
+                                                       unsigned int *lds_slots, unsigned int lds_cap) {
+    const unsigned int *g = src.blockColIdx + (unsigned long long) rb * width;
+    if (width > lds_cap) return g;
+    for (unsigned int k = threadIdx.x; k < width; k += blockDim.x) lds_slots[k] = g[k];
+    return lds_slots;
+}
+
+template<typename S, unsigned int U>
+__device__ __forceinline__ void fl_load_groups(const S *row, unsigned int ell_cols, unsigned int e0, fl_group<S> (&g)[U]) {
+    constexpr unsigned int PER = 16u / sizeof(S);
+    const unsigned int lane = lane_id();
+#pragma unroll
+    for (unsigned int u = 0; u < U; ++u) {
+        const unsigned int e = e0 + (u * 64u + lane) * PER;
+        g[u].q = e < ell_cols ? *(const uint4 *) (row + e) : make_uint4(0u, 0u, 0u, 0u);
+    }
+}
+
+// Count pass: counts[out_row] = surviving entries of the row; claim[] marks
+// output rows already taken (a second claim sets *dup).  Grouped: one
+// workgroup per row (its four waves take interleaved group rows, so a CU
+// streams few rows at a time); otherwise one wave per row.
+template<typename S, unsigned int U>
+__global__ void __launch_bounds__(FL_THREADS) fl_count_kernel(csamd_blocked_ell_view src, const S *__restrict__ val, fl_masks m,
+                                                              bool grouped, unsigned int slot_cap, unsigned int *__restrict__ counts,
+                                                              unsigned int *__restrict__ claim, unsigned int *__restrict__ dup) {
+    constexpr unsigned int PER = 16u / sizeof(S);
+    const unsigned int width = src.ell_cols / src.block_size;
+    const unsigned int lane = lane_id(), wave = threadIdx.x >> 6;
+    __shared__ unsigned int red[FL_WAVES];
+    extern __shared__ unsigned int lds_slots[];
+    if (grouped) {
+        unsigned int cur_rb = ~0u;
+        const unsigned int *slots = nullptr;
+        for (unsigned int r = blockIdx.x; r < src.rows; r += gridDim.x) {
+            const unsigned int o = fl_out_row(m, r);
+            if (o >= m.output_rows) continue;
+            const unsigned int rb = r / src.block_size;
+            if (rb != cur_rb) {
+                __syncthreads();
+                slots = fl_slots(src, width, rb, lds_slots, slot_cap);
+                cur_rb = rb;
+                __syncthreads();
+            }
+            const S *row = val + (unsigned long long) r * src.ell_cols;
+            unsigned int n = 0u;
+            for (unsigned int e0 = wave * 64u * U * PER; e0 < src.ell_cols; e0 += FL_WAVES * 64u * U * PER) {
+                fl_group<S> g[U];
+                fl_load_groups<S, U>(row, src.ell_cols, e0, g);
+#pragma unroll
+                for (unsigned int u = 0; u < U; ++u) {
+                    unsigned int oc[PER];
+                    n += (unsigned int) __popc(fl_group_mask<S>(src, m, slots, e0 + (u * 64u + lane) * PER, g[u], oc));
+                }
+            }
+            n = wave_sum_u32(n);
+            if (lane == 0u) red[wave] = n;
+            __syncthreads();
+            if (threadIdx.x == 0u) {
+                unsigned int t = 0u;
+                for (unsigned int k = 0; k < FL_WAVES; ++k) t += red[k];
+                if (atomicExch(&claim[o], 1u) != 0u) *dup = o + 1u;
+                counts[o] = t;
+            }
+            __syncthreads();
+        }
+        return;
+    }
+    for (unsigned int r = blockIdx.x * FL_WAVES + wave; r < src.rows; r += gridDim.x * FL_WAVES) {
+        const unsigned int o = fl_out_row(m, r);
+        if (o >= m.output_rows) continue;
+        const unsigned int rb = r / src.block_size;
+        unsigned int n = 0u;
+        for (unsigned int e = lane; e < src.ell_cols; e += 64u) {
+            S v;
+            n += fl_out_col<S>(src, val, m, width, rb, r, e, &v) != ~0u ? 1u : 0u;
+        }
+        n = wave_sum_u32(n);
+        if (lane == 0u) {
+            if (atomicExch(&claim[o], 1u) != 0u) *dup = o + 1u;
+            counts[o] = n;
+        }
+    }
+}
+
+// Emit pass, in position order.  Grouped: one workgroup per row, a block
+// prefix sum of the threads' surviving counts per pass of FL_THREADS x U
+// groups places each thread's entries; otherwise one wave per row with a
+// ballot per 64-position chunk.
+template<typename S, unsigned int U>
+__global__ void __launch_bounds__(FL_THREADS) fl_emit_kernel(csamd_blocked_ell_view src, const S *__restrict__ val, fl_masks m,
+                                                             bool grouped, unsigned int slot_cap, const unsigned int *__restrict__ ptr,
+                                                             unsigned int *__restrict__ out_idx, S *__restrict__ out_val) {
+    constexpr unsigned int PER = 16u / sizeof(S);
+    const unsigned int width = src.ell_cols / src.block_size;
+    const unsigned int lane = lane_id();
+    __shared__ unsigned int smem[16];
+    extern __shared__ unsigned int lds_slots[];
+    if (grouped) {
+        unsigned int cur_rb = ~0u;
+        const unsigned int *slots = nullptr;
+        for (unsigned int r = blockIdx.x; r < src.rows; r += gridDim.x) {
+            const unsigned int o = fl_out_row(m, r);
+            if (o >= m.output_rows) continue;
+            const unsigned int rb = r / src.block_size;
+            if (rb != cur_rb) {
+                __syncthreads();
+                slots = fl_slots(src, width, rb, lds_slots, slot_cap);
+                cur_rb = rb;
+                __syncthreads();
+            }
+            const S *row = val + (unsigned long long) r * src.ell_cols;
+            unsigned int w = ptr[o];
+            // thread t takes groups t, t + FL_THREADS, ... of the pass (position order = thread order per group row)
+            for (unsigned int e0 = 0; e0 < src.ell_cols; e0 += FL_THREADS * U * PER) {
+                fl_group<S> g[U];
+#pragma unroll
+                for (unsigned int u = 0; u < U; ++u) {
+                    const unsigned int e = e0 + (u * FL_THREADS + threadIdx.x) * PER;
+                    g[u].q = e < src.ell_cols ? *(const uint4 *) (row + e) : make_uint4(0u, 0u, 0u, 0u);
+                }
+#pragma unroll
+                for (unsigned int u = 0; u < U; ++u) {
+                    unsigned int oc[PER];
+                    const unsigned int mask = fl_group_mask<S>(src, m, slots, e0 + (u * FL_THREADS + threadIdx.x) * PER, g[u], oc);
+                    const unsigned int c = (unsigned int) __popc(mask);
+                    unsigned int total = 0u;
+                    unsigned int k = w + block_exclusive_scan_u32(c, smem, &total);
+#pragma unroll
+                    for (unsigned int j = 0; j < PER; ++j) {
+                        if ((mask >> j) & 1u) {
+                            out_idx[k] = oc[j];
+                            out_val[k] = g[u].v[j];
+                            ++k;
+                        }
+                    }
+                    w += total;
+                }
+            }
+        }
+        return;
+    }
+    for (unsigned int r = blockIdx.x * FL_WAVES + (threadIdx.x >> 6); r < src.rows; r += gridDim.x * FL_WAVES) {
+        const unsigned int o = fl_out_row(m, r);
+        if (o >= m.output_rows) continue;
+        const unsigned int rb = r / src.block_size;
+        unsigned int w = ptr[o];
+        for (unsigned int e0 = 0; e0 < src.ell_cols; e0 += 64u) {
+            const unsigned int e = e0 + lane;
+            S v = S(0);
+            const unsigned int oc = e < src.ell_cols ? fl_out_col<S>(src, val, m, width, rb, r, e, &v) : ~0u;
+            const unsigned long long bal = __ballot(oc != ~0u);
+            if (oc != ~0u) {
+                const unsigned int k = w + (unsigned int) __popcll(bal & lanemask_lt());
+                out_idx[k] = oc;
+                out_val[k] = v;
+            }
+            w += (unsigned int) __popcll(bal);
+        }
+    }
+}
+
+// Workspace-owned device memory (grow-only).
+struct fl_private {
+    void *ptr_block;     // counts [output_rows] | claim [output_rows] | dup | ptr [output_rows + 1] | scan scratch
+    size_t ptr_bytes;
+    void *val_block;     // minor idx [max_nnz] | values [max_nnz]
+    size_t val_bytes;
+    void *bucket_block;  // plan arrays, rebuilt ptr, offsets, sort / scan scratch
+    size_t bucket_bytes;
+    void *rebuilt_block; // rebuilt minor idx | values
+    size_t rebuilt_bytes;
+};
+
+int grow(void **p, size_t *have, size_t need) {
+    if (need <= *have && *p != nullptr) return 1;
+    if (*p != nullptr) (void) hipFree(*p);
+    *p = nullptr;
+    *have = 0u;
+    CSAMD_HIP(hipMalloc(p, need == 0u ? 256u : need), "hipMalloc filtered blocked ell workspace");
+    *have = need == 0u ? 256u : need;
+    return 1;
+}
+
+template<typename S> struct fl_launch {
+    static int run(const csamd_blocked_ell_view *src, const fl_masks &m, unsigned int *counts, unsigned int *claim, unsigned int *dup,
+                   const unsigned int *ptr, unsigned int *out_idx, void *out_val, bool emit, hipStream_t s) {
+        const bool grouped = ((unsigned long long) src->ell_cols * sizeof(S)) % 16ull == 0ull && ((uintptr_t) src->val & 15u) == 0u;
+        const unsigned int grid = grid_for(src->rows, grouped ? 1u : FL_WAVES, (unsigned int) device_cu_count() * 16u);
+        const unsigned int width = src->ell_cols / src->block_size;
+        const unsigned int cap = grouped && width <= FL_LDS_SLOTS ? width : 0u;  // slot table staged in LDS
+        const size_t lds = (size_t) cap * 4u;
+        if (!emit) {
+            fl_count_kernel<S, 4u><<<grid, FL_THREADS, lds, s>>>(*src, (const S *) src->val, m, grouped, cap, counts, claim, dup);
+            return hip_ok(hipGetLastError(), "fl_count_kernel");
+        }
+        fl_emit_kernel<S, 4u><<<grid, FL_THREADS, lds, s>>>(*src, (const S *) src->val, m, grouped, cap, ptr, out_idx, (S *) out_val);
+        return hip_ok(hipGetLastError(), "fl_emit_kernel");
+    }
+};
+
+template<template<typename> class K, typename... Args>
+int fl_dispatch(unsigned int vt, Args... args) {
+    switch (value_type_bytes(vt)) {
+        case 1: return K<uint8_t>::run(args...);
+        case 2: return K<uint16_t>::run(args...);
+        case 4: return K<float>::run(args...);
+        default: set_error("filter_blocked_ell_to_compressed: unsupported value type %u", vt); return 0;
+    }
+}
+
+int filter_impl(const csamd_blocked_ell_view *src, unsigned int value_type, unsigned int output_rows, unsigned int output_cols,
+                const unsigned char *d_keep_rows, const unsigned char *d_keep_cols, const unsigned int *d_row_remap,
+                const unsigned int *d_col_remap, csamd_filtered_blocked_ell_workspace *ws, csamd_filtered_blocked_ell_result *out) {
+    CSAMD_REQUIRE(src != nullptr && ws != nullptr, "filter_blocked_ell_to_compressed: null argument");
+    const int vb = value_type_bytes(value_type);
+    CSAMD_REQUIRE(vb == 1 || vb == 2 || vb == 4, "filter_blocked_ell_to_compressed: unsupported value type %u", value_type);
+    CSAMD_REQUIRE(src->rows == 0u || src->block_size != 0u, "filter_blocked_ell_to_compressed: block_size is 0");
+    CSAMD_REQUIRE(src->block_size == 0u || src->ell_cols % src->block_size == 0u,
+                  "filter_blocked_ell_to_compressed: ell_cols not a multiple of block_size");
+    CSAMD_HIP(hipSetDevice(ws->device >= 0 ? ws->device : 0), "hipSetDevice filtered blocked ell");
+    if (ws->private_workspace == nullptr) {
+        ws->private_workspace = std::calloc(1u, sizeof(fl_private));
+        CSAMD_REQUIRE(ws->private_workspace != nullptr, "filter_blocked_ell_to_compressed: host allocation failed");
+    }
+    fl_private *st = (fl_private *) ws->private_workspace;
+    hipStream_t s = (hipStream_t) ws->stream;
+    const unsigned int max_nnz = src->nnz;  // a kept entry is a nonzero stored value
+    const size_t off_claim = align_up((size_t) output_rows * 4u, 256u);
+    const size_t off_dup = off_claim + align_up((size_t) output_rows * 4u, 256u);
+    const size_t off_ptr = off_dup + 256u;
+    const size_t off_map = off_ptr + align_up(((size_t) output_rows + 1u) * 4u, 256u);
+    const size_t off_scan = off_map + align_up((size_t) src->cols * 4u, 256u);
+    if (!grow(&st->ptr_block, &st->ptr_bytes, off_scan + exclusive_scan_scratch_bytes(output_rows))) return 0;
+    const size_t off_val = align_up((size_t) max_nnz * 4u, 256u);
+    if (!grow(&st->val_block, &st->val_bytes, off_val + (size_t) max_nnz * (size_t) vb)) return 0;
+    char *pb = (char *) st->ptr_block;
+    unsigned int *counts = (unsigned int *) pb;
+    unsigned int *claim = (unsigned int *) (pb + off_claim);
+    unsigned int *dup = (unsigned int *) (pb + off_dup);
+    unsigned int *ptr = (unsigned int *) (pb + off_ptr);
+    unsigned int *out_idx = (unsigned int *) st->val_block;
+    void *out_val = (char *) st->val_block + off_val;
+
+    csamd_filtered_blocked_ell_result local;
+    std::memset(&local, 0, sizeof(local));
+    local.filtered.rows = output_rows;
+    local.filtered.cols = output_cols;
+    local.filtered.axis = CSAMD_COMPRESSED_BY_ROW;
+    local.filtered.majorPtr = ptr;
+    local.filtered.minorIdx = out_idx;
+    local.filtered.val = (csamd_half *) out_val;
+    local.stats.kept_rows = output_rows;  // as the reference reports it
+    local.stats.output_rows = output_rows;
+    local.stats.output_cols = output_cols;
+    local.stats.live_fill_ratio = 1.0;
+    unsigned int live = 0u;
+    if (output_rows != 0u && output_cols != 0u) {
+        CSAMD_REQUIRE(src->rows == 0u || (src->blockColIdx != nullptr && src->val != nullptr),
+                      "filter_blocked_ell_to_compressed: source arrays missing");
+        if (!zero_bytes(pb, off_ptr, s)) return 0;  // counts, claims, dup flag
+        unsigned int *col_map = (unsigned int *) (pb + off_map);
+        const fl_masks m{output_rows, output_cols, d_keep_rows, d_keep_cols, d_row_remap, d_col_remap, col_map};
+        if (src->cols != 0u) {
+            fl_col_map_kernel<<<grid_for(src->cols, 256u, 1024u), 256, 0, s>>>(src->cols, m, col_map);
+            CSAMD_HIP(hipGetLastError(), "fl_col_map_kernel");
+        }
+        if (src->rows != 0u && src->ell_cols != 0u &&
+            !fl_dispatch<fl_launch>(value_type, src, m, counts, claim, dup, (const unsigned int *) nullptr, out_idx, out_val, false, s))
+            return 0;
+        if (!exclusive_scan_u32(counts, output_rows, ptr, nullptr, pb + off_scan, exclusive_scan_scratch_bytes(output_rows), s)) return 0;
+        unsigned int h[2] = {0u, 0u};
+        CSAMD_HIP(hipMemcpyAsync(&h[0], ptr + output_rows, 4u, hipMemcpyDeviceToHost, s), "D2H filtered nnz");
+        CSAMD_HIP(hipMemcpyAsync(&h[1], dup, 4u, hipMemcpyDeviceToHost, s), "D2H filtered dup");
+        CSAMD_HIP(hipStreamSynchronize(s), "filtered blocked ell sync");
+        CSAMD_REQUIRE(h[1] == 0u, "filter_blocked_ell_to_compressed: row_remap maps two kept rows to output row %u", h[1] - 1u);
+        live = h[0];
+        CSAMD_REQUIRE(live <= max_nnz, "filter_blocked_ell_to_compressed: %u surviving entries exceed the source nnz %u", live, max_nnz);
+        if (src->rows != 0u && src->ell_cols != 0u &&
+            !fl_dispatch<fl_launch>(value_type, src, m, counts, claim, dup, (const unsigned int *) ptr, out_idx, out_val, true, s))
+            return 0;
+    } else if (output_rows != 0u) {
+        if (!zero_bytes(ptr, ((size_t) output_rows + 1u) * 4u, s)) return 0;
+    } else {
+        if (!zero_bytes(ptr, 4u, s)) return 0;
+    }
+    local.filtered.nnz = live;
+    local.stats.live_nnz = live;
+    if (output_rows != 0u && output_cols != 0u) {
+        const unsigned long long slots = (unsigned long long) output_rows * src->ell_cols;
+        local.stats.kept_row_slots = slots;
+        local.stats.dead_slots = slots > live ? slots - live : 0ull;
+        local.stats.live_fill_ratio = slots == 0ull ? 1.0 : (double) live / (double) slots;
+        local.stats.dead_value_bytes = (size_t) local.stats.dead_slots * (size_t) vb;
+    }
+    if (out != nullptr) *out = local;
+    return 1;
+}
+
+} // namespace
+} // namespace csamd
+
+using namespace csamd;
+
+extern "C" {
+
+void csamd_filtered_blocked_ell_init(csamd_filtered_blocked_ell_workspace *ws) {
+    if (ws == nullptr) return;
+    std::memset(ws, 0, sizeof(*ws));
+    ws->device = -1;
+}
+
+void csamd_filtered_blocked_ell_clear(csamd_filtered_blocked_ell_workspace *ws) {
+    if (ws == nullptr) return;
+    if (ws->device >= 0) (void) hipSetDevice(ws->device);
+    if (ws->private_workspace != nullptr) {
+        fl_private *st = (fl_private *) ws->private_workspace;
+        void *blocks[4] = {st->ptr_block, st->val_block, st->bucket_block, st->rebuilt_block};
+        for (void *b : blocks)
+            if (b != nullptr) (void) hipFree(b);
+        std::free(st);
+    }
+    if (ws->owns_stream && ws->stream != nullptr) (void) hipStreamDestroy((hipStream_t) ws->stream);
+    csamd_filtered_blocked_ell_init(ws);
+}
+
+int csamd_filtered_blocked_ell_setup(csamd_filtered_blocked_ell_workspace *ws, int device, csamd_stream_t stream) {
+    CSAMD_REQUIRE(ws != nullptr, "filtered blocked ell setup: null workspace");
+    csamd_filtered_blocked_ell_clear(ws);
+    CSAMD_HIP(hipSetDevice(device), "hipSetDevice filtered blocked ell setup");
+    ws->device = device;
+    if (stream == nullptr) {
+        hipStream_t s = nullptr;
+        CSAMD_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreateWithFlags filtered blocked ell");
+        ws->stream = (csamd_stream_t) s;
+        ws->owns_stream = 1;
+    } else {
+        ws->stream = stream;
+        ws->owns_stream = 0;
+    }
+    return 1;
+}
+
+int csamd_filter_blocked_ell_to_compressed(const csamd_blocked_ell_view *src, unsigned int value_type, unsigned int output_rows,
+                                           unsigned int output_cols, const unsigned char *d_keep_rows,
+                                           const unsigned char *d_keep_cols, const unsigned int *d_row_remap,
+                                           const unsigned int *d_col_remap, csamd_filtered_blocked_ell_workspace *ws,
+                                           csamd_filtered_blocked_ell_result *out) {
+    return filter_impl(src, value_type, output_rows, output_cols, d_keep_rows, d_keep_cols, d_row_remap, d_col_remap, ws, out);
+}
+
+int csamd_build_bucketed_filtered_blocked_ell_major_view(const csamd_blocked_ell_view *src, unsigned int value_type,
+                                                         unsigned int output_rows, unsigned int output_cols,
+                                                         const unsigned char *d_keep_rows, const unsigned char *d_keep_cols,
+                                                         const unsigned int *d_row_remap, const unsigned int *d_col_remap,
+                                                         unsigned int requested_bucket_count,
+                                                         csamd_filtered_blocked_ell_workspace *ws,
+                                                         csamd_filtered_blocked_ell_result *out) {
+    csamd_filtered_blocked_ell_result local;
+    if (!filter_impl(src, value_type, output_rows, output_cols, d_keep_rows, d_keep_cols, d_row_remap, d_col_remap, ws, &local))
+        return 0;
+    // bucket::build_bucketed_major_view: plan (stable ascending nnz order,
+    // equal-count offsets) and rebuild into workspace buffers
+    fl_private *st = (fl_private *) ws->private_workspace;
+    hipStream_t s = (hipStream_t) ws->stream;
+    const unsigned int n = output_rows;
+    const unsigned int bc = csamd_clamp_bucket_count(n, requested_bucket_count);
+    size_t sort_b = 0u, scan_b = 0u;
+    if (!csamd_major_nnz_bucket_sort_scratch_bytes(n, &sort_b) || !csamd_major_nnz_bucket_scan_scratch_bytes(n, &scan_b)) return 0;
+    const size_t a = align_up((size_t) n * 4u, 256u);
+    const size_t off_sorted = a, off_in = 2u * a, off_out = 3u * a, off_inv = 4u * a, off_ptr = 5u * a;
+    const size_t off_offsets = off_ptr + align_up(((size_t) n + 1u) * 4u, 256u);
+    const size_t off_sort = off_offsets + align_up(((size_t) bc + 1u) * 4u, 256u);
+    const size_t off_scan = off_sort + align_up(sort_b, 256u);
+    if (!grow(&st->bucket_block, &st->bucket_bytes, off_scan + scan_b)) return 0;
+    const int vb = value_type_bytes(value_type);
+    const size_t nnz = local.filtered.nnz;
+    const size_t off_rval = align_up(nnz * 4u, 256u);
+    if (!grow(&st->rebuilt_block, &st->rebuilt_bytes, off_rval + nnz * (size_t) vb)) return 0;
+    char *b = (char *) st->bucket_block;
+    unsigned int *major_nnz = (unsigned int *) b;
+    unsigned int *sorted = (unsigned int *) (b + off_sorted);
+    unsigned int *order_in = (unsigned int *) (b + off_in);
+    unsigned int *order_out = (unsigned int *) (b + off_out);
+    unsigned int *inverse = (unsigned int *) (b + off_inv);
+    unsigned int *rptr = (unsigned int *) (b + off_ptr);
+    unsigned int *offsets = (unsigned int *) (b + off_offsets);
+    unsigned int *ridx = (unsigned int *) st->rebuilt_block;
+    void *rval = (char *) st->rebuilt_block + off_rval;
+    if (!csamd_build_major_nnz_bucket_plan_raw(local.filtered.majorPtr, n, major_nnz, sorted, order_in, order_out, offsets, bc,
+                                               b + off_sort, sort_b, (csamd_stream_t) s))
+        return 0;
+    if (!csamd_rebuild_compressed_major_order_raw(local.filtered.majorPtr, local.filtered.minorIdx, local.filtered.val, value_type, n,
+                                                  order_out, rptr, ridx, rval, inverse, b + off_scan, scan_b, (csamd_stream_t) s))
+        return 0;
+    if (n == 0u && !zero_bytes(rptr, 4u, s)) return 0;
+    local.bucketed.rebuilt = local.filtered;
+    local.bucketed.rebuilt.majorPtr = rptr;
+    local.bucketed.rebuilt.minorIdx = ridx;
+    local.bucketed.rebuilt.val = (csamd_half *) rval;
+    local.bucketed.plan.major_dim = n;
+    local.bucketed.plan.bucket_count = bc;
+    local.bucketed.plan.major_order = order_out;
+    local.bucketed.plan.major_nnz_sorted = sorted;
+    local.bucketed.plan.bucket_offsets = offsets;
+    local.bucketed.plan.inverse_major_order = inverse;
+    local.has_bucketed = 1;
+    if (out != nullptr) *out = local;
+    return 1;
+}
+
+} // extern "C"

@@ -88,6 +88,53 @@ def blocked_ell_from_compressed(csr: DeviceCSR, block_size: int, ws: ConvertWork
     return out
 
 
+class FilteredBlockedEllWorkspace:
+    """filtered_blocked_ell_workspace: the filtered CSR (and its bucketed
+    rebuild) alias this workspace's memory until the next call."""
+
+    def __init__(self, device: int = 0, stream: int | None = None):
+        self.ws = A.filtered_blocked_ell_workspace()
+        lib = load()
+        lib.csamd_filtered_blocked_ell_init(C.byref(self.ws))
+        check(lib.csamd_filtered_blocked_ell_setup(C.byref(self.ws), device,
+                                                   stream if stream is not None else current_stream_handle()),
+              "filtered_blocked_ell_setup")
+
+    def close(self):
+        if self.ws is not None:
+            load().csamd_filtered_blocked_ell_clear(C.byref(self.ws))
+            self.ws = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def filter_blocked_ell_to_compressed(bell: DeviceBlockedELL, output_rows: int, output_cols: int, keep_rows=None, keep_cols=None,
+                                     row_remap=None, col_remap=None, ws: FilteredBlockedEllWorkspace | None = None,
+                                     bucket_count: int | None = None) -> A.filtered_blocked_ell_result:
+    """Masked Blocked-ELL -> CSR with row / column remaps
+    (convert/filtered_blocked_ell_to_compressed.cuh:264-373); with
+    bucket_count, also the major-nnz bucket plan and rebuild (:375-412).
+    keep_* are device uint8 tensors, *_remap device uint32/int32 tensors (or
+    None).  The result's views point into `ws`."""
+    lib = load()
+    if ws is None:
+        raise ValueError("filter_blocked_ell_to_compressed: pass a FilteredBlockedEllWorkspace (the result aliases it)")
+    out = A.filtered_blocked_ell_result()
+    v = bell.view()
+    args = [C.byref(v), bell.value_type, output_rows, output_cols] + [ptr(t) if t is not None else None
+                                                                        for t in (keep_rows, keep_cols, row_remap, col_remap)]
+    if bucket_count is None:
+        check(lib.csamd_filter_blocked_ell_to_compressed(*args, C.byref(ws.ws), C.byref(out)), "filter_blocked_ell_to_compressed")
+    else:
+        check(lib.csamd_build_bucketed_filtered_blocked_ell_major_view(*args, bucket_count, C.byref(ws.ws), C.byref(out)),
+              "build_bucketed_filtered_blocked_ell_major_view")
+    return out
+
+
 # ---------------------------------------------------------------------------
 # Sliced-ELL
 # ---------------------------------------------------------------------------

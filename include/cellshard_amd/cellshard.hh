@@ -441,6 +441,36 @@ static inline int build_transpose_cs_from_cs_raw(types::dim_t cDim, types::dim_t
     return build_compressed_transpose_raw(cDim, uDim, nnz, a, b, c, d, e, f, g, h, i, s);
 }
 
+// Masked Blocked-ELL -> CSR (convert/filtered_blocked_ell_to_compressed.cuh:125-415).
+using filtered_blocked_ell_stats = csamd_filtered_blocked_ell_stats;
+using filtered_blocked_ell_result = csamd_filtered_blocked_ell_result;
+using filtered_blocked_ell_workspace = csamd_filtered_blocked_ell_workspace;
+
+inline void init(filtered_blocked_ell_workspace *ws) { csamd_filtered_blocked_ell_init(ws); }
+inline void clear(filtered_blocked_ell_workspace *ws) { csamd_filtered_blocked_ell_clear(ws); }
+inline int setup(filtered_blocked_ell_workspace *ws, int device, hipStream_t stream = (hipStream_t) 0) {
+    return csamd_filtered_blocked_ell_setup(ws, device, (csamd_stream_t) stream);
+}
+inline int filter_blocked_ell_to_compressed(const device::blocked_ell_view *src, types::dim_t output_rows,
+                                                     types::dim_t output_cols, const unsigned char *d_keep_rows,
+                                                     const unsigned char *d_keep_cols, const types::idx_t *d_row_remap,
+                                                     const types::idx_t *d_col_remap, filtered_blocked_ell_workspace *ws,
+                                                     filtered_blocked_ell_result *out) {
+    return csamd_filter_blocked_ell_to_compressed(src, CSAMD_VALUE_F16, output_rows, output_cols, d_keep_rows, d_keep_cols,
+                                                  d_row_remap, d_col_remap, ws, out);
+}
+inline int build_bucketed_filtered_blocked_ell_major_view(const device::blocked_ell_view *src, types::dim_t output_rows,
+                                                                   types::dim_t output_cols, const unsigned char *d_keep_rows,
+                                                                   const unsigned char *d_keep_cols, const types::idx_t *d_row_remap,
+                                                                   const types::idx_t *d_col_remap,
+                                                                   types::idx_t requested_bucket_count,
+                                                                   filtered_blocked_ell_workspace *ws,
+                                                                   filtered_blocked_ell_result *out) {
+    return csamd_build_bucketed_filtered_blocked_ell_major_view(src, CSAMD_VALUE_F16, output_rows, output_cols, d_keep_rows,
+                                                                d_keep_cols, d_row_remap, d_col_remap, requested_bucket_count, ws,
+                                                                out);
+}
+
 } // namespace convert
 
 // ---------------------------------------------------------------------------

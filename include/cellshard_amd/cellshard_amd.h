@@ -768,6 +768,66 @@ int csamd_blocked_ell_bucket_plan_raw(const unsigned int *d_block_col_idx, unsig
                                       unsigned int *d_width_sorted, unsigned long long *d_keys_sorted,
                                       csamd_convert_workspace *ws);
 
+/* Masked Blocked-ELL -> CSR compaction with row / column remaps, and the same
+ * followed by the major-nnz bucket plan and rebuild (the step after the
+ * reduce: QC keep -> compact -> rebucket).
+ * convert/filtered_blocked_ell_to_compressed.cuh:125-415; bucketed view:
+ * bucket/operators/major_nnz.cuh:40-43,189-210.  An entry survives when its
+ * row is kept and maps below output_rows, its block column is valid, col <
+ * cols, its value is nonzero, its column is kept and maps below output_cols;
+ * output rows keep source ell order.  Row i of the output is [ptr[i],
+ * ptr[i + 1]) (the reference's scan shifts rows by one place; see the
+ * integration notes).  Outputs alias workspace memory until the next call.
+ * Two kept rows mapped to one output row fail the call. */
+typedef struct CSAMD_ALIGN16 csamd_filtered_blocked_ell_stats {
+    unsigned int kept_rows;          /* = output_rows, as the reference reports it (:362) */
+    unsigned int output_rows;
+    unsigned int output_cols;
+    unsigned int live_nnz;
+    unsigned long long kept_row_slots;  /* kept_rows x ell_cols */
+    unsigned long long dead_slots;
+    double live_fill_ratio;
+    size_t dead_value_bytes;
+} csamd_filtered_blocked_ell_stats;
+
+typedef struct CSAMD_ALIGN16 csamd_compressed_major_bucket_result {
+    csamd_compressed_view rebuilt;
+    csamd_major_nnz_bucket_plan_view plan;
+} csamd_compressed_major_bucket_result;
+
+typedef struct CSAMD_ALIGN16 csamd_filtered_blocked_ell_result {
+    csamd_compressed_view filtered;
+    csamd_compressed_major_bucket_result bucketed;
+    csamd_filtered_blocked_ell_stats stats;
+    int has_bucketed;
+} csamd_filtered_blocked_ell_result;
+
+typedef struct CSAMD_ALIGN16 csamd_filtered_blocked_ell_workspace {
+    int device;
+    csamd_stream_t stream;
+    int owns_stream;
+    void *private_workspace;
+} csamd_filtered_blocked_ell_workspace;
+
+/* :143-178 init / clear / setup (a null stream creates an owned non-blocking one) */
+void csamd_filtered_blocked_ell_init(csamd_filtered_blocked_ell_workspace *ws);
+void csamd_filtered_blocked_ell_clear(csamd_filtered_blocked_ell_workspace *ws);
+int csamd_filtered_blocked_ell_setup(csamd_filtered_blocked_ell_workspace *ws, int device, csamd_stream_t stream);
+/* :264-373 filter_blocked_ell_to_compressed (value_type: F16 as in the reference; F32 / BF16 / FP8 too) */
+int csamd_filter_blocked_ell_to_compressed(const csamd_blocked_ell_view *src, unsigned int value_type, unsigned int output_rows,
+                                           unsigned int output_cols, const unsigned char *d_keep_rows,
+                                           const unsigned char *d_keep_cols, const unsigned int *d_row_remap,
+                                           const unsigned int *d_col_remap, csamd_filtered_blocked_ell_workspace *ws,
+                                           csamd_filtered_blocked_ell_result *out);
+/* :375-412 build_bucketed_filtered_blocked_ell_major_view */
+int csamd_build_bucketed_filtered_blocked_ell_major_view(const csamd_blocked_ell_view *src, unsigned int value_type,
+                                                         unsigned int output_rows, unsigned int output_cols,
+                                                         const unsigned char *d_keep_rows, const unsigned char *d_keep_cols,
+                                                         const unsigned int *d_row_remap, const unsigned int *d_col_remap,
+                                                         unsigned int requested_bucket_count,
+                                                         csamd_filtered_blocked_ell_workspace *ws,
+                                                         csamd_filtered_blocked_ell_result *out);
+
 /* ---------------------------------------------------------------------------
  * Host matrix structs (reference layouts) and the host-struct converters.
  * formats/compressed.cuh:24-37, formats/blocked_ell.cuh:15-26,

@@ -357,6 +357,63 @@ def reoptimize(rows, cols, bs, ell_cols, bci, val16, row_keep, feature_keep, row
     return mp, mi[: nnz.value], mv[: nnz.value], nnz.value
 
 
+def _stored_nonzero(bits, vt):
+    bits = np.asarray(bits)
+    if vt == VT_F32:
+        return bits.view(np.float32) != 0.0
+    if vt == VT_FP8:
+        return (bits.view(np.uint8) & 0x7F) != 0
+    return (bits.view(np.uint16) & 0x7FFF) != 0
+
+
+def filter_blocked_ell(rows, cols, bs, ell_cols, bci, val_bits, vt, output_rows, output_cols, keep_rows=None, keep_cols=None,
+                       row_remap=None, col_remap=None):
+    """Masked Blocked-ELL -> CSR, restated from
+    src/convert/filtered_blocked_ell_to_compressed.cuh:29-123 (which entries
+    survive, in ell order) and :264-373 (stats).  Row pointers are the plain
+    exclusive scan of the per-row counts: row i = [ptr[i], ptr[i + 1]).
+    Returns (ptr, idx, val_bits, stats dict)."""
+    W = ell_cols // bs if bs else 0
+    bci = np.asarray(bci, np.uint32).astype(np.int64)
+    vals = np.asarray(val_bits).reshape(rows, ell_cols) if rows * ell_cols else np.zeros((rows, 0), np.asarray(val_bits).dtype)
+    per = [(np.zeros(0, np.uint32), vals[:0, 0] if ell_cols else np.zeros(0, vals.dtype))] * output_rows
+    claimed = set()
+    live = 0
+    if output_rows and output_cols:
+        e = np.arange(ell_cols)
+        slot, lane = (e // bs, e % bs) if bs else (e, e)
+        for r in range(rows):
+            if keep_rows is not None and keep_rows[r] == 0:
+                continue
+            o = int(row_remap[r]) if row_remap is not None else r
+            if o >= output_rows:
+                continue
+            if o in claimed:
+                raise ValueError(f"two kept rows map to output row {o}")
+            claimed.add(o)
+            bc = bci[(r // bs) * W + slot]
+            valid = bc != INVALID
+            col = np.where(valid, bc * bs + lane, cols)
+            m = valid & (col < cols) & _stored_nonzero(vals[r], vt)
+            cs = np.minimum(col, max(cols - 1, 0))
+            if keep_cols is not None:
+                m &= np.asarray(keep_cols)[cs] != 0
+            oc = np.asarray(col_remap, np.int64)[cs] if col_remap is not None else col
+            m &= oc < output_cols
+            per[o] = (oc[m].astype(np.uint32), vals[r][m])
+            live += int(m.sum())
+    counts = np.array([len(a) for a, _ in per], np.int64)
+    ptr = np.zeros(output_rows + 1, np.uint32)
+    ptr[1:] = np.cumsum(counts)
+    idx = np.concatenate([a for a, _ in per]) if per else np.zeros(0, np.uint32)
+    val = np.concatenate([v for _, v in per]) if per else np.zeros(0, vals.dtype)
+    slots = output_rows * ell_cols if output_rows and output_cols else 0
+    stats = {"kept_rows": output_rows, "output_rows": output_rows, "output_cols": output_cols, "live_nnz": live,
+             "kept_row_slots": slots, "dead_slots": max(slots - live, 0),
+             "live_fill_ratio": 1.0 if slots == 0 else live / slots}
+    return ptr, idx.astype(np.uint32), val, stats
+
+
 # ---------------------------------------------------------------------------
 # Reference CPU converter (oracle/_ref), fp16 values.
 def ref_choose_blocked_ell_block_size(rows, cols, ptr, idx, val16, candidates):

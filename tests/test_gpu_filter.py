@@ -1,0 +1,128 @@
+"""Masked Blocked-ELL -> CSR (filter_blocked_ell_to_compressed) and its
+bucketed variant on the GPU against the oracle restatement."""
+import numpy as np
+import pytest
+import torch
+
+from cellshard_amd import CellShardError
+from cellshard_amd.device import DeviceCSR
+from cellshard_amd.ops import (ConvertWorkspace, FilteredBlockedEllWorkspace, blocked_ell_from_compressed, device_to_numpy,
+                               filter_blocked_ell_to_compressed)
+from helpers import F16, F32, FP8, encode, random_csr
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+VB = {F16: 2, F32: 4, FP8: 1}
+NP_BITS = {F16: np.uint16, F32: np.uint32, FP8: np.uint8}
+
+
+@pytest.fixture(scope="module")
+def ws():
+    return ConvertWorkspace()
+
+
+@pytest.fixture(scope="module")
+def fws():
+    w = FilteredBlockedEllWorkspace()
+    yield w
+    w.close()
+
+
+def dev_u8(a):
+    return torch.as_tensor(np.asarray(a, np.uint8), device="cuda")
+
+
+def dev_u32(a):
+    return torch.as_tensor(np.asarray(a, np.uint32).view(np.int32), device="cuda")
+
+
+def make_bell(rows, cols, bs, vt, seed, ws, zeros=0.1):
+    ptr, idx, val = random_csr(rows, cols, 0.05, seed=seed, zeros=zeros, empty_rows=0.05, fp8_safe=(vt == FP8))
+    bits = encode(val, vt)
+    bell = blocked_ell_from_compressed(DeviceCSR.from_numpy(rows, cols, ptr, idx, bits, vt), bs, ws)
+    gb, gv = bell.to_numpy()
+    return bell, gb, np.ascontiguousarray(gv).view(NP_BITS[vt])
+
+
+def fetch_csr(view, vt):
+    torch.cuda.synchronize()
+    p = device_to_numpy(view.majorPtr, view.rows + 1, np.uint32)
+    n = int(view.nnz)
+    return p, device_to_numpy(view.minorIdx, n, np.uint32), device_to_numpy(view.val, n, NP_BITS[vt])
+
+
+def masks(rows, cols, seed, drop_rows=3, drop_cols=2):
+    rng = np.random.default_rng(seed)
+    rk = (rng.random(rows) < 0.8).astype(np.uint8)
+    fk = (rng.random(cols) < 0.6).astype(np.uint8)
+    rr = np.full(rows, 0xFFFFFFFF, np.uint32)
+    rr[rk != 0] = rng.permutation(int(rk.sum()))
+    cr = np.full(cols, 0xFFFFFFFF, np.uint32)
+    cr[fk != 0] = np.arange(int(fk.sum()))
+    return rk, fk, rr, cr, int(rk.sum()) - drop_rows, int(fk.sum()) - drop_cols
+
+
+@pytest.mark.parametrize("vt", [F16, F32, FP8])
+@pytest.mark.parametrize("bs", [4, 16])
+def test_filter_matches_oracle(ws, fws, vt, bs):
+    rows, cols = 701, 1300
+    bell, gb, gv = make_bell(rows, cols, bs, vt, 30 + bs + vt, ws)
+    rk, fk, rr, cr, orows, ocols = masks(rows, cols, bs + vt)
+    out = filter_blocked_ell_to_compressed(bell, orows, ocols, dev_u8(rk), dev_u8(fk), dev_u32(rr), dev_u32(cr), ws=fws)
+    p, i, v = fetch_csr(out.filtered, vt)
+    rp, ri, rv, st = O.filter_blocked_ell(rows, cols, bs, bell.ell_cols, gb, gv, vt, orows, ocols, rk, fk, rr, cr)
+    assert np.array_equal(p, rp) and np.array_equal(i, ri) and np.array_equal(v, rv.view(NP_BITS[vt]))
+    s = out.stats
+    assert (s.kept_rows, s.output_rows, s.output_cols, s.live_nnz) == (orows, orows, ocols, st["live_nnz"])
+    assert (s.kept_row_slots, s.dead_slots) == (st["kept_row_slots"], st["dead_slots"])
+    assert abs(s.live_fill_ratio - st["live_fill_ratio"]) < 1e-12 and s.dead_value_bytes == st["dead_slots"] * VB[vt]
+    assert out.has_bucketed == 0
+
+
+def test_filter_identity_and_empty_outputs(ws, fws):
+    rows, cols, bs = 300, 500, 8
+    bell, gb, gv = make_bell(rows, cols, bs, F16, 77, ws, zeros=0.3)
+    out = filter_blocked_ell_to_compressed(bell, rows, cols, ws=fws)  # no masks, no remaps
+    p, i, v = fetch_csr(out.filtered, F16)
+    rp, ri, rv, st = O.filter_blocked_ell(rows, cols, bs, bell.ell_cols, gb, gv, F16, rows, cols)
+    assert np.array_equal(p, rp) and np.array_equal(i, ri) and np.array_equal(v, rv)
+    assert int(out.filtered.nnz) == bell.nnz  # explicit zeros dropped, every nonzero kept
+    for orows, ocols in ((0, cols), (rows, 0)):
+        out = filter_blocked_ell_to_compressed(bell, orows, ocols, ws=fws)
+        assert int(out.filtered.nnz) == 0 and out.stats.live_fill_ratio == 1.0
+        if orows:
+            assert np.all(fetch_csr(out.filtered, F16)[0] == 0)
+
+
+def test_filter_rejects_two_rows_on_one_output_row(ws, fws):
+    rows, cols, bs = 40, 60, 4
+    bell, gb, gv = make_bell(rows, cols, bs, F16, 5, ws)
+    rr = np.arange(rows, dtype=np.uint32)
+    rr[7] = 3
+    with pytest.raises(CellShardError, match="output row 3"):
+        filter_blocked_ell_to_compressed(bell, rows, cols, row_remap=dev_u32(rr), ws=fws)
+
+
+@pytest.mark.parametrize("buckets", [1, 5, 64])
+def test_bucketed_filtered_view(ws, fws, buckets):
+    rows, cols, bs = 512, 900, 16
+    bell, gb, gv = make_bell(rows, cols, bs, F16, 91, ws)
+    rk, fk, rr, cr, orows, ocols = masks(rows, cols, 3, drop_rows=0, drop_cols=0)
+    out = filter_blocked_ell_to_compressed(bell, orows, ocols, dev_u8(rk), dev_u8(fk), dev_u32(rr), dev_u32(cr), ws=fws,
+                                           bucket_count=buckets)
+    assert out.has_bucketed == 1
+    rp, ri, rv, st = O.filter_blocked_ell(rows, cols, bs, bell.ell_cols, gb, gv, F16, orows, ocols, rk, fk, rr, cr)
+    ref = O.bucket_plan(rp, orows, buckets)
+    pl = out.bucketed.plan
+    torch.cuda.synchronize()
+    assert (pl.major_dim, pl.bucket_count) == (orows, ref["bucket_count"])
+    order = device_to_numpy(pl.major_order, orows, np.uint32)
+    assert np.array_equal(order, ref["major_order"])
+    assert np.array_equal(device_to_numpy(pl.bucket_offsets, pl.bucket_count + 1, np.uint32), ref["bucket_offsets"])
+    assert np.array_equal(device_to_numpy(pl.major_nnz_sorted, orows, np.uint32), ref["major_nnz_sorted"])
+    inv = device_to_numpy(pl.inverse_major_order, orows, np.uint32)
+    assert np.array_equal(inv[order], np.arange(orows))
+    bp, bi, bv = fetch_csr(out.bucketed.rebuilt, F16)
+    for k, r in enumerate(order):  # rebuilt row k = filtered row order[k]
+        assert np.array_equal(bi[bp[k]:bp[k + 1]], ri[rp[r]:rp[r + 1]])
+        assert np.array_equal(bv[bp[k]:bp[k + 1]], rv[rp[r]:rp[r + 1]])

@@ -364,6 +364,64 @@ def test_reoptimize_export_fixture():
     assert list(mv.view(np.float16).astype(float)) == e["expected_csr"]["data"]
 
 
+def test_filter_blocked_ell_reference_fixtures(mg):
+    # the reference test's masked reoptimize case (same keep / remap rules)
+    # and its Blocked-ELL -> CSR export case (all kept, identity maps)
+    b = mg["blocked_ell"]
+    r = mg["reoptimize"]
+    cfg = r["config"]
+    vals = f16_bits(b["values"]).view(np.uint16)
+    fp, fi, fv, st = O.filter_blocked_ell(mg["rows"], mg["cols"], b["block_size"], b["ell_cols"], b["block_col_idx"], vals, O.VT_F16,
+                                          cfg["output_rows"], cfg["output_cols"], mg["row_keep"], mg["feature_keep"], r["row_remap"],
+                                          r["feature_remap"])
+    assert st["live_nnz"] == r["expected"]["live_nnz"] and st["kept_rows"] == cfg["output_rows"]
+    mp, mi, mv, nnz = O.reoptimize(mg["rows"], mg["cols"], b["block_size"], b["ell_cols"], b["block_col_idx"], vals, mg["row_keep"],
+                                   mg["feature_keep"], r["row_remap"], r["feature_remap"], cfg["output_rows"], cfg["output_cols"])
+    assert np.array_equal(fp, mp) and np.array_equal(fi, mi) and np.array_equal(fv, mv)
+    e = json.loads((GOLD / "export_bell_fixture.json").read_text())
+    fp, fi, fv, st = O.filter_blocked_ell(e["rows"], e["cols"], e["block_size"], e["ell_cols"], e["block_col_idx"],
+                                          f16_bits(e["values"]).view(np.uint16), O.VT_F16, e["rows"], e["cols"])
+    assert list(fp) == e["expected_csr"]["indptr"] and list(fi) == e["expected_csr"]["indices"]
+    assert list(fv.view(np.float16).astype(float)) == e["expected_csr"]["data"]
+
+
+@pytest.mark.parametrize("vt", [F16, F32, FP8])
+def test_filter_blocked_ell_identity_is_csr_without_zeros(vt):
+    # keep everything, identity maps: Blocked-ELL -> CSR returns the source
+    # CSR minus its explicit zeros (any block size)
+    rows, cols = 61, 83
+    ptr, idx, val = random_csr(rows, cols, 0.1, seed=9, zeros=0.2, fp8_safe=(vt == FP8))
+    bits = encode(val, vt)
+    keep = val != 0
+    for bs in (1, 4, 8):
+        bci, v, ell, nnz = O.blocked_ell_from_compressed(rows, cols, ptr, idx, bits, bs)
+        fp, fi, fv, st = O.filter_blocked_ell(rows, cols, bs, ell, bci, v, vt, rows, cols)
+        cnt = np.array([keep[ptr[r]:ptr[r + 1]].sum() for r in range(rows)])
+        assert np.array_equal(fp[1:], np.cumsum(cnt)) and np.array_equal(fi, np.asarray(idx)[keep])
+        assert np.array_equal(np.ascontiguousarray(fv).view(np.uint8), np.ascontiguousarray(bits[keep]).view(np.uint8))
+        assert st["live_nnz"] == int(keep.sum()) == nnz
+
+
+def test_filter_blocked_ell_matches_reoptimize_restatement():
+    # two restatements of one rule (numpy here, C in oracle.c) on random
+    # masks and compacting remaps
+    rng = np.random.default_rng(4)
+    rows, cols, bs = 97, 150, 4
+    ptr, idx, val = random_csr(rows, cols, 0.08, seed=12, zeros=0.1)
+    bits = f16_bits(val)
+    bci, v, ell, nnz = O.blocked_ell_from_compressed(rows, cols, ptr, idx, bits, bs)
+    rk = (rng.random(rows) < 0.7).astype(np.uint8)
+    fk = (rng.random(cols) < 0.5).astype(np.uint8)
+    rr = np.full(rows, 0xFFFFFFFF, np.uint32)
+    rr[rk != 0] = np.arange(int(rk.sum()))
+    cr = np.full(cols, 0xFFFFFFFF, np.uint32)
+    cr[fk != 0] = rng.permutation(int(fk.sum()))
+    out_rows, out_cols = int(rk.sum()) - 3, int(fk.sum()) - 2  # some kept rows / columns map past the end
+    fp, fi, fv, st = O.filter_blocked_ell(rows, cols, bs, ell, bci, v, O.VT_F16, out_rows, out_cols, rk, fk, rr, cr)
+    mp, mi, mv, n2 = O.reoptimize(rows, cols, bs, ell, bci, v, rk, fk, rr, cr, out_rows, out_cols)
+    assert np.array_equal(fp, mp) and np.array_equal(fi, mi) and np.array_equal(fv, mv) and st["live_nnz"] == n2
+
+
 def test_planted_mistake_is_caught():
     """A dropped term in the oracle (simulated by zeroing one entry) must fail
     the dense comparison used above."""
