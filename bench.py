@@ -312,8 +312,9 @@ def run_plain_path(args, log):
 
     from cellshard_amd import abi as A
     from cellshard_amd.device import to_dev_u32, to_dev_u8
-    from cellshard_amd.ops import (ConvertWorkspace, FilteredBlockedEllWorkspace, LabelReduce, TransposeScratch,
-                                   blocked_ell_from_compressed, compressed_transpose, filter_blocked_ell_to_compressed, synth_csr)
+    from cellshard_amd.ops import (ConvertWorkspace, FilteredBlockedEllWorkspace, LabelReduce, RepackWorkspace, TransposeScratch,
+                                   blocked_ell_from_compressed, build_repacked_shard_blocked_ell, compressed_transpose,
+                                   filter_blocked_ell_to_compressed, synth_csr)
     from oracle import oracle as O
 
     dev = torch.device("cuda", 0)
@@ -350,6 +351,11 @@ def run_plain_path(args, log):
     f_args = (bell, int(rk.sum()), int(fk.sum()), to_dev_u8(rk, dev), to_dev_u8(fk, dev), to_dev_u32(rr, dev), to_dev_u32(cr, dev))
     fo, filt_ms = gpu_ms(lambda: filter_blocked_ell_to_compressed(*f_args, ws=fws))
     filt_live = int(fo.filtered.nnz)
+    # the same shard repacked: masked CSR -> 12 equal-count nnz buckets -> Blocked-ELL 16
+    rws = RepackWorkspace(stream=torch.cuda.current_stream().cuda_stream)
+    ro, repack_ms = gpu_ms(lambda: build_repacked_shard_blocked_ell([bell], *f_args[1:3], 16, 12, *f_args[3:], ws=rws))
+    repack_ell_width = int(ro.stats.ell_width)
+    rws.close()
     # algorithmic bytes: the kept value rows and the slot table read once, the CSR written
     # (the count and emit passes each read the kept rows)
     filt_bytes = int(rk.sum()) * bell.ell_cols * 4 + bell.row_blocks * (bell.ell_cols // 16) * 4 + filt_live * 8 + (int(rk.sum()) + 1) * 4
@@ -370,7 +376,7 @@ def run_plain_path(args, log):
            "gpu": {"blocked_ell_16_ms": round(bell_ms, 3), "blocked_ell_16_gb_s": round((csr_bytes + bell_bytes) / (bell_ms * 1e-3) / 1e9, 1),
                    "blocked_ell_bytes_over_csr": round(bell_bytes / csr_bytes, 2),
                    "filter_bell_to_csr_ms": round(filt_ms, 3), "filter_bell_to_csr_gb_s": round(filt_bytes / (filt_ms * 1e-3) / 1e9, 1),
-                   "filter_live_nnz": filt_live,
+                   "filter_live_nnz": filt_live, "repack_shard_bell16_ms": round(repack_ms, 3), "repack_ell_width": repack_ell_width,
                    "transpose_ms": round(tr_ms, 3), "transpose_gb_s": round(2 * csr_bytes / (tr_ms * 1e-3) / 1e9, 1),
                    "reduce_ms": round(red_ms, 4), "reduce_gb_s": round(red_bytes / (red_ms * 1e-3) / 1e9, 1)}}
     if not args.skip_cpu:

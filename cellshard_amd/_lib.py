@@ -64,6 +64,11 @@ PROTOTYPES = {
         P(A.filtered_blocked_ell_workspace), P(A.filtered_blocked_ell_result)]),
     "csamd_build_bucketed_filtered_blocked_ell_major_view": (i32, [P(A.blocked_ell_view), u32, u32, u32, vp, vp, vp, vp, u32,
         P(A.filtered_blocked_ell_workspace), P(A.filtered_blocked_ell_result)]),
+    "csamd_repack_init": (None, [P(A.sharded_blocked_ell_repack_workspace)]),
+    "csamd_repack_clear": (None, [P(A.sharded_blocked_ell_repack_workspace)]),
+    "csamd_repack_setup": (i32, [P(A.sharded_blocked_ell_repack_workspace), i32, vp]),
+    "csamd_build_repacked_blocked_ell": (i32, [P(A.blocked_ell_view), u32, u32, P(A.sharded_blocked_ell_repack_config), vp, vp, vp,
+        vp, C.c_ulong, P(A.sharded_blocked_ell_repack_workspace), P(A.sharded_blocked_ell_repack_result)]),
     # label reduce
     "csamd_label_reduce_init": (None, [P(A.label_reduce_workspace)]),
     "csamd_label_reduce_clear": (None, [P(A.label_reduce_workspace)]),

@@ -181,6 +181,25 @@ filtered_blocked_ell_workspace = _aligned16("filtered_blocked_ell_workspace", [
     ("device", C.c_int), ("stream", c_void_p), ("owns_stream", C.c_int), ("private_workspace", c_void_p),
 ])
 
+sharded_blocked_ell_repack_config = _aligned16("sharded_blocked_ell_repack_config", [
+    ("output_rows", c_u32), ("output_cols", c_u32), ("block_size", c_u32), ("requested_bucket_count", c_u32),
+])
+
+sharded_blocked_ell_repack_stats = _aligned16("sharded_blocked_ell_repack_stats", [
+    ("kept_rows", c_u32), ("output_rows", c_u32), ("output_cols", c_u32), ("live_nnz", c_u32),
+    ("kept_row_slots", C.c_uint64), ("dead_slots", C.c_uint64), ("live_fill_ratio", C.c_double),
+    ("dead_value_bytes", c_size_t), ("ell_width", c_u32),
+])
+
+sharded_blocked_ell_repack_result = _aligned16("sharded_blocked_ell_repack_result", [
+    ("filtered", compressed_view), ("bucketed", compressed_major_bucket_result), ("rebuilt", blocked_ell_view),
+    ("bucket_plan", major_nnz_bucket_plan_view), ("stats", sharded_blocked_ell_repack_stats), ("shard_id", C.c_ulong),
+])
+
+sharded_blocked_ell_repack_workspace = _aligned16("sharded_blocked_ell_repack_workspace", [
+    ("device", C.c_int), ("stream", c_void_p), ("owns_stream", C.c_int), ("private_workspace", c_void_p),
+])
+
 synth_params = _aligned16("synth_params", [
     ("rows", c_u32), ("cols", c_u32), ("value_type", c_u32), ("seed", c_u32),
     ("row_nnz_median", C.c_float), ("row_nnz_sigma", C.c_float), ("nb_alpha", C.c_float), ("reserved", c_u32),
@@ -300,6 +319,10 @@ ALL_STRUCTS = {
     "csamd_compressed_major_bucket_result": compressed_major_bucket_result,
     "csamd_filtered_blocked_ell_result": filtered_blocked_ell_result,
     "csamd_filtered_blocked_ell_workspace": filtered_blocked_ell_workspace,
+    "csamd_sharded_blocked_ell_repack_config": sharded_blocked_ell_repack_config,
+    "csamd_sharded_blocked_ell_repack_stats": sharded_blocked_ell_repack_stats,
+    "csamd_sharded_blocked_ell_repack_result": sharded_blocked_ell_repack_result,
+    "csamd_sharded_blocked_ell_repack_workspace": sharded_blocked_ell_repack_workspace,
     "csamd_synth_params": synth_params,
     "csamd_host_compressed": host_compressed,
     "csamd_host_blocked_ell": host_blocked_ell,

@@ -43,6 +43,7 @@ template<> __device__ __forceinline__ bool fl_nonzero<uint8_t>(uint8_t raw) { re
 
 struct fl_masks {
     unsigned int output_rows, output_cols;
+    unsigned int row_base;  // shard row of the part's first row (the output row without a remap)
     const unsigned char *keep_rows, *keep_cols;
     const unsigned int *row_remap, *col_remap;
     const unsigned int *col_map;  // keep and remap of every column in one lookup (~0u: dropped), or null
@@ -65,7 +66,7 @@ __global__ void fl_col_map_kernel(unsigned int cols, fl_masks m, unsigned int *_
 // Output row of source row r, or output_rows when the row is dropped.
 __device__ __forceinline__ unsigned int fl_out_row(const fl_masks &m, unsigned int r) {
     if (m.keep_rows != nullptr && m.keep_rows[r] == 0u) return m.output_rows;
-    const unsigned int o = m.row_remap != nullptr ? m.row_remap[r] : r;
+    const unsigned int o = m.row_remap != nullptr ? m.row_remap[r] : m.row_base + r;
     return o < m.output_rows ? o : m.output_rows;
 }
 
@@ -289,7 +290,7 @@ __global__ void __launch_bounds__(FL_THREADS) fl_emit_kernel(csamd_blocked_ell_v
 
 // Workspace-owned device memory (grow-only).
 struct fl_private {
-    void *ptr_block;     // counts [output_rows] | claim [output_rows] | dup | ptr [output_rows + 1] | scan scratch
+    void *ptr_block;     // counts [output_rows] | claim [output_rows] | dup, kept rows / slots | ptr [output_rows + 1] | col map | scan scratch
     size_t ptr_bytes;
     void *val_block;     // minor idx [max_nnz] | values [max_nnz]
     size_t val_bytes;
@@ -297,7 +298,38 @@ struct fl_private {
     size_t bucket_bytes;
     void *rebuilt_block; // rebuilt minor idx | values
     size_t rebuilt_bytes;
+    void *bell_block;    // repacked Blocked-ELL: blockColIdx | values
+    size_t bell_bytes;
+    csamd_convert_workspace cws;  // scratch of the Blocked-ELL build
+    int cws_ready;
 };
+
+void free_private(void *p) {
+    if (p == nullptr) return;
+    fl_private *st = (fl_private *) p;
+    void *blocks[5] = {st->ptr_block, st->val_block, st->bucket_block, st->rebuilt_block, st->bell_block};
+    for (void *b : blocks)
+        if (b != nullptr) (void) hipFree(b);
+    if (st->cws_ready) {
+        st->cws.stream = nullptr;  // the stream belongs to the caller's workspace
+        st->cws.owns_stream = 0;
+        csamd_convert_clear(&st->cws);
+    }
+    std::free(st);
+}
+
+// kept[0] += rows kept (keep null: all), kept[1] += ell_cols per kept row.
+__global__ void fl_kept_kernel(unsigned int rows, const unsigned char *__restrict__ keep_rows, unsigned int ell_cols,
+                               unsigned long long *__restrict__ kept) {
+    unsigned int n = 0u;
+    for (unsigned int r = blockIdx.x * blockDim.x + threadIdx.x; r < rows; r += gridDim.x * blockDim.x)
+        n += keep_rows == nullptr || keep_rows[r] != 0u ? 1u : 0u;
+    n = wave_sum_u32(n);
+    if (lane_id() == 0u && n != 0u) {
+        atomicAdd(&kept[0], (unsigned long long) n);
+        atomicAdd(&kept[1], (unsigned long long) n * ell_cols);
+    }
+}
 
 int grow(void **p, size_t *have, size_t need) {
     if (need <= *have && *p != nullptr) return 1;
@@ -336,28 +368,39 @@ int fl_dispatch(unsigned int vt, Args... args) {
     }
 }
 
-int filter_impl(const csamd_blocked_ell_view *src, unsigned int value_type, unsigned int output_rows, unsigned int output_cols,
-                const unsigned char *d_keep_rows, const unsigned char *d_keep_cols, const unsigned int *d_row_remap,
-                const unsigned int *d_col_remap, csamd_filtered_blocked_ell_workspace *ws, csamd_filtered_blocked_ell_result *out) {
-    CSAMD_REQUIRE(src != nullptr && ws != nullptr, "filter_blocked_ell_to_compressed: null argument");
+// Filtered CSR over consecutive partitions (partition p holds source rows
+// [row0_p, row0_p + parts[p].rows); keep_rows / row_remap index those
+// source rows).  Sets out->filtered and, in *kept, the kept row count and
+// the kept rows' ell slots.
+int filter_parts(const csamd_blocked_ell_view *parts, unsigned int part_count, unsigned int value_type, unsigned int output_rows,
+                 unsigned int output_cols, const unsigned char *d_keep_rows, const unsigned char *d_keep_cols,
+                 const unsigned int *d_row_remap, const unsigned int *d_col_remap, int device, hipStream_t s, void **priv,
+                 csamd_compressed_view *filtered, unsigned long long kept[2], const char *who) {
     const int vb = value_type_bytes(value_type);
-    CSAMD_REQUIRE(vb == 1 || vb == 2 || vb == 4, "filter_blocked_ell_to_compressed: unsupported value type %u", value_type);
-    CSAMD_REQUIRE(src->rows == 0u || src->block_size != 0u, "filter_blocked_ell_to_compressed: block_size is 0");
-    CSAMD_REQUIRE(src->block_size == 0u || src->ell_cols % src->block_size == 0u,
-                  "filter_blocked_ell_to_compressed: ell_cols not a multiple of block_size");
-    CSAMD_HIP(hipSetDevice(ws->device >= 0 ? ws->device : 0), "hipSetDevice filtered blocked ell");
-    if (ws->private_workspace == nullptr) {
-        ws->private_workspace = std::calloc(1u, sizeof(fl_private));
-        CSAMD_REQUIRE(ws->private_workspace != nullptr, "filter_blocked_ell_to_compressed: host allocation failed");
+    CSAMD_REQUIRE(vb == 1 || vb == 2 || vb == 4, "%s: unsupported value type %u", who, value_type);
+    CSAMD_REQUIRE(part_count == 0u || parts != nullptr, "%s: null partitions", who);
+    unsigned long long max_nnz = 0ull, cols = 0ull;
+    for (unsigned int p = 0; p < part_count; ++p) {
+        const csamd_blocked_ell_view &v = parts[p];
+        CSAMD_REQUIRE(v.rows == 0u || v.block_size != 0u, "%s: block_size is 0", who);
+        CSAMD_REQUIRE(v.block_size == 0u || v.ell_cols % v.block_size == 0u, "%s: ell_cols not a multiple of block_size", who);
+        CSAMD_REQUIRE(v.rows == 0u || v.ell_cols == 0u || (v.blockColIdx != nullptr && v.val != nullptr), "%s: source arrays missing",
+                      who);
+        max_nnz += v.nnz;  // a kept entry is a nonzero stored value
+        cols = v.cols > cols ? v.cols : cols;
     }
-    fl_private *st = (fl_private *) ws->private_workspace;
-    hipStream_t s = (hipStream_t) ws->stream;
-    const unsigned int max_nnz = src->nnz;  // a kept entry is a nonzero stored value
+    CSAMD_REQUIRE(max_nnz <= 0xffffffffull, "%s: %llu stored values exceed u32", who, max_nnz);
+    CSAMD_HIP(hipSetDevice(device >= 0 ? device : 0), "hipSetDevice filtered blocked ell");
+    if (*priv == nullptr) {
+        *priv = std::calloc(1u, sizeof(fl_private));
+        CSAMD_REQUIRE(*priv != nullptr, "%s: host allocation failed", who);
+    }
+    fl_private *st = (fl_private *) *priv;
     const size_t off_claim = align_up((size_t) output_rows * 4u, 256u);
     const size_t off_dup = off_claim + align_up((size_t) output_rows * 4u, 256u);
     const size_t off_ptr = off_dup + 256u;
     const size_t off_map = off_ptr + align_up(((size_t) output_rows + 1u) * 4u, 256u);
-    const size_t off_scan = off_map + align_up((size_t) src->cols * 4u, 256u);
+    const size_t off_scan = off_map + align_up((size_t) cols * 4u, 256u);
     if (!grow(&st->ptr_block, &st->ptr_bytes, off_scan + exclusive_scan_scratch_bytes(output_rows))) return 0;
     const size_t off_val = align_up((size_t) max_nnz * 4u, 256u);
     if (!grow(&st->val_block, &st->val_bytes, off_val + (size_t) max_nnz * (size_t) vb)) return 0;
@@ -365,61 +408,144 @@ int filter_impl(const csamd_blocked_ell_view *src, unsigned int value_type, unsi
     unsigned int *counts = (unsigned int *) pb;
     unsigned int *claim = (unsigned int *) (pb + off_claim);
     unsigned int *dup = (unsigned int *) (pb + off_dup);
+    unsigned long long *d_kept = (unsigned long long *) (pb + off_dup + 64u);
     unsigned int *ptr = (unsigned int *) (pb + off_ptr);
     unsigned int *out_idx = (unsigned int *) st->val_block;
     void *out_val = (char *) st->val_block + off_val;
-
-    csamd_filtered_blocked_ell_result local;
-    std::memset(&local, 0, sizeof(local));
-    local.filtered.rows = output_rows;
-    local.filtered.cols = output_cols;
-    local.filtered.axis = CSAMD_COMPRESSED_BY_ROW;
-    local.filtered.majorPtr = ptr;
-    local.filtered.minorIdx = out_idx;
-    local.filtered.val = (csamd_half *) out_val;
-    local.stats.kept_rows = output_rows;  // as the reference reports it
-    local.stats.output_rows = output_rows;
-    local.stats.output_cols = output_cols;
-    local.stats.live_fill_ratio = 1.0;
+    std::memset(filtered, 0, sizeof(*filtered));
+    filtered->rows = output_rows;
+    filtered->cols = output_cols;
+    filtered->axis = CSAMD_COMPRESSED_BY_ROW;
+    filtered->majorPtr = ptr;
+    filtered->minorIdx = out_idx;
+    filtered->val = (csamd_half *) out_val;
+    if (!zero_bytes(pb, off_ptr, s)) return 0;  // counts, claims, dup flag, kept totals
+    // kept rows and their ell slots (counted whatever the output range)
+    unsigned int row0 = 0u;
+    for (unsigned int p = 0; p < part_count; ++p) {
+        if (parts[p].rows != 0u) {
+            fl_kept_kernel<<<grid_for(parts[p].rows, 256u, 1024u), 256, 0, s>>>(
+                parts[p].rows, d_keep_rows != nullptr ? d_keep_rows + row0 : nullptr, parts[p].ell_cols, d_kept);
+            CSAMD_HIP(hipGetLastError(), "fl_kept_kernel");
+        }
+        row0 += parts[p].rows;
+    }
     unsigned int live = 0u;
     if (output_rows != 0u && output_cols != 0u) {
-        CSAMD_REQUIRE(src->rows == 0u || (src->blockColIdx != nullptr && src->val != nullptr),
-                      "filter_blocked_ell_to_compressed: source arrays missing");
-        if (!zero_bytes(pb, off_ptr, s)) return 0;  // counts, claims, dup flag
         unsigned int *col_map = (unsigned int *) (pb + off_map);
-        const fl_masks m{output_rows, output_cols, d_keep_rows, d_keep_cols, d_row_remap, d_col_remap, col_map};
-        if (src->cols != 0u) {
-            fl_col_map_kernel<<<grid_for(src->cols, 256u, 1024u), 256, 0, s>>>(src->cols, m, col_map);
+        if (cols != 0u) {
+            const fl_masks mc{output_rows, output_cols, 0u, nullptr, d_keep_cols, nullptr, d_col_remap, col_map};
+            fl_col_map_kernel<<<grid_for(cols, 256u, 1024u), 256, 0, s>>>((unsigned int) cols, mc, col_map);
             CSAMD_HIP(hipGetLastError(), "fl_col_map_kernel");
         }
-        if (src->rows != 0u && src->ell_cols != 0u &&
-            !fl_dispatch<fl_launch>(value_type, src, m, counts, claim, dup, (const unsigned int *) nullptr, out_idx, out_val, false, s))
-            return 0;
+        auto pass = [&](bool emit) -> int {
+            unsigned int r0 = 0u;
+            for (unsigned int p = 0; p < part_count; ++p) {
+                const csamd_blocked_ell_view &v = parts[p];
+                const fl_masks m{output_rows, output_cols, r0, d_keep_rows != nullptr ? d_keep_rows + r0 : nullptr, d_keep_cols,
+                                 d_row_remap != nullptr ? d_row_remap + r0 : nullptr, d_col_remap, col_map};
+                if (v.rows != 0u && v.ell_cols != 0u &&
+                    !fl_dispatch<fl_launch>(value_type, &v, m, counts, claim, dup, (const unsigned int *) ptr, out_idx, out_val, emit, s))
+                    return 0;
+                r0 += v.rows;
+            }
+            return 1;
+        };
+        if (!pass(false)) return 0;
         if (!exclusive_scan_u32(counts, output_rows, ptr, nullptr, pb + off_scan, exclusive_scan_scratch_bytes(output_rows), s)) return 0;
         unsigned int h[2] = {0u, 0u};
         CSAMD_HIP(hipMemcpyAsync(&h[0], ptr + output_rows, 4u, hipMemcpyDeviceToHost, s), "D2H filtered nnz");
         CSAMD_HIP(hipMemcpyAsync(&h[1], dup, 4u, hipMemcpyDeviceToHost, s), "D2H filtered dup");
         CSAMD_HIP(hipStreamSynchronize(s), "filtered blocked ell sync");
-        CSAMD_REQUIRE(h[1] == 0u, "filter_blocked_ell_to_compressed: row_remap maps two kept rows to output row %u", h[1] - 1u);
+        CSAMD_REQUIRE(h[1] == 0u, "%s: row_remap maps two kept rows to output row %u", who, h[1] - 1u);
         live = h[0];
-        CSAMD_REQUIRE(live <= max_nnz, "filter_blocked_ell_to_compressed: %u surviving entries exceed the source nnz %u", live, max_nnz);
-        if (src->rows != 0u && src->ell_cols != 0u &&
-            !fl_dispatch<fl_launch>(value_type, src, m, counts, claim, dup, (const unsigned int *) ptr, out_idx, out_val, true, s))
-            return 0;
+        CSAMD_REQUIRE(live <= max_nnz, "%s: %u surviving entries exceed the source nnz %llu", who, live, max_nnz);
+        if (!pass(true)) return 0;
     } else if (output_rows != 0u) {
         if (!zero_bytes(ptr, ((size_t) output_rows + 1u) * 4u, s)) return 0;
     } else {
         if (!zero_bytes(ptr, 4u, s)) return 0;
     }
-    local.filtered.nnz = live;
-    local.stats.live_nnz = live;
-    if (output_rows != 0u && output_cols != 0u) {
-        const unsigned long long slots = (unsigned long long) output_rows * src->ell_cols;
-        local.stats.kept_row_slots = slots;
-        local.stats.dead_slots = slots > live ? slots - live : 0ull;
-        local.stats.live_fill_ratio = slots == 0ull ? 1.0 : (double) live / (double) slots;
-        local.stats.dead_value_bytes = (size_t) local.stats.dead_slots * (size_t) vb;
-    }
+    filtered->nnz = live;
+    CSAMD_HIP(hipMemcpyAsync(kept, d_kept, 16u, hipMemcpyDeviceToHost, s), "D2H kept rows");
+    CSAMD_HIP(hipStreamSynchronize(s), "filtered blocked ell sync");
+    return 1;
+}
+
+// bucket::build_bucketed_major_view over the filtered CSR: plan (stable
+// ascending nnz order, equal-count offsets) and rebuild into workspace memory.
+int bucket_view(fl_private *st, const csamd_compressed_view &filtered, unsigned int value_type, unsigned int requested,
+                hipStream_t s, csamd_compressed_major_bucket_result *out) {
+    const unsigned int n = filtered.rows;
+    const unsigned int bc = csamd_clamp_bucket_count(n, requested);
+    size_t sort_b = 0u, scan_b = 0u;
+    if (!csamd_major_nnz_bucket_sort_scratch_bytes(n, &sort_b) || !csamd_major_nnz_bucket_scan_scratch_bytes(n, &scan_b)) return 0;
+    const size_t a = align_up((size_t) n * 4u, 256u);
+    const size_t off_sorted = a, off_in = 2u * a, off_out = 3u * a, off_inv = 4u * a, off_ptr = 5u * a;
+    const size_t off_offsets = off_ptr + align_up(((size_t) n + 1u) * 4u, 256u);
+    const size_t off_sort = off_offsets + align_up(((size_t) bc + 1u) * 4u, 256u);
+    const size_t off_scan = off_sort + align_up(sort_b, 256u);
+    if (!grow(&st->bucket_block, &st->bucket_bytes, off_scan + scan_b)) return 0;
+    const int vb = value_type_bytes(value_type);
+    const size_t nnz = filtered.nnz;
+    const size_t off_rval = align_up(nnz * 4u, 256u);
+    if (!grow(&st->rebuilt_block, &st->rebuilt_bytes, off_rval + nnz * (size_t) vb)) return 0;
+    char *b = (char *) st->bucket_block;
+    unsigned int *major_nnz = (unsigned int *) b;
+    unsigned int *sorted = (unsigned int *) (b + off_sorted);
+    unsigned int *order_in = (unsigned int *) (b + off_in);
+    unsigned int *order_out = (unsigned int *) (b + off_out);
+    unsigned int *inverse = (unsigned int *) (b + off_inv);
+    unsigned int *rptr = (unsigned int *) (b + off_ptr);
+    unsigned int *offsets = (unsigned int *) (b + off_offsets);
+    unsigned int *ridx = (unsigned int *) st->rebuilt_block;
+    void *rval = (char *) st->rebuilt_block + off_rval;
+    if (!csamd_build_major_nnz_bucket_plan_raw(filtered.majorPtr, n, major_nnz, sorted, order_in, order_out, offsets, bc,
+                                               b + off_sort, sort_b, (csamd_stream_t) s))
+        return 0;
+    if (!csamd_rebuild_compressed_major_order_raw(filtered.majorPtr, filtered.minorIdx, filtered.val, value_type, n, order_out, rptr,
+                                                  ridx, rval, inverse, b + off_scan, scan_b, (csamd_stream_t) s))
+        return 0;
+    if (n == 0u && !zero_bytes(rptr, 4u, s)) return 0;
+    out->rebuilt = filtered;
+    out->rebuilt.majorPtr = rptr;
+    out->rebuilt.minorIdx = ridx;
+    out->rebuilt.val = (csamd_half *) rval;
+    out->plan.major_dim = n;
+    out->plan.bucket_count = bc;
+    out->plan.major_order = order_out;
+    out->plan.major_nnz_sorted = sorted;
+    out->plan.bucket_offsets = offsets;
+    out->plan.inverse_major_order = inverse;
+    return 1;
+}
+
+void fill_stats(csamd_filtered_blocked_ell_stats *st, unsigned int kept_rows, unsigned int output_rows, unsigned int output_cols,
+                unsigned int live, unsigned long long slots, int vb) {
+    st->kept_rows = kept_rows;
+    st->output_rows = output_rows;
+    st->output_cols = output_cols;
+    st->live_nnz = live;
+    st->kept_row_slots = slots;
+    st->dead_slots = slots > live ? slots - live : 0ull;
+    st->live_fill_ratio = slots == 0ull ? 1.0 : (double) live / (double) slots;
+    st->dead_value_bytes = (size_t) st->dead_slots * (size_t) vb;
+}
+
+int filter_impl(const csamd_blocked_ell_view *src, unsigned int value_type, unsigned int output_rows, unsigned int output_cols,
+                const unsigned char *d_keep_rows, const unsigned char *d_keep_cols, const unsigned int *d_row_remap,
+                const unsigned int *d_col_remap, csamd_filtered_blocked_ell_workspace *ws, csamd_filtered_blocked_ell_result *out) {
+    CSAMD_REQUIRE(src != nullptr && ws != nullptr, "filter_blocked_ell_to_compressed: null argument");
+    csamd_filtered_blocked_ell_result local;
+    std::memset(&local, 0, sizeof(local));
+    unsigned long long kept[2] = {0ull, 0ull};
+    if (!filter_parts(src, 1u, value_type, output_rows, output_cols, d_keep_rows, d_keep_cols, d_row_remap, d_col_remap, ws->device,
+                      (hipStream_t) ws->stream, &ws->private_workspace, &local.filtered, kept, "filter_blocked_ell_to_compressed"))
+        return 0;
+    // the reference reports kept_rows = output_rows and prices kept_rows x ell_cols slots (:359-370)
+    const bool any = output_rows != 0u && output_cols != 0u;
+    fill_stats(&local.stats, output_rows, output_rows, output_cols, local.filtered.nnz,
+               any ? (unsigned long long) output_rows * src->ell_cols : 0ull, value_type_bytes(value_type));
     if (out != nullptr) *out = local;
     return 1;
 }
@@ -440,13 +566,7 @@ void csamd_filtered_blocked_ell_init(csamd_filtered_blocked_ell_workspace *ws) {
 void csamd_filtered_blocked_ell_clear(csamd_filtered_blocked_ell_workspace *ws) {
     if (ws == nullptr) return;
     if (ws->device >= 0) (void) hipSetDevice(ws->device);
-    if (ws->private_workspace != nullptr) {
-        fl_private *st = (fl_private *) ws->private_workspace;
-        void *blocks[4] = {st->ptr_block, st->val_block, st->bucket_block, st->rebuilt_block};
-        for (void *b : blocks)
-            if (b != nullptr) (void) hipFree(b);
-        std::free(st);
-    }
+    free_private(ws->private_workspace);
     if (ws->owns_stream && ws->stream != nullptr) (void) hipStreamDestroy((hipStream_t) ws->stream);
     csamd_filtered_blocked_ell_init(ws);
 }
@@ -486,53 +606,120 @@ int csamd_build_bucketed_filtered_blocked_ell_major_view(const csamd_blocked_ell
     csamd_filtered_blocked_ell_result local;
     if (!filter_impl(src, value_type, output_rows, output_cols, d_keep_rows, d_keep_cols, d_row_remap, d_col_remap, ws, &local))
         return 0;
-    // bucket::build_bucketed_major_view: plan (stable ascending nnz order,
-    // equal-count offsets) and rebuild into workspace buffers
-    fl_private *st = (fl_private *) ws->private_workspace;
-    hipStream_t s = (hipStream_t) ws->stream;
-    const unsigned int n = output_rows;
-    const unsigned int bc = csamd_clamp_bucket_count(n, requested_bucket_count);
-    size_t sort_b = 0u, scan_b = 0u;
-    if (!csamd_major_nnz_bucket_sort_scratch_bytes(n, &sort_b) || !csamd_major_nnz_bucket_scan_scratch_bytes(n, &scan_b)) return 0;
-    const size_t a = align_up((size_t) n * 4u, 256u);
-    const size_t off_sorted = a, off_in = 2u * a, off_out = 3u * a, off_inv = 4u * a, off_ptr = 5u * a;
-    const size_t off_offsets = off_ptr + align_up(((size_t) n + 1u) * 4u, 256u);
-    const size_t off_sort = off_offsets + align_up(((size_t) bc + 1u) * 4u, 256u);
-    const size_t off_scan = off_sort + align_up(sort_b, 256u);
-    if (!grow(&st->bucket_block, &st->bucket_bytes, off_scan + scan_b)) return 0;
-    const int vb = value_type_bytes(value_type);
-    const size_t nnz = local.filtered.nnz;
-    const size_t off_rval = align_up(nnz * 4u, 256u);
-    if (!grow(&st->rebuilt_block, &st->rebuilt_bytes, off_rval + nnz * (size_t) vb)) return 0;
-    char *b = (char *) st->bucket_block;
-    unsigned int *major_nnz = (unsigned int *) b;
-    unsigned int *sorted = (unsigned int *) (b + off_sorted);
-    unsigned int *order_in = (unsigned int *) (b + off_in);
-    unsigned int *order_out = (unsigned int *) (b + off_out);
-    unsigned int *inverse = (unsigned int *) (b + off_inv);
-    unsigned int *rptr = (unsigned int *) (b + off_ptr);
-    unsigned int *offsets = (unsigned int *) (b + off_offsets);
-    unsigned int *ridx = (unsigned int *) st->rebuilt_block;
-    void *rval = (char *) st->rebuilt_block + off_rval;
-    if (!csamd_build_major_nnz_bucket_plan_raw(local.filtered.majorPtr, n, major_nnz, sorted, order_in, order_out, offsets, bc,
-                                               b + off_sort, sort_b, (csamd_stream_t) s))
+    if (!bucket_view((fl_private *) ws->private_workspace, local.filtered, value_type, requested_bucket_count, (hipStream_t) ws->stream,
+                     &local.bucketed))
         return 0;
-    if (!csamd_rebuild_compressed_major_order_raw(local.filtered.majorPtr, local.filtered.minorIdx, local.filtered.val, value_type, n,
-                                                  order_out, rptr, ridx, rval, inverse, b + off_scan, scan_b, (csamd_stream_t) s))
-        return 0;
-    if (n == 0u && !zero_bytes(rptr, 4u, s)) return 0;
-    local.bucketed.rebuilt = local.filtered;
-    local.bucketed.rebuilt.majorPtr = rptr;
-    local.bucketed.rebuilt.minorIdx = ridx;
-    local.bucketed.rebuilt.val = (csamd_half *) rval;
-    local.bucketed.plan.major_dim = n;
-    local.bucketed.plan.bucket_count = bc;
-    local.bucketed.plan.major_order = order_out;
-    local.bucketed.plan.major_nnz_sorted = sorted;
-    local.bucketed.plan.bucket_offsets = offsets;
-    local.bucketed.plan.inverse_major_order = inverse;
     local.has_bucketed = 1;
     if (out != nullptr) *out = local;
+    return 1;
+}
+
+void csamd_repack_init(csamd_sharded_blocked_ell_repack_workspace *ws) {
+    if (ws == nullptr) return;
+    std::memset(ws, 0, sizeof(*ws));
+    ws->device = -1;
+}
+
+void csamd_repack_clear(csamd_sharded_blocked_ell_repack_workspace *ws) {
+    if (ws == nullptr) return;
+    if (ws->device >= 0) (void) hipSetDevice(ws->device);
+    free_private(ws->private_workspace);
+    if (ws->owns_stream && ws->stream != nullptr) (void) hipStreamDestroy((hipStream_t) ws->stream);
+    csamd_repack_init(ws);
+}
+
+int csamd_repack_setup(csamd_sharded_blocked_ell_repack_workspace *ws, int device, csamd_stream_t stream) {
+    CSAMD_REQUIRE(ws != nullptr, "repack setup: null workspace");
+    csamd_repack_clear(ws);
+    CSAMD_HIP(hipSetDevice(device), "hipSetDevice repack setup");
+    ws->device = device;
+    if (stream == nullptr) {
+        hipStream_t s = nullptr;
+        CSAMD_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreateWithFlags repack");
+        ws->stream = (csamd_stream_t) s;
+        ws->owns_stream = 1;
+    } else {
+        ws->stream = stream;
+        ws->owns_stream = 0;
+    }
+    return 1;
+}
+
+int csamd_build_repacked_blocked_ell(const csamd_blocked_ell_view *parts, unsigned int part_count, unsigned int value_type,
+                                     const csamd_sharded_blocked_ell_repack_config *cfg, const unsigned char *d_keep_rows,
+                                     const unsigned char *d_keep_cols, const unsigned int *d_row_remap,
+                                     const unsigned int *d_col_remap, unsigned long shard_id,
+                                     csamd_sharded_blocked_ell_repack_workspace *ws, csamd_sharded_blocked_ell_repack_result *out) {
+    const char *who = "build_repacked_shard_blocked_ell";
+    CSAMD_REQUIRE(cfg != nullptr && ws != nullptr && out != nullptr, "%s: null argument", who);
+    CSAMD_REQUIRE(cfg->block_size != 0u, "%s: block_size is 0", who);
+    hipStream_t s = (hipStream_t) ws->stream;
+    csamd_sharded_blocked_ell_repack_result local;
+    std::memset(&local, 0, sizeof(local));
+    unsigned long long kept[2] = {0ull, 0ull};
+    if (!filter_parts(parts, part_count, value_type, cfg->output_rows, cfg->output_cols, d_keep_rows, d_keep_cols, d_row_remap,
+                      d_col_remap, ws->device, s, &ws->private_workspace, &local.filtered, kept, who))
+        return 0;
+    fl_private *st = (fl_private *) ws->private_workspace;
+    const unsigned int live = local.filtered.nnz;
+    const int vb = value_type_bytes(value_type);
+    local.shard_id = shard_id;
+    // kept rows counted over the shard, slots priced at each kept row's own ell width
+    // (repack/kernels/sharded_blocked_ell.cuh:43-51)
+    csamd_filtered_blocked_ell_stats fs;
+    fill_stats(&fs, (unsigned int) kept[0], cfg->output_rows, cfg->output_cols, live, kept[1], vb);
+    local.stats.kept_rows = fs.kept_rows;
+    local.stats.output_rows = fs.output_rows;
+    local.stats.output_cols = fs.output_cols;
+    local.stats.live_nnz = fs.live_nnz;
+    local.stats.kept_row_slots = fs.kept_row_slots;
+    local.stats.dead_slots = fs.dead_slots;
+    local.stats.live_fill_ratio = fs.live_fill_ratio;
+    local.stats.dead_value_bytes = fs.dead_value_bytes;
+    local.stats.ell_width = 0u;
+    local.rebuilt.rows = cfg->output_rows;
+    local.rebuilt.cols = cfg->output_cols;
+    local.rebuilt.nnz = live;
+    local.rebuilt.block_size = cfg->block_size;
+    if (cfg->output_rows == 0u || cfg->output_cols == 0u || live == 0u) {
+        // nothing to bucket or pack (:541-560)
+        local.bucketed.rebuilt = local.filtered;
+        local.bucketed.plan.major_dim = cfg->output_rows;
+        local.bucketed.plan.bucket_count = 1u;
+        local.bucket_plan = local.bucketed.plan;
+        *out = local;
+        return 1;
+    }
+    if (!bucket_view(st, local.filtered, value_type, cfg->requested_bucket_count, s, &local.bucketed)) return 0;
+    local.bucket_plan = local.bucketed.plan;
+    // the bucketed rows repacked as Blocked-ELL of the requested block size
+    if (!st->cws_ready) {
+        csamd_convert_init(&st->cws);
+        if (!csamd_convert_setup(&st->cws, ws->device >= 0 ? ws->device : 0, ws->stream)) return 0;
+        st->cws_ready = 1;
+    }
+    st->cws.stream = ws->stream;
+    const unsigned int bs = cfg->block_size;
+    unsigned int width = 0u;
+    unsigned long long total = 0ull;
+    double fill = 0.0;
+    if (!csamd_blocked_ell_shape(&local.bucketed.rebuilt, bs, &st->cws, &width, &total, &fill)) return 0;
+    const unsigned long long row_blocks = (cfg->output_rows + (unsigned long long) bs - 1ull) / bs;
+    const unsigned long long ell_cols = (unsigned long long) width * bs;
+    CSAMD_REQUIRE(ell_cols <= 0xffffffffull, "%s: ell width %u x block %u exceeds u32", who, width, bs);
+    const size_t off_v = align_up((size_t) (row_blocks * width) * 4u, 256u);
+    if (!grow(&st->bell_block, &st->bell_bytes, off_v + (size_t) cfg->output_rows * (size_t) ell_cols * (size_t) vb)) return 0;
+    csamd_blocked_ell_view bell;
+    std::memset(&bell, 0, sizeof(bell));
+    bell.block_size = bs;
+    bell.ell_cols = (unsigned int) ell_cols;
+    bell.blockColIdx = (unsigned int *) st->bell_block;
+    bell.val = (csamd_half *) ((char *) st->bell_block + off_v);
+    if (!csamd_blocked_ell_from_compressed(&local.bucketed.rebuilt, value_type, &bell, &st->cws)) return 0;
+    local.rebuilt = bell;
+    local.rebuilt.nnz = live;  // as the reference reports it (:714)
+    local.stats.ell_width = width;
+    *out = local;
     return 1;
 }
 

@@ -135,6 +135,50 @@ def filter_blocked_ell_to_compressed(bell: DeviceBlockedELL, output_rows: int, o
     return out
 
 
+class RepackWorkspace:
+    """sharded_blocked_ell_repack_workspace: results alias its memory until
+    the next call."""
+
+    def __init__(self, device: int = 0, stream: int | None = None):
+        self.ws = A.sharded_blocked_ell_repack_workspace()
+        lib = load()
+        lib.csamd_repack_init(C.byref(self.ws))
+        check(lib.csamd_repack_setup(C.byref(self.ws), device, stream if stream is not None else current_stream_handle()),
+              "repack_setup")
+
+    def close(self):
+        if self.ws is not None:
+            load().csamd_repack_clear(C.byref(self.ws))
+            self.ws = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def build_repacked_shard_blocked_ell(parts: list, output_rows: int, output_cols: int, block_size: int, bucket_count: int,
+                                     keep_rows=None, keep_cols=None, row_remap=None, col_remap=None, shard_id: int = 0,
+                                     ws: RepackWorkspace | None = None) -> A.sharded_blocked_ell_repack_result:
+    """One shard's Blocked-ELL partitions (in shard order) -> masked CSR ->
+    bucketed CSR -> Blocked-ELL of `block_size`
+    (repack/operators/sharded_blocked_ell.cuh:393-723).  keep_rows /
+    row_remap index the shard's rows."""
+    if ws is None:
+        raise ValueError("build_repacked_shard_blocked_ell: pass a RepackWorkspace (the result aliases it)")
+    vts = {b.value_type for b in parts}
+    if len(vts) > 1:
+        raise ValueError("build_repacked_shard_blocked_ell: partitions of different value types")
+    views = (A.blocked_ell_view * max(len(parts), 1))(*[b.view() for b in parts])
+    cfg = A.sharded_blocked_ell_repack_config(output_rows, output_cols, block_size, bucket_count)
+    out = A.sharded_blocked_ell_repack_result()
+    check(load().csamd_build_repacked_blocked_ell(views, len(parts), vts.pop() if vts else A.VALUE_F16, C.byref(cfg),
+                                                 *[ptr(t) if t is not None else None for t in (keep_rows, keep_cols, row_remap, col_remap)],
+                                                 shard_id, C.byref(ws.ws), C.byref(out)), "build_repacked_shard_blocked_ell")
+    return out
+
+
 # ---------------------------------------------------------------------------
 # Sliced-ELL
 # ---------------------------------------------------------------------------

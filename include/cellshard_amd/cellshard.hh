@@ -473,6 +473,31 @@ inline int build_bucketed_filtered_blocked_ell_major_view(const device::blocked_
 
 } // namespace convert
 
+// Shard repack: masked CSR -> bucketed CSR -> Blocked-ELL
+// (repack/operators/sharded_blocked_ell.cuh:17-723).  The reference reads the
+// shard's partitions from a sharded_device state; here the caller passes
+// the partitions' views (host array, device arrays) in shard order.
+namespace repack {
+using sharded_blocked_ell_repack_config = csamd_sharded_blocked_ell_repack_config;
+using sharded_blocked_ell_repack_stats = csamd_sharded_blocked_ell_repack_stats;
+using sharded_blocked_ell_repack_result = csamd_sharded_blocked_ell_repack_result;
+using sharded_blocked_ell_repack_workspace = csamd_sharded_blocked_ell_repack_workspace;
+
+inline void init(sharded_blocked_ell_repack_workspace *ws) { csamd_repack_init(ws); }
+inline void clear(sharded_blocked_ell_repack_workspace *ws) { csamd_repack_clear(ws); }
+inline int setup(sharded_blocked_ell_repack_workspace *ws, int device, hipStream_t stream = (hipStream_t) 0) {
+    return csamd_repack_setup(ws, device, (csamd_stream_t) stream);
+}
+inline int build_repacked_shard_blocked_ell(const device::blocked_ell_view *part_views, unsigned long part_count,
+                                            unsigned long shard_id, const sharded_blocked_ell_repack_config *cfg,
+                                            const unsigned char *d_keep_rows, const unsigned char *d_keep_cols,
+                                            const types::idx_t *d_row_remap, const types::idx_t *d_col_remap,
+                                            sharded_blocked_ell_repack_workspace *ws, sharded_blocked_ell_repack_result *out) {
+    return csamd_build_repacked_blocked_ell(part_views, (unsigned int) part_count, CSAMD_VALUE_F16, cfg, d_keep_rows, d_keep_cols,
+                                            d_row_remap, d_col_remap, shard_id, ws, out);
+}
+} // namespace repack
+
 // ---------------------------------------------------------------------------
 // Bucketed execution partitions (src/io/csh5/execution_builders_part.hh:
 // 1206-1275, 1535-1596) over device views; segments stay on the device.

@@ -828,6 +828,61 @@ int csamd_build_bucketed_filtered_blocked_ell_major_view(const csamd_blocked_ell
                                                          csamd_filtered_blocked_ell_workspace *ws,
                                                          csamd_filtered_blocked_ell_result *out);
 
+/* Repack of one shard's Blocked-ELL partitions: masked compaction to CSR
+ * (kept rows counted over the shard, slots priced at each kept row's own
+ * ell width), the major-nnz bucket plan and rebuild, then Blocked-ELL of the
+ * requested block size over the bucketed rows.
+ * repack/operators/sharded_blocked_ell.cuh:17-47 (config, stats, result,
+ * workspace), :113-150 (init / clear / setup), :393-723
+ * (build_repacked_shard_blocked_ell).  The reference reads the shard's
+ * partitions from a sharded_device state; here the caller passes the
+ * partitions' views in shard order (host array of views over device
+ * arrays); keep_rows / row_remap index the shard's rows.  Row pointers as
+ * in the filter above.  Outputs alias workspace memory. */
+typedef struct CSAMD_ALIGN16 csamd_sharded_blocked_ell_repack_config {
+    unsigned int output_rows;
+    unsigned int output_cols;
+    unsigned int block_size;
+    unsigned int requested_bucket_count;
+} csamd_sharded_blocked_ell_repack_config;
+
+typedef struct CSAMD_ALIGN16 csamd_sharded_blocked_ell_repack_stats {
+    unsigned int kept_rows;
+    unsigned int output_rows;
+    unsigned int output_cols;
+    unsigned int live_nnz;
+    unsigned long long kept_row_slots;
+    unsigned long long dead_slots;
+    double live_fill_ratio;
+    size_t dead_value_bytes;
+    unsigned int ell_width;
+} csamd_sharded_blocked_ell_repack_stats;
+
+typedef struct CSAMD_ALIGN16 csamd_sharded_blocked_ell_repack_result {
+    csamd_compressed_view filtered;
+    csamd_compressed_major_bucket_result bucketed;
+    csamd_blocked_ell_view rebuilt;
+    csamd_major_nnz_bucket_plan_view bucket_plan;
+    csamd_sharded_blocked_ell_repack_stats stats;
+    unsigned long shard_id;
+} csamd_sharded_blocked_ell_repack_result;
+
+typedef struct CSAMD_ALIGN16 csamd_sharded_blocked_ell_repack_workspace {
+    int device;
+    csamd_stream_t stream;
+    int owns_stream;
+    void *private_workspace;
+} csamd_sharded_blocked_ell_repack_workspace;
+
+void csamd_repack_init(csamd_sharded_blocked_ell_repack_workspace *ws);
+void csamd_repack_clear(csamd_sharded_blocked_ell_repack_workspace *ws);
+int csamd_repack_setup(csamd_sharded_blocked_ell_repack_workspace *ws, int device, csamd_stream_t stream);
+int csamd_build_repacked_blocked_ell(const csamd_blocked_ell_view *parts, unsigned int part_count, unsigned int value_type,
+                                     const csamd_sharded_blocked_ell_repack_config *cfg, const unsigned char *d_keep_rows,
+                                     const unsigned char *d_keep_cols, const unsigned int *d_row_remap,
+                                     const unsigned int *d_col_remap, unsigned long shard_id,
+                                     csamd_sharded_blocked_ell_repack_workspace *ws, csamd_sharded_blocked_ell_repack_result *out);
+
 /* ---------------------------------------------------------------------------
  * Host matrix structs (reference layouts) and the host-struct converters.
  * formats/compressed.cuh:24-37, formats/blocked_ell.cuh:15-26,

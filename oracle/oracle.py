@@ -366,52 +366,111 @@ def _stored_nonzero(bits, vt):
     return (bits.view(np.uint16) & 0x7FFF) != 0
 
 
+def _filter_rows(part, vt, output_rows, output_cols, keep_rows, keep_cols, row_remap, col_remap, per, claimed, row0=0):
+    """Surviving entries of one Blocked-ELL part, per output row (in place
+    into `per`); returns (kept rows, kept rows' ell slots)."""
+    rows, cols, bs, ell_cols = part["rows"], part["cols"], part["block_size"], part["ell_cols"]
+    W = ell_cols // bs if bs else 0
+    bci = np.asarray(part["bci"], np.uint32).astype(np.int64)
+    vals = np.asarray(part["val"]).reshape(rows, ell_cols) if rows * ell_cols else None
+    kept = 0
+    e = np.arange(ell_cols)
+    slot, lane = (e // bs, e % bs) if bs else (e, e)
+    for r in range(rows):
+        if keep_rows is not None and keep_rows[r] == 0:
+            continue
+        kept += 1
+        o = int(row_remap[r]) if row_remap is not None else row0 + r  # shard row without a remap
+        if o >= output_rows or not output_cols:
+            continue
+        if o in claimed:
+            raise ValueError(f"two kept rows map to output row {o}")
+        claimed.add(o)
+        if vals is None:
+            continue
+        bc = bci[(r // bs) * W + slot]
+        valid = bc != INVALID
+        col = np.where(valid, bc * bs + lane, cols)
+        m = valid & (col < cols) & _stored_nonzero(vals[r], vt)
+        cs = np.minimum(col, max(cols - 1, 0))
+        if keep_cols is not None:
+            m &= np.asarray(keep_cols)[cs] != 0
+        oc = np.asarray(col_remap, np.int64)[cs] if col_remap is not None else col
+        m &= oc < output_cols
+        per[o] = (oc[m].astype(np.uint32), vals[r][m])
+    return kept, kept * ell_cols
+
+
+def _csr_of(per, output_rows, dtype):
+    counts = np.array([len(per[o][0]) if o in per else 0 for o in range(output_rows)], np.int64)
+    ptr = np.zeros(output_rows + 1, np.uint32)
+    ptr[1:] = np.cumsum(counts)
+    idx = np.concatenate([per[o][0] for o in range(output_rows) if o in per] or [np.zeros(0, np.uint32)])
+    val = np.concatenate([per[o][1] for o in range(output_rows) if o in per] or [np.zeros(0, dtype)])
+    return ptr, idx.astype(np.uint32), val.astype(dtype)
+
+
 def filter_blocked_ell(rows, cols, bs, ell_cols, bci, val_bits, vt, output_rows, output_cols, keep_rows=None, keep_cols=None,
                        row_remap=None, col_remap=None):
     """Masked Blocked-ELL -> CSR, restated from
     src/convert/filtered_blocked_ell_to_compressed.cuh:29-123 (which entries
-    survive, in ell order) and :264-373 (stats).  Row pointers are the plain
-    exclusive scan of the per-row counts: row i = [ptr[i], ptr[i + 1]).
+    survive, in ell order) and :264-373 (stats: kept_rows = output_rows,
+    slots = output_rows x ell_cols).  Row pointers are the plain exclusive
+    scan of the per-row counts: row i = [ptr[i], ptr[i + 1]).
     Returns (ptr, idx, val_bits, stats dict)."""
-    W = ell_cols // bs if bs else 0
-    bci = np.asarray(bci, np.uint32).astype(np.int64)
-    vals = np.asarray(val_bits).reshape(rows, ell_cols) if rows * ell_cols else np.zeros((rows, 0), np.asarray(val_bits).dtype)
-    per = [(np.zeros(0, np.uint32), vals[:0, 0] if ell_cols else np.zeros(0, vals.dtype))] * output_rows
-    claimed = set()
-    live = 0
+    val_bits = np.asarray(val_bits)
+    part = {"rows": rows, "cols": cols, "block_size": bs, "ell_cols": ell_cols, "bci": bci, "val": val_bits}
+    per = {}
     if output_rows and output_cols:
-        e = np.arange(ell_cols)
-        slot, lane = (e // bs, e % bs) if bs else (e, e)
-        for r in range(rows):
-            if keep_rows is not None and keep_rows[r] == 0:
-                continue
-            o = int(row_remap[r]) if row_remap is not None else r
-            if o >= output_rows:
-                continue
-            if o in claimed:
-                raise ValueError(f"two kept rows map to output row {o}")
-            claimed.add(o)
-            bc = bci[(r // bs) * W + slot]
-            valid = bc != INVALID
-            col = np.where(valid, bc * bs + lane, cols)
-            m = valid & (col < cols) & _stored_nonzero(vals[r], vt)
-            cs = np.minimum(col, max(cols - 1, 0))
-            if keep_cols is not None:
-                m &= np.asarray(keep_cols)[cs] != 0
-            oc = np.asarray(col_remap, np.int64)[cs] if col_remap is not None else col
-            m &= oc < output_cols
-            per[o] = (oc[m].astype(np.uint32), vals[r][m])
-            live += int(m.sum())
-    counts = np.array([len(a) for a, _ in per], np.int64)
-    ptr = np.zeros(output_rows + 1, np.uint32)
-    ptr[1:] = np.cumsum(counts)
-    idx = np.concatenate([a for a, _ in per]) if per else np.zeros(0, np.uint32)
-    val = np.concatenate([v for _, v in per]) if per else np.zeros(0, vals.dtype)
+        _filter_rows(part, vt, output_rows, output_cols, keep_rows, keep_cols, row_remap, col_remap, per, set())
+    ptr, idx, val = _csr_of(per, output_rows, val_bits.dtype)
+    live = len(idx)
     slots = output_rows * ell_cols if output_rows and output_cols else 0
     stats = {"kept_rows": output_rows, "output_rows": output_rows, "output_cols": output_cols, "live_nnz": live,
              "kept_row_slots": slots, "dead_slots": max(slots - live, 0),
              "live_fill_ratio": 1.0 if slots == 0 else live / slots}
-    return ptr, idx.astype(np.uint32), val, stats
+    return ptr, idx, val, stats
+
+
+def repack_blocked_ell(parts, vt, output_rows, output_cols, block_size, requested_bucket_count, keep_rows=None, keep_cols=None,
+                       row_remap=None, col_remap=None):
+    """Shard repack, restated from src/repack/operators/sharded_blocked_ell.cuh:393-723
+    and repack/kernels/sharded_blocked_ell.cuh:23-128: the parts' rows in
+    shard order are filtered to CSR (kept rows counted over the shard, slots
+    priced at each kept row's own ell width), bucketed by stable ascending
+    nnz (equal-count cuts), and the bucketed rows packed as Blocked-ELL of
+    `block_size` (ascending block columns, W = widest row block).  Row
+    pointers as in filter_blocked_ell."""
+    dtype = np.asarray(parts[0]["val"]).dtype if parts else np.uint16
+    per, claimed = {}, set()
+    kept = slots = 0
+    r0 = 0
+    for part in parts:
+        n = part["rows"]
+        sl = slice(r0, r0 + n)
+        k, s_ = _filter_rows(part, vt, output_rows, output_cols, None if keep_rows is None else np.asarray(keep_rows)[sl], keep_cols,
+                             None if row_remap is None else np.asarray(row_remap)[sl], col_remap, per, claimed, r0)
+        kept, slots, r0 = kept + k, slots + s_, r0 + n
+    if not (output_rows and output_cols):
+        per = {}
+    ptr, idx, val = _csr_of(per, output_rows, dtype)
+    live = len(idx)
+    stats = {"kept_rows": kept, "output_rows": output_rows, "output_cols": output_cols, "live_nnz": live, "kept_row_slots": slots,
+             "dead_slots": max(slots - live, 0), "live_fill_ratio": 1.0 if slots == 0 else live / slots, "ell_width": 0}
+    out = {"ptr": ptr, "idx": idx, "val": val, "stats": stats}
+    if not (output_rows and output_cols and live):
+        return out
+    plan = bucket_plan(ptr, output_rows, requested_bucket_count)
+    order = plan["major_order"]
+    lens = np.diff(ptr.astype(np.int64))[order]
+    rptr = np.zeros(output_rows + 1, np.uint32)
+    rptr[1:] = np.cumsum(lens)
+    ridx = np.concatenate([idx[ptr[o]:ptr[o + 1]] for o in order])
+    rval = np.concatenate([val[ptr[o]:ptr[o + 1]] for o in order])
+    bci, bv, ell, nnz = blocked_ell_from_compressed(output_rows, output_cols, rptr, ridx, rval, block_size)
+    stats["ell_width"] = ell // block_size
+    out.update(plan=plan, rptr=rptr, ridx=ridx, rval=rval, bell_bci=bci, bell_val=bv, bell_ell_cols=ell)
+    return out
 
 
 # ---------------------------------------------------------------------------

@@ -126,3 +126,63 @@ def test_bucketed_filtered_view(ws, fws, buckets):
     for k, r in enumerate(order):  # rebuilt row k = filtered row order[k]
         assert np.array_equal(bi[bp[k]:bp[k + 1]], ri[rp[r]:rp[r + 1]])
         assert np.array_equal(bv[bp[k]:bp[k + 1]], rv[rp[r]:rp[r + 1]])
+
+
+@pytest.fixture(scope="module")
+def rws():
+    from cellshard_amd.ops import RepackWorkspace
+    w = RepackWorkspace()
+    yield w
+    w.close()
+
+
+def shard_parts(ws, vt, specs, seed):
+    parts, oparts = [], []
+    for k, (rows, bs) in enumerate(specs):
+        bell, gb, gv = make_bell(rows, 1100, bs, vt, seed + k, ws)
+        parts.append(bell)
+        oparts.append({"rows": rows, "cols": 1100, "block_size": bs, "ell_cols": bell.ell_cols, "bci": gb, "val": gv})
+    return parts, oparts
+
+
+@pytest.mark.parametrize("vt", [F16, F32])
+@pytest.mark.parametrize("remap", [True, False])
+def test_repack_shard_matches_oracle(ws, rws, vt, remap):
+    from cellshard_amd.ops import build_repacked_shard_blocked_ell
+    parts, oparts = shard_parts(ws, vt, [(300, 8), (257, 16), (411, 32)], 60 + vt)
+    rows = sum(p.rows for p in parts)
+    if remap:
+        rk, fk, rr, cr, orows, ocols = masks(rows, 1100, 9)
+        args = (dev_u8(rk), dev_u8(fk), dev_u32(rr), dev_u32(cr))
+        oargs = dict(keep_rows=rk, keep_cols=fk, row_remap=rr, col_remap=cr)
+    else:
+        orows, ocols, args, oargs = rows, 1100, (None, None, None, None), {}
+    out = build_repacked_shard_blocked_ell(parts, orows, ocols, 16, 6, *args, shard_id=3, ws=rws)
+    ref = O.repack_blocked_ell(oparts, vt, orows, ocols, 16, 6, **oargs)
+    torch.cuda.synchronize()
+    p, i, v = fetch_csr(out.filtered, vt)
+    assert np.array_equal(p, ref["ptr"]) and np.array_equal(i, ref["idx"]) and np.array_equal(v, ref["val"].view(NP_BITS[vt]))
+    s, rs = out.stats, ref["stats"]
+    assert (s.kept_rows, s.output_rows, s.output_cols, s.live_nnz, s.kept_row_slots, s.dead_slots, s.ell_width) == (
+        rs["kept_rows"], rs["output_rows"], rs["output_cols"], rs["live_nnz"], rs["kept_row_slots"], rs["dead_slots"], rs["ell_width"])
+    assert abs(s.live_fill_ratio - rs["live_fill_ratio"]) < 1e-12 and out.shard_id == 3
+    pl = out.bucket_plan
+    assert pl.bucket_count == ref["plan"]["bucket_count"]
+    assert np.array_equal(device_to_numpy(pl.major_order, orows, np.uint32), ref["plan"]["major_order"])
+    bp, bi, bv = fetch_csr(out.bucketed.rebuilt, vt)
+    assert np.array_equal(bp, ref["rptr"]) and np.array_equal(bi, ref["ridx"])
+    b = out.rebuilt
+    assert (b.rows, b.cols, b.block_size, b.ell_cols, b.nnz) == (orows, ocols, 16, ref["bell_ell_cols"], rs["live_nnz"])
+    rb = (orows + 15) // 16
+    w = b.ell_cols // 16
+    assert np.array_equal(device_to_numpy(b.blockColIdx, rb * w, np.uint32), ref["bell_bci"])
+    assert np.array_equal(device_to_numpy(b.val, orows * b.ell_cols, NP_BITS[vt]), ref["bell_val"].view(NP_BITS[vt]))
+
+
+def test_repack_nothing_survives(ws, rws):
+    from cellshard_amd.ops import build_repacked_shard_blocked_ell
+    parts, oparts = shard_parts(ws, F16, [(100, 8), (60, 8)], 5)
+    rk = np.zeros(160, np.uint8)
+    out = build_repacked_shard_blocked_ell(parts, 50, 1100, 8, 4, keep_rows=dev_u8(rk), ws=rws)
+    assert out.stats.kept_rows == 0 and out.stats.live_nnz == 0 and out.rebuilt.ell_cols == 0
+    assert out.bucket_plan.bucket_count == 1 and out.bucketed.rebuilt.majorPtr == out.filtered.majorPtr

@@ -422,6 +422,41 @@ def test_filter_blocked_ell_matches_reoptimize_restatement():
     assert np.array_equal(fp, mp) and np.array_equal(fi, mi) and np.array_equal(fv, mv) and st["live_nnz"] == n2
 
 
+def _bell_dense(rows, cols, bs, ell_cols, bci, val):
+    d = np.zeros((rows, cols), np.float64)
+    W = ell_cols // bs if bs else 0
+    v = np.asarray(val).view(np.float16).astype(np.float64).reshape(rows, ell_cols) if rows * ell_cols else None
+    for r in range(rows):
+        for s_ in range(W):
+            c0 = int(bci[(r // bs) * W + s_])
+            if c0 == O.INVALID:
+                continue
+            for l in range(bs):
+                if c0 * bs + l < cols:
+                    d[r, c0 * bs + l] = v[r, s_ * bs + l]
+    return d
+
+
+def test_repack_is_a_row_permutation_of_the_kept_matrix():
+    # two parts of one shard, all rows and columns kept, identity maps: the
+    # repacked Blocked-ELL's row k is the shard's row order[k], order = the
+    # stable ascending-nnz bucket order; kept rows / slots by closed form
+    parts, dense = [], []
+    for k, (rows, bs) in enumerate(((37, 4), (29, 8))):
+        ptr, idx, val = random_csr(rows, 90, 0.1, seed=50 + k, zeros=0.0)
+        bits = f16_bits(val)
+        bci, v, ell, nnz = O.blocked_ell_from_compressed(rows, 90, ptr, idx, bits, bs)
+        parts.append({"rows": rows, "cols": 90, "block_size": bs, "ell_cols": ell, "bci": bci, "val": v})
+        dense.append(_bell_dense(rows, 90, bs, ell, bci, v))
+    D = np.vstack(dense)
+    out = O.repack_blocked_ell(parts, O.VT_F16, 66, 90, 16, 5)
+    order = out["plan"]["major_order"]
+    R = _bell_dense(66, 90, 16, out["bell_ell_cols"], out["bell_bci"], out["bell_val"])
+    assert np.array_equal(R, D[order])
+    assert out["stats"]["kept_rows"] == 66 and out["stats"]["kept_row_slots"] == sum(p["rows"] * p["ell_cols"] for p in parts)
+    assert np.array_equal(np.diff(out["ptr"].astype(np.int64))[order], np.sort(np.diff(out["ptr"].astype(np.int64)), kind="stable"))
+
+
 def test_planted_mistake_is_caught():
     """A dropped term in the oracle (simulated by zeroing one entry) must fail
     the dense comparison used above."""
