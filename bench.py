@@ -313,7 +313,8 @@ def run_plain_path(args, log):
     from cellshard_amd import abi as A
     from cellshard_amd.device import to_dev_u32, to_dev_u8
     from cellshard_amd.ops import (ConvertWorkspace, FilteredBlockedEllWorkspace, LabelReduce, RepackWorkspace, TransposeScratch,
-                                   blocked_ell_from_compressed, build_repacked_shard_blocked_ell, compressed_transpose,
+                                   blocked_ell_from_compressed, build_repacked_shard_blocked_ell, build_shard_column_maps,
+                                   compressed_transpose,
                                    filter_blocked_ell_to_compressed, synth_csr)
     from oracle import oracle as O
 
@@ -356,6 +357,8 @@ def run_plain_path(args, log):
     ro, repack_ms = gpu_ms(lambda: build_repacked_shard_blocked_ell([bell], *f_args[1:3], 16, 12, *f_args[3:], ws=rws))
     repack_ell_width = int(ro.stats.ell_width)
     rws.close()
+    # column order by row-block support signature over the same Blocked-ELL
+    _, colmap_ms = gpu_ms(lambda: build_shard_column_maps([bell], Cc, ws), reps=3)
     # algorithmic bytes: the kept value rows and the slot table read once, the CSR written
     # (the count and emit passes each read the kept rows)
     filt_bytes = int(rk.sum()) * bell.ell_cols * 4 + bell.row_blocks * (bell.ell_cols // 16) * 4 + filt_live * 8 + (int(rk.sum()) + 1) * 4
@@ -377,6 +380,7 @@ def run_plain_path(args, log):
                    "blocked_ell_bytes_over_csr": round(bell_bytes / csr_bytes, 2),
                    "filter_bell_to_csr_ms": round(filt_ms, 3), "filter_bell_to_csr_gb_s": round(filt_bytes / (filt_ms * 1e-3) / 1e9, 1),
                    "filter_live_nnz": filt_live, "repack_shard_bell16_ms": round(repack_ms, 3), "repack_ell_width": repack_ell_width,
+                   "shard_column_maps_ms": round(colmap_ms, 3),
                    "transpose_ms": round(tr_ms, 3), "transpose_gb_s": round(2 * csr_bytes / (tr_ms * 1e-3) / 1e9, 1),
                    "reduce_ms": round(red_ms, 4), "reduce_gb_s": round(red_bytes / (red_ms * 1e-3) / 1e9, 1)}}
     if not args.skip_cpu:

@@ -64,6 +64,9 @@ PROTOTYPES = {
         P(A.filtered_blocked_ell_workspace), P(A.filtered_blocked_ell_result)]),
     "csamd_build_bucketed_filtered_blocked_ell_major_view": (i32, [P(A.blocked_ell_view), u32, u32, u32, vp, vp, vp, vp, u32,
         P(A.filtered_blocked_ell_workspace), P(A.filtered_blocked_ell_result)]),
+    "csamd_build_shard_column_maps": (i32, [P(A.blocked_ell_view), u32, u32, u32, vp, vp, P(A.convert_workspace)]),
+    "csamd_build_column_order_from_sampled_coo": (i32, [P(A.host_coo), vp, i32, vp, vp]),
+    "csamd_build_row_order_from_coo": (i32, [P(A.host_coo), vp, i32, vp, vp]),
     "csamd_repack_init": (None, [P(A.sharded_blocked_ell_repack_workspace)]),
     "csamd_repack_clear": (None, [P(A.sharded_blocked_ell_repack_workspace)]),
     "csamd_repack_setup": (i32, [P(A.sharded_blocked_ell_repack_workspace), i32, vp]),

@@ -36,6 +36,7 @@ HIP_SOURCES = [
     "coo_convert.hip",
     "execution_layout.hip",
     "filter.hip",
+    "column_order.hip",
 ]
 
 HIPCC_FLAGS = [

@@ -179,6 +179,52 @@ def build_repacked_shard_blocked_ell(parts: list, output_rows: int, output_cols:
     return out
 
 
+def build_shard_column_maps(parts: list, cols: int, ws: ConvertWorkspace):
+    """Column order by row-block support signature over a shard's Blocked-ELL
+    partitions (convert_layout_support_part.hh:20-86).  Returns host
+    (exec_to_canonical, canonical_to_exec) uint32 arrays."""
+    vts = {b.value_type for b in parts}
+    if len(vts) > 1:
+        raise ValueError("build_shard_column_maps: partitions of different value types")
+    views = (A.blocked_ell_view * max(len(parts), 1))(*[b.view() for b in parts])
+    e2c = np.zeros(max(cols, 1), np.uint32)
+    c2e = np.zeros(max(cols, 1), np.uint32)
+    check(load().csamd_build_shard_column_maps(views, len(parts), vts.pop() if vts else A.VALUE_F16, cols,
+                                              e2c.ctypes.data_as(C.c_void_p), c2e.ctypes.data_as(C.c_void_p), C.byref(ws.ws)),
+          "build_shard_column_maps")
+    return e2c[:cols], c2e[:cols]
+
+
+def _coo_view(rows: int, cols: int, row_idx: np.ndarray, col_idx: np.ndarray):
+    r = np.ascontiguousarray(row_idx, np.uint32)
+    c = np.ascontiguousarray(col_idx, np.uint32)
+    coo = A.host_coo(rows=rows, cols=cols, nnz=len(r), storage=None, rowIdx=r.ctypes.data_as(C.c_void_p),
+                     colIdx=c.ctypes.data_as(C.c_void_p), val=None)
+    return coo, (r, c)
+
+
+def build_column_order_from_sampled_coo(rows: int, cols: int, row_idx, col_idx, sample_row_rank, device: int = 0):
+    """Columns by mean rank of their sampled rows (blocked_ell_bipartite_optimize.cuh:361-434)."""
+    coo, keep = _coo_view(rows, cols, row_idx, col_idx)
+    rk = np.ascontiguousarray(sample_row_rank, np.uint32)
+    e2c, c2e = np.zeros(max(cols, 1), np.uint32), np.zeros(max(cols, 1), np.uint32)
+    check(load().csamd_build_column_order_from_sampled_coo(C.byref(coo), rk.ctypes.data_as(C.c_void_p), device,
+                                                          e2c.ctypes.data_as(C.c_void_p), c2e.ctypes.data_as(C.c_void_p)),
+          "build_column_order_from_sampled_coo")
+    return e2c[:cols], c2e[:cols]
+
+
+def build_row_order_from_coo(rows: int, cols: int, row_idx, col_idx, canonical_to_exec_cols, device: int = 0):
+    """Rows by mean exec rank of their columns (blocked_ell_bipartite_optimize.cuh:436-500)."""
+    coo, keep = _coo_view(rows, cols, row_idx, col_idx)
+    ck = np.ascontiguousarray(canonical_to_exec_cols, np.uint32)
+    e2c, c2e = np.zeros(max(rows, 1), np.uint32), np.zeros(max(rows, 1), np.uint32)
+    check(load().csamd_build_row_order_from_coo(C.byref(coo), ck.ctypes.data_as(C.c_void_p), device, e2c.ctypes.data_as(C.c_void_p),
+                                               c2e.ctypes.data_as(C.c_void_p)),
+          "build_row_order_from_coo")
+    return e2c[:rows], c2e[:rows]
+
+
 # ---------------------------------------------------------------------------
 # Sliced-ELL
 # ---------------------------------------------------------------------------

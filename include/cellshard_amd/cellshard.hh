@@ -28,6 +28,7 @@
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
+#include <vector>
 
 #if !defined(__HIPCC__)
 #if defined(__has_include) && __has_include(<hip/hip_runtime_api.h>) && defined(__HIP_PLATFORM_AMD__)
@@ -572,6 +573,34 @@ static inline int rebuild_compressed_major_order_raw(const types::ptr_t *d_src_m
     return csamd_rebuild_compressed_major_order_raw(d_src_major_ptr, d_src_minor_idx, d_src_val, CSAMD_VALUE_F16, major_dim,
                                                     d_major_order, d_dst_major_ptr, d_dst_minor_idx, d_dst_val,
                                                     d_inverse_major_order, d_scan_tmp, scan_tmp_bytes, (csamd_stream_t) stream);
+}
+
+// Column / row orders by rank barycenter (operators/blocked_ell_bipartite_optimize.cuh:361-500).
+// The workspace and stream arguments are accepted for signature
+// compatibility; the library manages its own buffers and stream.
+inline int build_column_order_from_sampled_coo(const sparse::coo *sampled, const std::uint32_t *sample_row_rank, int device,
+                                               std::vector<std::uint32_t> *exec_to_canonical_cols,
+                                               std::vector<std::uint32_t> *canonical_to_exec_cols, void *workspace = nullptr,
+                                               hipStream_t stream = (hipStream_t) 0) {
+    (void) workspace;
+    (void) stream;
+    if (sampled == nullptr || exec_to_canonical_cols == nullptr || canonical_to_exec_cols == nullptr) return 0;
+    exec_to_canonical_cols->assign(sampled->cols, 0u);
+    canonical_to_exec_cols->assign(sampled->cols, 0u);
+    return csamd_build_column_order_from_sampled_coo(sampled, sample_row_rank, device, exec_to_canonical_cols->data(),
+                                                     canonical_to_exec_cols->data());
+}
+inline int build_row_order_from_coo(const sparse::coo *src, const std::uint32_t *canonical_to_exec_cols, int device,
+                                    std::vector<std::uint32_t> *exec_to_canonical_rows,
+                                    std::vector<std::uint32_t> *canonical_to_exec_rows, void *workspace = nullptr,
+                                    hipStream_t stream = (hipStream_t) 0) {
+    (void) workspace;
+    (void) stream;
+    if (src == nullptr || exec_to_canonical_rows == nullptr || canonical_to_exec_rows == nullptr) return 0;
+    exec_to_canonical_rows->assign(src->rows, 0u);
+    canonical_to_exec_rows->assign(src->rows, 0u);
+    return csamd_build_row_order_from_coo(src, canonical_to_exec_cols, device, exec_to_canonical_rows->data(),
+                                          canonical_to_exec_rows->data());
 }
 
 } // namespace bucket

@@ -969,6 +969,25 @@ int csamd_blocked_ell_from_coo(const csamd_host_coo *src, unsigned int cols, con
 int csamd_blocked_ell_from_coo_auto(const csamd_host_coo *src, unsigned int cols, const unsigned int *feature_to_global,
                                     const unsigned int *candidates, unsigned int candidate_count, csamd_host_blocked_ell *dst,
                                     int device, csamd_blocked_ell_tune_result *picked);
+
+/* Column / row orders against Blocked-ELL padding.
+ * ingest/internal/dataset/internal/convert_layout_support_part.hh:20-86
+ * (build_shard_column_maps): columns ordered by their row-block support
+ * signature over the shard's Blocked-ELL partitions (views in shard order;
+ * `cols` columns); host outputs [cols].  Synchronous. */
+int csamd_build_shard_column_maps(const csamd_blocked_ell_view *parts, unsigned int part_count, unsigned int value_type,
+                                  unsigned int cols, unsigned int *h_exec_to_canonical, unsigned int *h_canonical_to_exec,
+                                  csamd_convert_workspace *ws);
+/* bucket/operators/blocked_ell_bipartite_optimize.cuh:361-434: columns
+ * ordered by the mean rank of their sampled rows (h_sample_row_rank [rows],
+ * 0xFFFFFFFF: unranked); host outputs [cols].  :436-500: rows ordered by
+ * the mean exec rank of their columns; host outputs [rows].  Rank sums are
+ * exact (the reference accumulates them in float).  Synchronous. */
+int csamd_build_column_order_from_sampled_coo(const csamd_host_coo *sampled, const unsigned int *h_sample_row_rank, int device,
+                                              unsigned int *h_exec_to_canonical_cols, unsigned int *h_canonical_to_exec_cols);
+int csamd_build_row_order_from_coo(const csamd_host_coo *src, const unsigned int *h_canonical_to_exec_cols, int device,
+                                   unsigned int *h_exec_to_canonical_rows, unsigned int *h_canonical_to_exec_rows);
+
 /* Device COO -> device CSR (the sort those converters start with; project extension, the
  * reference's compressed_from_coo_raw.cuh:97-225 has the same role): stable by (row, column),
  * optional column remap, any value type; out_ptr[rows + 1], out_idx/out_val[nnz]. */

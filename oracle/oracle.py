@@ -473,6 +473,92 @@ def repack_blocked_ell(parts, vt, output_rows, output_cols, block_size, requeste
     return out
 
 
+_FNV_PRIME = 1099511628211
+_M64 = (1 << 64) - 1
+
+
+def _mix_signature(seed, value):
+    for k in range(8):
+        seed ^= (value >> (8 * k)) & 0xFF
+        seed = (seed * _FNV_PRIME) & _M64
+    return seed
+
+
+def shard_column_maps(parts, vt, cols):
+    """Column order by row-block support signature, restated from
+    include/CellShard/ingest/internal/dataset/internal/convert_layout_support_part.hh:20-86.
+    parts: Blocked-ELL dicts (rows, block_size, ell_cols, bci, val bits) in
+    shard order.  Returns (exec_to_canonical, canonical_to_exec)."""
+    support = [0] * cols
+    ha = [1469598103934665603] * cols
+    hb = [_FNV_PRIME] * cols
+    first = [INVALID] * cols
+    g = 0
+    for part in parts:
+        rows, bs, ell = part["rows"], part["block_size"], part["ell_cols"]
+        W = ell // bs if bs else 0
+        bci = np.asarray(part["bci"], np.uint32)
+        vals = np.asarray(part["val"]).reshape(rows, ell) if rows * ell else None
+        nzm = _stored_nonzero(vals, vt) if vals is not None else None
+        for rb in range((rows + bs - 1) // bs if bs else 0):
+            r0, r1 = rb * bs, min(rows, rb * bs + bs)
+            for slot in range(W):
+                bc = int(bci[rb * W + slot])
+                if bc == INVALID:
+                    continue
+                for lane in range(bs):
+                    col = bc * bs + lane
+                    if col >= cols or not nzm[r0:r1, slot * bs + lane].any():
+                        continue
+                    support[col] += 1
+                    ha[col] = _mix_signature(ha[col], g + 1)
+                    hb[col] = _mix_signature(hb[col], ((g + 1) * 1315423911) & _M64)
+                    first[col] = min(first[col], g)
+            g += 1
+    order = sorted(range(cols), key=lambda c: (support[c] == 0, first[c], ha[c], hb[c], -support[c], c))
+    e2c = np.array(order, np.uint32)
+    c2e = np.zeros(cols, np.uint32)
+    c2e[e2c] = np.arange(cols, dtype=np.uint32)
+    return e2c, c2e
+
+
+def _ordered_f32(v):
+    bits = np.asarray(v, np.float32).view(np.uint32).astype(np.uint64)
+    return np.where(bits & 0x80000000, bits ^ 0xFFFFFFFF, bits ^ 0x80000000)
+
+
+def barycenter_order(rows, cols, row_idx, col_idx, other_rank, by_col, float_sums=False):
+    """Entities (columns when by_col, else rows) sorted by the mean rank of
+    the entities they touch, then by index; FLT_MAX when they touch none.
+    Restated from src/bucket/operators/blocked_ell_bipartite_optimize.cuh:223-309.
+    float_sums: accumulate in float32 in COO order as the reference's atomics
+    would in that order; otherwise exact sums rounded once.
+    Returns (exec_to_canonical, canonical_to_exec)."""
+    n = cols if by_col else rows
+    ent = np.asarray(col_idx if by_col else row_idx, np.int64)
+    oth = np.asarray(row_idx if by_col else col_idx, np.int64)
+    rank = np.asarray(other_rank, np.uint64)[oth] if len(oth) else np.zeros(0, np.uint64)
+    ok = rank != INVALID
+    count = np.bincount(ent[ok], minlength=n).astype(np.uint32)
+    if float_sums:
+        s_ = np.zeros(n, np.float32)
+        for e_, r_ in zip(ent[ok], rank[ok]):
+            s_[e_] = np.float32(s_[e_] + np.float32(r_))
+        sums = s_
+    else:
+        sums = np.bincount(ent[ok], weights=None, minlength=n) * 0
+        acc = np.zeros(n, np.uint64)
+        np.add.at(acc, ent[ok], rank[ok])
+        sums = acc.astype(np.float32)  # one rounding of the exact integer
+    with np.errstate(divide="ignore", invalid="ignore"):
+        bary = np.where(count != 0, np.float32(sums) / np.float32(count), np.float32(np.finfo(np.float32).max)).astype(np.float32)
+    keys = (_ordered_f32(bary) << np.uint64(32)) | np.arange(n, dtype=np.uint64)
+    e2c = np.argsort(keys, kind="stable").astype(np.uint32)
+    c2e = np.zeros(n, np.uint32)
+    c2e[e2c] = np.arange(n, dtype=np.uint32)
+    return e2c, c2e
+
+
 # ---------------------------------------------------------------------------
 # Reference CPU converter (oracle/_ref), fp16 values.
 def ref_choose_blocked_ell_block_size(rows, cols, ptr, idx, val16, candidates):

@@ -457,6 +457,58 @@ def test_repack_is_a_row_permutation_of_the_kept_matrix():
     assert np.array_equal(np.diff(out["ptr"].astype(np.int64))[order], np.sort(np.diff(out["ptr"].astype(np.int64)), kind="stable"))
 
 
+def _fnv1a64(seed, data: bytes):
+    for b in data:
+        seed = ((seed ^ b) * 1099511628211) & ((1 << 64) - 1)
+    return seed
+
+
+def test_column_signature_mix_is_fnv1a():
+    # mix_signature is FNV-1a 64 over the value's 8 little-endian bytes: the
+    # published test vector for "a", then the 8-byte form
+    assert _fnv1a64(0xCBF29CE484222325, b"a") == 0xAF63DC4C8601EC8C
+    for seed, v in ((1469598103934665603, 1), (1099511628211, 1315423911 * 7), (12345, (1 << 64) - 1)):
+        assert O._mix_signature(seed, v) == _fnv1a64(seed, v.to_bytes(8, "little"))
+
+
+def test_shard_column_maps_known_answer():
+    # 2 row blocks of 2 rows, B = 2, 7 columns.  Row block 0 touches columns
+    # 0, 1, 4, 5 (column 1 only through row 1), row block 1 columns 2, 3, 4, 5;
+    # column 6 never.  Columns with the same support set tie on every key but
+    # the column: (0, 1), (2, 3), (4, 5) stay adjacent and in order; the
+    # set {0} and the set {0, 1} share the first row block, so their hashes
+    # decide; {1} comes after both; the unsupported column is last.
+    part = {"rows": 4, "block_size": 2, "ell_cols": 4, "bci": np.array([0, 2, 1, 2], np.uint32),
+            "val": np.array([1, 0, 1, 1, 0, 1, 0, 0, 1, 1, 1, 0, 0, 0, 0, 1], np.uint16) * 0x3C00}
+    e2c, c2e = O.shard_column_maps([part], O.VT_F16, 7)
+    init = 1469598103934665603
+    h01 = O._mix_signature(init, 1)
+    h45 = O._mix_signature(h01, 2)
+    first = [0, 1, 4, 5] if h01 < h45 else [4, 5, 0, 1]
+    assert list(e2c) == first + [2, 3, 6]
+    assert np.array_equal(c2e[e2c], np.arange(7))
+
+
+def test_barycenter_order_known_answer_and_float_sums():
+    # columns touched by rows of ranks {0, 2} (mean 1), {3} (3), {1} (1),
+    # none (FLT_MAX): order by mean, then column
+    rows, cols = 4, 4
+    ri = np.array([0, 2, 3, 1], np.uint32)
+    ci = np.array([0, 0, 1, 2], np.uint32)
+    rank = np.array([0, 1, 2, 3], np.uint32)
+    e2c, c2e = O.barycenter_order(rows, cols, ri, ci, rank, by_col=True)
+    assert list(e2c) == [0, 2, 1, 3]
+    # with sums below 2^24 the reference's float accumulation gives the same order
+    rng = np.random.default_rng(2)
+    ri = rng.integers(0, 500, 4000).astype(np.uint32)
+    ci = rng.integers(0, 300, 4000).astype(np.uint32)
+    rank = rng.permutation(500).astype(np.uint32)
+    rank[rng.random(500) < 0.1] = O.INVALID
+    a = O.barycenter_order(500, 300, ri, ci, rank, by_col=True)
+    b = O.barycenter_order(500, 300, ri, ci, rank, by_col=True, float_sums=True)
+    assert np.array_equal(a[0], b[0])
+
+
 def test_planted_mistake_is_caught():
     """A dropped term in the oracle (simulated by zeroing one entry) must fail
     the dense comparison used above."""
