@@ -96,6 +96,17 @@ PROTOTYPES = {
     "csamd_build_bucketed_sliced_ell": (i32, [P(A.sliced_ell_view), u32, u32, P(A.convert_workspace), P(A.bucketed_sliced_ell)]),
     "csamd_choose_bucket_count_sliced_ell": (i32, [P(A.sliced_ell_view), P(A.convert_workspace), P(u32), P(C.c_uint64)]),
     "csamd_bucketed_sliced_ell_clear": (None, [P(A.bucketed_sliced_ell)]),
+    # CSPACK01 shard packs
+    "csamd_cspack_init": (None, [P(A.cspack_workspace)]),
+    "csamd_cspack_clear": (None, [P(A.cspack_workspace)]),
+    "csamd_cspack_setup": (i32, [P(A.cspack_workspace), i32, vp]),
+    "csamd_cspack_write_blocked_ell": (i32, [C.c_char_p, u64, P(A.bucketed_blocked_ell), u32, u32, P(A.cspack_workspace)]),
+    "csamd_cspack_write_sliced_ell": (i32, [C.c_char_p, u64, P(A.bucketed_sliced_ell), u32, u32, P(A.cspack_workspace)]),
+    "csamd_cspack_open": (i32, [C.c_char_p, u64, P(A.cspack_file)]),
+    "csamd_cspack_close": (None, [P(A.cspack_file)]),
+    "csamd_cspack_stage_blocked_ell": (i32, [P(A.cspack_file), u64, u32, P(A.cspack_workspace), P(A.bucketed_blocked_ell)]),
+    "csamd_cspack_stage_sliced_ell": (i32, [P(A.cspack_file), u64, u32, P(A.cspack_workspace), P(A.bucketed_sliced_ell)]),
+    "csamd_cspack_stage_shard_blocked_ell": (i32, [P(A.cspack_file), u32, P(A.cspack_workspace), P(A.bucketed_blocked_ell)]),
     "csamd_compressed_from_coo_raw": (i32, [u32, u32, u32, vp, vp, vp, u32, vp, u32, vp, vp, vp, vp, sz, vp]),
     "csamd_label_reduce_set_timing": (i32, [P(A.label_reduce_workspace), i32]),
     "csamd_serve_init": (None, [P(A.serve_workspace)]),

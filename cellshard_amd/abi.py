@@ -283,6 +283,18 @@ partition_tune_result = _aligned16("partition_tune_result", [
     ("row_length_log2_hist", c_u32 * 32),
 ])
 
+CSPACK_ANY_SHARD = 0xFFFFFFFFFFFFFFFF
+
+cspack_file = _aligned16("cspack_file", [
+    ("shard_id", C.c_uint64), ("partition_count", C.c_uint64), ("file_bytes", C.c_uint64),
+    ("partition_offsets", C.POINTER(C.c_uint64)), ("partition_bytes", C.POINTER(C.c_uint64)), ("impl", c_void_p),
+])
+
+cspack_workspace = _aligned16("cspack_workspace", [
+    ("device", C.c_int), ("reader_threads", c_u32), ("stream", c_void_p), ("owns_stream", C.c_int),
+    ("chunk_bytes", c_u32), ("private_workspace", c_void_p),
+])
+
 # Structs whose layout tests/test_abi.py compares against the C header.
 ALL_STRUCTS = {
     "csamd_compressed_view": compressed_view,
@@ -311,6 +323,8 @@ ALL_STRUCTS = {
     "csamd_host_coo": host_coo,
     "csamd_bucketed_blocked_ell": bucketed_blocked_ell,
     "csamd_bucketed_sliced_ell": bucketed_sliced_ell,
+    "csamd_cspack_file": cspack_file,
+    "csamd_cspack_workspace": cspack_workspace,
     "csamd_convert_workspace": convert_workspace,
     "csamd_blocked_ell_tune_result": blocked_ell_tune_result,
     "csamd_sliced_ell_tune_result": sliced_ell_tune_result,

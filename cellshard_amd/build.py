@@ -37,6 +37,7 @@ HIP_SOURCES = [
     "execution_layout.hip",
     "filter.hip",
     "column_order.hip",
+    "cspack.hip",
 ]
 
 HIPCC_FLAGS = [

@@ -45,6 +45,18 @@ int ensure_device(int device);
 // Number of compute units on the current device (cached per device).
 int device_cu_count();
 
+// Owner of a bucketed execution partition's memory (the `storage` field of
+// csamd_bucketed_blocked_ell / csamd_bucketed_sliced_ell): device and host
+// blocks that the csamd_bucketed_*_clear calls free.  exec_storage_dev and
+// exec_storage_host return nullptr when the allocation fails.
+void *exec_storage_new();
+void *exec_storage_dev(void *storage, size_t bytes);
+void *exec_storage_host(void *storage, size_t bytes);
+// The reference's packed-byte estimates of one Blocked-ELL / Sliced-ELL
+// segment with 2-byte values (the `bytes` field of the bucketed partitions).
+unsigned long long exec_bell_packed_bytes(unsigned long long rows, unsigned long long ell_cols, unsigned long long block_size);
+unsigned long long exec_sell_packed_bytes(unsigned long long slices, unsigned long long slots);
+
 // ---------------------------------------------------------------------------
 // Value decoding (all to fp32).
 // ---------------------------------------------------------------------------

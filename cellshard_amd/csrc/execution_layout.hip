@@ -487,6 +487,16 @@ struct el_state {
 };
 
 } // namespace
+
+unsigned long long exec_bell_packed_bytes(unsigned long long rows, unsigned long long ell_cols, unsigned long long block_size) {
+    return bell_packed_bytes(rows, ell_cols, block_size);
+}
+unsigned long long exec_sell_packed_bytes(unsigned long long slices, unsigned long long slots) { return sell_packed_bytes(slices, slots); }
+
+void *exec_storage_new() { return new el_state(); }
+void *exec_storage_dev(void *storage, size_t bytes) { return ((el_state *) storage)->mem.d(bytes); }
+void *exec_storage_host(void *storage, size_t bytes) { return ((el_state *) storage)->mem.h(bytes); }
+
 } // namespace csamd
 
 using namespace csamd;

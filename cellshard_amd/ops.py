@@ -446,6 +446,212 @@ def choose_bucket_count_sliced_ell(sell: DeviceSlicedELL, ws: ConvertWorkspace):
     return int(n.value), int(b.value)
 
 
+# ---------------------------------------------------------------------------
+# Device-resident bucketed execution partitions and CSPACK01 shard packs
+# ---------------------------------------------------------------------------
+def _sliced_segment_to_host(sv, dt) -> dict:
+    n = int(sv.slice_count)
+    offs = device_to_numpy(sv.slice_row_offsets, n + 1, np.uint32) if sv.slice_row_offsets else np.zeros(1, np.uint32)
+    wid = device_to_numpy(sv.slice_widths, n, np.uint32)
+    slot = device_to_numpy(sv.slice_slot_offsets, n, np.uint32)
+    slots = int(((offs[1:].astype(np.int64) - offs[:-1]) * wid).sum()) if n else 0
+    return dict(rows=int(sv.rows), cols=int(sv.cols), nnz=int(sv.nnz), slice_count=n, slice_rows=int(sv.slice_rows),
+                slice_row_offsets=offs, slice_widths=wid, slice_slot_offsets=slot, width=int(wid[0]) if n else 0,
+                col_idx=device_to_numpy(sv.col_idx, slots, np.uint32), val=device_to_numpy(sv.val, slots, dt))
+
+
+class BucketedBlockedEll:
+    """An owned, device-resident bucketed Blocked-ELL execution partition
+    (csamd_bucketed_blocked_ell); freed by close()."""
+
+    def __init__(self, st, value_type: int):
+        self.st = st
+        self.value_type = value_type
+
+    def to_host(self) -> dict:
+        out, dt = self.st, _bits_dtype(self.value_type)
+        segs = []
+        for k in range(out.segment_count):
+            sv = out.segments[k]
+            rb = (sv.rows + sv.block_size - 1) // sv.block_size if sv.block_size else 0
+            w = sv.ell_cols // sv.block_size if sv.block_size else 0
+            segs.append(dict(rows=int(sv.rows), cols=int(sv.cols), nnz=int(sv.nnz), block_size=int(sv.block_size),
+                             ell_cols=int(sv.ell_cols), block_col_idx=device_to_numpy(sv.blockColIdx, rb * w, np.uint32),
+                             val=device_to_numpy(sv.val, sv.rows * sv.ell_cols, dt)))
+        return dict(rows=int(out.rows), cols=int(out.cols), nnz=int(out.nnz), segments=segs,
+                    segment_row_offsets=np.array([out.segment_row_offsets[k] for k in range(out.segment_count + 1)], np.uint32),
+                    exec_to_canonical=device_to_numpy(out.exec_to_canonical_rows, out.rows, np.uint32),
+                    canonical_to_exec=device_to_numpy(out.canonical_to_exec_rows, out.rows, np.uint32),
+                    col_maps=(device_to_numpy(out.exec_to_canonical_cols, out.cols, np.uint32),
+                              device_to_numpy(out.canonical_to_exec_cols, out.cols, np.uint32)),
+                    bytes=int(out.bytes))
+
+    def close(self):
+        if self.st is not None:
+            load().csamd_bucketed_blocked_ell_clear(C.byref(self.st))
+            self.st = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class BucketedSlicedEll:
+    """An owned, device-resident bucketed Sliced-ELL execution partition."""
+
+    def __init__(self, st, value_type: int):
+        self.st = st
+        self.value_type = value_type
+
+    def to_host(self) -> dict:
+        out, dt = self.st, _bits_dtype(self.value_type)
+        segs = [_sliced_segment_to_host(out.segments[k], dt) for k in range(out.segment_count)]
+        return dict(rows=int(out.rows), cols=int(out.cols), nnz=int(out.nnz), segments=segs,
+                    segment_row_offsets=np.array([out.segment_row_offsets[k] for k in range(out.segment_count + 1)], np.uint32),
+                    exec_to_canonical=device_to_numpy(out.exec_to_canonical_rows, out.rows, np.uint32),
+                    canonical_to_exec=device_to_numpy(out.canonical_to_exec_rows, out.rows, np.uint32),
+                    canonical_slice_row_offsets=device_to_numpy(out.canonical_slice_row_offsets, out.canonical_slice_count + 1,
+                                                                np.uint32),
+                    canonical_slice_widths=device_to_numpy(out.canonical_slice_widths, out.canonical_slice_count, np.uint32),
+                    bytes=int(out.bytes))
+
+    def close(self):
+        if self.st is not None:
+            load().csamd_bucketed_sliced_ell_clear(C.byref(self.st))
+            self.st = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def build_bucketed_blocked_ell_device(bell: DeviceBlockedELL, requested_bucket_count: int,
+                                      ws: ConvertWorkspace) -> BucketedBlockedEll:
+    out = A.bucketed_blocked_ell()
+    v = bell.view()
+    check(load().csamd_build_bucketed_blocked_ell(C.byref(v), bell.value_type, requested_bucket_count, C.byref(ws.ws), C.byref(out)),
+          "build_bucketed_blocked_ell")
+    return BucketedBlockedEll(out, bell.value_type)
+
+
+def build_bucketed_sliced_ell_device(sell: DeviceSlicedELL, requested_bucket_count: int, ws: ConvertWorkspace) -> BucketedSlicedEll:
+    out = A.bucketed_sliced_ell()
+    v = sell.view()
+    check(load().csamd_build_bucketed_sliced_ell(C.byref(v), sell.value_type, requested_bucket_count, C.byref(ws.ws), C.byref(out)),
+          "build_bucketed_sliced_ell")
+    return BucketedSlicedEll(out, sell.value_type)
+
+
+class CspackWorkspace:
+    """Owns a csamd_cspack_workspace (pinned and device staging buffers kept
+    between calls).  reader_threads / chunk_bytes of 0 take the library's
+    defaults (8 readers, 8 MiB pieces)."""
+
+    def __init__(self, device: int | None = None, reader_threads: int = 0, chunk_bytes: int = 0, stream: int | None = None):
+        lib = load()
+        self.ws = A.cspack_workspace()
+        lib.csamd_cspack_init(C.byref(self.ws))
+        self.ws.reader_threads = reader_threads
+        self.ws.chunk_bytes = chunk_bytes
+        dev = torch.cuda.current_device() if device is None else device
+        check(lib.csamd_cspack_setup(C.byref(self.ws), dev, stream), "cspack setup")
+
+    def close(self):
+        if self.ws is not None:
+            load().csamd_cspack_clear(C.byref(self.ws))
+            self.ws = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class CspackFile:
+    """An open CSPACK01 file (csamd_cspack_open): header checked, offsets
+    read; `shard_id` None accepts any shard."""
+
+    def __init__(self, path, shard_id: int | None = None):
+        self.f = A.cspack_file()
+        sid = A.CSPACK_ANY_SHARD if shard_id is None else int(shard_id)
+        check(load().csamd_cspack_open(str(path).encode(), sid, C.byref(self.f)), "cspack_open")
+
+    @property
+    def shard_id(self) -> int:
+        return int(self.f.shard_id)
+
+    @property
+    def partition_count(self) -> int:
+        return int(self.f.partition_count)
+
+    @property
+    def partition_offsets(self) -> list:
+        return [int(self.f.partition_offsets[i]) for i in range(self.partition_count)]
+
+    @property
+    def partition_bytes(self) -> list:
+        return [int(self.f.partition_bytes[i]) for i in range(self.partition_count)]
+
+    def close(self):
+        if self.f is not None:
+            load().csamd_cspack_close(C.byref(self.f))
+            self.f = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def write_cspack_blocked_ell(path, shard_id: int, parts: list, value_type: int, ws: CspackWorkspace) -> None:
+    arr = (A.bucketed_blocked_ell * max(len(parts), 1))()
+    for i, p in enumerate(parts):
+        arr[i] = p.st
+    check(load().csamd_cspack_write_blocked_ell(str(path).encode(), shard_id, arr, len(parts), value_type, C.byref(ws.ws)),
+          "cspack_write_blocked_ell")
+
+
+def write_cspack_sliced_ell(path, shard_id: int, parts: list, value_type: int, ws: CspackWorkspace) -> None:
+    arr = (A.bucketed_sliced_ell * max(len(parts), 1))()
+    for i, p in enumerate(parts):
+        arr[i] = p.st
+    check(load().csamd_cspack_write_sliced_ell(str(path).encode(), shard_id, arr, len(parts), value_type, C.byref(ws.ws)),
+          "cspack_write_sliced_ell")
+
+
+def stage_cspack_blocked_ell(f: CspackFile, partition: int, value_type: int, ws: CspackWorkspace) -> BucketedBlockedEll:
+    out = A.bucketed_blocked_ell()
+    check(load().csamd_cspack_stage_blocked_ell(C.byref(f.f), partition, value_type, C.byref(ws.ws), C.byref(out)),
+          "cspack_stage_blocked_ell")
+    return BucketedBlockedEll(out, value_type)
+
+
+def stage_cspack_sliced_ell(f: CspackFile, partition: int, value_type: int, ws: CspackWorkspace) -> BucketedSlicedEll:
+    out = A.bucketed_sliced_ell()
+    check(load().csamd_cspack_stage_sliced_ell(C.byref(f.f), partition, value_type, C.byref(ws.ws), C.byref(out)),
+          "cspack_stage_sliced_ell")
+    return BucketedSlicedEll(out, value_type)
+
+
+def stage_cspack_shard_blocked_ell(f: CspackFile, value_type: int, ws: CspackWorkspace) -> list:
+    n = f.partition_count
+    arr = (A.bucketed_blocked_ell * max(n, 1))()
+    check(load().csamd_cspack_stage_shard_blocked_ell(C.byref(f.f), value_type, C.byref(ws.ws), arr), "cspack_stage_shard_blocked_ell")
+    return [BucketedBlockedEll(A.bucketed_blocked_ell.from_buffer_copy(arr[i]), value_type) for i in range(n)]
+
+
 class TransposeScratch:
     def __init__(self):
         self.buf = None

@@ -542,6 +542,52 @@ inline void clear(bucketed_sliced_ell_partition *p) { csamd_bucketed_sliced_ell_
 } // namespace execution
 
 // ---------------------------------------------------------------------------
+// CSPACK01 shard packs of execution partitions (docs/SPEC_CSPACK_V1.md:87-155;
+// materialize / load: src/io/csh5/execution_runtime_part.hh:569-796).  The
+// reference keeps these behind its .csh5 runtime; here they take a path, a
+// shard id and the partitions directly, and loads stage to the device.
+namespace pack {
+
+using file = csamd_cspack_file;
+using workspace = csamd_cspack_workspace;
+constexpr unsigned long long any_shard = CSAMD_CSPACK_ANY_SHARD;
+
+inline void init(workspace *ws) { csamd_cspack_init(ws); }
+inline void clear(workspace *ws) { csamd_cspack_clear(ws); }
+inline int setup(workspace *ws, int device, hipStream_t stream = nullptr) {
+    return csamd_cspack_setup(ws, device, (csamd_stream_t) stream);
+}
+inline int materialize_blocked_ell_cspack(const char *path, std::uint64_t shard_id,
+                                          const execution::bucketed_blocked_ell_partition *parts, std::uint32_t count,
+                                          workspace *ws, unsigned int value_type = CSAMD_VALUE_F16) {
+    return csamd_cspack_write_blocked_ell(path, shard_id, parts, count, value_type, ws);
+}
+inline int materialize_sliced_ell_cspack(const char *path, std::uint64_t shard_id,
+                                         const execution::bucketed_sliced_ell_partition *parts, std::uint32_t count,
+                                         workspace *ws, unsigned int value_type = CSAMD_VALUE_F16) {
+    return csamd_cspack_write_sliced_ell(path, shard_id, parts, count, value_type, ws);
+}
+inline int open(const char *path, std::uint64_t shard_id, file *out) { return csamd_cspack_open(path, shard_id, out); }
+inline void close(file *f) { csamd_cspack_close(f); }
+inline int load_blocked_pack_partition_from_cspack(const file *f, std::uint64_t local_partition,
+                                                   execution::bucketed_blocked_ell_partition *out, workspace *ws,
+                                                   unsigned int value_type = CSAMD_VALUE_F16) {
+    return csamd_cspack_stage_blocked_ell(f, local_partition, value_type, ws, out);
+}
+inline int load_sliced_bucketed_partition_from_cspack(const file *f, std::uint64_t local_partition,
+                                                      execution::bucketed_sliced_ell_partition *out, workspace *ws,
+                                                      unsigned int value_type = CSAMD_VALUE_F16) {
+    return csamd_cspack_stage_sliced_ell(f, local_partition, value_type, ws, out);
+}
+// out: f->partition_count entries.
+inline int load_blocked_pack_shard_from_cspack(const file *f, execution::bucketed_blocked_ell_partition *out, workspace *ws,
+                                               unsigned int value_type = CSAMD_VALUE_F16) {
+    return csamd_cspack_stage_shard_blocked_ell(f, value_type, ws, out);
+}
+
+} // namespace pack
+
+// ---------------------------------------------------------------------------
 namespace bucket {
 
 using major_nnz_bucket_plan_view = csamd_major_nnz_bucket_plan_view;

@@ -630,6 +630,72 @@ int csamd_choose_bucket_count_sliced_ell(const csamd_sliced_ell_view *src, csamd
                                          unsigned int *bucket_count, unsigned long long *bucketed_bytes);
 void csamd_bucketed_sliced_ell_clear(csamd_bucketed_sliced_ell *m);
 
+/* ---------------------------------------------------------------------------
+ * CSPACK01 shard packs (docs/SPEC_CSPACK_V1.md:87-155): one file per shard holding its bucketed
+ * execution partitions.  Container: "CSPACK01", shard_id u64, partition_count u64,
+ * partition_offsets[partition_count] u64, then one blob per partition.  Blob layouts:
+ * Blocked-ELL execution_builders_part.hh:2116-2132, Sliced-ELL execution_runtime_part.hh:168-195;
+ * each segment is a raw payload (src/io/pack/packfile.cu:351-369 Blocked-ELL, :597-621
+ * Sliced-ELL: 1-byte format tag, rows, cols, nnz, then the arrays, unpadded).  Values are
+ * stored at the value type's width; files the reference reads carry 2-byte (F16) values.
+ *
+ * Staging goes straight to HBM: the partition's byte range is read into pinned host memory by
+ * parallel readers, each piece copied to a device staging buffer as soon as it lands, and one
+ * kernel moves every array from its unaligned place in the blob to an aligned place in the
+ * partition's single device allocation.  Headers are checked on the host before any device
+ * work; the result is the same device structure the bucketed builders produce (freed by
+ * csamd_bucketed_*_clear), with `bytes` set to the segments' packed bytes.
+ * ------------------------------------------------------------------------- */
+#define CSAMD_CSPACK_ANY_SHARD 0xffffffffffffffffull
+
+typedef struct CSAMD_ALIGN16 csamd_cspack_file {
+    unsigned long long shard_id;
+    unsigned long long partition_count;
+    unsigned long long file_bytes;
+    unsigned long long *partition_offsets;  /* host [partition_count] */
+    unsigned long long *partition_bytes;    /* host [partition_count]: bytes up to the next blob or the file end */
+    void *impl;
+} csamd_cspack_file;
+
+typedef struct CSAMD_ALIGN16 csamd_cspack_workspace {
+    int device;
+    unsigned int reader_threads;  /* parallel file readers while staging (0: 8) */
+    csamd_stream_t stream;
+    int owns_stream;
+    unsigned int chunk_bytes;     /* read / copy piece size (0: 8 MiB) */
+    void *private_workspace;
+} csamd_cspack_workspace;
+
+/* init / clear / setup (a null stream creates an owned non-blocking one).  The workspace keeps
+ * its pinned host buffers and device staging buffer between calls. */
+void csamd_cspack_init(csamd_cspack_workspace *ws);
+void csamd_cspack_clear(csamd_cspack_workspace *ws);
+int csamd_cspack_setup(csamd_cspack_workspace *ws, int device, csamd_stream_t stream);
+/* materialize_blocked_ell_cspack (execution_runtime_part.hh:643-733): writes `path` (through a
+ * temporary file renamed into place) from device-resident execution partitions. */
+int csamd_cspack_write_blocked_ell(const char *path, unsigned long long shard_id, const csamd_bucketed_blocked_ell *parts,
+                                   unsigned int part_count, unsigned int value_type, csamd_cspack_workspace *ws);
+/* materialize_sliced_ell_cspack (execution_runtime_part.hh:735-796) */
+int csamd_cspack_write_sliced_ell(const char *path, unsigned long long shard_id, const csamd_bucketed_sliced_ell *parts,
+                                  unsigned int part_count, unsigned int value_type, csamd_cspack_workspace *ws);
+/* The header checks of load_blocked_pack_partition_from_cspack (execution_runtime_part.hh:569-604):
+ * magic, the shard id (unless CSAMD_CSPACK_ANY_SHARD) and every offset inside the file. */
+int csamd_cspack_open(const char *path, unsigned long long shard_id, csamd_cspack_file *out);
+void csamd_cspack_close(csamd_cspack_file *f);
+/* load_blocked_pack_partition_from_cspack (execution_runtime_part.hh:569-604) with
+ * load_execution_partition_blob (execution_builders_part.hh:2146-2197), staged to the device. */
+int csamd_cspack_stage_blocked_ell(const csamd_cspack_file *f, unsigned long long local_partition, unsigned int value_type,
+                                   csamd_cspack_workspace *ws, csamd_bucketed_blocked_ell *out);
+/* load_sliced_bucketed_partition_from_cspack (execution_runtime_part.hh:606-641) with
+ * load_sliced_execution_partition_blob (:197-258), staged to the device. */
+int csamd_cspack_stage_sliced_ell(const csamd_cspack_file *f, unsigned long long local_partition, unsigned int value_type,
+                                  csamd_cspack_workspace *ws, csamd_bucketed_sliced_ell *out);
+/* fetch_cached_shard for Blocked-ELL (src/io/csh5/runtime/fetch.cc:20-36): every partition of
+ * the file into out[0 .. partition_count), reading partition p + 1 while p is copied and
+ * unpacked.  On failure every out[] is cleared. */
+int csamd_cspack_stage_shard_blocked_ell(const csamd_cspack_file *f, unsigned int value_type, csamd_cspack_workspace *ws,
+                                         csamd_bucketed_blocked_ell *out);
+
 
 
 void csamd_convert_init(csamd_convert_workspace *ws);

@@ -91,3 +91,73 @@ def dense_of(rows, cols, indptr, indices, values):
         for i in range(indptr[r], indptr[r + 1]):
             d[r, indices[i]] = values[i]  # last duplicate wins
     return d
+
+
+BITS = {1: np.uint8, 2: np.uint16, 4: np.uint32}
+
+
+def _cuts(rng, rows, nseg):
+    if nseg == 0:
+        return np.zeros(1, np.uint32)
+    inner = np.sort(rng.integers(0, rows + 1, nseg - 1)) if nseg > 1 else np.zeros(0, np.int64)
+    return np.concatenate([[0], inner, [rows]]).astype(np.uint32)
+
+
+def random_bell_partition(rng, rows, cols, nseg, vb, block_sizes=(1, 2, 4, 8, 16)):
+    """A bucketed Blocked-ELL execution partition as a dict of numpy arrays
+    (keys of BucketedBlockedEll.to_host()), with random structure: segment
+    cuts, per-segment block sizes and widths, invalid block slots."""
+    sro = _cuts(rng, rows, nseg if rows else nseg)
+    if rows == 0:
+        sro = np.zeros(nseg + 1, np.uint32)
+    segs = []
+    for k in range(nseg):
+        r = int(sro[k + 1] - sro[k])
+        bs = int(rng.choice(block_sizes))
+        w = int(rng.integers(0, 4)) if r else int(rng.integers(0, 2))
+        rb = (r + bs - 1) // bs
+        bci = rng.integers(0, max(cols // bs, 1), rb * w).astype(np.uint32)
+        bci[rng.random(rb * w) < 0.2] = 0xFFFFFFFF
+        val = rng.integers(0, 1 << (8 * vb), r * w * bs, dtype=np.uint64).astype(BITS[vb])
+        segs.append(dict(rows=r, cols=cols, nnz=int(rng.integers(0, r * w * bs + 1)), block_size=bs, ell_cols=w * bs,
+                         block_col_idx=bci, val=val))
+    perm = rng.permutation(rows).astype(np.uint32)
+    inv = np.empty(rows, np.uint32)
+    inv[perm] = np.arange(rows, dtype=np.uint32)
+    cperm = rng.permutation(cols).astype(np.uint32)
+    cinv = np.empty(cols, np.uint32)
+    cinv[cperm] = np.arange(cols, dtype=np.uint32)
+    return dict(rows=rows, cols=cols, nnz=sum(s["nnz"] for s in segs), segments=segs, segment_row_offsets=sro,
+                exec_to_canonical=perm, canonical_to_exec=inv, col_maps=(cperm, cinv))
+
+
+def random_sell_partition(rng, rows, cols, nseg, vb):
+    """A bucketed Sliced-ELL execution partition dict with several slices per
+    segment (uniform heights with a short last slice, or ragged heights)."""
+    sro = _cuts(rng, rows, nseg) if rows else np.zeros(nseg + 1, np.uint32)
+    segs = []
+    for k in range(nseg):
+        r = int(sro[k + 1] - sro[k])
+        if r == 0:
+            offs = np.zeros(1 + int(rng.integers(0, 2)), np.uint32)
+        elif rng.random() < 0.5:
+            h = int(rng.integers(1, 9))
+            offs = np.unique(np.concatenate([np.arange(0, r, h), [r]])).astype(np.uint32)
+        else:
+            offs = np.unique(np.concatenate([[0], rng.integers(0, r, int(rng.integers(0, 5))), [r]])).astype(np.uint32)
+        n = len(offs) - 1
+        wid = rng.integers(0, 6, n).astype(np.uint32)
+        slots = int(((offs[1:].astype(np.int64) - offs[:-1]) * wid).sum())
+        col = rng.integers(0, cols, slots).astype(np.uint32)
+        col[rng.random(slots) < 0.2] = 0xFFFFFFFF
+        val = rng.integers(0, 1 << (8 * vb), slots, dtype=np.uint64).astype(BITS[vb])
+        segs.append(dict(rows=r, cols=cols, nnz=int(rng.integers(0, slots + 1)), slice_count=n, slice_row_offsets=offs,
+                         slice_widths=wid, col_idx=col, val=val))
+    perm = rng.permutation(rows).astype(np.uint32)
+    inv = np.empty(rows, np.uint32)
+    inv[perm] = np.arange(rows, dtype=np.uint32)
+    ncs = int(rng.integers(0, 5))
+    cso = np.sort(rng.integers(0, rows + 1, ncs + 1)).astype(np.uint32)
+    return dict(rows=rows, cols=cols, nnz=sum(s["nnz"] for s in segs), segments=segs, segment_row_offsets=sro,
+                exec_to_canonical=perm, canonical_to_exec=inv, canonical_slice_row_offsets=cso,
+                canonical_slice_widths=rng.integers(0, 9, ncs).astype(np.uint32))

@@ -1,0 +1,228 @@
+"""CSPACK01 shard packs on the GPU: files the library writes from device
+partitions equal the restated encoder's bytes (which the CPU tests pin to
+the reference's own raw payload codec); staging from files the restatement
+wrote returns every array bit for bit (Blocked-ELL and Sliced-ELL, 1-, 2-
+and 4-byte values, empty partitions and segments, many segments, pieces
+smaller than a segment so reads and copies split arrays at odd places);
+the shard call equals the per-partition calls; damaged blobs are refused
+on the host before any device work, and the workspace stays usable."""
+import struct
+
+import numpy as np
+import pytest
+
+from cellshard_amd import _lib
+from cellshard_amd.device import DeviceBlockedELL, DeviceSlicedELL
+from cellshard_amd.ops import (ConvertWorkspace, CspackFile, CspackWorkspace, build_bucketed_blocked_ell_device,
+                               build_bucketed_sliced_ell_device, stage_cspack_blocked_ell, stage_cspack_shard_blocked_ell,
+                               stage_cspack_sliced_ell, write_cspack_blocked_ell, write_cspack_sliced_ell)
+from helpers import F16, encode, random_bell_partition, random_sell_partition
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+VT = {1: 7, 2: 3, 4: 4}  # FP8 e4m3, F16, F32 value types by width
+
+
+@pytest.fixture()
+def cws(gpu):
+    w = ConvertWorkspace()
+    yield w
+    w.close()
+
+
+def _bell_dev(rows, cols, bs, seed):
+    rng = np.random.default_rng(seed)
+    ptr, idx = [0], []
+    for r in range(rows):
+        k = int(rng.integers(0, cols // (2 if (r // bs) % 3 == 0 else 12)))
+        idx.append(np.sort(rng.choice(cols, k, replace=False)))
+        ptr.append(ptr[-1] + k)
+    ptr = np.asarray(ptr, np.uint32)
+    idx = np.concatenate(idx).astype(np.uint32)
+    bits = encode(rng.integers(0, 9, size=len(idx)).astype(np.float64), F16)
+    bci, val, ell_cols, nnz = O.blocked_ell_from_compressed(rows, cols, ptr, idx, bits, bs)
+    return DeviceBlockedELL.from_numpy(rows, cols, nnz, bs, ell_cols, bci, val)
+
+
+def _sell_dev(rows, cols, S, seed):
+    rng = np.random.default_rng(seed)
+    lens = np.where(rng.random(rows) < 0.7, rng.integers(0, 6, rows), rng.integers(20, 60, rows))
+    ptr = np.concatenate([[0], np.cumsum(lens)]).astype(np.uint32)
+    idx = np.concatenate([np.sort(rng.choice(cols, int(n), replace=False)) for n in lens]).astype(np.uint32)
+    bits = encode(rng.integers(1, 9, size=len(idx)).astype(np.float64), F16)
+    s = O.sliced_ell_from_compressed(rows, cols, 0, ptr, idx, bits, S)
+    return DeviceSlicedELL.from_numpy(rows, cols, len(idx), s["slice_rows"], s["offsets"], s["widths"], s["slot_offsets"],
+                                      s["col_idx"], s["val"])
+
+
+def _same_bell(got, want):
+    for k in ("rows", "cols", "nnz"):
+        assert got[k] == want[k], k
+    assert np.array_equal(got["segment_row_offsets"], want["segment_row_offsets"])
+    assert np.array_equal(got["exec_to_canonical"], want["exec_to_canonical"])
+    assert np.array_equal(got["canonical_to_exec"], want["canonical_to_exec"])
+    assert np.array_equal(got["col_maps"][0], want["col_maps"][0]) and np.array_equal(got["col_maps"][1], want["col_maps"][1])
+    assert len(got["segments"]) == len(want["segments"])
+    for a, b in zip(got["segments"], want["segments"]):
+        for k in ("rows", "cols", "nnz", "block_size", "ell_cols"):
+            assert a[k] == b[k], k
+        assert np.array_equal(a["block_col_idx"], b["block_col_idx"])
+        assert a["val"].dtype == b["val"].dtype and np.array_equal(a["val"], b["val"])
+
+
+def _same_sell(got, want):
+    for k in ("rows", "cols", "nnz"):
+        assert got[k] == want[k], k
+    for k in ("segment_row_offsets", "exec_to_canonical", "canonical_to_exec", "canonical_slice_row_offsets",
+              "canonical_slice_widths"):
+        assert np.array_equal(got[k], want[k]), k
+    assert len(got["segments"]) == len(want["segments"])
+    for a, b in zip(got["segments"], want["segments"]):
+        for k in ("rows", "cols", "nnz", "slice_count"):
+            assert a[k] == b[k], k
+        for k in ("slice_row_offsets", "slice_widths", "col_idx", "val"):
+            assert np.array_equal(a[k], b[k]), k
+        off, wid = b["slice_row_offsets"].astype(np.int64), b["slice_widths"].astype(np.int64)
+        slots = np.concatenate([[0], np.cumsum(np.diff(off) * wid)])[:-1] if b["slice_count"] else np.zeros(0)
+        assert np.array_equal(a["slice_slot_offsets"], slots)
+        h = np.diff(off)
+        uniform = b["slice_count"] > 0 and all(x == h[0] for x in h[:-1]) and h[-1] <= h[0]
+        assert a["slice_rows"] == (int(h[0]) if uniform else 0)
+
+
+def _bell_bytes(p):
+    return sum(O.bell_packed_bytes(s["rows"], s["ell_cols"], s["block_size"]) for s in p["segments"])
+
+
+def test_written_blocked_ell_pack_matches_restatement_and_stages_back(cws, tmp_path):
+    parts = [build_bucketed_blocked_ell_device(_bell_dev(rows, 300, bs, rows + bs), k, cws)
+             for rows, bs, k in [(203, 8, 3), (97, 16, 8), (40, 8, 1)]]
+    want = [p.to_host() for p in parts]
+    path = tmp_path / "shard.11.cspack"
+    ws = CspackWorkspace()
+    write_cspack_blocked_ell(path, 11, parts, 3, ws)
+    assert path.read_bytes() == O.cspack_bytes(11, [O.bell_execution_blob(w) for w in want])
+    assert not (tmp_path / "shard.11.cspack.tmp").exists()
+    with CspackFile(path, 11) as f:
+        assert f.partition_count == 3
+        for i, w in enumerate(want):
+            got = stage_cspack_blocked_ell(f, i, 3, ws)
+            g = got.to_host()
+            _same_bell(g, w)
+            assert g["bytes"] == w["bytes"] == _bell_bytes(w)
+            got.close()
+    for p in parts:
+        p.close()
+    ws.close()
+
+
+@pytest.mark.parametrize("vb", [1, 2, 4])
+@pytest.mark.parametrize("threads,chunk", [(0, 0), (4, 4096), (3, 1000)])
+def test_stage_blocked_ell_from_restated_pack(gpu, tmp_path, vb, threads, chunk):
+    rng = np.random.default_rng(vb * 100 + threads)
+    shapes = [(300, 500, 3), (0, 20, 0), (0, 20, 2), (1, 5, 1), (257, 64, 20), (1000, 2000, 5)]
+    want = [random_bell_partition(rng, r, c, n, vb) for r, c, n in shapes]
+    path = tmp_path / "shard.5.cspack"
+    path.write_bytes(O.cspack_bytes(5, [O.bell_execution_blob(w) for w in want]))
+    ws = CspackWorkspace(reader_threads=threads, chunk_bytes=chunk)
+    with CspackFile(path, 5) as f:
+        for i, w in enumerate(want):
+            print(f"partition {i} rows {w['rows']} segments {len(w['segments'])}", flush=True)
+            got = stage_cspack_blocked_ell(f, i, VT[vb], ws)
+            g = got.to_host()
+            _same_bell(g, w)
+            assert g["bytes"] == _bell_bytes(w)
+            got.close()
+        shard = stage_cspack_shard_blocked_ell(f, VT[vb], ws)
+        assert len(shard) == len(want)
+        for got, w in zip(shard, want):
+            _same_bell(got.to_host(), w)
+            got.close()
+    ws.close()
+
+
+def test_written_sliced_ell_pack_matches_restatement_and_stages_back(cws, tmp_path):
+    parts = [build_bucketed_sliced_ell_device(_sell_dev(rows, 200, S, rows * 7 + S), k, cws)
+             for rows, S, k in [(150, 32, 3), (64, 8, 8), (1, 32, 4)]]
+    want = [p.to_host() for p in parts]
+    path = tmp_path / "shard.2.cspack"
+    ws = CspackWorkspace()
+    write_cspack_sliced_ell(path, 2, parts, 3, ws)
+    assert path.read_bytes() == O.cspack_bytes(2, [O.sell_execution_blob(w) for w in want])
+    with CspackFile(path, 2) as f:
+        for i, w in enumerate(want):
+            got = stage_cspack_sliced_ell(f, i, 3, ws)
+            g = got.to_host()
+            _same_sell(g, w)
+            assert g["bytes"] == sum(O.sell_packed_bytes(s["slice_count"], len(s["col_idx"])) for s in w["segments"])
+            got.close()
+    for p in parts:
+        p.close()
+    ws.close()
+
+
+@pytest.mark.parametrize("vb", [1, 2, 4])
+def test_stage_sliced_ell_from_restated_pack(gpu, tmp_path, vb):
+    rng = np.random.default_rng(vb + 7)
+    want = [random_sell_partition(rng, r, c, n, vb) for r, c, n in [(200, 300, 4), (0, 9, 1), (31, 40, 12), (500, 1000, 2)]]
+    path = tmp_path / "s.cspack"
+    path.write_bytes(O.cspack_bytes(1, [O.sell_execution_blob(w) for w in want]))
+    ws = CspackWorkspace(reader_threads=2, chunk_bytes=777)
+    with CspackFile(path) as f:
+        for i, w in enumerate(want):
+            print(f"partition {i} rows {w['rows']} segments {len(w['segments'])}", flush=True)
+            got = stage_cspack_sliced_ell(f, i, VT[vb], ws)
+            _same_sell(got.to_host(), w)
+            got.close()
+    ws.close()
+
+
+def _corrupt(data, offs, case):
+    """Damages partition 0 (partition 1 for a truncated file); returns the
+    bytes and the damaged partition."""
+    d = bytearray(data)
+    o = offs[0]
+    rows, cols, _, nseg = struct.unpack_from("<4I", d, o)
+    seg0 = o + 16 + 4 * (nseg + 1) + 8 * rows + 8 * cols  # first segment's raw header
+    if case == "format":
+        d[seg0] = 7
+    elif case == "segment_rows":
+        struct.pack_into("<I", d, o + 20, struct.unpack_from("<I", d, o + 20)[0] + 1)
+    elif case == "ell_cols":
+        struct.pack_into("<I", d, seg0 + 17, struct.unpack_from("<I", d, seg0 + 17)[0] + 1)
+    elif case == "truncated":
+        return bytes(d[:-3]), 1
+    return bytes(d), 0
+
+
+@pytest.mark.parametrize("case", ["format", "segment_rows", "ell_cols", "truncated"])
+def test_damaged_blobs_are_refused(gpu, tmp_path, case):
+    rng = np.random.default_rng(3)
+    want = [random_bell_partition(rng, 64, 40, 2, 2, block_sizes=(4,)), random_bell_partition(rng, 30, 40, 1, 2)]
+    want[1]["segments"][0]["ell_cols"] = 0
+    want[1]["segments"][0]["block_col_idx"] = np.zeros(0, np.uint32)
+    want[1]["segments"][0]["val"] = np.zeros(0, np.uint16)
+    want[1]["segments"].append(dict(rows=0, cols=40, nnz=0, block_size=1, ell_cols=1, block_col_idx=np.zeros(0, np.uint32),
+                                    val=np.zeros(0, np.uint16)))
+    want[1]["segment_row_offsets"] = np.array([0, 30, 30], np.uint32)
+    data = O.cspack_bytes(0, [O.bell_execution_blob(w) for w in want])
+    _, offs = O.cspack_header(data)
+    bad, victim = _corrupt(data, offs, case)
+    path = tmp_path / "bad.cspack"
+    path.write_bytes(bad)
+    ws = CspackWorkspace()
+    with CspackFile(path) as f:
+        with pytest.raises(RuntimeError):
+            stage_cspack_blocked_ell(f, victim, 3, ws)
+        assert f"cspack partition {victim}" in _lib.last_error()
+        with pytest.raises(RuntimeError):
+            stage_cspack_shard_blocked_ell(f, 3, ws)
+    good = tmp_path / "good.cspack"
+    good.write_bytes(data)
+    with CspackFile(good) as f:
+        for i, w in enumerate(want):
+            got = stage_cspack_blocked_ell(f, i, 3, ws)
+            _same_bell(got.to_host(), w)
+            got.close()
+    ws.close()
