@@ -57,6 +57,8 @@ def parse_args():
     ap.add_argument("--batched-partitions", type=int, default=256)
     ap.add_argument("--skip-plain", action="store_true", help="skip the 20,000 x 33,538 plain-path measurement")
     ap.add_argument("--plain-only", action="store_true", help="only the plain-path measurement")
+    ap.add_argument("--skip-cspack", action="store_true", help="skip the CSPACK staging measurement")
+    ap.add_argument("--cspack-only", action="store_true", help="only the CSPACK staging measurement")
     return ap.parse_args()
 
 
@@ -402,6 +404,102 @@ def run_plain_path(args, log):
     return res
 
 
+def run_cspack(args, log):
+    """CSPACK01 staging: a shard of 4 bucketed Blocked-ELL execution
+    partitions (each 20,000 x 33,538, fp16 values, block 16, 4 buckets) is
+    written to a pack file, then staged back into device views from the warm
+    page cache (parallel reads into pinned memory, copies under the reads,
+    one unpack kernel).  Beside it: a plain pinned host-to-device copy of
+    1 GiB (the link ceiling) and the reference's own CPU partition load
+    (fread into host arrays, its raw payload codec compiled from its
+    sources) over the same file."""
+    import tempfile
+
+    import numpy as np
+    import torch
+
+    from cellshard_amd import abi as A
+    from cellshard_amd.ops import (ConvertWorkspace, CspackFile, CspackWorkspace, blocked_ell_from_compressed,
+                                   build_bucketed_blocked_ell_device, stage_cspack_shard_blocked_ell, synth_csr,
+                                   write_cspack_blocked_ell)
+
+    dev = torch.device("cuda", 0)
+    R, Cc, P = 20_000, 33_538, 4
+    ws = ConvertWorkspace()
+    parts = []
+    for k in range(P):
+        src = synth_csr(R, Cc, median=1500.0, sigma=0.5, seed=11 + k, value_type=A.VALUE_F16, ws=ws)
+        bell = blocked_ell_from_compressed(src, 16, ws)
+        del src
+        parts.append(build_bucketed_blocked_ell_device(bell, 4, ws))
+        del bell
+        torch.cuda.empty_cache()
+    cws = CspackWorkspace()
+    tmpdir = tempfile.mkdtemp(prefix="cspack_")
+    path = Path(tmpdir) / "shard.0.cspack"
+    t0 = time.perf_counter()
+    write_cspack_blocked_ell(path, 0, parts, A.VALUE_F16, cws)
+    write_s = time.perf_counter() - t0
+    file_bytes = path.stat().st_size
+    seg_count = sum(int(p.st.segment_count) for p in parts)
+    for p in parts:
+        p.close()
+    del parts
+    torch.cuda.empty_cache()
+
+    def stage(ws_):
+        t0 = time.perf_counter()
+        with CspackFile(path, 0) as f:
+            out = stage_cspack_shard_blocked_ell(f, A.VALUE_F16, ws_)
+        dt = time.perf_counter() - t0
+        for o in out:
+            o.close()
+        return dt
+
+    stage(cws)  # grows the pinned and device staging buffers
+    times = sorted(stage(cws) for _ in range(3))
+    one = CspackWorkspace(reader_threads=1)
+    stage(one)
+    one_s = stage(one)
+    one.close()
+    cws.close()
+    # link ceiling: 1 GiB pinned host -> device
+    h = torch.empty(1 << 30, dtype=torch.uint8, pin_memory=True)
+    d = torch.empty(1 << 30, dtype=torch.uint8, device=dev)
+    d.copy_(h, non_blocking=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        d.copy_(h, non_blocking=True)
+    torch.cuda.synchronize()
+    h2d = 3 * (1 << 30) / (time.perf_counter() - t0) / 1e9
+    del h, d
+    torch.cuda.empty_cache()
+    res = {"workload": "cspack_shard_4x_bucketed_bell16_20000x33538_fp16", "partitions": P, "segments": seg_count,
+           "file_bytes": file_bytes, "write_s": round(write_s, 3),
+           "stage_shard_ms": round(times[1] * 1e3, 2), "stage_shard_gb_s": round(file_bytes / times[1] / 1e9, 1),
+           "stage_shard_1_reader_gb_s": round(file_bytes / one_s / 1e9, 1), "pinned_h2d_gb_s": round(h2d, 1)}
+    if not args.skip_cpu:
+        from oracle import oracle as O
+        if O.ref_pack_available():
+            with CspackFile(path, 0) as f:
+                offs = f.partition_offsets
+            r = O.ref_pack()
+            r.ref_load_bell_partition(str(path).encode(), offs[0])  # warm
+            t0 = time.perf_counter()
+            got = sum(int(r.ref_load_bell_partition(str(path).encode(), o)) for o in offs)
+            cpu_s = time.perf_counter() - t0
+            res["cpu_reference_load"] = {"s": round(cpu_s, 3), "gb_s": round(file_bytes / cpu_s / 1e9, 2), "kind": "reference",
+                                         "cores": 1, "bytes": got,
+                                         "note": "fread of each blob into fresh host arrays, segments through the reference's "
+                                                 "load(FILE*, blocked_ell*) compiled from its sources; host memory only"}
+    path.unlink()
+    os.rmdir(tmpdir)
+    ws.close()
+    log(f"cspack: {json.dumps(res)}")
+    return res
+
+
 def run_batched_partitions(args, log):
     """Batched uneven partitions: 256 CSR partitions (rows U[512, 65,536],
     median row length U[300, 5,000], LogNormal sigma 0.7, 36,601 columns,
@@ -560,6 +658,10 @@ def main() -> int:
     if args.plain_only:
         torch.cuda.set_device(0)
         print(json.dumps({"plain_path": run_plain_path(args, log)}), flush=True)
+        return 0
+    if args.cspack_only:
+        torch.cuda.set_device(0)
+        print(json.dumps({"cspack": run_cspack(args, log)}), flush=True)
         return 0
     if args.batched_only:
         torch.cuda.set_device(0)
@@ -764,6 +866,9 @@ def main() -> int:
     batched = None
     if not args.skip_batched and not args.reduce_only:
         batched = run_batched_partitions(args, log)
+    cspack = None
+    if not args.skip_cspack and not args.reduce_only:
+        cspack = run_cspack(args, log)
     serving = None
     if not args.skip_serve and not args.reduce_only:
         serving = run_serving(args, log)
@@ -792,6 +897,7 @@ def main() -> int:
         "plain_path": plain,
         "reduced_precision": lowp,
         "batched_partitions": batched,
+        "cspack": cspack,
         "serving": serving,
         "generate_s": round(gen_s, 2),
         "device_copy_gb_s": round(copy_gbs, 1),

@@ -911,6 +911,7 @@ def ref_pack():
     if _ref_pack is None:
         _ref_pack = C.CDLL(str(REF_PACK_PATH))
         _ref_pack.ref_load_bell_partition.restype = C.c_longlong
+        _ref_pack.ref_load_bell_partition.argtypes = [C.c_char_p, C.c_ulonglong]
     return _ref_pack
 
 
