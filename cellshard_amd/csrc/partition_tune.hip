@@ -20,7 +20,11 @@
 // groups).  The group's distinct 8-column blocks are marked in 8 LDS
 // bitmaps, one per 8-row block; the 16- and 32-row blocks' 16- and
 // 32-column blocks are ORs of 2 / 4 bitmaps folded over bit pairs / quads,
-// so one pass over the index array serves all three block sizes.
+// so one pass over the index array serves all three block sizes.  A wave
+// reads its row in 16-byte units and ORs each unit's bits per bitmap word
+// in registers before the LDS atomic: the 4 indices of a sorted unit
+// mostly share a word, and per-index atomics on one word from many lanes
+// serialise.
 #include "common.hh"
 
 #include <vector>
@@ -53,6 +57,34 @@ __global__ void pt_zero_kernel(pt_acc *__restrict__ acc, unsigned int n) {
     for (unsigned int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         acc[i] = pt_acc{};
     }
+}
+
+typedef unsigned int pt_u32x4 __attribute__((ext_vector_type(4)));
+
+// Marks the 8-column blocks of one 16-byte unit of indices (a lane's 4
+// consecutive entries of a row): bits that fall in the same bitmap word are
+// ORed in registers first, so a sorted unit costs one LDS atomic.
+__device__ __forceinline__ void pt_mark4(unsigned int *bm, pt_u32x4 q, bool valid, unsigned int cols) {
+    unsigned int w[4], m[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const unsigned int c = q[k];
+        const bool ok = valid && c < cols;
+        w[k] = ok ? (c >> 8) : 0xffffffffu;
+        m[k] = ok ? 1u << ((c >> 3) & 31u) : 0u;
+    }
+    unsigned int cw = w[0], cm = m[0];
+#pragma unroll
+    for (int k = 1; k < 4; ++k) {
+        if (w[k] == cw) {
+            cm |= m[k];
+        } else {
+            if (cw != 0xffffffffu) atomicOr(&bm[cw], cm);
+            cw = w[k];
+            cm = m[k];
+        }
+    }
+    if (cw != 0xffffffffu) atomicOr(&bm[cw], cm);
 }
 
 __device__ __forceinline__ unsigned int fold2(unsigned int x) { return (x | (x >> 1)) & 0x55555555u; }
@@ -112,28 +144,35 @@ __global__ void __launch_bounds__(PT_THREADS) pt_tune_kernel(const pt_part *__re
             atomicAdd(&A->sell_padded[1], (unsigned long long) nr * m64);
         }
     }
-    // distinct 8-column blocks per 8-row block: one wave per row, PT_UNROLL
-    // independent loads in flight per lane
+    // distinct 8-column blocks per 8-row block: one wave per row.  The row is
+    // read in 16-byte units (4 indices) from its first 16-byte boundary, with
+    // PT_UNROLL units in flight per lane (loads clamped to the row's last
+    // unit, never under a branch); lanes 0-5 mark the up to 3 + 3 indices
+    // before the first and after the last whole unit.
     if (threadIdx.x <= nr) rp[threadIdx.x] = P.ptr[r0 + threadIdx.x];
     __syncthreads();
     const unsigned int wave = threadIdx.x >> 6, lane = lane_id();
+    const unsigned int mis = (unsigned int) (((uintptr_t) P.idx >> 2) & 3u);  // P.idx's distance past a 16-byte boundary, in indices
     for (unsigned int r = wave; r < nr; r += PT_THREADS / 64u) {
         unsigned int *bm = bits + (r >> 3) * nw;
         const unsigned int b = rp[r], e = rp[r + 1u];
-        for (unsigned int i0 = b + lane; i0 < e; i0 += 64u * PT_UNROLL) {
-            unsigned int c[PT_UNROLL];
-#pragma unroll
-            for (unsigned int u = 0; u < PT_UNROLL; ++u) {
-                const unsigned int i = i0 + 64u * u;
-                c[u] = i < e ? P.idx[i] : CSAMD_INVALID_COL;
+        const unsigned int a = min(b + ((4u - ((b + mis) & 3u)) & 3u), e);
+        const unsigned int units = (e - a) >> 2;
+        const unsigned int ea = a + 4u * units;
+        if (lane < 6u) {
+            const unsigned int i = lane < 3u ? b + lane : ea + lane - 3u;
+            if (lane < 3u ? i < a : i < e) {
+                const unsigned int c = P.idx[i];
+                if (c < cols) atomicOr(&bm[c >> 8], 1u << ((c >> 3) & 31u));
             }
+        }
+        const pt_u32x4 *U = (const pt_u32x4 *) (P.idx + a);
+        for (unsigned int j0 = lane; j0 < units; j0 += 64u * PT_UNROLL) {
+            pt_u32x4 q[PT_UNROLL];
 #pragma unroll
-            for (unsigned int u = 0; u < PT_UNROLL; ++u) {
-                if (c[u] < cols) {
-                    const unsigned int bc = c[u] >> 3;
-                    atomicOr(&bm[bc >> 5], 1u << (bc & 31u));
-                }
-            }
+            for (unsigned int u = 0; u < PT_UNROLL; ++u) q[u] = U[min(j0 + 64u * u, units - 1u)];
+#pragma unroll
+            for (unsigned int u = 0; u < PT_UNROLL; ++u) pt_mark4(bm, q[u], j0 + 64u * u < units, cols);
         }
     }
     __syncthreads();

@@ -81,3 +81,33 @@ def test_tuner_picks_each_layout(gpu):
     got = tune_partitions(parts, ws)
     assert [g["layout"] for g in got] == ["blocked_ell", "sliced_ell", "compressed"]
     ws.close()
+
+
+@pytest.mark.parametrize("shift", [1, 2, 3])
+def test_tuner_on_index_arrays_off_a_16_byte_boundary(gpu, shift):
+    """Partition views into the middle of an allocation: the index array
+    starts 4, 8 or 12 bytes past a 16-byte boundary, so the 16-byte unit
+    reads start at a different entry of every row."""
+    import torch
+
+    from cellshard_amd.device import to_dev_u32
+    ws = ConvertWorkspace()
+    rng = np.random.default_rng(shift)
+    cols = 777
+    cases, dev = [], []
+    for i, rows in enumerate([1, 7, 64, 130, 333]):
+        ptr, idx, val = random_csr(rows, cols, float(rng.uniform(0.01, 0.2)), seed=50 + i, duplicates=0.02, empty_rows=0.1,
+                                   sorted_rows=bool(i % 2))
+        cases.append((rows, ptr, idx))
+        big = to_dev_u32(np.concatenate([np.full(shift, 5, np.uint32), idx]), torch.device("cuda", 0))
+        d = DeviceCSR.from_numpy(rows, cols, ptr, idx, encode(val, F16), F16)
+        d.minor_idx = big[shift:]
+        d._keep = big
+        dev.append(d)
+    got = tune_partitions(dev, ws)
+    for (rows, ptr, idx), g in zip(cases, got):
+        bs, fill, w, sell, hist = _expect(rows, cols, ptr, idx, 2)
+        assert g["bell_block_size"] == bs and g["bell_ell_width"] == w, (rows, g, bs, w)
+        assert abs(g["bell_fill_ratio"] - fill) <= 1e-12
+        assert np.array_equal(np.asarray(g["row_length_log2_hist"]), hist)
+    ws.close()
