@@ -420,8 +420,8 @@ def run_cspack(args, log):
 
     from cellshard_amd import abi as A
     from cellshard_amd.ops import (ConvertWorkspace, CspackFile, CspackWorkspace, blocked_ell_from_compressed,
-                                   build_bucketed_blocked_ell_device, stage_cspack_shard_blocked_ell, synth_csr,
-                                   write_cspack_blocked_ell)
+                                   build_bucketed_blocked_ell_device, canonical_compressed_from_bucketed_blocked_ell,
+                                   stage_cspack_blocked_ell, stage_cspack_shard_blocked_ell, synth_csr, write_cspack_blocked_ell)
 
     dev = torch.device("cuda", 0)
     R, Cc, P = 20_000, 33_538, 4
@@ -458,6 +458,25 @@ def run_cspack(args, log):
 
     stage(cws)  # grows the pinned and device staging buffers
     times = sorted(stage(cws) for _ in range(3))
+    # the reference's fetch of a canonical Blocked-ELL partition: staged exec
+    # partition -> canonical CSR -> Blocked-ELL 16
+    with CspackFile(path, 0) as f:
+        part0 = stage_cspack_blocked_ell(f, 0, A.VALUE_F16, cws)
+    canonical_compressed_from_bucketed_blocked_ell(part0, ws)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ccsr = canonical_compressed_from_bucketed_blocked_ell(part0, ws)
+    torch.cuda.synchronize()
+    csr_ms = (time.perf_counter() - t0) * 1e3
+    t0 = time.perf_counter()
+    cbell = blocked_ell_from_compressed(ccsr, 16, ws)
+    torch.cuda.synchronize()
+    cbell_ms = (time.perf_counter() - t0) * 1e3
+    recon = {"canonical_csr_ms": round(csr_ms, 3), "canonical_nnz": ccsr.nnz, "blocked_ell_16_ms": round(cbell_ms, 3),
+             "exec_partition_bytes": int(part0.st.bytes)}
+    part0.close()
+    del ccsr, cbell
+    torch.cuda.empty_cache()
     one = CspackWorkspace(reader_threads=1)
     stage(one)
     one_s = stage(one)
@@ -478,7 +497,8 @@ def run_cspack(args, log):
     res = {"workload": "cspack_shard_4x_bucketed_bell16_20000x33538_fp16", "partitions": P, "segments": seg_count,
            "file_bytes": file_bytes, "write_s": round(write_s, 3),
            "stage_shard_ms": round(times[1] * 1e3, 2), "stage_shard_gb_s": round(file_bytes / times[1] / 1e9, 1),
-           "stage_shard_1_reader_gb_s": round(file_bytes / one_s / 1e9, 1), "pinned_h2d_gb_s": round(h2d, 1)}
+           "stage_shard_1_reader_gb_s": round(file_bytes / one_s / 1e9, 1), "pinned_h2d_gb_s": round(h2d, 1),
+           "reconstruct_partition": recon}
     if not args.skip_cpu:
         from oracle import oracle as O
         if O.ref_pack_available():

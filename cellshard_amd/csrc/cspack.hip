@@ -910,6 +910,209 @@ int write_pack(const char *path, unsigned long long shard_id, const Part *parts,
     return ok;
 }
 
+typedef unsigned int rc_u32x4 __attribute__((ext_vector_type(4)));
+
+// ---------------------------------------------------------------------------
+// Canonical CSR of a bucketed Blocked-ELL execution partition
+// (bucketed_partition_to_canonical_coo, execution_builders_part.hh:909-968):
+// one wave per exec row; the stored values that are not +-0 at exec columns
+// < cols, in slot order, land in the canonical row's CSR range with their
+// columns mapped to canonical ones.
+// ---------------------------------------------------------------------------
+struct rc_segment {
+    const unsigned int *bci;
+    const unsigned char *val;
+    unsigned int row0, rows, block_size, ell_cols, width;
+    unsigned int vec;  // 16-byte chunks of a value row never straddle a block (see rc_vec_ok)
+};
+
+// Values in a 16-byte chunk, and whether a segment can be read in chunks:
+// a block holds whole chunks and every value row starts on a chunk (with a
+// 16-byte aligned value array).
+template<int VB> constexpr unsigned int rc_per_chunk() { return 16u / (unsigned int) VB; }
+
+inline bool rc_vec_ok(unsigned int vb, unsigned int block_size, unsigned int ell_cols, const void *val) {
+    const unsigned int per = 16u / vb;
+    return block_size % per == 0u && ell_cols % per == 0u && ((uintptr_t) val & 15u) == 0u;
+}
+
+template<int VB>
+__device__ __forceinline__ unsigned int rc_elem(rc_u32x4 q, unsigned int e) {
+    if constexpr (VB == 4) return q[e];
+    else if constexpr (VB == 2) return (q[e >> 1] >> ((e & 1u) * 16u)) & 0xffffu;
+    else return (q[e >> 2] >> ((e & 3u) * 8u)) & 0xffu;
+}
+
+// Chunk c of exec row `local` of segment g: bit e of the result is set when
+// value e of the chunk is an entry; *col0 is the chunk's first exec column.
+template<int VB>
+__device__ __forceinline__ unsigned int rc_chunk(const rc_segment &g, unsigned int local, unsigned int c, unsigned int cols,
+                                                 unsigned int zero_mask, rc_u32x4 *q, unsigned long long *col0) {
+    constexpr unsigned int PER = rc_per_chunk<VB>();
+    const unsigned int t0 = c * PER, slot = t0 / g.block_size, cib0 = t0 - slot * g.block_size;
+    const unsigned int bc = g.bci[(size_t) (local / g.block_size) * g.width + slot];
+    *q = ((const rc_u32x4 *) (g.val + (size_t) local * g.ell_cols * VB))[c];
+    *col0 = (unsigned long long) bc * g.block_size + cib0;
+    unsigned int m = 0u;
+    if (bc != CSAMD_INVALID_COL) {
+#pragma unroll
+        for (unsigned int e = 0; e < PER; ++e)
+            if ((rc_elem<VB>(*q, e) & zero_mask) != 0u && *col0 + e < cols) m |= 1u << e;
+    }
+    return m;
+}
+
+struct rc_args {
+    const rc_segment *seg;
+    unsigned int nseg, rows, cols, zero_mask;
+    const unsigned int *e2c_rows;
+    const unsigned int *e2c_cols;  // null: identity
+    unsigned int *counts;          // [rows], by canonical row
+    unsigned int *hits;            // [rows], zeroed
+    unsigned int *flags;           // bit 0: row map not a permutation, bit 1: mapped column >= cols
+};
+
+template<int VB>
+__device__ __forceinline__ unsigned int rc_bits(const unsigned char *v, size_t i) {
+    if constexpr (VB == 1) return v[i];
+    else if constexpr (VB == 2) return ((const unsigned short *) v)[i];
+    else return ((const unsigned int *) v)[i];
+}
+
+__device__ __forceinline__ unsigned int rc_segment_of(const rc_segment *seg, unsigned int nseg, unsigned int x) {
+    unsigned int lo = 0u, hi = nseg - 1u;
+    while (lo < hi) {
+        const unsigned int mid = (lo + hi + 1u) >> 1;
+        if (seg[mid].row0 <= x) lo = mid;
+        else hi = mid - 1u;
+    }
+    return lo;
+}
+
+// Exec row x, slot position t of its value row: whether it holds an entry
+// and its exec column.
+template<int VB>
+__device__ __forceinline__ bool rc_entry(const rc_segment &g, unsigned int local, unsigned int t, unsigned int cols,
+                                         unsigned int zero_mask, unsigned long long *exec_col, unsigned int *bits) {
+    const unsigned int slot = t / g.block_size, cib = t - slot * g.block_size;
+    const unsigned int bc = g.bci[(size_t) (local / g.block_size) * g.width + slot];
+    *bits = rc_bits<VB>(g.val, (size_t) local * g.ell_cols + t);
+    *exec_col = (unsigned long long) bc * g.block_size + cib;
+    return bc != CSAMD_INVALID_COL && (*bits & zero_mask) != 0u && *exec_col < cols;
+}
+
+template<int VB>
+__global__ void rc_count_kernel(rc_args a) {
+    const unsigned int nw = (gridDim.x * blockDim.x) >> 6;
+    for (unsigned int x = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; x < a.rows; x += nw) {
+        const rc_segment g = a.seg[rc_segment_of(a.seg, a.nseg, x)];
+        const unsigned int local = x - g.row0;
+        unsigned int n = 0u;
+        if (g.vec) {
+            const unsigned int chunks = g.ell_cols / rc_per_chunk<VB>();
+            for (unsigned int c = lane_id(); c < chunks; c += 64u) {
+                rc_u32x4 q;
+                unsigned long long c0;
+                n += (unsigned int) __popc(rc_chunk<VB>(g, local, c, a.cols, a.zero_mask, &q, &c0));
+            }
+        } else {
+            for (unsigned int t = lane_id(); t < g.ell_cols; t += 64u) {
+                unsigned long long ec;
+                unsigned int bits;
+                n += rc_entry<VB>(g, local, t, a.cols, a.zero_mask, &ec, &bits) ? 1u : 0u;
+            }
+        }
+        n = wave_sum_u32(n);
+        if (lane_id() == 0u) {
+            const unsigned int c = a.e2c_rows[x];
+            if (c >= a.rows || atomicAdd(&a.hits[c], 1u) != 0u) atomicOr(a.flags, 1u);
+            else a.counts[c] = n;
+        }
+    }
+}
+
+template<int VB>
+__global__ void rc_emit_kernel(rc_args a, const unsigned int *__restrict__ ptr, unsigned int *__restrict__ out_idx,
+                               unsigned char *__restrict__ out_val, unsigned int limit) {
+    const unsigned int nw = (gridDim.x * blockDim.x) >> 6;
+    for (unsigned int x = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; x < a.rows; x += nw) {
+        const rc_segment g = a.seg[rc_segment_of(a.seg, a.nseg, x)];
+        const unsigned int local = x - g.row0;
+        const unsigned int c = a.e2c_rows[x];  // a permutation (checked by the count pass)
+        if (c >= a.rows) continue;             // uniform: one row per wave
+        const unsigned int base = ptr[c], end = min(ptr[c + 1u], limit);
+        unsigned int run = 0u;
+        if (g.vec) {
+            constexpr unsigned int PER = rc_per_chunk<VB>();
+            const unsigned int chunks = g.ell_cols / PER;
+            for (unsigned int k0 = 0u; k0 < chunks; k0 += 64u) {
+                const unsigned int k = k0 + lane_id();
+                rc_u32x4 q = rc_u32x4{0u, 0u, 0u, 0u};
+                unsigned long long c0 = 0u;
+                const unsigned int m = k < chunks ? rc_chunk<VB>(g, local, k, a.cols, a.zero_mask, &q, &c0) : 0u;
+                const unsigned int cnt = (unsigned int) __popc(m);
+                const unsigned int incl = wave_inclusive_scan_u32(cnt);
+                unsigned int pos = base + run + incl - cnt;
+#pragma unroll
+                for (unsigned int e = 0; e < PER; ++e) {
+                    if ((m >> e) & 1u) {
+                        const unsigned int col = a.e2c_cols != nullptr ? a.e2c_cols[c0 + e] : (unsigned int) (c0 + e);
+                        if (col >= a.cols) atomicOr(a.flags, 2u);
+                        if (pos < end) {
+                            const unsigned int bits = rc_elem<VB>(q, e);
+                            out_idx[pos] = col;
+                            if constexpr (VB == 1) out_val[pos] = (unsigned char) bits;
+                            else if constexpr (VB == 2) ((unsigned short *) out_val)[pos] = (unsigned short) bits;
+                            else ((unsigned int *) out_val)[pos] = bits;
+                        }
+                        ++pos;
+                    }
+                }
+                run += __shfl(incl, 63);
+            }
+            continue;
+        }
+        for (unsigned int t0 = 0u; t0 < g.ell_cols; t0 += 64u) {
+            const unsigned int t = t0 + lane_id();
+            unsigned long long ec = 0u;
+            unsigned int bits = 0u;
+            const bool ok = t < g.ell_cols && rc_entry<VB>(g, local, t, a.cols, a.zero_mask, &ec, &bits);
+            const unsigned long long m = __ballot(ok);
+            if (ok) {
+                const unsigned int pos = base + run + (unsigned int) __popcll(m & lanemask_lt());
+                const unsigned int col = a.e2c_cols != nullptr ? a.e2c_cols[ec] : (unsigned int) ec;
+                if (col >= a.cols) atomicOr(a.flags, 2u);
+                if (pos < end) {
+                    out_idx[pos] = col;
+                    if constexpr (VB == 1) out_val[pos] = (unsigned char) bits;
+                    else if constexpr (VB == 2) ((unsigned short *) out_val)[pos] = (unsigned short) bits;
+                    else ((unsigned int *) out_val)[pos] = bits;
+                }
+            }
+            run += (unsigned int) __popcll(m);
+        }
+    }
+}
+
+// Emit pass of the canonical CSR (row pointers in place), then the mapped
+// column check.
+int canonical_emit(const rc_args &a, int vb, unsigned int grid, const unsigned int *d_ptr, unsigned int *d_idx, void *d_val,
+                   unsigned int rows, unsigned int nnz, unsigned int *flags, unsigned int cols, hipStream_t s) {
+    if (rows != 0u && nnz != 0u) {
+        switch (vb) {
+            case 1: rc_emit_kernel<1><<<grid, 256, 0, s>>>(a, d_ptr, d_idx, (unsigned char *) d_val, nnz); break;
+            case 2: rc_emit_kernel<2><<<grid, 256, 0, s>>>(a, d_ptr, d_idx, (unsigned char *) d_val, nnz); break;
+            default: rc_emit_kernel<4><<<grid, 256, 0, s>>>(a, d_ptr, d_idx, (unsigned char *) d_val, nnz); break;
+        }
+        CSAMD_HIP(hipGetLastError(), "rc_emit_kernel");
+    }
+    unsigned int h = 0u;
+    CSAMD_HIP(hipMemcpyAsync(&h, flags, 4u, hipMemcpyDeviceToHost, s), "D2H flags");
+    CSAMD_HIP(hipStreamSynchronize(s), "canonical emit sync");
+    CSAMD_REQUIRE((h & 2u) == 0u, "canonical_compressed_from_bucketed_blocked_ell: a mapped column is >= cols %u", cols);
+    return 1;
+}
+
 } // namespace
 } // namespace csamd
 
@@ -1080,6 +1283,82 @@ int csamd_cspack_stage_shard_blocked_ell(const csamd_cspack_file *f, unsigned in
     if (!ok)
         for (unsigned long long p = 0; p < n; ++p) csamd_bucketed_blocked_ell_clear(&out[p]);
     return ok;
+}
+
+int csamd_canonical_compressed_from_bucketed_blocked_ell(const csamd_bucketed_blocked_ell *part, unsigned int value_type,
+                                                         unsigned int *d_out_ptr, int ptr_ready, unsigned int *d_out_idx,
+                                                         void *d_out_val, unsigned long long capacity, unsigned long long *nnz_out,
+                                                         csamd_convert_workspace *ws) {
+    CSAMD_REQUIRE(part != nullptr && d_out_ptr != nullptr && nnz_out != nullptr && ws != nullptr,
+                  "canonical_compressed_from_bucketed_blocked_ell: null argument");
+    const int vb = value_type_bytes(value_type);
+    CSAMD_REQUIRE(vb == 1 || vb == 2 || vb == 4, "canonical_compressed_from_bucketed_blocked_ell: unsupported value type %u", value_type);
+    const unsigned int rows = part->rows, nseg = part->segment_count;
+    CSAMD_REQUIRE(rows == 0u || (part->exec_to_canonical_rows != nullptr && nseg != 0u && part->segments != nullptr &&
+                                 part->segment_row_offsets != nullptr),
+                  "canonical_compressed_from_bucketed_blocked_ell: partition without segments or row map");
+    CSAMD_REQUIRE(rows == 0u || (part->segment_row_offsets[0] == 0u && part->segment_row_offsets[nseg] == rows),
+                  "canonical_compressed_from_bucketed_blocked_ell: segment row offsets do not cover rows [0, %u)", rows);
+    std::vector<rc_segment> segs(nseg != 0u ? nseg : 1u);
+    for (unsigned int k = 0; k < nseg; ++k) {
+        const csamd_blocked_ell_view &v = part->segments[k];
+        const unsigned int r0 = part->segment_row_offsets[k], r1 = part->segment_row_offsets[k + 1u];
+        CSAMD_REQUIRE(r1 >= r0 && r1 - r0 == v.rows && r1 <= rows, "canonical_compressed_from_bucketed_blocked_ell: segment %u rows", k);
+        CSAMD_REQUIRE(v.rows == 0u || (v.block_size != 0u && v.ell_cols % v.block_size == 0u &&
+                                       (v.ell_cols == 0u || (v.blockColIdx != nullptr && v.val != nullptr))),
+                      "canonical_compressed_from_bucketed_blocked_ell: segment %u shape", k);
+        segs[k] = rc_segment{v.blockColIdx, (const unsigned char *) v.val, r0, v.rows, v.block_size != 0u ? v.block_size : 1u,
+                             v.ell_cols, v.block_size != 0u ? v.ell_cols / v.block_size : 0u,
+                             v.block_size != 0u && rc_vec_ok((unsigned int) vb, v.block_size, v.ell_cols, v.val) ? 1u : 0u};
+    }
+    if (!ensure_device(ws->device >= 0 ? ws->device : 0)) return 0;
+    hipStream_t s = (hipStream_t) ws->stream;
+    const size_t seg_bytes = align_up((size_t) segs.size() * sizeof(rc_segment), 256u);
+    const size_t row_bytes = align_up((size_t) rows * 4u + 4u, 256u);
+    const size_t scan_bytes = exclusive_scan_scratch_bytes(rows);
+    char *scratch = (char *) convert_scratch(ws, seg_bytes + 2u * row_bytes + 256u + scan_bytes);
+    if (scratch == nullptr) return 0;
+    rc_segment *d_seg = (rc_segment *) scratch;
+    unsigned int *counts = (unsigned int *) (scratch + seg_bytes);
+    unsigned int *hits = (unsigned int *) (scratch + seg_bytes + row_bytes);
+    unsigned int *flags = (unsigned int *) (scratch + seg_bytes + 2u * row_bytes);
+    void *scan_scratch = scratch + seg_bytes + 2u * row_bytes + 256u;
+    CSAMD_HIP(hipMemcpyAsync(d_seg, segs.data(), segs.size() * sizeof(rc_segment), hipMemcpyHostToDevice, s), "H2D segments");
+    if (!zero_bytes(hits, row_bytes + 256u, s)) return 0;  // hits and flags
+    const unsigned int zero_mask = vb == 1 ? 0x7fu : vb == 2 ? 0x7fffu : 0x7fffffffu;
+    const rc_args a{d_seg, nseg, rows, part->cols, zero_mask, part->exec_to_canonical_rows, part->exec_to_canonical_cols, counts, hits,
+                    flags};
+    const unsigned int grid = grid_for((unsigned long long) rows * 64u, 256u, (unsigned int) device_cu_count() * 16u);
+    unsigned int h[3] = {0u, 0u, 0u};
+    if (ptr_ready) {
+        // the row pointers of a previous sizing call on this partition
+        CSAMD_REQUIRE(d_out_idx != nullptr, "canonical_compressed_from_bucketed_blocked_ell: ptr_ready without outputs");
+        CSAMD_HIP(hipMemcpyAsync(h + 2, d_out_ptr + rows, 4u, hipMemcpyDeviceToHost, s), "D2H nnz");
+        CSAMD_HIP(hipStreamSynchronize(s), "canonical nnz sync");
+        *nnz_out = h[2];
+        CSAMD_REQUIRE(capacity >= h[2] && (h[2] == 0u || d_out_val != nullptr),
+                      "canonical_compressed_from_bucketed_blocked_ell: capacity %llu < nnz %u", capacity, h[2]);
+        return canonical_emit(a, vb, grid, d_out_ptr, d_out_idx, d_out_val, rows, h[2], flags, part->cols, s);
+    }
+    if (rows != 0u) {
+        switch (vb) {
+            case 1: rc_count_kernel<1><<<grid, 256, 0, s>>>(a); break;
+            case 2: rc_count_kernel<2><<<grid, 256, 0, s>>>(a); break;
+            default: rc_count_kernel<4><<<grid, 256, 0, s>>>(a); break;
+        }
+        CSAMD_HIP(hipGetLastError(), "rc_count_kernel");
+    }
+    if (!exclusive_scan_u32(counts, rows, d_out_ptr, flags + 1, scan_scratch, scan_bytes, s, true)) return 0;
+    CSAMD_HIP(hipMemcpyAsync(h, flags, 8u, hipMemcpyDeviceToHost, s), "D2H flags");
+    CSAMD_HIP(hipMemcpyAsync(h + 2, d_out_ptr + rows, 4u, hipMemcpyDeviceToHost, s), "D2H nnz");
+    CSAMD_HIP(hipStreamSynchronize(s), "canonical count sync");
+    CSAMD_REQUIRE((h[0] & 1u) == 0u, "canonical_compressed_from_bucketed_blocked_ell: exec_to_canonical_rows is not a permutation");
+    CSAMD_REQUIRE(h[1] == 0u, "canonical_compressed_from_bucketed_blocked_ell: 2^32 entries or more");
+    *nnz_out = h[2];
+    if (d_out_idx == nullptr) return 1;
+    CSAMD_REQUIRE(capacity >= h[2] && (h[2] == 0u || d_out_val != nullptr),
+                  "canonical_compressed_from_bucketed_blocked_ell: capacity %llu < nnz %u", capacity, h[2]);
+    return canonical_emit(a, vb, grid, d_out_ptr, d_out_idx, d_out_val, rows, h[2], flags, part->cols, s);
 }
 
 } // extern "C"

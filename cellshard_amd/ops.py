@@ -652,6 +652,32 @@ def stage_cspack_shard_blocked_ell(f: CspackFile, value_type: int, ws: CspackWor
     return [BucketedBlockedEll(A.bucketed_blocked_ell.from_buffer_copy(arr[i]), value_type) for i in range(n)]
 
 
+def canonical_compressed_from_bucketed_blocked_ell(part: BucketedBlockedEll, ws: ConvertWorkspace) -> DeviceCSR:
+    """The partition's canonical-order CSR (the reference's canonical COO,
+    rows in canonical order); Blocked-ELL of it via
+    blocked_ell_from_compressed is the reference's reconstructed partition."""
+    lib = load()
+    rows = int(part.st.rows)
+    dev = default_device()
+    d_ptr = _empty_u32(rows + 1, dev)
+    n = C.c_uint64(0)
+    check(lib.csamd_canonical_compressed_from_bucketed_blocked_ell(C.byref(part.st), part.value_type, ptr(d_ptr), 0, None, None, 0,
+                                                                   C.byref(n), C.byref(ws.ws)), "canonical_compressed (size)")
+    nnz = int(n.value)
+    idx = _empty_u32(nnz, dev)
+    val = torch.empty(nnz, dtype=torch_bits_dtype(part.value_type), device=dev)
+    if nnz:
+        check(lib.csamd_canonical_compressed_from_bucketed_blocked_ell(C.byref(part.st), part.value_type, ptr(d_ptr), 1, ptr(idx),
+                                                                       ptr(val), nnz, C.byref(n), C.byref(ws.ws)), "canonical_compressed")
+    return DeviceCSR(rows, int(part.st.cols), d_ptr, idx, val, part.value_type, A.COMPRESSED_BY_ROW)
+
+
+def reconstruct_canonical_blocked_ell(part: BucketedBlockedEll, block_size: int, ws: ConvertWorkspace) -> DeviceBlockedELL:
+    """reconstruct_canonical_blocked_ell_part (execution_builders_part.hh
+    :1631-1652) on the device."""
+    return blocked_ell_from_compressed(canonical_compressed_from_bucketed_blocked_ell(part, ws), block_size, ws)
+
+
 class TransposeScratch:
     def __init__(self):
         self.buf = None

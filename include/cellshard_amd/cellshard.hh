@@ -584,6 +584,19 @@ inline int load_blocked_pack_shard_from_cspack(const file *f, execution::buckete
                                                unsigned int value_type = CSAMD_VALUE_F16) {
     return csamd_cspack_stage_shard_blocked_ell(f, value_type, ws, out);
 }
+// bucketed_partition_to_canonical_coo as a device CSR in canonical row order
+// (d_idx null: sizing only); blocked_ell_from_compressed of it is the
+// reference's reconstruct_canonical_blocked_ell_part.
+inline int canonical_compressed_from_bucketed_partition(const execution::bucketed_blocked_ell_partition *part, unsigned int *d_ptr,
+                                                        unsigned int *d_idx, void *d_val, std::uint64_t capacity,
+                                                        std::uint64_t *nnz_out, csamd_convert_workspace *ws,
+                                                        unsigned int value_type = CSAMD_VALUE_F16, bool ptr_ready = false) {
+    unsigned long long n = 0ull;
+    const int ok = csamd_canonical_compressed_from_bucketed_blocked_ell(part, value_type, d_ptr, ptr_ready ? 1 : 0, d_idx, d_val,
+                                                                        capacity, &n, ws);
+    *nnz_out = n;
+    return ok;
+}
 
 } // namespace pack
 

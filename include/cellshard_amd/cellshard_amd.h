@@ -695,6 +695,21 @@ int csamd_cspack_stage_sliced_ell(const csamd_cspack_file *f, unsigned long long
  * unpacked.  On failure every out[] is cleared. */
 int csamd_cspack_stage_shard_blocked_ell(const csamd_cspack_file *f, unsigned int value_type, csamd_cspack_workspace *ws,
                                          csamd_bucketed_blocked_ell *out);
+/* bucketed_partition_to_canonical_coo (execution_builders_part.hh:909-968), as a device CSR in
+ * canonical row order: the stored values that are not +-0 at exec columns < cols, in slot order
+ * within a row, columns mapped through exec_to_canonical_cols (identity when null).  The reference
+ * then builds the canonical Blocked-ELL with blocked_ell_from_coo (blocked_ell_from_compressed.cuh
+ * :349-434, used by load_blocked_ell_part_from_cspack, execution_runtime_part.hh:496-528);
+ * csamd_blocked_ell_from_compressed on this CSR gives the same Blocked-ELL.  d_out_ptr [rows + 1]
+ * is written and *nnz_out set; with d_out_idx null the call stops there (sizing), else capacity
+ * must hold nnz.  ptr_ready != 0 skips the counting pass: d_out_ptr then holds a sizing call's
+ * result for the same partition.  Fails when exec_to_canonical_rows is not a permutation or a
+ * mapped column is >= cols; writes never leave [0, nnz) of the outputs.  Synchronizes the
+ * workspace stream. */
+int csamd_canonical_compressed_from_bucketed_blocked_ell(const csamd_bucketed_blocked_ell *part, unsigned int value_type,
+                                                         unsigned int *d_out_ptr, int ptr_ready, unsigned int *d_out_idx,
+                                                         void *d_out_val, unsigned long long capacity, unsigned long long *nnz_out,
+                                                         csamd_convert_workspace *ws);
 
 
 

@@ -962,3 +962,50 @@ def ref_load_sliced_ell_raw(path, vb):
     _ok(ref_pack().ref_load_sliced_ell_raw(str(path).encode(), C.c_size_t(vb), _p(hdr), _p(off), _p(wid), _p(col), _p(val),
                                            C.c_ulonglong(slots)), "ref load sliced_ell")
     return hdr, off, wid[:n], col[:slots], val[:slots]
+
+
+def canonical_coo_of_bucketed_bell(part, vb):
+    """bucketed_partition_to_canonical_coo (src/io/csh5/execution_builders_part
+    .hh:909-968): per segment, per exec row, per slot with a valid block, per
+    column in the block, the stored values that are not +-0 at exec columns
+    < cols; rows mapped exec -> canonical, columns through the column map.
+    Returns (row_idx, col_idx, val bits) in that order."""
+    zero_mask = {1: 0x7F, 2: 0x7FFF, 4: 0x7FFFFFFF}[vb]
+    cols = part["cols"]
+    e2c = np.asarray(part["exec_to_canonical"], np.int64)
+    cmap = part.get("col_maps")
+    cmap = None if cmap is None else np.asarray(cmap[0], np.int64)
+    out_r, out_c, out_v = [], [], []
+    sro = part["segment_row_offsets"]
+    for k, s in enumerate(part["segments"]):
+        bs, ell = s["block_size"], s["ell_cols"]
+        w = ell // bs if bs else 0
+        bci = np.asarray(s["block_col_idx"], np.int64)
+        val = np.asarray(s["val"])
+        for r in range(s["rows"]):
+            crow = int(e2c[int(sro[k]) + r])
+            rb = r // bs
+            for slot in range(w):
+                bc = int(bci[rb * w + slot])
+                if bc == INVALID:
+                    continue
+                for cib in range(bs):
+                    ec = bc * bs + cib
+                    v = int(val[r * ell + slot * bs + cib])
+                    if (v & zero_mask) == 0 or ec >= cols:
+                        continue
+                    out_r.append(crow)
+                    out_c.append(int(cmap[ec]) if cmap is not None else ec)
+                    out_v.append(v)
+    dt = {1: np.uint8, 2: np.uint16, 4: np.uint32}[vb]
+    return np.asarray(out_r, np.uint32), np.asarray(out_c, np.uint32), np.asarray(out_v, dt)
+
+
+def canonical_csr_of_bucketed_bell(part, vb):
+    """The canonical COO grouped by row with a stable sort (each canonical
+    row comes from one exec row, so its entries keep the COO order)."""
+    r, c, v = canonical_coo_of_bucketed_bell(part, vb)
+    order = np.argsort(r, kind="stable")
+    ptr = np.zeros(part["rows"] + 1, np.int64)
+    np.add.at(ptr, r.astype(np.int64) + 1, 1)
+    return np.cumsum(ptr).astype(np.uint32), c[order], v[order]

@@ -167,3 +167,31 @@ def test_open_rejects_bad_headers(tmp_path, case):
     msg = _lib.last_error()
     assert "cspack open" in msg
     assert f.impl is None and f.partition_count == 0
+
+
+@pytest.mark.parametrize("bs,req", [(8, 3), (16, 2), (4, 5)])
+def test_canonical_coo_restatement_undoes_the_bucketing(bs, req):
+    """The canonical COO restatement inverts the bucketed execution layout
+    (restated from the reference's builder, pinned against the GPU builder
+    by the execution-layout tests): canonical Blocked-ELL -> execution
+    partition -> canonical COO -> Blocked-ELL gives back the original when
+    no stored value is zero."""
+    from helpers import F16, encode, random_csr
+    rows, cols = 203, 300
+    ptr, idx, _ = random_csr(rows, cols, 0.05, seed=bs + req)
+    bits = encode(np.random.default_rng(bs).integers(1, 9, size=len(idx)).astype(np.float64), F16)
+    bci, val, ell_cols, nnz = O.blocked_ell_from_compressed(rows, cols, ptr, idx, bits, bs)
+    lay = O.bell_execution_layout(rows, bs, ell_cols, bci, req)
+    segs, e2c = O.bell_fill_reference(rows, bs, ell_cols, bci, val, lay)
+    c2e = np.empty(rows, np.uint32)
+    c2e[e2c] = np.arange(rows, dtype=np.uint32)
+    sro = np.concatenate([[0], np.cumsum([s[0] for s in segs])]).astype(np.uint32)
+    part = dict(rows=rows, cols=cols, nnz=nnz, segment_row_offsets=sro, exec_to_canonical=e2c, canonical_to_exec=c2e,
+                col_maps=(np.arange(cols, dtype=np.uint32), np.arange(cols, dtype=np.uint32)),
+                segments=[dict(rows=r, cols=cols, nnz=0, block_size=bs, ell_cols=w * bs, block_col_idx=b, val=v)
+                          for (r, b, v), w in zip(segs, lay["width"])])
+    cptr, cidx, cval = O.canonical_csr_of_bucketed_bell(part, 2)
+    assert np.array_equal(cptr, ptr) and len(cidx) == len(idx)
+    b2, v2, e2, n2 = O.blocked_ell_from_compressed(rows, cols, cptr, cidx, cval, bs)
+    assert (e2, n2) == (ell_cols, nnz)
+    assert np.array_equal(b2, bci) and np.array_equal(v2, val)

@@ -226,3 +226,112 @@ def test_damaged_blobs_are_refused(gpu, tmp_path, case):
             _same_bell(got.to_host(), w)
             got.close()
     ws.close()
+
+
+# ---------------------------------------------------------------------------
+# Canonical reconstruction (the reference's fetch of a Blocked-ELL partition
+# from a pack: exec partition -> canonical COO -> Blocked-ELL)
+# ---------------------------------------------------------------------------
+def _exec_partition_no_repeats(rng, rows, cols, nseg, vb, block_sizes=(4, 8, 16)):
+    """Random exec partition whose row blocks never repeat a block column
+    (so every (row, column) appears once), with zero and negative-zero
+    values, invalid slots, and non-identity row and column maps."""
+    p = random_bell_partition(rng, rows, cols, nseg, vb, block_sizes)
+    for s in p["segments"]:
+        bs, w = s["block_size"], s["ell_cols"] // s["block_size"] if s["block_size"] else 0
+        nb = max(cols // bs, 1)
+        rb = (s["rows"] + bs - 1) // bs
+        w = min(w, nb)
+        s["ell_cols"] = w * bs
+        bci = np.full(rb * w, 0xFFFFFFFF, np.uint32)
+        for b in range(rb):
+            pick = rng.choice(nb, w, replace=False).astype(np.uint32)
+            pick[rng.random(w) < 0.2] = 0xFFFFFFFF
+            bci[b * w:(b + 1) * w] = pick
+        s["block_col_idx"] = bci
+        val = rng.integers(0, 1 << (8 * vb), s["rows"] * w * bs, dtype=np.uint64)
+        val[rng.random(val.size) < 0.3] = 0
+        val[rng.random(val.size) < 0.05] = 1 << (8 * vb - 1)  # -0
+        s["val"] = val.astype({1: np.uint8, 2: np.uint16, 4: np.uint32}[vb])
+    return p
+
+
+@pytest.mark.parametrize("vb", [1, 2, 4])
+def test_canonical_csr_matches_restatement(cws, tmp_path, vb):
+    rng = np.random.default_rng(60 + vb)
+    from cellshard_amd.ops import canonical_compressed_from_bucketed_blocked_ell
+    want = [_exec_partition_no_repeats(rng, r, c, n, vb) for r, c, n in [(150, 200, 3), (33, 47, 1), (0, 10, 0), (64, 500, 5)]]
+    path = tmp_path / "c.cspack"
+    path.write_bytes(O.cspack_bytes(0, [O.bell_execution_blob(w) for w in want]))
+    ws = CspackWorkspace()
+    with CspackFile(path) as f:
+        for i, w in enumerate(want):
+            print(f"partition {i} rows {w['rows']}", flush=True)
+            part = stage_cspack_blocked_ell(f, i, VT[vb], ws)
+            csr = canonical_compressed_from_bucketed_blocked_ell(part, cws)
+            p_, i_, v_ = csr.to_numpy()
+            ptr, idx, val = O.canonical_csr_of_bucketed_bell(w, vb)
+            assert np.array_equal(p_.view(np.uint32), ptr)
+            assert np.array_equal(i_.view(np.uint32), idx)
+            assert np.array_equal(v_.view(val.dtype), val)
+            if vb == 2 and w["rows"]:
+                from cellshard_amd.ops import reconstruct_canonical_blocked_ell
+                bell = reconstruct_canonical_blocked_ell(part, 8, cws)
+                bci, bval, ell, nnz = O.blocked_ell_from_compressed(w["rows"], w["cols"], ptr, idx, val, 8)
+                gb, gv = bell.to_numpy()
+                assert bell.ell_cols == ell and bell.nnz == nnz
+                assert np.array_equal(gb.view(np.uint32), bci) and np.array_equal(gv.view(np.uint16), bval)
+            part.close()
+    ws.close()
+
+
+def test_reconstructed_partition_equals_the_original(cws, tmp_path):
+    """canonical Blocked-ELL -> bucketed exec partition -> pack -> staged ->
+    reconstructed: the original Blocked-ELL bit for bit (no stored zeros)."""
+    from cellshard_amd.ops import reconstruct_canonical_blocked_ell
+    rng = np.random.default_rng(8)
+    rows, cols, bs = 301, 400, 8
+    ptr, idx = [0], []
+    for r in range(rows):
+        k = int(rng.integers(0, cols // (2 if (r // bs) % 3 == 0 else 12)))
+        idx.append(np.sort(rng.choice(cols, k, replace=False)))
+        ptr.append(ptr[-1] + k)
+    ptr = np.asarray(ptr, np.uint32)
+    idx = np.concatenate(idx).astype(np.uint32)
+    bits = encode(rng.integers(1, 9, size=len(idx)).astype(np.float64), F16)
+    bci, val, ell_cols, nnz = O.blocked_ell_from_compressed(rows, cols, ptr, idx, bits, bs)
+    src = DeviceBlockedELL.from_numpy(rows, cols, nnz, bs, ell_cols, bci, val)
+    parts = [build_bucketed_blocked_ell_device(src, 3, cws)]
+    path = tmp_path / "r.cspack"
+    ws = CspackWorkspace()
+    write_cspack_blocked_ell(path, 4, parts, 3, ws)
+    with CspackFile(path, 4) as f:
+        staged = stage_cspack_blocked_ell(f, 0, 3, ws)
+    back = reconstruct_canonical_blocked_ell(staged, bs, cws)
+    gb, gv = back.to_numpy()
+    assert back.ell_cols == ell_cols and back.nnz == nnz
+    assert np.array_equal(gb.view(np.uint32), bci) and np.array_equal(gv.view(np.uint16), val)
+    staged.close()
+    parts[0].close()
+    ws.close()
+
+
+@pytest.mark.parametrize("case", ["row_map", "col_map"])
+def test_reconstruction_rejects_bad_maps(cws, tmp_path, case):
+    from cellshard_amd.ops import canonical_compressed_from_bucketed_blocked_ell
+    rng = np.random.default_rng(2)
+    w = _exec_partition_no_repeats(rng, 40, 64, 2, 2)
+    if case == "row_map":
+        w["exec_to_canonical"][1] = w["exec_to_canonical"][0]
+    else:
+        w["col_maps"][0][:] = 64  # every mapped column past the end
+    path = tmp_path / "bad.cspack"
+    path.write_bytes(O.cspack_bytes(0, [O.bell_execution_blob(w)]))
+    ws = CspackWorkspace()
+    with CspackFile(path) as f:
+        part = stage_cspack_blocked_ell(f, 0, 3, ws)
+    with pytest.raises(RuntimeError):
+        canonical_compressed_from_bucketed_blocked_ell(part, cws)
+    assert ("permutation" if case == "row_map" else "mapped column") in _lib.last_error()
+    part.close()
+    ws.close()
