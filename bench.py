@@ -815,13 +815,21 @@ def main() -> int:
 
     if not args.skip_ops and not args.reduce_only:
         csr_bytes = (R + 1) * 4 + nnz * 6
+        # each op once untimed first (kernel code loading, allocator growth)
+        plan = build_major_nnz_bucket_plan(src, 12)
+        log2_bucket_offsets(plan["major_nnz_sorted"], 12, 5)
+        del plan
         plan, ms = timed(lambda: build_major_nnz_bucket_plan(src, 12))
         l2, ms2 = timed(lambda: log2_bucket_offsets(plan["major_nnz_sorted"], 12, 5))
         ops["bucket_plan_12_log2"] = {"ms": round(ms + ms2, 3), "gb_s": round(((R + 1) * 4 + 2 * R * 4) / ((ms + ms2) * 1e-3) / 1e9, 1),
                                       "bucket_sizes": np.diff(l2.cpu().numpy().view(np.uint32).astype(np.int64)).tolist()}
+        (reb, inv) = rebuild_compressed_major_order(src, plan["major_order"])
+        del reb, inv
         (reb, inv), ms = timed(lambda: rebuild_compressed_major_order(src, plan["major_order"]))
         ops["bucket_rebuild"] = {"ms": round(ms, 3), "gb_s": round(2 * csr_bytes / (ms * 1e-3) / 1e9, 1)}
         del inv
+        sell = sliced_ell_from_compressed(reb, 32, ws)
+        del sell
         sell, ms = timed(lambda: sliced_ell_from_compressed(reb, 32, ws))
         ops["csr_to_sell_s32_sorted"] = {"ms": round(ms, 3), "gb_s": round((csr_bytes + (2 * sell.slice_count + 1) * 4 +
                                                                             sell.total_slots * 6) / (ms * 1e-3) / 1e9, 1),
