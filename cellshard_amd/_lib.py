@@ -109,6 +109,8 @@ PROTOTYPES = {
     "csamd_cspack_stage_shard_blocked_ell": (i32, [P(A.cspack_file), u32, P(A.cspack_workspace), P(A.bucketed_blocked_ell)]),
     "csamd_canonical_compressed_from_bucketed_blocked_ell": (i32, [P(A.bucketed_blocked_ell), u32, vp, i32, vp, vp, u64, P(u64),
                                                                     P(A.convert_workspace)]),
+    "csamd_canonical_sliced_ell_from_bucketed_sliced_ell": (i32, [P(A.bucketed_sliced_ell), u32, vp, vp, vp, u64, P(u64),
+                                                                   P(A.convert_workspace)]),
     "csamd_compressed_from_coo_raw": (i32, [u32, u32, u32, vp, vp, vp, u32, vp, u32, vp, vp, vp, vp, sz, vp]),
     "csamd_label_reduce_set_timing": (i32, [P(A.label_reduce_workspace), i32]),
     "csamd_serve_init": (None, [P(A.serve_workspace)]),

@@ -1094,6 +1094,92 @@ __global__ void rc_emit_kernel(rc_args a, const unsigned int *__restrict__ ptr, 
     }
 }
 
+// ---------------------------------------------------------------------------
+// Canonical Sliced-ELL of a bucketed Sliced-ELL execution partition
+// (rebuild_canonical_sliced_partition, execution_runtime_part.hh:275-318):
+// one wave per exec row; its valid slots (column != invalid), in slot order,
+// fill the front of its canonical row, the rest of the row is invalid / 0.
+// ---------------------------------------------------------------------------
+struct rs_segment {
+    const unsigned int *offsets, *widths, *slots, *col;
+    const unsigned char *val;
+    unsigned int row0, rows, slice_count, pad;
+};
+
+struct rs_args {
+    const rs_segment *seg;
+    unsigned int nseg, rows, cslices, pad;
+    const unsigned int *e2c_rows;
+    const unsigned int *c_off, *c_wid, *c_slot;  // canonical slices (device)
+    unsigned int *hits;                          // [rows], zeroed
+    unsigned int *flags;                         // bit 0: row map, bit 1: a row's valid slots exceed its canonical width
+};
+
+__device__ __forceinline__ unsigned int rs_find(const unsigned int *off, unsigned int n, unsigned int row) {
+    unsigned int lo = 0u, hi = n - 1u;  // last slice whose start <= row
+    while (lo < hi) {
+        const unsigned int mid = (lo + hi + 1u) >> 1;
+        if (off[mid] <= row) lo = mid;
+        else hi = mid - 1u;
+    }
+    return lo;
+}
+
+template<int VB>
+__global__ void rs_rebuild_kernel(rs_args a, unsigned int *__restrict__ out_col, unsigned char *__restrict__ out_val) {
+    const unsigned int nw = (gridDim.x * blockDim.x) >> 6;
+    for (unsigned int x = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; x < a.rows; x += nw) {
+        const unsigned int c = a.e2c_rows[x];
+        unsigned int bad = 0u;
+        if (lane_id() == 0u) bad = (c >= a.rows || atomicAdd(&a.hits[c], 1u) != 0u) ? 1u : 0u;
+        bad = __shfl(bad, 0);
+        if (bad) {
+            if (lane_id() == 0u) atomicOr(a.flags, 1u);
+            continue;
+        }
+        // the canonical slices cover [0, rows) (checked on the host)
+        const unsigned int ds = rs_find(a.c_off, a.cslices, c);
+        const unsigned int dw = a.c_wid[ds];
+        const size_t dbase = (size_t) a.c_slot[ds] + (size_t) (c - a.c_off[ds]) * dw;
+        unsigned int lo = 0u, hi = a.nseg - 1u;
+        while (lo < hi) {
+            const unsigned int mid = (lo + hi + 1u) >> 1;
+            if (a.seg[mid].row0 <= x) lo = mid;
+            else hi = mid - 1u;
+        }
+        const rs_segment g = a.seg[lo];
+        const unsigned int local = x - g.row0;
+        const unsigned int ss = rs_find(g.offsets, g.slice_count, local);
+        const unsigned int sw = g.widths[ss];
+        const size_t sbase = (size_t) g.slots[ss] + (size_t) (local - g.offsets[ss]) * sw;
+        unsigned int run = 0u;
+        for (unsigned int t0 = 0u; t0 < sw; t0 += 64u) {
+            const unsigned int t = t0 + lane_id();
+            const unsigned int col = t < sw ? g.col[sbase + t] : CSAMD_INVALID_COL;
+            const bool ok = col != CSAMD_INVALID_COL;
+            const unsigned long long m = __ballot(ok);
+            if (ok) {
+                const unsigned int pos = run + (unsigned int) __popcll(m & lanemask_lt());
+                if (pos < dw) {
+                    out_col[dbase + pos] = col;
+                    if constexpr (VB == 1) out_val[dbase + pos] = g.val[sbase + t];
+                    else if constexpr (VB == 2) ((unsigned short *) out_val)[dbase + pos] = ((const unsigned short *) g.val)[sbase + t];
+                    else ((unsigned int *) out_val)[dbase + pos] = ((const unsigned int *) g.val)[sbase + t];
+                } else {
+                    atomicOr(a.flags, 2u);
+                }
+            }
+            run += (unsigned int) __popcll(m);
+        }
+        for (unsigned int j = run + lane_id(); j < dw; j += 64u) {
+            out_col[dbase + j] = CSAMD_INVALID_COL;
+            if constexpr (VB == 1) out_val[dbase + j] = 0u;
+            else if constexpr (VB == 2) ((unsigned short *) out_val)[dbase + j] = 0u;
+            else ((unsigned int *) out_val)[dbase + j] = 0u;
+        }
+    }
+}
+
 // Emit pass of the canonical CSR (row pointers in place), then the mapped
 // column check.
 int canonical_emit(const rc_args &a, int vb, unsigned int grid, const unsigned int *d_ptr, unsigned int *d_idx, void *d_val,
@@ -1359,6 +1445,86 @@ int csamd_canonical_compressed_from_bucketed_blocked_ell(const csamd_bucketed_bl
     CSAMD_REQUIRE(capacity >= h[2] && (h[2] == 0u || d_out_val != nullptr),
                   "canonical_compressed_from_bucketed_blocked_ell: capacity %llu < nnz %u", capacity, h[2]);
     return canonical_emit(a, vb, grid, d_out_ptr, d_out_idx, d_out_val, rows, h[2], flags, part->cols, s);
+}
+
+int csamd_canonical_sliced_ell_from_bucketed_sliced_ell(const csamd_bucketed_sliced_ell *part, unsigned int value_type,
+                                                        unsigned int *d_slot_offsets, unsigned int *d_col_idx, void *d_val,
+                                                        unsigned long long capacity, unsigned long long *total_slots_out,
+                                                        csamd_convert_workspace *ws) {
+    CSAMD_REQUIRE(part != nullptr && d_slot_offsets != nullptr && total_slots_out != nullptr && ws != nullptr,
+                  "canonical_sliced_ell_from_bucketed_sliced_ell: null argument");
+    const int vb = value_type_bytes(value_type);
+    CSAMD_REQUIRE(vb == 1 || vb == 2 || vb == 4, "canonical_sliced_ell_from_bucketed_sliced_ell: unsupported value type %u", value_type);
+    const unsigned int rows = part->rows, nseg = part->segment_count, ncs = part->canonical_slice_count;
+    CSAMD_REQUIRE(part->canonical_slice_row_offsets != nullptr && (ncs == 0u || part->canonical_slice_widths != nullptr),
+                  "canonical_sliced_ell_from_bucketed_sliced_ell: no canonical slice tables");
+    CSAMD_REQUIRE(rows == 0u || (part->exec_to_canonical_rows != nullptr && nseg != 0u && part->segments != nullptr &&
+                                 part->segment_row_offsets != nullptr && part->segment_row_offsets[0] == 0u &&
+                                 part->segment_row_offsets[nseg] == rows && ncs != 0u),
+                  "canonical_sliced_ell_from_bucketed_sliced_ell: partition without segments, row map or slices");
+    if (!ensure_device(ws->device >= 0 ? ws->device : 0)) return 0;
+    hipStream_t s = (hipStream_t) ws->stream;
+    std::vector<unsigned int> off(ncs + 1u), wid(ncs != 0u ? ncs : 1u), slot(ncs != 0u ? ncs : 1u);
+    CSAMD_HIP(hipMemcpyAsync(off.data(), part->canonical_slice_row_offsets, 4ull * (ncs + 1u), hipMemcpyDeviceToHost, s),
+              "D2H canonical slice offsets");
+    if (ncs != 0u)
+        CSAMD_HIP(hipMemcpyAsync(wid.data(), part->canonical_slice_widths, 4ull * ncs, hipMemcpyDeviceToHost, s), "D2H canonical widths");
+    std::vector<rs_segment> segs(nseg != 0u ? nseg : 1u);
+    for (unsigned int k = 0; k < nseg; ++k) {
+        const csamd_sliced_ell_view &v = part->segments[k];
+        const unsigned int r0 = part->segment_row_offsets[k], r1 = part->segment_row_offsets[k + 1u];
+        CSAMD_REQUIRE(r1 >= r0 && r1 - r0 == v.rows, "canonical_sliced_ell_from_bucketed_sliced_ell: segment %u rows", k);
+        CSAMD_REQUIRE(v.rows == 0u || (v.slice_count != 0u && v.slice_row_offsets != nullptr && v.slice_widths != nullptr &&
+                                       v.slice_slot_offsets != nullptr),
+                      "canonical_sliced_ell_from_bucketed_sliced_ell: segment %u has rows but no slices", k);
+        segs[k] = rs_segment{v.slice_row_offsets, v.slice_widths, v.slice_slot_offsets, v.col_idx, (const unsigned char *) v.val, r0,
+                             v.rows, v.slice_count, 0u};
+    }
+    CSAMD_HIP(hipStreamSynchronize(s), "canonical slices sync");
+    CSAMD_REQUIRE(off[0] == 0u && off[ncs] == rows, "canonical_sliced_ell_from_bucketed_sliced_ell: canonical slices cover [%u, %u), "
+                  "not [0, %u)", off[0], off[ncs], rows);
+    unsigned long long total = 0u;
+    for (unsigned int q = 0; q < ncs; ++q) {
+        CSAMD_REQUIRE(off[q + 1u] >= off[q], "canonical_sliced_ell_from_bucketed_sliced_ell: canonical slice offsets decrease");
+        slot[q] = (unsigned int) total;
+        total += (unsigned long long) (off[q + 1u] - off[q]) * wid[q];
+        CSAMD_REQUIRE(total <= 0xffffffffull, "canonical_sliced_ell_from_bucketed_sliced_ell: 2^32 slots or more");
+    }
+    *total_slots_out = total;
+    if (ncs != 0u)
+        CSAMD_HIP(hipMemcpyAsync(d_slot_offsets, slot.data(), 4ull * ncs, hipMemcpyHostToDevice, s), "H2D canonical slot offsets");
+    if (d_col_idx == nullptr) {
+        CSAMD_HIP(hipStreamSynchronize(s), "canonical slot offsets sync");
+        return 1;
+    }
+    CSAMD_REQUIRE(capacity >= total && (total == 0u || d_val != nullptr),
+                  "canonical_sliced_ell_from_bucketed_sliced_ell: capacity %llu < %llu slots", capacity, total);
+    const size_t seg_bytes = align_up(segs.size() * sizeof(rs_segment), 256u);
+    const size_t row_bytes = align_up((size_t) rows * 4u + 4u, 256u);
+    char *scratch = (char *) convert_scratch(ws, seg_bytes + row_bytes + 256u);
+    if (scratch == nullptr) return 0;
+    rs_segment *d_seg = (rs_segment *) scratch;
+    unsigned int *hits = (unsigned int *) (scratch + seg_bytes);
+    unsigned int *flags = (unsigned int *) (scratch + seg_bytes + row_bytes);
+    CSAMD_HIP(hipMemcpyAsync(d_seg, segs.data(), segs.size() * sizeof(rs_segment), hipMemcpyHostToDevice, s), "H2D sliced segments");
+    if (!zero_bytes(hits, row_bytes + 256u, s)) return 0;  // hits and flags
+    if (rows != 0u) {
+        const rs_args a{d_seg, nseg, rows, ncs, 0u, part->exec_to_canonical_rows, part->canonical_slice_row_offsets,
+                        part->canonical_slice_widths, d_slot_offsets, hits, flags};
+        const unsigned int grid = grid_for((unsigned long long) rows * 64u, 256u, (unsigned int) device_cu_count() * 16u);
+        switch (vb) {
+            case 1: rs_rebuild_kernel<1><<<grid, 256, 0, s>>>(a, d_col_idx, (unsigned char *) d_val); break;
+            case 2: rs_rebuild_kernel<2><<<grid, 256, 0, s>>>(a, d_col_idx, (unsigned char *) d_val); break;
+            default: rs_rebuild_kernel<4><<<grid, 256, 0, s>>>(a, d_col_idx, (unsigned char *) d_val); break;
+        }
+        CSAMD_HIP(hipGetLastError(), "rs_rebuild_kernel");
+    }
+    unsigned int h = 0u;
+    CSAMD_HIP(hipMemcpyAsync(&h, flags, 4u, hipMemcpyDeviceToHost, s), "D2H flags");
+    CSAMD_HIP(hipStreamSynchronize(s), "canonical sliced sync");
+    CSAMD_REQUIRE((h & 1u) == 0u, "canonical_sliced_ell_from_bucketed_sliced_ell: exec_to_canonical_rows is not a permutation");
+    CSAMD_REQUIRE((h & 2u) == 0u, "canonical_sliced_ell_from_bucketed_sliced_ell: a row's valid slots exceed its canonical width");
+    return 1;
 }
 
 } // extern "C"

@@ -13,7 +13,7 @@ import torch
 
 from . import abi as A
 from ._lib import check, load
-from .device import (DeviceBlockedELL, DeviceCSR, DeviceSlicedELL, default_device, ptr, to_dev_f32, torch_bits_dtype,
+from .device import (DeviceBlockedELL, DeviceCSR, DeviceSlicedELL, default_device, ptr, to_dev_f32, to_dev_u32, torch_bits_dtype,
                      value_bytes)
 
 
@@ -676,6 +676,32 @@ def reconstruct_canonical_blocked_ell(part: BucketedBlockedEll, block_size: int,
     """reconstruct_canonical_blocked_ell_part (execution_builders_part.hh
     :1631-1652) on the device."""
     return blocked_ell_from_compressed(canonical_compressed_from_bucketed_blocked_ell(part, ws), block_size, ws)
+
+
+def rebuild_canonical_sliced_ell(part: BucketedSlicedEll, ws: ConvertWorkspace) -> DeviceSlicedELL:
+    """rebuild_canonical_sliced_partition (execution_runtime_part.hh
+    :275-318) on the device: the canonical Sliced-ELL in the partition's
+    canonical slice layout."""
+    lib = load()
+    st = part.st
+    dev = default_device()
+    ncs = int(st.canonical_slice_count)
+    slot = _empty_u32(max(ncs, 1), dev)
+    n = C.c_uint64(0)
+    check(lib.csamd_canonical_sliced_ell_from_bucketed_sliced_ell(C.byref(st), part.value_type, ptr(slot), None, None, 0, C.byref(n),
+                                                                  C.byref(ws.ws)), "canonical_sliced_ell (size)")
+    total = int(n.value)
+    col = _empty_u32(total, dev)
+    val = torch.empty(total, dtype=torch_bits_dtype(part.value_type), device=dev)
+    if total:
+        check(lib.csamd_canonical_sliced_ell_from_bucketed_sliced_ell(C.byref(st), part.value_type, ptr(slot), ptr(col), ptr(val), total,
+                                                                      C.byref(n), C.byref(ws.ws)), "canonical_sliced_ell")
+    off = device_to_numpy(st.canonical_slice_row_offsets, ncs + 1, np.uint32)
+    wid = device_to_numpy(st.canonical_slice_widths, ncs, np.uint32)
+    h = np.diff(off.astype(np.int64))
+    uniform = ncs > 0 and bool(np.all(h[:-1] == h[0])) and h[-1] <= h[0]
+    return DeviceSlicedELL(int(st.rows), int(st.cols), int(st.nnz), ncs, int(h[0]) if uniform else 0, to_dev_u32(off, dev),
+                           to_dev_u32(wid, dev), slot[:ncs], col, val, part.value_type)
 
 
 class TransposeScratch:

@@ -584,6 +584,15 @@ inline int load_blocked_pack_shard_from_cspack(const file *f, execution::buckete
                                                unsigned int value_type = CSAMD_VALUE_F16) {
     return csamd_cspack_stage_shard_blocked_ell(f, value_type, ws, out);
 }
+// rebuild_canonical_sliced_partition on the device (d_col null: sizing only).
+inline int rebuild_canonical_sliced_partition(const execution::bucketed_sliced_ell_partition *part, unsigned int *d_slot_offsets,
+                                              unsigned int *d_col, void *d_val, std::uint64_t capacity, std::uint64_t *slots_out,
+                                              csamd_convert_workspace *ws, unsigned int value_type = CSAMD_VALUE_F16) {
+    unsigned long long n = 0ull;
+    const int ok = csamd_canonical_sliced_ell_from_bucketed_sliced_ell(part, value_type, d_slot_offsets, d_col, d_val, capacity, &n, ws);
+    *slots_out = n;
+    return ok;
+}
 // bucketed_partition_to_canonical_coo as a device CSR in canonical row order
 // (d_idx null: sizing only); blocked_ell_from_compressed of it is the
 // reference's reconstruct_canonical_blocked_ell_part.

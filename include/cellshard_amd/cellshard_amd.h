@@ -710,6 +710,20 @@ int csamd_canonical_compressed_from_bucketed_blocked_ell(const csamd_bucketed_bl
                                                          unsigned int *d_out_ptr, int ptr_ready, unsigned int *d_out_idx,
                                                          void *d_out_val, unsigned long long capacity, unsigned long long *nnz_out,
                                                          csamd_convert_workspace *ws);
+/* rebuild_canonical_sliced_partition (execution_runtime_part.hh:275-318, used by
+ * load_sliced_ell_part_from_cspack, :464-494): the canonical Sliced-ELL of a bucketed Sliced-ELL
+ * partition in the canonical slice layout the partition carries (canonical_slice_row_offsets /
+ * widths).  Each canonical row holds its exec row's valid slots (column != invalid) in slot order,
+ * then invalid slots with zero values.  d_slot_offsets [canonical_slice_count] is written and
+ * *total_slots_out set; with d_col_idx null the call stops there (sizing), else capacity must hold
+ * the slots.  Fails when a row's valid slots exceed its canonical width (as the reference does), when
+ * exec_to_canonical_rows is not a permutation, or when the canonical slices do not cover the rows.
+ * Segments may hold several slices (the reference reads only the first slice's width).
+ * Synchronizes the workspace stream. */
+int csamd_canonical_sliced_ell_from_bucketed_sliced_ell(const csamd_bucketed_sliced_ell *part, unsigned int value_type,
+                                                        unsigned int *d_slot_offsets, unsigned int *d_col_idx, void *d_val,
+                                                        unsigned long long capacity, unsigned long long *total_slots_out,
+                                                        csamd_convert_workspace *ws);
 
 
 

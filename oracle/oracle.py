@@ -1009,3 +1009,49 @@ def canonical_csr_of_bucketed_bell(part, vb):
     ptr = np.zeros(part["rows"] + 1, np.int64)
     np.add.at(ptr, r.astype(np.int64) + 1, 1)
     return np.cumsum(ptr).astype(np.uint32), c[order], v[order]
+
+
+def _slot_offsets(off, wid):
+    off = np.asarray(off, np.int64)
+    wid = np.asarray(wid, np.int64)
+    return np.concatenate([[0], np.cumsum(np.diff(off) * wid)])[:-1] if len(wid) else np.zeros(0, np.int64)
+
+
+def canonical_sliced_of_bucketed_sell(part, vb):
+    """rebuild_canonical_sliced_partition (src/io/csh5/execution_runtime_part
+    .hh:275-318): the canonical Sliced-ELL in the partition's canonical slice
+    layout; exec row x's valid slots (column != invalid), in slot order, at
+    the front of canonical row exec_to_canonical[x], invalid / 0 after.  The
+    exec row is found in its segment's own slices (the reference reads the
+    first slice's width, the same for the one-slice segments its builder
+    emits).  Returns (slot offsets, col_idx, val bits); raises when a row's
+    valid slots exceed its canonical width, as the reference fails."""
+    off = np.asarray(part["canonical_slice_row_offsets"], np.int64)
+    wid = np.asarray(part["canonical_slice_widths"], np.int64)
+    slots = _slot_offsets(off, wid)
+    total = int(slots[-1] + (off[-1] - off[-2]) * wid[-1]) if len(wid) else 0
+    dt = {1: np.uint8, 2: np.uint16, 4: np.uint32}[vb]
+    col = np.full(total, INVALID, np.uint32)
+    val = np.zeros(total, dt)
+    sro = np.asarray(part["segment_row_offsets"], np.int64)
+    e2c = np.asarray(part["exec_to_canonical"], np.int64)
+    for x in range(part["rows"]):
+        c = int(e2c[x])
+        ds = int(np.searchsorted(off[:-1], c, side="right") - 1)
+        dbase = int(slots[ds] + (c - off[ds]) * wid[ds])
+        k = next(k for k in range(len(part["segments"])) if sro[k] <= x < sro[k + 1])
+        s = part["segments"][k]
+        local = x - int(sro[k])
+        so = np.asarray(s["slice_row_offsets"], np.int64)
+        sw = np.asarray(s["slice_widths"], np.int64)
+        q = int(np.searchsorted(so[:-1], local, side="right") - 1)
+        sbase = int(_slot_offsets(so, sw)[q] + (local - so[q]) * sw[q])
+        sc = np.asarray(s["col_idx"], np.uint32)[sbase:sbase + int(sw[q])]
+        sv = np.asarray(s["val"])[sbase:sbase + int(sw[q])]
+        keep = sc != INVALID
+        n = int(keep.sum())
+        if n > wid[ds]:
+            raise ValueError(f"row {c}: {n} valid slots exceed width {wid[ds]}")
+        col[dbase:dbase + n] = sc[keep]
+        val[dbase:dbase + n] = sv[keep]
+    return slots.astype(np.uint32), col, val

@@ -195,3 +195,35 @@ def test_canonical_coo_restatement_undoes_the_bucketing(bs, req):
     b2, v2, e2, n2 = O.blocked_ell_from_compressed(rows, cols, cptr, cidx, cval, bs)
     assert (e2, n2) == (ell_cols, nnz)
     assert np.array_equal(b2, bci) and np.array_equal(v2, val)
+
+
+def test_canonical_sliced_restatement_undoes_a_row_permutation():
+    """A canonical Sliced-ELL moved into one permuted exec segment (one slice
+    as wide as its widest row) is rebuilt slot for slot by the restatement
+    of rebuild_canonical_sliced_partition."""
+    from helpers import F16, encode, random_csr
+    rows, cols = 90, 120
+    ptr, idx, _ = random_csr(rows, cols, 0.08, seed=3, empty_rows=0.1)
+    bits = encode(np.random.default_rng(3).integers(1, 9, size=len(idx)).astype(np.float64), F16)
+    s = O.sliced_ell_from_compressed(rows, cols, 0, ptr, idx, bits, 16)
+    off, wid, slots = s["offsets"].astype(np.int64), s["widths"].astype(np.int64), s["slot_offsets"].astype(np.int64)
+    W = int(wid.max())
+    perm = np.random.default_rng(4).permutation(rows).astype(np.uint32)
+    col = np.full(rows * W, 0xFFFFFFFF, np.uint32)
+    val = np.zeros(rows * W, np.uint16)
+    for x, c in enumerate(perm):
+        q = int(np.searchsorted(off[:-1], c, side="right") - 1)
+        b = int(slots[q] + (c - off[q]) * wid[q])
+        cc, vv = s["col_idx"][b:b + int(wid[q])], s["val"][b:b + int(wid[q])]
+        k = cc != 0xFFFFFFFF
+        col[x * W:x * W + int(k.sum())] = cc[k]
+        val[x * W:x * W + int(k.sum())] = vv[k]
+    inv = np.empty(rows, np.uint32)
+    inv[perm] = np.arange(rows, dtype=np.uint32)
+    part = dict(rows=rows, cols=cols, nnz=len(idx), segment_row_offsets=np.array([0, rows], np.uint32), exec_to_canonical=perm,
+                canonical_to_exec=inv, canonical_slice_row_offsets=s["offsets"], canonical_slice_widths=s["widths"],
+                segments=[dict(rows=rows, cols=cols, nnz=len(idx), slice_count=1, slice_row_offsets=np.array([0, rows], np.uint32),
+                               slice_widths=np.array([W], np.uint32), col_idx=col, val=val)])
+    got_slots, got_col, got_val = O.canonical_sliced_of_bucketed_sell(part, 2)
+    assert np.array_equal(got_slots, s["slot_offsets"])
+    assert np.array_equal(got_col, s["col_idx"]) and np.array_equal(got_val, s["val"])

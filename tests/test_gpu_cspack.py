@@ -335,3 +335,89 @@ def test_reconstruction_rejects_bad_maps(cws, tmp_path, case):
     assert ("permutation" if case == "row_map" else "mapped column") in _lib.last_error()
     part.close()
     ws.close()
+
+
+def _with_canonical_slices(rng, p):
+    """Gives a random Sliced-ELL exec partition a canonical slice layout
+    that covers its rows: uniform heights (short last slice), each slice as
+    wide as its rows' valid slots need plus 0-2 spare."""
+    rows = p["rows"]
+    need = np.zeros(rows, np.int64)
+    sro = p["segment_row_offsets"]
+    for k, s in enumerate(p["segments"]):
+        so = s["slice_row_offsets"].astype(np.int64)
+        sw = s["slice_widths"].astype(np.int64)
+        slots = O._slot_offsets(so, sw)
+        for local in range(s["rows"]):
+            q = int(np.searchsorted(so[:-1], local, side="right") - 1)
+            b = int(slots[q] + (local - so[q]) * sw[q])
+            need[p["exec_to_canonical"][int(sro[k]) + local]] = int((s["col_idx"][b:b + int(sw[q])] != 0xFFFFFFFF).sum())
+    h = int(rng.integers(1, 9))
+    off = np.unique(np.concatenate([np.arange(0, rows, h), [rows]])).astype(np.uint32) if rows else np.zeros(1, np.uint32)
+    wid = np.array([need[off[i]:off[i + 1]].max() + int(rng.integers(0, 3)) for i in range(len(off) - 1)], np.uint32)
+    p["canonical_slice_row_offsets"] = off
+    p["canonical_slice_widths"] = wid
+    return p
+
+
+@pytest.mark.parametrize("vb", [1, 2, 4])
+def test_canonical_sliced_rebuild_matches_restatement(cws, tmp_path, vb):
+    from cellshard_amd.ops import rebuild_canonical_sliced_ell
+    rng = np.random.default_rng(70 + vb)
+    want = [_with_canonical_slices(rng, random_sell_partition(rng, r, c, n, vb)) for r, c, n in [(200, 300, 4), (31, 40, 7), (1, 5, 1)]]
+    path = tmp_path / "s.cspack"
+    path.write_bytes(O.cspack_bytes(0, [O.sell_execution_blob(w) for w in want]))
+    ws = CspackWorkspace()
+    with CspackFile(path) as f:
+        for i, w in enumerate(want):
+            print(f"partition {i} rows {w['rows']}", flush=True)
+            part = stage_cspack_sliced_ell(f, i, VT[vb], ws)
+            got = rebuild_canonical_sliced_ell(part, cws)
+            slots, col, val = O.canonical_sliced_of_bucketed_sell(w, vb)
+            o, wd, so, gc, gv = got.to_numpy()
+            assert np.array_equal(o.view(np.uint32), w["canonical_slice_row_offsets"])
+            assert np.array_equal(wd.view(np.uint32), w["canonical_slice_widths"])
+            assert np.array_equal(so.view(np.uint32), slots)
+            assert np.array_equal(gc.view(np.uint32), col) and np.array_equal(gv.view(val.dtype), val)
+            part.close()
+    ws.close()
+
+
+def test_canonical_sliced_round_trip(cws, tmp_path):
+    """canonical Sliced-ELL -> bucketed exec partition -> pack -> staged ->
+    rebuilt: the original, slot for slot."""
+    from cellshard_amd.ops import rebuild_canonical_sliced_ell
+    dev = _sell_dev(150, 200, 32, 99)
+    orig = dev.to_numpy()
+    part = build_bucketed_sliced_ell_device(dev, 3, cws)
+    path = tmp_path / "rt.cspack"
+    ws = CspackWorkspace()
+    write_cspack_sliced_ell(path, 1, [part], 3, ws)
+    with CspackFile(path, 1) as f:
+        staged = stage_cspack_sliced_ell(f, 0, 3, ws)
+    back = rebuild_canonical_sliced_ell(staged, cws)
+    assert back.slice_rows == dev.slice_rows and back.nnz == dev.nnz
+    for a, b in zip(back.to_numpy(), orig):
+        assert np.array_equal(a, b)
+    staged.close()
+    part.close()
+    ws.close()
+
+
+def test_canonical_sliced_rebuild_rejects_narrow_slices(cws, tmp_path):
+    from cellshard_amd.ops import rebuild_canonical_sliced_ell
+    rng = np.random.default_rng(5)
+    w = _with_canonical_slices(rng, random_sell_partition(rng, 60, 50, 2, 2))
+    w["canonical_slice_widths"] = np.maximum(w["canonical_slice_widths"].astype(np.int64) - 3, 0).astype(np.uint32)
+    with pytest.raises(ValueError):
+        O.canonical_sliced_of_bucketed_sell(w, 2)
+    path = tmp_path / "n.cspack"
+    path.write_bytes(O.cspack_bytes(0, [O.sell_execution_blob(w)]))
+    ws = CspackWorkspace()
+    with CspackFile(path) as f:
+        part = stage_cspack_sliced_ell(f, 0, 3, ws)
+    with pytest.raises(RuntimeError):
+        rebuild_canonical_sliced_ell(part, cws)
+    assert "exceed" in _lib.last_error()
+    part.close()
+    ws.close()
