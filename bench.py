@@ -58,6 +58,8 @@ def parse_args():
     ap.add_argument("--skip-plain", action="store_true", help="skip the 20,000 x 33,538 plain-path measurement")
     ap.add_argument("--plain-only", action="store_true", help="only the plain-path measurement")
     ap.add_argument("--skip-cspack", action="store_true", help="skip the CSPACK staging measurement")
+    ap.add_argument("--skip-refabi", action="store_true", help="skip the reference-ABI entry point measurement")
+    ap.add_argument("--refabi-only", action="store_true", help="only the reference-ABI entry point measurement")
     ap.add_argument("--cspack-only", action="store_true", help="only the CSPACK staging measurement")
     return ap.parse_args()
 
@@ -520,6 +522,104 @@ def run_cspack(args, log):
     return res
 
 
+def run_reference_abi(args, log):
+    """The reference-ABI entry points on the plain-path shape: CSR 20,000 x
+    33,538 with fp16 values (the reference's storage type), its Blocked-ELL 16
+    and Sliced-ELL 32.  compute_sparse_group_reduce over 32 feature groups
+    (row keep 0.9, 2,000 kept features) on each layout, the masked
+    reoptimize of the Blocked-ELL to a compacted CSR, and Blocked-ELL ->
+    Sliced-ELL.  Each timed after an untimed first call."""
+    import numpy as np
+    import torch
+
+    from cellshard_amd import abi as A
+    from cellshard_amd import runtime as rt
+    from cellshard_amd.device import ptr, to_dev_u32, to_dev_u8
+    from cellshard_amd.ops import (ConvertWorkspace, blocked_ell_from_compressed, sliced_ell_from_blocked_ell,
+                                   sliced_ell_from_compressed, synth_csr)
+
+    dev = torch.device("cuda", 0)
+    R, Cc, G, FK = 20_000, 33_538, 32, 2000
+    ws = ConvertWorkspace()
+    src = synth_csr(R, Cc, median=1500.0, sigma=0.5, seed=1, value_type=A.VALUE_F16, ws=ws)
+    bell = blocked_ell_from_compressed(src, 16, ws)
+    sell = sliced_ell_from_compressed(src, 32, ws)
+    rng = np.random.default_rng(3)
+    rk = (rng.random(R) < 0.9).astype(np.uint8)
+    fk = np.zeros(Cc, np.uint8)
+    fk[rng.choice(Cc, FK, replace=False)] = 1
+    fgm = rng.integers(0, 2 ** 32, size=Cc, dtype=np.uint64).astype(np.uint32)
+    d_rk, d_fk, d_fgm = to_dev_u8(rk, dev), to_dev_u8(fk, dev), to_dev_u32(fgm, dev)
+    gws = A.sparse_group_reduce_workspace()
+    rt.init(gws)
+    assert rt.setup(gws, 0, torch.cuda.current_stream().cuda_stream) == 1
+    groups = A.group_mask_config_view(group_count=G, feature_group_masks=ptr(d_fgm))
+    masks = A.row_feature_mask_view(row_keep=ptr(d_rk), feature_keep=ptr(d_fk))
+    filt = A.sparse_group_filter_params(0.0, 0, 1.0, 0)
+    res = A.sparse_group_reduce_result()
+
+    def ms(fn, reps=5):
+        assert fn() == 1
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            assert fn() == 1
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / reps
+
+    row_len = np.diff(src.major_ptr.cpu().numpy().view(np.uint32).astype(np.int64))
+    kept = rk == 1
+    extra = R * (1 + 4 + 4 + 4 + 1) + Cc * (1 + 4) + R * G * 8  # row keep, outputs, feature keep + group masks, group outputs
+    out = {}
+    t = ms(lambda: rt.compute_sparse_group_reduce_compressed_fallback(src.view(), gws, groups, masks, filt, res))
+    b = int(row_len[kept].sum()) * 6 + (R + 1) * 4 + extra
+    out["group_reduce_csr"] = {"ms": round(t, 3), "gb_s": round(b / (t * 1e-3) / 1e9, 1)}
+    sw = sell.slice_widths.cpu().numpy().view(np.uint32).astype(np.int64)
+    so = sell.slice_row_offsets.cpu().numpy().view(np.uint32).astype(np.int64)
+    row_w = np.repeat(sw, np.diff(so))
+    t = ms(lambda: rt.compute_sparse_group_reduce(sell.view(), gws, groups, masks, filt, res))
+    b = int(row_w[kept].sum()) * 6 + sell.slice_count * 12 + extra
+    out["group_reduce_sliced_ell_32"] = {"ms": round(t, 3), "gb_s": round(b / (t * 1e-3) / 1e9, 1)}
+    t = ms(lambda: rt.compute_sparse_group_reduce(bell.view(), gws, groups, masks, filt, res))
+    b = int(kept.sum()) * bell.ell_cols * 2 + bell.row_blocks * (bell.ell_cols // 16) * 4 + extra
+    out["group_reduce_blocked_ell_16"] = {"ms": round(t, 3), "gb_s": round(b / (t * 1e-3) / 1e9, 1),
+                                          "bytes_over_csr_reduce": round(b / (int(row_len[kept].sum()) * 6 + (R + 1) * 4 + extra), 2)}
+    rt.clear(gws)
+    # masked reoptimize: kept rows and features compacted by remaps into a CSR
+    rr = np.full(R, 0xFFFFFFFF, np.uint32)
+    rr[kept] = np.arange(int(kept.sum()))
+    cr = np.full(Cc, 0xFFFFFFFF, np.uint32)
+    cr[fk == 1] = np.arange(FK)
+    d_rr, d_cr = to_dev_u32(rr, dev), to_dev_u32(cr, dev)
+    rws = A.masked_sparse_reoptimize_workspace()
+    rt.init(rws)
+    assert rt.setup(rws, 0, torch.cuda.current_stream().cuda_stream) == 1
+    rmasks = A.row_feature_mask_view(row_keep=ptr(d_rk), feature_keep=ptr(d_fk), row_remap=ptr(d_rr), feature_remap=ptr(d_cr))
+    cfg = A.masked_sparse_reoptimize_config(int(kept.sum()), FK, 0, 16, A.LAYOUT_COMPRESSED)
+    ro = A.masked_sparse_reoptimize_result()
+    t = ms(lambda: rt.manual_reoptimize_masked_sparse(bell.view(), rmasks, cfg, rws, ro))
+    live = int(ro.metadata.live_nnz)
+    b = int(kept.sum()) * bell.ell_cols * 2 + bell.row_blocks * (bell.ell_cols // 16) * 4 + live * 6 + (int(kept.sum()) + 1) * 4
+    out["reoptimize_bell16_to_csr"] = {"ms": round(t, 3), "gb_s": round(b / (t * 1e-3) / 1e9, 1), "live_nnz": live}
+    rt.clear(rws)
+    del sell
+    torch.cuda.empty_cache()
+    sliced_ell_from_blocked_ell(bell, 32, ws)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s2 = sliced_ell_from_blocked_ell(bell, 32, ws)
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - t0) * 1e3
+    b = R * bell.ell_cols * 2 + bell.row_blocks * (bell.ell_cols // 16) * 4 + s2.total_slots * 6 + (2 * s2.slice_count + 1) * 4
+    out["blocked_ell_16_to_sliced_ell_32"] = {"ms": round(t, 3), "gb_s": round(b / (t * 1e-3) / 1e9, 1)}
+    del s2, bell, src
+    ws.close()
+    torch.cuda.empty_cache()
+    res_ = {"workload": "reference_abi_csr_20000x33538_fp16_32groups_bell16_sell32", **out}
+    log(f"reference abi: {json.dumps(res_)}")
+    return res_
+
+
 def run_batched_partitions(args, log):
     """Batched uneven partitions: 256 CSR partitions (rows U[512, 65,536],
     median row length U[300, 5,000], LogNormal sigma 0.7, 36,601 columns,
@@ -678,6 +778,10 @@ def main() -> int:
     if args.plain_only:
         torch.cuda.set_device(0)
         print(json.dumps({"plain_path": run_plain_path(args, log)}), flush=True)
+        return 0
+    if args.refabi_only:
+        torch.cuda.set_device(0)
+        print(json.dumps({"reference_abi": run_reference_abi(args, log)}), flush=True)
         return 0
     if args.cspack_only:
         torch.cuda.set_device(0)
@@ -894,6 +998,9 @@ def main() -> int:
     batched = None
     if not args.skip_batched and not args.reduce_only:
         batched = run_batched_partitions(args, log)
+    refabi = None
+    if not args.skip_refabi and not args.reduce_only:
+        refabi = run_reference_abi(args, log)
     cspack = None
     if not args.skip_cspack and not args.reduce_only:
         cspack = run_cspack(args, log)
@@ -925,6 +1032,7 @@ def main() -> int:
         "plain_path": plain,
         "reduced_precision": lowp,
         "batched_partitions": batched,
+        "reference_abi": refabi,
         "cspack": cspack,
         "serving": serving,
         "generate_s": round(gen_s, 2),

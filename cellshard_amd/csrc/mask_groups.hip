@@ -36,25 +36,61 @@ __device__ __forceinline__ bool feature_participates(const csamd_row_feature_mas
 __device__ __forceinline__ void lds_fence() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup"); }
 
 // Per-element accumulation shared by all three layouts.
+// Group sums live in each lane's registers (one per group, combined by
+// wave reductions at the row's end): an entry adds its value to the groups
+// of its feature mask without LDS atomics, which serialise when many lanes
+// add to one group.
 struct row_acc {
     float sum;
     float vmax;
     unsigned int detected;
+    float g[CSAMD_MAX_MASK_GROUPS];
 };
+
+__device__ __forceinline__ row_acc row_acc_zero() {
+    row_acc a;
+    a.sum = 0.0f;
+    a.vmax = 0.0f;
+    a.detected = 0u;
+#pragma unroll
+    for (unsigned int k = 0; k < CSAMD_MAX_MASK_GROUPS; ++k) a.g[k] = 0.0f;
+    return a;
+}
+
+__device__ __forceinline__ void add_groups(row_acc &a, uint32_t m, float v) {
+#pragma unroll
+    for (unsigned int k = 0; k < CSAMD_MAX_MASK_GROUPS; ++k) a.g[k] += ((m >> k) & 1u) != 0u ? v : 0.0f;
+}
+
+__device__ __forceinline__ uint32_t group_bits(unsigned int group_count) {
+    return group_count >= 32u ? 0xffffffffu : (1u << group_count) - 1u;
+}
+
+// Feature keep flags: a bitmap staged in LDS (kl) when the call provides
+// one, else the mask bytes.  col must be < cols.
+__device__ __forceinline__ bool keep_of(const unsigned int *kl, const csamd_row_feature_mask_view &masks, unsigned int col) {
+    if (kl != nullptr) return ((kl[col >> 5] >> (col & 31u)) & 1u) != 0u;
+    return feature_participates(masks, col);
+}
+
+// An entry whose column test (col < cols, feature kept) is known; gbits
+// selects the call's groups (0: no group masks).
+__device__ __forceinline__ void accumulate_kept(row_acc &a, bool kept, unsigned int col, float v, const uint32_t *__restrict__ fgm,
+                                                uint32_t gbits) {
+    if (kept && v != 0.0f) {
+        a.sum += v;
+        if (gbits != 0u) add_groups(a, fgm[col] & gbits, v);
+        a.vmax = fmaxf(a.vmax, v);
+        a.detected += 1u;
+    }
+}
 
 __device__ __forceinline__ void accumulate(row_acc &a, unsigned int col, float v, unsigned int cols,
                                            const csamd_row_feature_mask_view &masks, const uint32_t *__restrict__ fgm,
                                            unsigned int group_count, float *wave_groups) {
     if (col < cols && v != 0.0f && feature_participates(masks, col)) {
         a.sum += v;
-        if (fgm != nullptr && group_count != 0u) {
-            uint32_t m = fgm[col];
-            while (m != 0u) {
-                const unsigned int g = (unsigned int) __builtin_ctz(m);
-                if (g < group_count) atomicAdd(wave_groups + g, v);
-                m &= m - 1u;
-            }
-        }
+        if (fgm != nullptr && group_count != 0u) add_groups(a, fgm[col] & group_bits(group_count), v);
         a.vmax = fmaxf(a.vmax, v);
         a.detected += 1u;
     }
@@ -84,6 +120,16 @@ __device__ __forceinline__ void finish_row(unsigned int row, row_acc a, unsigned
     const float row_sum = wave_sum_f32(a.sum);
     const float vmax = wave_max_f32(a.vmax);
     const unsigned int detected = wave_sum_u32(a.detected);
+    if (group_count != 0u) {
+        // the lanes' group sums to the wave's bins: lane k writes group k
+        float mine = 0.0f;
+#pragma unroll
+        for (unsigned int k = 0; k < CSAMD_MAX_MASK_GROUPS; ++k) {
+            const float t = wave_sum_f32(a.g[k]);
+            if (lane == k) mine = t;
+        }
+        if (lane < CSAMD_MAX_MASK_GROUPS) ((float *) wave_groups)[lane] = mine;
+    }
     lds_fence();
     if (lane == 0u) {
         o.row_totals[row] = row_sum;
@@ -103,16 +149,120 @@ __device__ __forceinline__ void finish_row(unsigned int row, row_acc a, unsigned
     lds_fence();
 }
 
+// Units of 4 consecutive values: one 4 / 8 / 16-byte load, decoded after
+// all of a pass's loads are issued.
+template<int VT> struct mg_unit;
+template<> struct mg_unit<CSAMD_VALUE_F16> {
+    using raw = uint2;
+    static __device__ __forceinline__ float get(raw r, unsigned int k) {
+        const unsigned int w = k < 2u ? r.x : r.y;
+        return f16_bits_to_float((uint16_t) (w >> (16u * (k & 1u))));
+    }
+};
+template<> struct mg_unit<CSAMD_VALUE_BF16> {
+    using raw = uint2;
+    static __device__ __forceinline__ float get(raw r, unsigned int k) {
+        const unsigned int w = k < 2u ? r.x : r.y;
+        return bf16_bits_to_float((uint16_t) (w >> (16u * (k & 1u))));
+    }
+};
+template<> struct mg_unit<CSAMD_VALUE_F32> {
+    using raw = float4;
+    static __device__ __forceinline__ float get(raw r, unsigned int k) { return k == 0u ? r.x : k == 1u ? r.y : k == 2u ? r.z : r.w; }
+};
+template<> struct mg_unit<CSAMD_VALUE_FP8_E4M3> {
+    using raw = unsigned int;
+    static __device__ __forceinline__ float get(raw r, unsigned int k) { return fp8_e4m3_decode_hw(r, (int) k); }
+};
+
+constexpr unsigned int MG_UNROLL = 4u;  // units in flight per lane
+
+// Whether entry arrays idx (u32) and val (VT) can be read in units: both
+// 4-byte / value aligned and the same number of entries past a unit
+// boundary (then unit-aligned entries are aligned in both).
+template<int VT>
+__device__ __forceinline__ bool mg_units_ok(const void *idx, const void *val, unsigned int *mis) {
+    constexpr unsigned int VB = sizeof(typename value_traits<VT>::storage);
+    const uintptr_t ia = (uintptr_t) idx, va = (uintptr_t) val;
+    *mis = (unsigned int) ((ia >> 2) & 3u);
+    return (ia & 3u) == 0u && (va % VB) == 0u && (unsigned int) ((va / VB) & 3u) == *mis;
+}
+
+// Entries [b, e) of paired arrays (column, value), accumulated: the
+// unit-aligned middle by 16-byte index loads and 4-value loads, MG_UNROLL
+// units in flight per lane (loads clamped to the last unit, never under a
+// branch); the up to 3 + 3 entries outside whole units by lanes 0-5.
+template<int VT>
+__device__ __forceinline__ void mg_stream(row_acc &a, const unsigned int *__restrict__ idx,
+                                          const typename value_traits<VT>::storage *__restrict__ val, unsigned long long b,
+                                          unsigned long long e, bool units_ok, unsigned int mis, unsigned int cols,
+                                          const csamd_row_feature_mask_view &masks, const uint32_t *__restrict__ fgm,
+                                          unsigned int group_count, float *wg, const unsigned int *kl) {
+    const unsigned int lane = lane_id();
+    if (!units_ok) {
+        for (unsigned long long i = b + lane; i < e; i += 64u)
+            accumulate(a, idx[i], value_traits<VT>::decode(val[i]), cols, masks, fgm, group_count, wg);
+        return;
+    }
+    const unsigned long long ua = b + ((4u - (unsigned int) ((b + mis) & 3u)) & 3u);
+    const unsigned long long a0 = ua < e ? ua : e;
+    const unsigned long long units = (e - a0) >> 2;
+    const unsigned long long ea = a0 + 4u * units;
+    if (lane < 6u) {
+        const unsigned long long i = lane < 3u ? b + lane : ea + lane - 3u;
+        if (lane < 3u ? i < a0 : i < e) accumulate(a, idx[i], value_traits<VT>::decode(val[i]), cols, masks, fgm, group_count, wg);
+    }
+    using raw = typename mg_unit<VT>::raw;
+    const uint32_t gbits = fgm != nullptr && group_count != 0u ? group_bits(group_count) : 0u;
+    const uint4 *U = (const uint4 *) (idx + a0);
+    const raw *V = (const raw *) (val + a0);
+    for (unsigned long long j0 = lane; j0 < units; j0 += 64u * MG_UNROLL) {
+        uint4 q[MG_UNROLL];
+        raw r[MG_UNROLL];
+#pragma unroll
+        for (unsigned int u = 0; u < MG_UNROLL; ++u) {
+            const unsigned long long j = j0 + 64u * u < units ? j0 + 64u * u : units - 1u;
+            q[u] = U[j];
+            r[u] = V[j];
+        }
+        // every keep flag of the pass read before any is used
+        bool kp[MG_UNROLL][4];
+#pragma unroll
+        for (unsigned int u = 0; u < MG_UNROLL; ++u) {
+            const unsigned int c[4] = {q[u].x, q[u].y, q[u].z, q[u].w};
+#pragma unroll
+            for (unsigned int k = 0; k < 4u; ++k) kp[u][k] = c[k] < cols && keep_of(kl, masks, c[k] < cols ? c[k] : 0u);
+        }
+#pragma unroll
+        for (unsigned int u = 0; u < MG_UNROLL; ++u) {
+            if (j0 + 64u * u >= units) continue;
+            const unsigned int c[4] = {q[u].x, q[u].y, q[u].z, q[u].w};
+#pragma unroll
+            for (unsigned int k = 0; k < 4u; ++k) accumulate_kept(a, kp[u][k], c[k], mg_unit<VT>::get(r[u], k), fgm, gbits);
+        }
+    }
+}
+
 template<int VT>
 __global__ void __launch_bounds__(MG_THREADS) group_reduce_compressed_kernel(csamd_compressed_view src, const uint32_t *__restrict__ fgm,
                                                                              csamd_row_feature_mask_view masks, unsigned int group_count,
-                                                                             csamd_sparse_group_filter_params filter, reduce_outputs o) {
+                                                                             csamd_sparse_group_filter_params filter, reduce_outputs o, const unsigned int *__restrict__ keep_bits,
+    unsigned int keep_words) {
     using S = typename value_traits<VT>::storage;
     __shared__ float groups[MG_WAVES][CSAMD_MAX_MASK_GROUPS];
+    extern __shared__ unsigned int keep_lds[];
+    const unsigned int *kl = nullptr;
+    if (keep_bits != nullptr) {  // the packed feature keep flags, staged once per workgroup
+        for (unsigned int w = threadIdx.x; w < keep_words; w += MG_THREADS) keep_lds[w] = keep_bits[w];
+        __syncthreads();
+        kl = keep_lds;
+    }
     const unsigned int wave = threadIdx.x >> 6;
     const unsigned int lane = lane_id();
     float *wg = groups[wave];
     const S *__restrict__ val = (const S *) src.val;
+    unsigned int mis = 0u;
+    const bool units_ok = mg_units_ok<VT>(src.minorIdx, src.val, &mis);
     for (unsigned int row = blockIdx.x * MG_WAVES + wave; row < src.rows; row += gridDim.x * MG_WAVES) {
         if (lane < CSAMD_MAX_MASK_GROUPS) wg[lane] = 0.0f;
         lds_fence();
@@ -122,10 +272,8 @@ __global__ void __launch_bounds__(MG_THREADS) group_reduce_compressed_kernel(csa
         }
         const unsigned int begin = src.majorPtr[row];
         const unsigned int end = src.majorPtr[row + 1u];
-        row_acc a{0.0f, 0.0f, 0u};
-        for (unsigned int i = begin + lane; i < end; i += 64u) {
-            accumulate(a, src.minorIdx[i], value_traits<VT>::decode(val[i]), src.cols, masks, fgm, group_count, wg);
-        }
+        row_acc a = row_acc_zero();
+        mg_stream<VT>(a, src.minorIdx, val, begin, end, units_ok, mis, src.cols, masks, fgm, group_count, wg, kl);
         finish_row(row, a, group_count, filter, wg, o);
     }
 }
@@ -154,13 +302,23 @@ __device__ __forceinline__ unsigned int find_slice_of_row(const csamd_sliced_ell
 template<int VT>
 __global__ void __launch_bounds__(MG_THREADS) group_reduce_sliced_kernel(csamd_sliced_ell_view src, const uint32_t *__restrict__ fgm,
                                                                          csamd_row_feature_mask_view masks, unsigned int group_count,
-                                                                         csamd_sparse_group_filter_params filter, reduce_outputs o) {
+                                                                         csamd_sparse_group_filter_params filter, reduce_outputs o, const unsigned int *__restrict__ keep_bits,
+    unsigned int keep_words) {
     using S = typename value_traits<VT>::storage;
     __shared__ float groups[MG_WAVES][CSAMD_MAX_MASK_GROUPS];
+    extern __shared__ unsigned int keep_lds[];
+    const unsigned int *kl = nullptr;
+    if (keep_bits != nullptr) {  // the packed feature keep flags, staged once per workgroup
+        for (unsigned int w = threadIdx.x; w < keep_words; w += MG_THREADS) keep_lds[w] = keep_bits[w];
+        __syncthreads();
+        kl = keep_lds;
+    }
     const unsigned int wave = threadIdx.x >> 6;
     const unsigned int lane = lane_id();
     float *wg = groups[wave];
     const S *__restrict__ val = (const S *) src.val;
+    unsigned int mis = 0u;
+    const bool units_ok = mg_units_ok<VT>(src.col_idx, src.val, &mis);
     for (unsigned int row = blockIdx.x * MG_WAVES + wave; row < src.rows; row += gridDim.x * MG_WAVES) {
         if (lane < CSAMD_MAX_MASK_GROUPS) wg[lane] = 0.0f;
         lds_fence();
@@ -176,11 +334,8 @@ __global__ void __launch_bounds__(MG_THREADS) group_reduce_sliced_kernel(csamd_s
             width = src.slice_widths[slice];
             slot_base = (unsigned long long) src.slice_slot_offsets[slice] + (unsigned long long) (row - row_begin) * width;
         }
-        row_acc a{0.0f, 0.0f, 0u};
-        for (unsigned int slot = lane; slot < width; slot += 64u) {
-            accumulate(a, src.col_idx[slot_base + slot], value_traits<VT>::decode(val[slot_base + slot]), src.cols, masks,
-                       fgm, group_count, wg);
-        }
+        row_acc a = row_acc_zero();
+        mg_stream<VT>(a, src.col_idx, val, slot_base, slot_base + width, units_ok, mis, src.cols, masks, fgm, group_count, wg, kl);
         finish_row(row, a, group_count, filter, wg, o);
     }
 }
@@ -188,15 +343,26 @@ __global__ void __launch_bounds__(MG_THREADS) group_reduce_sliced_kernel(csamd_s
 template<int VT>
 __global__ void __launch_bounds__(MG_THREADS) group_reduce_blocked_kernel(csamd_blocked_ell_view src, const uint32_t *__restrict__ fgm,
                                                                           csamd_row_feature_mask_view masks, unsigned int group_count,
-                                                                          csamd_sparse_group_filter_params filter, reduce_outputs o) {
+                                                                          csamd_sparse_group_filter_params filter, reduce_outputs o, const unsigned int *__restrict__ keep_bits,
+    unsigned int keep_words) {
     using S = typename value_traits<VT>::storage;
     __shared__ float groups[MG_WAVES][CSAMD_MAX_MASK_GROUPS];
+    extern __shared__ unsigned int keep_lds[];
+    const unsigned int *kl = nullptr;
+    if (keep_bits != nullptr) {  // the packed feature keep flags, staged once per workgroup
+        for (unsigned int w = threadIdx.x; w < keep_words; w += MG_THREADS) keep_lds[w] = keep_bits[w];
+        __syncthreads();
+        kl = keep_lds;
+    }
     const unsigned int wave = threadIdx.x >> 6;
     const unsigned int lane = lane_id();
     float *wg = groups[wave];
     const S *__restrict__ val = (const S *) src.val;
     const unsigned int bs = src.block_size;
     const unsigned int width = bs != 0u ? src.ell_cols / bs : 0u;
+    constexpr unsigned int VB = sizeof(S);
+    const bool units_ok = bs != 0u && bs % 4u == 0u && src.ell_cols % 4u == 0u && ((uintptr_t) src.val % (4u * VB)) == 0u;
+    const uint32_t gbits = fgm != nullptr && group_count != 0u ? group_bits(group_count) : 0u;
     for (unsigned int row = blockIdx.x * MG_WAVES + wave; row < src.rows; row += gridDim.x * MG_WAVES) {
         if (lane < CSAMD_MAX_MASK_GROUPS) wg[lane] = 0.0f;
         lds_fence();
@@ -206,7 +372,42 @@ __global__ void __launch_bounds__(MG_THREADS) group_reduce_blocked_kernel(csamd_
         }
         const unsigned int rb = bs != 0u ? row / bs : 0u;
         const unsigned long long row_base = (unsigned long long) row * src.ell_cols;
-        row_acc a{0.0f, 0.0f, 0u};
+        row_acc a = row_acc_zero();
+        if (units_ok) {
+            // 4 values of one block per unit (bs % 4 == 0), MG_UNROLL units in flight per lane
+            using raw = typename mg_unit<VT>::raw;
+            const raw *V = (const raw *) (val + row_base);
+            const unsigned int units = src.ell_cols >> 2;
+            const unsigned int *bcr = src.blockColIdx + (unsigned long long) rb * width;
+            for (unsigned int j0 = lane; j0 < units; j0 += 64u * MG_UNROLL) {
+                raw r[MG_UNROLL];
+                unsigned int bc[MG_UNROLL];
+#pragma unroll
+                for (unsigned int u = 0; u < MG_UNROLL; ++u) {
+                    const unsigned int j = j0 + 64u * u < units ? j0 + 64u * u : units - 1u;
+                    r[u] = V[j];
+                    bc[u] = bcr[(4u * j) / bs];
+                }
+                bool kp[MG_UNROLL][4];
+                unsigned int c0[MG_UNROLL];
+#pragma unroll
+                for (unsigned int u = 0; u < MG_UNROLL; ++u) {
+                    const unsigned int j = j0 + 64u * u < units ? j0 + 64u * u : units - 1u;
+                    c0[u] = bc[u] != CSAMD_INVALID_COL ? bc[u] * bs + (4u * j) % bs : src.cols;
+#pragma unroll
+                    for (unsigned int k = 0; k < 4u; ++k)
+                        kp[u][k] = c0[u] + k < src.cols && keep_of(kl, masks, c0[u] + k < src.cols ? c0[u] + k : 0u);
+                }
+#pragma unroll
+                for (unsigned int u = 0; u < MG_UNROLL; ++u) {
+                    if (j0 + 64u * u >= units) continue;
+#pragma unroll
+                    for (unsigned int k = 0; k < 4u; ++k) accumulate_kept(a, kp[u][k], c0[u] + k, mg_unit<VT>::get(r[u], k), fgm, gbits);
+                }
+            }
+            finish_row(row, a, group_count, filter, wg, o);
+            continue;
+        }
         for (unsigned int e = lane; e < src.ell_cols; e += 64u) {
             const unsigned int slot = bs != 0u ? e / bs : 0u;
             const unsigned int lib = bs != 0u ? e % bs : 0u;
@@ -256,7 +457,22 @@ struct reduce_prelude {
     unsigned int group_count;
     csamd_row_feature_mask_view masks;
     const uint32_t *fgm;
+    const unsigned int *keep_bits;  // packed feature_keep (null: none, or too many columns for LDS)
+    unsigned int keep_words;
 };
+
+constexpr unsigned int MG_KEEP_WORDS_MAX = 8192u;  // 32 KB of LDS: up to 262,144 columns
+
+// feature_keep bytes -> bits (bit c of word c / 32 set when column c is kept).
+__global__ void mg_pack_keep_kernel(const unsigned char *__restrict__ keep, unsigned int cols, unsigned int *__restrict__ bits,
+                                    unsigned int words) {
+    for (unsigned int w = blockIdx.x * blockDim.x + threadIdx.x; w < words; w += gridDim.x * blockDim.x) {
+        unsigned int b = 0u;
+        const unsigned int c0 = w * 32u;
+        for (unsigned int k = 0; k < 32u && c0 + k < cols; ++k) b |= (keep[c0 + k] != 0u ? 1u : 0u) << k;
+        bits[w] = b;
+    }
+}
 
 int prepare_reduce(const void *src, csamd_sparse_group_reduce_workspace *ws, const csamd_group_mask_config_view *groups,
                    const csamd_row_feature_mask_view *masks, const csamd_sparse_group_filter_params *filter,
@@ -271,6 +487,27 @@ int prepare_reduce(const void *src, csamd_sparse_group_reduce_workspace *ws, con
     if (masks != nullptr) p->masks = *masks;
     else std::memset(&p->masks, 0, sizeof(p->masks));
     p->fgm = groups != nullptr && groups->feature_group_masks != nullptr ? groups->feature_group_masks : ws->feature_group_masks;
+    p->keep_bits = nullptr;
+    p->keep_words = 0u;
+    const unsigned int words = (cols + 31u) / 32u;
+    if (p->masks.feature_keep != nullptr && cols != 0u && words <= MG_KEEP_WORDS_MAX && rows != 0u) {
+        const size_t need = (size_t) words * 4u;
+        if (need > ws->scratch_bytes) {
+            // an earlier call's kernels may still read the old block
+            CSAMD_HIP(hipStreamSynchronize((hipStream_t) ws->stream), "sparse group keep bits sync");
+            if (ws->scratch_block != nullptr) (void) hipFree(ws->scratch_block);
+            ws->scratch_block = nullptr;
+            ws->scratch_bytes = 0u;
+            CSAMD_HIP(hipMalloc(&ws->scratch_block, align_up(need, 4096u)), "hipMalloc sparse group keep bits");
+            ws->scratch_bytes = align_up(need, 4096u);
+        }
+        hipStream_t s = (hipStream_t) ws->stream;
+        mg_pack_keep_kernel<<<grid_for(words, 256u, 1024u), 256, 0, s>>>(p->masks.feature_keep, cols, (unsigned int *) ws->scratch_block,
+                                                                        words);
+        CSAMD_HIP(hipGetLastError(), "mg_pack_keep_kernel");
+        p->keep_bits = (const unsigned int *) ws->scratch_block;
+        p->keep_words = words;
+    }
     return 1;
 }
 
@@ -290,19 +527,19 @@ int dispatch_value_type(unsigned int vt, const View &v, unsigned int grid, hipSt
 template<int VT> struct launch_compressed {
     static void run(const csamd_compressed_view &v, unsigned int grid, hipStream_t s, const reduce_prelude &p,
                     const csamd_sparse_group_filter_params &f, const reduce_outputs &o) {
-        group_reduce_compressed_kernel<VT><<<grid, MG_THREADS, 0, s>>>(v, p.fgm, p.masks, p.group_count, f, o);
+        group_reduce_compressed_kernel<VT><<<grid, MG_THREADS, p.keep_words * 4u, s>>>(v, p.fgm, p.masks, p.group_count, f, o, p.keep_bits, p.keep_words);
     }
 };
 template<int VT> struct launch_sliced {
     static void run(const csamd_sliced_ell_view &v, unsigned int grid, hipStream_t s, const reduce_prelude &p,
                     const csamd_sparse_group_filter_params &f, const reduce_outputs &o) {
-        group_reduce_sliced_kernel<VT><<<grid, MG_THREADS, 0, s>>>(v, p.fgm, p.masks, p.group_count, f, o);
+        group_reduce_sliced_kernel<VT><<<grid, MG_THREADS, p.keep_words * 4u, s>>>(v, p.fgm, p.masks, p.group_count, f, o, p.keep_bits, p.keep_words);
     }
 };
 template<int VT> struct launch_blocked {
     static void run(const csamd_blocked_ell_view &v, unsigned int grid, hipStream_t s, const reduce_prelude &p,
                     const csamd_sparse_group_filter_params &f, const reduce_outputs &o) {
-        group_reduce_blocked_kernel<VT><<<grid, MG_THREADS, 0, s>>>(v, p.fgm, p.masks, p.group_count, f, o);
+        group_reduce_blocked_kernel<VT><<<grid, MG_THREADS, p.keep_words * 4u, s>>>(v, p.fgm, p.masks, p.group_count, f, o, p.keep_bits, p.keep_words);
     }
 };
 

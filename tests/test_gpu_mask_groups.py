@@ -196,3 +196,44 @@ def test_fleet_single_gpu(gpu):
     ref = O.group_reduce_bell(rows, cols, 4, ell, bci, bv, F16, None, None, None, 0, [0, 0, 1, 0])
     np.testing.assert_allclose(got, ref["row_totals"], rtol=1e-6)
     csr_rt.clear(fleet)
+
+
+@pytest.mark.parametrize("shift_idx,shift_val", [(0, 0), (1, 1), (3, 3), (1, 2)])
+@pytest.mark.parametrize("vt", [F16, F32, FP8])
+def test_compressed_views_off_unit_boundaries(ws, vt, shift_idx, shift_val):
+    """CSR views into the middle of allocations (indices and values a few
+    entries past a 16-byte boundary, equal or different shifts) and long
+    rows: the unit-load path, its head / tail entries and the per-entry
+    fallback all give the oracle's sums."""
+    rows, cols, G = 64, 3000, 32
+    ptr_, idx, val = random_csr(rows, cols, 0.45, seed=5 + vt, zeros=0.03, empty_rows=0.05, fp8_safe=(vt == FP8))
+    bits = encode(val, vt)
+    rng = np.random.default_rng(vt)
+    rk = (rng.random(rows) < 0.9).astype(np.uint8)
+    fk = (rng.random(cols) < 0.3).astype(np.uint8)
+    fgm = rng.integers(0, 2 ** 32, size=cols, dtype=np.uint64).astype(np.uint32)
+    filt_t = [10.0, 3, 0.4, 2]
+    src = DeviceCSR.from_numpy(rows, cols, ptr_, idx, bits, vt)
+    big_i = to_dev_u32(np.concatenate([np.zeros(shift_idx, np.uint32), idx]))
+    big_v = torch.cat([torch.zeros(shift_val, dtype=src.val.dtype, device=src.val.device), src.val])
+    src.minor_idx = big_i[shift_idx:]
+    src.val = big_v[shift_val:]
+    d_rk, d_fk, d_fgm = to_dev_u8(rk), to_dev_u8(fk), to_dev_u32(fgm)
+    groups = A.group_mask_config_view(group_count=G, feature_group_masks=ptr(d_fgm))
+    masks = A.row_feature_mask_view(row_keep=ptr(d_rk), feature_keep=ptr(d_fk))
+    filt = A.sparse_group_filter_params(*filt_t)
+    res = A.sparse_group_reduce_result()
+    assert csr_rt.compute_sparse_group_reduce_compressed_fallback(src.view(), ws, groups, masks, filt, res, vt) == 1
+    ref = O.group_reduce_csr(rows, cols, ptr_, idx, bits, vt, rk, fk, fgm, G, filt_t)
+    _compare(fetch(ws, res), ref, G, vt)
+    s = O.sliced_ell_from_compressed(rows, cols, 0, ptr_, idx, bits, 32)
+    sell = DeviceSlicedELL.from_numpy(rows, cols, len(idx), s["slice_rows"], s["offsets"], s["widths"], s["slot_offsets"], s["col_idx"],
+                                      s["val"], vt)
+    assert csr_rt.compute_sparse_group_reduce(sell.view(), ws, groups, masks, filt, res, vt) == 1
+    _compare(fetch(ws, res), O.group_reduce_sell(rows, cols, s["slice_rows"], s["offsets"], s["widths"], s["slot_offsets"],
+                                                 s["col_idx"], s["val"], vt, rk, fk, fgm, G, filt_t), G, vt)
+    for bs in (4, 6, 16):
+        bci, bv, ell, nnz = O.blocked_ell_from_compressed(rows, cols, ptr_, idx, bits, bs)
+        bell = DeviceBlockedELL.from_numpy(rows, cols, nnz, bs, ell, bci, bv, vt)
+        assert csr_rt.compute_sparse_group_reduce(bell.view(), ws, groups, masks, filt, res, vt) == 1
+        _compare(fetch(ws, res), O.group_reduce_bell(rows, cols, bs, ell, bci, bv, vt, rk, fk, fgm, G, filt_t), G, vt)
