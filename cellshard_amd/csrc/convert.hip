@@ -311,6 +311,39 @@ __global__ void csc_row_nnz_kernel(const unsigned int *__restrict__ minor, unsig
     }
 }
 
+// 16-byte chunks of a Blocked-ELL value row, when every chunk lies in one
+// block (bs and ell_cols multiples of the chunk, an aligned value array):
+// bit e of the mask is set when value e of chunk c is nonzero at a valid
+// column < cols; *col0 is the chunk's first column.
+template<typename S> union bell_chunk_vals {
+    uint4 q;
+    S v[16u / sizeof(S)];
+};
+
+template<typename S>
+__device__ __forceinline__ bool bell_chunks_ok(const csamd_blocked_ell_view &src) {
+    constexpr unsigned int PER = 16u / sizeof(S);
+    return src.block_size != 0u && src.block_size % PER == 0u && src.ell_cols % PER == 0u && ((uintptr_t) src.val & 15u) == 0u;
+}
+
+template<typename S>
+__device__ __forceinline__ unsigned int bell_chunk_mask(const csamd_blocked_ell_view &src, unsigned int width, unsigned int row,
+                                                        unsigned int c, bell_chunk_vals<S> *g, unsigned int *col0) {
+    constexpr unsigned int PER = 16u / sizeof(S);
+    const unsigned int bs = src.block_size;
+    const unsigned int t0 = c * PER, slot = t0 / bs;
+    const unsigned int bc = src.blockColIdx[(unsigned long long) (row / bs) * width + slot];
+    g->q = ((const uint4 *) ((const S *) src.val + (unsigned long long) row * src.ell_cols))[c];
+    *col0 = bc * bs + (t0 - slot * bs);
+    unsigned int m = 0u;
+    if (bc != CSAMD_INVALID_COL) {
+#pragma unroll
+        for (unsigned int e = 0; e < PER; ++e)
+            if (*col0 + e < src.cols && stored_nonzero<S>(g->v[e])) m |= 1u << e;
+    }
+    return m;
+}
+
 // Blocked-ELL row nnz: nonzero values at valid columns (sliced_ell_from_blocked_ell.cuh:36-62).
 template<typename S>
 __global__ void bell_row_nnz_kernel(csamd_blocked_ell_view src, unsigned int *__restrict__ row_nnz) {
@@ -319,9 +352,21 @@ __global__ void bell_row_nnz_kernel(csamd_blocked_ell_view src, unsigned int *__
     const unsigned int bs = src.block_size;
     const unsigned int width = bs != 0u ? src.ell_cols / bs : 0u;
     const S *val = (const S *) src.val;
+    const bool chunks = bell_chunks_ok<S>(src);
     for (unsigned int row = blockIdx.x * nw + wave; row < src.rows; row += gridDim.x * nw) {
         const unsigned int rb = row / bs;
         unsigned int count = 0u;
+        if (chunks) {
+            const unsigned int nc = src.ell_cols / (16u / (unsigned int) sizeof(S));
+            for (unsigned int c = lane_id(); c < nc; c += 64u) {
+                bell_chunk_vals<S> g;
+                unsigned int c0;
+                count += (unsigned int) __popc(bell_chunk_mask<S>(src, width, row, c, &g, &c0));
+            }
+            count = wave_sum_u32(count);
+            if (lane_id() == 0u) row_nnz[row] = count;
+            continue;
+        }
         for (unsigned int e = lane_id(); e < width * bs; e += 64u) {
             const unsigned int slot = e / bs;
             const unsigned int bc = src.blockColIdx[(unsigned long long) rb * width + slot];
@@ -416,6 +461,7 @@ __global__ void sell_emit_bell_kernel(csamd_blocked_ell_view src, csamd_sliced_e
     const unsigned int bs = src.block_size;
     const unsigned int width_blocks = bs != 0u ? src.ell_cols / bs : 0u;
     const S *sval = (const S *) src.val;
+    const bool chunks = bell_chunks_ok<S>(src);
     for (unsigned int row = blockIdx.x * nw + wave; row < src.rows; row += gridDim.x * nw) {
         const unsigned int slice = sell_slice_of(dst, row);
         if (slice >= dst.slice_count) continue;
@@ -425,6 +471,31 @@ __global__ void sell_emit_bell_kernel(csamd_blocked_ell_view src, csamd_sliced_e
         const unsigned int rb = row / bs;
         const unsigned int total = width_blocks * bs;
         unsigned int written = 0u;
+        if (chunks) {
+            constexpr unsigned int PER = 16u / sizeof(S);
+            const unsigned int nc = src.ell_cols / PER;
+            for (unsigned int c0i = 0u; c0i < nc; c0i += 64u) {
+                const unsigned int c = c0i + lane_id();
+                bell_chunk_vals<S> g;
+                g.q = make_uint4(0u, 0u, 0u, 0u);
+                unsigned int col0 = 0u;
+                const unsigned int m = c < nc ? bell_chunk_mask<S>(src, width_blocks, row, c, &g, &col0) : 0u;
+                const unsigned int cnt = (unsigned int) __popc(m);
+                const unsigned int incl = wave_inclusive_scan_u32(cnt);
+                unsigned int pos = written + incl - cnt;
+#pragma unroll
+                for (unsigned int e = 0; e < PER; ++e) {
+                    if ((m >> e) & 1u) {
+                        if (pos < width) {
+                            dst.col_idx[base + pos] = col0 + e;
+                            dval[base + pos] = g.v[e];
+                        }
+                        ++pos;
+                    }
+                }
+                written += __shfl(incl, 63);
+            }
+        } else {
         for (unsigned int e0 = 0u; e0 < total; e0 += 64u) {
             const unsigned int e = e0 + lane_id();
             bool keep = false;
@@ -446,6 +517,7 @@ __global__ void sell_emit_bell_kernel(csamd_blocked_ell_view src, csamd_sliced_e
                 dval[base + pos] = v;
             }
             written += (unsigned int) __popcll(bal);
+        }
         }
         for (unsigned int k = written + lane_id(); k < width; k += 64u) {
             dst.col_idx[base + k] = CSAMD_INVALID_COL;

@@ -269,3 +269,22 @@ def test_synth_generator_properties(ws):
     # Zipf popularity: the most popular column appears in far more rows than the median one
     counts = np.bincount(idx, minlength=4000)
     assert counts.max() > 20 * max(np.median(counts), 1)
+
+
+@pytest.mark.parametrize("bs", [4, 8, 16])
+@pytest.mark.parametrize("vt", [F16, F32])
+def test_sliced_from_blocked_chunk_paths(ws, bs, vt):
+    """16-byte chunk reads (block size a multiple of the chunk) and the
+    per-value path give the oracle's Sliced-ELL, with explicit zeros and a
+    column count that is not a multiple of the block size."""
+    rows, cols = 300, 403
+    ptr, idx, val = random_csr(rows, cols, 0.3, seed=bs + vt, zeros=0.1, empty_rows=0.05)
+    bits = encode(val, vt)
+    src = DeviceCSR.from_numpy(rows, cols, ptr, idx, bits, vt)
+    bell = blocked_ell_from_compressed(src, bs, ws)
+    out = sliced_ell_from_blocked_ell(bell, 32, ws)
+    gb, gv = bell.to_numpy()
+    ref = O.sliced_ell_from_blocked_ell(rows, cols, bs, bell.ell_cols, gb, gv, 32)
+    offs, widths, slots, col, v = out.to_numpy()
+    assert np.array_equal(widths, ref["widths"]) and np.array_equal(slots, ref["slot_offsets"])
+    assert np.array_equal(col, ref["col_idx"]) and np.array_equal(v.view(np.uint8), np.asarray(ref["val"]).view(np.uint8))
