@@ -574,18 +574,69 @@ __device__ __forceinline__ bool reopt_keep(const csamd_blocked_ell_view &src, co
     return false;
 }
 
+// Output column of every source column (feature kept, remapped, below
+// output_cols) or ~0u, so that a chunk of entries needs one gather each.
+__global__ void reopt_col_map_kernel(csamd_row_feature_mask_view masks, unsigned int cols, unsigned int output_cols,
+                                     unsigned int *__restrict__ col_map) {
+    for (unsigned int c = blockIdx.x * blockDim.x + threadIdx.x; c < cols; c += gridDim.x * blockDim.x) {
+        unsigned int o = ~0u;
+        if (feature_participates(masks, c)) {
+            o = masks.feature_remap != nullptr ? masks.feature_remap[c] : c;
+            if (o >= output_cols) o = ~0u;
+        }
+        col_map[c] = o;
+    }
+}
+
+// 16-byte chunks of 8 f16 values, when every chunk lies in one block: bit e
+// of the mask is set when value e is nonzero at a column that maps to an
+// output column (oc[e]).
+__device__ __forceinline__ bool reopt_chunks_ok(const csamd_blocked_ell_view &src) {
+    return src.block_size != 0u && src.block_size % 8u == 0u && src.ell_cols % 8u == 0u && ((uintptr_t) src.val & 15u) == 0u;
+}
+
+__device__ __forceinline__ unsigned int reopt_chunk(const csamd_blocked_ell_view &src, const unsigned int *__restrict__ col_map,
+                                                    unsigned int width, unsigned int row, unsigned int c, uint4 *q,
+                                                    unsigned int (&oc)[8]) {
+    const unsigned int bs = src.block_size;
+    const unsigned int t0 = c * 8u, slot = t0 / bs;
+    const unsigned int bc = src.blockColIdx[(unsigned long long) (row / bs) * width + slot];
+    *q = ((const uint4 *) ((const uint16_t *) src.val + (unsigned long long) row * src.ell_cols))[c];
+    const unsigned int col0 = bc != CSAMD_INVALID_COL ? bc * bs + (t0 - slot * bs) : src.cols;
+#pragma unroll
+    for (unsigned int e = 0; e < 8u; ++e) oc[e] = col0 + e < src.cols ? col_map[col0 + e] : ~0u;  // gathers issued together
+    const unsigned int w[4] = {q->x, q->y, q->z, q->w};
+    unsigned int m = 0u;
+#pragma unroll
+    for (unsigned int e = 0; e < 8u; ++e)
+        if (oc[e] != ~0u && ((w[e >> 1] >> (16u * (e & 1u))) & 0x7fffu) != 0u) m |= 1u << e;
+    return m;
+}
+
 __global__ void __launch_bounds__(MG_THREADS) reopt_count_kernel(csamd_blocked_ell_view src, csamd_row_feature_mask_view masks,
                                                                  unsigned int output_rows, unsigned int output_cols,
-                                                                 unsigned int *major_ptr) {
+                                                                 unsigned int *major_ptr, const unsigned int *__restrict__ col_map) {
     const unsigned int wave = threadIdx.x >> 6;
     const unsigned int bs = src.block_size;
     const unsigned int width = bs != 0u ? src.ell_cols / bs : 0u;
+    const bool chunks = reopt_chunks_ok(src);
     for (unsigned int row = blockIdx.x * MG_WAVES + wave; row < src.rows; row += gridDim.x * MG_WAVES) {
         if (!row_participates(masks, row)) continue;
         const unsigned int out_row = masks.row_remap != nullptr ? masks.row_remap[row] : row;
         if (out_row >= output_rows) continue;
         const unsigned int rb = bs != 0u ? row / bs : 0u;
         unsigned int count = 0u;
+        if (chunks) {
+            const unsigned int nc = src.ell_cols / 8u;
+            for (unsigned int c = lane_id(); c < nc; c += 64u) {
+                uint4 q;
+                unsigned int oc[8];
+                count += (unsigned int) __popc(reopt_chunk(src, col_map, width, row, c, &q, oc));
+            }
+            count = wave_sum_u32(count);
+            if (lane_id() == 0u) major_ptr[out_row + 1u] = count;
+            continue;
+        }
         for (unsigned int e = lane_id(); e < src.ell_cols; e += 64u) {
             unsigned int oc;
             uint16_t bits;
@@ -622,15 +673,45 @@ __global__ void __launch_bounds__(MG_THREADS) reopt_scatter_kernel(csamd_blocked
                                                                    unsigned int output_rows, unsigned int output_cols,
                                                                    const unsigned int *__restrict__ major_ptr,
                                                                    unsigned int *__restrict__ row_write,
-                                                                   unsigned int *__restrict__ minor_idx, uint16_t *__restrict__ val) {
+                                                                   unsigned int *__restrict__ minor_idx, uint16_t *__restrict__ val,
+                                                                   const unsigned int *__restrict__ col_map) {
     const unsigned int wave = threadIdx.x >> 6;
     const unsigned int bs = src.block_size;
     const unsigned int width = bs != 0u ? src.ell_cols / bs : 0u;
+    const bool chunks = reopt_chunks_ok(src);
     for (unsigned int row = blockIdx.x * MG_WAVES + wave; row < src.rows; row += gridDim.x * MG_WAVES) {
         if (!row_participates(masks, row)) continue;
         const unsigned int out_row = masks.row_remap != nullptr ? masks.row_remap[row] : row;
         if (out_row >= output_rows) continue;
         const unsigned int rb = bs != 0u ? row / bs : 0u;
+        if (chunks) {
+            // per 64 chunks: positions by a wave scan, one cursor claim per group
+            const unsigned int nc = src.ell_cols / 8u;
+            for (unsigned int c0 = 0u; c0 < nc; c0 += 64u) {
+                const unsigned int c = c0 + lane_id();
+                uint4 q = make_uint4(0u, 0u, 0u, 0u);
+                unsigned int oc[8];
+                const unsigned int m = c < nc ? reopt_chunk(src, col_map, width, row, c, &q, oc) : 0u;
+                const unsigned int cnt = (unsigned int) __popc(m);
+                const unsigned int incl = wave_inclusive_scan_u32(cnt);
+                const unsigned int total = __shfl(incl, 63);
+                if (total == 0u) continue;
+                unsigned int base = 0u;
+                if (lane_id() == 0u) base = atomicAdd(row_write + out_row, total);
+                base = __shfl(base, 0, 64);
+                unsigned int w = major_ptr[out_row] + base + incl - cnt;
+                const unsigned int qw[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+                for (unsigned int e = 0; e < 8u; ++e) {
+                    if ((m >> e) & 1u) {
+                        minor_idx[w] = oc[e];
+                        val[w] = (uint16_t) (qw[e >> 1] >> (16u * (e & 1u)));
+                        ++w;
+                    }
+                }
+            }
+            continue;
+        }
         const unsigned int ell_round = (src.ell_cols + 63u) & ~63u;
         for (unsigned int e0 = 0u; e0 < ell_round; e0 += 64u) {
             const unsigned int e = e0 + lane_id();
@@ -1029,7 +1110,8 @@ int csamd_manual_reoptimize_masked_sparse(const csamd_blocked_ell_view *src, con
         CSAMD_HIP(hipMalloc(&st->compressed_block, bytes), "hipMalloc masked compressed output");
         st->compressed_bytes = bytes;
     }
-    const size_t scratch_needed = (size_t) (output_rows == 0u ? 1u : output_rows) * sizeof(unsigned int);
+    const size_t map_off = align_up((size_t) (output_rows == 0u ? 1u : output_rows) * sizeof(unsigned int), 256u);
+    const size_t scratch_needed = map_off + (size_t) (src->cols == 0u ? 1u : src->cols) * sizeof(unsigned int);
     if (scratch_needed > st->scratch_bytes) {
         if (st->scratch_block != nullptr) (void) hipFree(st->scratch_block);
         st->scratch_block = nullptr;
@@ -1047,19 +1129,24 @@ int csamd_manual_reoptimize_masked_sparse(const csamd_blocked_ell_view *src, con
     c.minorIdx = (unsigned int *) (base + minor_off);
     c.val = (csamd_half *) (base + val_off);
     unsigned int *row_write = (unsigned int *) st->scratch_block;
+    unsigned int *col_map = (unsigned int *) ((char *) st->scratch_block + map_off);
 
     if (!fill_u32(c.majorPtr, (unsigned long long) output_rows + 1ull, 0u, s)) return 0;
     if (!fill_u32(row_write, output_rows, 0u, s)) return 0;
     const unsigned int grid = reduce_grid(src->rows);
+    if (src->cols != 0u) {
+        reopt_col_map_kernel<<<grid_for(src->cols, 256u, 1024u), 256, 0, s>>>(m, src->cols, output_cols, col_map);
+        CSAMD_HIP(hipGetLastError(), "reopt_col_map_kernel");
+    }
     if (src->rows != 0u) {
-        reopt_count_kernel<<<grid, MG_THREADS, 0, s>>>(*src, m, output_rows, output_cols, c.majorPtr);
+        reopt_count_kernel<<<grid, MG_THREADS, 0, s>>>(*src, m, output_rows, output_cols, c.majorPtr, col_map);
         CSAMD_HIP(hipGetLastError(), "reopt_count_kernel");
     }
     reopt_scan_kernel<<<1, 1024, 0, s>>>(output_rows, c.majorPtr);
     CSAMD_HIP(hipGetLastError(), "reopt_scan_kernel");
     if (src->rows != 0u) {
         reopt_scatter_kernel<<<grid, MG_THREADS, 0, s>>>(*src, m, output_rows, output_cols, c.majorPtr, row_write, c.minorIdx,
-                                                         (uint16_t *) c.val);
+                                                         (uint16_t *) c.val, col_map);
         CSAMD_HIP(hipGetLastError(), "reopt_scatter_kernel");
     }
     unsigned int host_nnz = 0u;

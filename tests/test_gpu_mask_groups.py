@@ -237,3 +237,37 @@ def test_compressed_views_off_unit_boundaries(ws, vt, shift_idx, shift_val):
         bell = DeviceBlockedELL.from_numpy(rows, cols, nnz, bs, ell, bci, bv, vt)
         assert csr_rt.compute_sparse_group_reduce(bell.view(), ws, groups, masks, filt, res, vt) == 1
         _compare(fetch(ws, res), O.group_reduce_bell(rows, cols, bs, ell, bci, bv, vt, rk, fk, fgm, G, filt_t), G, vt)
+
+
+@pytest.mark.parametrize("bs", [4, 8, 16])
+def test_reoptimize_random_parity(gpu, bs):
+    """manual_reoptimize_masked_sparse on random Blocked-ELL (16-byte chunk
+    reads when the block size is a multiple of 8, per value otherwise) with
+    row / feature masks and compacting remaps, against the oracle."""
+    rows, cols = 250, 403
+    ptr_, idx, val = random_csr(rows, cols, 0.2, seed=bs, zeros=0.1, empty_rows=0.05)
+    bits = f16_bits(val)
+    bci, bv, ell, nnz = O.blocked_ell_from_compressed(rows, cols, ptr_, idx, bits, bs)
+    bell = DeviceBlockedELL.from_numpy(rows, cols, nnz, bs, ell, bci, bv)
+    rng = np.random.default_rng(bs)
+    rk = (rng.random(rows) < 0.8).astype(np.uint8)
+    fk = (rng.random(cols) < 0.5).astype(np.uint8)
+    out_rows, out_cols = int(rk.sum()), int(fk.sum())
+    rr = np.full(rows, 0xFFFFFFFF, np.uint32)
+    rr[rk == 1] = rng.permutation(out_rows)
+    fr = np.full(cols, 0xFFFFFFFF, np.uint32)
+    fr[fk == 1] = rng.permutation(out_cols)
+    d_rk, d_fk, d_rr, d_fr = to_dev_u8(rk), to_dev_u8(fk), to_dev_u32(rr), to_dev_u32(fr)
+    rws = A.masked_sparse_reoptimize_workspace()
+    csr_rt.init(rws)
+    assert csr_rt.setup(rws, 0) == 1
+    masks = A.row_feature_mask_view(row_keep=ptr(d_rk), feature_keep=ptr(d_fk), row_remap=ptr(d_rr), feature_remap=ptr(d_fr))
+    cfg = A.masked_sparse_reoptimize_config(out_rows, out_cols, 0, bs, A.LAYOUT_COMPRESSED)
+    out = A.masked_sparse_reoptimize_result()
+    assert csr_rt.manual_reoptimize_masked_sparse(bell.view(), masks, cfg, rws, out) == 1
+    mp, mi, mv, n = O.reoptimize(rows, cols, bs, ell, bci, np.asarray(bv).view(np.uint16), rk, fk, rr, fr, out_rows, out_cols)
+    assert out.metadata.live_nnz == n
+    assert np.array_equal(device_to_numpy(out.compressed.majorPtr, out_rows + 1, np.uint32), mp)
+    assert np.array_equal(device_to_numpy(out.compressed.minorIdx, n, np.uint32), mi)
+    assert np.array_equal(device_to_numpy(out.compressed.val, n, np.uint16), mv)
+    csr_rt.clear(rws)
