@@ -113,22 +113,27 @@ __device__ __forceinline__ void emit_empty_row(unsigned int row, unsigned int gr
     }
 }
 
+constexpr unsigned int MG_TSTRIDE = CSAMD_MAX_MASK_GROUPS + 1u;  // padded row of the group transpose (no bank conflicts)
+
 __device__ __forceinline__ void finish_row(unsigned int row, row_acc a, unsigned int group_count,
                                            const csamd_sparse_group_filter_params &filter, const float *wave_groups,
-                                           const reduce_outputs &o) {
+                                           const reduce_outputs &o, float *tbuf) {
     const unsigned int lane = lane_id();
     const float row_sum = wave_sum_f32(a.sum);
     const float vmax = wave_max_f32(a.vmax);
     const unsigned int detected = wave_sum_u32(a.detected);
     if (group_count != 0u) {
-        // the lanes' group sums to the wave's bins: lane k writes group k
-        float mine = 0.0f;
+        // the lanes' group sums to the wave's bins through an LDS transpose:
+        // lane l writes its row of partials, lane k < group_count adds up
+        // column k in lane order
 #pragma unroll
-        for (unsigned int k = 0; k < CSAMD_MAX_MASK_GROUPS; ++k) {
-            const float t = wave_sum_f32(a.g[k]);
-            if (lane == k) mine = t;
+        for (unsigned int k = 0; k < CSAMD_MAX_MASK_GROUPS; ++k) tbuf[lane * MG_TSTRIDE + k] = a.g[k];
+        lds_fence();
+        if (lane < group_count) {
+            float t = 0.0f;
+            for (unsigned int l = 0; l < 64u; ++l) t += tbuf[l * MG_TSTRIDE + lane];
+            ((float *) wave_groups)[lane] = t;
         }
-        if (lane < CSAMD_MAX_MASK_GROUPS) ((float *) wave_groups)[lane] = mine;
     }
     lds_fence();
     if (lane == 0u) {
@@ -250,6 +255,7 @@ __global__ void __launch_bounds__(MG_THREADS) group_reduce_compressed_kernel(csa
     unsigned int keep_words) {
     using S = typename value_traits<VT>::storage;
     __shared__ float groups[MG_WAVES][CSAMD_MAX_MASK_GROUPS];
+    __shared__ float transpose[MG_WAVES][64u * MG_TSTRIDE];
     extern __shared__ unsigned int keep_lds[];
     const unsigned int *kl = nullptr;
     if (keep_bits != nullptr) {  // the packed feature keep flags, staged once per workgroup
@@ -260,6 +266,7 @@ __global__ void __launch_bounds__(MG_THREADS) group_reduce_compressed_kernel(csa
     const unsigned int wave = threadIdx.x >> 6;
     const unsigned int lane = lane_id();
     float *wg = groups[wave];
+    float *tb = transpose[wave];
     const S *__restrict__ val = (const S *) src.val;
     unsigned int mis = 0u;
     const bool units_ok = mg_units_ok<VT>(src.minorIdx, src.val, &mis);
@@ -274,7 +281,7 @@ __global__ void __launch_bounds__(MG_THREADS) group_reduce_compressed_kernel(csa
         const unsigned int end = src.majorPtr[row + 1u];
         row_acc a = row_acc_zero();
         mg_stream<VT>(a, src.minorIdx, val, begin, end, units_ok, mis, src.cols, masks, fgm, group_count, wg, kl);
-        finish_row(row, a, group_count, filter, wg, o);
+        finish_row(row, a, group_count, filter, wg, o, tb);
     }
 }
 
@@ -306,6 +313,7 @@ __global__ void __launch_bounds__(MG_THREADS) group_reduce_sliced_kernel(csamd_s
     unsigned int keep_words) {
     using S = typename value_traits<VT>::storage;
     __shared__ float groups[MG_WAVES][CSAMD_MAX_MASK_GROUPS];
+    __shared__ float transpose[MG_WAVES][64u * MG_TSTRIDE];
     extern __shared__ unsigned int keep_lds[];
     const unsigned int *kl = nullptr;
     if (keep_bits != nullptr) {  // the packed feature keep flags, staged once per workgroup
@@ -316,6 +324,7 @@ __global__ void __launch_bounds__(MG_THREADS) group_reduce_sliced_kernel(csamd_s
     const unsigned int wave = threadIdx.x >> 6;
     const unsigned int lane = lane_id();
     float *wg = groups[wave];
+    float *tb = transpose[wave];
     const S *__restrict__ val = (const S *) src.val;
     unsigned int mis = 0u;
     const bool units_ok = mg_units_ok<VT>(src.col_idx, src.val, &mis);
@@ -336,7 +345,7 @@ __global__ void __launch_bounds__(MG_THREADS) group_reduce_sliced_kernel(csamd_s
         }
         row_acc a = row_acc_zero();
         mg_stream<VT>(a, src.col_idx, val, slot_base, slot_base + width, units_ok, mis, src.cols, masks, fgm, group_count, wg, kl);
-        finish_row(row, a, group_count, filter, wg, o);
+        finish_row(row, a, group_count, filter, wg, o, tb);
     }
 }
 
@@ -347,6 +356,7 @@ __global__ void __launch_bounds__(MG_THREADS) group_reduce_blocked_kernel(csamd_
     unsigned int keep_words) {
     using S = typename value_traits<VT>::storage;
     __shared__ float groups[MG_WAVES][CSAMD_MAX_MASK_GROUPS];
+    __shared__ float transpose[MG_WAVES][64u * MG_TSTRIDE];
     extern __shared__ unsigned int keep_lds[];
     const unsigned int *kl = nullptr;
     if (keep_bits != nullptr) {  // the packed feature keep flags, staged once per workgroup
@@ -357,6 +367,7 @@ __global__ void __launch_bounds__(MG_THREADS) group_reduce_blocked_kernel(csamd_
     const unsigned int wave = threadIdx.x >> 6;
     const unsigned int lane = lane_id();
     float *wg = groups[wave];
+    float *tb = transpose[wave];
     const S *__restrict__ val = (const S *) src.val;
     const unsigned int bs = src.block_size;
     const unsigned int width = bs != 0u ? src.ell_cols / bs : 0u;
@@ -405,7 +416,7 @@ __global__ void __launch_bounds__(MG_THREADS) group_reduce_blocked_kernel(csamd_
                     for (unsigned int k = 0; k < 4u; ++k) accumulate_kept(a, kp[u][k], c0[u] + k, mg_unit<VT>::get(r[u], k), fgm, gbits);
                 }
             }
-            finish_row(row, a, group_count, filter, wg, o);
+            finish_row(row, a, group_count, filter, wg, o, tb);
             continue;
         }
         for (unsigned int e = lane; e < src.ell_cols; e += 64u) {
@@ -415,7 +426,7 @@ __global__ void __launch_bounds__(MG_THREADS) group_reduce_blocked_kernel(csamd_
             const unsigned int gene = bc != CSAMD_INVALID_COL ? bc * bs + lib : src.cols;
             accumulate(a, gene, value_traits<VT>::decode(val[row_base + e]), src.cols, masks, fgm, group_count, wg);
         }
-        finish_row(row, a, group_count, filter, wg, o);
+        finish_row(row, a, group_count, filter, wg, o, tb);
     }
 }
 
@@ -461,7 +472,7 @@ struct reduce_prelude {
     unsigned int keep_words;
 };
 
-constexpr unsigned int MG_KEEP_WORDS_MAX = 8192u;  // 32 KB of LDS: up to 262,144 columns
+constexpr unsigned int MG_KEEP_WORDS_MAX = 6144u;  // 24 KB of LDS (with the 34 KB group transpose): up to 196,608 columns
 
 // feature_keep bytes -> bits (bit c of word c / 32 set when column c is kept).
 __global__ void mg_pack_keep_kernel(const unsigned char *__restrict__ keep, unsigned int cols, unsigned int *__restrict__ bits,
