@@ -575,12 +575,13 @@ int launch_scatter(const tr_layout &L, bool sorted, unsigned int cDim, unsigned 
         const char *genv = std::getenv("CSAMD_TR_G");
         const unsigned int gw = genv != nullptr ? (unsigned int) std::atoi(genv) : 8u;
         const tr_chunk_order order{L.tiles, L.bands, (unsigned int) per, gw >= 1u && gw <= 64u ? gw : 8u};
-        // staging of K x 256 entries per batch (CSAMD_TR_K = 8 / 12 / 16 for sweeps)
+        // staging of K x 256 entries per batch (CSAMD_TR_K = 8 / 12 / 16 / 20 / 24 for sweeps)
         const char *env = std::getenv("CSAMD_TR_K");
-        const int kw = env != nullptr ? std::atoi(env) : 16;
-        const unsigned int K = kw == 8 ? 8u : kw == 16 ? 16u : 12u;
+        const int kw = env != nullptr ? std::atoi(env) : 20;
+        const unsigned int K = kw == 8 ? 8u : kw == 12 ? 12u : kw == 16 ? 16u : kw == 24 ? 24u : 20u;
         const size_t lds = band_lds_bytes(K * TR_SORT_THREADS, sizeof(S));
-        auto k = K == 8u ? tr_band_kernel<S, 8u> : K == 16u ? tr_band_kernel<S, 16u> : tr_band_kernel<S, 12u>;
+        auto k = K == 8u ? tr_band_kernel<S, 8u> : K == 12u ? tr_band_kernel<S, 12u> : K == 16u ? tr_band_kernel<S, 16u>
+               : K == 24u ? tr_band_kernel<S, 24u> : tr_band_kernel<S, 20u>;
         CSAMD_HIP(hipFuncSetAttribute((const void *) k, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds), "set lds");
         const unsigned long long blocks = per * 8ull;
         CSAMD_REQUIRE(blocks <= 0x7fffffffull, "transpose: too many chunks");
