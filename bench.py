@@ -415,6 +415,7 @@ def run_cspack(args, log):
     1 GiB (the link ceiling) and the reference's own CPU partition load
     (fread into host arrays, its raw payload codec compiled from its
     sources) over the same file."""
+    import shutil
     import tempfile
 
     import numpy as np
@@ -439,84 +440,95 @@ def run_cspack(args, log):
     cws = CspackWorkspace()
     tmpdir = tempfile.mkdtemp(prefix="cspack_")
     path = Path(tmpdir) / "shard.0.cspack"
-    t0 = time.perf_counter()
-    write_cspack_blocked_ell(path, 0, parts, A.VALUE_F16, cws)
-    write_s = time.perf_counter() - t0
-    file_bytes = path.stat().st_size
-    seg_count = sum(int(p.st.segment_count) for p in parts)
-    for p in parts:
-        p.close()
-    del parts
-    torch.cuda.empty_cache()
-
-    def stage(ws_):
+    free = shutil.disk_usage(tmpdir).free
+    if free < 3 * (1 << 30) * P:
+        for p in parts:
+            p.close()
+        os.rmdir(tmpdir)
+        cws.close()
+        ws.close()
+        raise RuntimeError(f"{tmpdir}: {free} bytes free, too few for the shard pack")
+    try:
         t0 = time.perf_counter()
-        with CspackFile(path, 0) as f:
-            out = stage_cspack_shard_blocked_ell(f, A.VALUE_F16, ws_)
-        dt = time.perf_counter() - t0
-        for o in out:
-            o.close()
-        return dt
+        write_cspack_blocked_ell(path, 0, parts, A.VALUE_F16, cws)
+        write_s = time.perf_counter() - t0
+        file_bytes = path.stat().st_size
+        seg_count = sum(int(p.st.segment_count) for p in parts)
+        for p in parts:
+            p.close()
+        del parts
+        torch.cuda.empty_cache()
 
-    stage(cws)  # grows the pinned and device staging buffers
-    times = sorted(stage(cws) for _ in range(3))
-    # the reference's fetch of a canonical Blocked-ELL partition: staged exec
-    # partition -> canonical CSR -> Blocked-ELL 16
-    with CspackFile(path, 0) as f:
-        part0 = stage_cspack_blocked_ell(f, 0, A.VALUE_F16, cws)
-    canonical_compressed_from_bucketed_blocked_ell(part0, ws)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    ccsr = canonical_compressed_from_bucketed_blocked_ell(part0, ws)
-    torch.cuda.synchronize()
-    csr_ms = (time.perf_counter() - t0) * 1e3
-    t0 = time.perf_counter()
-    cbell = blocked_ell_from_compressed(ccsr, 16, ws)
-    torch.cuda.synchronize()
-    cbell_ms = (time.perf_counter() - t0) * 1e3
-    recon = {"canonical_csr_ms": round(csr_ms, 3), "canonical_nnz": ccsr.nnz, "blocked_ell_16_ms": round(cbell_ms, 3),
-             "exec_partition_bytes": int(part0.st.bytes)}
-    part0.close()
-    del ccsr, cbell
-    torch.cuda.empty_cache()
-    one = CspackWorkspace(reader_threads=1)
-    stage(one)
-    one_s = stage(one)
-    one.close()
-    cws.close()
-    # link ceiling: 1 GiB pinned host -> device
-    h = torch.empty(1 << 30, dtype=torch.uint8, pin_memory=True)
-    d = torch.empty(1 << 30, dtype=torch.uint8, device=dev)
-    d.copy_(h, non_blocking=True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(3):
-        d.copy_(h, non_blocking=True)
-    torch.cuda.synchronize()
-    h2d = 3 * (1 << 30) / (time.perf_counter() - t0) / 1e9
-    del h, d
-    torch.cuda.empty_cache()
-    res = {"workload": "cspack_shard_4x_bucketed_bell16_20000x33538_fp16", "partitions": P, "segments": seg_count,
-           "file_bytes": file_bytes, "write_s": round(write_s, 3),
-           "stage_shard_ms": round(times[1] * 1e3, 2), "stage_shard_gb_s": round(file_bytes / times[1] / 1e9, 1),
-           "stage_shard_1_reader_gb_s": round(file_bytes / one_s / 1e9, 1), "pinned_h2d_gb_s": round(h2d, 1),
-           "reconstruct_partition": recon}
-    if not args.skip_cpu:
-        from oracle import oracle as O
-        if O.ref_pack_available():
-            with CspackFile(path, 0) as f:
-                offs = f.partition_offsets
-            r = O.ref_pack()
-            r.ref_load_bell_partition(str(path).encode(), offs[0])  # warm
+        def stage(ws_):
             t0 = time.perf_counter()
-            got = sum(int(r.ref_load_bell_partition(str(path).encode(), o)) for o in offs)
-            cpu_s = time.perf_counter() - t0
-            res["cpu_reference_load"] = {"s": round(cpu_s, 3), "gb_s": round(file_bytes / cpu_s / 1e9, 2), "kind": "reference",
-                                         "cores": 1, "bytes": got,
-                                         "note": "fread of each blob into fresh host arrays, segments through the reference's "
-                                                 "load(FILE*, blocked_ell*) compiled from its sources; host memory only"}
-    path.unlink()
-    os.rmdir(tmpdir)
+            with CspackFile(path, 0) as f:
+                out = stage_cspack_shard_blocked_ell(f, A.VALUE_F16, ws_)
+            dt = time.perf_counter() - t0
+            for o in out:
+                o.close()
+            return dt
+
+        stage(cws)  # grows the pinned and device staging buffers
+        times = sorted(stage(cws) for _ in range(3))
+        # the reference's fetch of a canonical Blocked-ELL partition: staged exec
+        # partition -> canonical CSR -> Blocked-ELL 16
+        with CspackFile(path, 0) as f:
+            part0 = stage_cspack_blocked_ell(f, 0, A.VALUE_F16, cws)
+        canonical_compressed_from_bucketed_blocked_ell(part0, ws)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ccsr = canonical_compressed_from_bucketed_blocked_ell(part0, ws)
+        torch.cuda.synchronize()
+        csr_ms = (time.perf_counter() - t0) * 1e3
+        t0 = time.perf_counter()
+        cbell = blocked_ell_from_compressed(ccsr, 16, ws)
+        torch.cuda.synchronize()
+        cbell_ms = (time.perf_counter() - t0) * 1e3
+        recon = {"canonical_csr_ms": round(csr_ms, 3), "canonical_nnz": ccsr.nnz, "blocked_ell_16_ms": round(cbell_ms, 3),
+                 "exec_partition_bytes": int(part0.st.bytes)}
+        part0.close()
+        del ccsr, cbell
+        torch.cuda.empty_cache()
+        one = CspackWorkspace(reader_threads=1)
+        stage(one)
+        one_s = stage(one)
+        one.close()
+        cws.close()
+        # link ceiling: 1 GiB pinned host -> device
+        h = torch.empty(1 << 30, dtype=torch.uint8, pin_memory=True)
+        d = torch.empty(1 << 30, dtype=torch.uint8, device=dev)
+        d.copy_(h, non_blocking=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(3):
+            d.copy_(h, non_blocking=True)
+        torch.cuda.synchronize()
+        h2d = 3 * (1 << 30) / (time.perf_counter() - t0) / 1e9
+        del h, d
+        torch.cuda.empty_cache()
+        res = {"workload": "cspack_shard_4x_bucketed_bell16_20000x33538_fp16", "partitions": P, "segments": seg_count,
+               "file_bytes": file_bytes, "write_s": round(write_s, 3),
+               "stage_shard_ms": round(times[1] * 1e3, 2), "stage_shard_gb_s": round(file_bytes / times[1] / 1e9, 1),
+               "stage_shard_1_reader_gb_s": round(file_bytes / one_s / 1e9, 1), "pinned_h2d_gb_s": round(h2d, 1),
+               "reconstruct_partition": recon}
+        if not args.skip_cpu:
+            from oracle import oracle as O
+            if O.ref_pack_available():
+                with CspackFile(path, 0) as f:
+                    offs = f.partition_offsets
+                r = O.ref_pack()
+                r.ref_load_bell_partition(str(path).encode(), offs[0])  # warm
+                t0 = time.perf_counter()
+                got = sum(int(r.ref_load_bell_partition(str(path).encode(), o)) for o in offs)
+                cpu_s = time.perf_counter() - t0
+                res["cpu_reference_load"] = {"s": round(cpu_s, 3), "gb_s": round(file_bytes / cpu_s / 1e9, 2), "kind": "reference",
+                                             "cores": 1, "bytes": got,
+                                             "note": "fread of each blob into fresh host arrays, segments through the reference's "
+                                                     "load(FILE*, blocked_ell*) compiled from its sources; host memory only"}
+    finally:
+        if path.exists():
+            path.unlink()
+        os.rmdir(tmpdir)
     ws.close()
     log(f"cspack: {json.dumps(res)}")
     return res
@@ -760,6 +772,20 @@ def run_batched_partitions(args, log):
     return res
 
 
+def guarded(section, args, log):
+    """Runs one of the sections reported beside the headline; a failure is
+    logged and recorded in the JSON line instead of ending the run."""
+    import torch
+
+    try:
+        return section(args, log)
+    except Exception as e:  # the headline line must still be printed
+        log(f"{section.__name__} failed: {type(e).__name__}: {e}")
+        torch.cuda.synchronize()
+        torch.cuda.empty_cache()
+        return {"error": f"{type(e).__name__}: {e}"}
+
+
 def main() -> int:
     args = parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -991,22 +1017,22 @@ def main() -> int:
     torch.cuda.empty_cache()
     lowp = None
     if not args.skip_lowp and not args.reduce_only:
-        lowp = run_reduced_precision(args, log)
+        lowp = guarded(run_reduced_precision, args, log)
     plain = None
     if not args.skip_plain and not args.reduce_only:
-        plain = run_plain_path(args, log)
+        plain = guarded(run_plain_path, args, log)
     batched = None
     if not args.skip_batched and not args.reduce_only:
-        batched = run_batched_partitions(args, log)
+        batched = guarded(run_batched_partitions, args, log)
     refabi = None
     if not args.skip_refabi and not args.reduce_only:
-        refabi = run_reference_abi(args, log)
+        refabi = guarded(run_reference_abi, args, log)
     cspack = None
     if not args.skip_cspack and not args.reduce_only:
-        cspack = run_cspack(args, log)
+        cspack = guarded(run_cspack, args, log)
     serving = None
     if not args.skip_serve and not args.reduce_only:
-        serving = run_serving(args, log)
+        serving = guarded(run_serving, args, log)
     line = {
         "metric": "masked group-reduce HBM GB/s (% of peak)",
         "value": round(value, 1),
