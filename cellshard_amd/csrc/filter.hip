@@ -22,7 +22,9 @@
 // positions (16-byte value groups, four in flight per thread); a count
 // pass, one exclusive scan, then an emit pass that places each thread's
 // entries with a block prefix sum.  Padding (zero values) is rejected
-// before any gather.  Rows that are not whole 16-byte groups take a
+// before any gather; for block sizes up to 32 that hold whole 16-byte
+// groups, a group's kept columns come from one word of per-block-column
+// keep bits, so the column map is read only for surviving values.  Rows that are not whole 16-byte groups take a
 // one-wave-per-row path.
 #include "common.hh"
 
@@ -47,6 +49,11 @@ struct fl_masks {
     const unsigned char *keep_rows, *keep_cols;
     const unsigned int *row_remap, *col_remap;
     const unsigned int *col_map;  // keep and remap of every column in one lookup (~0u: dropped), or null
+    // Kept-column bits of each block column (bit j: column bc * block_size + j
+    // survives), for block sizes up to 32 that are whole 16-byte groups; null
+    // otherwise.  keep_blocks = ceil(cols / block_size).
+    const unsigned int *block_keep;
+    unsigned int keep_blocks;
 };
 
 // col_map[c] = output column of source column c, or ~0u when it is dropped
@@ -60,6 +67,20 @@ __global__ void fl_col_map_kernel(unsigned int cols, fl_masks m, unsigned int *_
             if (o >= m.output_cols) o = ~0u;
         }
         col_map[c] = o;
+    }
+}
+
+// block_keep[b] bit j = column b * bs + j is kept (below cols, col_map not ~0u).
+__global__ void fl_block_keep_kernel(unsigned int cols, unsigned int bs, const unsigned int *__restrict__ col_map,
+                                     unsigned int *__restrict__ block_keep) {
+    const unsigned int blocks = (unsigned int) (((unsigned long long) cols + bs - 1u) / bs);
+    for (unsigned int b = blockIdx.x * blockDim.x + threadIdx.x; b < blocks; b += gridDim.x * blockDim.x) {
+        unsigned int w = 0u;
+        for (unsigned int j = 0; j < bs; ++j) {
+            const unsigned long long c = (unsigned long long) b * bs + j;
+            if (c < cols && col_map[c] != ~0u) w |= 1u << j;
+        }
+        block_keep[b] = w;
     }
 }
 
@@ -105,6 +126,26 @@ __device__ __forceinline__ unsigned int fl_group_mask(const csamd_blocked_ell_vi
                                                       unsigned int e, const fl_group<S> &g, unsigned int *oc) {
     constexpr unsigned int PER = 16u / sizeof(S);
     unsigned int mask = 0u;
+    if (m.block_keep != nullptr) {
+        // The group lies in one block column (block_size % PER == 0): one
+        // lookup of its kept-column bits, and col_map only for survivors.
+        unsigned int nz = 0u;
+#pragma unroll
+        for (unsigned int k = 0; k < PER; ++k) {
+            oc[k] = ~0u;
+            if (e + k < src.ell_cols && fl_nonzero<S>(g.v[k])) nz |= 1u << k;
+        }
+        if (nz == 0u) return 0u;
+        const unsigned int slot = e / src.block_size;
+        const unsigned int bc = slots[slot];
+        if (bc >= m.keep_blocks) return 0u;  // CSAMD_INVALID_COL included
+        const unsigned int j0 = e - slot * src.block_size;
+        mask = nz & (m.block_keep[bc] >> j0);
+#pragma unroll
+        for (unsigned int k = 0; k < PER; ++k)
+            if ((mask >> k) & 1u) oc[k] = m.col_map[bc * src.block_size + j0 + k];
+        return mask;
+    }
 #pragma unroll
     for (unsigned int k = 0; k < PER; ++k) {
         oc[k] = ~0u;
@@ -400,7 +441,8 @@ int filter_parts(const csamd_blocked_ell_view *parts, unsigned int part_count, u
     const size_t off_dup = off_claim + align_up((size_t) output_rows * 4u, 256u);
     const size_t off_ptr = off_dup + 256u;
     const size_t off_map = off_ptr + align_up(((size_t) output_rows + 1u) * 4u, 256u);
-    const size_t off_scan = off_map + align_up((size_t) cols * 4u, 256u);
+    const size_t off_keep = off_map + align_up((size_t) cols * 4u, 256u);
+    const size_t off_scan = off_keep + align_up((size_t) cols * 4u, 256u);
     if (!grow(&st->ptr_block, &st->ptr_bytes, off_scan + exclusive_scan_scratch_bytes(output_rows))) return 0;
     const size_t off_val = align_up((size_t) max_nnz * 4u, 256u);
     if (!grow(&st->val_block, &st->val_bytes, off_val + (size_t) max_nnz * (size_t) vb)) return 0;
@@ -438,12 +480,24 @@ int filter_parts(const csamd_blocked_ell_view *parts, unsigned int part_count, u
             fl_col_map_kernel<<<grid_for(cols, 256u, 1024u), 256, 0, s>>>((unsigned int) cols, mc, col_map);
             CSAMD_HIP(hipGetLastError(), "fl_col_map_kernel");
         }
+        unsigned int *block_keep = (unsigned int *) (pb + off_keep);
+        unsigned int keep_bs = 0u;  // block size block_keep currently holds (stream order keeps reuse safe)
         auto pass = [&](bool emit) -> int {
             unsigned int r0 = 0u;
             for (unsigned int p = 0; p < part_count; ++p) {
                 const csamd_blocked_ell_view &v = parts[p];
+                const bool keep_bits = cols != 0u && v.cols == cols && v.block_size != 0u && v.block_size <= 32u &&
+                                       v.block_size % (16u / (unsigned int) vb) == 0u;
+                if (keep_bits && v.rows != 0u && v.ell_cols != 0u && v.block_size != keep_bs) {
+                    fl_block_keep_kernel<<<grid_for((cols + v.block_size - 1u) / v.block_size, 256u, 1024u), 256, 0, s>>>(
+                        (unsigned int) cols, v.block_size, col_map, block_keep);
+                    CSAMD_HIP(hipGetLastError(), "fl_block_keep_kernel");
+                    keep_bs = v.block_size;
+                }
                 const fl_masks m{output_rows, output_cols, r0, d_keep_rows != nullptr ? d_keep_rows + r0 : nullptr, d_keep_cols,
-                                 d_row_remap != nullptr ? d_row_remap + r0 : nullptr, d_col_remap, col_map};
+                                 d_row_remap != nullptr ? d_row_remap + r0 : nullptr, d_col_remap, col_map,
+                                 keep_bits ? block_keep : nullptr,
+                                 keep_bits ? (unsigned int) ((cols + v.block_size - 1u) / v.block_size) : 0u};
                 if (v.rows != 0u && v.ell_cols != 0u &&
                     !fl_dispatch<fl_launch>(value_type, &v, m, counts, claim, dup, (const unsigned int *) ptr, out_idx, out_val, emit, s))
                     return 0;
