@@ -624,9 +624,46 @@ __device__ __forceinline__ unsigned int reopt_chunk(const csamd_blocked_ell_view
     return m;
 }
 
+// block_keep[b] bit j = column b * bs + j is below cols and maps to an
+// output column (bs <= 32).
+__global__ void reopt_block_keep_kernel(unsigned int cols, unsigned int bs, const unsigned int *__restrict__ col_map,
+                                        unsigned int *__restrict__ block_keep) {
+    const unsigned int blocks = (unsigned int) (((unsigned long long) cols + bs - 1u) / bs);
+    for (unsigned int b = blockIdx.x * blockDim.x + threadIdx.x; b < blocks; b += gridDim.x * blockDim.x) {
+        unsigned int w = 0u;
+        for (unsigned int j = 0; j < bs; ++j) {
+            const unsigned long long c = (unsigned long long) b * bs + j;
+            if (c < cols && col_map[c] != ~0u) w |= 1u << j;
+        }
+        block_keep[b] = w;
+    }
+}
+
+// reopt_chunk with the chunk's kept columns taken from one word of its block
+// column's keep bits: col_map is read only for surviving values.
+__device__ __forceinline__ unsigned int reopt_chunk_keep(const csamd_blocked_ell_view &src, const unsigned int *__restrict__ col_map,
+                                                         const unsigned int *__restrict__ block_keep, unsigned int keep_blocks,
+                                                         unsigned int width, unsigned int row, unsigned int c, uint4 *q,
+                                                         unsigned int (&oc)[8]) {
+    const unsigned int bs = src.block_size;
+    const unsigned int t0 = c * 8u, slot = t0 / bs, j0 = t0 - slot * bs;
+    const unsigned int bc = src.blockColIdx[(unsigned long long) (row / bs) * width + slot];
+    *q = ((const uint4 *) ((const uint16_t *) src.val + (unsigned long long) row * src.ell_cols))[c];
+    const unsigned int w[4] = {q->x, q->y, q->z, q->w};
+    unsigned int nz = 0u;
+#pragma unroll
+    for (unsigned int e = 0; e < 8u; ++e)
+        if (((w[e >> 1] >> (16u * (e & 1u))) & 0x7fffu) != 0u) nz |= 1u << e;
+    const unsigned int m = bc < keep_blocks ? nz & (block_keep[bc] >> j0) : 0u;  // CSAMD_INVALID_COL fails the test
+#pragma unroll
+    for (unsigned int e = 0; e < 8u; ++e) oc[e] = (m >> e) & 1u ? col_map[bc * bs + j0 + e] : ~0u;
+    return m;
+}
+
 __global__ void __launch_bounds__(MG_THREADS) reopt_count_kernel(csamd_blocked_ell_view src, csamd_row_feature_mask_view masks,
                                                                  unsigned int output_rows, unsigned int output_cols,
-                                                                 unsigned int *major_ptr, const unsigned int *__restrict__ col_map) {
+                                                                 unsigned int *major_ptr, const unsigned int *__restrict__ col_map,
+                                                                 const unsigned int *__restrict__ block_keep, unsigned int keep_blocks) {
     const unsigned int wave = threadIdx.x >> 6;
     const unsigned int bs = src.block_size;
     const unsigned int width = bs != 0u ? src.ell_cols / bs : 0u;
@@ -642,7 +679,9 @@ __global__ void __launch_bounds__(MG_THREADS) reopt_count_kernel(csamd_blocked_e
             for (unsigned int c = lane_id(); c < nc; c += 64u) {
                 uint4 q;
                 unsigned int oc[8];
-                count += (unsigned int) __popc(reopt_chunk(src, col_map, width, row, c, &q, oc));
+                count += (unsigned int) __popc(block_keep != nullptr
+                                                   ? reopt_chunk_keep(src, col_map, block_keep, keep_blocks, width, row, c, &q, oc)
+                                                   : reopt_chunk(src, col_map, width, row, c, &q, oc));
             }
             count = wave_sum_u32(count);
             if (lane_id() == 0u) major_ptr[out_row + 1u] = count;
@@ -685,7 +724,8 @@ __global__ void __launch_bounds__(MG_THREADS) reopt_scatter_kernel(csamd_blocked
                                                                    const unsigned int *__restrict__ major_ptr,
                                                                    unsigned int *__restrict__ row_write,
                                                                    unsigned int *__restrict__ minor_idx, uint16_t *__restrict__ val,
-                                                                   const unsigned int *__restrict__ col_map) {
+                                                                   const unsigned int *__restrict__ col_map,
+                                                                   const unsigned int *__restrict__ block_keep, unsigned int keep_blocks) {
     const unsigned int wave = threadIdx.x >> 6;
     const unsigned int bs = src.block_size;
     const unsigned int width = bs != 0u ? src.ell_cols / bs : 0u;
@@ -702,7 +742,10 @@ __global__ void __launch_bounds__(MG_THREADS) reopt_scatter_kernel(csamd_blocked
                 const unsigned int c = c0 + lane_id();
                 uint4 q = make_uint4(0u, 0u, 0u, 0u);
                 unsigned int oc[8];
-                const unsigned int m = c < nc ? reopt_chunk(src, col_map, width, row, c, &q, oc) : 0u;
+                const unsigned int m = c >= nc ? 0u
+                                       : block_keep != nullptr
+                                           ? reopt_chunk_keep(src, col_map, block_keep, keep_blocks, width, row, c, &q, oc)
+                                           : reopt_chunk(src, col_map, width, row, c, &q, oc);
                 const unsigned int cnt = (unsigned int) __popc(m);
                 const unsigned int incl = wave_inclusive_scan_u32(cnt);
                 const unsigned int total = __shfl(incl, 63);
@@ -1122,7 +1165,8 @@ int csamd_manual_reoptimize_masked_sparse(const csamd_blocked_ell_view *src, con
         st->compressed_bytes = bytes;
     }
     const size_t map_off = align_up((size_t) (output_rows == 0u ? 1u : output_rows) * sizeof(unsigned int), 256u);
-    const size_t scratch_needed = map_off + (size_t) (src->cols == 0u ? 1u : src->cols) * sizeof(unsigned int);
+    const size_t keep_off = map_off + align_up((size_t) (src->cols == 0u ? 1u : src->cols) * sizeof(unsigned int), 256u);
+    const size_t scratch_needed = keep_off + (size_t) (src->cols == 0u ? 1u : src->cols) * sizeof(unsigned int);
     if (scratch_needed > st->scratch_bytes) {
         if (st->scratch_block != nullptr) (void) hipFree(st->scratch_block);
         st->scratch_block = nullptr;
@@ -1141,6 +1185,10 @@ int csamd_manual_reoptimize_masked_sparse(const csamd_blocked_ell_view *src, con
     c.val = (csamd_half *) (base + val_off);
     unsigned int *row_write = (unsigned int *) st->scratch_block;
     unsigned int *col_map = (unsigned int *) ((char *) st->scratch_block + map_off);
+    // keep bits for the 16-byte chunk path (block_size a multiple of 8, at most 32)
+    const bool keep_bits = src->cols != 0u && src->block_size != 0u && src->block_size % 8u == 0u && src->block_size <= 32u;
+    unsigned int *block_keep = keep_bits ? (unsigned int *) ((char *) st->scratch_block + keep_off) : nullptr;
+    const unsigned int keep_blocks = keep_bits ? (unsigned int) (((unsigned long long) src->cols + src->block_size - 1u) / src->block_size) : 0u;
 
     if (!fill_u32(c.majorPtr, (unsigned long long) output_rows + 1ull, 0u, s)) return 0;
     if (!fill_u32(row_write, output_rows, 0u, s)) return 0;
@@ -1149,15 +1197,20 @@ int csamd_manual_reoptimize_masked_sparse(const csamd_blocked_ell_view *src, con
         reopt_col_map_kernel<<<grid_for(src->cols, 256u, 1024u), 256, 0, s>>>(m, src->cols, output_cols, col_map);
         CSAMD_HIP(hipGetLastError(), "reopt_col_map_kernel");
     }
+    if (keep_bits) {
+        reopt_block_keep_kernel<<<grid_for(keep_blocks, 256u, 1024u), 256, 0, s>>>(src->cols, src->block_size, col_map, block_keep);
+        CSAMD_HIP(hipGetLastError(), "reopt_block_keep_kernel");
+    }
     if (src->rows != 0u) {
-        reopt_count_kernel<<<grid, MG_THREADS, 0, s>>>(*src, m, output_rows, output_cols, c.majorPtr, col_map);
+        reopt_count_kernel<<<grid, MG_THREADS, 0, s>>>(*src, m, output_rows, output_cols, c.majorPtr, col_map, block_keep,
+                                                       keep_blocks);
         CSAMD_HIP(hipGetLastError(), "reopt_count_kernel");
     }
     reopt_scan_kernel<<<1, 1024, 0, s>>>(output_rows, c.majorPtr);
     CSAMD_HIP(hipGetLastError(), "reopt_scan_kernel");
     if (src->rows != 0u) {
         reopt_scatter_kernel<<<grid, MG_THREADS, 0, s>>>(*src, m, output_rows, output_cols, c.majorPtr, row_write, c.minorIdx,
-                                                         (uint16_t *) c.val, col_map);
+                                                         (uint16_t *) c.val, col_map, block_keep, keep_blocks);
         CSAMD_HIP(hipGetLastError(), "reopt_scatter_kernel");
     }
     unsigned int host_nnz = 0u;
