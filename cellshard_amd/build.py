@@ -94,6 +94,30 @@ def build_hip(verbose: bool = False) -> Path:
     return LIB
 
 
+CPP_TESTS = ["drop_in_boundary"]
+CPP_TEST_DIR = ROOT / "tests" / "cpp"
+
+
+def build_cpp_tests(verbose: bool = False) -> list:
+    """C++ callers of the drop-in header (tests/cpp/*.cpp), linked against the
+    HIP library by a relative rpath: tests/cpp/bin/<name>."""
+    hipcc = _hipcc()
+    out_dir = CPP_TEST_DIR / "bin"
+    out_dir.mkdir(exist_ok=True)
+    outs = []
+    for name in CPP_TESTS:
+        out = out_dir / name
+        tmp = out.with_suffix(".tmp")
+        _run([hipcc, "--offload-arch=gfx950", "-std=c++17", "-O1", "-Wall", "-I", str(ROOT / "include"),
+              str(CPP_TEST_DIR / f"{name}.cpp"), "-o", str(tmp), "-L", str(PKG), "-lcellshard_amd",
+              "-Wl,-rpath,$ORIGIN/../../../cellshard_amd"])
+        os.replace(tmp, out)
+        outs.append(out)
+        if verbose:
+            print(f"built {out}")
+    return outs
+
+
 def build_oracle(verbose: bool = False) -> Path:
     """Build the CPU oracle (plain C) and, if the reference tree exists, the
     reference CPU converter probe under oracle/_ref."""
@@ -112,6 +136,7 @@ def build_oracle(verbose: bool = False) -> Path:
 
 def main() -> None:
     build_hip(verbose=True)
+    build_cpp_tests(verbose=True)
     build_oracle(verbose=True)
 
 

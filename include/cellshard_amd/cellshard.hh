@@ -188,6 +188,55 @@ inline types::u32 uniform_slice_rows(const sliced_ell *m) {
     return step;
 }
 
+// formats/sliced_ell.cuh:70-79: first slot of slice `slice_id` (slices are
+// stored back to back); 0 past the last slice boundary.
+inline std::size_t slice_slot_base(const sliced_ell *m, types::u32 slice_id) {
+    if (m == nullptr || m->slice_row_offsets == nullptr || m->slice_widths == nullptr || slice_id > m->slice_count) return 0u;
+    std::size_t base = 0u;
+    for (types::u32 s = 0; s < slice_id; ++s)
+        base += (std::size_t) (m->slice_row_offsets[s + 1u] - m->slice_row_offsets[s]) * m->slice_widths[s];
+    return base;
+}
+// formats/sliced_ell.cuh:81-88: the slice holding `row` (slice_count when
+// none).  Offsets ascend, so a binary search gives the reference's answer.
+inline types::u32 find_slice(const sliced_ell *m, types::dim_t row) {
+    if (m == nullptr) return 0u;
+    if (m->slice_row_offsets == nullptr || row >= m->rows) return m->slice_count;
+    types::u32 lo = 0u, hi = m->slice_count;  // first slice whose end is > row
+    while (lo < hi) {
+        const types::u32 mid = lo + ((hi - lo) >> 1);
+        if (row < m->slice_row_offsets[mid + 1u]) hi = mid;
+        else lo = mid + 1u;
+    }
+    return lo;
+}
+// formats/sliced_ell.cuh:113-125: valid (column != invalid) slots of a row.
+inline types::u32 row_nnz(const sliced_ell *m, types::dim_t row) {
+    const types::u32 slice = find_slice(m, row);
+    if (m == nullptr || slice >= m->slice_count) return 0u;
+    const types::u32 width = m->slice_widths[slice];
+    const std::size_t base = slice_slot_base(m, slice) + (std::size_t) (row - m->slice_row_offsets[slice]) * width;
+    types::u32 n = 0u;
+    for (types::u32 k = 0; k < width; ++k) n += m->col_idx[base + k] != sliced_ell_invalid_col ? 1u : 0u;
+    return n;
+}
+
+// Host footprint of one materialized part (formats/compressed.cuh:69-74,
+// blocked_ell.cuh:78-82, sliced_ell.cuh:100-111): the struct plus its arrays.
+inline std::size_t bytes(const compressed *m) {
+    return sizeof(*m) + (std::size_t) (major_dim(m) + 1u) * 4u + (std::size_t) m->nnz * 4u + (std::size_t) m->nnz * 2u;
+}
+inline std::size_t bytes(const blocked_ell *m) {
+    return sizeof(*m) + (std::size_t) row_block_count(m) * ell_width_blocks(m) * 4u + (std::size_t) m->rows * m->ell_cols * 2u;
+}
+inline std::size_t bytes(const sliced_ell *m) {
+    const std::size_t slots = total_slots(m);
+    const std::size_t w_off = align_up_(((std::size_t) m->slice_count + 1u) * 4u, 4u);
+    const std::size_t col_off = align_up_(w_off + (std::size_t) m->slice_count * 4u, 4u);
+    const std::size_t val_off = align_up_(col_off + slots * 4u, 2u);
+    return sizeof(*m) + val_off + slots * 2u;
+}
+
 // formats/sliced_ell.cuh:165-224
 inline int allocate(sliced_ell *m, types::u32 slice_count, const types::u32 *offsets, const types::u32 *widths) {
     if (m == nullptr || offsets == nullptr || widths == nullptr) return 0;

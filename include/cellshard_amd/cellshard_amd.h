@@ -1170,6 +1170,146 @@ int csamd_upload_compressed_shard(const csamd_host_compressed *const *parts, uns
                                   csamd_allocation_cache *cache, csamd_stream_t stream);
 
 /* ---------------------------------------------------------------------------
+ * sharded<MatrixT>: partition / shard metadata of a row-partitioned matrix.
+ * runtime/layout/sharded.cuh:30-50 (layout, field for field), :71-124
+ * (partition_nnz / partition_aux), :172-252 (lookups), :262-344 (bytes);
+ * runtime/host/sharded_host.cuh:14-396 (metadata operations).  The layout
+ * does not depend on MatrixT (parts is an array of part pointers), so one C
+ * struct serves every format; calls that read a part or its aux word take
+ * the part format (CSAMD_FORMAT_*).  Host memory only; nothing here touches
+ * a device.  parts[i] may be NULL (metadata only); a non-NULL part is a host
+ * struct of the format, owned by the sharded matrix (destroyed by clear and
+ * the drop calls: its storage released, the struct itself freed).
+ * ------------------------------------------------------------------------- */
+enum {
+    CSAMD_FORMAT_COMPRESSED = 1,
+    CSAMD_FORMAT_BLOCKED_ELL = 2,
+    CSAMD_FORMAT_SLICED_ELL = 3
+};
+
+typedef struct CSAMD_ALIGN16 csamd_sharded {
+    unsigned long rows;
+    unsigned long cols;
+    unsigned long nnz;
+    unsigned long num_partitions;
+    unsigned long partition_capacity;
+    void **parts;
+    unsigned long *partition_offsets;  /* [capacity + 1] */
+    unsigned long *partition_rows;
+    unsigned long *partition_nnz;
+    unsigned long *partition_aux;
+    unsigned long num_shards;
+    unsigned long shard_capacity;
+    unsigned long *shard_offsets;      /* [shard capacity + 1] row offsets */
+    unsigned long *shard_parts;        /* [shard capacity + 1] first partition of each shard */
+} csamd_sharded;
+
+/* sharded.cuh:54-69 */
+void csamd_sharded_init(csamd_sharded *m);
+/* sharded_host.cuh:21-36: destroys the parts, frees the metadata, re-inits */
+void csamd_sharded_clear(csamd_sharded *m, unsigned int format);
+/* sharded_host.cuh:38-83, :85-109 (grow-only; existing entries kept) */
+int csamd_sharded_reserve_partitions(csamd_sharded *m, unsigned long capacity);
+int csamd_sharded_reserve_shards(csamd_sharded *m, unsigned long capacity);
+/* sharded_host.cuh:111-126, :128-141 */
+void csamd_sharded_rebuild_shard_parts(csamd_sharded *m);
+void csamd_sharded_rebuild_partition_offsets(csamd_sharded *m);
+/* sharded_host.cuh:143-167 (rows / nnz / aux arrays may be NULL: zeros); shards = partitions */
+int csamd_sharded_define_partitions(csamd_sharded *m, unsigned long cols, unsigned long num_partitions,
+                                    const unsigned long *partition_rows, const unsigned long *partition_nnz,
+                                    const unsigned long *partition_aux);
+/* sharded_host.cuh:169-176 */
+int csamd_sharded_set_shards_to_partitions(csamd_sharded *m);
+/* sharded_host.cuh:178-198: takes ownership of `part` (a host struct of `format`, or NULL) */
+int csamd_sharded_append_partition(csamd_sharded *m, unsigned int format, void *part);
+/* sharded_host.cuh:200-228: moves src's parts to the end of dst; src keeps no partitions */
+int csamd_sharded_concatenate(csamd_sharded *dst, csamd_sharded *src);
+/* sharded_host.cuh:230-266 */
+int csamd_sharded_set_equal_shards(csamd_sharded *m, unsigned long count);
+/* sharded_host.cuh:268-308: *out_offsets is malloc'ed ([*out_count + 1]); the caller frees it */
+int csamd_sharded_build_shard_offsets_by_rows(const csamd_sharded *m, unsigned long target_rows_per_shard,
+                                              unsigned long **out_offsets, unsigned long *out_count);
+/* sharded_host.cuh:310-327: cuts must be partition boundaries, 0 first, rows last */
+int csamd_sharded_reshard(csamd_sharded *m, unsigned long count, const unsigned long *offsets);
+/* sharded_host.cuh:329-357, :359-389 */
+int csamd_sharded_set_shards_by_nnz(csamd_sharded *m, unsigned long max_nnz);
+int csamd_sharded_set_shards_by_part_bytes(csamd_sharded *m, unsigned int format, size_t max_bytes);
+/* sharded_host.cuh:514-531, :533-548 (host materialization only) */
+int csamd_sharded_drop_partition(csamd_sharded *m, unsigned int format, unsigned long part);
+int csamd_sharded_drop_shard(csamd_sharded *m, unsigned int format, unsigned long shard);
+/* sharded.cuh:101-124 (aux word of a host part: compressed axis, packed Blocked-ELL
+ * (width << 16 | block size), packed Sliced-ELL (total slots << 32 | slice count)) */
+unsigned long csamd_partition_aux(unsigned int format, const void *part);
+/* sharded.cuh:127-139, :178-190 etc.: row -> partition / shard (count when outside) */
+unsigned long csamd_sharded_find_partition(const csamd_sharded *m, unsigned long row);
+unsigned long csamd_sharded_find_shard(const csamd_sharded *m, unsigned long row);
+unsigned long csamd_sharded_first_partition_in_shard(const csamd_sharded *m, unsigned long shard);
+unsigned long csamd_sharded_last_partition_in_shard(const csamd_sharded *m, unsigned long shard);
+int csamd_sharded_row_is_partition_boundary(const csamd_sharded *m, unsigned long row);
+/* sharded.cuh:262-344: host bytes of a partition (its struct + arrays; from the metadata
+ * when the part is not materialized), of a shard, and of the whole matrix */
+size_t csamd_sharded_partition_bytes(const csamd_sharded *m, unsigned int format, unsigned long part);
+size_t csamd_sharded_shard_bytes(const csamd_sharded *m, unsigned int format, unsigned long shard);
+size_t csamd_sharded_bytes(const csamd_sharded *m, unsigned int format);
+
+/* ---------------------------------------------------------------------------
+ * sharded_device<MatrixT>: device residency of a sharded matrix's partitions.
+ * runtime/device/sharded_device.cuh:43-50 (layout, field for field), :119-178
+ * (init / clear / reserve), :1098-1140 (device bytes, shards by device bytes),
+ * :1142-1198 (upload / release partition), :1200-1432 (packed shard upload),
+ * :1434-1545 (upload / release shard), :1555-1722 (stage / swap).
+ * Device bytes are this library's footprint: one allocation per partition
+ * (or per packed shard) with the device view at offset 0 and every array
+ * 256-byte aligned.  Stream NULL = synchronous.
+ * ------------------------------------------------------------------------- */
+typedef struct csamd_sharded_device {
+    unsigned long capacity;
+    csamd_partition_record *parts;       /* [capacity] */
+    unsigned long cache_count;           /* the allocation cache (csamd_allocation_cache fields) */
+    unsigned long cache_capacity;
+    csamd_cached_allocation *cache;
+} csamd_sharded_device;
+
+void csamd_sharded_device_init(csamd_sharded_device *s);
+/* releases every resident partition, frees the cached allocations and the table */
+int csamd_sharded_device_clear(csamd_sharded_device *s);
+int csamd_sharded_device_reserve(csamd_sharded_device *s, unsigned long capacity);
+/* device bytes one partition / shard takes once uploaded (from the metadata) */
+size_t csamd_sharded_device_partition_bytes(const csamd_sharded *view, unsigned int format, unsigned long part);
+size_t csamd_sharded_device_shard_bytes(const csamd_sharded *view, unsigned int format, unsigned long shard);
+/* sharded_device.cuh:1111-1140: shards cut so each holds at most max_bytes of device bytes */
+int csamd_sharded_set_shards_by_device_bytes(csamd_sharded *view, unsigned int format, size_t max_bytes);
+/* sharded_device.cuh:1142-1165: the host part must be materialized; a partition already
+ * resident on `device` is kept, one resident elsewhere is released first */
+int csamd_sharded_device_upload_partition(csamd_sharded_device *s, const csamd_sharded *view, unsigned int format,
+                                          unsigned long part, int device, csamd_stream_t stream);
+/* sharded_device.cuh:1167-1198: a partition of a packed shard releases its whole group;
+ * the allocation goes to the cache (16 entries) or is freed */
+int csamd_sharded_device_release_partition(csamd_sharded_device *s, unsigned long part);
+/* sharded_device.cuh:1434-1524: compressed and Blocked-ELL shards go up packed into one
+ * allocation (views back to back at its start); Sliced-ELL shards one partition at a time.
+ * A shard whose partitions are all resident on `device` is kept as it is. */
+int csamd_sharded_device_upload_shard(csamd_sharded_device *s, const csamd_sharded *view, unsigned int format,
+                                      unsigned long shard, int device, csamd_stream_t stream);
+/* sharded_device.cuh:1526-1545 */
+int csamd_sharded_device_release_shard(csamd_sharded_device *s, const csamd_sharded *view, unsigned long shard);
+/* sharded_device.cuh:1631-1674 (stage_shard / stage_shard_async).  When every host part of
+ * the shard is materialized: upload_shard.  Otherwise the parts come from `pack` (Blocked-ELL
+ * only, a CSPACK01 file of the shard's bucketed execution partitions in shard order; the
+ * reference's fetch_shard reads the same blobs): each partition is staged from the file
+ * straight into HBM and rebuilt there as its canonical Blocked-ELL partition at the block
+ * size of its aux word, which becomes the resident device partition, with no host copy.
+ * drop_host_after_upload drops materialized host parts afterwards. */
+int csamd_sharded_device_stage_shard(csamd_sharded_device *s, csamd_sharded *view, unsigned int format,
+                                     const csamd_cspack_file *pack, unsigned long shard, int device, csamd_stream_t stream,
+                                     int drop_host_after_upload);
+/* sharded_device.cuh:1676-1722: stage `in_shard`, then release `out_shard` (both resident
+ * for a moment) and optionally drop the outgoing shard's host parts */
+int csamd_sharded_device_swap_shard(csamd_sharded_device *s, csamd_sharded *view, unsigned int format,
+                                    const csamd_cspack_file *pack, unsigned long out_shard, unsigned long in_shard, int device,
+                                    csamd_stream_t stream, int drop_host_after_upload, int drop_host_after_release);
+
+/* ---------------------------------------------------------------------------
  * Synthetic CSR generator for measurement and tests (not a reference path).
  * Rows: nnz ~ round(LogNormal(ln median, sigma)) capped at cols; columns
  * drawn without replacement with weights w_c (Efraimidis-Spirakis keys

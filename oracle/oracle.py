@@ -156,6 +156,53 @@ def label_reduce_csr(rows, cols, ptr, idx, val, vt, labels, row_keep, feature_ke
     return _label_finish(G, F, s, q, n, gr, ids)
 
 
+def worker_threads() -> int:
+    """Host threads for the chunked oracle: the cores this process may run on,
+    capped by OMP_NUM_THREADS when that is set (the CPU share of a GPU box)."""
+    import os
+
+    try:
+        n = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        n = os.cpu_count() or 1
+    try:
+        cap = int(os.environ.get("OMP_NUM_THREADS", "0"))
+    except ValueError:
+        cap = 0
+    return max(1, min(n, cap) if cap > 0 else n)
+
+
+def label_reduce_csr_chunked(rows, cols, ptr, idx, val, vt, labels, row_keep, feature_keep, G, scale=None, threads=None,
+                             chunks_per_thread=4):
+    """label_reduce_csr over contiguous row chunks on `threads` host threads
+    (ctypes releases the GIL during each call).  Chunk partials are added in
+    FP64 in chunk order: counts are exact, sums equal the one-pass oracle's
+    up to FP64 rounding.  ptr may hold offsets up to 2^32 - 1 (any integer
+    dtype); idx / val are the whole arrays."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    threads = threads or worker_threads()
+    ptr = np.asarray(ptr).astype(np.int64, copy=False)
+    n = max(1, min(rows, threads * chunks_per_thread)) if rows else 1
+    cuts = np.linspace(0, rows, n + 1).astype(np.int64)
+    labels, row_keep = _u32(labels), _u8(row_keep)
+
+    def chunk(k):
+        a, b = int(cuts[k]), int(cuts[k + 1])
+        pa = (ptr[a:b + 1] - ptr[a]).astype(np.uint32)
+        return label_reduce_csr(b - a, cols, pa, idx[ptr[a]:ptr[b]], val[ptr[a]:ptr[b]], vt, labels[a:b],
+                                None if row_keep is None else row_keep[a:b], feature_keep, G, scale)
+
+    with ThreadPoolExecutor(threads) as ex:
+        parts = list(ex.map(chunk, range(n)))
+    out = dict(parts[0])
+    for key in ("sum", "sumsq"):
+        out[key] = np.sum([p[key] for p in parts], axis=0, dtype=np.float64) if len(parts) > 1 else parts[0][key]
+    for key in ("nnz", "group_rows"):
+        out[key] = np.sum([p[key].astype(np.uint64) for p in parts], axis=0).astype(np.uint32)
+    return out
+
+
 def label_reduce_sell(rows, cols, slice_rows, offsets, widths, slot_offsets, col_idx, val, vt, labels, row_keep, feature_keep, G,
                       scale=None):
     ids = kept_features(cols, feature_keep)

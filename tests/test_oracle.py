@@ -283,6 +283,37 @@ def test_sliced_ell_tuner_closed_form():
     assert abs(out["score"] - best[0]) < 1e-12
 
 
+def test_sliced_ell_tuner_ties():
+    # S = 8 and S = 4 score the same (136/128 + 0.25 * 1/8 == 132/128 + 0.25 * 2/8):
+    # the one with fewer padded bytes (S = 4: 812 < 828) wins, in either order
+    rn = np.array([16, 16, 16, 16, 17, 17, 15, 15], np.uint32)
+    for cands in ([8, 4], [4, 8]):
+        out = O.choose_sliced_ell_slice_rows(rn, len(rn), int(rn.sum()), cands)
+        assert out["slice_rows"] == 4 and out["padded_slots"] == 132
+    # S = 8 and S = 2 tie on score (1.15625) and on padded bytes (892): the
+    # larger slice height wins, in either order
+    rn = np.array([7, 7, 7, 7, 7, 7, 1, 1] + [6] * 7 + [2] + [5] * 8, np.uint32)
+    assert rn.sum() == 128
+    for cands in ([2, 8], [8, 2]):
+        out = O.choose_sliced_ell_slice_rows(rn, len(rn), 128, cands)
+        assert out["slice_rows"] == 8 and out["padded_slots"] == 144
+
+
+def test_label_reduce_chunked_equals_one_pass():
+    rows, cols, G = 300, 90, 7
+    ptr, idx, val = random_csr(rows, cols, 0.2, seed=33, zeros=0.05, empty_rows=0.1)
+    rng = np.random.default_rng(5)
+    labels = rng.integers(0, G + 1, size=rows).astype(np.uint32)
+    rk = (rng.random(rows) < 0.8).astype(np.uint8)
+    fk = (rng.random(cols) < 0.5).astype(np.uint8)
+    bits = f16_bits(val)
+    a = O.label_reduce_csr(rows, cols, ptr, idx, bits, F16, labels, rk, fk, G)
+    for threads in (1, 3):
+        b = O.label_reduce_csr_chunked(rows, cols, ptr, idx, bits, F16, labels, rk, fk, G, threads=threads)
+        for k in ("sum", "sumsq", "nnz", "group_rows", "kept_feature_ids"):
+            assert np.array_equal(a[k], b[k]), k  # integer data: FP64 sums are exact in any order
+
+
 def test_transpose_matches_scipy():
     for seed, (rows, cols) in enumerate([(40, 30), (1, 7), (25, 1), (60, 200)]):
         ptr, idx, val = random_csr(rows, cols, 0.2, seed=seed, empty_rows=0.1)
