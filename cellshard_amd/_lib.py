@@ -174,6 +174,49 @@ PROTOTYPES = {
                                              P(A.allocation_cache), vp]),
     "csamd_upload_compressed_shard": (i32, [P(P(A.host_compressed)), u32, P(A.partition_record), P(A.compressed_view),
                                             P(A.allocation_cache), vp]),
+    # sharded<T> metadata and sharded_device<T> residency
+    "csamd_sharded_init": (None, [P(A.sharded)]),
+    "csamd_sharded_clear": (None, [P(A.sharded), u32]),
+    "csamd_host_part_alloc": (vp, [u32]),
+    "csamd_host_compressed_allocate": (i32, [P(A.host_compressed)]),
+    "csamd_sharded_reserve_partitions": (i32, [P(A.sharded), C.c_ulong]),
+    "csamd_sharded_reserve_shards": (i32, [P(A.sharded), C.c_ulong]),
+    "csamd_sharded_rebuild_shard_parts": (None, [P(A.sharded)]),
+    "csamd_sharded_rebuild_partition_offsets": (None, [P(A.sharded)]),
+    "csamd_sharded_define_partitions": (i32, [P(A.sharded), u32, C.c_ulong, C.c_ulong, vp, vp, vp]),
+    "csamd_sharded_set_shards_to_partitions": (i32, [P(A.sharded)]),
+    "csamd_sharded_append_partition": (i32, [P(A.sharded), u32, vp]),
+    "csamd_sharded_concatenate": (i32, [P(A.sharded), P(A.sharded)]),
+    "csamd_sharded_set_equal_shards": (i32, [P(A.sharded), C.c_ulong]),
+    "csamd_sharded_build_shard_offsets_by_rows": (i32, [P(A.sharded), C.c_ulong, P(P(C.c_ulong)), P(C.c_ulong)]),
+    "csamd_sharded_reshard": (i32, [P(A.sharded), C.c_ulong, vp]),
+    "csamd_sharded_set_shards_by_nnz": (i32, [P(A.sharded), C.c_ulong]),
+    "csamd_sharded_set_shards_by_part_bytes": (i32, [P(A.sharded), u32, sz]),
+    "csamd_sharded_drop_partition": (i32, [P(A.sharded), u32, C.c_ulong]),
+    "csamd_sharded_drop_shard": (i32, [P(A.sharded), u32, C.c_ulong]),
+    "csamd_partition_aux": (C.c_ulong, [u32, vp]),
+    "csamd_sharded_find_partition": (C.c_ulong, [P(A.sharded), C.c_ulong]),
+    "csamd_sharded_find_shard": (C.c_ulong, [P(A.sharded), C.c_ulong]),
+    "csamd_sharded_first_partition_in_shard": (C.c_ulong, [P(A.sharded), C.c_ulong]),
+    "csamd_sharded_last_partition_in_shard": (C.c_ulong, [P(A.sharded), C.c_ulong]),
+    "csamd_sharded_row_is_partition_boundary": (i32, [P(A.sharded), C.c_ulong]),
+    "csamd_sharded_partition_bytes": (sz, [P(A.sharded), u32, C.c_ulong]),
+    "csamd_sharded_shard_bytes": (sz, [P(A.sharded), u32, C.c_ulong]),
+    "csamd_sharded_bytes": (sz, [P(A.sharded), u32]),
+    "csamd_sharded_device_init": (None, [P(A.sharded_device)]),
+    "csamd_sharded_device_clear": (i32, [P(A.sharded_device)]),
+    "csamd_sharded_device_reserve": (i32, [P(A.sharded_device), C.c_ulong]),
+    "csamd_sharded_device_partition_bytes": (sz, [P(A.sharded), u32, C.c_ulong]),
+    "csamd_sharded_device_shard_bytes": (sz, [P(A.sharded), u32, C.c_ulong]),
+    "csamd_sharded_set_shards_by_device_bytes": (i32, [P(A.sharded), u32, sz]),
+    "csamd_sharded_device_upload_partition": (i32, [P(A.sharded_device), P(A.sharded), u32, C.c_ulong, i32, vp]),
+    "csamd_sharded_device_release_partition": (i32, [P(A.sharded_device), C.c_ulong]),
+    "csamd_sharded_device_upload_shard": (i32, [P(A.sharded_device), P(A.sharded), u32, C.c_ulong, i32, vp]),
+    "csamd_sharded_device_release_shard": (i32, [P(A.sharded_device), P(A.sharded), C.c_ulong]),
+    "csamd_sharded_device_stage_shard": (i32, [P(A.sharded_device), P(A.sharded), u32, P(A.cspack_file), C.c_ulong, i32, vp, i32]),
+    "csamd_sharded_device_stage_partition": (i32, [P(A.sharded_device), P(A.sharded), u32, P(A.cspack_file), C.c_ulong, i32, vp, i32]),
+    "csamd_sharded_device_swap_shard": (i32, [P(A.sharded_device), P(A.sharded), u32, P(A.cspack_file), C.c_ulong, C.c_ulong, i32,
+                                              vp, i32, i32]),
     # synth / scan
     "csamd_synth_row_nnz": (i32, [P(A.synth_params), vp, vp]),
     "csamd_exclusive_scan_u32": (i32, [vp, u32, vp, P(A.convert_workspace), P(u64)]),

@@ -252,6 +252,25 @@ partition_record = _aligned16("partition_record", [
 
 cached_allocation = _aligned16("cached_allocation", [("ptr", c_void_p), ("bytes", c_size_t), ("device_id", C.c_int)])
 
+FORMAT_COMPRESSED = 1
+FORMAT_BLOCKED_ELL = 2
+FORMAT_SLICED_ELL = 3
+
+c_ulong = C.c_ulong
+c_ulong_p = C.POINTER(C.c_ulong)
+
+sharded = _aligned16("sharded", [
+    ("rows", c_ulong), ("cols", c_ulong), ("nnz", c_ulong), ("num_partitions", c_ulong), ("partition_capacity", c_ulong),
+    ("parts", C.POINTER(c_void_p)), ("partition_offsets", c_ulong_p), ("partition_rows", c_ulong_p),
+    ("partition_nnz", c_ulong_p), ("partition_aux", c_ulong_p), ("num_shards", c_ulong), ("shard_capacity", c_ulong),
+    ("shard_offsets", c_ulong_p), ("shard_parts", c_ulong_p),
+])
+
+sharded_device = type("sharded_device", (C.Structure,), {"_fields_": [
+    ("capacity", c_ulong), ("parts", C.POINTER(partition_record)), ("cache_count", c_ulong), ("cache_capacity", c_ulong),
+    ("cache", C.POINTER(cached_allocation)),
+]})
+
 allocation_cache = _aligned16("allocation_cache", [
     ("cache_count", C.c_ulong), ("cache_capacity", C.c_ulong), ("cache", C.POINTER(cached_allocation)),
 ])
@@ -344,6 +363,8 @@ ALL_STRUCTS = {
     "csamd_partition_record": partition_record,
     "csamd_cached_allocation": cached_allocation,
     "csamd_allocation_cache": allocation_cache,
+    "csamd_sharded": sharded,
+    "csamd_sharded_device": sharded_device,
     "csamd_serve_query": serve_query,
     "csamd_serve_result": serve_result,
     "csamd_serve_config": serve_config,

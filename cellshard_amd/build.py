@@ -38,6 +38,7 @@ HIP_SOURCES = [
     "filter.hip",
     "column_order.hip",
     "cspack.hip",
+    "sharded.hip",
 ]
 
 HIPCC_FLAGS = [

@@ -723,7 +723,11 @@ inline int build_row_order_from_coo(const sparse::coo *src, const std::uint32_t 
 } // namespace bucket
 
 // ---------------------------------------------------------------------------
-// sharded<MatrixT>: partition / shard metadata (runtime/layout/sharded.cuh:30-252).
+// sharded<MatrixT>: partition / shard metadata (runtime/layout/sharded.cuh:30-344,
+// runtime/host/sharded_host.cuh:14-548).  Layout identical to csamd_sharded;
+// the metadata logic lives in the library (csamd_sharded_*).  Parts are the
+// caller's `new MatrixT` objects: clear() / drop_*() destroy them with
+// sparse::clear + delete, as the reference does.
 template<typename MatrixT>
 struct alignas(16) sharded {
     unsigned long rows;
@@ -742,21 +746,95 @@ struct alignas(16) sharded {
     unsigned long *shard_parts;
 };
 
+template<typename MatrixT> struct part_format;
+template<> struct part_format<sparse::compressed> { static constexpr unsigned int value = CSAMD_FORMAT_COMPRESSED; };
+template<> struct part_format<sparse::blocked_ell> { static constexpr unsigned int value = CSAMD_FORMAT_BLOCKED_ELL; };
+template<> struct part_format<sparse::sliced_ell> { static constexpr unsigned int value = CSAMD_FORMAT_SLICED_ELL; };
+
 template<typename MatrixT>
-inline void init(sharded<MatrixT> *m) {
-    std::memset(m, 0, sizeof(*m));
+inline csamd_sharded *c_sharded(sharded<MatrixT> *m) { return reinterpret_cast<csamd_sharded *>(m); }
+template<typename MatrixT>
+inline const csamd_sharded *c_sharded(const sharded<MatrixT> *m) { return reinterpret_cast<const csamd_sharded *>(m); }
+
+template<typename MatrixT>
+inline void init(sharded<MatrixT> *m) { csamd_sharded_init(c_sharded(m)); }
+
+// sharded_host.cuh:14-19
+template<typename MatrixT>
+inline void destroy(MatrixT *m) {
+    if (m == nullptr) return;
+    sparse::clear(m);
+    delete m;
 }
 
 template<typename MatrixT>
-inline unsigned long find_partition(const sharded<MatrixT> *m, unsigned long row) {
-    if (m->partition_offsets == nullptr || m->num_partitions == 0) return m->num_partitions;
-    return find_offset_span(row, m->partition_offsets, m->num_partitions);
+inline void clear(sharded<MatrixT> *m) {
+    if (m->parts != nullptr)
+        for (unsigned long i = 0; i < m->num_partitions; ++i) destroy(m->parts[i]);
+    csamd_sharded_clear(c_sharded(m), 0u);
 }
 template<typename MatrixT>
-inline unsigned long find_shard(const sharded<MatrixT> *m, unsigned long row) {
-    if (m->shard_offsets == nullptr || m->num_shards == 0) return m->num_shards;
-    return find_offset_span(row, m->shard_offsets, m->num_shards);
+inline int reserve_partitions(sharded<MatrixT> *m, unsigned long capacity) { return csamd_sharded_reserve_partitions(c_sharded(m), capacity); }
+template<typename MatrixT>
+inline int reserve_shards(sharded<MatrixT> *m, unsigned long capacity) { return csamd_sharded_reserve_shards(c_sharded(m), capacity); }
+template<typename MatrixT>
+inline void rebuild_shard_parts(sharded<MatrixT> *m) { csamd_sharded_rebuild_shard_parts(c_sharded(m)); }
+template<typename MatrixT>
+inline void rebuild_partition_offsets(sharded<MatrixT> *m) { csamd_sharded_rebuild_partition_offsets(c_sharded(m)); }
+template<typename MatrixT>
+inline int define_partitions(sharded<MatrixT> *m, unsigned long cols, unsigned long num_partitions, const unsigned long *partition_rows,
+                             const unsigned long *partition_nnz, const unsigned long *partition_aux) {
+    clear(m);
+    return csamd_sharded_define_partitions(c_sharded(m), 0u, cols, num_partitions, partition_rows, partition_nnz, partition_aux);
 }
+template<typename MatrixT>
+inline int set_shards_to_partitions(sharded<MatrixT> *m) { return csamd_sharded_set_shards_to_partitions(c_sharded(m)); }
+// takes ownership of `part` (a `new MatrixT`)
+template<typename MatrixT>
+inline int append_partition(sharded<MatrixT> *m, MatrixT *part) {
+    return csamd_sharded_append_partition(c_sharded(m), part_format<MatrixT>::value, part);
+}
+template<typename MatrixT>
+inline int concatenate(sharded<MatrixT> *dst, sharded<MatrixT> *src) { return csamd_sharded_concatenate(c_sharded(dst), c_sharded(src)); }
+template<typename MatrixT>
+inline int set_equal_shards(sharded<MatrixT> *m, unsigned long count) { return csamd_sharded_set_equal_shards(c_sharded(m), count); }
+// *out_offsets is malloc'ed; free() it
+template<typename MatrixT>
+inline int build_shard_offsets_by_rows(const sharded<MatrixT> *m, unsigned long target_rows_per_shard, unsigned long **out_offsets,
+                                       unsigned long *out_count) {
+    return csamd_sharded_build_shard_offsets_by_rows(c_sharded(m), target_rows_per_shard, out_offsets, out_count);
+}
+template<typename MatrixT>
+inline int reshard(sharded<MatrixT> *m, unsigned long count, const unsigned long *offsets) { return csamd_sharded_reshard(c_sharded(m), count, offsets); }
+template<typename MatrixT>
+inline int set_shards_by_nnz(sharded<MatrixT> *m, unsigned long max_nnz) { return csamd_sharded_set_shards_by_nnz(c_sharded(m), max_nnz); }
+template<typename MatrixT>
+inline int set_shards_by_part_bytes(sharded<MatrixT> *m, std::size_t max_bytes) {
+    return csamd_sharded_set_shards_by_part_bytes(c_sharded(m), part_format<MatrixT>::value, max_bytes);
+}
+template<typename MatrixT>
+inline int drop_partition(sharded<MatrixT> *m, unsigned long partId) {
+    if (partId >= m->num_partitions) return 0;
+    destroy(m->parts[partId]);
+    m->parts[partId] = nullptr;
+    return 1;
+}
+template<typename MatrixT>
+inline int drop_all_partitions(sharded<MatrixT> *m) {
+    for (unsigned long i = 0; i < m->num_partitions; ++i)
+        if (!drop_partition(m, i)) return 0;
+    return 1;
+}
+
+template<typename MatrixT>
+inline unsigned long partition_nnz(const MatrixT *m) { return m->nnz; }
+template<typename MatrixT>
+inline unsigned long partition_aux(const MatrixT *m) { return csamd_partition_aux(part_format<MatrixT>::value, m); }
+
+template<typename MatrixT>
+inline unsigned long find_partition(const sharded<MatrixT> *m, unsigned long row) { return csamd_sharded_find_partition(c_sharded(m), row); }
+template<typename MatrixT>
+inline unsigned long find_shard(const sharded<MatrixT> *m, unsigned long row) { return csamd_sharded_find_shard(c_sharded(m), row); }
 template<typename MatrixT>
 inline unsigned long first_row_in_partition(const sharded<MatrixT> *m, unsigned long p) {
     return (p >= m->num_partitions || m->partition_offsets == nullptr) ? m->rows : m->partition_offsets[p];
@@ -766,12 +844,180 @@ inline unsigned long last_row_in_partition(const sharded<MatrixT> *m, unsigned l
     return (p >= m->num_partitions || m->partition_offsets == nullptr) ? m->rows : m->partition_offsets[p + 1];
 }
 template<typename MatrixT>
+inline int row_is_partition_boundary(const sharded<MatrixT> *m, unsigned long row) {
+    return csamd_sharded_row_is_partition_boundary(c_sharded(m), row);
+}
+template<typename MatrixT>
 inline unsigned long first_partition_in_shard(const sharded<MatrixT> *m, unsigned long s) {
-    return (s >= m->num_shards || m->shard_parts == nullptr) ? m->num_partitions : m->shard_parts[s];
+    return csamd_sharded_first_partition_in_shard(c_sharded(m), s);
 }
 template<typename MatrixT>
 inline unsigned long last_partition_in_shard(const sharded<MatrixT> *m, unsigned long s) {
-    return (s >= m->num_shards || m->shard_parts == nullptr) ? m->num_partitions : m->shard_parts[s + 1];
+    return csamd_sharded_last_partition_in_shard(c_sharded(m), s);
 }
+template<typename MatrixT>
+inline unsigned long first_row_in_shard(const sharded<MatrixT> *m, unsigned long s) {
+    return (s >= m->num_shards || m->shard_offsets == nullptr) ? m->rows : m->shard_offsets[s];
+}
+template<typename MatrixT>
+inline unsigned long last_row_in_shard(const sharded<MatrixT> *m, unsigned long s) {
+    return (s >= m->num_shards || m->shard_offsets == nullptr) ? m->rows : m->shard_offsets[s + 1];
+}
+template<typename MatrixT>
+inline unsigned long partition_count_in_shard(const sharded<MatrixT> *m, unsigned long s) {
+    const unsigned long b = first_partition_in_shard(m, s), e = last_partition_in_shard(m, s);
+    return b >= e ? 0ul : e - b;
+}
+template<typename MatrixT>
+inline unsigned long rows_in_shard(const sharded<MatrixT> *m, unsigned long s) {
+    const unsigned long b = first_row_in_shard(m, s), e = last_row_in_shard(m, s);
+    return b >= e ? 0ul : e - b;
+}
+template<typename MatrixT>
+inline int partition_loaded(const sharded<MatrixT> *m, unsigned long p) { return p < m->num_partitions && m->parts[p] != nullptr; }
+template<typename MatrixT>
+inline int shard_loaded(const sharded<MatrixT> *m, unsigned long s) {
+    if (s >= m->num_shards) return 0;
+    for (unsigned long i = first_partition_in_shard(m, s); i < last_partition_in_shard(m, s); ++i)
+        if (!partition_loaded(m, i)) return 0;
+    return 1;
+}
+template<typename MatrixT>
+inline unsigned long nnz_in_shard(const sharded<MatrixT> *m, unsigned long s) {
+    if (s >= m->num_shards) return 0ul;
+    unsigned long t = 0ul;
+    for (unsigned long i = first_partition_in_shard(m, s); i < last_partition_in_shard(m, s); ++i) t += m->partition_nnz[i];
+    return t;
+}
+template<typename MatrixT>
+inline int drop_shard(sharded<MatrixT> *m, unsigned long s) {
+    if (s >= m->num_shards) return 0;
+    for (unsigned long i = first_partition_in_shard(m, s); i < last_partition_in_shard(m, s); ++i)
+        if (!drop_partition(m, i)) return 0;
+    return 1;
+}
+template<typename MatrixT>
+inline std::size_t partition_bytes(const sharded<MatrixT> *m, unsigned long p) {
+    return csamd_sharded_partition_bytes(c_sharded(m), part_format<MatrixT>::value, p);
+}
+template<typename MatrixT>
+inline std::size_t shard_bytes(const sharded<MatrixT> *m, unsigned long s) {
+    return csamd_sharded_shard_bytes(c_sharded(m), part_format<MatrixT>::value, s);
+}
+template<typename MatrixT>
+inline std::size_t bytes(const sharded<MatrixT> *m) { return csamd_sharded_bytes(c_sharded(m), part_format<MatrixT>::value); }
+
+// ---------------------------------------------------------------------------
+// sharded_device<MatrixT>: device residency (runtime/device/sharded_device.cuh:43-50,
+// 119-178, 1098-1722).  Layout identical to csamd_sharded_device.  The
+// reference's `const shard_storage *files` (its .csh5 fetch) is a CSPACK01
+// shard pack here (pack::file, may be null when the host parts are
+// materialized); hipError_t 0 = success, 1 = failure (csamd_last_error()).
+namespace device {
+
+template<typename MatrixT>
+struct sharded_device {
+    unsigned long capacity;
+    partition_record<MatrixT> *parts;
+    unsigned long cache_count;
+    unsigned long cache_capacity;
+    cached_allocation *cache;
+};
+
+template<typename MatrixT>
+inline csamd_sharded_device *c_state(sharded_device<MatrixT> *s) { return reinterpret_cast<csamd_sharded_device *>(s); }
+inline hipError_t status_(int ok) { return ok ? (hipError_t) 0 : (hipError_t) 1; }
+
+template<typename MatrixT>
+inline void init(sharded_device<MatrixT> *s) { csamd_sharded_device_init(c_state(s)); }
+template<typename MatrixT>
+inline void clear(sharded_device<MatrixT> *s) { (void) csamd_sharded_device_clear(c_state(s)); }
+template<typename MatrixT>
+inline int reserve(sharded_device<MatrixT> *s, unsigned long capacity) { return csamd_sharded_device_reserve(c_state(s), capacity); }
+template<typename MatrixT>
+inline std::size_t device_partition_bytes(const sharded<MatrixT> *v, unsigned long p) {
+    return csamd_sharded_device_partition_bytes(c_sharded(v), part_format<MatrixT>::value, p);
+}
+template<typename MatrixT>
+inline std::size_t device_shard_bytes(const sharded<MatrixT> *v, unsigned long s) {
+    return csamd_sharded_device_shard_bytes(c_sharded(v), part_format<MatrixT>::value, s);
+}
+template<typename MatrixT>
+inline int set_shards_by_device_bytes(sharded<MatrixT> *v, std::size_t max_bytes) {
+    return csamd_sharded_set_shards_by_device_bytes(c_sharded(v), part_format<MatrixT>::value, max_bytes);
+}
+template<typename MatrixT>
+inline hipError_t upload_partition(sharded_device<MatrixT> *s, const sharded<MatrixT> *v, unsigned long partId, int deviceId) {
+    return status_(csamd_sharded_device_upload_partition(c_state(s), c_sharded(v), part_format<MatrixT>::value, partId, deviceId, nullptr));
+}
+template<typename MatrixT>
+inline hipError_t upload_partition_async(sharded_device<MatrixT> *s, const sharded<MatrixT> *v, unsigned long partId, int deviceId,
+                                         hipStream_t stream) {
+    return status_(csamd_sharded_device_upload_partition(c_state(s), c_sharded(v), part_format<MatrixT>::value, partId, deviceId,
+                                                         (csamd_stream_t) stream));
+}
+template<typename MatrixT>
+inline hipError_t release_partition(sharded_device<MatrixT> *s, unsigned long partId) {
+    return status_(csamd_sharded_device_release_partition(c_state(s), partId));
+}
+template<typename MatrixT>
+inline hipError_t upload_shard(sharded_device<MatrixT> *s, const sharded<MatrixT> *v, unsigned long shardId, int deviceId) {
+    return status_(csamd_sharded_device_upload_shard(c_state(s), c_sharded(v), part_format<MatrixT>::value, shardId, deviceId, nullptr));
+}
+template<typename MatrixT>
+inline hipError_t upload_shard_async(sharded_device<MatrixT> *s, const sharded<MatrixT> *v, unsigned long shardId, int deviceId,
+                                     hipStream_t stream) {
+    return status_(csamd_sharded_device_upload_shard(c_state(s), c_sharded(v), part_format<MatrixT>::value, shardId, deviceId,
+                                                     (csamd_stream_t) stream));
+}
+template<typename MatrixT>
+inline hipError_t release_shard(sharded_device<MatrixT> *s, const sharded<MatrixT> *v, unsigned long shardId) {
+    return status_(csamd_sharded_device_release_shard(c_state(s), c_sharded(v), shardId));
+}
+template<typename MatrixT>
+inline hipError_t stage_partition_async(sharded_device<MatrixT> *s, sharded<MatrixT> *v, const csamd_cspack_file *files,
+                                        unsigned long partId, int deviceId, hipStream_t stream, int drop_host_after_upload) {
+    if (!csamd_sharded_device_stage_partition(c_state(s), c_sharded(v), part_format<MatrixT>::value, files, partId, deviceId,
+                                              (csamd_stream_t) stream, 0))
+        return (hipError_t) 1;
+    return status_(!drop_host_after_upload || v->parts[partId] == nullptr || drop_partition(v, partId));
+}
+template<typename MatrixT>
+inline hipError_t stage_partition(sharded_device<MatrixT> *s, sharded<MatrixT> *v, const csamd_cspack_file *files, unsigned long partId,
+                                  int deviceId, int drop_host_after_upload) {
+    return stage_partition_async(s, v, files, partId, deviceId, (hipStream_t) nullptr, drop_host_after_upload);
+}
+template<typename MatrixT>
+inline hipError_t stage_shard_async(sharded_device<MatrixT> *s, sharded<MatrixT> *v, const csamd_cspack_file *files, unsigned long shardId,
+                                    int deviceId, hipStream_t stream, int drop_host_after_upload) {
+    if (!csamd_sharded_device_stage_shard(c_state(s), c_sharded(v), part_format<MatrixT>::value, files, shardId, deviceId,
+                                          (csamd_stream_t) stream, 0))
+        return (hipError_t) 1;
+    return status_(!drop_host_after_upload || drop_shard(v, shardId));
+}
+template<typename MatrixT>
+inline hipError_t stage_shard(sharded_device<MatrixT> *s, sharded<MatrixT> *v, const csamd_cspack_file *files, unsigned long shardId,
+                              int deviceId, int drop_host_after_upload) {
+    return stage_shard_async(s, v, files, shardId, deviceId, (hipStream_t) nullptr, drop_host_after_upload);
+}
+// stages the incoming shard before the outgoing one is released (both resident for a moment)
+template<typename MatrixT>
+inline hipError_t swap_shard_async(sharded_device<MatrixT> *s, sharded<MatrixT> *v, const csamd_cspack_file *files, unsigned long outShardId,
+                                   unsigned long inShardId, int deviceId, hipStream_t stream, int drop_host_after_upload,
+                                   int drop_host_after_release) {
+    hipError_t e = stage_shard_async(s, v, files, inShardId, deviceId, stream, drop_host_after_upload);
+    if (e != (hipError_t) 0) return e;
+    e = release_shard(s, v, outShardId);
+    if (e != (hipError_t) 0) return e;
+    return status_(!drop_host_after_release || drop_shard(v, outShardId));
+}
+template<typename MatrixT>
+inline hipError_t swap_shard(sharded_device<MatrixT> *s, sharded<MatrixT> *v, const csamd_cspack_file *files, unsigned long outShardId,
+                             unsigned long inShardId, int deviceId, int drop_host_after_upload, int drop_host_after_release) {
+    return swap_shard_async(s, v, files, outShardId, inShardId, deviceId, (hipStream_t) nullptr, drop_host_after_upload,
+                            drop_host_after_release);
+}
+
+} // namespace device
 
 } // namespace cellshard

@@ -1206,16 +1206,23 @@ typedef struct CSAMD_ALIGN16 csamd_sharded {
 
 /* sharded.cuh:54-69 */
 void csamd_sharded_init(csamd_sharded *m);
-/* sharded_host.cuh:21-36: destroys the parts, frees the metadata, re-inits */
+/* sharded_host.cuh:21-36: destroys the parts (format 0: parts are left to the caller),
+ * frees the metadata, re-inits */
 void csamd_sharded_clear(csamd_sharded *m, unsigned int format);
+/* A zeroed host part struct of `format` (calloc), for parts a sharded matrix will own. */
+void *csamd_host_part_alloc(unsigned int format);
+/* formats/compressed.cuh:122-153 (allocate): rows, cols, nnz and axis set; one malloc block
+ * [ptr | idx | val] (2-byte values), any previous storage released. */
+int csamd_host_compressed_allocate(csamd_host_compressed *m);
 /* sharded_host.cuh:38-83, :85-109 (grow-only; existing entries kept) */
 int csamd_sharded_reserve_partitions(csamd_sharded *m, unsigned long capacity);
 int csamd_sharded_reserve_shards(csamd_sharded *m, unsigned long capacity);
 /* sharded_host.cuh:111-126, :128-141 */
 void csamd_sharded_rebuild_shard_parts(csamd_sharded *m);
 void csamd_sharded_rebuild_partition_offsets(csamd_sharded *m);
-/* sharded_host.cuh:143-167 (rows / nnz / aux arrays may be NULL: zeros); shards = partitions */
-int csamd_sharded_define_partitions(csamd_sharded *m, unsigned long cols, unsigned long num_partitions,
+/* sharded_host.cuh:143-167 (rows / nnz / aux arrays may be NULL: zeros); shards = partitions.
+ * Existing parts are destroyed as csamd_sharded_clear(m, format) does (format 0: left alone). */
+int csamd_sharded_define_partitions(csamd_sharded *m, unsigned int format, unsigned long cols, unsigned long num_partitions,
                                     const unsigned long *partition_rows, const unsigned long *partition_nnz,
                                     const unsigned long *partition_aux);
 /* sharded_host.cuh:169-176 */
@@ -1303,6 +1310,12 @@ int csamd_sharded_device_release_shard(csamd_sharded_device *s, const csamd_shar
 int csamd_sharded_device_stage_shard(csamd_sharded_device *s, csamd_sharded *view, unsigned int format,
                                      const csamd_cspack_file *pack, unsigned long shard, int device, csamd_stream_t stream,
                                      int drop_host_after_upload);
+/* sharded_device.cuh:1555-1629 (stage_partition / _async): keep a partition resident on
+ * `device`, else upload its host part, else (Blocked-ELL) rebuild it from the pack of the shard
+ * holding it as stage_shard does; optionally drop the host part afterwards. */
+int csamd_sharded_device_stage_partition(csamd_sharded_device *s, csamd_sharded *view, unsigned int format,
+                                         const csamd_cspack_file *pack, unsigned long part, int device, csamd_stream_t stream,
+                                         int drop_host_after_upload);
 /* sharded_device.cuh:1676-1722: stage `in_shard`, then release `out_shard` (both resident
  * for a moment) and optionally drop the outgoing shard's host parts */
 int csamd_sharded_device_swap_shard(csamd_sharded_device *s, csamd_sharded *view, unsigned int format,

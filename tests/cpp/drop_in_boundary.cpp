@@ -12,7 +12,8 @@
 // Sliced-ELL matrices store it at column 4, group 2).
 // Part 2 calls the host-struct converters (convert/) on the same matrix and
 // checks them against known answers.
-// Part 3 exercises the sharded<T> metadata and sharded_device<T> staging.
+// Part 3 exercises the sharded<T> metadata (the only part that runs with
+// --host-only, without a device) and part 4 the sharded_device<T> residency.
 //
 // Exit status 0 when every check passes; each failed check is printed.
 #include <cellshard_amd/cellshard.hh>
@@ -270,7 +271,87 @@ static void part2_host_converters() {
     std::printf("part 2 (host converters): done\n");
 }
 
-int main() {
+// ---------------------------------------------------------------------------
+// three CSR parts of 5 columns: 3 rows (5 entries), 0 rows, 2 rows (3 entries)
+static sp::compressed *make_part(unsigned int which) {
+    static const unsigned int p0[4] = {0u, 2u, 2u, 5u}, i0[5] = {0u, 4u, 1u, 2u, 3u};
+    static const unsigned int p1[1] = {0u};
+    static const unsigned int p2[3] = {0u, 1u, 3u}, i2[3] = {2u, 0u, 4u};
+    static const float v0[5] = {1, 2, 3, 4, 5}, v2[3] = {6, 7, 8};
+    sp::compressed *m = new sp::compressed;
+    if (which == 0u) fill_compressed(m, 3u, 5u, sp::compressed_by_row, p0, i0, v0, 5u);
+    else if (which == 1u) fill_compressed(m, 0u, 5u, sp::compressed_by_row, p1, nullptr, nullptr, 0u);
+    else fill_compressed(m, 2u, 5u, sp::compressed_by_row, p2, i2, v2, 3u);
+    return m;
+}
+
+static void part3_sharded_metadata(cellshard::sharded<sp::compressed> *m) {
+    cellshard::init(m);
+    for (unsigned int k = 0; k < 3u; ++k) CHECK(cellshard::append_partition(m, make_part(k)) == 1);
+    CHECK(m->rows == 5u && m->cols == 5u && m->nnz == 8u && m->num_partitions == 3u && m->num_shards == 3u);
+    CHECK(m->partition_offsets[1] == 3u && m->partition_offsets[2] == 3u && m->partition_offsets[3] == 5u);
+    CHECK(cellshard::find_partition(m, 3ul) == 2u && cellshard::find_partition(m, 5ul) == 3u);
+    CHECK(cellshard::partition_aux(m->parts[0]) == sp::compressed_by_row && cellshard::partition_nnz(m->parts[2]) == 3u);
+    CHECK(cellshard::partition_bytes(m, 0ul) == sp::bytes(m->parts[0]));
+    // two shards: rows {0..2} (partition 0) and {3, 4} (the empty partition 1 and partition 2)
+    CHECK(cellshard::set_equal_shards(m, 2ul) == 1);
+    CHECK(m->num_shards == 2u && m->shard_offsets[1] == 3u && cellshard::first_partition_in_shard(m, 1ul) == 1u);
+    CHECK(cellshard::partition_count_in_shard(m, 1ul) == 2u && cellshard::rows_in_shard(m, 1ul) == 2u && cellshard::nnz_in_shard(m, 1ul) == 3u);
+    CHECK(cellshard::shard_bytes(m, 0ul) == sp::bytes(m->parts[0]) && cellshard::shard_loaded(m, 1ul) == 1);
+    unsigned long *offs = nullptr, count = 0ul;
+    CHECK(cellshard::build_shard_offsets_by_rows(m, 2ul, &offs, &count) == 1);
+    CHECK(count == 2u && offs != nullptr && offs[0] == 0u && offs[1] == 3u && offs[2] == 5u);
+    std::free(offs);
+    const unsigned long bad[3] = {0ul, 4ul, 5ul};
+    CHECK(cellshard::reshard(m, 2ul, bad) == 0);  // row 4 is inside partition 2
+    std::printf("part 3 (sharded metadata): done\n");
+}
+
+static void part4_sharded_device(cellshard::sharded<sp::compressed> *m) {
+    dv::sharded_device<sp::compressed> st;
+    dv::init(&st);
+    CHECK(dv::reserve(&st, m->num_partitions) == 1);
+    CHECK(dv::upload_shard(&st, m, 1ul, 0) == hipSuccess);
+    // partitions 1 and 2 share one packed allocation, owned by the first
+    CHECK(st.parts[1].view != nullptr && st.parts[2].view != nullptr && st.parts[0].view == nullptr);
+    CHECK(st.parts[1].allocation != nullptr && st.parts[2].allocation == nullptr && st.parts[2].group_begin == 1u);
+    CHECK(st.parts[1].allocation_bytes <= dv::device_shard_bytes(m, 1ul));
+    dv::compressed_view dview{};
+    CHECK(hipMemcpy(&dview, st.parts[2].view, sizeof(dview), hipMemcpyDeviceToHost) == hipSuccess);
+    CHECK(dview.rows == 2u && dview.nnz == 3u);
+    std::vector<unsigned int> idx = to_host(dview.minorIdx, 3);
+    CHECK(idx[0] == 2u && idx[1] == 0u && idx[2] == 4u);
+    std::vector<__half> val = to_host(dview.val, 3);
+    CHECK(__half2float(val[2]) == 8.0f);
+    // swap: shard 0 in, shard 1 out (its allocation goes to the cache)
+    CHECK(dv::swap_shard(&st, m, (const csamd_cspack_file *) nullptr, 1ul, 0ul, 0, 0, 0) == hipSuccess);
+    CHECK(st.parts[0].view != nullptr && st.parts[1].view == nullptr && st.parts[2].view == nullptr && st.cache_count == 1u);
+    CHECK(hipMemcpy(&dview, st.parts[0].view, sizeof(dview), hipMemcpyDeviceToHost) == hipSuccess);
+    std::vector<unsigned int> p0 = to_host(dview.majorPtr, 4);
+    CHECK(dview.rows == 3u && p0[3] == 5u);
+    // releasing one partition of a packed shard releases the group
+    CHECK(dv::upload_shard(&st, m, 1ul, 0) == hipSuccess && st.cache_count == 0u);
+    CHECK(dv::release_partition(&st, 2ul) == hipSuccess && st.parts[1].view == nullptr && st.cache_count == 1u);
+    // stage with drop: the host parts of shard 0 go away, the device copy stays
+    CHECK(dv::stage_shard(&st, m, (const csamd_cspack_file *) nullptr, 0ul, 0, 1) == hipSuccess);
+    CHECK(m->parts[0] == nullptr && st.parts[0].view != nullptr);
+    // a shard with parts that are not materialized and no pack to read fails cleanly
+    CHECK(dv::release_shard(&st, m, 0ul) == hipSuccess);
+    CHECK(dv::stage_shard(&st, m, (const csamd_cspack_file *) nullptr, 0ul, 0, 0) != hipSuccess);
+    dv::clear(&st);
+    CHECK(st.parts == nullptr && st.cache_count == 0u);
+    std::printf("part 4 (sharded_device residency): done\n");
+}
+
+int main(int argc, char **argv) {
+    const bool host_only = argc > 1 && std::strcmp(argv[1], "--host-only") == 0;
+    cellshard::sharded<sp::compressed> m;
+    if (host_only) {
+        part3_sharded_metadata(&m);
+        cellshard::clear(&m);
+        std::printf("%d checks, %d failed\n", g_checked, g_failed);
+        return g_failed == 0 ? 0 : 1;
+    }
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
         std::fprintf(stderr, "no HIP device\n");
@@ -278,6 +359,9 @@ int main() {
     }
     part1_mask_groups();
     part2_host_converters();
+    part3_sharded_metadata(&m);
+    part4_sharded_device(&m);
+    cellshard::clear(&m);
     std::printf("%d checks, %d failed\n", g_checked, g_failed);
     return g_failed == 0 ? 0 : 1;
 }

@@ -109,3 +109,14 @@ def test_cpp_wrappers_compile(tmp_path, path):
     src = tmp_path / "use.cpp"
     src.write_text(f'#include "{path.split("include/")[1]}"\nint main() {{ return 0; }}\n')
     subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-I", str(ROOT / "include"), str(src)], check=True)
+
+
+def test_cpp_program_host_part_runs_without_a_device():
+    """The sharded<T> metadata part of the C++ drop-in program (compiled
+    against the library, host code only) runs here."""
+    exe = ROOT / "tests" / "cpp" / "bin" / "drop_in_boundary"
+    if not exe.exists():
+        from cellshard_amd import build
+        build.build_cpp_tests()
+    r = subprocess.run([str(exe), "--host-only"], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=60)
+    assert r.returncode == 0 and ", 0 failed" in r.stdout, r.stdout
