@@ -62,6 +62,7 @@ def parse_args():
     ap.add_argument("--skip-cspack", action="store_true", help="skip the CSPACK staging measurement")
     ap.add_argument("--skip-refabi", action="store_true", help="skip the reference-ABI entry point measurement")
     ap.add_argument("--refabi-only", action="store_true", help="only the reference-ABI entry point measurement")
+    ap.add_argument("--skip-refabi-large", action="store_true", help="skip the 1M-row reference-ABI reduce measurement")
     ap.add_argument("--cspack-only", action="store_true", help="only the CSPACK staging measurement")
     return ap.parse_args()
 
@@ -648,8 +649,89 @@ def run_reference_abi(args, log):
     ws.close()
     torch.cuda.empty_cache()
     res_ = {"workload": "reference_abi_csr_20000x33538_fp16_32groups_bell16_sell32", **out}
+    if not args.skip_refabi_large:
+        res_["large"] = run_reference_abi_large(args, log)
     log(f"reference abi: {json.dumps(res_)}")
     return res_
+
+
+def run_reference_abi_large(args, log):
+    """compute_sparse_group_reduce at the single-large-partition shape:
+    1,000,000 x 36,601 fp16 counts (median 2,000 entries per row), 32 feature
+    groups with random masks, row keep 0.8, 3,000 kept features; over the
+    CSR and over Sliced-ELL 32 of its nnz-sorted rebuild.  Algorithmic bytes:
+    kept rows' stored entries (or slots) x 6, row pointers (or slice arrays),
+    row keep bytes, feature keep bytes + group masks, and the outputs (13 +
+    8 G bytes per row)."""
+    import numpy as np
+    import torch
+
+    from cellshard_amd import abi as A
+    from cellshard_amd import runtime as rt
+    from cellshard_amd.device import ptr, to_dev_u32, to_dev_u8
+    from cellshard_amd.ops import (ConvertWorkspace, build_major_nnz_bucket_plan, rebuild_compressed_major_order,
+                                   sliced_ell_from_compressed, synth_csr)
+
+    dev = torch.device("cuda", 0)
+    R, Cc, G, FK = 1_000_000, 36_601, 32, 3000
+    ws = ConvertWorkspace()
+    src = synth_csr(R, Cc, median=2000.0, sigma=0.6, seed=2, value_type=A.VALUE_F16, ws=ws)
+    rng = np.random.default_rng(5)
+    rk = (rng.random(R) < 0.8).astype(np.uint8)
+    fk = np.zeros(Cc, np.uint8)
+    fk[rng.choice(Cc, FK, replace=False)] = 1
+    fgm = rng.integers(0, 2 ** 32, size=Cc, dtype=np.uint64).astype(np.uint32)
+    d_rk, d_fk, d_fgm = to_dev_u8(rk, dev), to_dev_u8(fk, dev), to_dev_u32(fgm, dev)
+    gws = A.sparse_group_reduce_workspace()
+    rt.init(gws)
+    assert rt.setup(gws, 0, torch.cuda.current_stream().cuda_stream) == 1
+    groups = A.group_mask_config_view(group_count=G, feature_group_masks=ptr(d_fgm))
+    masks = A.row_feature_mask_view(row_keep=ptr(d_rk), feature_keep=ptr(d_fk))
+    filt = A.sparse_group_filter_params(0.0, 0, 1.0, 0)
+    res = A.sparse_group_reduce_result()
+
+    def ms(fn, reps=10):
+        assert fn() == 1
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            assert fn() == 1
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / reps
+
+    row_len = np.diff(src.major_ptr.cpu().numpy().view(np.uint32).astype(np.int64))
+    kept = rk == 1
+    extra = R * (1 + 13 + 8 * G) + Cc * (1 + 4)
+    out = {"workload": "reference_abi_group_reduce_csr_1000000x36601_fp16_32groups", "rows": R, "cols": Cc, "groups": G,
+           "kept_features": FK, "row_keep_p": 0.8, "nnz": src.nnz}
+    t = ms(lambda: rt.compute_sparse_group_reduce_compressed_fallback(src.view(), gws, groups, masks, filt, res))
+    b = int(row_len[kept].sum()) * 6 + (R + 1) * 4 + extra
+    out["csr"] = {"ms": round(t, 3), "gb_s": round(b / (t * 1e-3) / 1e9, 1), "pct_of_peak": round(100 * b / (t * 1e-3) / 1e9 / PEAK_HBM_GBS, 1),
+                  "algorithmic_bytes": b}
+    log(f"reference abi large: csr {out['csr']}")
+    plan = build_major_nnz_bucket_plan(src, 12)
+    reb, _inv = rebuild_compressed_major_order(src, plan["major_order"])
+    order = plan["major_order"].cpu().numpy().view(np.uint32)
+    del src, _inv
+    torch.cuda.empty_cache()
+    sell = sliced_ell_from_compressed(reb, 32, ws)
+    del reb
+    torch.cuda.empty_cache()
+    rk_s = rk[order]
+    d_rk_s = to_dev_u8(rk_s, dev)
+    masks_s = A.row_feature_mask_view(row_keep=ptr(d_rk_s), feature_keep=ptr(d_fk))
+    sw = sell.slice_widths.cpu().numpy().view(np.uint32).astype(np.int64)
+    so = sell.slice_row_offsets.cpu().numpy().view(np.uint32).astype(np.int64)
+    row_w = np.repeat(sw, np.diff(so))
+    t = ms(lambda: rt.compute_sparse_group_reduce(sell.view(), gws, groups, masks_s, filt, res))
+    b = int(row_w[rk_s == 1].sum()) * 6 + sell.slice_count * 12 + extra
+    out["sliced_ell_32_nnz_sorted"] = {"ms": round(t, 3), "gb_s": round(b / (t * 1e-3) / 1e9, 1),
+                                       "pct_of_peak": round(100 * b / (t * 1e-3) / 1e9 / PEAK_HBM_GBS, 1), "slots": sell.total_slots}
+    rt.clear(gws)
+    del sell
+    ws.close()
+    torch.cuda.empty_cache()
+    return out
 
 
 def run_batched_partitions(args, log):

@@ -39,6 +39,7 @@ HIP_SOURCES = [
     "column_order.hip",
     "cspack.hip",
     "sharded.hip",
+    "group_reduce.hip",
 ]
 
 HIPCC_FLAGS = [

@@ -8,6 +8,7 @@
 // Work mapping is gfx950-native: one wave64 per row, lane-strided coalesced
 // streams, per-wave group bins in LDS, wave64 butterfly reductions.
 #include "common.hh"
+#include "group_reduce.hh"
 
 #include <cstdlib>
 
@@ -485,10 +486,22 @@ __global__ void mg_pack_keep_kernel(const unsigned char *__restrict__ keep, unsi
     }
 }
 
+// Grow-only workspace scratch (an earlier call's kernels may still read the old block).
+int ensure_scratch(csamd_sparse_group_reduce_workspace *ws, size_t need) {
+    if (need <= ws->scratch_bytes) return 1;
+    CSAMD_HIP(hipStreamSynchronize((hipStream_t) ws->stream), "sparse group scratch sync");
+    if (ws->scratch_block != nullptr) (void) hipFree(ws->scratch_block);
+    ws->scratch_block = nullptr;
+    ws->scratch_bytes = 0u;
+    CSAMD_HIP(hipMalloc(&ws->scratch_block, align_up(need, 4096u)), "hipMalloc sparse group scratch");
+    ws->scratch_bytes = align_up(need, 4096u);
+    return 1;
+}
+
 int prepare_reduce(const void *src, csamd_sparse_group_reduce_workspace *ws, const csamd_group_mask_config_view *groups,
                    const csamd_row_feature_mask_view *masks, const csamd_sparse_group_filter_params *filter,
                    unsigned int rows, unsigned int cols, unsigned long long values, unsigned int value_type,
-                   reduce_prelude *p) {
+                   reduce_prelude *p, bool pack_keep = true) {
     CSAMD_REQUIRE(src != nullptr && ws != nullptr && filter != nullptr, "compute_sparse_group_reduce: null argument");
     CSAMD_REQUIRE(value_type_decodable(value_type), "compute_sparse_group_reduce: unsupported value type %u", value_type);
     p->group_count = groups != nullptr ? groups->group_count : 0u;
@@ -501,17 +514,8 @@ int prepare_reduce(const void *src, csamd_sparse_group_reduce_workspace *ws, con
     p->keep_bits = nullptr;
     p->keep_words = 0u;
     const unsigned int words = (cols + 31u) / 32u;
-    if (p->masks.feature_keep != nullptr && cols != 0u && words <= MG_KEEP_WORDS_MAX && rows != 0u) {
-        const size_t need = (size_t) words * 4u;
-        if (need > ws->scratch_bytes) {
-            // an earlier call's kernels may still read the old block
-            CSAMD_HIP(hipStreamSynchronize((hipStream_t) ws->stream), "sparse group keep bits sync");
-            if (ws->scratch_block != nullptr) (void) hipFree(ws->scratch_block);
-            ws->scratch_block = nullptr;
-            ws->scratch_bytes = 0u;
-            CSAMD_HIP(hipMalloc(&ws->scratch_block, align_up(need, 4096u)), "hipMalloc sparse group keep bits");
-            ws->scratch_bytes = align_up(need, 4096u);
-        }
+    if (pack_keep && p->masks.feature_keep != nullptr && cols != 0u && words <= MG_KEEP_WORDS_MAX && rows != 0u) {
+        if (!ensure_scratch(ws, (size_t) words * 4u)) return 0;
         hipStream_t s = (hipStream_t) ws->stream;
         mg_pack_keep_kernel<<<grid_for(words, 256u, 1024u), 256, 0, s>>>(p->masks.feature_keep, cols, (unsigned int *) ws->scratch_block,
                                                                         words);
@@ -520,6 +524,46 @@ int prepare_reduce(const void *src, csamd_sparse_group_reduce_workspace *ws, con
         p->keep_words = words;
     }
     return 1;
+}
+
+gr_source stream_source(const csamd_compressed_view &v, unsigned int value_type) {
+    gr_source g;
+    std::memset(&g, 0, sizeof(g));
+    g.rows = v.rows;
+    g.cols = v.cols;
+    g.ptr = v.majorPtr;
+    g.idx = v.minorIdx;
+    g.val = v.val;
+    g.value_type = value_type;
+    return g;
+}
+
+gr_source stream_source(const csamd_sliced_ell_view &v, unsigned int value_type) {
+    gr_source g;
+    std::memset(&g, 0, sizeof(g));
+    g.rows = v.rows;
+    g.cols = v.cols;
+    g.slice_count = v.slice_count;
+    g.slice_rows = v.slice_rows;
+    g.slice_offsets = v.slice_row_offsets;
+    g.slice_widths = v.slice_widths;
+    g.slot_offsets = v.slice_slot_offsets;
+    g.idx = v.col_idx;
+    g.val = v.val;
+    g.value_type = value_type;
+    return g;
+}
+
+gr_outputs stream_outputs(const reduce_outputs &o) {
+    return gr_outputs{o.row_totals, o.max_values, o.detected_features, o.row_keep, o.group_counts, o.group_percentages};
+}
+
+// The streaming path over compressed rows / Sliced-ELL (group_reduce.hip).
+int run_stream(const gr_source &gs, csamd_sparse_group_reduce_workspace *ws, const reduce_prelude &p,
+               const csamd_sparse_group_filter_params &filter) {
+    if (!ensure_scratch(ws, gr_scratch_bytes(gs.cols))) return 0;
+    return gr_reduce_rows(gs, p.group_count != 0u ? p.fgm : nullptr, p.group_count, p.masks.row_keep, p.masks.feature_keep, filter,
+                          stream_outputs(outputs_of(ws, p.group_count)), ws->scratch_block, ws->scratch_bytes, (hipStream_t) ws->stream);
 }
 
 template<template<int> class Launch, typename View>
@@ -902,8 +946,12 @@ int csamd_compute_sparse_group_reduce_compressed_typed(const csamd_compressed_vi
     reduce_prelude p;
     CSAMD_REQUIRE(src != nullptr, "compute_sparse_group_reduce_compressed: null src");
     CSAMD_REQUIRE(src->axis == CSAMD_COMPRESSED_BY_ROW, "compute_sparse_group_reduce_compressed: axis must be row");
-    if (!prepare_reduce(src, ws, groups, masks, filter, src->rows, src->cols, src->nnz, value_type, &p)) return 0;
-    if (src->rows != 0u) {
+    const gr_source gs = stream_source(*src, value_type);
+    const bool stream = gr_eligible(gs);
+    if (!prepare_reduce(src, ws, groups, masks, filter, src->rows, src->cols, src->nnz, value_type, &p, !stream)) return 0;
+    if (stream) {
+        if (!run_stream(gs, ws, p, *filter)) return 0;
+    } else if (src->rows != 0u) {
         if (!dispatch_value_type<launch_compressed>(value_type, *src, reduce_grid(src->rows), (hipStream_t) ws->stream, p, *filter,
                                                     outputs_of(ws, p.group_count)))
             return 0;
@@ -928,11 +976,15 @@ int csamd_compute_sparse_group_reduce_sliced_ell_typed(const csamd_sliced_ell_vi
                                                        csamd_sparse_group_reduce_result *out) {
     reduce_prelude p;
     CSAMD_REQUIRE(src != nullptr, "compute_sparse_group_reduce(sliced_ell): null src");
-    if (!prepare_reduce(src, ws, groups, masks, filter, src->rows, src->cols, src->nnz, value_type, &p)) return 0;
     CSAMD_REQUIRE(src->slice_count == 0u || (src->slice_row_offsets != nullptr && src->slice_widths != nullptr &&
                                              src->slice_slot_offsets != nullptr),
                   "compute_sparse_group_reduce(sliced_ell): slice arrays missing");
-    if (src->rows != 0u) {
+    const gr_source gs = stream_source(*src, value_type);
+    const bool stream = gr_eligible(gs);
+    if (!prepare_reduce(src, ws, groups, masks, filter, src->rows, src->cols, src->nnz, value_type, &p, !stream)) return 0;
+    if (stream) {
+        if (!run_stream(gs, ws, p, *filter)) return 0;
+    } else if (src->rows != 0u) {
         if (!dispatch_value_type<launch_sliced>(value_type, *src, reduce_grid(src->rows), (hipStream_t) ws->stream, p, *filter,
                                                 outputs_of(ws, p.group_count)))
             return 0;

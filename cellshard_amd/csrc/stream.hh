@@ -1,0 +1,81 @@
+// Unit loads shared by the row-streaming kernels (label reduce, group reduce).
+//
+// A unit is 4 consecutive entries of a row-major sparse layout: one 16-byte
+// index load and one 4-value load (8 bytes for 2-byte values, 16 for f32, 4
+// for fp8), both non-temporal.  Loaded words stay packed in registers until
+// they are consumed: any instruction that reads them at load time would make
+// the wave wait for the load right there and void the prefetch.
+#pragma once
+
+#include "common.hh"
+
+namespace csamd {
+
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+
+template<int VT> struct unit_vals;
+template<> struct unit_vals<CSAMD_VALUE_F16> {
+    using raw = u32x2;
+    static __device__ __forceinline__ raw vload(const uint16_t *v, unsigned int i0) {
+        return __builtin_nontemporal_load((const u32x2 *) (v + i0));
+    }
+    static __device__ __forceinline__ raw sload(const uint16_t *v, unsigned int i0, unsigned int last) {
+        raw r;
+        r.x = (unsigned int) v[min(i0, last)] | ((unsigned int) v[min(i0 + 1u, last)] << 16);
+        r.y = (unsigned int) v[min(i0 + 2u, last)] | ((unsigned int) v[min(i0 + 3u, last)] << 16);
+        return r;
+    }
+    static __device__ __forceinline__ float get(const raw &r, unsigned int k) {
+        const unsigned int w = k < 2u ? r.x : r.y;
+        return f16_bits_to_float((uint16_t) ((k & 1u) ? (w >> 16) : (w & 0xffffu)));
+    }
+};
+template<> struct unit_vals<CSAMD_VALUE_BF16> {
+    using raw = u32x2;
+    static __device__ __forceinline__ raw vload(const uint16_t *v, unsigned int i0) {
+        return __builtin_nontemporal_load((const u32x2 *) (v + i0));
+    }
+    static __device__ __forceinline__ raw sload(const uint16_t *v, unsigned int i0, unsigned int last) {
+        return unit_vals<CSAMD_VALUE_F16>::sload(v, i0, last);
+    }
+    static __device__ __forceinline__ float get(const raw &r, unsigned int k) {
+        const unsigned int w = k < 2u ? r.x : r.y;
+        return __uint_as_float((k & 1u) ? (w & 0xffff0000u) : (w << 16));
+    }
+};
+template<> struct unit_vals<CSAMD_VALUE_F32> {
+    using raw = u32x4;
+    static __device__ __forceinline__ raw vload(const float *v, unsigned int i0) {
+        return __builtin_nontemporal_load((const u32x4 *) (v + i0));
+    }
+    static __device__ __forceinline__ raw sload(const float *v, unsigned int i0, unsigned int last) {
+        raw r;
+        r.x = __float_as_uint(v[min(i0, last)]);
+        r.y = __float_as_uint(v[min(i0 + 1u, last)]);
+        r.z = __float_as_uint(v[min(i0 + 2u, last)]);
+        r.w = __float_as_uint(v[min(i0 + 3u, last)]);
+        return r;
+    }
+    static __device__ __forceinline__ float get(const raw &r, unsigned int k) {
+        return __uint_as_float(k == 0u ? r.x : k == 1u ? r.y : k == 2u ? r.z : r.w);
+    }
+};
+template<> struct unit_vals<CSAMD_VALUE_FP8_E4M3> {
+    using raw = unsigned int;
+    static __device__ __forceinline__ raw vload(const uint8_t *v, unsigned int i0) {
+        return __builtin_nontemporal_load((const unsigned int *) (v + i0));
+    }
+    static __device__ __forceinline__ raw sload(const uint8_t *v, unsigned int i0, unsigned int last) {
+        return (unsigned int) v[min(i0, last)] | ((unsigned int) v[min(i0 + 1u, last)] << 8) |
+               ((unsigned int) v[min(i0 + 2u, last)] << 16) | ((unsigned int) v[min(i0 + 3u, last)] << 24);
+    }
+    static __device__ __forceinline__ float get(const raw &r, unsigned int k) { return fp8_e4m3_decode_hw(r, (int) k); }
+};
+
+
+__device__ __forceinline__ unsigned int lane_of(const u32x4 &w, unsigned int k) {
+    return k == 0u ? w.x : k == 1u ? w.y : k == 2u ? w.z : w.w;
+}
+
+} // namespace csamd

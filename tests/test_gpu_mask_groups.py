@@ -271,3 +271,97 @@ def test_reoptimize_random_parity(gpu, bs):
     assert np.array_equal(device_to_numpy(out.compressed.minorIdx, n, np.uint32), mi)
     assert np.array_equal(device_to_numpy(out.compressed.val, n, np.uint16), mv)
     csr_rt.clear(rws)
+
+
+def _fast_csr(rows, cols, mean_nnz, seed, vt, long_rows=0, empty=0.05):
+    """Vectorised random CSR (columns may repeat inside a row, as stored
+    entries may): integer counts, some explicit zeros and empty rows."""
+    rng = np.random.default_rng(seed)
+    n = rng.poisson(mean_nnz, rows).astype(np.int64)
+    n[rng.random(rows) < empty] = 0
+    if long_rows:
+        n[rng.choice(rows, long_rows, replace=False)] = rng.integers(3000, 9000, long_rows)
+    ptr_ = np.concatenate([[0], np.cumsum(n)]).astype(np.uint32)
+    idx = rng.integers(0, cols, int(n.sum())).astype(np.uint32)
+    vals = rng.choice([0.0, 1.0, 2.0, 3.0, 4.0, 6.0, 8.0], size=len(idx), p=[0.03, 0.37, 0.2, 0.15, 0.1, 0.1, 0.05])
+    return ptr_, idx, encode(vals, vt)
+
+
+def _sell_ragged(rows, cols, ptr_, idx, bits, seed):
+    """Sliced-ELL with slices of random heights (no uniform height)."""
+    rng = np.random.default_rng(seed)
+    cuts = np.unique(np.concatenate([[0], rng.integers(1, rows, max(rows // 7, 1)), [rows]])).astype(np.uint32)
+    rn = np.diff(ptr_.astype(np.int64))
+    widths = np.array([rn[a:b].max(initial=0) for a, b in zip(cuts[:-1], cuts[1:])], np.uint32)
+    heights = np.diff(cuts.astype(np.int64))
+    slots = np.concatenate([[0], np.cumsum(heights * widths)]).astype(np.int64)
+    col = np.full(int(slots[-1]), 0xFFFFFFFF, np.uint32)
+    val = np.zeros(int(slots[-1]), bits.dtype)
+    for s_, (a, b) in enumerate(zip(cuts[:-1], cuts[1:])):
+        for r in range(a, b):
+            base = slots[s_] + (r - a) * widths[s_]
+            k = int(rn[r])
+            col[base:base + k] = idx[ptr_[r]:ptr_[r + 1]]
+            val[base:base + k] = bits[ptr_[r]:ptr_[r + 1]]
+    return cuts, widths, slots[:-1].astype(np.uint32), col, val
+
+
+STREAM_CASES = {
+    # name: rows, cols, mean nnz, long rows, G, vt, feature keep p (None: no mask), row keep p (None: no mask)
+    "windows": (600_000, 500, 9, 0, 32, F16, 0.6, 0.8),
+    "long_rows": (300, 20_000, 40, 40, 7, BF16, 0.3, 0.9),
+    "tail_fp8": (1001, 300, 5, 0, 1, FP8, 0.7, 0.7),
+    "global_masks": (3000, 40_000, 60, 3, 5, F32, None, 0.9),
+    "no_groups": (5000, 1000, 30, 2, 0, F16, 0.5, None),
+    "all_masked": (700, 100, 10, 0, 3, F16, 0.5, 0.0),
+}
+
+
+@pytest.mark.parametrize("case", sorted(STREAM_CASES))
+def test_streaming_group_reduce_parity(ws, case):
+    """The streaming path (LDS row windows, two-buffer chunk stream, group
+    queue): many windows per workgroup, rows over several chunks, rows in
+    the array's last partial unit, masks of every group count (1 to 32
+    lanes per queued entry), kept-feature tables too large for LDS, no
+    groups, no masks, every row masked; compressed rows and Sliced-ELL with
+    uniform and ragged slices, against the oracle."""
+    rows, cols, mean, longr, G, vt, fkp, rkp = STREAM_CASES[case]
+    ptr_, idx, bits = _fast_csr(rows, cols, mean, seed=hash(case) % 1000, vt=vt, long_rows=longr)
+    if len(idx) % 4 == 0:  # make the array end inside a unit (rows read entry by entry)
+        ptr_[-1] += 1
+        idx = np.append(idx, np.uint32(1)).astype(np.uint32)
+        bits = np.append(bits, encode(np.ones(1), vt)).astype(bits.dtype)
+    rows = len(ptr_) - 1
+    rng = np.random.default_rng(7)
+    rk = None if rkp is None else (rng.random(rows) < rkp).astype(np.uint8)
+    fk = None if fkp is None else (rng.random(cols) < fkp).astype(np.uint8)
+    fgm = rng.integers(0, 2 ** 32, size=cols, dtype=np.uint64).astype(np.uint32)
+    if case == "long_rows":
+        fgm &= np.uint32(0x41)  # sparse masks: at most two groups per feature
+    filt_t = [5.0, 2, 0.5, 0]
+    d_rk = to_dev_u8(rk) if rk is not None else None
+    d_fk = to_dev_u8(fk) if fk is not None else None
+    d_fgm = to_dev_u32(fgm)
+    groups = A.group_mask_config_view(group_count=G, feature_group_masks=ptr(d_fgm) if G else None)
+    masks = A.row_feature_mask_view(row_keep=ptr(d_rk), feature_keep=ptr(d_fk))
+    filt = A.sparse_group_filter_params(*filt_t)
+    res = A.sparse_group_reduce_result()
+    src = DeviceCSR.from_numpy(rows, cols, ptr_, idx, bits, vt)
+    assert csr_rt.compute_sparse_group_reduce_compressed_fallback(src.view(), ws, groups, masks, filt, res, vt) == 1
+    ref = O.group_reduce_csr(rows, cols, ptr_, idx, bits, vt, rk, fk, fgm if G else None, G, filt_t)
+    _compare(fetch(ws, res), ref, G, vt)
+    if rows > 100_000:
+        return
+    for ragged in (False, True):
+        if ragged:
+            offs, wid, slot, col, sval = _sell_ragged(rows, cols, ptr_, idx, bits, seed=rows)
+            sell = DeviceSlicedELL.from_numpy(rows, cols, len(idx), 0, offs, wid, slot, col, sval, vt)
+            sref = O.group_reduce_sell(rows, cols, 0, offs, wid, slot, col, sval, vt, rk, fk, fgm if G else None, G, filt_t)
+        else:
+            s = O.sliced_ell_from_compressed(rows, cols, 0, ptr_, idx, bits, 32)
+            sell = DeviceSlicedELL.from_numpy(rows, cols, len(idx), s["slice_rows"], s["offsets"], s["widths"], s["slot_offsets"],
+                                              s["col_idx"], s["val"], vt)
+            sref = O.group_reduce_sell(rows, cols, s["slice_rows"], s["offsets"], s["widths"], s["slot_offsets"], s["col_idx"],
+                                       s["val"], vt, rk, fk, fgm if G else None, G, filt_t)
+        assert csr_rt.compute_sparse_group_reduce(sell.view(), ws, groups, masks, filt, res, vt) == 1
+        _compare(fetch(ws, res), sref, G, vt)
