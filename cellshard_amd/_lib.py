@@ -40,6 +40,8 @@ PROTOTYPES = {
         P(A.group_mask_config_view), P(A.row_feature_mask_view), P(A.sparse_group_filter_params), P(A.sparse_group_reduce_result)]),
     "csamd_compute_sparse_group_reduce_compressed_typed": (i32, [P(A.compressed_view), u32, P(A.sparse_group_reduce_workspace),
         P(A.group_mask_config_view), P(A.row_feature_mask_view), P(A.sparse_group_filter_params), P(A.sparse_group_reduce_result)]),
+    "csamd_compute_sparse_group_reduce_compressed_noalloc": (i32, [P(A.compressed_view), u32, P(A.group_mask_config_view),
+        P(A.row_feature_mask_view), P(A.sparse_group_filter_params), P(A.sparse_group_reduce_result), vp]),
     "csamd_compute_sparse_group_reduce_blocked_ell_typed": (i32, [P(A.blocked_ell_view), u32, P(A.sparse_group_reduce_workspace),
         P(A.group_mask_config_view), P(A.row_feature_mask_view), P(A.sparse_group_filter_params), P(A.sparse_group_reduce_result)]),
     "csamd_compute_sparse_group_reduce_sliced_ell_typed": (i32, [P(A.sliced_ell_view), u32, P(A.sparse_group_reduce_workspace),

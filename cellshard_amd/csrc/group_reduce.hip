@@ -35,6 +35,8 @@
 //     row's last chunk is consumed; the outputs are written then.
 // Rows whose last unit would pass the array's last whole 4-entry unit (at
 // most three per matrix) are read entry by entry.
+#include <mutex>
+
 #include "group_reduce.hh"
 #include "stream.hh"
 
@@ -721,4 +723,89 @@ int gr_reduce_rows(const gr_source &src, const uint32_t *fgm, unsigned int group
     }
 }
 
+// ---------------------------------------------------------------------------
+// Allocation-free entry point: the streaming path's scratch (workgroup row
+// bounds, feature map, kept-feature group masks) in one fixed device array.
+// ---------------------------------------------------------------------------
+constexpr size_t GR_NOALLOC_SCRATCH = 512u * 1024u;  // every source of up to 122,303 columns
+constexpr int GR_NOALLOC_DEVICES = 64;
+
+// Each GPU the library runs on holds its own copy until the process ends.
+// Host code never passes this name to a kernel: it is a host-side stand-in;
+// the device address comes from hipGetSymbolAddress, per device.
+static __device__ __align__(256) unsigned char noalloc_scratch[GR_NOALLOC_SCRATCH];
+
+namespace {
+std::mutex noalloc_mu;
+void *noalloc_ptr[GR_NOALLOC_DEVICES];
+
+// Device address of the current device's noalloc_scratch, looked up on the
+// first call on that device (with it current) and kept.
+int noalloc_scratch_address(void **out) {
+    int dev = -1;
+    CSAMD_HIP(hipGetDevice(&dev), "hipGetDevice (allocation-free group reduce)");
+    CSAMD_REQUIRE(dev >= 0 && dev < GR_NOALLOC_DEVICES, "allocation-free group reduce: device %d outside [0, %d)", dev,
+                  GR_NOALLOC_DEVICES);
+    std::lock_guard<std::mutex> lock(noalloc_mu);
+    if (noalloc_ptr[dev] == nullptr) {
+        void *p = nullptr;
+        CSAMD_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(noalloc_scratch)), "hipGetSymbolAddress (allocation-free group reduce scratch)");
+        CSAMD_REQUIRE(p != nullptr, "allocation-free group reduce: no device address for the scratch array on device %d", dev);
+        noalloc_ptr[dev] = p;
+    }
+    *out = noalloc_ptr[dev];
+    return 1;
+}
+} // namespace
+
 } // namespace csamd
+
+using namespace csamd;
+
+extern "C" int csamd_compute_sparse_group_reduce_compressed_noalloc(const csamd_compressed_view *src, unsigned int value_type,
+                                                                    const csamd_group_mask_config_view *groups,
+                                                                    const csamd_row_feature_mask_view *masks,
+                                                                    const csamd_sparse_group_filter_params *filter,
+                                                                    const csamd_sparse_group_reduce_result *out,
+                                                                    csamd_stream_t stream) {
+    CSAMD_REQUIRE(src != nullptr && filter != nullptr && out != nullptr,
+                  "compute_sparse_group_reduce_compressed_noalloc: null argument");
+    CSAMD_REQUIRE(src->axis == CSAMD_COMPRESSED_BY_ROW, "compute_sparse_group_reduce_compressed_noalloc: axis must be row");
+    CSAMD_REQUIRE(value_type_decodable(value_type), "compute_sparse_group_reduce_compressed_noalloc: unsupported value type %u",
+                  value_type);
+    const unsigned int G = groups != nullptr ? groups->group_count : 0u;
+    CSAMD_REQUIRE(G <= CSAMD_MAX_MASK_GROUPS, "compute_sparse_group_reduce_compressed_noalloc: group_count %u > %u", G,
+                  CSAMD_MAX_MASK_GROUPS);
+    const uint32_t *fgm = groups != nullptr ? groups->feature_group_masks : nullptr;
+    CSAMD_REQUIRE(G == 0u || fgm != nullptr,
+                  "compute_sparse_group_reduce_compressed_noalloc: %u groups need device feature_group_masks", G);
+    CSAMD_REQUIRE(out->rows >= src->rows && out->group_count == G,
+                  "compute_sparse_group_reduce_compressed_noalloc: outputs hold %u rows x %u groups, the call needs %u x %u",
+                  out->rows, out->group_count, src->rows, G);
+    CSAMD_REQUIRE(out->row_totals != nullptr && out->max_values != nullptr && out->detected_features != nullptr &&
+                      out->row_keep != nullptr && (G == 0u || (out->group_counts != nullptr && out->group_percentages != nullptr)),
+                  "compute_sparse_group_reduce_compressed_noalloc: null output array");
+    if (src->rows == 0u) return 1;
+    gr_source gs;
+    std::memset(&gs, 0, sizeof(gs));
+    gs.rows = src->rows;
+    gs.cols = src->cols;
+    gs.ptr = src->majorPtr;
+    gs.idx = src->minorIdx;
+    gs.val = src->val;
+    gs.value_type = value_type;
+    CSAMD_REQUIRE(gr_eligible(gs),
+                  "compute_sparse_group_reduce_compressed_noalloc: source not taken by the streaming path (more than %u columns, "
+                  "or index / value arrays not aligned to 4-entry units)",
+                  GR_MAX_WORDS * 32u);
+    const size_t need = gr_scratch_bytes(src->cols);
+    CSAMD_REQUIRE(need <= GR_NOALLOC_SCRATCH,
+                  "compute_sparse_group_reduce_compressed_noalloc: %u columns need %zu scratch bytes, above the fixed %zu",
+                  src->cols, need, GR_NOALLOC_SCRATCH);
+    void *scratch = nullptr;
+    if (!noalloc_scratch_address(&scratch)) return 0;
+    const gr_outputs o{out->row_totals, out->max_values, out->detected_features, out->row_keep, G != 0u ? out->group_counts : nullptr,
+                       G != 0u ? out->group_percentages : nullptr};
+    return gr_reduce_rows(gs, fgm, G, masks != nullptr ? masks->row_keep : nullptr, masks != nullptr ? masks->feature_keep : nullptr,
+                          *filter, o, scratch, GR_NOALLOC_SCRATCH, (hipStream_t) stream);
+}

@@ -108,6 +108,16 @@ def compute_sparse_group_reduce_compressed_fallback(src, workspace, groups, mask
     return lib.csamd_compute_sparse_group_reduce_compressed_typed(C.byref(src), value_type, *args)
 
 
+def compute_sparse_group_reduce_compressed_noalloc(src, value_type: int, groups, masks, filter_, out, stream) -> int:
+    """compute_sparse_group_reduce over compressed rows into caller-owned
+    device outputs (`out` describes them), allocating nothing: the scratch
+    is a fixed per-GPU device array.  Returns 1, or 0 with the error set
+    (more than 122,303 columns, unaligned arrays, mismatched outputs)."""
+    lib = load()
+    return lib.csamd_compute_sparse_group_reduce_compressed_noalloc(C.byref(src), value_type, _ref(groups), _ref(masks),
+                                                                    _ref(filter_), C.byref(out), stream)
+
+
 def compute_sparse_group_reduce_fleet(src_by_slot, fleet, groups, masks, filter_, out) -> int:
     lib = load()
     first = src_by_slot[0]

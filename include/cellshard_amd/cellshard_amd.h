@@ -290,6 +290,27 @@ int csamd_compute_sparse_group_reduce_compressed_typed(const csamd_compressed_vi
                                                        const csamd_row_feature_mask_view *masks,
                                                        const csamd_sparse_group_filter_params *filter,
                                                        csamd_sparse_group_reduce_result *out);
+/* No counterpart in the reference.  compute_sparse_group_reduce over
+ * compressed rows for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc): the same
+ * streaming kernels as csamd_compute_sparse_group_reduce_compressed_typed,
+ * with their scratch in a fixed 512 KiB device array per GPU instead of a
+ * workspace, so that no call allocates or frees anything.
+ *   - out: caller-owned device arrays for out->rows >= src->rows rows and
+ *     out->group_count == groups->group_count groups (row-major
+ *     group_counts / group_percentages); written, never rebound.
+ *   - groups->feature_group_masks must be a device array when groups are
+ *     asked for (there is no workspace to hold uploaded masks).
+ *   - refused with an error: more than 122,303 columns (the fixed scratch),
+ *     or a source the streaming path does not take (index / value arrays not
+ *     aligned to 4-entry units).
+ *   - enqueued on `stream` with the calling thread's current device; calls
+ *     that overlap on one GPU share its scratch array and race. */
+int csamd_compute_sparse_group_reduce_compressed_noalloc(const csamd_compressed_view *src, unsigned int value_type,
+                                                         const csamd_group_mask_config_view *groups,
+                                                         const csamd_row_feature_mask_view *masks,
+                                                         const csamd_sparse_group_filter_params *filter,
+                                                         const csamd_sparse_group_reduce_result *out, csamd_stream_t stream);
 int csamd_compute_sparse_group_reduce_blocked_ell_typed(const csamd_blocked_ell_view *src, unsigned int value_type,
                                                         csamd_sparse_group_reduce_workspace *workspace,
                                                         const csamd_group_mask_config_view *groups,
