@@ -555,6 +555,22 @@ def run_cspack(args, log):
     return res
 
 
+def calibrate_copy(log):
+    """A plain device copy of known size for counter calibration: 2 GiB read
+    and 2 GiB written per launch, twice."""
+    import torch
+
+    dev = torch.device("cuda", 0)
+    a_ = torch.empty(1 << 31, dtype=torch.uint8, device=dev).fill_(1)
+    b_ = torch.empty_like(a_)
+    for _ in range(2):
+        b_.copy_(a_)
+    torch.cuda.synchronize()
+    del a_, b_
+    torch.cuda.empty_cache()
+    log("copy calibration: 2 x (2 GiB read + 2 GiB write)")
+
+
 def run_reference_abi(args, log):
     """The reference-ABI entry points on the plain-path shape: CSR 20,000 x
     33,538 with fp16 values (the reference's storage type), its Blocked-ELL 16
@@ -912,6 +928,8 @@ def main() -> int:
         return 0
     if args.refabi_only:
         torch.cuda.set_device(0)
+        if args.calibrate_copy:
+            calibrate_copy(log)
         print(json.dumps({"reference_abi": run_reference_abi(args, log)}), flush=True)
         return 0
     if args.cspack_only:
@@ -940,15 +958,7 @@ def main() -> int:
     log(f"device {props.name} CUs={props.multi_processor_count}")
 
     if args.calibrate_copy:
-        # plain copy of known size: 2 GiB read + 2 GiB written per launch
-        a_ = torch.empty(1 << 31, dtype=torch.uint8, device=dev).fill_(1)
-        b_ = torch.empty_like(a_)
-        for _ in range(2):
-            b_.copy_(a_)
-        torch.cuda.synchronize()
-        del a_, b_
-        torch.cuda.empty_cache()
-        log("copy calibration: 2 x (2 GiB read + 2 GiB write)")
+        calibrate_copy(log)
 
     # plain device copy of 2 GiB (read + write), for scale: what a trivial
     # stream reaches on this box

@@ -139,6 +139,22 @@ inline int value_type_decodable(unsigned int vt) {
 }
 
 // ---------------------------------------------------------------------------
+// Global-memory views of generic pointers.  Accesses through a generic
+// pointer compile to flat_* instructions; a flat access may touch LDS, so the
+// compiler counts it on both the vector-memory and the LDS counters and then
+// waits for every outstanding load (vmcnt(0) lgkmcnt(0)) wherever it needs
+// one of them, which voids any prefetch.  Through these views the same
+// accesses are global_* and the waits count only what they need.
+// ---------------------------------------------------------------------------
+#define CSAMD_GLOBAL __attribute__((address_space(1)))
+template<typename T> __device__ __forceinline__ CSAMD_GLOBAL T *gmem(T *p) {
+    return (CSAMD_GLOBAL T *) p;
+}
+template<typename T> __device__ __forceinline__ const CSAMD_GLOBAL T *gmem(const T *p) {
+    return (const CSAMD_GLOBAL T *) p;
+}
+
+// ---------------------------------------------------------------------------
 // Wave64 helpers.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ unsigned int lane_id() { return __lane_id(); }

@@ -54,6 +54,7 @@ constexpr unsigned int GR_QPAD = 128u;                 // zero entries after the
 constexpr unsigned int GR_MAX_WORDS = 6144u;           // feature map words (196,608 columns)
 constexpr unsigned int GR_MAX_GRID = 1024u;
 constexpr unsigned int GR_TAIL = 4u;                   // rows read entry by entry, per window
+constexpr unsigned int GR_ROW_COST = 384u;             // per-row cost in entries, for the workgroup ranges
 constexpr size_t GR_LDS_BUDGET = 160u * 1024u - 1024u;
 
 struct gr_params {
@@ -69,14 +70,18 @@ struct gr_params {
     unsigned int logP;        // P = 2^logP lanes per queued entry: P >= G, or 16 (two groups per lane) for G > 16
     csamd_sparse_group_filter_params filter;
     gr_outputs o;
-    const unsigned int *bounds;  // [gridDim.x + 1]
+    const unsigned int *bounds;  // [gridDim.x + 1] (compressed rows / Sliced-ELL)
+    // Blocked-ELL: block columns [row_blocks x width], block size 2^lbs, values [rows x ell_cols]
+    const unsigned int *bcol;
+    unsigned int lbs, ell_cols, width;
+    unsigned int nrb;  // row blocks per window (block columns staged in LDS)
 };
 
 // Entry range [b, e) of row r: compressed rows, or the row's slots of its slice.
 __device__ __forceinline__ void gr_range(const gr_source &s, unsigned int r, unsigned int &b, unsigned int &e) {
     if (s.ptr != nullptr) {
-        b = s.ptr[r];
-        e = s.ptr[r + 1u];
+        b = gmem(s.ptr)[r];
+        e = gmem(s.ptr)[r + 1u];
         return;
     }
     if (s.slice_count == 0u) {
@@ -91,13 +96,13 @@ __device__ __forceinline__ void gr_range(const gr_source &s, unsigned int r, uns
         unsigned int lo = 0u, hi = s.slice_count - 1u;
         while (lo < hi) {
             const unsigned int mid = (lo + hi + 1u) >> 1;
-            if (s.slice_offsets[mid] <= r) lo = mid;
+            if (gmem(s.slice_offsets)[mid] <= r) lo = mid;
             else hi = mid - 1u;
         }
         sl = lo;
     }
-    const unsigned int w = s.slice_widths[sl];
-    b = s.slot_offsets[sl] + (r - s.slice_offsets[sl]) * w;
+    const unsigned int w = gmem(s.slice_widths)[sl];
+    b = gmem(s.slot_offsets)[sl] + (r - gmem(s.slice_offsets)[sl]) * w;
     e = b + w;
 }
 
@@ -146,13 +151,16 @@ __global__ void gr_ktab_kernel(const uint2 *__restrict__ fmap, const uint32_t *_
     }
 }
 
-// Workgroup row ranges with about the same number of entries: bounds[k] =
-// the first row starting at or after T_k = B0 + k (E - B0) / grid, where B0 /
-// E are the first row's start and the last row's end: row r + 1 when T_k
-// lies in (start of r, start of r + 1], the start of row `rows` taken as E.
-// The host fills bounds with `rows` first, so a target no row takes (a
-// layout whose row starts do not ascend) leaves an empty range behind, and
-// every bound stays within [0, rows].
+// Workgroup row ranges of about the same cost, a row costing its entries
+// plus GR_ROW_COST (its share of a partly filled chunk, the row's window
+// bookkeeping and its folds and writes), so that ranges of short rows are
+// not overloaded.  With position p(r) = start(r) - B0 + r GR_ROW_COST (B0:
+// the first row's start; strictly ascending when row starts ascend) and W =
+// E - B0 + rows GR_ROW_COST (E: the last row's end), bounds[k] = the first
+// row whose position is at or after T_k = floor(k W / grid): row r + 1 when
+// T_k lies in (p(r), p(r + 1)].  The host fills bounds with `rows` first, so
+// a target no row takes (row starts that do not ascend) leaves an empty
+// range behind, and every bound stays within [0, rows].
 __global__ void gr_bounds_kernel(gr_source s, unsigned int grid, unsigned int *__restrict__ bounds) {
     __shared__ unsigned int sB0, sE;
     if (threadIdx.x == 0u) {
@@ -163,30 +171,29 @@ __global__ void gr_bounds_kernel(gr_source s, unsigned int grid, unsigned int *_
         sE = e;
     }
     __syncthreads();
-    const unsigned long long B0 = sB0, W = sE > sB0 ? (unsigned long long) sE - sB0 : 0ull;
+    const unsigned long long B0 = sB0;
+    const unsigned long long W = (sE > sB0 ? (unsigned long long) sE - sB0 : 0ull) + (unsigned long long) s.rows * GR_ROW_COST;
     const unsigned int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r == 0u) {
         bounds[0] = 0u;
         bounds[grid] = s.rows;
-        // targets at B0 itself (fewer entries than workgroups) start at row 0
+        // targets at position 0 (W < grid) start at row 0
         for (unsigned int k = 1u; k < grid && (unsigned long long) k * W < grid; ++k) bounds[k] = 0u;
-    }
-    if (W == 0ull) {  // no entries anywhere: every row to the first workgroup
-        if (r >= 1u && r < grid) bounds[r] = s.rows;
-        return;
     }
     if (r >= s.rows) return;
     unsigned int b, e, b1 = sE;
     gr_range(s, r, b, e);
     if (r + 1u < s.rows) gr_range(s, r + 1u, b1, e);
-    if (b1 <= b || b < sB0) return;
-    // every k with b < T_k <= b1, T_k = B0 + floor(k W / grid), starts at row r + 1
-    unsigned int k = (unsigned int) (((unsigned long long) (b - sB0) * grid) / W);
+    if (b1 < b || b < B0) return;
+    const unsigned long long p0 = (unsigned long long) b - B0 + (unsigned long long) r * GR_ROW_COST;
+    const unsigned long long p1 = (unsigned long long) b1 - B0 + (unsigned long long) (r + 1u) * GR_ROW_COST;
+    // every k with p0 < T_k <= p1 starts at row r + 1 (T_0 = 0 is row 0's)
+    unsigned int k = (unsigned int) ((p0 * grid) / W);
     if (k < 1u) k = 1u;
     for (; k < grid; ++k) {
-        const unsigned long long T = B0 + (unsigned long long) k * W / grid;
-        if (T > b1) break;
-        if (T > b) bounds[k] = r + 1u;
+        const unsigned long long T = (unsigned long long) k * W / grid;
+        if (T > p1) break;
+        if (T > p0) bounds[k] = r + 1u;
     }
 }
 
@@ -222,23 +229,23 @@ __device__ __forceinline__ void gr_write_row(const gr_params &a, const gr_sink &
     const unsigned int lane = lane_id();
     const gr_outputs &o = k.o;
     if (lane == 0u) {
-        o.row_totals[row] = rs;
-        o.max_values[row] = mx;
-        o.detected_features[row] = det;
+        gmem(o.row_totals)[row] = rs;
+        gmem(o.max_values)[row] = mx;
+        gmem(o.detected_features)[row] = det;
         const float frac = rs > 0.0f ? gsel / rs : 0.0f;
-        o.row_keep[row] = (unsigned char) (kept_row && rs >= k.filter.min_total && det >= k.filter.min_detected_features &&
-                                           frac <= k.filter.max_group_fraction);
+        gmem(o.row_keep)[row] = (unsigned char) (kept_row && rs >= k.filter.min_total && det >= k.filter.min_detected_features &&
+                                                 frac <= k.filter.max_group_fraction);
     }
     if (o.group_counts != nullptr) {
         const unsigned long long base = (unsigned long long) row * a.G;
         const unsigned int P = 1u << a.logP;
         if (lane < P && lane < a.G) {
-            o.group_counts[base + lane] = g0;
-            o.group_percentages[base + lane] = rs > 0.0f ? 100.0f * g0 / rs : 0.0f;
+            gmem(o.group_counts)[base + lane] = g0;
+            gmem(o.group_percentages)[base + lane] = rs > 0.0f ? 100.0f * g0 / rs : 0.0f;
         }
         if (lane < P && lane + 16u < a.G) {
-            o.group_counts[base + lane + 16u] = g1;
-            o.group_percentages[base + lane + 16u] = rs > 0.0f ? 100.0f * g1 / rs : 0.0f;
+            gmem(o.group_counts)[base + lane + 16u] = g1;
+            gmem(o.group_percentages)[base + lane + 16u] = rs > 0.0f ? 100.0f * g1 / rs : 0.0f;
         }
     }
 }
@@ -247,16 +254,16 @@ __device__ __forceinline__ void gr_write_row(const gr_params &a, const gr_sink &
 // emit_empty_row) or kept with no stored entry (keep from the filter).
 __device__ __forceinline__ void gr_write_empty(const gr_params &a, const gr_sink &k, unsigned int row, bool kept_row) {
     const gr_outputs &o = k.o;
-    o.row_totals[row] = 0.0f;
-    o.max_values[row] = 0.0f;
-    o.detected_features[row] = 0u;
-    o.row_keep[row] = (unsigned char) (kept_row && 0.0f >= k.filter.min_total && 0u >= k.filter.min_detected_features &&
-                                       0.0f <= k.filter.max_group_fraction);
+    gmem(o.row_totals)[row] = 0.0f;
+    gmem(o.max_values)[row] = 0.0f;
+    gmem(o.detected_features)[row] = 0u;
+    gmem(o.row_keep)[row] = (unsigned char) (kept_row && 0.0f >= k.filter.min_total && 0u >= k.filter.min_detected_features &&
+                                             0.0f <= k.filter.max_group_fraction);
     if (o.group_counts != nullptr) {
         const unsigned long long base = (unsigned long long) row * a.G;
         for (unsigned int g = 0; g < a.G; ++g) {
-            o.group_counts[base + g] = 0.0f;
-            o.group_percentages[base + g] = 0.0f;
+            gmem(o.group_counts)[base + g] = 0.0f;
+            gmem(o.group_percentages)[base + g] = 0.0f;
         }
     }
 }
@@ -275,8 +282,8 @@ __device__ __forceinline__ float gr_bit(uint32_t m, unsigned int b, float v) {
     return __uint_as_float(__float_as_uint(v) & (uint32_t) __builtin_amdgcn_sbfe((int) m, b, 1u));
 }
 
-// K entries per lane (c = CSAMD_INVALID_COL for none).  A stored entry is
-// kept when its column's feature is kept and its value is not zero.
+// K entries per lane, kept[k] when entry k's column is kept and its value is
+// not zero (c[k] < cols then).
 //   * G = 0: the kept entries add to the lane's row partials directly.
 //   * G > 0: the wave's kept entries go to its LDS queue as {value, column}
 //     (ballot + mbcnt); pass 1 takes them one per lane, adds them to the row
@@ -288,14 +295,8 @@ __device__ __forceinline__ float gr_bit(uint32_t m, unsigned int b, float v) {
 // The lookups and group work thus scale with the kept entries, not the
 // stored ones.
 template<unsigned int K, unsigned int TB>
-__device__ __forceinline__ void gr_take(const gr_params &a, const gr_lds &L, const unsigned int (&c)[K], const float (&v)[K],
-                                        gr_acc &A) {
-    unsigned int pw[K];
-#pragma unroll
-    for (unsigned int k = 0; k < K; ++k) pw[k] = L.fmap[min(c[k] >> 5, a.words)].x;
-    bool kept[K];
-#pragma unroll
-    for (unsigned int k = 0; k < K; ++k) kept[k] = __builtin_amdgcn_ubfe(pw[k], c[k] & 31u, 1u) != 0u && v[k] != 0.0f;
+__device__ __forceinline__ void gr_accum(const gr_params &a, const gr_lds &L, const unsigned int (&c)[K], const float (&v)[K],
+                                         const bool (&kept)[K], gr_acc &A) {
     if (a.G == 0u) {
 #pragma unroll
         for (unsigned int k = 0; k < K; ++k) {
@@ -333,7 +334,7 @@ __device__ __forceinline__ void gr_take(const gr_params &a, const gr_lds &L, con
             A.sum += x;
             A.vmax = x > A.vmax ? x : A.vmax;
             if constexpr (TB == 0u) {
-                m = a.fgm[col] & a.gbits;
+                m = gmem(a.fgm)[col] & a.gbits;
             } else {
                 const uint2 pm = L.fmap[col >> 5];
                 m = gr_tab<TB>(L.ktab, pm.y + (unsigned int) __popc(pm.x & ((1u << (col & 31u)) - 1u)));
@@ -363,6 +364,20 @@ __device__ __forceinline__ void gr_take(const gr_params &a, const gr_lds &L, con
         }
     }
     __builtin_amdgcn_wave_barrier();
+}
+
+// K entries per lane with their columns: the feature test from the LDS
+// feature map, then gr_accum.
+template<unsigned int K, unsigned int TB>
+__device__ __forceinline__ void gr_take(const gr_params &a, const gr_lds &L, const unsigned int (&c)[K], const float (&v)[K],
+                                        gr_acc &A) {
+    unsigned int pw[K];
+#pragma unroll
+    for (unsigned int k = 0; k < K; ++k) pw[k] = L.fmap[min(c[k] >> 5, a.words)].x;
+    bool kept[K];
+#pragma unroll
+    for (unsigned int k = 0; k < K; ++k) kept[k] = __builtin_amdgcn_ubfe(pw[k], c[k] & 31u, 1u) != 0u && v[k] != 0.0f;
+    gr_accum<K, TB>(a, L, c, v, kept, A);
 }
 
 __device__ __forceinline__ void gr_finish(const gr_params &a, const gr_sink &k, unsigned int row, gr_acc &A) {
@@ -404,7 +419,7 @@ __device__ __forceinline__ void gr_load(const gr_params &a, const gr_cursor &c, 
 #pragma unroll
     for (unsigned int j = 0; j < GR_UNITS; ++j) {
         const unsigned int uu = min(c.u + lane + 64u * j, c.ue - 1u);
-        R.c[j] = __builtin_nontemporal_load((const u32x4 *) (a.src.idx + (size_t) uu * 4u));
+        R.c[j] = __builtin_nontemporal_load(gmem((const u32x4 *) (a.src.idx + (size_t) uu * 4u)));
         R.v[j] = unit_vals<VT>::vload(val, uu * 4u);
     }
 }
@@ -475,7 +490,7 @@ __device__ __forceinline__ void gr_stream_rows(const gr_params &a, const gr_lds 
             rr[h] = r;
             rb[h] = re[h] = 0u;
             if (r < w1) {
-                const bool kept = a.row_keep == nullptr || a.row_keep[r] != 0u;
+                const bool kept = a.row_keep == nullptr || gmem(a.row_keep)[r] != 0u;
                 gr_range(a.src, r, rb[h], re[h]);
                 if (!kept || rb[h] == re[h]) {
                     gr_write_empty(a, sink, r, kept);
@@ -565,13 +580,44 @@ __device__ __forceinline__ void gr_stream_rows(const gr_params &a, const gr_lds 
             const S *val = (const S *) a.src.val;
             for (unsigned int i0 = b; i0 < e; i0 += 64u) {
                 const unsigned int i = i0 + lane;
-                unsigned int col[1] = {i < e ? a.src.idx[i] : CSAMD_INVALID_COL};
-                float v[1] = {i < e ? value_traits<VT>::decode(val[i]) : 0.0f};
+                unsigned int col[1] = {i < e ? gmem(a.src.idx)[i] : CSAMD_INVALID_COL};
+                float v[1] = {i < e ? value_traits<VT>::decode(gmem(val)[i]) : 0.0f};
                 gr_take<1u, TB>(a, L, col, v, A);
             }
             gr_finish(a, sink, row, A);
         }
     }
+}
+
+// LDS of one workgroup: the feature map, the kept features' group masks when
+// TB != 0 and they fit (F <= cap), the waves' queues, the window lists.
+struct gr_staged {
+    gr_lds L;
+    unsigned int *wrow;
+    bool in_lds;
+};
+
+template<unsigned int TB>
+__device__ __forceinline__ gr_staged gr_stage(const gr_params &a, unsigned char *lds, gr_sink &sink) {
+    const unsigned int tid = threadIdx.x, wave = tid >> 6;
+    uint2 *fm = (uint2 *) lds;
+    unsigned char *kt = lds + align_up((size_t) (a.words + 1u) * 8u, 16u);
+    uint2 *queues = (uint2 *) (kt + align_up((size_t) a.cap * (TB != 0u ? TB : 1u), 16u));
+    unsigned int *wrow = (unsigned int *) (queues + GR_WAVES * (GR_QCAP + GR_QPAD));
+    if (tid == 0u) {
+        sink.o = a.o;
+        sink.filter = a.filter;
+    }
+    for (unsigned int w = tid; w <= a.words; w += GR_THREADS) fm[w] = a.fmap[w];
+    const unsigned int F = __builtin_amdgcn_readfirstlane(gmem((const unsigned int *) a.fmap)[2u * a.words + 1u]);
+    const bool in_lds = TB != 0u && F <= a.cap;
+    for (unsigned int k = tid; k < F && in_lds; k += GR_THREADS) {
+        const uint32_t m = gmem(a.ktab)[k];
+        if (TB == 1u) ((uint8_t *) kt)[k] = (uint8_t) m;
+        else if (TB == 2u) ((uint16_t *) kt)[k] = (uint16_t) m;
+        else ((uint32_t *) kt)[k] = m;
+    }
+    return gr_staged{gr_lds{fm, kt, queues + wave * (GR_QCAP + GR_QPAD)}, wrow, in_lds};
 }
 
 // TB = 0: group masks from memory (or no groups); TB = 1, 2, 4: the kept
@@ -583,31 +629,203 @@ __global__ void __launch_bounds__(GR_THREADS) gr_stream_kernel(gr_params a) {
     __shared__ gr_sink sink;
     const unsigned int lo = min(a.bounds[blockIdx.x], a.src.rows), hi = min(a.bounds[blockIdx.x + 1u], a.src.rows);
     if (lo >= hi) return;  // uniform across the workgroup
-    const unsigned int tid = threadIdx.x, wave = tid >> 6;
-    uint2 *fm = (uint2 *) lds;
-    unsigned char *kt = lds + align_up((size_t) (a.words + 1u) * 8u, 16u);
-    uint2 *queues = (uint2 *) (kt + align_up((size_t) a.cap * (TB != 0u ? TB : 1u), 16u));
-    unsigned int *wrow = (unsigned int *) (queues + GR_WAVES * (GR_QCAP + GR_QPAD));
-
-    if (tid == 0u) {
-        sink.o = a.o;
-        sink.filter = a.filter;
-    }
-    for (unsigned int w = tid; w <= a.words; w += GR_THREADS) fm[w] = a.fmap[w];
-    const unsigned int F = __builtin_amdgcn_readfirstlane(a.fmap[a.words].y);
-    const bool in_lds = TB != 0u && F <= a.cap;
-    for (unsigned int k = tid; k < F && in_lds; k += GR_THREADS) {
-        const uint32_t m = a.ktab[k];
-        if (TB == 1u) ((uint8_t *) kt)[k] = (uint8_t) m;
-        else if (TB == 2u) ((uint16_t *) kt)[k] = (uint16_t) m;
-        else ((uint32_t *) kt)[k] = m;
-    }
-    const gr_lds L{fm, kt, queues + wave * (GR_QCAP + GR_QPAD)};
+    const gr_staged st = gr_stage<TB>(a, lds, sink);
     if constexpr (TB == 0u) {
-        gr_stream_rows<VT, 0u>(a, L, sink, wrow, lo, hi);
+        gr_stream_rows<VT, 0u>(a, st.L, sink, st.wrow, lo, hi);
     } else {
-        if (in_lds) gr_stream_rows<VT, TB>(a, L, sink, wrow, lo, hi);
-        else gr_stream_rows<VT, 0u>(a, L, sink, wrow, lo, hi);
+        if (st.in_lds) gr_stream_rows<VT, TB>(a, st.L, sink, st.wrow, lo, hi);
+        else gr_stream_rows<VT, 0u>(a, st.L, sink, st.wrow, lo, hi);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Blocked-ELL: a lane takes 8 consecutive values of a row per chunk (one
+// block holds them: block size 8, 16 or 32), one 16-byte load for 2-byte
+// values.  The row block's block column of the lane's slot is loaded one chunk
+// ahead of the values; its 8 keep bits come from the LDS feature map, and the
+// value load is issued only when one of them is set, so slots of padding
+// (CSAMD_INVALID_COL) and blocks of no kept feature are not read.
+// ---------------------------------------------------------------------------
+template<int VT> struct oct_vals;
+// Raw buffer loads (non-temporal): an offset at or past the resource's
+// num_records reads zeros and touches no memory.
+__device__ __forceinline__ u32x4 gb_load16(__amdgpu_buffer_rsrc_t r, unsigned int off) {
+    return __builtin_bit_cast(u32x4, __builtin_amdgcn_raw_buffer_load_b128(r, (int) off, 0, 2));
+}
+__device__ __forceinline__ u32x2 gb_load8(__amdgpu_buffer_rsrc_t r, unsigned int off) {
+    return __builtin_bit_cast(u32x2, __builtin_amdgcn_raw_buffer_load_b64(r, (int) off, 0, 2));
+}
+
+template<> struct oct_vals<CSAMD_VALUE_F16> {
+    using raw = u32x4;
+    static constexpr unsigned int bytes = 2u;
+    static __device__ __forceinline__ raw load(__amdgpu_buffer_rsrc_t r, unsigned int off) { return gb_load16(r, off); }
+    static __device__ __forceinline__ float get(const raw &r, unsigned int k) {
+        const unsigned int w = lane_of(r, k >> 1);
+        return f16_bits_to_float((uint16_t) ((k & 1u) ? (w >> 16) : (w & 0xffffu)));
+    }
+};
+template<> struct oct_vals<CSAMD_VALUE_BF16> {
+    using raw = u32x4;
+    static constexpr unsigned int bytes = 2u;
+    static __device__ __forceinline__ raw load(__amdgpu_buffer_rsrc_t r, unsigned int off) { return gb_load16(r, off); }
+    static __device__ __forceinline__ float get(const raw &r, unsigned int k) {
+        const unsigned int w = lane_of(r, k >> 1);
+        return __uint_as_float((k & 1u) ? (w & 0xffff0000u) : (w << 16));
+    }
+};
+template<> struct oct_vals<CSAMD_VALUE_F32> {
+    struct raw {
+        u32x4 a, b;
+    };
+    static constexpr unsigned int bytes = 4u;
+    static __device__ __forceinline__ raw load(__amdgpu_buffer_rsrc_t r, unsigned int off) {
+        return raw{gb_load16(r, off), gb_load16(r, off + 16u)};
+    }
+    static __device__ __forceinline__ float get(const raw &r, unsigned int k) {
+        return __uint_as_float(k < 4u ? lane_of(r.a, k) : lane_of(r.b, k - 4u));
+    }
+};
+template<> struct oct_vals<CSAMD_VALUE_FP8_E4M3> {
+    using raw = u32x2;
+    static constexpr unsigned int bytes = 1u;
+    static __device__ __forceinline__ raw load(__amdgpu_buffer_rsrc_t r, unsigned int off) { return gb_load8(r, off); }
+    static __device__ __forceinline__ float get(const raw &r, unsigned int k) {
+        return fp8_e4m3_decode_hw(k < 4u ? r.x : r.y, (int) (k & 3u));
+    }
+};
+
+constexpr unsigned int GB_CHUNK = 64u * 8u;  // entries per wave chunk
+
+struct gb_cursor {
+    unsigned int row;
+    unsigned int e;  // first entry of the chunk within the row
+    bool live;
+};
+
+// One chunk's lane state: keep bits of the lane's 8 columns, the first
+// column, the values.
+template<int VT> struct gb_lane {
+    unsigned int kw;
+    unsigned int c0;
+    typename oct_vals<VT>::raw v;
+};
+
+// The workgroup's rows [lo, hi) in windows of a.nrb row blocks: the windows'
+// block columns are staged in LDS (bcl), so a chunk's value loads depend on
+// LDS reads only and stay in flight while the previous chunk is consumed.
+template<int VT, unsigned int TB>
+__device__ __forceinline__ void gr_bell_rows(const gr_params &a, const gr_lds &L, const gr_sink &sink, unsigned int *wrow,
+                                             unsigned int *bcl, unsigned int lo, unsigned int hi) {
+    __shared__ unsigned int s_next, s_nk, s_scan[16];
+    using O = oct_vals<VT>;
+    const unsigned int tid = threadIdx.x, lane = lane_id();
+    const unsigned int lbs = a.lbs, bs = 1u << lbs, ell = a.ell_cols, width = a.width;
+    const unsigned int nblocks = (a.src.cols + bs - 1u) >> lbs;  // CSAMD_INVALID_COL is past it
+    const unsigned int row_bytes = ell * O::bytes;
+    gr_acc A{0.0f, 0.0f, 0u, 0.0f, 0.0f};
+
+    for (unsigned int rb0 = lo >> lbs, rb_end = ((hi - 1u) >> lbs) + 1u; rb0 < rb_end; rb0 += a.nrb) {
+        const unsigned int rb1 = rb_end - rb0 < a.nrb ? rb_end : rb0 + a.nrb;
+        const unsigned int r0 = max(lo, rb0 << lbs), r1 = min(hi, rb1 << lbs);  // r1 - r0 <= GR_THREADS
+        __syncthreads();  // tables staged / the previous window consumed
+        const unsigned int nbc = (rb1 - rb0) * width;
+        for (unsigned int i = tid; i < nbc; i += GR_THREADS) bcl[i] = gmem(a.bcol)[(size_t) rb0 * width + i];
+        const unsigned int r = r0 + tid;
+        unsigned int take = 0u;
+        if (r < r1) {
+            const bool kept = a.row_keep == nullptr || gmem(a.row_keep)[r] != 0u;
+            if (!kept || ell == 0u) gr_write_empty(a, sink, r, kept);
+            else take = 1u;
+        }
+        unsigned int nk = 0u;
+        const unsigned int pos = block_exclusive_scan_u32(take, s_scan, &nk);
+        if (take) wrow[pos] = r;
+        if (tid == 0u) {
+            s_next = 0u;
+            s_nk = nk;
+        }
+        __syncthreads();
+        const unsigned int n_rows = __builtin_amdgcn_readfirstlane(s_nk);
+        auto next_row = [&](gb_cursor &c) {
+            unsigned int i = 0u;
+            if (lane == 0u) i = atomicAdd(&s_next, 1u);
+            i = __builtin_amdgcn_readfirstlane(i);
+            c.live = i < n_rows;
+            c.row = c.live ? __builtin_amdgcn_readfirstlane(wrow[i]) : 0u;
+            c.e = 0u;
+        };
+        auto advance = [&](gb_cursor &c) {
+            if (!c.live) return;
+            if (c.e + GB_CHUNK < ell) c.e += GB_CHUNK;
+            else next_row(c);
+        };
+        // the lane's block column from LDS, its 8 keep bits from the feature
+        // map (8 columns inside one 32-bit word), then the values; a lane with
+        // none kept reads past the row's buffer (zeros, no memory traffic)
+        auto load_vals = [&](const gb_cursor &c, gb_lane<VT> &s) {
+            const unsigned int row = __builtin_amdgcn_readfirstlane(c.row);
+            const unsigned int e0 = __builtin_amdgcn_readfirstlane(c.e) + 8u * lane;
+            const bool in_row = c.live && e0 < ell;
+            const unsigned int bc = bcl[in_row ? ((row >> lbs) - rb0) * width + (e0 >> lbs) : 0u];
+            const bool in = in_row && bc < nblocks;
+            s.c0 = (bc << lbs) + (e0 & (bs - 1u));
+            const unsigned int w = L.fmap[min(s.c0 >> 5, a.words)].x;
+            s.kw = in ? (w >> (s.c0 & 31u)) & 0xffu : 0u;
+            const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+                (void *) ((const unsigned char *) a.src.val + (size_t) row * row_bytes), (short) 0, (int) row_bytes, 0x00020000);
+            s.v = O::load(rs, s.kw != 0u ? e0 * O::bytes : row_bytes);
+        };
+        auto consume = [&](const gb_cursor &c, const gb_lane<VT> &s) {
+            unsigned int col[8];
+            float v[8];
+            bool kept[8];
+#pragma unroll
+            for (unsigned int k = 0; k < 8u; ++k) {
+                v[k] = O::get(s.v, k);
+                col[k] = s.c0 + k;
+                kept[k] = ((s.kw >> k) & 1u) != 0u && v[k] != 0.0f;
+            }
+            gr_accum<8u, TB>(a, L, col, v, kept, A);
+            if (c.e + GB_CHUNK >= ell) gr_finish(a, sink, c.row, A);
+        };
+        gb_cursor ca, cb;
+        next_row(ca);
+        if (ca.live) {
+            gb_lane<VT> RA, RB;
+            load_vals(ca, RA);
+            while (true) {
+                cb = ca;
+                advance(cb);
+                load_vals(cb, RB);
+                consume(ca, RA);
+                if (!cb.live) break;
+                ca = cb;
+                advance(ca);
+                load_vals(ca, RA);
+                consume(cb, RB);
+                if (!ca.live) break;
+            }
+        }
+    }
+}
+
+// Blocked-ELL: workgroup b takes rows [rows b / grid, rows (b + 1) / grid)
+// (every row holds ell_cols slots' values); TB as in gr_stream_kernel.
+template<int VT, unsigned int TB>
+__global__ void __launch_bounds__(GR_THREADS) gr_bell_kernel(gr_params a) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    __shared__ gr_sink sink;
+    const unsigned int lo = (unsigned int) ((unsigned long long) a.src.rows * blockIdx.x / gridDim.x);
+    const unsigned int hi = (unsigned int) ((unsigned long long) a.src.rows * (blockIdx.x + 1u) / gridDim.x);
+    if (lo >= hi) return;  // uniform across the workgroup
+    const gr_staged st = gr_stage<TB>(a, lds, sink);
+    unsigned int *bcl = st.wrow + GR_WINDOW;  // [nrb x width] block columns of a window
+    if constexpr (TB == 0u) {
+        gr_bell_rows<VT, 0u>(a, st.L, sink, st.wrow, bcl, lo, hi);
+    } else {
+        if (st.in_lds) gr_bell_rows<VT, TB>(a, st.L, sink, st.wrow, bcl, lo, hi);
+        else gr_bell_rows<VT, 0u>(a, st.L, sink, st.wrow, bcl, lo, hi);
     }
 }
 
@@ -615,26 +833,98 @@ size_t fixed_lds_bytes() {
     return (size_t) GR_WAVES * (GR_QCAP + GR_QPAD) * 8u + (size_t) 3u * GR_WINDOW * 4u;
 }
 
-template<int VT, unsigned int TB>
+template<int VT, unsigned int TB, bool BELL>
 int launch_stream(const gr_params &p, unsigned int grid, size_t lds, hipStream_t s) {
+    const void *fn = BELL ? (const void *) gr_bell_kernel<VT, TB> : (const void *) gr_stream_kernel<VT, TB>;
     static bool attr_set = false;
     if (!attr_set) {
-        CSAMD_HIP(hipFuncSetAttribute((const void *) gr_stream_kernel<VT, TB>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      (int) GR_LDS_BUDGET),
-                  "gr_stream_kernel lds");
+        CSAMD_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int) GR_LDS_BUDGET), "group reduce lds");
         attr_set = true;
     }
-    gr_stream_kernel<VT, TB><<<grid, GR_THREADS, lds, s>>>(p);
-    return hip_ok(hipGetLastError(), "gr_stream_kernel");
+    if constexpr (BELL) gr_bell_kernel<VT, TB><<<grid, GR_THREADS, lds, s>>>(p);
+    else gr_stream_kernel<VT, TB><<<grid, GR_THREADS, lds, s>>>(p);
+    return hip_ok(hipGetLastError(), BELL ? "gr_bell_kernel" : "gr_stream_kernel");
 }
 
-template<int VT> int launch_stream_tb(const gr_params &p, unsigned int tb, unsigned int grid, size_t lds, hipStream_t s) {
-    switch (tb) {
-        case 1u: return launch_stream<VT, 1u>(p, grid, lds, s);
-        case 2u: return launch_stream<VT, 2u>(p, grid, lds, s);
-        case 4u: return launch_stream<VT, 4u>(p, grid, lds, s);
-        default: return launch_stream<VT, 0u>(p, grid, lds, s);
+template<bool BELL> int launch_any(const gr_params &p, unsigned int tb, unsigned int grid, hipStream_t s) {
+    // the whole budget, so that exactly one workgroup runs per CU
+    const size_t lds = GR_LDS_BUDGET;
+#define GR_TB_CASES(VT)                                                                                                          \
+    switch (tb) {                                                                                                                \
+        case 1u: return launch_stream<VT, 1u, BELL>(p, grid, lds, s);                                                           \
+        case 2u: return launch_stream<VT, 2u, BELL>(p, grid, lds, s);                                                           \
+        case 4u: return launch_stream<VT, 4u, BELL>(p, grid, lds, s);                                                           \
+        default: return launch_stream<VT, 0u, BELL>(p, grid, lds, s);                                                           \
     }
+    switch (p.src.value_type) {
+        case CSAMD_VALUE_F16: GR_TB_CASES(CSAMD_VALUE_F16)
+        case CSAMD_VALUE_BF16: GR_TB_CASES(CSAMD_VALUE_BF16)
+        case CSAMD_VALUE_F32: GR_TB_CASES(CSAMD_VALUE_F32)
+        case CSAMD_VALUE_FP8_E4M3: GR_TB_CASES(CSAMD_VALUE_FP8_E4M3)
+        default: set_error("group reduce: value type %u", p.src.value_type); return 0;
+    }
+#undef GR_TB_CASES
+}
+
+size_t feature_lds_bytes(unsigned int cols) {
+    return align_up((((size_t) cols + 31u) / 32u + 1u) * 8u, 16u);
+}
+
+// The kernels' common parameters; enqueues the feature map and the kept
+// features' group masks.  Returns the LDS table entry bytes (0: no groups)
+// through *tb.
+int gr_prepare(unsigned int rows, unsigned int cols, const void *val, unsigned int value_type, const uint32_t *fgm,
+               unsigned int group_count, const unsigned char *row_keep, const unsigned char *feature_keep,
+               const csamd_sparse_group_filter_params &filter, const gr_outputs &out, void *scratch, size_t scratch_bytes,
+               hipStream_t s, gr_params *p, unsigned int *tb) {
+    CSAMD_REQUIRE(scratch != nullptr && scratch_bytes >= gr_scratch_bytes(cols), "group reduce: scratch %zu < %zu bytes",
+                  scratch_bytes, gr_scratch_bytes(cols));
+    CSAMD_REQUIRE(group_count <= CSAMD_MAX_MASK_GROUPS, "group reduce: group_count %u > %u", group_count, CSAMD_MAX_MASK_GROUPS);
+    const unsigned int G = fgm != nullptr ? group_count : 0u;
+    const unsigned int words = (cols + 31u) / 32u;
+    char *base = (char *) scratch;
+    uint2 *fmap = (uint2 *) (base + align_up(((size_t) GR_MAX_GRID + 1u) * 4u, 256u));
+    uint32_t *ktab = (uint32_t *) ((char *) fmap + align_up(((size_t) words + 1u) * 8u, 256u));
+    const uint32_t gbits = G >= 32u ? 0xffffffffu : (1u << G) - 1u;
+    gr_fmap_kernel<<<1, 1024, 0, s>>>(feature_keep, cols, words, fmap);
+    CSAMD_HIP(hipGetLastError(), "gr_fmap_kernel");
+    if (G != 0u && cols != 0u) {
+        gr_ktab_kernel<<<grid_for(cols, 256u, 1024u), 256, 0, s>>>(fmap, fgm, cols, gbits, ktab);
+        CSAMD_HIP(hipGetLastError(), "gr_ktab_kernel");
+    }
+    std::memset(p, 0, sizeof(*p));
+    p->src.rows = rows;
+    p->src.cols = cols;
+    p->src.val = val;
+    p->src.value_type = value_type;
+    p->row_keep = row_keep;
+    p->fmap = fmap;
+    p->words = words;
+    p->ktab = ktab;
+    p->fgm = fgm;
+    p->gbits = gbits;
+    p->G = G;
+    unsigned int logP = 0u;  // P = min(2^ceil(log2 G), 16) lanes per queued entry
+    while ((1u << logP) < G && logP < 4u) ++logP;
+    p->logP = logP;
+    p->filter = filter;
+    p->o = out;
+    if (G == 0u) {
+        p->o.group_counts = nullptr;
+        p->o.group_percentages = nullptr;
+    }
+    const size_t room = GR_LDS_BUDGET - feature_lds_bytes(cols) - fixed_lds_bytes();
+    // LDS table entry bytes: the narrowest that holds G bits
+    *tb = G == 0u ? 0u : G <= 8u ? 1u : G <= 16u ? 2u : 4u;
+    p->cap = *tb != 0u ? (unsigned int) (room / *tb < cols ? room / *tb : cols) : 0u;
+    p->cap &= ~15u;
+    return 1;
+}
+
+unsigned int gr_grid() {
+    unsigned int grid = (unsigned int) device_cu_count();
+    if (grid < 1u) grid = 1u;
+    return grid > GR_MAX_GRID ? GR_MAX_GRID : grid;
 }
 
 } // namespace
@@ -660,67 +950,70 @@ int gr_reduce_rows(const gr_source &src, const uint32_t *fgm, unsigned int group
                    const unsigned char *feature_keep, const csamd_sparse_group_filter_params &filter, const gr_outputs &out,
                    void *scratch, size_t scratch_bytes, hipStream_t s) {
     CSAMD_REQUIRE(gr_eligible(src), "group reduce: source not eligible for the streaming path");
-    CSAMD_REQUIRE(scratch != nullptr && scratch_bytes >= gr_scratch_bytes(src.cols), "group reduce: scratch %zu < %zu bytes",
-                  scratch_bytes, gr_scratch_bytes(src.cols));
-    CSAMD_REQUIRE(group_count <= CSAMD_MAX_MASK_GROUPS, "group reduce: group_count %u > %u", group_count, CSAMD_MAX_MASK_GROUPS);
-    const unsigned int G = fgm != nullptr ? group_count : 0u;
-    const unsigned int words = (src.cols + 31u) / 32u;
-    unsigned int grid = (unsigned int) device_cu_count();
-    if (grid < 1u) grid = 1u;
-    if (grid > GR_MAX_GRID) grid = GR_MAX_GRID;
-    char *base = (char *) scratch;
-    unsigned int *bounds = (unsigned int *) base;
-    uint2 *fmap = (uint2 *) (base + align_up(((size_t) GR_MAX_GRID + 1u) * 4u, 256u));
-    uint32_t *ktab = (uint32_t *) ((char *) fmap + align_up(((size_t) words + 1u) * 8u, 256u));
-    const uint32_t gbits = G >= 32u ? 0xffffffffu : (1u << G) - 1u;
-
-    gr_fmap_kernel<<<1, 1024, 0, s>>>(feature_keep, src.cols, words, fmap);
-    CSAMD_HIP(hipGetLastError(), "gr_fmap_kernel");
-    if (G != 0u && src.cols != 0u) {
-        gr_ktab_kernel<<<grid_for(src.cols, 256u, 1024u), 256, 0, s>>>(fmap, fgm, src.cols, gbits, ktab);
-        CSAMD_HIP(hipGetLastError(), "gr_ktab_kernel");
-    }
+    gr_params p;
+    unsigned int tb = 0u;
+    if (!gr_prepare(src.rows, src.cols, src.val, src.value_type, fgm, group_count, row_keep, feature_keep, filter, out, scratch,
+                    scratch_bytes, s, &p, &tb))
+        return 0;
+    p.src = src;
+    const unsigned int grid = gr_grid();
+    unsigned int *bounds = (unsigned int *) scratch;
     gr_fill_kernel<<<grid_for(grid + 1u, 256u, 16u), 256, 0, s>>>(bounds, grid + 1u, src.rows);
     CSAMD_HIP(hipGetLastError(), "gr_fill_kernel");
     const unsigned int bthreads = src.rows > grid ? src.rows : grid;
     gr_bounds_kernel<<<grid_for(bthreads, 256u, 0x7fffffffu), 256, 0, s>>>(src, grid, bounds);
     CSAMD_HIP(hipGetLastError(), "gr_bounds_kernel");
-
-    gr_params p;
-    std::memset(&p, 0, sizeof(p));
-    p.src = src;
-    p.row_keep = row_keep;
-    p.fmap = fmap;
-    p.words = words;
-    p.ktab = ktab;
-    p.fgm = fgm;
-    p.gbits = gbits;
-    p.G = G;
-    unsigned int logP = 0u;  // P = min(2^ceil(log2 G), 16) lanes per queued entry
-    while ((1u << logP) < G && logP < 4u) ++logP;
-    p.logP = logP;
-    p.filter = filter;
-    p.o = out;
-    if (G == 0u) {
-        p.o.group_counts = nullptr;
-        p.o.group_percentages = nullptr;
-    }
     p.bounds = bounds;
-    const size_t fmap_lds = align_up(((size_t) words + 1u) * 8u, 16u);
-    size_t room = GR_LDS_BUDGET - fmap_lds - fixed_lds_bytes();
-    // LDS table entry bytes: the narrowest that holds G bits (0: no groups)
-    const unsigned int tb = G == 0u ? 0u : G <= 8u ? 1u : G <= 16u ? 2u : 4u;
-    p.cap = tb != 0u ? (unsigned int) (room / tb < src.cols ? room / tb : src.cols) : 0u;
-    p.cap &= ~15u;
-    // the whole budget, so that exactly one workgroup runs per CU
-    const size_t lds = GR_LDS_BUDGET;
-    switch (src.value_type) {
-        case CSAMD_VALUE_F16: return launch_stream_tb<CSAMD_VALUE_F16>(p, tb, grid, lds, s);
-        case CSAMD_VALUE_BF16: return launch_stream_tb<CSAMD_VALUE_BF16>(p, tb, grid, lds, s);
-        case CSAMD_VALUE_F32: return launch_stream_tb<CSAMD_VALUE_F32>(p, tb, grid, lds, s);
-        case CSAMD_VALUE_FP8_E4M3: return launch_stream_tb<CSAMD_VALUE_FP8_E4M3>(p, tb, grid, lds, s);
-        default: set_error("group reduce: value type %u", src.value_type); return 0;
-    }
+    return launch_any<false>(p, tb, grid, s);
+}
+
+// LDS left for a Blocked-ELL window's block columns and the group-mask table
+// (the row list takes GR_WINDOW words; the rest of the window area is reused).
+size_t bell_room(unsigned int cols) {
+    const size_t used = feature_lds_bytes(cols) + (size_t) GR_WAVES * (GR_QCAP + GR_QPAD) * 8u + (size_t) GR_WINDOW * 4u;
+    return used < GR_LDS_BUDGET ? GR_LDS_BUDGET - used : 0u;
+}
+
+bool gr_bell_eligible(const gr_bell_source &src) {
+    if (!value_type_decodable(src.value_type) || src.rows == 0u || src.val == nullptr) return false;
+    const unsigned int bs = src.block_size;
+    if (bs != 8u && bs != 16u && bs != 32u) return false;
+    if (src.ell_cols % bs != 0u || (src.ell_cols != 0u && src.block_col_idx == nullptr)) return false;
+    if (((size_t) src.cols + 31u) / 32u > GR_MAX_WORDS || feature_lds_bytes(src.cols) + fixed_lds_bytes() > GR_LDS_BUDGET) return false;
+    if ((size_t) (src.ell_cols / bs) * 4u + 16u > bell_room(src.cols)) return false;  // one row block's block columns
+    if ((size_t) src.ell_cols * value_type_bytes(src.value_type) > 0x7fffffffu) return false;  // a row's buffer range
+    const size_t align = src.value_type == CSAMD_VALUE_FP8_E4M3 ? 8u : 16u;  // 8-value loads
+    return ((uintptr_t) src.val % align) == 0u;
+}
+
+int gr_reduce_bell(const gr_bell_source &src, const uint32_t *fgm, unsigned int group_count, const unsigned char *row_keep,
+                   const unsigned char *feature_keep, const csamd_sparse_group_filter_params &filter, const gr_outputs &out,
+                   void *scratch, size_t scratch_bytes, hipStream_t s) {
+    CSAMD_REQUIRE(gr_bell_eligible(src), "group reduce: Blocked-ELL source not eligible for the streaming path");
+    gr_params p;
+    unsigned int tb = 0u;
+    if (!gr_prepare(src.rows, src.cols, src.val, src.value_type, fgm, group_count, row_keep, feature_keep, filter, out, scratch,
+                    scratch_bytes, s, &p, &tb))
+        return 0;
+    p.bcol = src.block_col_idx;
+    p.lbs = src.block_size == 8u ? 3u : src.block_size == 16u ? 4u : 5u;
+    p.ell_cols = src.ell_cols;
+    p.width = src.ell_cols / src.block_size;
+    // a window stages nrb row blocks' block columns: up to half of the LDS
+    // left (at least one row block), the rest holds the group-mask table
+    const size_t room = bell_room(src.cols), one = (size_t) p.width * 4u;
+    const unsigned int row_blocks = (src.rows + src.block_size - 1u) >> p.lbs;
+    size_t nrb = one != 0u ? (room / 2u) / one : (size_t) row_blocks;
+    if (nrb < 1u) nrb = 1u;
+    if (nrb > (GR_WINDOW >> p.lbs)) nrb = GR_WINDOW >> p.lbs;
+    if (nrb > row_blocks) nrb = row_blocks;
+    p.nrb = (unsigned int) nrb;
+    const size_t left = room - align_up(nrb * one, 16u);
+    p.cap = tb != 0u ? (unsigned int) (left / tb < src.cols ? left / tb : src.cols) & ~15u : 0u;
+    unsigned int grid = gr_grid();
+    const unsigned int by_rows = (src.rows + GR_WAVES - 1u) / GR_WAVES;  // at least a row per wave
+    if (grid > by_rows) grid = by_rows;
+    return launch_any<true>(p, tb, grid, s);
 }
 
 // ---------------------------------------------------------------------------

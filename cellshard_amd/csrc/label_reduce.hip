@@ -589,7 +589,7 @@ __device__ __forceinline__ void load_chunk(const unsigned int *__restrict__ idx,
 #pragma unroll
         for (unsigned int j = 0; j < LR_UNITS; ++j) {
             const unsigned int uu = min(u + lane + 64u * j, u_end - 1u);
-            r.c[j] = __builtin_nontemporal_load((const u32x4 *) (idx + uu * 4u));
+            r.c[j] = __builtin_nontemporal_load(gmem((const u32x4 *) (idx + uu * 4u)));
             r.v[j] = unit_vals<VT>::vload(val, uu * 4u);
         }
     } else {

@@ -1009,10 +1009,18 @@ int csamd_compute_sparse_group_reduce_blocked_ell_typed(const csamd_blocked_ell_
                                                         csamd_sparse_group_reduce_result *out) {
     reduce_prelude p;
     CSAMD_REQUIRE(src != nullptr, "compute_sparse_group_reduce(blocked_ell): null src");
+    const gr_bell_source gb{src->rows, src->cols, src->block_size, src->ell_cols, src->blockColIdx, src->val, value_type};
+    const bool stream = gr_bell_eligible(gb);
     if (!prepare_reduce(src, ws, groups, masks, filter, src->rows, src->cols,
-                        (unsigned long long) src->rows * src->ell_cols, value_type, &p))
+                        (unsigned long long) src->rows * src->ell_cols, value_type, &p, !stream))
         return 0;
-    if (src->rows != 0u) {
+    if (stream) {
+        if (!ensure_scratch(ws, gr_scratch_bytes(gb.cols))) return 0;
+        if (!gr_reduce_bell(gb, p.group_count != 0u ? p.fgm : nullptr, p.group_count, p.masks.row_keep, p.masks.feature_keep, *filter,
+                            stream_outputs(outputs_of(ws, p.group_count)), ws->scratch_block, ws->scratch_bytes,
+                            (hipStream_t) ws->stream))
+            return 0;
+    } else if (src->rows != 0u) {
         if (!dispatch_value_type<launch_blocked>(value_type, *src, reduce_grid(src->rows), (hipStream_t) ws->stream, p, *filter,
                                                  outputs_of(ws, p.group_count)))
             return 0;

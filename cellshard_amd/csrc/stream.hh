@@ -18,7 +18,7 @@ template<int VT> struct unit_vals;
 template<> struct unit_vals<CSAMD_VALUE_F16> {
     using raw = u32x2;
     static __device__ __forceinline__ raw vload(const uint16_t *v, unsigned int i0) {
-        return __builtin_nontemporal_load((const u32x2 *) (v + i0));
+        return __builtin_nontemporal_load(gmem((const u32x2 *) (v + i0)));
     }
     static __device__ __forceinline__ raw sload(const uint16_t *v, unsigned int i0, unsigned int last) {
         raw r;
@@ -34,7 +34,7 @@ template<> struct unit_vals<CSAMD_VALUE_F16> {
 template<> struct unit_vals<CSAMD_VALUE_BF16> {
     using raw = u32x2;
     static __device__ __forceinline__ raw vload(const uint16_t *v, unsigned int i0) {
-        return __builtin_nontemporal_load((const u32x2 *) (v + i0));
+        return __builtin_nontemporal_load(gmem((const u32x2 *) (v + i0)));
     }
     static __device__ __forceinline__ raw sload(const uint16_t *v, unsigned int i0, unsigned int last) {
         return unit_vals<CSAMD_VALUE_F16>::sload(v, i0, last);
@@ -47,7 +47,7 @@ template<> struct unit_vals<CSAMD_VALUE_BF16> {
 template<> struct unit_vals<CSAMD_VALUE_F32> {
     using raw = u32x4;
     static __device__ __forceinline__ raw vload(const float *v, unsigned int i0) {
-        return __builtin_nontemporal_load((const u32x4 *) (v + i0));
+        return __builtin_nontemporal_load(gmem((const u32x4 *) (v + i0)));
     }
     static __device__ __forceinline__ raw sload(const float *v, unsigned int i0, unsigned int last) {
         raw r;
@@ -64,7 +64,7 @@ template<> struct unit_vals<CSAMD_VALUE_F32> {
 template<> struct unit_vals<CSAMD_VALUE_FP8_E4M3> {
     using raw = unsigned int;
     static __device__ __forceinline__ raw vload(const uint8_t *v, unsigned int i0) {
-        return __builtin_nontemporal_load((const unsigned int *) (v + i0));
+        return __builtin_nontemporal_load(gmem((const unsigned int *) (v + i0)));
     }
     static __device__ __forceinline__ raw sload(const uint8_t *v, unsigned int i0, unsigned int last) {
         return (unsigned int) v[min(i0, last)] | ((unsigned int) v[min(i0 + 1u, last)] << 8) |
