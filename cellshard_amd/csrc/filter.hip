@@ -119,10 +119,22 @@ template<typename S> union fl_group {
     S v[16u / sizeof(S)];
 };
 
+// The row block's block columns, staged in LDS or read in place.  The two
+// are read through pointers of their own address spaces: a generic pointer
+// that may point at either would make every read a flat access, which the
+// compiler counts against both memory counters (it then waits for all
+// outstanding value loads at each read).
+struct fl_slots_ref {
+    const __attribute__((address_space(3))) unsigned int *lds;
+    const CSAMD_GLOBAL unsigned int *glob;
+    bool in_lds;
+    __device__ __forceinline__ unsigned int operator[](unsigned int k) const { return in_lds ? lds[k] : glob[k]; }
+};
+
 // Surviving entries of group g (bit k: position k of the lane's group);
-// slots: the row block's block columns (LDS or global).
+// slots: the row block's block columns.
 template<typename S>
-__device__ __forceinline__ unsigned int fl_group_mask(const csamd_blocked_ell_view &src, const fl_masks &m, const unsigned int *slots,
+__device__ __forceinline__ unsigned int fl_group_mask(const csamd_blocked_ell_view &src, const fl_masks &m, const fl_slots_ref &slots,
                                                       unsigned int e, const fl_group<S> &g, unsigned int *oc) {
     constexpr unsigned int PER = 16u / sizeof(S);
     unsigned int mask = 0u;
@@ -166,12 +178,13 @@ __device__ __forceinline__ unsigned int fl_group_mask(const csamd_blocked_ell_vi
 
 // The row block's block columns: staged in LDS when they fit (the caller
 // syncs), else read in place.
-__device__ __forceinline__ const unsigned int *fl_slots(const csamd_blocked_ell_view &src, unsigned int width, unsigned int rb,
-                                                       unsigned int *lds_slots, unsigned int lds_cap) {
-    const unsigned int *g = src.blockColIdx + (unsigned long long) rb * width;
-    if (width > lds_cap) return g;
-    for (unsigned int k = threadIdx.x; k < width; k += blockDim.x) lds_slots[k] = g[k];
-    return lds_slots;
+__device__ __forceinline__ fl_slots_ref fl_slots(const csamd_blocked_ell_view &src, unsigned int width, unsigned int rb,
+                                                 unsigned int *lds_slots, unsigned int lds_cap) {
+    const CSAMD_GLOBAL unsigned int *g = gmem((const unsigned int *) src.blockColIdx) + (unsigned long long) rb * width;
+    const bool in_lds = width <= lds_cap;
+    if (in_lds)
+        for (unsigned int k = threadIdx.x; k < width; k += blockDim.x) lds_slots[k] = g[k];
+    return fl_slots_ref{(const __attribute__((address_space(3))) unsigned int *) lds_slots, g, in_lds};
 }
 
 template<typename S, unsigned int U>
@@ -200,7 +213,7 @@ __global__ void __launch_bounds__(FL_THREADS) fl_count_kernel(csamd_blocked_ell_
     extern __shared__ unsigned int lds_slots[];
     if (grouped) {
         unsigned int cur_rb = ~0u;
-        const unsigned int *slots = nullptr;
+        fl_slots_ref slots{nullptr, nullptr, false};
         for (unsigned int r = blockIdx.x; r < src.rows; r += gridDim.x) {
             const unsigned int o = fl_out_row(m, r);
             if (o >= m.output_rows) continue;
@@ -267,7 +280,7 @@ __global__ void __launch_bounds__(FL_THREADS) fl_emit_kernel(csamd_blocked_ell_v
     extern __shared__ unsigned int lds_slots[];
     if (grouped) {
         unsigned int cur_rb = ~0u;
-        const unsigned int *slots = nullptr;
+        fl_slots_ref slots{nullptr, nullptr, false};
         for (unsigned int r = blockIdx.x; r < src.rows; r += gridDim.x) {
             const unsigned int o = fl_out_row(m, r);
             if (o >= m.output_rows) continue;

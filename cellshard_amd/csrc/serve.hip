@@ -66,8 +66,8 @@ constexpr unsigned int SERVE_UNROLL = 4u;
 template<int VT>
 __device__ __forceinline__ void serve_column(const query_dev &Q, unsigned int b, unsigned int e, double *acc_d, unsigned int *acc_u) {
     using S = typename value_traits<VT>::storage;
-    const S *__restrict__ val = (const S *) Q.val;
-    const unsigned int *__restrict__ rowidx = Q.rowidx;
+    const CSAMD_GLOBAL S *__restrict__ val = gmem((const S *) Q.val);
+    const CSAMD_GLOBAL unsigned int *__restrict__ rowidx = gmem(Q.rowidx);
     for (unsigned int i0 = b + threadIdx.x; i0 < e; i0 += SERVE_THREADS * SERVE_UNROLL) {
         unsigned int r[SERVE_UNROLL];
         float v[SERVE_UNROLL];
@@ -81,9 +81,9 @@ __device__ __forceinline__ void serve_column(const query_dev &Q, unsigned int b,
 #pragma unroll
         for (unsigned int u = 0; u < SERVE_UNROLL; ++u) {
             if (r[u] >= Q.rows || v[u] == 0.0f) continue;
-            const bool kept = Q.keep != nullptr ? Q.keep[r[u]] != 0u : seeded_keep(Q.mask_seed, r[u], Q.keep_thr24);
+            const bool kept = Q.keep != nullptr ? gmem(Q.keep)[r[u]] != 0u : seeded_keep(Q.mask_seed, r[u], Q.keep_thr24);
             if (!kept) continue;
-            const unsigned int g = Q.labels != nullptr ? Q.labels[r[u]] : seeded_label(Q.label_seed, r[u], Q.G);
+            const unsigned int g = Q.labels != nullptr ? gmem(Q.labels)[r[u]] : seeded_label(Q.label_seed, r[u], Q.G);
             if (g >= Q.G) continue;
             if (Q.op == CSAMD_SERVE_NNZ) atomicAdd(acc_u + g, 1u);
             else {
@@ -117,8 +117,8 @@ __global__ void __launch_bounds__(SERVE_THREADS) serve_batch_kernel(const query_
         }
         const query_dev Q = qs[lo];
         const unsigned int j = item - pre[lo];
-        const unsigned int c = Q.features[j];
-        const unsigned int b = Q.colptr[c], e = Q.colptr[c + 1u];
+        const unsigned int c = gmem(Q.features)[j];
+        const unsigned int b = gmem(Q.colptr)[c], e = gmem(Q.colptr)[c + 1u];
         for (unsigned int g = threadIdx.x; g < Q.G; g += blockDim.x) acc_d[g] = 0.0;
         __syncthreads();
         switch (Q.value_type) {
@@ -130,9 +130,9 @@ __global__ void __launch_bounds__(SERVE_THREADS) serve_batch_kernel(const query_
         __syncthreads();
         const unsigned long long o = (unsigned long long) j * Q.G;
         if (Q.op == CSAMD_SERVE_NNZ)
-            for (unsigned int g = threadIdx.x; g < Q.G; g += blockDim.x) ((unsigned int *) Q.out)[o + g] = acc_u[g];
+            for (unsigned int g = threadIdx.x; g < Q.G; g += blockDim.x) gmem((unsigned int *) Q.out)[o + g] = acc_u[g];
         else
-            for (unsigned int g = threadIdx.x; g < Q.G; g += blockDim.x) ((double *) Q.out)[o + g] = acc_d[g];
+            for (unsigned int g = threadIdx.x; g < Q.G; g += blockDim.x) gmem((double *) Q.out)[o + g] = acc_d[g];
         __syncthreads();  // accumulators and item_s are reused
     }
 }
