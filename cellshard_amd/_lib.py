@@ -46,6 +46,29 @@ PROTOTYPES = {
         P(A.group_mask_config_view), P(A.row_feature_mask_view), P(A.sparse_group_filter_params), P(A.sparse_group_reduce_result)]),
     "csamd_compute_sparse_group_reduce_sliced_ell_typed": (i32, [P(A.sliced_ell_view), u32, P(A.sparse_group_reduce_workspace),
         P(A.group_mask_config_view), P(A.row_feature_mask_view), P(A.sparse_group_filter_params), P(A.sparse_group_reduce_result)]),
+    "csamd_major_bucket_init": (None, [P(A.compressed_major_bucket_workspace)]),
+    "csamd_major_bucket_clear": (None, [P(A.compressed_major_bucket_workspace)]),
+    "csamd_major_bucket_setup": (i32, [P(A.compressed_major_bucket_workspace), i32, vp]),
+    "csamd_major_bucket_reserve": (i32, [P(A.compressed_major_bucket_workspace), u32, u32, u32]),
+    "csamd_major_bucket_reserve_typed": (i32, [P(A.compressed_major_bucket_workspace), u32, u32, u32, u32]),
+    "csamd_major_bucket_build_plan": (i32, [P(A.compressed_view), u32, P(A.compressed_major_bucket_workspace),
+        P(A.major_nnz_bucket_plan_view)]),
+    "csamd_major_bucket_rebuild": (i32, [P(A.compressed_view), P(A.major_nnz_bucket_plan_view), P(A.compressed_major_bucket_workspace),
+        P(A.compressed_major_bucket_result)]),
+    "csamd_major_bucket_rebuild_typed": (i32, [P(A.compressed_view), u32, P(A.major_nnz_bucket_plan_view),
+        P(A.compressed_major_bucket_workspace), P(A.compressed_major_bucket_result)]),
+    "csamd_build_bucketed_major_view": (i32, [P(A.compressed_view), u32, P(A.compressed_major_bucket_workspace),
+        P(A.compressed_major_bucket_result)]),
+    "csamd_build_bucketed_major_view_typed": (i32, [P(A.compressed_view), u32, u32, P(A.compressed_major_bucket_workspace),
+        P(A.compressed_major_bucket_result)]),
+    "csamd_sharded_major_bucket_init": (None, [P(A.sharded_major_bucket_workspace)]),
+    "csamd_sharded_major_bucket_clear": (None, [P(A.sharded_major_bucket_workspace)]),
+    "csamd_sharded_major_bucket_setup": (i32, [P(A.sharded_major_bucket_workspace), i32, vp]),
+    "csamd_sharded_major_bucket_reserve_parts": (i32, [P(A.sharded_major_bucket_workspace), C.c_ulong]),
+    "csamd_build_bucketed_shard_major_view": (i32, [P(A.sharded_device), P(A.sharded), C.c_ulong, u32,
+        P(A.sharded_major_bucket_workspace), P(A.sharded_major_bucket_result)]),
+    "csamd_build_bucketed_shard_major_view_typed": (i32, [P(A.sharded_device), P(A.sharded), C.c_ulong, u32, u32,
+        P(A.sharded_major_bucket_workspace), P(A.sharded_major_bucket_result)]),
     "csamd_fleet_init": (None, [P(A.sparse_group_reduce_fleet_workspace)]),
     "csamd_fleet_clear": (None, [P(A.sparse_group_reduce_fleet_workspace)]),
     "csamd_setup_fleet": (i32, [P(A.sparse_group_reduce_fleet_workspace), P(A.sparse_group_reduce_fleet_config)]),

@@ -271,6 +271,26 @@ sharded_device = type("sharded_device", (C.Structure,), {"_fields_": [
     ("cache", C.POINTER(cached_allocation)),
 ]})
 
+compressed_major_bucket_workspace = _aligned16("compressed_major_bucket_workspace", [
+    ("device", C.c_int), ("stream", c_void_p), ("major_capacity", c_u32), ("nnz_capacity", c_u32), ("bucket_capacity", c_u32),
+    ("sort_capacity", c_size_t), ("scan_capacity", c_size_t), ("d_major_block", c_void_p), ("d_rebuilt_block", c_void_p),
+    ("d_major_nnz", c_void_p), ("d_major_nnz_sorted", c_void_p), ("d_major_order_in", c_void_p), ("d_major_order_out", c_void_p),
+    ("d_inverse_major_order", c_void_p), ("d_bucket_offsets", c_void_p), ("d_rebuilt_major_ptr", c_void_p),
+    ("d_rebuilt_minor_idx", c_void_p), ("d_rebuilt_val", c_void_p), ("d_sort_tmp", c_void_p), ("d_scan_tmp", c_void_p),
+    ("owns_stream", C.c_int), ("value_bytes_capacity", c_u32),
+])
+
+sharded_major_bucket_workspace = _aligned16("sharded_major_bucket_workspace", [
+    ("base", compressed_major_bucket_workspace), ("part_capacity", c_ulong), ("h_part_block", c_void_p), ("d_part_block", c_void_p),
+    ("h_part_major_ptr", c_void_p), ("h_part_minor_idx", c_void_p), ("h_part_val", c_void_p), ("h_part_row_offsets", c_void_p),
+    ("d_part_major_ptr", c_void_p), ("d_part_minor_idx", c_void_p), ("d_part_val", c_void_p), ("d_part_row_offsets", c_void_p),
+    ("tables_event", c_void_p),
+])
+
+sharded_major_bucket_result = _aligned16("sharded_major_bucket_result", [
+    ("rebuilt", compressed_view), ("plan", major_nnz_bucket_plan_view), ("shard_id", c_ulong),
+])
+
 allocation_cache = _aligned16("allocation_cache", [
     ("cache_count", C.c_ulong), ("cache_capacity", C.c_ulong), ("cache", C.POINTER(cached_allocation)),
 ])
@@ -350,6 +370,9 @@ ALL_STRUCTS = {
     "csamd_major_nnz_bucket_plan_view": major_nnz_bucket_plan_view,
     "csamd_filtered_blocked_ell_stats": filtered_blocked_ell_stats,
     "csamd_compressed_major_bucket_result": compressed_major_bucket_result,
+    "csamd_compressed_major_bucket_workspace": compressed_major_bucket_workspace,
+    "csamd_sharded_major_bucket_workspace": sharded_major_bucket_workspace,
+    "csamd_sharded_major_bucket_result": sharded_major_bucket_result,
     "csamd_filtered_blocked_ell_result": filtered_blocked_ell_result,
     "csamd_filtered_blocked_ell_workspace": filtered_blocked_ell_workspace,
     "csamd_sharded_blocked_ell_repack_config": sharded_blocked_ell_repack_config,

@@ -40,6 +40,7 @@ HIP_SOURCES = [
     "cspack.hip",
     "sharded.hip",
     "group_reduce.hip",
+    "bucket_ops.hip",
 ]
 
 HIPCC_FLAGS = [

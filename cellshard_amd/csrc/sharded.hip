@@ -276,6 +276,18 @@ int upload_one(csamd_sharded_device *s, const csamd_sharded *v, unsigned int for
     return 1;
 }
 
+// The partition's device arrays in its record (a0.. as the single-partition
+// uploads set them), for callers that read partitions of a packed shard.
+inline void set_record_arrays(csamd_partition_record &r, const csamd_compressed_view &v) {
+    r.a0 = v.majorPtr;
+    r.a1 = v.minorIdx;
+    r.a2 = v.val;
+}
+inline void set_record_arrays(csamd_partition_record &r, const csamd_blocked_ell_view &v) {
+    r.a0 = v.blockColIdx;
+    r.a1 = v.val;
+}
+
 template<typename HostT, typename ViewT, typename Upload>
 int upload_packed(csamd_sharded_device *s, const csamd_sharded *v, unsigned long b, unsigned long e, int device, csamd_stream_t stream,
                   Upload upload) {
@@ -297,6 +309,7 @@ int upload_packed(csamd_sharded_device *s, const csamd_sharded *v, unsigned long
         r.allocation_bytes = i == 0u ? rec.allocation_bytes : 0u;
         r.storage = i == 0u ? rec.storage : nullptr;
         r.view = (char *) rec.view + i * sizeof(ViewT);
+        set_record_arrays(r, views[i]);
         r.group_begin = b;
         r.group_end = e;
         r.device_id = device;

@@ -48,6 +48,49 @@ class ConvertWorkspace:
             pass
 
 
+class MajorBucketWorkspace:
+    """A csamd_compressed_major_bucket_workspace (bucket/operators/major_nnz.cuh:11-213)
+    on torch's current stream: build_plan / rebuild / build_bucketed_major_view.
+    Results alias the workspace until its next call."""
+
+    def __init__(self, device: int | None = None, stream: int | None = None):
+        lib = load()
+        self.ws = A.compressed_major_bucket_workspace()
+        lib.csamd_major_bucket_init(C.byref(self.ws))
+        dev = torch.cuda.current_device() if device is None else device
+        check(lib.csamd_major_bucket_setup(C.byref(self.ws), dev, stream if stream is not None else current_stream_handle()),
+              "major bucket setup")
+
+    def build_plan(self, src: DeviceCSR, requested_bucket_count: int) -> A.major_nnz_bucket_plan_view:
+        plan = A.major_nnz_bucket_plan_view()
+        check(load().csamd_major_bucket_build_plan(C.byref(src.view()), requested_bucket_count, C.byref(self.ws), C.byref(plan)),
+              "build_plan")
+        return plan
+
+    def rebuild(self, src: DeviceCSR, plan) -> A.compressed_major_bucket_result:
+        out = A.compressed_major_bucket_result()
+        check(load().csamd_major_bucket_rebuild_typed(C.byref(src.view()), src.value_type, C.byref(plan), C.byref(self.ws),
+                                                      C.byref(out)), "rebuild")
+        return out
+
+    def build_bucketed_major_view(self, src: DeviceCSR, requested_bucket_count: int) -> A.compressed_major_bucket_result:
+        out = A.compressed_major_bucket_result()
+        check(load().csamd_build_bucketed_major_view_typed(C.byref(src.view()), src.value_type, requested_bucket_count,
+                                                           C.byref(self.ws), C.byref(out)), "build_bucketed_major_view")
+        return out
+
+    def close(self):
+        if self.ws is not None:
+            load().csamd_major_bucket_clear(C.byref(self.ws))
+            self.ws = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 # ---------------------------------------------------------------------------
 # Blocked-ELL
 # ---------------------------------------------------------------------------

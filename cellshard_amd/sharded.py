@@ -236,3 +236,33 @@ class ShardedDevice:
                                                      C.byref(pack) if pack is not None else None, int(out_shard), int(in_shard),
                                                      device, stream, 1 if drop_host_after_upload else 0,
                                                      1 if drop_host_after_release else 0), "swap_shard")
+
+
+class ShardMajorBucket:
+    """A csamd_sharded_major_bucket_workspace (bucket/operators/sharded_major_nnz.cuh:35-273):
+    the nnz bucket plan and rebuild of one resident shard of a compressed
+    sharded matrix.  Results alias the workspace until its next call."""
+
+    def __init__(self, device: int = 0, stream=None):
+        self.ws = A.sharded_major_bucket_workspace()
+        load().csamd_sharded_major_bucket_init(C.byref(self.ws))
+        check(load().csamd_sharded_major_bucket_setup(C.byref(self.ws), device, stream), "shard bucket setup")
+
+    def build(self, state: ShardedDevice, view: Sharded, shard_id: int, requested_bucket_count: int,
+              value_type: int = A.VALUE_F16) -> A.sharded_major_bucket_result:
+        out = A.sharded_major_bucket_result()
+        check(load().csamd_build_bucketed_shard_major_view_typed(C.byref(state.s), C.byref(view.m), int(shard_id), value_type,
+                                                                 requested_bucket_count, C.byref(self.ws), C.byref(out)),
+              "build_bucketed_shard_major_view")
+        return out
+
+    def close(self):
+        if self.ws is not None:
+            load().csamd_sharded_major_bucket_clear(C.byref(self.ws))
+            self.ws = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1020,4 +1020,49 @@ inline hipError_t swap_shard(sharded_device<MatrixT> *s, sharded<MatrixT> *v, co
 
 } // namespace device
 
+// ---------------------------------------------------------------------------
+// Workspace-owned major-nnz bucketing (bucket/operators/major_nnz.cuh:11-213,
+// sharded_major_nnz.cuh:35-273).  Result views alias the workspace.
+namespace bucket {
+
+using compressed_major_bucket_workspace = csamd_compressed_major_bucket_workspace;
+using compressed_major_bucket_result = csamd_compressed_major_bucket_result;
+using sharded_major_bucket_workspace = csamd_sharded_major_bucket_workspace;
+using sharded_major_bucket_result = csamd_sharded_major_bucket_result;
+
+inline void init(compressed_major_bucket_workspace *ws) { csamd_major_bucket_init(ws); }
+inline void clear(compressed_major_bucket_workspace *ws) { csamd_major_bucket_clear(ws); }
+// as the reference: the workspace creates its own non-blocking stream
+inline int setup(compressed_major_bucket_workspace *ws, int device) { return csamd_major_bucket_setup(ws, device, nullptr); }
+inline int reserve(compressed_major_bucket_workspace *ws, types::dim_t major_dim, types::nnz_t nnz, types::idx_t bucket_count) {
+    return csamd_major_bucket_reserve(ws, major_dim, nnz, bucket_count);
+}
+inline int build_plan(const device::compressed_view *src, types::idx_t requested_bucket_count, compressed_major_bucket_workspace *ws,
+                      major_nnz_bucket_plan_view *out) {
+    return csamd_major_bucket_build_plan(src, requested_bucket_count, ws, out);
+}
+inline int rebuild(const device::compressed_view *src, const major_nnz_bucket_plan_view *plan, compressed_major_bucket_workspace *ws,
+                   compressed_major_bucket_result *out) {
+    return csamd_major_bucket_rebuild(src, plan, ws, out);
+}
+inline int build_bucketed_major_view(const device::compressed_view *src, types::idx_t requested_bucket_count,
+                                     compressed_major_bucket_workspace *ws, compressed_major_bucket_result *out) {
+    return csamd_build_bucketed_major_view(src, requested_bucket_count, ws, out);
+}
+
+inline void init(sharded_major_bucket_workspace *ws) { csamd_sharded_major_bucket_init(ws); }
+inline void clear(sharded_major_bucket_workspace *ws) { csamd_sharded_major_bucket_clear(ws); }
+inline int setup(sharded_major_bucket_workspace *ws, int device) { return csamd_sharded_major_bucket_setup(ws, device, nullptr); }
+inline int reserve_parts(sharded_major_bucket_workspace *ws, unsigned long part_count) {
+    return csamd_sharded_major_bucket_reserve_parts(ws, part_count);
+}
+inline int build_bucketed_shard_major_view(const device::sharded_device<sparse::compressed> *state, const sharded<sparse::compressed> *view,
+                                           unsigned long shard_id, types::idx_t requested_bucket_count,
+                                           sharded_major_bucket_workspace *ws, sharded_major_bucket_result *out) {
+    return csamd_build_bucketed_shard_major_view(reinterpret_cast<const csamd_sharded_device *>(state), c_sharded(view), shard_id,
+                                                 requested_bucket_count, ws, out);
+}
+
+} // namespace bucket
+
 } // namespace cellshard

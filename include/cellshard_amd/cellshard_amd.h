@@ -876,6 +876,71 @@ int csamd_rebuild_compressed_major_order_raw(const unsigned int *d_src_major_ptr
                                              unsigned int *d_inverse_major_order, void *d_scan_tmp,
                                              size_t scan_tmp_bytes, csamd_stream_t stream);
 
+/* Workspace-owned major-nnz bucketing (bucket/operators/major_nnz.cuh:11-213):
+ * packed grow-only device blocks for the per-major counts, the sort and
+ * scan scratch and the rebuilt compressed arrays.  Result views alias the
+ * workspace until its next call; work runs on the workspace stream.  The
+ * fields up to d_scan_tmp are the reference's; owns_stream and
+ * value_bytes_capacity (rebuilt value bytes per entry reserved: the reference
+ * stores 2-byte values only) are extensions. */
+typedef struct CSAMD_ALIGN16 csamd_compressed_major_bucket_workspace {
+    int device;
+    csamd_stream_t stream;
+    unsigned int major_capacity;
+    unsigned int nnz_capacity;
+    unsigned int bucket_capacity;
+    size_t sort_capacity;
+    size_t scan_capacity;
+    void *d_major_block;
+    void *d_rebuilt_block;
+    unsigned int *d_major_nnz;
+    unsigned int *d_major_nnz_sorted;
+    unsigned int *d_major_order_in;
+    unsigned int *d_major_order_out;
+    unsigned int *d_inverse_major_order;
+    unsigned int *d_bucket_offsets;
+    unsigned int *d_rebuilt_major_ptr;
+    unsigned int *d_rebuilt_minor_idx;
+    void *d_rebuilt_val;
+    void *d_sort_tmp;
+    void *d_scan_tmp;
+    int owns_stream;
+    unsigned int value_bytes_capacity;
+} csamd_compressed_major_bucket_workspace;
+
+/* major_nnz.cuh:40-43 */
+typedef struct CSAMD_ALIGN16 csamd_compressed_major_bucket_result {
+    csamd_compressed_view rebuilt;
+    csamd_major_nnz_bucket_plan_view plan;
+} csamd_compressed_major_bucket_result;
+
+/* major_nnz.cuh:45-65: init / clear / setup (stream 0: an owned non-blocking stream) */
+void csamd_major_bucket_init(csamd_compressed_major_bucket_workspace *ws);
+void csamd_major_bucket_clear(csamd_compressed_major_bucket_workspace *ws);
+int csamd_major_bucket_setup(csamd_compressed_major_bucket_workspace *ws, int device, csamd_stream_t stream);
+/* major_nnz.cuh:67-143 (2-byte values); _typed reserves value_type's bytes per entry */
+int csamd_major_bucket_reserve(csamd_compressed_major_bucket_workspace *ws, unsigned int major_dim, unsigned int nnz,
+                               unsigned int bucket_count);
+int csamd_major_bucket_reserve_typed(csamd_compressed_major_bucket_workspace *ws, unsigned int major_dim, unsigned int nnz,
+                                     unsigned int bucket_count, unsigned int value_type);
+/* major_nnz.cuh:145-177: stable ascending nnz order and equal-count bucket
+ * offsets of src's major axis; out aliases the workspace (inverse order null). */
+int csamd_major_bucket_build_plan(const csamd_compressed_view *src, unsigned int requested_bucket_count,
+                                  csamd_compressed_major_bucket_workspace *ws, csamd_major_nnz_bucket_plan_view *out);
+/* major_nnz.cuh:179-202: src's segments in plan order (rebuilt row i = src
+ * row major_order[i]) and the inverse order; 2-byte values, or value_type's. */
+int csamd_major_bucket_rebuild(const csamd_compressed_view *src, const csamd_major_nnz_bucket_plan_view *plan,
+                               csamd_compressed_major_bucket_workspace *ws, csamd_compressed_major_bucket_result *out);
+int csamd_major_bucket_rebuild_typed(const csamd_compressed_view *src, unsigned int value_type,
+                                     const csamd_major_nnz_bucket_plan_view *plan, csamd_compressed_major_bucket_workspace *ws,
+                                     csamd_compressed_major_bucket_result *out);
+/* major_nnz.cuh:204-210: build_plan then rebuild */
+int csamd_build_bucketed_major_view(const csamd_compressed_view *src, unsigned int requested_bucket_count,
+                                    csamd_compressed_major_bucket_workspace *ws, csamd_compressed_major_bucket_result *out);
+int csamd_build_bucketed_major_view_typed(const csamd_compressed_view *src, unsigned int value_type,
+                                          unsigned int requested_bucket_count, csamd_compressed_major_bucket_workspace *ws,
+                                          csamd_compressed_major_bucket_result *out);
+
 /* Blocked-ELL row-block bucket plan: key = (valid slots << 32) | hash of the
  * valid block columns, stable ascending order.  Writes order and the sorted
  * widths (device, [row_block_count]).  bucket/operators/blocked_ell_bucket.cuh:77-96,149-210 */
@@ -905,11 +970,6 @@ typedef struct CSAMD_ALIGN16 csamd_filtered_blocked_ell_stats {
     double live_fill_ratio;
     size_t dead_value_bytes;
 } csamd_filtered_blocked_ell_stats;
-
-typedef struct CSAMD_ALIGN16 csamd_compressed_major_bucket_result {
-    csamd_compressed_view rebuilt;
-    csamd_major_nnz_bucket_plan_view plan;
-} csamd_compressed_major_bucket_result;
 
 typedef struct CSAMD_ALIGN16 csamd_filtered_blocked_ell_result {
     csamd_compressed_view filtered;
@@ -1342,6 +1402,54 @@ int csamd_sharded_device_stage_partition(csamd_sharded_device *s, csamd_sharded 
 int csamd_sharded_device_swap_shard(csamd_sharded_device *s, csamd_sharded *view, unsigned int format,
                                     const csamd_cspack_file *pack, unsigned long out_shard, unsigned long in_shard, int device,
                                     csamd_stream_t stream, int drop_host_after_upload, int drop_host_after_release);
+
+/* Shard bucketing over the partitions of one shard resident on the device
+ * (bucket/operators/sharded_major_nnz.cuh:35-135): the base workspace plus
+ * the shard's part pointer tables (pinned host, device).  tables_event
+ * (extension) orders rewrites of the pinned tables after the copies that
+ * read them. */
+typedef struct CSAMD_ALIGN16 csamd_sharded_major_bucket_workspace {
+    csamd_compressed_major_bucket_workspace base;
+    unsigned long part_capacity;
+    void *h_part_block;
+    void *d_part_block;
+    const unsigned int **h_part_major_ptr;
+    const unsigned int **h_part_minor_idx;
+    const void **h_part_val;
+    unsigned int *h_part_row_offsets;
+    const unsigned int **d_part_major_ptr;
+    const unsigned int **d_part_minor_idx;
+    const void **d_part_val;
+    unsigned int *d_part_row_offsets;
+    void *tables_event;
+} csamd_sharded_major_bucket_workspace;
+
+/* sharded_major_nnz.cuh:53-57 */
+typedef struct CSAMD_ALIGN16 csamd_sharded_major_bucket_result {
+    csamd_compressed_view rebuilt;
+    csamd_major_nnz_bucket_plan_view plan;
+    unsigned long shard_id;
+} csamd_sharded_major_bucket_result;
+
+/* sharded_major_nnz.cuh:59-87 */
+void csamd_sharded_major_bucket_init(csamd_sharded_major_bucket_workspace *ws);
+void csamd_sharded_major_bucket_clear(csamd_sharded_major_bucket_workspace *ws);
+int csamd_sharded_major_bucket_setup(csamd_sharded_major_bucket_workspace *ws, int device, csamd_stream_t stream);
+/* sharded_major_nnz.cuh:89-135 */
+int csamd_sharded_major_bucket_reserve_parts(csamd_sharded_major_bucket_workspace *ws, unsigned long part_count);
+/* sharded_major_nnz.cuh:137-273 with major_nnz_raw.cuh:122-186,252-320: the
+ * bucket plan and rebuild of shard shard_id of `view` (compressed by row)
+ * from its partitions resident in `state` on the workspace's device, as
+ * one matrix of the shard's rows (row i of the shard = row i -
+ * partition_offsets[p] of partition p).  2-byte values, or value_type's. */
+int csamd_build_bucketed_shard_major_view(const csamd_sharded_device *state, const csamd_sharded *view, unsigned long shard_id,
+                                          unsigned int requested_bucket_count, csamd_sharded_major_bucket_workspace *ws,
+                                          csamd_sharded_major_bucket_result *out);
+int csamd_build_bucketed_shard_major_view_typed(const csamd_sharded_device *state, const csamd_sharded *view,
+                                                unsigned long shard_id, unsigned int value_type,
+                                                unsigned int requested_bucket_count, csamd_sharded_major_bucket_workspace *ws,
+                                                csamd_sharded_major_bucket_result *out);
+
 
 /* ---------------------------------------------------------------------------
  * Synthetic CSR generator for measurement and tests (not a reference path).
