@@ -61,6 +61,9 @@ PROTOTYPES = {
         P(A.compressed_major_bucket_result)]),
     "csamd_build_bucketed_major_view_typed": (i32, [P(A.compressed_view), u32, u32, P(A.compressed_major_bucket_workspace),
         P(A.compressed_major_bucket_result)]),
+    "csamd_repack_sharded_compressed_to_blocked_ell": (i32, [P(A.sharded), u32, C.c_ulong, P(A.sharded)]),
+    "csamd_repack_sharded_compressed_to_blocked_ell_auto": (i32, [P(A.sharded), P(u32), u32, C.c_ulong, P(A.sharded),
+        P(A.blocked_ell_tune_result)]),
     "csamd_sharded_major_bucket_init": (None, [P(A.sharded_major_bucket_workspace)]),
     "csamd_sharded_major_bucket_clear": (None, [P(A.sharded_major_bucket_workspace)]),
     "csamd_sharded_major_bucket_setup": (i32, [P(A.sharded_major_bucket_workspace), i32, vp]),

@@ -949,4 +949,115 @@ int csamd_sharded_device_swap_shard(csamd_sharded_device *s, csamd_sharded *view
     return 1;
 }
 
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Repack a sharded CSR into a sharded Blocked-ELL of parts of
+// target_block_rows row blocks (convert/blocked_ell_from_compressed.cuh:480-600).
+// The rows of a part are stitched from the source partitions into one host
+// CSR (a copy per partition span, not per row), which the GPU converter turns
+// into the part.
+// ---------------------------------------------------------------------------
+
+namespace csamd {
+namespace {
+
+struct stitched_csr {
+    std::vector<unsigned int> ptr, idx;
+    std::vector<uint16_t> val;
+    csamd_host_compressed h;
+};
+
+// Rows [r0, r1) of src (compressed by row) as one CSR.
+int stitch_rows(const csamd_sharded *src, unsigned long r0, unsigned long r1, stitched_csr &out) {
+    const unsigned long rows = r1 - r0;
+    CSAMD_REQUIRE(rows <= 0xffffffffull, "repack: %lu rows in one part", rows);
+    out.ptr.assign(rows + 1u, 0u);
+    unsigned long long nnz = 0ull;
+    // first pass: the row pointers, partition span by partition span
+    for (unsigned long p = r0 < r1 ? csamd_sharded_find_partition(src, r0) : src->num_partitions; p < src->num_partitions; ++p) {
+        const unsigned long pb = src->partition_offsets[p], pe = pb + src->partition_rows[p];
+        if (pb >= r1) break;
+        const csamd_host_compressed *part = (const csamd_host_compressed *) src->parts[p];
+        CSAMD_REQUIRE(part != nullptr, "repack: partition %lu is not loaded", p);
+        CSAMD_REQUIRE(part->axis == CSAMD_COMPRESSED_BY_ROW, "repack: partition %lu is not compressed by row", p);
+        const unsigned long a = pb > r0 ? pb : r0, b = pe < r1 ? pe : r1;
+        for (unsigned long r = a; r < b; ++r) {
+            const unsigned long local = r - pb;
+            nnz += part->majorPtr[local + 1u] - part->majorPtr[local];
+            CSAMD_REQUIRE(nnz <= 0xffffffffull, "repack: part of more than 2^32 entries");
+            out.ptr[r - r0 + 1u] = (unsigned int) nnz;
+        }
+    }
+    out.idx.resize(nnz);
+    out.val.resize(nnz);
+    for (unsigned long p = r0 < r1 ? csamd_sharded_find_partition(src, r0) : src->num_partitions; p < src->num_partitions; ++p) {
+        const unsigned long pb = src->partition_offsets[p], pe = pb + src->partition_rows[p];
+        if (pb >= r1) break;
+        const csamd_host_compressed *part = (const csamd_host_compressed *) src->parts[p];
+        const unsigned long a = pb > r0 ? pb : r0, b = pe < r1 ? pe : r1;
+        if (a >= b) continue;
+        const unsigned int sb = part->majorPtr[a - pb], se = part->majorPtr[b - pb];
+        const unsigned int ob = out.ptr[a - r0];
+        if (se > sb) {
+            std::memcpy(out.idx.data() + ob, part->minorIdx + sb, (size_t) (se - sb) * 4u);
+            std::memcpy(out.val.data() + ob, part->val + sb, (size_t) (se - sb) * 2u);
+        }
+    }
+    std::memset(&out.h, 0, sizeof(out.h));
+    out.h.rows = (unsigned int) rows;
+    out.h.cols = (unsigned int) src->cols;
+    out.h.nnz = (unsigned int) nnz;
+    out.h.axis = CSAMD_COMPRESSED_BY_ROW;
+    out.h.majorPtr = out.ptr.data();
+    out.h.minorIdx = nnz != 0u ? out.idx.data() : nullptr;
+    out.h.val = nnz != 0u ? (csamd_half *) out.val.data() : nullptr;
+    return 1;
+}
+
+} // namespace
+} // namespace csamd
+
+extern "C" {
+
+int csamd_repack_sharded_compressed_to_blocked_ell(const csamd_sharded *src, unsigned int block_size, unsigned long target_block_rows,
+                                                   csamd_sharded *dst) {
+    CSAMD_REQUIRE(src != nullptr && dst != nullptr && block_size != 0u, "repack_sharded_compressed_to_blocked_ell: bad arguments");
+    CSAMD_REQUIRE(src->cols <= 0xffffffffull, "repack_sharded_compressed_to_blocked_ell: %lu columns", src->cols);
+    const unsigned long rows_per_part = target_block_rows == 0ul ? src->rows : target_block_rows * (unsigned long) block_size;
+    csamd_sharded_clear(dst, CSAMD_FORMAT_BLOCKED_ELL);
+    dst->cols = src->cols;
+    for (unsigned long r0 = 0ul; r0 < src->rows;) {
+        const unsigned long r1 = rows_per_part == 0ul || r0 + rows_per_part >= src->rows ? src->rows : r0 + rows_per_part;
+        stitched_csr part_src;
+        if (!stitch_rows(src, r0, r1, part_src)) return 0;
+        csamd_host_blocked_ell *part = (csamd_host_blocked_ell *) csamd_host_part_alloc(CSAMD_FORMAT_BLOCKED_ELL);
+        CSAMD_REQUIRE(part != nullptr, "repack_sharded_compressed_to_blocked_ell: out of host memory");
+        if (!csamd_host_blocked_ell_from_compressed(&part_src.h, block_size, part) ||
+            !csamd_sharded_append_partition(dst, CSAMD_FORMAT_BLOCKED_ELL, part)) {
+            csamd_host_blocked_ell_clear(part);
+            std::free(part);
+            return 0;
+        }
+        r0 = r1;
+    }
+    return 1;
+}
+
+int csamd_repack_sharded_compressed_to_blocked_ell_auto(const csamd_sharded *src, const unsigned int *candidates,
+                                                        unsigned int candidate_count, unsigned long target_block_rows,
+                                                        csamd_sharded *dst, csamd_blocked_ell_tune_result *picked) {
+    CSAMD_REQUIRE(src != nullptr && dst != nullptr && candidates != nullptr && candidate_count != 0u,
+                  "repack_sharded_compressed_to_blocked_ell_auto: bad arguments");
+    // the block size is chosen over the whole matrix, as the reference does
+    stitched_csr whole;
+    if (!stitch_rows(src, 0ul, src->rows, whole)) return 0;
+    csamd_blocked_ell_tune_result tune;
+    std::memset(&tune, 0, sizeof(tune));
+    if (!csamd_host_choose_blocked_ell_block_size(&whole.h, candidates, candidate_count, &tune)) return 0;
+    if (!csamd_repack_sharded_compressed_to_blocked_ell(src, tune.block_size, target_block_rows, dst)) return 0;
+    if (picked != nullptr) *picked = tune;
+    return 1;
+}
+
 } // extern "C"

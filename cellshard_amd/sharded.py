@@ -180,6 +180,25 @@ class Sharded:
         return host_part(self.format, addr) if addr else None
 
 
+def repack_to_blocked_ell(src: "Sharded", block_size: int | None = None, target_block_rows: int = 0, candidates=None):
+    """repack_sharded_compressed_to_blocked_ell(_auto) (convert/blocked_ell_from_compressed.cuh:480-600):
+    a new Sharded(FORMAT_BLOCKED_ELL) of parts of target_block_rows x block_size
+    rows; with block_size None the size is chosen from `candidates` over the
+    whole matrix.  Returns (dst, picked tune result or None)."""
+    dst = Sharded(A.FORMAT_BLOCKED_ELL)
+    lib = load()
+    if block_size is not None:
+        check(lib.csamd_repack_sharded_compressed_to_blocked_ell(C.byref(src.m), int(block_size), int(target_block_rows),
+                                                                 C.byref(dst.m)), "repack_sharded_compressed_to_blocked_ell")
+        return dst, None
+    cand = np.ascontiguousarray(candidates, dtype=np.uint32)
+    tune = A.blocked_ell_tune_result()
+    check(lib.csamd_repack_sharded_compressed_to_blocked_ell_auto(C.byref(src.m), cand.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                                                  len(cand), int(target_block_rows), C.byref(dst.m), C.byref(tune)),
+          "repack_sharded_compressed_to_blocked_ell_auto")
+    return dst, tune
+
+
 class ShardedDevice:
     """Owns a csamd_sharded_device (residency table + allocation cache)."""
 

@@ -1021,6 +1021,21 @@ inline hipError_t swap_shard(sharded_device<MatrixT> *s, sharded<MatrixT> *v, co
 } // namespace device
 
 // ---------------------------------------------------------------------------
+// Sharded CSR -> sharded Blocked-ELL repack (convert/blocked_ell_from_compressed.cuh:480-600).
+namespace convert {
+inline int repack_sharded_compressed_to_blocked_ell(const sharded<sparse::compressed> *src, unsigned int block_size,
+                                                    unsigned long target_block_rows, sharded<sparse::blocked_ell> *dst) {
+    return csamd_repack_sharded_compressed_to_blocked_ell(c_sharded(src), block_size, target_block_rows, c_sharded(dst));
+}
+inline int repack_sharded_compressed_to_blocked_ell_auto(const sharded<sparse::compressed> *src, const unsigned int *candidates,
+                                                         unsigned int candidate_count, unsigned long target_block_rows,
+                                                         sharded<sparse::blocked_ell> *dst, blocked_ell_tune_result *picked = nullptr) {
+    return csamd_repack_sharded_compressed_to_blocked_ell_auto(c_sharded(src), candidates, candidate_count, target_block_rows,
+                                                               c_sharded(dst), picked);
+}
+} // namespace convert
+
+// ---------------------------------------------------------------------------
 // Workspace-owned major-nnz bucketing (bucket/operators/major_nnz.cuh:11-213,
 // sharded_major_nnz.cuh:35-273).  Result views alias the workspace.
 namespace bucket {

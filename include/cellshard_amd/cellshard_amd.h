@@ -1403,6 +1403,19 @@ int csamd_sharded_device_swap_shard(csamd_sharded_device *s, csamd_sharded *view
                                     const csamd_cspack_file *pack, unsigned long out_shard, unsigned long in_shard, int device,
                                     csamd_stream_t stream, int drop_host_after_upload, int drop_host_after_release);
 
+/* convert/blocked_ell_from_compressed.cuh:480-551: dst (cleared first; its
+ * parts are host Blocked-ELL the matrix owns) gets the rows of src (host CSR
+ * partitions, compressed by row) in parts of target_block_rows x block_size
+ * rows (0: one part), each converted as blocked_ell_from_compressed does
+ * (on the GPU, through the host-struct converter). */
+int csamd_repack_sharded_compressed_to_blocked_ell(const csamd_sharded *src, unsigned int block_size, unsigned long target_block_rows,
+                                                   csamd_sharded *dst);
+/* :553-600: the block size chosen over the whole matrix
+ * (choose_blocked_ell_block_size), then the repack. */
+int csamd_repack_sharded_compressed_to_blocked_ell_auto(const csamd_sharded *src, const unsigned int *candidates,
+                                                        unsigned int candidate_count, unsigned long target_block_rows,
+                                                        csamd_sharded *dst, csamd_blocked_ell_tune_result *picked);
+
 /* Shard bucketing over the partitions of one shard resident on the device
  * (bucket/operators/sharded_major_nnz.cuh:35-135): the base workspace plus
  * the shard's part pointer tables (pinned host, device).  tables_event
