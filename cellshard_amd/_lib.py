@@ -46,6 +46,8 @@ PROTOTYPES = {
         P(A.group_mask_config_view), P(A.row_feature_mask_view), P(A.sparse_group_filter_params), P(A.sparse_group_reduce_result)]),
     "csamd_compute_sparse_group_reduce_sliced_ell_typed": (i32, [P(A.sliced_ell_view), u32, P(A.sparse_group_reduce_workspace),
         P(A.group_mask_config_view), P(A.row_feature_mask_view), P(A.sparse_group_filter_params), P(A.sparse_group_reduce_result)]),
+    "csamd_label_reduce_pipeline_capture": (i32, [P(A.pipeline_partition), u32, P(A.label_reduce_query), i32,
+        P(A.label_reduce_graph)]),
     "csamd_major_bucket_init": (None, [P(A.compressed_major_bucket_workspace)]),
     "csamd_major_bucket_clear": (None, [P(A.compressed_major_bucket_workspace)]),
     "csamd_major_bucket_setup": (i32, [P(A.compressed_major_bucket_workspace), i32, vp]),

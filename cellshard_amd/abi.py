@@ -314,6 +314,13 @@ SERVE_SUM, SERVE_SUMSQ, SERVE_NNZ = 0, 1, 2
 label_reduce_partition = _aligned16("label_reduce_partition", [
     ("layout", c_u32), ("reserved", c_u32 * 3), ("compressed", compressed_view), ("sliced_ell", sliced_ell_view),
 ])
+
+pipeline_partition = _aligned16("pipeline_partition", [
+    ("layout", c_u32), ("slice_rows", c_u32), ("block_size", c_u32), ("ell_width", c_u32),
+    ("staged_from", C.POINTER(host_compressed)), ("d_nnz", c_void_p), ("slots_capacity", C.c_uint64),
+    ("_align16", C.c_uint64),  # the C compiler places the 16-byte-aligned view at offset 48
+    ("compressed", compressed_view), ("sliced_ell", sliced_ell_view), ("blocked_ell", blocked_ell_view),
+])
 label_reduce_graph = _aligned16("label_reduce_graph", [("device", C.c_int), ("state", c_void_p)])
 partition_tune_result = _aligned16("partition_tune_result", [
     ("layout", c_u32), ("bell_block_size", c_u32), ("bell_ell_width", c_u32), ("sell_slice_rows", c_u32),
@@ -371,6 +378,7 @@ ALL_STRUCTS = {
     "csamd_filtered_blocked_ell_stats": filtered_blocked_ell_stats,
     "csamd_compressed_major_bucket_result": compressed_major_bucket_result,
     "csamd_compressed_major_bucket_workspace": compressed_major_bucket_workspace,
+    "csamd_pipeline_partition": pipeline_partition,
     "csamd_sharded_major_bucket_workspace": sharded_major_bucket_workspace,
     "csamd_sharded_major_bucket_result": sharded_major_bucket_result,
     "csamd_filtered_blocked_ell_result": filtered_blocked_ell_result,

@@ -41,6 +41,7 @@ HIP_SOURCES = [
     "sharded.hip",
     "group_reduce.hip",
     "bucket_ops.hip",
+    "pipeline_convert.hip",
 ]
 
 HIPCC_FLAGS = [

@@ -29,6 +29,7 @@
 //   5. a finish kernel sums, per (g, f), the slots of the workgroups that
 //      hold label g, applies the feature scale, and derives mean/variance.
 #include "common.hh"
+#include "pipeline.hh"
 #include "stream.hh"
 
 #include <cstdlib>
@@ -1283,6 +1284,7 @@ struct lr_graph_state {
     hipEvent_t done;
     int launched;
     int device;
+    pl_state pipe;  // the pipeline's conversion tables (zero for a reduce-only graph)
 };
 
 void free_private(label_private *P) {
@@ -1494,6 +1496,13 @@ int csamd_label_reduce_graph_result(csamd_label_reduce_graph *graph, csamd_label
     CSAMD_HIP(hipEventSynchronize(G->done), "label reduce graph sync");
     unsigned int counts[2] = {0u, 0u};
     CSAMD_HIP(hipMemcpy(counts, G->plan.gstart + G->plan.G, sizeof(counts), hipMemcpyDeviceToHost), "D2H graph counts");
+    if (G->pipe.block != nullptr) {
+        unsigned int err = 0u;
+        CSAMD_HIP(hipMemcpy(&err, G->pipe.error, 4u, hipMemcpyDeviceToHost), "D2H pipeline error word");
+        CSAMD_REQUIRE(err == 0u, "label_reduce pipeline: %s",
+                      (err & 1u) ? "a row block holds more block columns than its Blocked-ELL ell width"
+                                 : "a Sliced-ELL layout needs more slots than its arrays hold");
+    }
     fill_result(G->plan, counts[0], counts[1], out);
     return 1;
 }
@@ -1507,8 +1516,78 @@ void csamd_label_reduce_graph_destroy(csamd_label_reduce_graph *graph) {
     if (G->graph != nullptr) (void) hipGraphDestroy(G->graph);
     if (G->done != nullptr) (void) hipEventDestroy(G->done);
     free_private(&G->priv);
+    pl_free(&G->pipe);
     std::free(G);
     graph->state = nullptr;
+}
+
+int csamd_label_reduce_pipeline_capture(const csamd_pipeline_partition *parts, unsigned int partition_count,
+                                        const csamd_label_reduce_query *query, int device, csamd_label_reduce_graph *graph) {
+    CSAMD_REQUIRE(graph != nullptr && query != nullptr && parts != nullptr, "label_reduce_pipeline_capture: null argument");
+    CSAMD_REQUIRE(partition_count >= 1u && partition_count <= LR_MAX_PARTS, "label_reduce_pipeline_capture: 1..%u partitions",
+                  LR_MAX_PARTS);
+    std::memset(graph, 0, sizeof(*graph));
+    const int vb = value_type_bytes(query->value_type);
+    // the reduce's partitions: Sliced-ELL ones after conversion, the rest as CSR
+    std::vector<part_in> v;
+    v.reserve(partition_count);
+    for (unsigned int p = 0; p < partition_count; ++p) {
+        const csamd_pipeline_partition &P = parts[p];
+        CSAMD_REQUIRE(P.layout == CSAMD_LAYOUT_COMPRESSED || P.layout == CSAMD_LAYOUT_SLICED_ELL || P.layout == CSAMD_LAYOUT_BLOCKED_ELL,
+                      "label_reduce_pipeline_capture: partition %u layout %u", p, P.layout);
+        CSAMD_REQUIRE(P.compressed.axis == CSAMD_COMPRESSED_BY_ROW && (P.compressed.rows == 0u || P.compressed.majorPtr != nullptr),
+                      "label_reduce_pipeline_capture: partition %u needs a row-major device CSR", p);
+        if (P.staged_from != nullptr) {
+            const csamd_host_compressed &h = *P.staged_from;
+            CSAMD_REQUIRE(h.rows == P.compressed.rows && h.nnz == P.compressed.nnz && h.axis == CSAMD_COMPRESSED_BY_ROW &&
+                              h.majorPtr != nullptr && (h.nnz == 0u || (h.minorIdx != nullptr && h.val != nullptr)),
+                          "label_reduce_pipeline_capture: partition %u staging source does not match its device CSR", p);
+        }
+        v.push_back(P.layout == CSAMD_LAYOUT_SLICED_ELL ? sell_part(&P.sliced_ell) : csr_part(&P.compressed));
+    }
+    CSAMD_HIP(hipSetDevice(device), "hipSetDevice label reduce pipeline");
+    lr_graph_state *G = (lr_graph_state *) std::calloc(1u, sizeof(lr_graph_state));
+    CSAMD_REQUIRE(G != nullptr, "label_reduce_pipeline_capture: host allocation failed");
+    G->device = device;
+    graph->device = device;
+    graph->state = G;
+    hipStream_t cs = nullptr;
+    int ok = plan_label_reduce(v, query->value_type, query, &G->priv, G->plan) &&
+             pl_prepare(parts, partition_count, query->value_type, &G->pipe) &&
+             hip_ok(hipEventCreateWithFlags(&G->done, hipEventDisableTiming), "hipEventCreate pipeline");
+    if (ok && !G->plan.inline_parts)
+        ok = hip_ok(hipMemcpy(G->plan.d_parts, G->priv.parts_host, G->plan.parts_bytes, hipMemcpyHostToDevice), "H2D pipeline partitions");
+    if (ok) ok = hip_ok(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking), "hipStreamCreate capture");
+    if (ok) ok = hip_ok(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    if (ok) {
+        int enq = 1;
+        // 1. staging: pinned host CSR arrays into the partitions' device arrays
+        for (unsigned int p = 0; enq && p < partition_count; ++p) {
+            const csamd_pipeline_partition &P = parts[p];
+            if (P.staged_from == nullptr) continue;
+            const csamd_host_compressed &h = *P.staged_from;
+            enq = hip_ok(hipMemcpyAsync(P.compressed.majorPtr, h.majorPtr, ((size_t) h.rows + 1u) * 4u, hipMemcpyHostToDevice, cs),
+                         "H2D staged row pointers");
+            if (enq && h.nnz != 0u)
+                enq = hip_ok(hipMemcpyAsync(P.compressed.minorIdx, h.minorIdx, (size_t) h.nnz * 4u, hipMemcpyHostToDevice, cs),
+                             "H2D staged indices") &&
+                      hip_ok(hipMemcpyAsync(P.compressed.val, h.val, (size_t) h.nnz * (size_t) vb, hipMemcpyHostToDevice, cs),
+                             "H2D staged values");
+        }
+        // 2. conversion of every Sliced-ELL / Blocked-ELL partition; 3. the reduce
+        enq = enq && pl_enqueue(G->pipe, cs) && enqueue_label_reduce(G->plan, &G->priv, cs, false, false);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(cs, &g);  // always ends the capture
+        G->graph = g;
+        ok = enq && hip_ok(e, "hipStreamEndCapture");
+    }
+    if (ok) ok = hip_ok(hipGraphInstantiate(&G->exec, G->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+    if (cs != nullptr) (void) hipStreamDestroy(cs);
+    if (!ok) {
+        csamd_label_reduce_graph_destroy(graph);
+        return 0;
+    }
+    return 1;
 }
 
 int csamd_label_reduce_set_timing(csamd_label_reduce_workspace *ws, int enable) {

@@ -997,6 +997,102 @@ class LabelReduceGraph:
             pass
 
 
+class LabelReducePipeline:
+    """Staging -> conversion -> reduce of batched partitions captured into one
+    HIP graph (csamd_label_reduce_pipeline_capture).
+
+    parts: the partitions' device CSRs (conversion sources; with `staged`,
+    also the staging destinations).  layouts: per partition a dict from
+    tune_partitions (or the same keys): layout "compressed", "sliced_ell"
+    (sell_slice_rows, sell_slice_count, sell_padded_slots) or "blocked_ell"
+    (bell_block_size, bell_ell_width).  staged: True to stage each CSR from a
+    pinned host copy inside the graph (made here from the device CSR).  The
+    Sliced-ELL / Blocked-ELL outputs are allocated here and filled by every
+    launch; sell(i) / bell(i) give them as device layouts."""
+
+    def __init__(self, parts, layouts, labels: torch.Tensor, group_count: int, row_keep: torch.Tensor | None = None,
+                 feature_keep: torch.Tensor | None = None, staged: bool = False, moments: bool = True, device: int | None = None):
+        if not parts or len(parts) != len(layouts):
+            raise ValueError("one layout per partition required")
+        vt = parts[0].value_type
+        if any(p.value_type != vt for p in parts):
+            raise ValueError("partitions must share the value type")
+        dev = default_device() if device is None else torch.device("cuda", device)
+        bits = torch_bits_dtype(vt)
+        self.parts, self.layouts = list(parts), list(layouts)
+        self._sell, self._bell, self._host = {}, {}, []
+        arr = (A.pipeline_partition * len(parts))()
+        for i, (p, lay) in enumerate(zip(parts, layouts)):
+            e = arr[i]
+            e.compressed = p.view()
+            name = lay["layout"]
+            if name == "sliced_ell":
+                S = min(int(lay["sell_slice_rows"]), p.rows)
+                ns = -(-p.rows // S)
+                slots = int(lay["sell_padded_slots"])
+                u32 = lambda n: torch.empty(max(n, 1), dtype=torch.int32, device=dev)
+                sell = DeviceSlicedELL(p.rows, p.cols, p.nnz, ns, S, u32(ns + 1), u32(ns), u32(ns), u32(slots),
+                                       torch.empty(max(slots, 1), dtype=bits, device=dev), vt)
+                self._sell[i] = (sell, slots)
+                e.layout, e.slice_rows, e.slots_capacity, e.sliced_ell = A.LAYOUT_SLICED_ELL, S, slots, sell.view()
+            elif name == "blocked_ell":
+                bs, W = int(lay["bell_block_size"]), int(lay["bell_ell_width"])
+                rb = -(-p.rows // bs)
+                bell = DeviceBlockedELL(p.rows, p.cols, 0, bs, W * bs, torch.empty(max(rb * W, 1), dtype=torch.int32, device=dev),
+                                        torch.empty(max(p.rows * W * bs, 1), dtype=bits, device=dev), vt)
+                nnz = torch.zeros(1, dtype=torch.int32, device=dev)
+                self._bell[i] = (bell, nnz)
+                e.layout, e.block_size, e.ell_width, e.blocked_ell, e.d_nnz = A.LAYOUT_BLOCKED_ELL, bs, W, bell.view(), ptr(nnz)
+            elif name == "compressed":
+                e.layout = A.LAYOUT_COMPRESSED
+            else:
+                raise ValueError(f"partition {i}: layout {name}")
+            if staged:
+                hp = p.major_ptr.cpu().pin_memory()
+                hi = p.minor_idx.cpu().pin_memory()
+                hv = p.val.cpu().pin_memory()
+                h = A.host_compressed(rows=p.rows, cols=p.cols, nnz=p.nnz, axis=A.COMPRESSED_BY_ROW)
+                h.majorPtr, h.minorIdx, h.val = hp.data_ptr(), hi.data_ptr() if p.nnz else None, hv.data_ptr() if p.nnz else None
+                self._host.append((hp, hi, hv, h))
+                e.staged_from = C.pointer(h)
+        self._table = arr
+        self._keep = (labels, row_keep, feature_keep)
+        q = LabelReduce._query(vt, labels, group_count, row_keep, feature_keep, None, moments)
+        self.g = A.label_reduce_graph()
+        check(load().csamd_label_reduce_pipeline_capture(arr, len(parts), C.byref(q), dev.index, C.byref(self.g)),
+              "label_reduce_pipeline_capture")
+
+    def launch(self, stream: int | None = None) -> None:
+        check(load().csamd_label_reduce_graph_launch(C.byref(self.g), stream if stream is not None else current_stream_handle()),
+              "label_reduce_graph_launch")
+
+    def result(self):
+        res = A.label_reduce_result()
+        check(load().csamd_label_reduce_graph_result(C.byref(self.g), C.byref(res)), "label_reduce_graph_result")
+        return res
+
+    def sell(self, i) -> DeviceSlicedELL:
+        return self._sell[i][0]
+
+    def bell(self, i) -> DeviceBlockedELL:
+        """The converted Blocked-ELL of partition i with its stored nonzero count (synchronizes)."""
+        bell, nnz = self._bell[i]
+        torch.cuda.synchronize()
+        bell.nnz = int(nnz.cpu().numpy().view(np.uint32)[0])
+        return bell
+
+    def close(self):
+        if self.g is not None:
+            load().csamd_label_reduce_graph_destroy(C.byref(self.g))
+            self.g = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def _copy_d2h(host_u8: torch.Tensor, dev_ptr: int, nbytes: int) -> None:
     """Copy nbytes from a raw device pointer into a host uint8 tensor."""
     hip = _hip()

@@ -444,6 +444,40 @@ typedef struct CSAMD_ALIGN16 csamd_label_reduce_graph {
 
 int csamd_label_reduce_graph_capture(const csamd_label_reduce_partition *partitions, unsigned int partition_count,
                                      const csamd_label_reduce_query *query, int device, csamd_label_reduce_graph *graph);
+/* Staging -> conversion -> reduce in one captured graph (project extension:
+ * the batched-partition pipeline).  Per partition: the layout the reduce
+ * streams after conversion, and where its CSR comes from.
+ *   - staged_from (optional): a host CSR in pinned memory whose arrays the
+ *     graph copies into `compressed`'s device arrays on every launch.
+ *   - layout CSAMD_LAYOUT_SLICED_ELL: the graph converts `compressed` into
+ *     `sliced_ell` (arrays preallocated by the caller for slice_rows and the
+ *     padded slot count the layout tuner reports: slice_row_offsets
+ *     [slices + 1], slice_widths / slice_slot_offsets [slices], col_idx / val
+ *     [slots]); all such partitions in one batched conversion; the reduce
+ *     streams the Sliced-ELL.  A layout that needs more slots than
+ *     slots_capacity is not written past it and fails the graph's result.
+ *   - layout CSAMD_LAYOUT_BLOCKED_ELL: the graph converts `compressed` into
+ *     `blocked_ell` (block_size in {8, 16, 32} and ell_cols = ell_width x
+ *     block_size from the tuner; blockColIdx [row blocks x ell_width], val
+ *     [rows x ell_cols]) in one batched conversion, as blocked_ell_from_compressed
+ *     does, and writes the stored nonzero count to *d_nnz (device); the
+ *     reduce streams the CSR (the batched reduce reads row-major layouts).
+ *   - layout CSAMD_LAYOUT_COMPRESSED: reduced as staged. */
+typedef struct CSAMD_ALIGN16 csamd_pipeline_partition {
+    unsigned int layout;
+    unsigned int slice_rows;
+    unsigned int block_size;
+    unsigned int ell_width;
+    const struct csamd_host_compressed *staged_from;
+    unsigned int *d_nnz;
+    unsigned long long slots_capacity;  /* entries sliced_ell.col_idx / val hold */
+    csamd_compressed_view compressed;
+    csamd_sliced_ell_view sliced_ell;
+    csamd_blocked_ell_view blocked_ell;
+} csamd_pipeline_partition;
+
+int csamd_label_reduce_pipeline_capture(const csamd_pipeline_partition *partitions, unsigned int partition_count,
+                                        const csamd_label_reduce_query *query, int device, csamd_label_reduce_graph *graph);
 int csamd_label_reduce_graph_launch(csamd_label_reduce_graph *graph, csamd_stream_t stream);
 /* Waits for the last launch, then fills out (kept rows / features included). */
 int csamd_label_reduce_graph_result(csamd_label_reduce_graph *graph, csamd_label_reduce_result *out);
