@@ -764,8 +764,8 @@ def run_batched_partitions(args, log):
 
     from cellshard_amd import abi as A
     from cellshard_amd.device import to_dev_u32, to_dev_u8
-    from cellshard_amd.ops import (ConvertWorkspace, LabelReduce, LabelReduceGraph, sliced_ell_from_compressed, synth_csr,
-                                   tune_partitions)
+    from cellshard_amd.ops import (ConvertWorkspace, LabelReduce, LabelReduceGraph, LabelReducePipeline, sliced_ell_from_compressed,
+                                   synth_csr, tune_partitions)
 
     dev = torch.device("cuda", 0)
     cols, G, FK, n = 36_601, 32, 3000, args.batched_partitions
@@ -790,6 +790,7 @@ def run_batched_partitions(args, log):
     layouts = {k: sum(1 for t in tune if t["layout"] == k) for k in ("compressed", "sliced_ell", "blocked_ell")}
     chosen_bytes = sum(min(t["csr_bytes"], t["sell_bytes"], t["bell_bytes"]) for t in tune)
     log(f"batched: tuner {tune_ms:.1f} ms, layouts {layouts}")
+    csr_parts = list(parts)  # the pipeline below converts from these
     conv_ms = 0.0
     for i, t in enumerate(tune):
         if t["layout"] == "sliced_ell":
@@ -799,7 +800,7 @@ def run_batched_partitions(args, log):
             torch.cuda.synchronize()
             conv_ms += (time.perf_counter() - s0) * 1e3
     # Blocked-ELL partitions are reduced from their CSR (the batched reduce
-    # streams compressed and sliced rows); none are chosen on this workload
+    # streams compressed and sliced rows)
     torch.cuda.empty_cache()
 
     props_g = rng.dirichlet(np.full(G, 0.5))
@@ -853,6 +854,79 @@ def run_batched_partitions(args, log):
     big = float(np.max(eager["sumsq"], initial=0.0))
     graph.close()
 
+    # the pipeline graph: staging of partitions from pinned host copies (the
+    # first ones, up to 4 GB), conversion of every partition whose layout is
+    # Sliced-ELL or Blocked-ELL in the batched conversion launches, then the
+    # reduce of all partitions.  The tuner keeps CSR for these ragged rows,
+    # so the conversion leg is exercised by forcing the partitions with the
+    # least Sliced-ELL padding to their Sliced-ELL candidate (up to 12 GB of
+    # output) and the smallest Blocked-ELL candidates to Blocked-ELL (up to
+    # 2 GB); those are reduced as converted (Sliced-ELL) or from CSR.
+    pipeline = None
+    try:
+        lay = [dict(t) for t in tune]
+        sell_budget, bell_budget, forced = 12e9, 2e9, {"sliced_ell": 0, "blocked_ell": 0}
+        conv_out = 0
+        for i in sorted(range(n), key=lambda k: tune[k]["sell_bytes"] / tune[k]["csr_bytes"]):
+            if lay[i]["layout"] == "compressed" and tune[i]["sell_bytes"] <= sell_budget:
+                lay[i]["layout"] = "sliced_ell"
+                sell_budget -= tune[i]["sell_bytes"]
+                conv_out += tune[i]["sell_bytes"]
+                forced["sliced_ell"] += 1
+        for i in sorted(range(n), key=lambda k: tune[k]["bell_bytes"]):
+            if lay[i]["layout"] == "compressed" and tune[i]["bell_bytes"] <= bell_budget:
+                lay[i]["layout"] = "blocked_ell"
+                bell_budget -= tune[i]["bell_bytes"]
+                conv_out += tune[i]["bell_bytes"]
+                forced["blocked_ell"] += 1
+        conv_in = sum(csr_parts[i].format_bytes() for i in range(n) if lay[i]["layout"] != "compressed")
+        staged, sb = [], 0
+        for p in csr_parts:
+            staged.append(sb + p.format_bytes() <= 4e9)
+            sb += p.format_bytes() if staged[-1] else 0
+        pl = LabelReducePipeline(csr_parts, lay, d_labels, G, d_rk, d_fk, staged=staged)
+        for _ in range(max(1, args.warmup)):
+            pl.launch(stream)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            pl.launch(stream)
+        torch.cuda.synchronize()
+        pl_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+        pres = LabelReduce.fetch(pl.result())
+        pl_same = all(np.array_equal(np.asarray(eager[k]), np.asarray(pres[k]))
+                      for k in ("nnz", "group_rows", "kept_feature_ids", "kept_rows", "kept_features"))
+        pl_rel = max(float(np.max(np.abs(eager[k] - pres[k]) / np.maximum(np.abs(eager[k]), 1e-300), initial=0.0))
+                     for k in ("sum", "sumsq"))
+        pl.close()
+        del pl
+        torch.cuda.empty_cache()
+        # the same graph without staging: the conversion's share
+        pl = LabelReducePipeline(csr_parts, lay, d_labels, G, d_rk, d_fk, staged=False)
+        pl.launch(stream)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            pl.launch(stream)
+        torch.cuda.synchronize()
+        cv_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+        pl.close()
+        del pl
+        torch.cuda.empty_cache()
+        conv_only = cv_ms - graph_ms
+        stage_only = pl_ms - cv_ms
+        pipeline = {"ms_per_replay": round(pl_ms, 3), "staged_partitions": int(sum(staged)), "staged_bytes": int(sb),
+                    "converted": forced, "conversion_read_bytes": int(conv_in), "conversion_written_bytes": int(conv_out),
+                    "reduce_only_graph_ms": round(graph_ms, 3), "convert_and_reduce_graph_ms": round(cv_ms, 3),
+                    "conversion_ms": round(conv_only, 3),
+                    "conversion_gb_s": round((conv_in + conv_out) / max(conv_only * 1e-3, 1e-9) / 1e9, 1),
+                    "staging_ms": round(stage_only, 3), "staging_gb_s": round(sb / max(stage_only * 1e-3, 1e-9) / 1e9, 1),
+                    "counts_identical_to_eager": bool(pl_same), "max_rel_diff_sums_vs_eager": pl_rel}
+        log(f"batched pipeline: {json.dumps(pipeline)}")
+    except Exception as e:  # reported, the rest of the section stands
+        log(f"batched pipeline failed: {type(e).__name__}: {e}")
+        pipeline = {"error": f"{type(e).__name__}: {e}"}
+
     # one reduce call per partition (the reference's per-partition launches)
     offs = np.concatenate([[0], np.cumsum(rows_p)])
     torch.cuda.synchronize()
@@ -882,10 +956,11 @@ def run_batched_partitions(args, log):
                   "counts_identical_to_eager": bool(counts_same), "max_rel_diff_sums_vs_eager": rel,
                   "max_sumsq": big, "max_sumsq_over_2p53": round(big / 2.0 ** 53, 3)},
         "per_partition_calls": {"ms_per_pass": round(per_ms, 3), "gb_s": round(b_alg / (per_ms * 1e-3) / 1e9, 1)},
+        "pipeline_graph": pipeline,
         "group_rows_match": group_ok,
     }
     log(f"batched: {json.dumps(res)}")
-    del parts, d_labels, d_rk, d_fk
+    del parts, csr_parts, d_labels, d_rk, d_fk
     torch.cuda.empty_cache()
     return res
 

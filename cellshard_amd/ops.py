@@ -1005,22 +1005,24 @@ class LabelReducePipeline:
     also the staging destinations).  layouts: per partition a dict from
     tune_partitions (or the same keys): layout "compressed", "sliced_ell"
     (sell_slice_rows, sell_slice_count, sell_padded_slots) or "blocked_ell"
-    (bell_block_size, bell_ell_width).  staged: True to stage each CSR from a
-    pinned host copy inside the graph (made here from the device CSR).  The
+    (bell_block_size, bell_ell_width).  staged: True (every partition) or a
+    list of flags, to stage those CSRs from pinned host copies inside the
+    graph (made here from the device CSRs).  The
     Sliced-ELL / Blocked-ELL outputs are allocated here and filled by every
     launch; sell(i) / bell(i) give them as device layouts."""
 
     def __init__(self, parts, layouts, labels: torch.Tensor, group_count: int, row_keep: torch.Tensor | None = None,
-                 feature_keep: torch.Tensor | None = None, staged: bool = False, moments: bool = True, device: int | None = None):
+                 feature_keep: torch.Tensor | None = None, staged=False, moments: bool = True, device: int | None = None):
         if not parts or len(parts) != len(layouts):
             raise ValueError("one layout per partition required")
+        staged = [bool(staged)] * len(parts) if isinstance(staged, bool) else [bool(x) for x in staged]
         vt = parts[0].value_type
         if any(p.value_type != vt for p in parts):
             raise ValueError("partitions must share the value type")
         dev = default_device() if device is None else torch.device("cuda", device)
         bits = torch_bits_dtype(vt)
         self.parts, self.layouts = list(parts), list(layouts)
-        self._sell, self._bell, self._host = {}, {}, []
+        self._sell, self._bell, self._host = {}, {}, {}
         arr = (A.pipeline_partition * len(parts))()
         for i, (p, lay) in enumerate(zip(parts, layouts)):
             e = arr[i]
@@ -1047,13 +1049,13 @@ class LabelReducePipeline:
                 e.layout = A.LAYOUT_COMPRESSED
             else:
                 raise ValueError(f"partition {i}: layout {name}")
-            if staged:
+            if staged[i]:
                 hp = p.major_ptr.cpu().pin_memory()
                 hi = p.minor_idx.cpu().pin_memory()
                 hv = p.val.cpu().pin_memory()
                 h = A.host_compressed(rows=p.rows, cols=p.cols, nnz=p.nnz, axis=A.COMPRESSED_BY_ROW)
                 h.majorPtr, h.minorIdx, h.val = hp.data_ptr(), hi.data_ptr() if p.nnz else None, hv.data_ptr() if p.nnz else None
-                self._host.append((hp, hi, hv, h))
+                self._host[i] = (hp, hi, hv, h)
                 e.staged_from = C.pointer(h)
         self._table = arr
         self._keep = (labels, row_keep, feature_keep)
