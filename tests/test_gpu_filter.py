@@ -63,7 +63,7 @@ def masks(rows, cols, seed, drop_rows=3, drop_cols=2):
 
 
 @pytest.mark.parametrize("vt", [F16, F32, FP8])
-@pytest.mark.parametrize("bs", [4, 16])
+@pytest.mark.parametrize("bs", [4, 16, 32])
 def test_filter_matches_oracle(ws, fws, vt, bs):
     rows, cols = 701, 1300
     bell, gb, gv = make_bell(rows, cols, bs, vt, 30 + bs + vt, ws)

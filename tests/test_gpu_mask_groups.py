@@ -239,7 +239,7 @@ def test_compressed_views_off_unit_boundaries(ws, vt, shift_idx, shift_val):
         _compare(fetch(ws, res), O.group_reduce_bell(rows, cols, bs, ell, bci, bv, vt, rk, fk, fgm, G, filt_t), G, vt)
 
 
-@pytest.mark.parametrize("bs", [4, 8, 16])
+@pytest.mark.parametrize("bs", [4, 8, 16, 24, 32, 64])
 def test_reoptimize_random_parity(gpu, bs):
     """manual_reoptimize_masked_sparse on random Blocked-ELL (16-byte chunk
     reads when the block size is a multiple of 8, per value otherwise) with
