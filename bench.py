@@ -45,6 +45,7 @@ def parse_args():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-ops", action="store_true")
     ap.add_argument("--reduce-only", action="store_true", help="only the reduce steps (for profiling)")
+    ap.add_argument("--ops-only", action="store_true", help="only the headline matrix's conversions (for profiling)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 2 GiB device copy (reference point for HBM byte counters)")
     ap.add_argument("--skip-serve", action="store_true", help="skip the serving-loop measurement")
@@ -965,6 +966,64 @@ def run_batched_partitions(args, log):
     return res
 
 
+def run_conversions(src, ws, log):
+    """The headline matrix's conversions, each timed after an untimed first
+    call: 12-bucket major-nnz plan + log2 offsets, the nnz-sorted rebuild,
+    CSR -> Sliced-ELL 32 of the rebuild, and CSR -> CSC."""
+    import numpy as np
+    import torch
+
+    from cellshard_amd.ops import (TransposeScratch, build_major_nnz_bucket_plan, compressed_transpose, log2_bucket_offsets,
+                                   rebuild_compressed_major_order, sliced_ell_from_compressed)
+
+    R, Cc, nnz = src.rows, src.cols, src.nnz
+    ops = {}
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        s = time.perf_counter()
+        out = fn()
+        torch.cuda.synchronize()
+        return out, (time.perf_counter() - s) * 1e3
+
+    csr_bytes = (R + 1) * 4 + nnz * 6
+    # each op once untimed first (kernel code loading, allocator growth)
+    plan = build_major_nnz_bucket_plan(src, 12)
+    log2_bucket_offsets(plan["major_nnz_sorted"], 12, 5)
+    del plan
+    plan, ms = timed(lambda: build_major_nnz_bucket_plan(src, 12))
+    l2, ms2 = timed(lambda: log2_bucket_offsets(plan["major_nnz_sorted"], 12, 5))
+    ops["bucket_plan_12_log2"] = {"ms": round(ms + ms2, 3), "gb_s": round(((R + 1) * 4 + 2 * R * 4) / ((ms + ms2) * 1e-3) / 1e9, 1),
+                                  "bucket_sizes": np.diff(l2.cpu().numpy().view(np.uint32).astype(np.int64)).tolist()}
+    (reb, inv) = rebuild_compressed_major_order(src, plan["major_order"])
+    del reb, inv
+    (reb, inv), ms = timed(lambda: rebuild_compressed_major_order(src, plan["major_order"]))
+    ops["bucket_rebuild"] = {"ms": round(ms, 3), "gb_s": round(2 * csr_bytes / (ms * 1e-3) / 1e9, 1)}
+    del inv
+    sell = sliced_ell_from_compressed(reb, 32, ws)
+    del sell
+    sell, ms = timed(lambda: sliced_ell_from_compressed(reb, 32, ws))
+    ops["csr_to_sell_s32_sorted"] = {"ms": round(ms, 3), "gb_s": round((csr_bytes + (2 * sell.slice_count + 1) * 4 +
+                                                                        sell.total_slots * 6) / (ms * 1e-3) / 1e9, 1),
+                                     "slots_per_nnz": round(sell.total_slots / nnz, 3)}
+    del sell, reb
+    torch.cuda.empty_cache()
+    scratch = TransposeScratch()
+    compressed_transpose(src, scratch)  # warm
+    reps = []
+    for _ in range(3):
+        csc, ms = timed(lambda: compressed_transpose(src, scratch))
+        reps.append(ms)
+        del csc
+    ms = min(reps)
+    ops["csr_to_csc"] = {"ms": round(ms, 3), "gb_s": round((csr_bytes + (Cc + 1) * 4 + nnz * 6) / (ms * 1e-3) / 1e9, 1),
+                         "runs_ms": [round(x, 3) for x in reps]}
+    del scratch
+    torch.cuda.empty_cache()
+    log(f"ops: {json.dumps(ops)}")
+    return ops
+
+
 def guarded(section, args, log):
     """Runs one of the sections reported beside the headline; a failure is
     logged and recorded in the JSON line instead of ending the run."""
@@ -1022,9 +1081,17 @@ def main() -> int:
 
     from cellshard_amd import abi as A
     from cellshard_amd.device import to_dev_u32, to_dev_u8
-    from cellshard_amd.ops import (ConvertWorkspace, LabelReduce, TransposeScratch, build_major_nnz_bucket_plan,
-                                   compressed_transpose, log2_bucket_offsets, rebuild_compressed_major_order,
-                                   sliced_ell_from_compressed, synth_csr)
+    from cellshard_amd.ops import ConvertWorkspace, LabelReduce, synth_csr
+
+    if args.ops_only:
+        torch.cuda.set_device(0)
+        if args.calibrate_copy:
+            calibrate_copy(log)
+        ws = ConvertWorkspace()
+        src = synth_csr(args.rows, args.cols, median=2000.0, sigma=0.6, seed=args.seed, value_type=A.VALUE_F16, ws=ws)
+        torch.cuda.synchronize()
+        print(json.dumps({"ops": run_conversions(src, ws, log)}), flush=True)
+        return 0
 
     torch.cuda.set_device(0)
     dev = torch.device("cuda", 0)
@@ -1125,44 +1192,8 @@ def main() -> int:
 
     # ---------------- conversions of the same workload ----------------
     ops = {}
-
-    def timed(fn):
-        torch.cuda.synchronize()
-        s = time.perf_counter()
-        out = fn()
-        torch.cuda.synchronize()
-        return out, (time.perf_counter() - s) * 1e3
-
     if not args.skip_ops and not args.reduce_only:
-        csr_bytes = (R + 1) * 4 + nnz * 6
-        # each op once untimed first (kernel code loading, allocator growth)
-        plan = build_major_nnz_bucket_plan(src, 12)
-        log2_bucket_offsets(plan["major_nnz_sorted"], 12, 5)
-        del plan
-        plan, ms = timed(lambda: build_major_nnz_bucket_plan(src, 12))
-        l2, ms2 = timed(lambda: log2_bucket_offsets(plan["major_nnz_sorted"], 12, 5))
-        ops["bucket_plan_12_log2"] = {"ms": round(ms + ms2, 3), "gb_s": round(((R + 1) * 4 + 2 * R * 4) / ((ms + ms2) * 1e-3) / 1e9, 1),
-                                      "bucket_sizes": np.diff(l2.cpu().numpy().view(np.uint32).astype(np.int64)).tolist()}
-        (reb, inv) = rebuild_compressed_major_order(src, plan["major_order"])
-        del reb, inv
-        (reb, inv), ms = timed(lambda: rebuild_compressed_major_order(src, plan["major_order"]))
-        ops["bucket_rebuild"] = {"ms": round(ms, 3), "gb_s": round(2 * csr_bytes / (ms * 1e-3) / 1e9, 1)}
-        del inv
-        sell = sliced_ell_from_compressed(reb, 32, ws)
-        del sell
-        sell, ms = timed(lambda: sliced_ell_from_compressed(reb, 32, ws))
-        ops["csr_to_sell_s32_sorted"] = {"ms": round(ms, 3), "gb_s": round((csr_bytes + (2 * sell.slice_count + 1) * 4 +
-                                                                            sell.total_slots * 6) / (ms * 1e-3) / 1e9, 1),
-                                         "slots_per_nnz": round(sell.total_slots / nnz, 3)}
-        del sell, reb
-        torch.cuda.empty_cache()
-        scratch = TransposeScratch()
-        compressed_transpose(src, scratch)  # warm
-        csc, ms = timed(lambda: compressed_transpose(src, scratch))
-        ops["csr_to_csc"] = {"ms": round(ms, 3), "gb_s": round((csr_bytes + (Cc + 1) * 4 + nnz * 6) / (ms * 1e-3) / 1e9, 1)}
-        del csc, scratch
-        torch.cuda.empty_cache()
-        log(f"ops: {json.dumps(ops)}")
+        ops = run_conversions(src, ws, log)
 
     # ---------------- headline parity + CPU baseline (oracle) ----------------
     # The C oracle over EVERY row (row chunks on the host threads, FP64

@@ -160,11 +160,14 @@ __global__ void __launch_bounds__(TR_COUNT_THREADS) tr_count_kernel(unsigned int
 // K1 (packed, uDim < 65536 and the tile's tables fit in LDS): two u16
 // counters per LDS word (a strictly ascending row adds at most 1 per
 // segment, so a tile's count stays <= TR_ROWS), and the band starts as u16
-// offsets from the row start in LDS, written out coalesced at the end.
-// Scattered 4-byte stores straight into rbt reach HBM as partial lines;
-// this way every rbt line is written once and whole.  A tile with a row
-// that is not strictly ascending may overflow a u16 counter: flags[0] is
-// then set and the host counts again with tr_count_kernel.
+// offsets from the row start in LDS: an entry whose band differs from the
+// previous entry's (or that starts its row) stores its offset at its band
+// (one predicated store, no loop); bands a row skips keep a sentinel and
+// take the next band's start in a backward pass per row at the end, when
+// the table is written out coalesced (every rbt line once and whole).  A
+// tile with a row that is not strictly ascending may overflow a u16
+// counter: flags[0] is then set and the host counts again with
+// tr_count_kernel.
 //
 // The tile's entries are walked flat, not row by row, in groups of 4
 // consecutive entries (one 16-byte load per lane and group): pass p covers
@@ -185,20 +188,18 @@ __global__ void __launch_bounds__(TR_COUNT_THREADS) tr_count_packed_kernel(unsig
     constexpr unsigned int NW = TR_COUNT_THREADS / 64u;
     extern __shared__ unsigned int lds_words[];
     const unsigned int hw = (uDim + 1u) >> 1;
+    constexpr unsigned short NONE = 0xffffu;                          // band without entries in the row (so far)
     unsigned int *hist = lds_words;                                  // [hw] two u16 counters each
     unsigned int *rp = hist + hw;                                    // [TR_ROWS + 1] row starts (+ pad)
-    unsigned short *off = (unsigned short *) (rp + TR_ROWS + 4u);    // [(bands + 1) * TR_ROWS]
+    unsigned short *off = (unsigned short *) (rp + TR_ROWS + 4u);    // [bands][TR_ROWS] band starts - row start
     const unsigned int t = blockIdx.x;
     const unsigned int r0 = t * TR_ROWS;
     const unsigned int nr = cDim - r0 < TR_ROWS ? cDim - r0 : TR_ROWS;
     const unsigned int tid = threadIdx.x;
     for (unsigned int w = tid; w < hw; w += blockDim.x) hist[w] = 0u;
+    for (unsigned int w = tid; w < bands * (TR_ROWS / 2u); w += blockDim.x) ((unsigned int *) off)[w] = 0xffffffffu;
     for (unsigned int r = tid; r <= nr; r += blockDim.x) rp[r] = ptr[r0 + r];
     __syncthreads();
-    // empty rows: every band starts at offset 0
-    for (unsigned int r = tid; r < nr; r += blockDim.x)
-        if (rp[r] == rp[r + 1u])
-            for (unsigned int k = 0; k <= bands; ++k) off[k * TR_ROWS + r] = 0u;
     const unsigned int E0 = rp[0], E1 = rp[nr];
     const unsigned int wave = tid >> 6, lane = lane_id();
     const unsigned int g0 = E0 >> 2, g1 = (E1 + 3u) >> 2;  // groups [g0, g1) hold the tile
@@ -219,28 +220,16 @@ __global__ void __launch_bounds__(TR_COUNT_THREADS) tr_count_packed_kernel(unsig
         before = idx[gp > 0u ? gp * 4u - 1u : 0u];
     };
     // one entry: i in [E0, E1), its column cc, the previous entry's column pc
-    // (used only when i is not its row's first), row state rs / re
-    auto visit = [&](unsigned int i, unsigned int cc, unsigned int pc, unsigned int row, unsigned int rs, unsigned int re,
-                     bool &bad, bool &unsorted) {
+    // (used only when i is not its row's first), its row's start rs
+    auto visit = [&](unsigned int i, unsigned int cc, unsigned int pc, unsigned int row, unsigned int rs, bool &bad,
+                     bool &unsorted) {
         bad |= cc >= uDim;
         const unsigned int cs = cc < uDim ? cc : 0u;  // out of range: reported, not counted right
         atomicAdd(&hist[cs >> 1], 1u << ((cs & 1u) << 4));
-        // entry i opens every band in (band of entry i - 1, band of cc]
-        const int band = (int) (cs / TR_BAND);
         const bool inner = i > rs;
         unsorted |= inner && pc >= cc;
-        const int prev = inner ? (int) ((pc < uDim ? pc : uDim - 1u) / TR_BAND) : -1;
-        unsigned short *orow = off + row;
-        if (band > prev) {
-            const unsigned short o = (unsigned short) (i - rs);
-#pragma unroll 1
-            for (int k = prev + 1; k <= band; ++k) orow[k * TR_ROWS] = o;
-        }
-        if (i == re - 1u) {  // the row's last entry: the bands after it start at the row end
-            const unsigned short o = (unsigned short) (re - rs);
-#pragma unroll 1
-            for (unsigned int k = (unsigned int) band + 1u; k <= bands; ++k) orow[k * TR_ROWS] = o;
-        }
+        const unsigned int band = cs / TR_BAND;
+        if (!inner || band != (pc < uDim ? pc : 0u) / TR_BAND) off[band * TR_ROWS + row] = (unsigned short) (i - rs);
     };
     // last row with rp[row] <= i (i in [E0, E1))
     auto row_of = [&](unsigned int i) -> unsigned int {
@@ -276,7 +265,7 @@ __global__ void __launch_bounds__(TR_COUNT_THREADS) tr_count_packed_kernel(unsig
                     rs = re;
                     re = rp[row + 1u];
                 }
-                visit(i, col[e], e == 0u ? first_prev : col[e == 0u ? 0u : e - 1u], row, rs, re, bad, unsorted);
+                visit(i, col[e], e == 0u ? first_prev : col[e == 0u ? 0u : e - 1u], row, rs, bad, unsorted);
             }
         }
         if (bad) flags[1] = 1u;
@@ -308,7 +297,7 @@ __global__ void __launch_bounds__(TR_COUNT_THREADS) tr_count_packed_kernel(unsig
         bool bad = false, unsorted = false;
         for (unsigned int i = Ev > E0 ? Ev : E0; i < E1; ++i) {
             const unsigned int row = row_of(i);
-            visit(i, idx[i], idx[i > 0u ? i - 1u : 0u], row, rp[row], rp[row + 1u], bad, unsorted);
+            visit(i, idx[i], idx[i > 0u ? i - 1u : 0u], row, rp[row], bad, unsorted);
         }
         if (bad) flags[1] = 1u;
         if (unsorted) flags[0] = 1u;
@@ -316,16 +305,23 @@ __global__ void __launch_bounds__(TR_COUNT_THREADS) tr_count_packed_kernel(unsig
     __syncthreads();
     unsigned int *tile_counts = counts + (size_t) t * uDim;
     for (unsigned int c = tid; c < uDim; c += blockDim.x) tile_counts[c] = (hist[c >> 1] >> ((c & 1u) << 4)) & 0xffffu;
-    unsigned int *tile_rbt = rbt + (size_t) t * (bands + 1u) * TR_ROWS;
-    const unsigned int n = (bands + 1u) * TR_ROWS;
-    for (unsigned int j = tid; j < n; j += blockDim.x) {
-        const unsigned int rl = j & (TR_ROWS - 1u);
-        if (rl < nr) tile_rbt[j] = rp[rl] + off[j];
+    // band starts, last band first: a band the row skips starts where the
+    // next one does; band `bands` is the row end
+    if (tid < nr) {
+        unsigned int *tile_rbt = rbt + (size_t) t * (bands + 1u) * TR_ROWS + tid;
+        const unsigned int rs = rp[tid];
+        unsigned int next = rp[tid + 1u];
+        tile_rbt[(size_t) bands * TR_ROWS] = next;
+        for (unsigned int k = bands; k-- > 0u;) {
+            const unsigned int o = off[k * TR_ROWS + tid];
+            if (o != NONE) next = rs + o;
+            tile_rbt[(size_t) k * TR_ROWS] = next;
+        }
     }
 }
 
 size_t count_packed_lds_bytes(unsigned int uDim, unsigned int bands) {
-    return (size_t) ((uDim + 1u) >> 1) * 4u + (size_t) (TR_ROWS + 4u) * 4u + (size_t) (bands + 1u) * TR_ROWS * 2u;
+    return (size_t) ((uDim + 1u) >> 1) * 4u + (size_t) (TR_ROWS + 4u) * 4u + (size_t) bands * TR_ROWS * 2u;
 }
 
 // K2: counts[t][c] <- sum_{t' < t} counts[t'][c] (in place), coltot[c] = total.
@@ -406,8 +402,8 @@ __global__ void __launch_bounds__(TR_SORT_THREADS) tr_band_kernel(unsigned int c
                                                                  // different segments hit different banks
     unsigned int *seg_b = masks + TR_BAND * TR_MASK_WORDS;       // [TR_ROWS] row starts in the band
     unsigned int *seg_o = seg_b + TR_ROWS;                       // [TR_ROWS + 1] chunk offsets of the rows
-    unsigned int *lstart = seg_o + TR_ROWS + 1u;                 // [TR_BAND + 1] run starts in the staging
-    unsigned int *gcur = lstart + TR_BAND + 1u;                  // [TR_BAND] output cursors
+    unsigned int *delta = seg_o + TR_ROWS + 1u;                  // [TR_BAND + 1] output position - staging slot
+    unsigned int *gcur = delta + TR_BAND + 1u;                   // [TR_BAND] output cursors
     unsigned short *pre = (unsigned short *) (gcur + TR_BAND + 2u);  // [TR_MASK_WORDS][TR_BAND] run start + rows in lower words
     unsigned char *s_in = (unsigned char *) (pre + TR_MASK_WORDS * TR_BAND);  // [cap] batch row, input order
     unsigned short *s_rs = (unsigned short *) (s_in + cap);      // [cap] batch row | segment << 8, output order
@@ -427,7 +423,8 @@ __global__ void __launch_bounds__(TR_SORT_THREADS) tr_band_kernel(unsigned int c
     const unsigned int *rb_lo = rbt + ((size_t) t * (bands + 1u) + band) * TR_ROWS;
 
     for (unsigned int w = tid; w < TR_BAND * TR_MASK_WORDS; w += T) masks[w] = 0u;
-    if (tid < nb) gcur[tid] = out_ptr[c0 + tid] + tile_off[(size_t) t * uDim + c0 + tid];
+    // segments past uDim start at 0 and hold no entries
+    gcur[tid] = tid < nb ? out_ptr[c0 + tid] + tile_off[(size_t) t * uDim + c0 + tid] : 0u;
     {
         const unsigned int b = tid < nr ? rb_lo[tid] : 0u;
         const unsigned int n = tid < nr ? rb_lo[tid + TR_ROWS] - b : 0u;
@@ -438,9 +435,6 @@ __global__ void __launch_bounds__(TR_SORT_THREADS) tr_band_kernel(unsigned int c
         if (tid == 0u) seg_o[TR_ROWS] = total;
     }
     __syncthreads();
-    // every entry of the chunk is at or after its first row's start, and a
-    // chunk spans less than 4 GiB of the source (rows of one tile, at most
-    // 4096 bands of 256 columns): one uniform base, 32-bit offsets
     const unsigned int first = seg_b[0];
     const char *ib = (const char *) (idx + first);
     const char *vbase = (const char *) (val + first);
@@ -452,6 +446,8 @@ __global__ void __launch_bounds__(TR_SORT_THREADS) tr_band_kernel(unsigned int c
         // entries of a band, so a batch holds at least one row)
         const unsigned int base = seg_o[rb0];
         const int fits = tid >= rb0 && tid < nr && seg_o[tid + 1u] - base <= cap;
+        // (also the barrier after the previous batch: its D and E, and its
+        // reads of s_rs / s_val / delta, finish before this batch writes them)
         const unsigned int rb1 = rb0 + (unsigned int) __syncthreads_count(fits);
         const unsigned int n = seg_o[rb1] - base;
         if (tid >= rb0 && tid < rb1) {
@@ -477,10 +473,12 @@ __global__ void __launch_bounds__(TR_SORT_THREADS) tr_band_kernel(unsigned int c
         for (unsigned int k = 0; k < K; ++k)
             if (tid + k * T < n) atomicOr(&masks[(r[k] >> 5) * TR_BAND + (c[k] - c0)], 1u << (r[k] & 31u));
         __syncthreads();
-        // B: run lengths -> run starts in the staging, and each mask word's
-        // first slot (run start + rows in the lower words)
+        // B: run lengths -> run starts in the staging, each mask word's
+        // first slot (run start + rows in the lower words), and each
+        // segment's output position minus its run start
+        unsigned int cnt = 0u;
         {
-            unsigned int cw[TR_MASK_WORDS], cnt = 0u;
+            unsigned int cw[TR_MASK_WORDS];
 #pragma unroll
             for (unsigned int w = 0; w < TR_MASK_WORDS; ++w) {
                 cw[w] = (unsigned int) __popc(masks[w * TR_BAND + tid]);
@@ -488,8 +486,7 @@ __global__ void __launch_bounds__(TR_SORT_THREADS) tr_band_kernel(unsigned int c
             }
             unsigned int total = 0u;
             unsigned int o = block_exclusive_scan_u32(cnt, scan_smem, &total);
-            lstart[tid] = o;
-            if (tid == 0u) lstart[TR_BAND] = total;
+            delta[tid] = gcur[tid] - o;
 #pragma unroll
             for (unsigned int w = 0; w < TR_MASK_WORDS; ++w) {
                 pre[w * TR_BAND + tid] = (unsigned short) o;
@@ -512,17 +509,16 @@ __global__ void __launch_bounds__(TR_SORT_THREADS) tr_band_kernel(unsigned int c
         // are consecutive output positions)
         for (unsigned int k = tid; k < n; k += T) {
             const unsigned int rs = s_rs[k];
-            const unsigned int cl = rs >> 8;
-            const unsigned int pos = gcur[cl] + (k - lstart[cl]);
+            const unsigned int pos = delta[rs >> 8] + k;
             out_idx[pos] = r0 + rb0 + (rs & 255u);
             out_val[pos] = s_val[k];
         }
-        __syncthreads();
-        // E: advance the cursors, clear the masks
-        if (tid < nb) gcur[tid] += lstart[tid + 1u] - lstart[tid];
+        // E: advance the cursors, clear the masks.  No barrier before: D reads
+        // neither (gcur was folded into delta in B, the masks were last read
+        // in C); the next batch's first barrier orders E before its A.
+        gcur[tid] += cnt;
 #pragma unroll
         for (unsigned int w = 0; w < TR_MASK_WORDS; ++w) masks[w * TR_BAND + tid] = 0u;
-        __syncthreads();
         rb0 = rb1;
     }
 }
