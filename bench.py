@@ -41,7 +41,8 @@ def parse_args():
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--cpu-sample-rows", type=int, default=200_000)
     ap.add_argument("--cpu-repeats", type=int, default=5, help="single-threaded CPU baseline: best of this many runs")
-    ap.add_argument("--cpu-repeats-all", type=int, default=3, help="all-cores CPU baseline / headline check: best of this many")
+    ap.add_argument("--cpu-repeats-all", type=int, default=5, help="all-cores CPU baseline / headline check: best of this many")
+    ap.add_argument("--cpu-conv-rows", type=int, default=50_000, help="row sample of the host conversion timings")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-ops", action="store_true")
     ap.add_argument("--reduce-only", action="store_true", help="only the reduce steps (for profiling)")
@@ -84,6 +85,42 @@ def host_info() -> dict:
         affinity = None
     return {"nproc": os.cpu_count(), "affinity_cores": affinity, "omp_num_threads": os.environ.get("OMP_NUM_THREADS"),
             "cpu_model": model}
+
+
+def cpu_best(fn, repeats):
+    """Best-of-`repeats` wall seconds of fn() on the host."""
+    best = float("inf")
+    for _ in range(max(1, repeats)):
+        t0 = time.perf_counter()
+        fn()
+        best = min(best, time.perf_counter() - t0)
+    return best
+
+
+def cpu_rows_parallel(fn, rows, threads, align=1, chunks_per_thread=4):
+    """fn(a, b) over contiguous row ranges [a, b) (boundaries multiples of
+    `align`) on `threads` host threads (the oracle's ctypes calls release the
+    GIL); returns wall seconds."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    import numpy as np
+
+    n = max(1, min((rows + align - 1) // align, threads * chunks_per_thread))
+    cuts = sorted(set(int(c) for c in (np.linspace(0, (rows + align - 1) // align, n + 1).astype(np.int64) * align)))
+    cuts[-1] = rows
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(threads) as ex:
+        list(ex.map(lambda k: fn(cuts[k], cuts[k + 1]), range(len(cuts) - 1)))
+    return time.perf_counter() - t0
+
+
+def sub_csr(ptr, idx, val, a, b):
+    """Rows [a, b) of a CSR as its own CSR (row pointers rebased)."""
+    import numpy as np
+
+    ptr = np.asarray(ptr)
+    lo, hi = int(ptr[a]), int(ptr[b])
+    return (ptr[a:b + 1].astype(np.int64) - lo).astype(np.uint32), idx[lo:hi], val[lo:hi]
 
 
 def pct(a, q):
@@ -216,6 +253,26 @@ def run_serving(args, log):
     out["plain_row_scan"] = {"saturated_qps": round(len(times) / sum(times), 1), "mean_ms": round(1e3 * float(np.mean(times)), 3),
                              "queries": len(times)}
     out["speedup_saturated_qps_vs_row_scan"] = round(out["batched"]["saturated_qps"] / out["plain_row_scan"]["saturated_qps"], 1)
+    # host baseline: the per-query reduce (C oracle row scan of the shard's
+    # CSR, single-threaded) for the queries on the first query's shard
+    if not args.skip_cpu:
+        from cellshard_amd.serve import keep_threshold
+        from oracle import oracle as O
+
+        s0 = int(q_shard[0])
+        qs = [i for i in range(nq) if int(q_shard[i]) == s0][:10]
+        p_h, i_h, v_h = csr[s0].to_numpy()
+        lat_cpu = []
+        for i in qs:
+            t1 = time.perf_counter()
+            O.serve_query(int(shard_rows[s0]), cols, p_h, i_h, v_h, A.VALUE_F16, feats[i], int(q_G[i]), int(q_op[i]),
+                          label_seed=int(q_seed[i, 0]), mask_seed=int(q_seed[i, 1]), thr24=keep_threshold(float(q_dens[i])))
+            lat_cpu.append(time.perf_counter() - t1)
+        del p_h, i_h, v_h
+        out["cpu_baseline"] = {"kind": "port", "cores": 1, "shard": s0, "shard_rows": int(shard_rows[s0]), "queries": len(lat_cpu),
+                               "p50_ms": round(1e3 * pct(lat_cpu, 50), 2), "max_ms": round(1e3 * max(lat_cpu), 2),
+                               "qps": round(len(lat_cpu) / sum(lat_cpu), 2),
+                               "note": "one query at a time: a row scan of the shard's CSR with the query's seeded labels and row mask"}
     ws.close()
     log(f"serve: {json.dumps(out)}")
     return out
@@ -290,8 +347,19 @@ def run_reduced_precision(args, log):
         sub = type(src)(S, Cc, src.major_ptr[: S + 1], src.minor_idx[:end], src.val[:end], vt, src.axis)
         got = LabelReduce.fetch(lr.run(sub, d_l[:S], G, d_rk[:S], d_fk, scale))
         sc = scale.cpu().numpy() if scale is not None else None
-        ref = O.label_reduce_csr(S, Cc, p_s, idx_s, src.val[:end].cpu().numpy(), vt, labels[:S], rk[:S], fk, G, sc)
+        v_s = src.val[:end].cpu().numpy()
+        ref = O.label_reduce_csr(S, Cc, p_s, idx_s, v_s, vt, labels[:S], rk[:S], fk, G, sc)
         assert np.array_equal(got["nnz"], ref["nnz"]) and np.array_equal(got["group_rows"], ref["group_rows"])
+        cpu = None
+        if not args.skip_cpu:  # host baseline on the error sample: single-threaded and row chunks on all threads
+            nt = O.worker_threads()
+            k_s = kept[:S]
+            b_s = int(np.diff(ptr_h[: S + 1])[k_s].sum()) * (4 + vb) + (S + 1) * 4 + S * 5 + Cc * 4 + G * FK * 36
+            t1 = cpu_best(lambda: O.label_reduce_csr(S, Cc, p_s, idx_s, v_s, vt, labels[:S], rk[:S], fk, G, sc), args.cpu_repeats)
+            tn = cpu_best(lambda: O.label_reduce_csr_chunked(S, Cc, p_s, idx_s, v_s, vt, labels[:S], rk[:S], fk, G, sc, threads=nt),
+                          args.cpu_repeats_all)
+            cpu = {"kind": "port", "sample": f"first {S} rows ({end} stored entries)", "cores": 1, "s": round(t1, 4),
+                   "gb_s": round(b_s / t1 / 1e9, 3), "all_cores": {"cores": nt, "s": round(tn, 4), "gb_s": round(b_s / tn / 1e9, 3)}}
         n = np.maximum(ref["group_rows"], 1)[:, None].astype(np.float64)
         r_mean, r_var = ref["sum"] / n, np.maximum(ref["sumsq"] / n - (ref["sum"] / n) ** 2, 0)
         e = {k: rel(got[k], v).max(axis=1) for k, v in (("sum", ref["sum"]), ("sumsq", ref["sumsq"]), ("mean", r_mean), ("var", r_var))}
@@ -303,7 +371,8 @@ def run_reduced_precision(args, log):
                      "rel_err_vs_fp64_oracle": {k: {"max_over_groups": float(v.max()), "median_over_groups": float(np.median(v))}
                                                 for k, v in e.items()},
                      "storage_rel_err_sum_vs_fp32_values": {"max_over_groups": float(qm.max()),
-                                                            "median_over_groups": float(np.median(qm))}}
+                                                            "median_over_groups": float(np.median(qm))},
+                     "cpu_baseline": cpu}
         del src, scale, sub
         torch.cuda.empty_cache()
     lr.close()
@@ -410,15 +479,35 @@ def run_plain_path(args, log):
                    "transpose_ms": round(tr_ms, 3), "transpose_gb_s": round(2 * csr_bytes / (tr_ms * 1e-3) / 1e9, 1),
                    "reduce_ms": round(red_ms, 4), "reduce_gb_s": round(red_bytes / (red_ms * 1e-3) / 1e9, 1)}}
     if not args.skip_cpu:
-        t0 = time.perf_counter()
-        O.label_reduce_csr(R, Cc, ptr, idx, val, A.VALUE_F32, labels, rk, fk, G)
-        res["cpu_reduce"] = {"s": round(time.perf_counter() - t0, 3), "kind": "port", "cores": 1}
+        # host baselines on the same matrix: best of cpu_repeats single-threaded
+        # runs, and row chunks on all host threads (best of cpu_repeats_all)
+        nt = O.worker_threads()
+        t1 = cpu_best(lambda: O.label_reduce_csr(R, Cc, ptr, idx, val, A.VALUE_F32, labels, rk, fk, G), args.cpu_repeats)
+        tn = cpu_best(lambda: O.label_reduce_csr_chunked(R, Cc, ptr, idx, val, A.VALUE_F32, labels, rk, fk, G, threads=nt),
+                      args.cpu_repeats_all)
+        res["cpu_reduce"] = {"s": round(t1, 4), "gb_s": round(red_bytes / t1 / 1e9, 3), "kind": "port", "cores": 1,
+                             "all_cores": {"s": round(tn, 4), "gb_s": round(red_bytes / tn / 1e9, 3), "cores": nt},
+                             "gpu_speedup_vs_all_cores": round(tn * 1e3 / red_ms, 1)}
+        # the reference-ABI form: 16 feature groups as per-column bitmasks
+        fgm = rng.integers(0, 1 << 16, size=Cc, dtype=np.uint64).astype(np.uint32)
+        filt = (0.0, 0, 1.0, 0)
+
+        def gr_rows(a, b):
+            O.group_reduce_csr(b - a, Cc, *sub_csr(ptr, idx, val, a, b), A.VALUE_F32, rk[a:b], fk, fgm, G, filt)
+
+        gr_bytes = int(row_len[kept].sum()) * 8 + (R + 1) * 4 + R * (1 + 13 + 8 * G) + Cc * (1 + 4)
+        t1 = cpu_best(lambda: gr_rows(0, R), args.cpu_repeats)
+        tn = min(cpu_rows_parallel(gr_rows, R, nt) for _ in range(args.cpu_repeats_all))
+        res["cpu_group_reduce_reference_abi"] = {"s": round(t1, 4), "gb_s": round(gr_bytes / t1 / 1e9, 3), "kind": "port", "cores": 1,
+                                                 "all_cores": {"s": round(tn, 4), "gb_s": round(gr_bytes / tn / 1e9, 3), "cores": nt},
+                                                 "groups": G}
+        t1 = cpu_best(lambda: O.transpose(R, Cc, ptr, idx, val), args.cpu_repeats)
+        res["cpu_transpose"] = {"s": round(t1, 4), "gb_s": round(2 * csr_bytes / t1 / 1e9, 3), "kind": "port", "cores": 1,
+                                "gpu_speedup": round(t1 * 1e3 / tr_ms, 1)}
         if O.ref_available():
             v16 = val.view(np.float32).astype(np.float16)
-            t0 = time.perf_counter()
-            O.ref_blocked_ell_from_compressed(R, Cc, ptr, idx, v16, 16)
-            cpu_s = time.perf_counter() - t0
-            res["cpu_blocked_ell_16"] = {"s": round(cpu_s, 3), "kind": "reference", "cores": 1,
+            cpu_s = cpu_best(lambda: O.ref_blocked_ell_from_compressed(R, Cc, ptr, idx, v16, 16), args.cpu_repeats)
+            res["cpu_blocked_ell_16"] = {"s": round(cpu_s, 4), "kind": "reference", "cores": 1,
                                          "note": "reference convert::blocked_ell_from_compressed (CPU, fp16 values), compiled from its sources",
                                          "gpu_speedup": round(cpu_s * 1e3 / bell_ms, 1)}
     ws.close()
@@ -930,6 +1019,40 @@ def run_batched_partitions(args, log):
 
     # one reduce call per partition (the reference's per-partition launches)
     offs = np.concatenate([[0], np.cumsum(rows_p)])
+    # host baseline on a seeded subset: each partition's conversion to the
+    # layout the pipeline gave it, then its masked grouped reduce (C oracle,
+    # single-threaded, best of cpu_repeats); the rate is scaled to the whole set
+    cpu = None
+    if not args.skip_cpu:
+        from oracle import oracle as O
+
+        layout_of = [t["layout"] for t in (lay if pipeline is not None and "error" not in pipeline else tune)]
+        pick = sorted(int(i) for i in np.random.default_rng(7).choice(n, size=min(4, n), replace=False))
+        cpu_s, cpu_bytes, done = 0.0, 0, []
+        for i in pick:
+            ptr_i, idx_i, val_i = csr_parts[i].to_numpy()
+            a, b = int(offs[i]), int(offs[i + 1])
+            lay_i = layout_of[i]
+
+            def one(i=i, ptr_i=ptr_i, idx_i=idx_i, val_i=val_i, a=a, b=b, lay_i=lay_i):
+                if lay_i == "sliced_ell":
+                    O.sliced_ell_from_compressed(b - a, cols, 0, ptr_i, idx_i, val_i, tune[i]["sell_slice_rows"])
+                elif lay_i == "blocked_ell":
+                    O.blocked_ell_from_compressed(b - a, cols, ptr_i, idx_i, val_i, tune[i]["bell_block_size"])
+                O.label_reduce_csr(b - a, cols, ptr_i, idx_i, val_i, A.VALUE_F16, labels[a:b], row_keep[a:b], feature_keep, G)
+
+            cpu_s += cpu_best(one, args.cpu_repeats)
+            k_i = kept[a:b]
+            red_b = int(row_len[a:b][k_i].sum()) * 6 + (b - a + 1) * 4 + (b - a) * 5
+            conv_b = 0 if lay_i == "compressed" else int(tune[i]["csr_bytes"]) + int(
+                tune[i]["sell_bytes"] if lay_i == "sliced_ell" else tune[i]["bell_bytes"])
+            cpu_bytes += red_b + conv_b
+            done.append({"partition": i, "rows": b - a, "layout": lay_i})
+        gbs = cpu_bytes / cpu_s / 1e9
+        cpu = {"kind": "port", "cores": 1, "subset": done, "s": round(cpu_s, 3), "gb_s": round(gbs, 3),
+               "scaled_whole_set_s": round(b_alg / (gbs * 1e9), 2),
+               "note": "conversion (the partition's pipeline layout) + masked grouped reduce per partition; the whole-set "
+                       "time scales the subset's rate to the reduce's algorithmic bytes"}
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for i, p in enumerate(parts):
@@ -959,6 +1082,7 @@ def run_batched_partitions(args, log):
         "per_partition_calls": {"ms_per_pass": round(per_ms, 3), "gb_s": round(b_alg / (per_ms * 1e-3) / 1e9, 1)},
         "pipeline_graph": pipeline,
         "group_rows_match": group_ok,
+        "cpu_baseline": cpu,
     }
     log(f"batched: {json.dumps(res)}")
     del parts, csr_parts, d_labels, d_rk, d_fk
@@ -1242,11 +1366,41 @@ def main() -> int:
         cpu_s = min(st_times)
         k = kept[:S]
         b_cpu = int(row_nnz[:S][k].sum()) * 6 + (S + 1) * 4 + S * 5 + Cc * 4 + out_bytes
+        # the same matrix's conversions on the host: the 12-bucket plan over
+        # every row, Sliced-ELL 32 and CSR -> CSC over a row sample
+        cpu_ops = {}
+        t1 = cpu_best(lambda: O.log2_offsets(O.bucket_plan(p_all, R, 12)["major_nnz_sorted"], 12, 5), args.cpu_repeats)
+        cpu_ops["bucket_plan_12_log2"] = {"s": round(t1, 4), "gb_s": round(((R + 1) * 4 + 2 * R * 4) / t1 / 1e9, 3), "cores": 1,
+                                          "sample": f"all {R} rows"}
+        Sc = min(args.cpu_conv_rows, R) // 32 * 32
+        ps, is_, vs = sub_csr(p_all, idx_all, val_all, 0, Sc)
+        n_s = len(is_)
+        sell_h = O.sliced_ell_from_compressed(Sc, Cc, 0, ps, is_, vs, 32)
+        sell_bytes = (Sc + 1) * 4 + n_s * 6 + len(sell_h["col_idx"]) * 6 + (2 * len(sell_h["widths"]) + 1) * 4
+        del sell_h
+
+        def sell_rows(a, b):
+            O.sliced_ell_from_compressed(b - a, Cc, 0, *sub_csr(ps, is_, vs, a, b), 32)
+
+        t1 = cpu_best(lambda: sell_rows(0, Sc), args.cpu_repeats)
+        tn = min(cpu_rows_parallel(sell_rows, Sc, nt, align=32) for _ in range(args.cpu_repeats_all))
+        cpu_ops["csr_to_sell_s32"] = {"s": round(t1, 4), "gb_s": round(sell_bytes / t1 / 1e9, 3), "cores": 1,
+                                      "all_cores": {"s": round(tn, 4), "gb_s": round(sell_bytes / tn / 1e9, 3), "cores": nt},
+                                      "sample": f"first {Sc} rows ({n_s} stored entries)"}
+        t1 = cpu_best(lambda: O.transpose(Sc, Cc, ps, is_, vs), max(1, args.cpu_repeats - 2))
+        tr_bytes = (Sc + 1) * 4 + (Cc + 1) * 4 + n_s * 12
+        cpu_ops["csr_to_csc"] = {"s": round(t1, 4), "gb_s": round(tr_bytes / t1 / 1e9, 3), "cores": 1,
+                                 "sample": f"first {Sc} rows ({n_s} stored entries)",
+                                 "note": "one counting sort; no multi-core form in the restatement"}
+        del ps, is_, vs
         cpu = {"value": round(b_cpu / cpu_s / 1e9, 3), "unit": "GB/s", "cores": 1, "kind": "port",
                "sample": f"first {S} of {R} rows ({end} stored entries), same masks/labels; best of {len(st_times)} "
                          f"single-threaded C oracle runs ({cpu_s:.2f} s)",
                "all_cores": {"value": round(b_alg_step / min(mt_times) / 1e9, 3), "cores": nt, "s": round(min(mt_times), 3),
                              "sample": f"all {R} rows, row chunks on {nt} threads, best of {len(mt_times)}"},
+               "ops": cpu_ops,
+               "threads_note": "all-cores runs use the cores this process may run on, capped by OMP_NUM_THREADS "
+                               "(the host share of one GPU)",
                "host": host, "d2h_copy_s": round(d2h_s, 2)}
         del idx_all, val_all
         log(f"cpu baseline: {cpu}")
@@ -1294,6 +1448,8 @@ def main() -> int:
                    "algorithmic_bytes_per_step": b_alg_step, "pct_of_peak": round(100.0 * value / PEAK_HBM_GBS, 2)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
+                     "traffic_source": "profiles/label_reduce_csr_1Mx36601_pmc.txt: rocprofv3 FETCH_SIZE + WRITE_SIZE of this "
+                                       "kernel on this workload, copy-calibrated; committed, not measured in this run",
                      "kernel": "label_reduce_rows_kernel", "kernel_ms": round(k_ms, 4), "algorithmic_bytes": b_alg_kernel},
         "cpu_baseline": cpu,
         "headline_matches_oracle": headline_check,
