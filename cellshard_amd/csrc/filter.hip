@@ -342,6 +342,95 @@ __global__ void __launch_bounds__(FL_THREADS) fl_emit_kernel(csamd_blocked_ell_v
     }
 }
 
+// Gather pass, the one read of the kept value rows when every row's
+// survivors fit a fixed region: one workgroup per row, as the grouped emit
+// pass, but a row's surviving entries go in position order to its own
+// region of a scratch array (source row r at r * cap, at most cap entries;
+// a row with more sets *overflow and the caller falls back to count + emit).
+// The row's count, its source row and the output-row claim are recorded as
+// in the count pass.
+template<typename S, unsigned int U>
+__global__ void __launch_bounds__(FL_THREADS) fl_gather_kernel(csamd_blocked_ell_view src, const S *__restrict__ val, fl_masks m,
+                                                               unsigned int slot_cap, unsigned int row0, unsigned int cap,
+                                                               unsigned int *__restrict__ counts, unsigned int *__restrict__ claim,
+                                                               unsigned int *__restrict__ dup, unsigned int *__restrict__ src_row,
+                                                               unsigned int *__restrict__ overflow, unsigned int *__restrict__ t_idx,
+                                                               S *__restrict__ t_val) {
+    constexpr unsigned int PER = 16u / sizeof(S);
+    const unsigned int width = src.ell_cols / src.block_size;
+    __shared__ unsigned int smem[16];
+    extern __shared__ unsigned int lds_slots[];
+    unsigned int cur_rb = ~0u;
+    fl_slots_ref slots{nullptr, nullptr, false};
+    CSAMD_GLOBAL unsigned int *gi = gmem(t_idx);
+    CSAMD_GLOBAL S *gv = gmem(t_val);
+    for (unsigned int r = blockIdx.x; r < src.rows; r += gridDim.x) {
+        const unsigned int o = fl_out_row(m, r);
+        if (o >= m.output_rows) continue;
+        const unsigned int rb = r / src.block_size;
+        if (rb != cur_rb) {
+            __syncthreads();
+            slots = fl_slots(src, width, rb, lds_slots, slot_cap);
+            cur_rb = rb;
+            __syncthreads();
+        }
+        const S *row = val + (unsigned long long) r * src.ell_cols;
+        const unsigned long long base = (unsigned long long) (row0 + r) * cap;
+        unsigned int w = 0u;
+        for (unsigned int e0 = 0; e0 < src.ell_cols; e0 += FL_THREADS * U * PER) {
+            fl_group<S> g[U];
+#pragma unroll
+            for (unsigned int u = 0; u < U; ++u) {
+                const unsigned int e = e0 + (u * FL_THREADS + threadIdx.x) * PER;
+                g[u].q = e < src.ell_cols ? *(const uint4 *) (row + e) : make_uint4(0u, 0u, 0u, 0u);
+            }
+#pragma unroll
+            for (unsigned int u = 0; u < U; ++u) {
+                unsigned int oc[PER];
+                const unsigned int mask = fl_group_mask<S>(src, m, slots, e0 + (u * FL_THREADS + threadIdx.x) * PER, g[u], oc);
+                unsigned int total = 0u;
+                unsigned int k = w + block_exclusive_scan_u32((unsigned int) __popc(mask), smem, &total);
+#pragma unroll
+                for (unsigned int j = 0; j < PER; ++j) {
+                    if ((mask >> j) & 1u) {
+                        if (k < cap) {
+                            gi[base + k] = oc[j];
+                            gv[base + k] = g[u].v[j];
+                        }
+                        ++k;
+                    }
+                }
+                w += total;
+            }
+        }
+        if (threadIdx.x == 0u) {
+            if (atomicExch(&claim[o], 1u) != 0u) *dup = o + 1u;
+            counts[o] = w;
+            src_row[o] = row0 + r;
+            if (w > cap) *overflow = 1u;
+        }
+    }
+}
+
+// Copy pass after the gather: output row o's entries from its source row's
+// region, one wave per row.
+template<typename S>
+__global__ void fl_copy_kernel(unsigned int output_rows, unsigned int cap, const unsigned int *__restrict__ ptr,
+                               const unsigned int *__restrict__ src_row, const unsigned int *__restrict__ t_idx,
+                               const S *__restrict__ t_val, unsigned int *__restrict__ out_idx, S *__restrict__ out_val) {
+    const unsigned int lane = lane_id();
+    const unsigned int nw = blockDim.x >> 6;
+    for (unsigned int o = blockIdx.x * nw + (threadIdx.x >> 6); o < output_rows; o += gridDim.x * nw) {
+        const unsigned int b = ptr[o], n = ptr[o + 1u] - b;
+        if (n == 0u) continue;
+        const unsigned long long base = (unsigned long long) src_row[o] * cap;
+        for (unsigned int k = lane; k < n; k += 64u) {
+            gmem(out_idx)[b + k] = gmem(t_idx)[base + k];
+            gmem(out_val)[b + k] = gmem(t_val)[base + k];
+        }
+    }
+}
+
 // Workspace-owned device memory (grow-only).
 struct fl_private {
     void *ptr_block;     // counts [output_rows] | claim [output_rows] | dup, kept rows / slots | ptr [output_rows + 1] | col map | scan scratch
@@ -354,6 +443,8 @@ struct fl_private {
     size_t rebuilt_bytes;
     void *bell_block;    // repacked Blocked-ELL: blockColIdx | values
     size_t bell_bytes;
+    void *tmp_block;     // gather pass: per-row regions of surviving indices | values
+    size_t tmp_bytes;
     csamd_convert_workspace cws;  // scratch of the Blocked-ELL build
     int cws_ready;
 };
@@ -361,7 +452,7 @@ struct fl_private {
 void free_private(void *p) {
     if (p == nullptr) return;
     fl_private *st = (fl_private *) p;
-    void *blocks[5] = {st->ptr_block, st->val_block, st->bucket_block, st->rebuilt_block, st->bell_block};
+    void *blocks[6] = {st->ptr_block, st->val_block, st->bucket_block, st->rebuilt_block, st->bell_block, st->tmp_block};
     for (void *b : blocks)
         if (b != nullptr) (void) hipFree(b);
     if (st->cws_ready) {
@@ -412,6 +503,28 @@ template<typename S> struct fl_launch {
     }
 };
 
+// Gather / copy launches (rows whose value rows are whole 16-byte groups).
+template<typename S> struct fl_gather_launch {
+    static int run(const csamd_blocked_ell_view *src, const fl_masks &m, unsigned int row0, unsigned int cap, unsigned int *counts,
+                   unsigned int *claim, unsigned int *dup, unsigned int *src_row, unsigned int *overflow, unsigned int *t_idx,
+                   void *t_val, hipStream_t s) {
+        const unsigned int grid = grid_for(src->rows, 1u, (unsigned int) device_cu_count() * 16u);
+        const unsigned int width = src->ell_cols / src->block_size;
+        const unsigned int slot_cap = width <= FL_LDS_SLOTS ? width : 0u;
+        fl_gather_kernel<S, 4u><<<grid, FL_THREADS, (size_t) slot_cap * 4u, s>>>(*src, (const S *) src->val, m, slot_cap, row0, cap,
+                                                                                counts, claim, dup, src_row, overflow, t_idx, (S *) t_val);
+        return hip_ok(hipGetLastError(), "fl_gather_kernel");
+    }
+};
+template<typename S> struct fl_copy_launch {
+    static int run(unsigned int output_rows, unsigned int cap, const unsigned int *ptr, const unsigned int *src_row,
+                   const unsigned int *t_idx, const void *t_val, unsigned int *out_idx, void *out_val, hipStream_t s) {
+        fl_copy_kernel<S><<<grid_for(output_rows, FL_WAVES, (unsigned int) device_cu_count() * 16u), FL_THREADS, 0, s>>>(
+            output_rows, cap, ptr, src_row, t_idx, (const S *) t_val, out_idx, (S *) out_val);
+        return hip_ok(hipGetLastError(), "fl_copy_kernel");
+    }
+};
+
 template<template<typename> class K, typename... Args>
 int fl_dispatch(unsigned int vt, Args... args) {
     switch (value_type_bytes(vt)) {
@@ -455,7 +568,8 @@ int filter_parts(const csamd_blocked_ell_view *parts, unsigned int part_count, u
     const size_t off_ptr = off_dup + 256u;
     const size_t off_map = off_ptr + align_up(((size_t) output_rows + 1u) * 4u, 256u);
     const size_t off_keep = off_map + align_up((size_t) cols * 4u, 256u);
-    const size_t off_scan = off_keep + align_up((size_t) cols * 4u, 256u);
+    const size_t off_src = off_keep + align_up((size_t) cols * 4u, 256u);
+    const size_t off_scan = off_src + align_up((size_t) output_rows * 4u, 256u);
     if (!grow(&st->ptr_block, &st->ptr_bytes, off_scan + exclusive_scan_scratch_bytes(output_rows))) return 0;
     const size_t off_val = align_up((size_t) max_nnz * 4u, 256u);
     if (!grow(&st->val_block, &st->val_bytes, off_val + (size_t) max_nnz * (size_t) vb)) return 0;
@@ -463,7 +577,9 @@ int filter_parts(const csamd_blocked_ell_view *parts, unsigned int part_count, u
     unsigned int *counts = (unsigned int *) pb;
     unsigned int *claim = (unsigned int *) (pb + off_claim);
     unsigned int *dup = (unsigned int *) (pb + off_dup);
+    unsigned int *overflow = (unsigned int *) (pb + off_dup + 4u);  // read back with dup
     unsigned long long *d_kept = (unsigned long long *) (pb + off_dup + 64u);
+    unsigned int *src_row = (unsigned int *) (pb + off_src);
     unsigned int *ptr = (unsigned int *) (pb + off_ptr);
     unsigned int *out_idx = (unsigned int *) st->val_block;
     void *out_val = (char *) st->val_block + off_val;
@@ -495,7 +611,37 @@ int filter_parts(const csamd_blocked_ell_view *parts, unsigned int part_count, u
         }
         unsigned int *block_keep = (unsigned int *) (pb + off_keep);
         unsigned int keep_bs = 0u;  // block size block_keep currently holds (stream order keeps reuse safe)
-        auto pass = [&](bool emit) -> int {
+        // One read of the kept rows instead of two (count, then emit): when
+        // every part's value rows are whole aligned 16-byte groups, a row's
+        // survivors go to a region of cap = min(ell_cols, output_cols)
+        // entries (enough while a row's columns are distinct and the column
+        // remap is one-to-one; a row with more falls back to count + emit),
+        // and a copy pass packs the regions once the row pointers are known.
+        unsigned long long rows_all = 0ull, max_ell = 0ull;
+        bool aligned = true;
+        for (unsigned int p = 0; p < part_count; ++p) {
+            const csamd_blocked_ell_view &v = parts[p];
+            rows_all += v.rows;
+            max_ell = v.ell_cols > max_ell ? v.ell_cols : max_ell;
+            if (v.rows != 0u && v.ell_cols != 0u)
+                aligned = aligned && ((unsigned long long) v.ell_cols * (unsigned long long) vb) % 16ull == 0ull &&
+                          ((uintptr_t) v.val & 15u) == 0u;
+        }
+        const unsigned long long cap = max_ell < output_cols ? max_ell : output_cols;
+        const unsigned long long tmp_need = rows_all * cap * (4ull + (unsigned long long) vb);
+        const unsigned long long tmp_budget = 2ull * max_nnz * (4ull + (unsigned long long) vb) > (256ull << 20)
+                                                  ? 2ull * max_nnz * (4ull + (unsigned long long) vb) : (256ull << 20);
+        const bool gather = aligned && cap != 0u && tmp_need <= tmp_budget && rows_all <= 0xffffffffull;
+        unsigned int *t_idx = nullptr;
+        void *t_val = nullptr;
+        if (gather) {
+            const size_t off_tv = align_up((size_t) (rows_all * cap) * 4u, 256u);
+            if (!grow(&st->tmp_block, &st->tmp_bytes, off_tv + (size_t) (rows_all * cap) * (size_t) vb)) return 0;
+            t_idx = (unsigned int *) st->tmp_block;
+            t_val = (char *) st->tmp_block + off_tv;
+        }
+        // mode 0: count, 1: emit, 2: gather
+        auto pass = [&](int mode) -> int {
             unsigned int r0 = 0u;
             for (unsigned int p = 0; p < part_count; ++p) {
                 const csamd_blocked_ell_view &v = parts[p];
@@ -511,23 +657,49 @@ int filter_parts(const csamd_blocked_ell_view *parts, unsigned int part_count, u
                                  d_row_remap != nullptr ? d_row_remap + r0 : nullptr, d_col_remap, col_map,
                                  keep_bits ? block_keep : nullptr,
                                  keep_bits ? (unsigned int) ((cols + v.block_size - 1u) / v.block_size) : 0u};
-                if (v.rows != 0u && v.ell_cols != 0u &&
-                    !fl_dispatch<fl_launch>(value_type, &v, m, counts, claim, dup, (const unsigned int *) ptr, out_idx, out_val, emit, s))
-                    return 0;
+                if (v.rows != 0u && v.ell_cols != 0u) {
+                    const int ok = mode == 2 ? fl_dispatch<fl_gather_launch>(value_type, &v, m, r0, (unsigned int) cap, counts, claim, dup,
+                                                                             src_row, overflow, t_idx, t_val, s)
+                                             : fl_dispatch<fl_launch>(value_type, &v, m, counts, claim, dup, (const unsigned int *) ptr,
+                                                                      out_idx, out_val, mode == 1, s);
+                    if (!ok) return 0;
+                }
                 r0 += v.rows;
             }
             return 1;
         };
-        if (!pass(false)) return 0;
-        if (!exclusive_scan_u32(counts, output_rows, ptr, nullptr, pb + off_scan, exclusive_scan_scratch_bytes(output_rows), s)) return 0;
-        unsigned int h[2] = {0u, 0u};
-        CSAMD_HIP(hipMemcpyAsync(&h[0], ptr + output_rows, 4u, hipMemcpyDeviceToHost, s), "D2H filtered nnz");
-        CSAMD_HIP(hipMemcpyAsync(&h[1], dup, 4u, hipMemcpyDeviceToHost, s), "D2H filtered dup");
-        CSAMD_HIP(hipStreamSynchronize(s), "filtered blocked ell sync");
-        CSAMD_REQUIRE(h[1] == 0u, "%s: row_remap maps two kept rows to output row %u", who, h[1] - 1u);
-        live = h[0];
-        CSAMD_REQUIRE(live <= max_nnz, "%s: %u surviving entries exceed the source nnz %llu", who, live, max_nnz);
-        if (!pass(true)) return 0;
+        // counts -> row pointers; the live count, duplicate-row and overflow flags to the host
+        auto finish_counts = [&](unsigned int h[3]) -> int {
+            if (!exclusive_scan_u32(counts, output_rows, ptr, nullptr, pb + off_scan, exclusive_scan_scratch_bytes(output_rows), s))
+                return 0;
+            CSAMD_HIP(hipMemcpyAsync(&h[0], ptr + output_rows, 4u, hipMemcpyDeviceToHost, s), "D2H filtered nnz");
+            CSAMD_HIP(hipMemcpyAsync(&h[1], dup, 8u, hipMemcpyDeviceToHost, s), "D2H filtered flags");
+            CSAMD_HIP(hipStreamSynchronize(s), "filtered blocked ell sync");
+            CSAMD_REQUIRE(h[1] == 0u, "%s: row_remap maps two kept rows to output row %u", who, h[1] - 1u);
+            return 1;
+        };
+        unsigned int h[3] = {0u, 0u, 0u};
+        bool done = false;
+        if (gather) {
+            if (!pass(2) || !finish_counts(h)) return 0;
+            if (h[2] == 0u) {  // every row fit its region
+                live = h[0];
+                CSAMD_REQUIRE(live <= max_nnz, "%s: %u surviving entries exceed the source nnz %llu", who, live, max_nnz);
+                if (!fl_dispatch<fl_copy_launch>(value_type, output_rows, (unsigned int) cap, (const unsigned int *) ptr,
+                                                 (const unsigned int *) src_row, (const unsigned int *) t_idx, (const void *) t_val,
+                                                 out_idx, out_val, s))
+                    return 0;
+                done = true;
+            } else if (!zero_bytes(pb, off_dup + 64u, s)) {  // counts, claims and flags again (not the kept totals) for count + emit
+                return 0;
+            }
+        }
+        if (!done) {
+            if (!pass(0) || !finish_counts(h)) return 0;
+            live = h[0];
+            CSAMD_REQUIRE(live <= max_nnz, "%s: %u surviving entries exceed the source nnz %llu", who, live, max_nnz);
+            if (!pass(1)) return 0;
+        }
     } else if (output_rows != 0u) {
         if (!zero_bytes(ptr, ((size_t) output_rows + 1u) * 4u, s)) return 0;
     } else {

@@ -94,6 +94,29 @@ def test_filter_identity_and_empty_outputs(ws, fws):
             assert np.all(fetch_csr(out.filtered, F16)[0] == 0)
 
 
+@pytest.mark.parametrize("vt", [F16, F32])
+def test_filter_many_to_one_column_remap(ws, fws, vt):
+    # every column onto one of 7 output columns: a row keeps more entries than
+    # there are output columns, so the one-read gather overflows its per-row
+    # region and the call counts and emits instead; the result is the same
+    rows, cols, bs = 300, 512, 8
+    bell, gb, gv = make_bell(rows, cols, bs, vt, 12 + vt, ws)
+    cr = (np.arange(cols) % 7).astype(np.uint32)
+    out = filter_blocked_ell_to_compressed(bell, rows, 7, col_remap=dev_u32(cr), ws=fws)
+    p, i, v = fetch_csr(out.filtered, vt)
+    rp, ri, rv, st = O.filter_blocked_ell(rows, cols, bs, bell.ell_cols, gb, gv, vt, rows, 7, None, None, None, cr)
+    assert int(np.diff(rp).max()) > 7
+    assert np.array_equal(p, rp) and np.array_equal(i, ri) and np.array_equal(v, rv.view(NP_BITS[vt]))
+    assert out.stats.live_nnz == st["live_nnz"]
+    # then a call that fits its regions again, on the same workspace
+    rk, fk, rr, cr2, orows, ocols = masks(rows, cols, 4)
+    out = filter_blocked_ell_to_compressed(bell, orows, ocols, dev_u8(rk), dev_u8(fk), dev_u32(rr), dev_u32(cr2), ws=fws)
+    p, i, v = fetch_csr(out.filtered, vt)
+    rp, ri, rv, st = O.filter_blocked_ell(rows, cols, bs, bell.ell_cols, gb, gv, vt, orows, ocols, rk, fk, rr, cr2)
+    assert np.array_equal(p, rp) and np.array_equal(i, ri) and np.array_equal(v, rv.view(NP_BITS[vt]))
+    assert (out.stats.kept_rows, out.stats.live_nnz) == (orows, st["live_nnz"])
+
+
 def test_filter_rejects_two_rows_on_one_output_row(ws, fws):
     rows, cols, bs = 40, 60, 4
     bell, gb, gv = make_bell(rows, cols, bs, F16, 5, ws)
