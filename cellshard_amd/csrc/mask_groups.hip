@@ -606,6 +606,8 @@ struct reopt_private {
     size_t compressed_bytes;
     void *scratch_block;
     size_t scratch_bytes;
+    void *gather_block;  // per-row regions of surviving indices | values (one-read path)
+    size_t gather_bytes;
 };
 
 __device__ __forceinline__ bool reopt_keep(const csamd_blocked_ell_view &src, const csamd_row_feature_mask_view &masks,
@@ -761,6 +763,79 @@ __global__ void __launch_bounds__(1024) reopt_scan_kernel(unsigned int rows, uns
         __syncthreads();
     }
     if (threadIdx.x == 0) major_ptr[rows] = carry;
+}
+
+// One-read form of count + scatter for the 16-byte chunk path: a wave takes
+// a row and writes its surviving entries, in position order, to the row's
+// own region of a scratch array (row r at r * cap, at most cap entries; a
+// row with more sets *overflow and the caller runs count + scatter
+// instead), with the count in major_ptr[out_row + 1] and the source row in
+// src_row[out_row].
+__global__ void __launch_bounds__(MG_THREADS) reopt_gather_kernel(csamd_blocked_ell_view src, csamd_row_feature_mask_view masks,
+                                                                  unsigned int output_rows, unsigned int cap,
+                                                                  unsigned int *__restrict__ major_ptr, unsigned int *__restrict__ src_row,
+                                                                  unsigned int *__restrict__ overflow, unsigned int *__restrict__ t_idx,
+                                                                  uint16_t *__restrict__ t_val, const unsigned int *__restrict__ col_map,
+                                                                  const unsigned int *__restrict__ block_keep, unsigned int keep_blocks) {
+    const unsigned int wave = threadIdx.x >> 6;
+    const unsigned int bs = src.block_size;
+    const unsigned int width = src.ell_cols / bs;
+    const unsigned int nc = src.ell_cols / 8u;
+    CSAMD_GLOBAL unsigned int *gi = gmem(t_idx);
+    CSAMD_GLOBAL uint16_t *gv = gmem(t_val);
+    for (unsigned int row = blockIdx.x * MG_WAVES + wave; row < src.rows; row += gridDim.x * MG_WAVES) {
+        if (!row_participates(masks, row)) continue;
+        const unsigned int out_row = masks.row_remap != nullptr ? masks.row_remap[row] : row;
+        if (out_row >= output_rows) continue;
+        const unsigned long long base = (unsigned long long) row * cap;
+        unsigned int w = 0u;
+        for (unsigned int c0 = 0u; c0 < nc; c0 += 64u) {
+            const unsigned int c = c0 + lane_id();
+            uint4 q = make_uint4(0u, 0u, 0u, 0u);
+            unsigned int oc[8];
+            const unsigned int m = c >= nc ? 0u
+                                   : block_keep != nullptr ? reopt_chunk_keep(src, col_map, block_keep, keep_blocks, width, row, c, &q, oc)
+                                                           : reopt_chunk(src, col_map, width, row, c, &q, oc);
+            const unsigned int cnt = (unsigned int) __popc(m);
+            const unsigned int incl = wave_inclusive_scan_u32(cnt);
+            unsigned int k = w + incl - cnt;
+            w += __shfl(incl, 63);
+            const unsigned int qw[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+            for (unsigned int e = 0; e < 8u; ++e) {
+                if ((m >> e) & 1u) {
+                    if (k < cap) {
+                        gi[base + k] = oc[e];
+                        gv[base + k] = (uint16_t) (qw[e >> 1] >> (16u * (e & 1u)));
+                    }
+                    ++k;
+                }
+            }
+        }
+        if (lane_id() == 0u) {
+            major_ptr[out_row + 1u] = w;
+            src_row[out_row] = row;
+            if (w > cap) *overflow = 1u;
+        }
+    }
+}
+
+// Copy pass after the gather: output row o's entries from its source row's
+// region, one wave per row.
+__global__ void __launch_bounds__(MG_THREADS) reopt_copy_kernel(unsigned int output_rows, unsigned int cap,
+                                                                const unsigned int *__restrict__ major_ptr,
+                                                                const unsigned int *__restrict__ src_row,
+                                                                const unsigned int *__restrict__ t_idx, const uint16_t *__restrict__ t_val,
+                                                                unsigned int *__restrict__ minor_idx, uint16_t *__restrict__ val) {
+    for (unsigned int o = blockIdx.x * MG_WAVES + (threadIdx.x >> 6); o < output_rows; o += gridDim.x * MG_WAVES) {
+        const unsigned int b = major_ptr[o], n = major_ptr[o + 1u] - b;
+        if (n == 0u) continue;
+        const unsigned long long base = (unsigned long long) src_row[o] * cap;
+        for (unsigned int k = lane_id(); k < n; k += 64u) {
+            gmem(minor_idx)[b + k] = gmem(t_idx)[base + k];
+            gmem(val)[b + k] = gmem(t_val)[base + k];
+        }
+    }
 }
 
 __global__ void __launch_bounds__(MG_THREADS) reopt_scatter_kernel(csamd_blocked_ell_view src, csamd_row_feature_mask_view masks,
@@ -1167,6 +1242,7 @@ void csamd_reoptimize_clear(csamd_masked_sparse_reoptimize_workspace *ws) {
     if (st != nullptr) {
         if (st->compressed_block != nullptr) (void) hipFree(st->compressed_block);
         if (st->scratch_block != nullptr) (void) hipFree(st->scratch_block);
+        if (st->gather_block != nullptr) (void) hipFree(st->gather_block);
         std::free(st);
     }
     csamd_reoptimize_init(ws);
@@ -1226,7 +1302,9 @@ int csamd_manual_reoptimize_masked_sparse(const csamd_blocked_ell_view *src, con
     }
     const size_t map_off = align_up((size_t) (output_rows == 0u ? 1u : output_rows) * sizeof(unsigned int), 256u);
     const size_t keep_off = map_off + align_up((size_t) (src->cols == 0u ? 1u : src->cols) * sizeof(unsigned int), 256u);
-    const size_t scratch_needed = keep_off + (size_t) (src->cols == 0u ? 1u : src->cols) * sizeof(unsigned int);
+    const size_t src_off = keep_off + align_up((size_t) (src->cols == 0u ? 1u : src->cols) * sizeof(unsigned int), 256u);
+    const size_t flag_off = src_off + align_up((size_t) (output_rows == 0u ? 1u : output_rows) * sizeof(unsigned int), 256u);
+    const size_t scratch_needed = flag_off + 256u;
     if (scratch_needed > st->scratch_bytes) {
         if (st->scratch_block != nullptr) (void) hipFree(st->scratch_block);
         st->scratch_block = nullptr;
@@ -1250,8 +1328,12 @@ int csamd_manual_reoptimize_masked_sparse(const csamd_blocked_ell_view *src, con
     unsigned int *block_keep = keep_bits ? (unsigned int *) ((char *) st->scratch_block + keep_off) : nullptr;
     const unsigned int keep_blocks = keep_bits ? (unsigned int) (((unsigned long long) src->cols + src->block_size - 1u) / src->block_size) : 0u;
 
+    unsigned int *src_row = (unsigned int *) ((char *) st->scratch_block + src_off);
+    unsigned int *overflow = (unsigned int *) ((char *) st->scratch_block + flag_off);
+
     if (!fill_u32(c.majorPtr, (unsigned long long) output_rows + 1ull, 0u, s)) return 0;
     if (!fill_u32(row_write, output_rows, 0u, s)) return 0;
+    if (!fill_u32(overflow, 1u, 0u, s)) return 0;
     const unsigned int grid = reduce_grid(src->rows);
     if (src->cols != 0u) {
         reopt_col_map_kernel<<<grid_for(src->cols, 256u, 1024u), 256, 0, s>>>(m, src->cols, output_cols, col_map);
@@ -1261,22 +1343,65 @@ int csamd_manual_reoptimize_masked_sparse(const csamd_blocked_ell_view *src, con
         reopt_block_keep_kernel<<<grid_for(keep_blocks, 256u, 1024u), 256, 0, s>>>(src->cols, src->block_size, col_map, block_keep);
         CSAMD_HIP(hipGetLastError(), "reopt_block_keep_kernel");
     }
-    if (src->rows != 0u) {
-        reopt_count_kernel<<<grid, MG_THREADS, 0, s>>>(*src, m, output_rows, output_cols, c.majorPtr, col_map, block_keep,
-                                                       keep_blocks);
-        CSAMD_HIP(hipGetLastError(), "reopt_count_kernel");
-    }
-    reopt_scan_kernel<<<1, 1024, 0, s>>>(output_rows, c.majorPtr);
-    CSAMD_HIP(hipGetLastError(), "reopt_scan_kernel");
-    if (src->rows != 0u) {
-        reopt_scatter_kernel<<<grid, MG_THREADS, 0, s>>>(*src, m, output_rows, output_cols, c.majorPtr, row_write, c.minorIdx,
-                                                         (uint16_t *) c.val, col_map, block_keep, keep_blocks);
-        CSAMD_HIP(hipGetLastError(), "reopt_scatter_kernel");
-    }
+    // One read of the kept rows (16-byte chunk path): each row's survivors to
+    // a region of cap = min(ell_cols, output_cols) entries, enough while a
+    // row's columns are distinct and the feature remap is one-to-one, then
+    // packed once the row pointers are known; a row with more survivors
+    // sends the call through count + scatter.
+    const unsigned long long cap = src->ell_cols < output_cols ? src->ell_cols : output_cols;
+    const unsigned long long gather_bytes = align_up((size_t) (src->rows * cap) * 4u, 256u) + (size_t) (src->rows * cap) * 2u;
+    const bool chunk_path = src->block_size != 0u && src->block_size % 8u == 0u && src->ell_cols % 8u == 0u &&
+                            ((uintptr_t) src->val & 15u) == 0u;
+    const bool gather = chunk_path && src->rows != 0u && cap != 0u && gather_bytes <= (1ull << 30);
     unsigned int host_nnz = 0u;
-    CSAMD_HIP(hipMemcpyAsync(&host_nnz, c.majorPtr + output_rows, sizeof(unsigned int), hipMemcpyDeviceToHost, s),
-              "hipMemcpyAsync masked nnz");
-    CSAMD_HIP(hipStreamSynchronize(s), "hipStreamSynchronize masked reoptimize");
+    bool done = false;
+    if (gather) {
+        if (gather_bytes > st->gather_bytes) {
+            if (st->gather_block != nullptr) (void) hipFree(st->gather_block);
+            st->gather_block = nullptr;
+            st->gather_bytes = 0u;
+            CSAMD_HIP(hipMalloc(&st->gather_block, gather_bytes), "hipMalloc masked reoptimize gather");
+            st->gather_bytes = gather_bytes;
+        }
+        unsigned int *t_idx = (unsigned int *) st->gather_block;
+        uint16_t *t_val = (uint16_t *) ((char *) st->gather_block + align_up((size_t) (src->rows * cap) * 4u, 256u));
+        reopt_gather_kernel<<<grid, MG_THREADS, 0, s>>>(*src, m, output_rows, (unsigned int) cap, c.majorPtr, src_row, overflow, t_idx,
+                                                        t_val, col_map, block_keep, keep_blocks);
+        CSAMD_HIP(hipGetLastError(), "reopt_gather_kernel");
+        reopt_scan_kernel<<<1, 1024, 0, s>>>(output_rows, c.majorPtr);
+        CSAMD_HIP(hipGetLastError(), "reopt_scan_kernel");
+        unsigned int flag = 0u;
+        CSAMD_HIP(hipMemcpyAsync(&flag, overflow, sizeof(unsigned int), hipMemcpyDeviceToHost, s), "hipMemcpyAsync reoptimize flag");
+        CSAMD_HIP(hipMemcpyAsync(&host_nnz, c.majorPtr + output_rows, sizeof(unsigned int), hipMemcpyDeviceToHost, s),
+                  "hipMemcpyAsync masked nnz");
+        CSAMD_HIP(hipStreamSynchronize(s), "hipStreamSynchronize masked reoptimize");
+        if (flag == 0u) {
+            reopt_copy_kernel<<<grid_for(output_rows, MG_WAVES, (unsigned int) device_cu_count() * 16u), MG_THREADS, 0, s>>>(
+                output_rows, (unsigned int) cap, c.majorPtr, src_row, t_idx, t_val, c.minorIdx, (uint16_t *) c.val);
+            CSAMD_HIP(hipGetLastError(), "reopt_copy_kernel");
+            CSAMD_HIP(hipStreamSynchronize(s), "hipStreamSynchronize masked reoptimize");
+            done = true;
+        } else if (!fill_u32(c.majorPtr, (unsigned long long) output_rows + 1ull, 0u, s)) {
+            return 0;
+        }
+    }
+    if (!done) {
+        if (src->rows != 0u) {
+            reopt_count_kernel<<<grid, MG_THREADS, 0, s>>>(*src, m, output_rows, output_cols, c.majorPtr, col_map, block_keep,
+                                                           keep_blocks);
+            CSAMD_HIP(hipGetLastError(), "reopt_count_kernel");
+        }
+        reopt_scan_kernel<<<1, 1024, 0, s>>>(output_rows, c.majorPtr);
+        CSAMD_HIP(hipGetLastError(), "reopt_scan_kernel");
+        if (src->rows != 0u) {
+            reopt_scatter_kernel<<<grid, MG_THREADS, 0, s>>>(*src, m, output_rows, output_cols, c.majorPtr, row_write, c.minorIdx,
+                                                             (uint16_t *) c.val, col_map, block_keep, keep_blocks);
+            CSAMD_HIP(hipGetLastError(), "reopt_scatter_kernel");
+        }
+        CSAMD_HIP(hipMemcpyAsync(&host_nnz, c.majorPtr + output_rows, sizeof(unsigned int), hipMemcpyDeviceToHost, s),
+                  "hipMemcpyAsync masked nnz");
+        CSAMD_HIP(hipStreamSynchronize(s), "hipStreamSynchronize masked reoptimize");
+    }
     c.nnz = host_nnz;
     out->compressed = c;
     std::memset(&out->blocked_ell, 0, sizeof(out->blocked_ell));

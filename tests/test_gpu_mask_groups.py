@@ -273,6 +273,40 @@ def test_reoptimize_random_parity(gpu, bs):
     csr_rt.clear(rws)
 
 
+@pytest.mark.parametrize("many_to_one", [False, True])
+def test_reoptimize_gather_and_fallback(gpu, many_to_one):
+    """The one-read gather (16-byte chunks) against the oracle, and the
+    count + scatter fallback when a row keeps more entries than its region
+    holds (a feature remap onto 5 output columns), on one workspace."""
+    rows, cols, bs = 300, 520, 16
+    ptr_, idx, val = random_csr(rows, cols, 0.15, seed=7, zeros=0.1, empty_rows=0.05)
+    bits = f16_bits(val)
+    bci, bv, ell, nnz = O.blocked_ell_from_compressed(rows, cols, ptr_, idx, bits, bs)
+    bell = DeviceBlockedELL.from_numpy(rows, cols, nnz, bs, ell, bci, bv)
+    rk = np.ones(rows, np.uint8)
+    fk = np.ones(cols, np.uint8)
+    rr = np.arange(rows, dtype=np.uint32)
+    out_cols = 5 if many_to_one else cols
+    fr = (np.arange(cols) % out_cols).astype(np.uint32)
+    d_rk, d_fk, d_rr, d_fr = to_dev_u8(rk), to_dev_u8(fk), to_dev_u32(rr), to_dev_u32(fr)
+    rws = A.masked_sparse_reoptimize_workspace()
+    csr_rt.init(rws)
+    assert csr_rt.setup(rws, 0) == 1
+    masks = A.row_feature_mask_view(row_keep=ptr(d_rk), feature_keep=ptr(d_fk), row_remap=ptr(d_rr), feature_remap=ptr(d_fr))
+    for _ in range(2):
+        cfg = A.masked_sparse_reoptimize_config(rows, out_cols, 0, bs, A.LAYOUT_COMPRESSED)
+        out = A.masked_sparse_reoptimize_result()
+        assert csr_rt.manual_reoptimize_masked_sparse(bell.view(), masks, cfg, rws, out) == 1
+        mp, mi, mv, n = O.reoptimize(rows, cols, bs, ell, bci, np.asarray(bv).view(np.uint16), rk, fk, rr, fr, rows, out_cols)
+        if many_to_one:
+            assert int(np.diff(mp).max()) > out_cols
+        assert out.metadata.live_nnz == n
+        assert np.array_equal(device_to_numpy(out.compressed.majorPtr, rows + 1, np.uint32), mp)
+        assert np.array_equal(device_to_numpy(out.compressed.minorIdx, n, np.uint32), mi)
+        assert np.array_equal(device_to_numpy(out.compressed.val, n, np.uint16), mv)
+    csr_rt.clear(rws)
+
+
 def _fast_csr(rows, cols, mean_nnz, seed, vt, long_rows=0, empty=0.05):
     """Vectorised random CSR (columns may repeat inside a row, as stored
     entries may): integer counts, some explicit zeros and empty rows."""
