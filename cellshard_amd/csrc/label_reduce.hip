@@ -285,6 +285,15 @@ __global__ void __launch_bounds__(1024) label_offsets_kernel(unsigned int *__res
                                                              unsigned int *__restrict__ f_total) {
     __shared__ unsigned int smem[16];
     __shared__ unsigned int stage[1024u * LR_OFF_ITEMS];
+    // the table's first pass is loaded before the feature map is built, and
+    // each later pass while the previous one is scanned
+    const unsigned long long n = (unsigned long long) (G + 1u) * nb;
+    unsigned int nxt[LR_OFF_ITEMS];
+#pragma unroll
+    for (unsigned int k = 0; k < LR_OFF_ITEMS; ++k) {
+        const unsigned long long i = k * 1024ull + threadIdx.x;
+        nxt[k] = i < n ? hist[i] : 0u;
+    }
     unsigned int fcarry = 0u;  // same value in every thread
     for (unsigned int start = 0; start < nwords; start += blockDim.x) {
         const unsigned int w = start + threadIdx.x;
@@ -304,13 +313,15 @@ __global__ void __launch_bounds__(1024) label_offsets_kernel(unsigned int *__res
     if (threadIdx.x == 0) *f_total = fcarry;
     // the table in passes of 1024 x LR_OFF_ITEMS entries, loaded and stored
     // coalesced through LDS; each thread scans LR_OFF_ITEMS consecutive ones
-    const unsigned long long n = (unsigned long long) (G + 1u) * nb;
     unsigned int carry = 0u;  // same value in every thread
     for (unsigned long long start = 0; start < n; start += 1024ull * LR_OFF_ITEMS) {
 #pragma unroll
+        for (unsigned int k = 0; k < LR_OFF_ITEMS; ++k) stage[k * 1024u + threadIdx.x] = nxt[k];
+        const unsigned long long next = start + 1024ull * LR_OFF_ITEMS;
+#pragma unroll
         for (unsigned int k = 0; k < LR_OFF_ITEMS; ++k) {
-            const unsigned long long i = start + k * 1024u + threadIdx.x;
-            stage[k * 1024u + threadIdx.x] = i < n ? hist[i] : 0u;
+            const unsigned long long i = next + k * 1024u + threadIdx.x;
+            nxt[k] = i < n ? hist[i] : 0u;
         }
         __syncthreads();
         unsigned int v[LR_OFF_ITEMS], own = 0u;
