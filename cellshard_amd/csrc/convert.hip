@@ -14,6 +14,7 @@
 // blocks (popcount ranks give the ascending slot of every block column);
 // one wave64 per row for Sliced-ELL emits (coalesced copy + padding).
 #include "common.hh"
+#include "sell_write.hh"
 
 #include <cstdlib>
 
@@ -433,6 +434,7 @@ __global__ void sell_emit_csr_kernel(csamd_compressed_view src, const S *__restr
                                      S *__restrict__ dval) {
     const unsigned int wave = threadIdx.x >> 6;
     const unsigned int nw = blockDim.x >> 6;
+    const bool quads = sell_quads_ok<S>(dst.col_idx, dval);
     for (unsigned int row = blockIdx.x * nw + wave; row < src.rows; row += gridDim.x * nw) {
         const unsigned int slice = sell_slice_of(dst, row);
         if (slice >= dst.slice_count) continue;
@@ -441,15 +443,8 @@ __global__ void sell_emit_csr_kernel(csamd_compressed_view src, const S *__restr
         const unsigned long long base = (unsigned long long) dst.slice_slot_offsets[slice] + (unsigned long long) (row - r0) * width;
         const unsigned int b = src.majorPtr[row], e = src.majorPtr[row + 1u];
         const unsigned int n = e - b < width ? e - b : width;
-        for (unsigned int k = lane_id(); k < width; k += 64u) {
-            if (k < n) {
-                dst.col_idx[base + k] = src.minorIdx[b + k];
-                dval[base + k] = sval[b + k];
-            } else {
-                dst.col_idx[base + k] = CSAMD_INVALID_COL;
-                dval[base + k] = (S) 0;
-            }
-        }
+        sell_write_row<S>(gmem((const unsigned int *) src.minorIdx), gmem(sval), b, n, width, base, gmem(dst.col_idx), gmem(dval), quads,
+                          ~0ull);
     }
 }
 

@@ -14,6 +14,7 @@
 // INVALID, values at row * ell_cols + slot * B + col % B, the last of
 // repeated (row, col) entries kept, nnz = stored nonzero values).
 #include "pipeline.hh"
+#include "sell_write.hh"
 
 namespace csamd {
 namespace {
@@ -109,7 +110,7 @@ __global__ void __launch_bounds__(1024) pl_sell_scan_kernel(const pl_sell *__res
 template<typename S>
 __global__ void pl_sell_emit_kernel(const pl_sell *__restrict__ tab, unsigned int n, const unsigned int *__restrict__ row_base,
                                     unsigned int total_rows) {
-    const unsigned int wave = threadIdx.x >> 6, nw = blockDim.x >> 6, lane = lane_id();
+    const unsigned int wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
     for (unsigned int g = blockIdx.x * nw + wave; g < total_rows; g += gridDim.x * nw) {
         const unsigned int p = pl_find(row_base, n, g);
         const pl_sell &P = tab[p];
@@ -118,16 +119,9 @@ __global__ void pl_sell_emit_kernel(const pl_sell *__restrict__ tab, unsigned in
         const unsigned int width = P.widths[s];
         const unsigned long long base = (unsigned long long) P.slot_offsets[s] + (unsigned long long) (r - s * P.S) * width;
         const unsigned int b = gmem(P.ptr)[r], e = gmem(P.ptr)[r + 1u];
-        const unsigned int len = min(e - b, width);
-        const CSAMD_GLOBAL unsigned int *idx = gmem(P.idx);
-        const CSAMD_GLOBAL S *val = gmem((const S *) P.val);
-        CSAMD_GLOBAL unsigned int *oc = gmem(P.col_idx);
-        CSAMD_GLOBAL S *ov = gmem((S *) P.out_val);
-        for (unsigned int k = lane; k < width; k += 64u) {
-            if (base + k >= P.slot_cap) break;  // over capacity: flagged by the scan
-            oc[base + k] = k < len ? idx[b + k] : CSAMD_INVALID_COL;
-            ov[base + k] = k < len ? val[b + k] : S(0);
-        }
+        // slots past the capacity are not written (over capacity: flagged by the scan)
+        sell_write_row<S>(gmem(P.idx), gmem((const S *) P.val), b, min(e - b, width), width, base, gmem(P.col_idx), gmem((S *) P.out_val),
+                          sell_quads_ok<S>(P.col_idx, P.out_val), P.slot_cap);
     }
 }
 
@@ -184,17 +178,19 @@ __global__ void __launch_bounds__(PL_THREADS) pl_bell_kernel(const pl_bell *__re
         }
         for (unsigned int k = count + tid; k < W; k += PL_THREADS) bci[k] = CSAMD_INVALID_COL;
         const unsigned long long zb = (unsigned long long) r0 * ell, zn = (unsigned long long) (r1 - r0) * ell;
-        for (unsigned long long i = tid; i < zn; i += PL_THREADS) out[zb + i] = S(0);
+        constexpr unsigned int Z = 16u / sizeof(S);  // values per 16-byte store
+        if (((zb * sizeof(S)) & 15u) == 0u && ((uintptr_t) P.out_val & 15u) == 0u) {
+            CSAMD_GLOBAL u32x4 *z4 = (CSAMD_GLOBAL u32x4 *) (out + zb);
+            const unsigned long long nz4 = zn / Z;
+            for (unsigned long long i = tid; i < nz4; i += PL_THREADS) z4[i] = u32x4{0u, 0u, 0u, 0u};
+            for (unsigned long long i = nz4 * Z + tid; i < zn; i += PL_THREADS) out[zb + i] = S(0);
+        } else {
+            for (unsigned long long i = tid; i < zn; i += PL_THREADS) out[zb + i] = S(0);
+        }
         __syncthreads();  // zeros written before the entries
         unsigned int nz = 0u;
         for (unsigned int r = r0 + wave; r < r1; r += nwaves) {
             const unsigned int b = ptr[r], e = ptr[r + 1u];
-            bool simple = true;
-            for (unsigned int i0 = b; i0 < e; i0 += 64u) {
-                const unsigned int i = i0 + lane;
-                const bool bad = i + 1u < e && idx[i] >= idx[i + 1u];
-                if (__ballot(bad) != 0ull) simple = false;
-            }
             const unsigned long long ro = (unsigned long long) r * ell;
             auto pos_of = [&](unsigned int c, unsigned int *pos) -> bool {
                 if (c >= P.cols) return false;
@@ -204,14 +200,39 @@ __global__ void __launch_bounds__(PL_THREADS) pl_bell_kernel(const pl_bell *__re
                 *pos = (slot << lbs) + (c & (bs - 1u));
                 return true;
             };
-            if (simple) {
-                for (unsigned int i = b + lane; i < e; i += 64u) {
-                    unsigned int pos;
-                    if (!pos_of(idx[i], &pos)) continue;
-                    const S v = val[i];
-                    out[ro + pos] = v;
-                    nz += pl_nonzero<S>(v) ? 1u : 0u;
+            // Scatter, assuming the row strictly ascends, PL_U entries per
+            // lane per step with every load issued before any use (one
+            // memory round trip per step, not per entry); a step that finds
+            // a column not above the previous one marks the row for the
+            // ordered replay below, which rewrites all of its entries.
+            constexpr unsigned int PL_U = 16u;
+            bool simple = true;
+            unsigned int rnz = 0u;
+            for (unsigned int i0 = b; i0 < e; i0 += 64u * PL_U) {
+                unsigned int c[PL_U], cn[PL_U];
+                S v[PL_U];
+#pragma unroll
+                for (unsigned int u = 0; u < PL_U; ++u) {
+                    const unsigned int i = i0 + u * 64u + lane;
+                    c[u] = i < e ? idx[i] : CSAMD_INVALID_COL;
+                    cn[u] = i + 1u < e ? idx[i + 1u] : CSAMD_INVALID_COL;
+                    v[u] = i < e ? val[i] : S(0);
                 }
+                bool bad = false;
+#pragma unroll
+                for (unsigned int u = 0; u < PL_U; ++u) {
+                    const unsigned int i = i0 + u * 64u + lane;
+                    bad |= i + 1u < e && c[u] >= cn[u];
+                    unsigned int pos;
+                    if (i < e && pos_of(c[u], &pos)) {
+                        out[ro + pos] = v[u];
+                        rnz += pl_nonzero<S>(v[u]) ? 1u : 0u;
+                    }
+                }
+                if (__ballot(bad) != 0ull) simple = false;
+            }
+            if (simple) {
+                nz += rnz;
             } else {
                 if (lane == 0u)
                     for (unsigned int i = b; i < e; ++i) {
