@@ -150,9 +150,19 @@ __global__ void __launch_bounds__(PL_THREADS) pl_bell_kernel(const pl_bell *__re
         for (unsigned int w = tid; w < nwords; w += PL_THREADS) bits[w] = 0u;
         __syncthreads();
         const unsigned int eb = ptr[r0], ee = ptr[r1];
-        for (unsigned int i = eb + tid; i < ee; i += PL_THREADS) {
-            const unsigned int c = idx[i];
-            if (c < P.cols) atomicOr(&bits[(c >> lbs) >> 5], 1u << ((c >> lbs) & 31u));
+        // the row block's entries, PL_BITS_U per thread per step with the
+        // loads issued before the atomics (one memory round trip per step)
+        constexpr unsigned int PL_BITS_U = 16u;
+        for (unsigned int i0 = eb; i0 < ee; i0 += PL_THREADS * PL_BITS_U) {
+            unsigned int c[PL_BITS_U];
+#pragma unroll
+            for (unsigned int u = 0; u < PL_BITS_U; ++u) {
+                const unsigned int i = i0 + u * PL_THREADS + tid;
+                c[u] = i < ee ? idx[i] : CSAMD_INVALID_COL;
+            }
+#pragma unroll
+            for (unsigned int u = 0; u < PL_BITS_U; ++u)
+                if (c[u] < P.cols) atomicOr(&bits[(c[u] >> lbs) >> 5], 1u << ((c[u] >> lbs) & 31u));
         }
         __syncthreads();
         unsigned int carry = 0u;
