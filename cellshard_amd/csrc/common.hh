@@ -190,13 +190,19 @@ __device__ __forceinline__ unsigned long long lanemask_lt() {
     return l == 0u ? 0ull : (~0ull >> (64u - l));
 }
 
-// Inclusive prefix sum over the wave (Hillis-Steele with shuffles).
+// Inclusive prefix sum over the wave with DPP moves (no LDS round trips):
+// within each row of 16 lanes, shifts right by 1, 2, 4 and 8 (lanes whose
+// source lies outside the row add 0); then lane 15 of rows 0 and 2 is added
+// to rows 1 and 3 (row_bcast:15), and lane 31 to rows 2 and 3
+// (row_bcast:31).  Disabled rows and invalid sources read the `old` operand,
+// 0 here.
 __device__ __forceinline__ unsigned int wave_inclusive_scan_u32(unsigned int v) {
-    const unsigned int l = lane_id();
-    for (int o = 1; o < 64; o <<= 1) {
-        unsigned int t = __shfl_up(v, o, 64);
-        if (l >= (unsigned int) o) v += t;
-    }
+    v += (unsigned int) __builtin_amdgcn_update_dpp(0, (int) v, 0x111, 0xf, 0xf, false);  // row_shr:1
+    v += (unsigned int) __builtin_amdgcn_update_dpp(0, (int) v, 0x112, 0xf, 0xf, false);  // row_shr:2
+    v += (unsigned int) __builtin_amdgcn_update_dpp(0, (int) v, 0x114, 0xf, 0xf, false);  // row_shr:4
+    v += (unsigned int) __builtin_amdgcn_update_dpp(0, (int) v, 0x118, 0xf, 0xf, false);  // row_shr:8
+    v += (unsigned int) __builtin_amdgcn_update_dpp(0, (int) v, 0x142, 0xa, 0xf, false);  // row_bcast:15, rows 1, 3
+    v += (unsigned int) __builtin_amdgcn_update_dpp(0, (int) v, 0x143, 0xc, 0xf, false);  // row_bcast:31, rows 2, 3
     return v;
 }
 __device__ __forceinline__ unsigned long long wave_inclusive_scan_u64(unsigned long long v) {

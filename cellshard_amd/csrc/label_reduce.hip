@@ -475,7 +475,8 @@ struct acc_lds {
     double *sum;
     double *sumsq;
     unsigned int *cnt;
-    const uint2 *lpairs;  // [nwords + 1], {0, 0} sentinel last
+    const uint2 *lpairs;  // [nwords + 1] the feature tile's map, {0, 0} sentinel last
+    uint2 *queues;        // [waves x LR_QCAP] kept entries {tile feature, value bits}
 };
 
 struct reduce_args {
@@ -502,6 +503,35 @@ __device__ __forceinline__ void acc_add(const acc_lds &acc, unsigned int f, floa
     atomicAdd(acc.sum + f, d);
     atomicAdd(acc.sumsq + f, d * d);
     atomicAdd(acc.cnt + f, 1u);
+}
+
+// Feature map of one LDS feature tile [f0, f1): per 32-column word, the
+// kept columns whose feature rank lies in the tile, and the tile-relative
+// rank of the first of them.  A column c is then in the tile iff bit c & 31
+// of x is set, and its tile feature is y + popcount(x below bit c & 31), so
+// the streaming loop needs neither the tile's first feature nor its size.
+__device__ __forceinline__ uint2 tile_pairs(uint2 pm, unsigned int f0, unsigned int f1) {
+    const unsigned int n = (unsigned int) __popc(pm.x);
+    const unsigned int lo = pm.y > f0 ? pm.y : f0;
+    const unsigned int hi = pm.y + n < f1 ? pm.y + n : f1;
+    if (lo >= hi) return make_uint2(0u, 0u);
+    unsigned int x = pm.x;
+    for (unsigned int k = pm.y; k < lo; ++k) x &= x - 1u;  // drop the kept columns ranked before the tile
+    unsigned int keep = 0u;
+    for (unsigned int k = lo; k < hi; ++k) {                // keep the next hi - lo of them
+        const unsigned int low = x & (0u - x);
+        keep |= low;
+        x ^= low;
+    }
+    return make_uint2(keep, lo - f0);
+}
+
+// Column c against its word's tile map pm: true iff c is a kept feature of
+// the tile, with f its tile feature.  The bit-field extracts read only the
+// low 5 bits of c.
+__device__ __forceinline__ bool tile_feature(uint2 pm, unsigned int c, unsigned int &f) {
+    f = pm.y + (unsigned int) __popc(__builtin_amdgcn_ubfe(pm.x, 0u, c));
+    return __builtin_amdgcn_ubfe(pm.x, c, 1u) != 0u;
 }
 
 // The workgroup's sorted-row range [bounds[b], bounds[b + 1]), walked one
@@ -531,6 +561,11 @@ __device__ __forceinline__ void walk_segments(const reduce_args &a, const acc_ld
     for (unsigned int tile = 0u; tile * a.ftile < F; ++tile) {
         t.f0 = tile * a.ftile;
         t.f_count = F - t.f0 < a.ftile ? F - t.f0 : a.ftile;
+        // the tile's feature map (the previous tile's last segment ended
+        // with a barrier; the first segment's barrier publishes it)
+        uint2 *lp = (uint2 *) acc.lpairs;
+        for (unsigned int w = threadIdx.x; w <= a.nwords; w += blockDim.x)
+            lp[w] = w < a.nwords ? tile_pairs(a.fpairs[w], t.f0, t.f0 + t.f_count) : make_uint2(0u, 0u);
         for (unsigned int g = g0; g < a.G; ++g) {
             const unsigned int gs = a.gstart[g], ge = a.gstart[g + 1u];
             const unsigned int s0 = gs > lo ? gs : lo;
@@ -547,10 +582,8 @@ __device__ __forceinline__ void walk_segments(const reduce_args &a, const acc_ld
             for (unsigned int k = threadIdx.x; k < a.ntail; k += blockDim.x) {
                 const tail_record rec = a.tail[k];
                 if (rec.pos < s0 || rec.pos >= s1 || rec.col >= a.cols || rec.v == 0.0f) continue;
-                const uint2 pm = acc.lpairs[rec.col >> 5];
-                const unsigned int bit = rec.col & 31u;
-                const unsigned int f = pm.y + (unsigned int) __popc(pm.x & ((1u << bit) - 1u)) - t.f0;
-                if (((pm.x >> bit) & 1u) != 0u && f < t.f_count) acc_add(acc, f, rec.v);
+                unsigned int f;
+                if (tile_feature(acc.lpairs[rec.col >> 5], rec.col, f)) acc_add(acc, f, rec.v);
             }
             __syncthreads();
             const unsigned long long slot = tile * tile_slots + (unsigned long long) (blockIdx.x + g) * a.ftile;
@@ -568,13 +601,50 @@ template<int VT>
 __device__ __forceinline__ void consume(unsigned int c, typename value_traits<VT>::storage raw, const reduce_args &a,
                                         const acc_lds &acc) {
     if (c >= a.cols) return;
-    const uint2 pm = acc.lpairs[c >> 5];
-    const unsigned int bit = c & 31u;
-    if (((pm.x >> bit) & 1u) == 0u) return;
-    const unsigned int f = pm.y + (unsigned int) __popc(pm.x & ((1u << bit) - 1u)) - a.f0;
-    if (f >= a.f_count) return;
+    unsigned int f;
+    if (!tile_feature(acc.lpairs[c >> 5], c, f)) return;
     const float v = value_traits<VT>::decode(raw);
     if (v != 0.0f) acc_add(acc, f, v);
+}
+
+// A wave's kept entries go through a ring of LR_QCAP slots in LDS: each
+// entry slot of a chunk appends the lanes whose entry is kept (ballot +
+// mbcnt), and whenever 64 are waiting, every lane takes one and adds it to
+// the accumulators.  With ~8 % of entries kept, the accumulator atomics then
+// run on full waves instead of once per entry slot on a few lanes.  The ring
+// holds < 64 waiting entries plus one slot's 64 new ones.  A wave's LDS
+// operations complete in order, so a lane reads what another lane of the
+// wave appended before it, with no barrier.
+constexpr unsigned int LR_QCAP = 128u;
+
+struct kept_queue {
+    uint2 *q;        // this wave's ring
+    unsigned int n;  // entries appended (wave-uniform)
+    unsigned int h;  // entries added to the accumulators (wave-uniform)
+};
+
+__device__ __forceinline__ void queue_push(kept_queue &Q, bool in_tile, unsigned int f, float v, const acc_lds &acc) {
+    // two ballots straight from the compares, combined on the scalar unit
+    const unsigned long long bal = __builtin_amdgcn_ballot_w64(in_tile) & __builtin_amdgcn_ballot_w64(v != 0.0f);
+    const bool kept = in_tile && v != 0.0f;
+    const unsigned int pos = __builtin_amdgcn_mbcnt_hi((unsigned int) (bal >> 32), __builtin_amdgcn_mbcnt_lo((unsigned int) bal, Q.n));
+    if (kept) Q.q[pos & (LR_QCAP - 1u)] = make_uint2(f, __float_as_uint(v));
+    Q.n += (unsigned int) __popcll(bal);
+    if (Q.n - Q.h >= 64u) {
+        const uint2 e = Q.q[(Q.h + lane_id()) & (LR_QCAP - 1u)];
+        acc_add(acc, e.x, __uint_as_float(e.y));
+        Q.h += 64u;
+    }
+}
+
+// The waiting entries (fewer than 64), before the segment's accumulators are read.
+__device__ __forceinline__ void queue_drain(kept_queue &Q, const acc_lds &acc) {
+    const unsigned int lane = lane_id();
+    if (lane < Q.n - Q.h) {
+        const uint2 e = Q.q[(Q.h + lane) & (LR_QCAP - 1u)];
+        acc_add(acc, e.x, __uint_as_float(e.y));
+    }
+    Q.h = Q.n;
 }
 
 constexpr unsigned int LR_UNITS = 4u;               // 4-entry units per lane per chunk (even)
@@ -621,8 +691,7 @@ __device__ __forceinline__ void load_chunk(const unsigned int *__restrict__ idx,
 // Consume one chunk of the row whose entries are [b, e), starting at unit u,
 // two units (8 entries per lane) at a time.  Per pair, the 8 bitmap lookups
 // are issued first (independent LDS reads); keep tests, ranks and values are
-// computed without branches; one branch per entry guards the three
-// accumulator atomics.  Nothing here reads global memory, so the prefetched
+// computed without branches; kept entries go to the wave's queue.  Nothing here reads global memory, so the prefetched
 // chunk stays in flight (a feature scale is applied by the finish).
 // lpairs[nwords] is a {0, 0} sentinel, so columns >= cols (and
 // CSAMD_INVALID_COL) are "not kept".  Entries outside [b, e) exist only in a
@@ -632,7 +701,7 @@ __device__ __forceinline__ void load_chunk(const unsigned int *__restrict__ idx,
 // written.
 template<int VT>
 __device__ __forceinline__ void consume_chunk(const chunk_regs<VT> &r, unsigned int u, unsigned int b, unsigned int e,
-                                              const reduce_args &a, const acc_lds &acc) {
+                                              const reduce_args &a, const acc_lds &acc, kept_queue &Q) {
     const unsigned long long first = 4ull * u;
     const bool edge = first < b || first + 4ull * LR_CHUNK_UNITS > e;
     const unsigned int lane = lane_id();
@@ -654,11 +723,10 @@ __device__ __forceinline__ void consume_chunk(const chunk_regs<VT> &r, unsigned 
         for (unsigned int k = 0; k < 8u; ++k) pm[k] = acc.lpairs[min(c[k] >> 5, a.nwords)];
 #pragma unroll
         for (unsigned int k = 0; k < 8u; ++k) {
-            const unsigned int bit = c[k] & 31u;
-            const unsigned int f = pm[k].y + (unsigned int) __popc(pm[k].x & ((1u << bit) - 1u)) - a.f0;
+            unsigned int f;
+            const bool in_tile = tile_feature(pm[k], c[k], f);
             const float v = unit_vals<VT>::get(r.v[2u * p + (k >> 2)], k & 3u);
-            const bool kept = ((pm[k].x >> bit) & 1u) != 0u && f < a.f_count && v != 0.0f;
-            if (kept) acc_add(acc, f, v);
+            queue_push(Q, in_tile, f, v, acc);
         }
     }
 }
@@ -734,6 +802,7 @@ __device__ __forceinline__ void stream_rows(const row_range *__restrict__ ranges
     st.take();
     row_cursor ca, cb;
     if (!st.next_row(ca)) return;
+    kept_queue Q{acc.queues + (threadIdx.x >> 6) * LR_QCAP, 0u, 0u};
     // Loads are issued unconditionally (after the last chunk the wave
     // re-loads it, in bounds and unused): a load under a branch makes the
     // compiler merge old and new buffer contents, which waits for the load.
@@ -744,15 +813,16 @@ __device__ __forceinline__ void stream_rows(const row_range *__restrict__ ranges
         bool more = st.advance(cb);
         if (!more) cb = ca;
         load_cursor<VT, VEC>(cb, B);
-        consume_chunk<VT>(A, ca.u, ca.rng.b, ca.rng.e, a, acc);
+        consume_chunk<VT>(A, ca.u, ca.rng.b, ca.rng.e, a, acc, Q);
         if (!more) break;
         ca = cb;
         more = st.advance(ca);
         if (!more) ca = cb;
         load_cursor<VT, VEC>(ca, A);
-        consume_chunk<VT>(B, cb.u, cb.rng.b, cb.rng.e, a, acc);
+        consume_chunk<VT>(B, cb.u, cb.rng.b, cb.rng.e, a, acc, Q);
         if (!more) break;
     }
+    queue_drain(Q, acc);
 }
 
 __device__ __forceinline__ acc_lds lds_layout(unsigned char *lds, const reduce_args &a) {
@@ -760,7 +830,8 @@ __device__ __forceinline__ acc_lds lds_layout(unsigned char *lds, const reduce_a
     double *lsum = (double *) (lds + align_up((size_t) (a.nwords + 1u) * sizeof(uint2), 16u));
     double *lsumsq = lsum + a.ftile;
     unsigned int *lcnt = (unsigned int *) (lsumsq + a.ftile);
-    return acc_lds{lsum, lsumsq, lcnt, lpairs};
+    uint2 *queues = (uint2 *) (lcnt + ((a.ftile + 1u) & ~1u));  // 8-byte aligned
+    return acc_lds{lsum, lsumsq, lcnt, lpairs, queues};
 }
 
 template<int VT, bool VEC>
@@ -768,8 +839,6 @@ __global__ void label_reduce_rows_kernel(const row_range *__restrict__ ranges, r
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ unsigned int next_row;
     const acc_lds acc = lds_layout(lds, a);
-    uint2 *lpairs = (uint2 *) acc.lpairs;
-    for (unsigned int w = threadIdx.x; w <= a.nwords; w += blockDim.x) lpairs[w] = w < a.nwords ? a.fpairs[w] : make_uint2(0u, 0u);
     walk_segments(a, acc, [&](unsigned int s0, unsigned int s1, const reduce_args &t) {
         if (threadIdx.x == 0u) next_row = s0;
         __syncthreads();  // also publishes the zeroed accumulators
@@ -785,12 +854,10 @@ __global__ void label_reduce_blocked_kernel(csamd_blocked_ell_view src, reduce_a
     const unsigned int nwaves = blockDim.x >> 6;
     const unsigned int wave = threadIdx.x >> 6;
     const acc_lds acc = lds_layout(lds, a);
-    uint2 *lpairs = (uint2 *) acc.lpairs;
     const S *__restrict__ val = (const S *) src.val;
     const unsigned int bs = src.block_size;
     const unsigned int width = bs != 0u ? src.ell_cols / bs : 0u;
 
-    for (unsigned int w = threadIdx.x; w <= a.nwords; w += blockDim.x) lpairs[w] = w < a.nwords ? a.fpairs[w] : make_uint2(0u, 0u);
     walk_segments(a, acc, [&](unsigned int s0, unsigned int s1, const reduce_args &t) {
         __syncthreads();  // publishes the zeroed accumulators
         for (unsigned int i = s0 + wave; i < s1; i += nwaves) {
@@ -1035,6 +1102,10 @@ int plan_label_reduce(const std::vector<part_in> &parts, unsigned int value_type
     const unsigned int cu = (unsigned int) device_cu_count();
     unsigned int threads = env_u("CSAMD_LR_THREADS", 1024u);
     if (threads < 256u || threads > 1024u || threads % 64u != 0u) threads = 1024u;
+    // the waves' kept-entry rings, after the accumulators (16-byte aligned)
+    const size_t queue_lds = 16u + (size_t) (threads / 64u) * LR_QCAP * sizeof(uint2);
+    CSAMD_REQUIRE(fixed_lds + queue_lds + 64u * LR_ACC_BYTES <= lds_avail,
+                  "label_reduce: %u columns need %zu bytes of LDS for the feature map", cols, fixed_lds);
     unsigned int per_cu = env_u("CSAMD_LR_PER_CU", 1u);
     unsigned int split = env_u("CSAMD_LR_SPLIT", 1u);
     if (split < 1u || split > 16u) split = 1u;
@@ -1130,16 +1201,16 @@ int plan_label_reduce(const std::vector<part_in> &parts, unsigned int value_type
     // the LDS feature tile is sized for all columns (at most F of them are
     // kept; the device reads F), so no host wait is needed
     const unsigned int fmax = cols == 0u ? 1u : cols;
-    if (per_cu != 2u || fixed_lds + 64u * LR_ACC_BYTES > lds_avail / 2u) per_cu = 1u;
+    if (per_cu != 2u || fixed_lds + queue_lds + 64u * LR_ACC_BYTES > lds_avail / 2u) per_cu = 1u;
     const size_t lds_budget = per_cu == 2u ? lds_avail / 2u : lds_avail;
-    unsigned int ftile = (unsigned int) ((lds_budget - fixed_lds) / LR_ACC_BYTES);
+    unsigned int ftile = (unsigned int) ((lds_budget - fixed_lds - queue_lds) / LR_ACC_BYTES);
     if (ftile >= fmax) ftile = fmax;
     else ftile &= ~63u;
     const unsigned int nftiles = (fmax + ftile - 1u) / ftile;
     const unsigned int W = cu * per_cu * split;
     L.fmax = fmax;
     L.W = W;
-    L.lds = fixed_lds + (size_t) ftile * LR_ACC_BYTES;
+    L.lds = fixed_lds + (size_t) ftile * LR_ACC_BYTES + queue_lds;
     L.hist_lds = (size_t) (G + 1u) * sizeof(unsigned int);
     L.scatter_lds = (2u * (size_t) (G + 1u) + nparts + 1u) * sizeof(unsigned int);
     L.meta_lds = (size_t) (G + 1u + W + 1u) * sizeof(unsigned int);

@@ -25,6 +25,7 @@
 // Sources with unsorted or repeated minor indices take a fallback scatter
 // (K4u: LDS cursors, rows in order, entries of one row in any order).
 #include "common.hh"
+#include "stream.hh"
 
 #include <cstdlib>
 
@@ -170,14 +171,18 @@ __global__ void __launch_bounds__(TR_COUNT_THREADS) tr_count_kernel(unsigned int
 // tr_count_kernel.
 //
 // The tile's entries are walked flat, not row by row, in groups of 4
-// consecutive entries (one 16-byte load per lane and group): pass p covers
-// 64 U groups from the tile's first group + 64 U p, wave w takes passes w,
-// w + 16, ..., and the loads of a wave's next pass are in flight while the
-// current one is counted.  Each lane finds its entries' row from the row
-// starts in LDS (binary search once per pass, then a forward walk).  The
-// group holding the tile's first entry may start in the previous tile and
-// the one holding its last may end in the next (those entries are skipped);
-// the array's final partial group, if any, is read one entry at a time.
+// consecutive entries (16-byte loads): pass p of a wave covers 64 U groups,
+// lane l taking U consecutive ones (4 U consecutive entries), and the loads
+// of the wave's next pass are in flight while the current one is counted.
+// A lane finds the row of its first entry once per pass (binary search over
+// the row starts in LDS); a lane's run crosses at most a row boundary or
+// two, so the per-entry work is the count, the band test and the order
+// test, and the row advance sits behind a wave-uniform branch that is
+// rarely taken.  Only the tile's first and last passes test entries
+// against the tile's bounds: the group holding the tile's first entry may
+// start in the previous tile and the one holding its last may end in the
+// next (those entries are skipped); the array's final partial group, if
+// any, is read one entry at a time.
 template<unsigned int U>
 __global__ void __launch_bounds__(TR_COUNT_THREADS) tr_count_packed_kernel(unsigned int cDim, unsigned int uDim, unsigned int nnz,
                                                                            const unsigned int *__restrict__ ptr,
@@ -186,6 +191,7 @@ __global__ void __launch_bounds__(TR_COUNT_THREADS) tr_count_packed_kernel(unsig
                                                                            unsigned int *__restrict__ flags) {
     constexpr unsigned int PG = 64u * U;  // groups per pass
     constexpr unsigned int NW = TR_COUNT_THREADS / 64u;
+    static_assert(TR_BAND == 256u, "band = column >> 8");
     extern __shared__ unsigned int lds_words[];
     const unsigned int hw = (uDim + 1u) >> 1;
     constexpr unsigned short NONE = 0xffffu;                          // band without entries in the row (so far)
@@ -205,31 +211,24 @@ __global__ void __launch_bounds__(TR_COUNT_THREADS) tr_count_packed_kernel(unsig
     const unsigned int g0 = E0 >> 2, g1 = (E1 + 3u) >> 2;  // groups [g0, g1) hold the tile
     const unsigned int ng = E1 > E0 ? g1 - g0 : 0u;
     const unsigned int gfull = nnz >> 2;                     // groups wholly inside the array
-    const uint4 *idx4 = (const uint4 *) idx;
+    const unsigned int Ev = E1 < gfull * 4u ? E1 : gfull * 4u;  // entries read by the group loads: [E0, Ev)
+    const CSAMD_GLOBAL u32x4 *idx4 = (const CSAMD_GLOBAL u32x4 *) gmem(idx);
+    const CSAMD_GLOBAL unsigned int *idxg = gmem(idx);
 
-    // 16-byte loads only (groups past the array's last whole group read that
-    // group; their entries are skipped here and visited one by one at the end)
-    auto load = [&](unsigned int p, uint4 (&c)[U], unsigned int &before) {
-        const unsigned int gp = g0 + p * PG;
+    // lane's groups of pass p: [g0 + p PG + lane U, + U), clamped into the
+    // tile's groups and the array's whole groups; `before` is the column of
+    // the entry before the lane's first (lane 0: loaded; others: the
+    // previous lane's last, moved over once the loads land)
+    auto load = [&](unsigned int p, u32x4 (&c)[U], unsigned int &before) {
+        const unsigned int gp = g0 + p * PG + lane * U;
 #pragma unroll
         for (unsigned int u = 0; u < U; ++u) {
-            const unsigned int g = gp + 64u * u + lane;
+            const unsigned int g = gp + u;
             const unsigned int gc = g < g1 ? g : g1 - 1u;
             c[u] = idx4[gc < gfull ? gc : gfull - 1u];
         }
-        before = idx[gp > 0u ? gp * 4u - 1u : 0u];
-    };
-    // one entry: i in [E0, E1), its column cc, the previous entry's column pc
-    // (used only when i is not its row's first), its row's start rs
-    auto visit = [&](unsigned int i, unsigned int cc, unsigned int pc, unsigned int row, unsigned int rs, bool &bad,
-                     bool &unsorted) {
-        bad |= cc >= uDim;
-        const unsigned int cs = cc < uDim ? cc : 0u;  // out of range: reported, not counted right
-        atomicAdd(&hist[cs >> 1], 1u << ((cs & 1u) << 4));
-        const bool inner = i > rs;
-        unsorted |= inner && pc >= cc;
-        const unsigned int band = cs / TR_BAND;
-        if (!inner || band != (pc < uDim ? pc : 0u) / TR_BAND) off[band * TR_ROWS + row] = (unsigned short) (i - rs);
+        const unsigned int pb = g0 + p * PG;
+        before = idxg[pb > 0u ? pb * 4u - 1u : 0u];
     };
     // last row with rp[row] <= i (i in [E0, E1))
     auto row_of = [&](unsigned int i) -> unsigned int {
@@ -241,40 +240,84 @@ __global__ void __launch_bounds__(TR_COUNT_THREADS) tr_count_packed_kernel(unsig
         }
         return lo;
     };
-    const unsigned int Ev = E1 < gfull * 4u ? E1 : gfull * 4u;  // entries read by the group loads: [E0, Ev)
-    auto count = [&](unsigned int p, const uint4 (&c)[U], unsigned int before) {
-        const unsigned int ib = (g0 + p * PG) * 4u;  // first entry of the pass
-        unsigned int i = ib + 4u * lane;
-        unsigned int row = row_of(i < E0 ? E0 : (i < E1 ? i : E1 - 1u));
+    unsigned int cmax = 0u;   // largest column seen (range check at the end)
+    bool unsorted = false;
+    // unused band-start slot of this lane (stores of entries that open no band)
+    unsigned short *dummy = off + (size_t) bands * TR_ROWS + 2u * tid;
+    auto count = [&](unsigned int p, const u32x4 (&c)[U], unsigned int before) {
+        const unsigned int ib = (g0 + p * PG + lane * U) * 4u;  // lane's first entry
+        const unsigned int up = (unsigned int) __shfl_up((int) c[U - 1u].w, 1);
+        unsigned int pc = lane == 0u ? before : up;  // column of the entry before this one
+        unsigned int row = row_of(ib < E0 ? E0 : (ib < E1 ? ib : E1 - 1u));
         unsigned int rs = rp[row], re = rp[row + 1u];
-        unsigned int carry = before;  // column of the entry before the group row u
-        bool bad = false, unsorted = false;
+        // wave-uniform: does the pass reach outside [E0, Ev)?
+        const bool edge = (g0 + p * PG) * 4u < E0 || (g0 + (p + 1u) * PG) * 4u > Ev;
+        // Fast form: the lane's run lies in its row and at most one more row
+        // b (the row of entry re); an entry then takes row / rs, or b / re
+        // once it reaches re, with no branch.
+        unsigned int rowb = row, rsb = rs;
+        bool one_more = true;
+        if (!edge && re < ib + 4u * U) {
+            rowb = row_of(re);
+            rsb = rp[rowb];
+            one_more = rp[rowb + 1u] >= ib + 4u * U;
+        }
+        if (!edge && __builtin_amdgcn_ballot_w64(!one_more) == 0ull) {
+            bool uns = false;
+#pragma unroll
+            for (unsigned int u = 0; u < U; ++u) {
+                const unsigned int col[4] = {c[u].x, c[u].y, c[u].z, c[u].w};
+#pragma unroll
+                for (unsigned int e = 0; e < 4u; ++e) {
+                    const unsigned int i = ib + 4u * u + e;
+                    const unsigned int cc = col[e];
+                    const bool nb = i >= re;
+                    const unsigned int r = nb ? rowb : row;
+                    const unsigned int rsi = nb ? rsb : rs;
+                    cmax = cc > cmax ? cc : cmax;
+                    const unsigned int cs = cc < uDim ? cc : 0u;  // out of range: reported, not counted right
+                    atomicAdd(&hist[cs >> 1], 1u << ((cs & 1u) << 4));
+                    const bool inner = i != rsi;
+                    uns |= inner && pc >= cc;
+                    const bool open = !inner || (cs >> 8) != (pc >> 8);
+                    unsigned short *dst = open ? off + (cs >> 8) * TR_ROWS + r : dummy;
+                    *dst = (unsigned short) (i - rsi);
+                    pc = cc;
+                }
+            }
+            unsorted |= uns;
+            return;
+        }
 #pragma unroll
         for (unsigned int u = 0; u < U; ++u) {
-            const unsigned int up = (unsigned int) __shfl_up((int) c[u].w, 1);
-            const unsigned int first_prev = lane == 0u ? carry : up;
-            carry = (unsigned int) __shfl((int) c[u].w, 63);
             const unsigned int col[4] = {c[u].x, c[u].y, c[u].z, c[u].w};
-            const unsigned int base = ib + 256u * u + 4u * lane;
 #pragma unroll
             for (unsigned int e = 0; e < 4u; ++e) {
-                i = base + e;
-                if (i < E0 || i >= Ev) continue;
-                while (i >= re) {  // forward to the entry's row (empty rows skipped)
+                const unsigned int i = ib + 4u * u + e;
+                const unsigned int cc = col[e];
+                while (i >= re && i < E1) {  // forward to the entry's row (empty rows skipped)
                     ++row;
                     rs = re;
                     re = rp[row + 1u];
                 }
-                visit(i, col[e], e == 0u ? first_prev : col[e == 0u ? 0u : e - 1u], row, rs, bad, unsorted);
+                const bool valid = !edge || (i >= E0 && i < Ev);
+                if (valid) {
+                    cmax = cc > cmax ? cc : cmax;
+                    const unsigned int cs = cc < uDim ? cc : 0u;
+                    atomicAdd(&hist[cs >> 1], 1u << ((cs & 1u) << 4));
+                    const bool inner = i != rs;
+                    unsorted |= inner && pc >= cc;
+                    const unsigned int band = cs >> 8;
+                    if (!inner || band != (pc >> 8)) off[band * TR_ROWS + row] = (unsigned short) (i - rs);
+                }
+                pc = cc;
             }
         }
-        if (bad) flags[1] = 1u;
-        if (unsorted) flags[0] = 1u;
     };
     // two register sets in turn: the next pass loads while this one counts
     unsigned int p = wave;
     if (p * PG < ng) {
-        uint4 ca[U], cb[U];
+        u32x4 ca[U], cb[U];
         unsigned int ba, bb;
         load(p, ca, ba);
         for (;;) {
@@ -292,15 +335,25 @@ __global__ void __launch_bounds__(TR_COUNT_THREADS) tr_count_packed_kernel(unsig
             p = pn;
         }
     }
+    if (cmax >= uDim) flags[1] = 1u;
+    if (unsorted) flags[0] = 1u;
     // the array's final partial group (at most 3 entries, last tile only)
     if (tid == 0u && Ev < E1) {
-        bool bad = false, unsorted = false;
+        bool bad = false, uns = false;
         for (unsigned int i = Ev > E0 ? Ev : E0; i < E1; ++i) {
             const unsigned int row = row_of(i);
-            visit(i, idx[i], idx[i > 0u ? i - 1u : 0u], row, rp[row], bad, unsorted);
+            const unsigned int rs = rp[row];
+            const unsigned int cc = idx[i], pc = idx[i > 0u ? i - 1u : 0u];
+            bad |= cc >= uDim;
+            const unsigned int cs = cc < uDim ? cc : 0u;
+            atomicAdd(&hist[cs >> 1], 1u << ((cs & 1u) << 4));
+            const bool inner = i > rs;
+            uns |= inner && pc >= cc;
+            const unsigned int band = cs >> 8;
+            if (!inner || band != (pc < uDim ? pc : 0u) >> 8) off[band * TR_ROWS + row] = (unsigned short) (i - rs);
         }
         if (bad) flags[1] = 1u;
-        if (unsorted) flags[0] = 1u;
+        if (uns) flags[0] = 1u;
     }
     __syncthreads();
     unsigned int *tile_counts = counts + (size_t) t * uDim;
@@ -321,7 +374,9 @@ __global__ void __launch_bounds__(TR_COUNT_THREADS) tr_count_packed_kernel(unsig
 }
 
 size_t count_packed_lds_bytes(unsigned int uDim, unsigned int bands) {
-    return (size_t) ((uDim + 1u) >> 1) * 4u + (size_t) (TR_ROWS + 4u) * 4u + (size_t) bands * TR_ROWS * 2u;
+    // counters, row starts, band-start table, one unused slot per thread
+    return (size_t) ((uDim + 1u) >> 1) * 4u + (size_t) (TR_ROWS + 4u) * 4u + (size_t) bands * TR_ROWS * 2u +
+           (size_t) TR_COUNT_THREADS * 4u;
 }
 
 // K2: counts[t][c] <- sum_{t' < t} counts[t'][c] (in place), coltot[c] = total.
@@ -387,6 +442,33 @@ __device__ __forceinline__ void tr_chunk(const tr_chunk_order &o, unsigned int L
     t = full * o.group + (rem - band * last);
 }
 
+// Staged entry of the chunk sort: batch row | segment << 8 | value bits,
+// one register for 1- and 2-byte values, two for 4-byte values.
+template<typename S> struct tr_rec {
+    using T = unsigned int;
+    static __device__ __forceinline__ T pack(unsigned int r, unsigned int seg, S v) { return r | (seg << 8) | ((unsigned int) v << 16); }
+    static __device__ __forceinline__ unsigned int key(T x) { return x; }
+    static __device__ __forceinline__ S val(T x) { return (S) (x >> 16); }
+};
+template<> struct tr_rec<float> {
+    using T = uint2;
+    static __device__ __forceinline__ T pack(unsigned int r, unsigned int seg, float v) { return make_uint2(r | (seg << 8), __float_as_uint(v)); }
+    static __device__ __forceinline__ unsigned int key(T x) { return x.x; }
+    static __device__ __forceinline__ float val(T x) { return __uint_as_float(x.y); }
+};
+
+// LDS of tr_band_kernel: {rows mask, run start} per (mask word, segment),
+// per-row band starts and run offsets, per-segment output deltas and
+// cursors, then the batch's entry map, which the sorted staging reuses.
+struct tr_band_lds {
+    static constexpr size_t mp = 0;
+    static constexpr size_t seg_b = mp + (size_t) TR_MASK_WORDS * TR_BAND * 8u;
+    static constexpr size_t seg_o = seg_b + TR_ROWS * 4u;
+    static constexpr size_t delta = seg_o + (TR_ROWS + 1u) * 4u;
+    static constexpr size_t gcur = delta + TR_BAND * 4u;
+    static constexpr size_t stage = (gcur + TR_BAND * 4u + 15u) & ~(size_t) 15u;
+};
+
 template<typename S, unsigned int K>
 __global__ void __launch_bounds__(TR_SORT_THREADS) tr_band_kernel(unsigned int cDim, unsigned int uDim, const unsigned int *__restrict__ idx,
                                                                   const S *__restrict__ val, const unsigned int *__restrict__ tile_off,
@@ -395,19 +477,19 @@ __global__ void __launch_bounds__(TR_SORT_THREADS) tr_band_kernel(unsigned int c
                                                                   unsigned int *__restrict__ out_idx, S *__restrict__ out_val) {
     static_assert(TR_SORT_THREADS == TR_ROWS && TR_SORT_THREADS == TR_BAND, "one row and one segment per thread");
     static_assert(TR_ROWS <= 256u && TR_BAND <= 256u, "row and segment packed in 8 bits each");
+    using R = tr_rec<S>;
+    using RT = typename R::T;
     constexpr unsigned int T = TR_SORT_THREADS;
     constexpr unsigned int cap = K * T;
     extern __shared__ __align__(16) unsigned char lds[];
-    unsigned int *masks = (unsigned int *) lds;                  // [TR_MASK_WORDS][TR_BAND] word-major: lanes on
-                                                                 // different segments hit different banks
-    unsigned int *seg_b = masks + TR_BAND * TR_MASK_WORDS;       // [TR_ROWS] row starts in the band
-    unsigned int *seg_o = seg_b + TR_ROWS;                       // [TR_ROWS + 1] chunk offsets of the rows
-    unsigned int *delta = seg_o + TR_ROWS + 1u;                  // [TR_BAND + 1] output position - staging slot
-    unsigned int *gcur = delta + TR_BAND + 1u;                   // [TR_BAND] output cursors
-    unsigned short *pre = (unsigned short *) (gcur + TR_BAND + 2u);  // [TR_MASK_WORDS][TR_BAND] run start + rows in lower words
-    unsigned char *s_in = (unsigned char *) (pre + TR_MASK_WORDS * TR_BAND);  // [cap] batch row, input order
-    unsigned short *s_rs = (unsigned short *) (s_in + cap);      // [cap] batch row | segment << 8, output order
-    S *s_val = (S *) (s_rs + cap);                               // [cap]
+    uint2 *mp = (uint2 *) (lds + tr_band_lds::mp);                       // [TR_MASK_WORDS][TR_BAND] word-major: lanes
+                                                                         // on different segments hit different banks
+    unsigned int *seg_b = (unsigned int *) (lds + tr_band_lds::seg_b);   // [TR_ROWS] row's band start (entry index)
+    unsigned int *seg_o = (unsigned int *) (lds + tr_band_lds::seg_o);   // [TR_ROWS + 1] chunk offsets of the rows' runs
+    unsigned int *delta = (unsigned int *) (lds + tr_band_lds::delta);   // [TR_BAND] output position - staging slot
+    unsigned int *gcur = (unsigned int *) (lds + tr_band_lds::gcur);     // [TR_BAND] output cursors
+    unsigned int *s_map = (unsigned int *) (lds + tr_band_lds::stage);   // [cap] entry -> (offset from the batch start) << 8 | row
+    RT *s_out = (RT *) (lds + tr_band_lds::stage);                       // [cap] sorted entries (after the map is read)
     __shared__ unsigned int scan_smem[16];
 
     const unsigned int tid = threadIdx.x;
@@ -422,7 +504,7 @@ __global__ void __launch_bounds__(TR_SORT_THREADS) tr_band_kernel(unsigned int c
     const unsigned int nr = cDim - r0 < TR_ROWS ? cDim - r0 : TR_ROWS;
     const unsigned int *rb_lo = rbt + ((size_t) t * (bands + 1u) + band) * TR_ROWS;
 
-    for (unsigned int w = tid; w < TR_BAND * TR_MASK_WORDS; w += T) masks[w] = 0u;
+    for (unsigned int w = tid; w < TR_BAND * TR_MASK_WORDS; w += T) mp[w] = make_uint2(0u, 0u);
     // segments past uDim start at 0 and hold no entries
     gcur[tid] = tid < nb ? out_ptr[c0 + tid] + tile_off[(size_t) t * uDim + c0 + tid] : 0u;
     {
@@ -435,43 +517,65 @@ __global__ void __launch_bounds__(TR_SORT_THREADS) tr_band_kernel(unsigned int c
         if (tid == 0u) seg_o[TR_ROWS] = total;
     }
     __syncthreads();
-    const unsigned int first = seg_b[0];
-    const char *ib = (const char *) (idx + first);
-    const char *vbase = (const char *) (val + first);
 
     unsigned int rb0 = 0u;
     while (rb0 < nr) {
         // batch: rows [rb0, rb1), the most whole rows within cap entries
-        // (seg_o is nondecreasing, and one row holds at most TR_BAND <= cap
-        // entries of a band, so a batch holds at least one row)
+        // whose span in the arrays stays below 2^24 entries (seg_o and the
+        // band ends are nondecreasing; one row holds at most TR_BAND <= cap
+        // entries of a band and spans at most uDim < 2^24 entries, so a
+        // batch holds at least one row).  The barrier also ends the previous
+        // batch: its D and E, and its reads of the staging and delta, finish
+        // before this batch writes them.
         const unsigned int base = seg_o[rb0];
-        const int fits = tid >= rb0 && tid < nr && seg_o[tid + 1u] - base <= cap;
-        // (also the barrier after the previous batch: its D and E, and its
-        // reads of s_rs / s_val / delta, finish before this batch writes them)
+        const unsigned int b0 = seg_b[rb0];
+        const int fits = tid >= rb0 && tid < nr && seg_o[tid + 1u] - base <= cap &&
+                         seg_b[tid] + (seg_o[tid + 1u] - seg_o[tid]) - b0 < (1u << 24);
         const unsigned int rb1 = rb0 + (unsigned int) __syncthreads_count(fits);
+        if (rb1 == rb0) break;  // unreachable for a valid band table (uniform): never loop without progress
         const unsigned int n = seg_o[rb1] - base;
+        if (n == 0u) {  // rows without entries in the band (uniform)
+            rb0 = rb1;
+            continue;
+        }
+        // map: the row owner lists its entries' offsets from the batch start
         if (tid >= rb0 && tid < rb1) {
             const unsigned int o = seg_o[tid] - base, len = seg_o[tid + 1u] - seg_o[tid];
-            for (unsigned int k = 0; k < len; ++k) s_in[o + k] = (unsigned char) (tid - rb0);
+            const unsigned int tag = ((seg_b[tid] - b0) << 8) | (tid - rb0);
+            for (unsigned int k = 0; k < len; ++k) s_map[o + k] = tag + (k << 8);
         }
         __syncthreads();
         // A: load this thread's entries (all loads issued before any use),
-        // then mark which rows hold each segment
-        unsigned int c[K], r[K];
-        S v[K];
+        // pack each as one staged record, mark which rows hold each segment
+        // (32-bit byte offsets from the batch start: the offsets stay below
+        // 2^26, so each load is one base register pair + one offset)
+        const CSAMD_GLOBAL char *ib = (const CSAMD_GLOBAL char *) gmem(idx + b0);
+        const CSAMD_GLOBAL char *vbp = (const CSAMD_GLOBAL char *) gmem(val + b0);
+        RT x[K];
+        {
+            unsigned int m[K];
 #pragma unroll
-        for (unsigned int k = 0; k < K; ++k) {
-            const unsigned int j = tid + k * T;
-            const unsigned int jj = j < n ? j : (n != 0u ? n - 1u : 0u);
-            r[k] = n != 0u ? s_in[jj] : 0u;
-            const unsigned int row = rb0 + r[k];
-            const unsigned int li = n != 0u ? seg_b[row] + (base + jj - seg_o[row]) - first : 0u;
-            c[k] = n != 0u ? *(const unsigned int *) (ib + li * 4u) : c0;
-            v[k] = n != 0u ? *(const S *) (vbase + li * (unsigned int) sizeof(S)) : S(0);
+            for (unsigned int k = 0; k < K; ++k) {
+                const unsigned int j = tid + k * T;
+                m[k] = s_map[j < n ? j : n - 1u];
+            }
+            unsigned int c[K];
+            S v[K];
+#pragma unroll
+            for (unsigned int k = 0; k < K; ++k) {
+                c[k] = *(const CSAMD_GLOBAL unsigned int *) (ib + (m[k] >> 8) * 4u);
+                v[k] = *(const CSAMD_GLOBAL S *) (vbp + (m[k] >> 8) * (unsigned int) sizeof(S));
+            }
+#pragma unroll
+            for (unsigned int k = 0; k < K; ++k) x[k] = R::pack(m[k] & 255u, c[k] - c0, v[k]);
         }
 #pragma unroll
-        for (unsigned int k = 0; k < K; ++k)
-            if (tid + k * T < n) atomicOr(&masks[(r[k] >> 5) * TR_BAND + (c[k] - c0)], 1u << (r[k] & 31u));
+        for (unsigned int k = 0; k < K; ++k) {
+            if (tid + k * T >= n) continue;
+            const unsigned int key = R::key(x[k]);
+            const unsigned int r = key & 255u, seg = (key >> 8) & 255u;
+            atomicOr(&mp[(r >> 5) * TR_BAND + seg].x, 1u << (r & 31u));
+        }
         __syncthreads();
         // B: run lengths -> run starts in the staging, each mask word's
         // first slot (run start + rows in the lower words), and each
@@ -481,7 +585,7 @@ __global__ void __launch_bounds__(TR_SORT_THREADS) tr_band_kernel(unsigned int c
             unsigned int cw[TR_MASK_WORDS];
 #pragma unroll
             for (unsigned int w = 0; w < TR_MASK_WORDS; ++w) {
-                cw[w] = (unsigned int) __popc(masks[w * TR_BAND + tid]);
+                cw[w] = (unsigned int) __popc(mp[w * TR_BAND + tid].x);
                 cnt += cw[w];
             }
             unsigned int total = 0u;
@@ -489,36 +593,40 @@ __global__ void __launch_bounds__(TR_SORT_THREADS) tr_band_kernel(unsigned int c
             delta[tid] = gcur[tid] - o;
 #pragma unroll
             for (unsigned int w = 0; w < TR_MASK_WORDS; ++w) {
-                pre[w * TR_BAND + tid] = (unsigned short) o;
+                mp[w * TR_BAND + tid].y = o;
                 o += cw[w];
             }
         }
         __syncthreads();
-        // C: every entry to its run start + rank among earlier rows
+        // C: every entry to its run start + rank among earlier rows (the map
+        // was last read in A, so the staging may overwrite it).  The key is
+        // made opaque so that A's mask addresses and bits are recomputed here
+        // rather than held in registers across the barriers.
 #pragma unroll
         for (unsigned int k = 0; k < K; ++k) {
             if (tid + k * T >= n) continue;
-            const unsigned int cl = c[k] - c0;
-            const unsigned int wi = (r[k] >> 5) * TR_BAND + cl;
-            const unsigned int d = pre[wi] + (unsigned int) __popc(masks[wi] & ((1u << (r[k] & 31u)) - 1u));
-            s_rs[d] = (unsigned short) (r[k] | (cl << 8));
-            s_val[d] = v[k];
+            unsigned int key = R::key(x[k]);
+            asm volatile("" : "+v"(key));
+            const unsigned int r = key & 255u, seg = (key >> 8) & 255u;
+            const uint2 u = mp[(r >> 5) * TR_BAND + seg];
+            s_out[u.y + (unsigned int) __popc(u.x & ((1u << (r & 31u)) - 1u))] = x[k];
         }
         __syncthreads();
         // D: runs to their final places (consecutive staging slots of one run
         // are consecutive output positions)
         for (unsigned int k = tid; k < n; k += T) {
-            const unsigned int rs = s_rs[k];
-            const unsigned int pos = delta[rs >> 8] + k;
-            out_idx[pos] = r0 + rb0 + (rs & 255u);
-            out_val[pos] = s_val[k];
+            const RT e = s_out[k];
+            const unsigned int key = R::key(e);
+            const unsigned int pos = delta[(key >> 8) & 255u] + k;
+            out_idx[pos] = r0 + rb0 + (key & 255u);
+            out_val[pos] = R::val(e);
         }
         // E: advance the cursors, clear the masks.  No barrier before: D reads
         // neither (gcur was folded into delta in B, the masks were last read
         // in C); the next batch's first barrier orders E before its A.
         gcur[tid] += cnt;
 #pragma unroll
-        for (unsigned int w = 0; w < TR_MASK_WORDS; ++w) masks[w * TR_BAND + tid] = 0u;
+        for (unsigned int w = 0; w < TR_MASK_WORDS; ++w) mp[w * TR_BAND + tid] = make_uint2(0u, 0u);
         rb0 = rb1;
     }
 }
@@ -554,9 +662,7 @@ __global__ void __launch_bounds__(64) tr_scatter_kernel(unsigned int cDim, unsig
 }
 
 size_t band_lds_bytes(unsigned int cap, size_t vb) {
-    const size_t words = (size_t) TR_BAND * TR_MASK_WORDS + TR_ROWS + (TR_ROWS + 1u) + (TR_BAND + 1u) + TR_BAND + 2u +
-                         (size_t) TR_MASK_WORDS * TR_BAND / 2u;
-    return align_up(words * 4u + (size_t) cap * 3u + (size_t) cap * vb, 16u);
+    return tr_band_lds::stage + (size_t) cap * (vb == 4u ? 8u : 4u);
 }
 
 template<typename S>
