@@ -457,16 +457,20 @@ template<> struct tr_rec<float> {
     static __device__ __forceinline__ float val(T x) { return __uint_as_float(x.y); }
 };
 
-// LDS of tr_band_kernel: {rows mask, run start} per (mask word, segment),
-// per-row band starts and run offsets, per-segment output deltas and
-// cursors, then the batch's entry map, which the sorted staging reuses.
+// LDS of tr_band_kernel: the rows mask and the run start per (mask word,
+// segment), per-row band starts and run offsets, per-segment output deltas,
+// then the batch's entry map, which the sorted staging reuses (about 40 KB
+// at 6,144 entries of 2-byte values, so four workgroups fit in a CU's LDS;
+// a batch that holds most chunks whole matters more than a fifth
+// workgroup: 4,096-entry batches with five workgroups per CU were 8 %
+// slower).
 struct tr_band_lds {
-    static constexpr size_t mp = 0;
-    static constexpr size_t seg_b = mp + (size_t) TR_MASK_WORDS * TR_BAND * 8u;
+    static constexpr size_t masks = 0;
+    static constexpr size_t pre = masks + (size_t) TR_MASK_WORDS * TR_BAND * 4u;
+    static constexpr size_t seg_b = pre + (size_t) TR_MASK_WORDS * TR_BAND * 2u;
     static constexpr size_t seg_o = seg_b + TR_ROWS * 4u;
     static constexpr size_t delta = seg_o + (TR_ROWS + 1u) * 4u;
-    static constexpr size_t gcur = delta + TR_BAND * 4u;
-    static constexpr size_t stage = (gcur + TR_BAND * 4u + 15u) & ~(size_t) 15u;
+    static constexpr size_t stage = (delta + TR_BAND * 4u + 15u) & ~(size_t) 15u;
 };
 
 template<typename S, unsigned int K>
@@ -482,12 +486,13 @@ __global__ void __launch_bounds__(TR_SORT_THREADS) tr_band_kernel(unsigned int c
     constexpr unsigned int T = TR_SORT_THREADS;
     constexpr unsigned int cap = K * T;
     extern __shared__ __align__(16) unsigned char lds[];
-    uint2 *mp = (uint2 *) (lds + tr_band_lds::mp);                       // [TR_MASK_WORDS][TR_BAND] word-major: lanes
-                                                                         // on different segments hit different banks
+    unsigned int *masks = (unsigned int *) (lds + tr_band_lds::masks);   // [TR_MASK_WORDS][TR_BAND] rows holding each segment,
+                                                                         // word-major: lanes on different segments hit
+                                                                         // different banks
+    unsigned short *pre = (unsigned short *) (lds + tr_band_lds::pre);   // [TR_MASK_WORDS][TR_BAND] run start of each mask word
     unsigned int *seg_b = (unsigned int *) (lds + tr_band_lds::seg_b);   // [TR_ROWS] row's band start (entry index)
     unsigned int *seg_o = (unsigned int *) (lds + tr_band_lds::seg_o);   // [TR_ROWS + 1] chunk offsets of the rows' runs
     unsigned int *delta = (unsigned int *) (lds + tr_band_lds::delta);   // [TR_BAND] output position - staging slot
-    unsigned int *gcur = (unsigned int *) (lds + tr_band_lds::gcur);     // [TR_BAND] output cursors
     unsigned int *s_map = (unsigned int *) (lds + tr_band_lds::stage);   // [cap] entry -> (offset from the batch start) << 8 | row
     RT *s_out = (RT *) (lds + tr_band_lds::stage);                       // [cap] sorted entries (after the map is read)
     __shared__ unsigned int scan_smem[16];
@@ -504,9 +509,10 @@ __global__ void __launch_bounds__(TR_SORT_THREADS) tr_band_kernel(unsigned int c
     const unsigned int nr = cDim - r0 < TR_ROWS ? cDim - r0 : TR_ROWS;
     const unsigned int *rb_lo = rbt + ((size_t) t * (bands + 1u) + band) * TR_ROWS;
 
-    for (unsigned int w = tid; w < TR_BAND * TR_MASK_WORDS; w += T) mp[w] = make_uint2(0u, 0u);
+    for (unsigned int w = tid; w < TR_BAND * TR_MASK_WORDS; w += T) masks[w] = 0u;
     // segments past uDim start at 0 and hold no entries
-    gcur[tid] = tid < nb ? out_ptr[c0 + tid] + tile_off[(size_t) t * uDim + c0 + tid] : 0u;
+    // thread tid owns segment tid's output cursor
+    unsigned int gcur = tid < nb ? out_ptr[c0 + tid] + tile_off[(size_t) t * uDim + c0 + tid] : 0u;
     {
         const unsigned int b = tid < nr ? rb_lo[tid] : 0u;
         const unsigned int n = tid < nr ? rb_lo[tid + TR_ROWS] - b : 0u;
@@ -574,7 +580,7 @@ __global__ void __launch_bounds__(TR_SORT_THREADS) tr_band_kernel(unsigned int c
             if (tid + k * T >= n) continue;
             const unsigned int key = R::key(x[k]);
             const unsigned int r = key & 255u, seg = (key >> 8) & 255u;
-            atomicOr(&mp[(r >> 5) * TR_BAND + seg].x, 1u << (r & 31u));
+            atomicOr(&masks[(r >> 5) * TR_BAND + seg], 1u << (r & 31u));
         }
         __syncthreads();
         // B: run lengths -> run starts in the staging, each mask word's
@@ -585,15 +591,15 @@ __global__ void __launch_bounds__(TR_SORT_THREADS) tr_band_kernel(unsigned int c
             unsigned int cw[TR_MASK_WORDS];
 #pragma unroll
             for (unsigned int w = 0; w < TR_MASK_WORDS; ++w) {
-                cw[w] = (unsigned int) __popc(mp[w * TR_BAND + tid].x);
+                cw[w] = (unsigned int) __popc(masks[w * TR_BAND + tid]);
                 cnt += cw[w];
             }
             unsigned int total = 0u;
             unsigned int o = block_exclusive_scan_u32(cnt, scan_smem, &total);
-            delta[tid] = gcur[tid] - o;
+            delta[tid] = gcur - o;
 #pragma unroll
             for (unsigned int w = 0; w < TR_MASK_WORDS; ++w) {
-                mp[w * TR_BAND + tid].y = o;
+                pre[w * TR_BAND + tid] = (unsigned short) o;
                 o += cw[w];
             }
         }
@@ -608,8 +614,8 @@ __global__ void __launch_bounds__(TR_SORT_THREADS) tr_band_kernel(unsigned int c
             unsigned int key = R::key(x[k]);
             asm volatile("" : "+v"(key));
             const unsigned int r = key & 255u, seg = (key >> 8) & 255u;
-            const uint2 u = mp[(r >> 5) * TR_BAND + seg];
-            s_out[u.y + (unsigned int) __popc(u.x & ((1u << (r & 31u)) - 1u))] = x[k];
+            const unsigned int wi = (r >> 5) * TR_BAND + seg;
+            s_out[pre[wi] + (unsigned int) __popc(masks[wi] & ((1u << (r & 31u)) - 1u))] = x[k];
         }
         __syncthreads();
         // D: runs to their final places (consecutive staging slots of one run
@@ -624,9 +630,9 @@ __global__ void __launch_bounds__(TR_SORT_THREADS) tr_band_kernel(unsigned int c
         // E: advance the cursors, clear the masks.  No barrier before: D reads
         // neither (gcur was folded into delta in B, the masks were last read
         // in C); the next batch's first barrier orders E before its A.
-        gcur[tid] += cnt;
+        gcur += cnt;
 #pragma unroll
-        for (unsigned int w = 0; w < TR_MASK_WORDS; ++w) mp[w * TR_BAND + tid] = make_uint2(0u, 0u);
+        for (unsigned int w = 0; w < TR_MASK_WORDS; ++w) masks[w * TR_BAND + tid] = 0u;
         rb0 = rb1;
     }
 }
@@ -679,7 +685,7 @@ int launch_scatter(const tr_layout &L, bool sorted, unsigned int cDim, unsigned 
         const tr_chunk_order order{L.tiles, L.bands, (unsigned int) per, gw >= 1u && gw <= 64u ? gw : 8u};
         // staging of K x 256 entries per batch (CSAMD_TR_K = 8 / 12 / 16 / 20 / 24 for sweeps)
         const char *env = std::getenv("CSAMD_TR_K");
-        const int kw = env != nullptr ? std::atoi(env) : 20;
+        const int kw = env != nullptr ? std::atoi(env) : 24;
         const unsigned int K = kw == 8 ? 8u : kw == 12 ? 12u : kw == 16 ? 16u : kw == 24 ? 24u : 20u;
         const size_t lds = band_lds_bytes(K * TR_SORT_THREADS, sizeof(S));
         auto k = K == 8u ? tr_band_kernel<S, 8u> : K == 12u ? tr_band_kernel<S, 12u> : K == 16u ? tr_band_kernel<S, 16u>
