@@ -8,12 +8,15 @@
 // counts iff its value is nonzero (mask_groups.cu:155).
 //
 // Device side: one workgroup per (query, feature) item, taken from a
-// per-launch counter.  The item's column of the shard's CSC is streamed
-// once (coalesced row index / value loads, four entries in flight per
-// thread); each kept entry lands in an LDS accumulator indexed by the
-// row's label; the G accumulators are then stored contiguously at
-// out[j * G] (feature-major output).  A query's work is a few tens of MB,
-// so batching many queries per launch is what keeps HBM busy.
+// per-launch counter (the next item is taken when the current one starts,
+// so the counter's round trip overlaps the item).  The item's column of the
+// shard's CSC is streamed once in chunks of 8 entries per thread through two
+// register buffers (coalesced row index / value loads, the next chunk in
+// flight while the current one is consumed); each kept entry lands in an LDS
+// accumulator indexed by the row's label; the G accumulators are then
+// stored contiguously at out[j * G] (feature-major output).  A query's work
+// is a few tens of MB, so batching many queries per launch is what keeps
+// HBM busy.
 #include "common.hh"
 
 #include <chrono>
@@ -61,36 +64,69 @@ struct query_dev {
 
 constexpr unsigned int SERVE_THREADS = 256u;
 constexpr unsigned int SERVE_MAX_BATCH = 256u;
-constexpr unsigned int SERVE_UNROLL = 4u;
+constexpr unsigned int SERVE_UNROLL = 8u;
+
+// One item's column [b, e): chunks of SERVE_THREADS x SERVE_UNROLL entries
+// (thread t takes entries t, t + SERVE_THREADS, ...) through two register
+// buffers in turn, the next chunk's loads in flight while the current one
+// is consumed.  Loads are unconditional (indices clamped to the column's
+// last entry; entries past the end are masked when consumed), so the
+// compiler waits only for the chunk it consumes.
+template<int VT> struct serve_chunk {
+    unsigned int r[SERVE_UNROLL];
+    typename value_traits<VT>::storage v[SERVE_UNROLL];
+};
+
+template<int VT>
+__device__ __forceinline__ void serve_load(const CSAMD_GLOBAL unsigned int *rowidx, const CSAMD_GLOBAL typename value_traits<VT>::storage *val,
+                                           unsigned int i0, unsigned int last, serve_chunk<VT> &c) {
+#pragma unroll
+    for (unsigned int u = 0; u < SERVE_UNROLL; ++u) {
+        const unsigned int i = min(i0 + u * SERVE_THREADS, last);
+        c.r[u] = rowidx[i];
+        c.v[u] = val[i];
+    }
+}
+
+template<int VT>
+__device__ __forceinline__ void serve_consume(const query_dev &Q, unsigned int i0, unsigned int e, const serve_chunk<VT> &c, double *acc_d,
+                                              unsigned int *acc_u) {
+#pragma unroll
+    for (unsigned int u = 0; u < SERVE_UNROLL; ++u) {
+        const unsigned int r = c.r[u];
+        const float v = value_traits<VT>::decode(c.v[u]);
+        if (i0 + u * SERVE_THREADS >= e || r >= Q.rows || v == 0.0f) continue;
+        const bool kept = Q.keep != nullptr ? gmem(Q.keep)[r] != 0u : seeded_keep(Q.mask_seed, r, Q.keep_thr24);
+        if (!kept) continue;
+        const unsigned int g = Q.labels != nullptr ? gmem(Q.labels)[r] : seeded_label(Q.label_seed, r, Q.G);
+        if (g >= Q.G) continue;
+        if (Q.op == CSAMD_SERVE_NNZ) atomicAdd(acc_u + g, 1u);
+        else {
+            const double d = (double) v;
+            atomicAdd(acc_d + g, Q.op == CSAMD_SERVE_SUMSQ ? d * d : d);
+        }
+    }
+}
 
 template<int VT>
 __device__ __forceinline__ void serve_column(const query_dev &Q, unsigned int b, unsigned int e, double *acc_d, unsigned int *acc_u) {
     using S = typename value_traits<VT>::storage;
+    if (b >= e) return;  // uniform
     const CSAMD_GLOBAL S *__restrict__ val = gmem((const S *) Q.val);
     const CSAMD_GLOBAL unsigned int *__restrict__ rowidx = gmem(Q.rowidx);
-    for (unsigned int i0 = b + threadIdx.x; i0 < e; i0 += SERVE_THREADS * SERVE_UNROLL) {
-        unsigned int r[SERVE_UNROLL];
-        float v[SERVE_UNROLL];
-#pragma unroll
-        for (unsigned int u = 0; u < SERVE_UNROLL; ++u) {  // loads first
-            const unsigned int i = i0 + u * SERVE_THREADS;
-            const bool in = i < e;
-            r[u] = in ? rowidx[i] : 0xffffffffu;
-            v[u] = in ? value_traits<VT>::decode(val[i]) : 0.0f;
-        }
-#pragma unroll
-        for (unsigned int u = 0; u < SERVE_UNROLL; ++u) {
-            if (r[u] >= Q.rows || v[u] == 0.0f) continue;
-            const bool kept = Q.keep != nullptr ? gmem(Q.keep)[r[u]] != 0u : seeded_keep(Q.mask_seed, r[u], Q.keep_thr24);
-            if (!kept) continue;
-            const unsigned int g = Q.labels != nullptr ? gmem(Q.labels)[r[u]] : seeded_label(Q.label_seed, r[u], Q.G);
-            if (g >= Q.G) continue;
-            if (Q.op == CSAMD_SERVE_NNZ) atomicAdd(acc_u + g, 1u);
-            else {
-                const double d = (double) v[u];
-                atomicAdd(acc_d + g, Q.op == CSAMD_SERVE_SUMSQ ? d * d : d);
-            }
-        }
+    constexpr unsigned int step = SERVE_THREADS * SERVE_UNROLL;
+    const unsigned int last = e - 1u;
+    unsigned int i0 = b + threadIdx.x;
+    serve_chunk<VT> A, B;
+    serve_load<VT>(rowidx, val, i0, last, A);
+    // chunks start at b, b + step, ... (uniform count)
+    for (unsigned int c0 = b; c0 < e; c0 += 2u * step) {
+        serve_load<VT>(rowidx, val, i0 + step, last, B);
+        serve_consume<VT>(Q, i0, e, A, acc_d, acc_u);
+        if (c0 + step >= e) break;
+        serve_load<VT>(rowidx, val, i0 + 2u * step, last, A);
+        serve_consume<VT>(Q, i0 + step, e, B, acc_d, acc_u);
+        i0 += 2u * step;
     }
 }
 
@@ -104,11 +140,14 @@ __global__ void __launch_bounds__(SERVE_THREADS) serve_batch_kernel(const query_
     for (unsigned int i = threadIdx.x; i <= nq; i += blockDim.x) pre[i] = prefix[i];
     __syncthreads();
     const unsigned int total = pre[nq];
-    while (true) {
-        if (threadIdx.x == 0u) item_s = atomicAdd(counter, 1u);
-        __syncthreads();
-        const unsigned int item = item_s;
-        if (item >= total) break;  // uniform
+    if (threadIdx.x == 0u) item_s = atomicAdd(counter, 1u);
+    __syncthreads();
+    unsigned int item = item_s;
+    while (item < total) {  // uniform
+        // the next item, taken now so that the counter's round trip
+        // overlaps this item's work
+        unsigned int nxt = 0u;
+        if (threadIdx.x == 0u) nxt = atomicAdd(counter, 1u);
         unsigned int lo = 0u, hi = nq - 1u;  // last q with pre[q] <= item
         while (lo < hi) {
             const unsigned int mid = (lo + hi + 1u) >> 1;
@@ -133,7 +172,9 @@ __global__ void __launch_bounds__(SERVE_THREADS) serve_batch_kernel(const query_
             for (unsigned int g = threadIdx.x; g < Q.G; g += blockDim.x) gmem((unsigned int *) Q.out)[o + g] = acc_u[g];
         else
             for (unsigned int g = threadIdx.x; g < Q.G; g += blockDim.x) gmem((double *) Q.out)[o + g] = acc_d[g];
-        __syncthreads();  // accumulators and item_s are reused
+        if (threadIdx.x == 0u) item_s = nxt;  // read by all after this item's first barrier
+        __syncthreads();                        // accumulators are reused; item_s published
+        item = item_s;
     }
 }
 
