@@ -15,6 +15,7 @@
 // one wave64 per row for Sliced-ELL emits (coalesced copy + padding).
 #include "common.hh"
 #include "sell_write.hh"
+#include "stream.hh"
 
 #include <cstdlib>
 
@@ -345,6 +346,83 @@ __device__ __forceinline__ unsigned int bell_chunk_mask(const csamd_blocked_ell_
     return m;
 }
 
+// Value e of a 16-byte chunk.
+template<typename S> __device__ __forceinline__ S chunk_val(const u32x4 &q, unsigned int e);
+template<> __device__ __forceinline__ uint16_t chunk_val<uint16_t>(const u32x4 &q, unsigned int e) {
+    const unsigned int w = lane_of(q, e >> 1);
+    return (uint16_t) ((e & 1u) ? (w >> 16) : (w & 0xffffu));
+}
+template<> __device__ __forceinline__ float chunk_val<float>(const u32x4 &q, unsigned int e) { return __uint_as_float(lane_of(q, e)); }
+template<> __device__ __forceinline__ uint8_t chunk_val<uint8_t>(const u32x4 &q, unsigned int e) {
+    return (uint8_t) (lane_of(q, e >> 2) >> ((e & 3u) * 8u));
+}
+
+// Bit e set when value e of the chunk is nonzero at a valid column < cols
+// (block column bc, first column col0).
+template<typename S>
+__device__ __forceinline__ unsigned int chunk_mask(const u32x4 &q, unsigned int bc, unsigned int col0, unsigned int cols) {
+    constexpr unsigned int PER = 16u / sizeof(S);
+    unsigned int m = 0u;
+#pragma unroll
+    for (unsigned int e = 0; e < PER; ++e)
+        if (col0 + e < cols && stored_nonzero<S>(chunk_val<S>(q, e))) m |= 1u << e;
+    return bc != CSAMD_INVALID_COL ? m : 0u;
+}
+
+template<typename S, unsigned int U> struct bell_row_buf {
+    u32x4 q[U];
+    unsigned int bc[U];
+};
+
+// Walk the 16-byte chunks of one Blocked-ELL value row with the whole wave,
+// U chunks per lane per step (chunk c0 + 64 u + lane), through two register
+// buffers in turn: the next step's value and block-column loads are in
+// flight while visit(c, q, bc) runs for the current step, u = 0, 1, ... in
+// order.  Loads past the row's last chunk re-read it (visit sees c >= nc
+// for them), so every load is unconditional.  Chunks per block must be a
+// power of two (2^cpb_log2).
+template<typename S, unsigned int U, typename Visit>
+__device__ __forceinline__ void bell_stream_row(const csamd_blocked_ell_view &src, unsigned int width, unsigned int cpb_log2,
+                                                unsigned int row, Visit visit) {
+    constexpr unsigned int PER = 16u / sizeof(S);
+    constexpr unsigned int STEP = 64u * U;
+    const unsigned int nc = src.ell_cols / PER;
+    if (nc == 0u) return;
+    const unsigned int lane = lane_id();
+    const CSAMD_GLOBAL u32x4 *vrow = (const CSAMD_GLOBAL u32x4 *) gmem((const S *) src.val + (unsigned long long) row * src.ell_cols);
+    const CSAMD_GLOBAL unsigned int *bcrow = gmem(src.blockColIdx + (unsigned long long) (row / src.block_size) * width);
+    auto load = [&](unsigned int c0, bell_row_buf<S, U> &B) {
+#pragma unroll
+        for (unsigned int u = 0; u < U; ++u) {
+            const unsigned int c = min(c0 + 64u * u + lane, nc - 1u);
+            B.q[u] = vrow[c];
+            B.bc[u] = bcrow[c >> cpb_log2];
+        }
+    };
+    bell_row_buf<S, U> A, B;
+    load(0u, A);
+    for (unsigned int c0 = 0u; c0 < nc; c0 += 2u * STEP) {
+        load(c0 + STEP, B);
+#pragma unroll
+        for (unsigned int u = 0; u < U; ++u) visit(c0 + 64u * u + lane, A.q[u], A.bc[u]);
+        if (c0 + STEP >= nc) break;
+        load(c0 + 2u * STEP, A);
+#pragma unroll
+        for (unsigned int u = 0; u < U; ++u) visit(c0 + STEP + 64u * u + lane, B.q[u], B.bc[u]);
+    }
+}
+
+// log2 of the 16-byte chunks per block when the streaming walk applies
+// (chunked layout, a power-of-two number of chunks per block), else -1.
+template<typename S>
+__device__ __forceinline__ int bell_stream_log2(const csamd_blocked_ell_view &src) {
+    if (!bell_chunks_ok<S>(src)) return -1;
+    const unsigned int cpb = src.block_size / (16u / (unsigned int) sizeof(S));
+    return (cpb & (cpb - 1u)) == 0u ? __ffs((int) cpb) - 1 : -1;
+}
+
+constexpr unsigned int BELL_STREAM_U = 4u;  // chunks per lane per streaming step
+
 // Blocked-ELL row nnz: nonzero values at valid columns (sliced_ell_from_blocked_ell.cuh:36-62).
 template<typename S>
 __global__ void bell_row_nnz_kernel(csamd_blocked_ell_view src, unsigned int *__restrict__ row_nnz) {
@@ -353,17 +431,18 @@ __global__ void bell_row_nnz_kernel(csamd_blocked_ell_view src, unsigned int *__
     const unsigned int bs = src.block_size;
     const unsigned int width = bs != 0u ? src.ell_cols / bs : 0u;
     const S *val = (const S *) src.val;
-    const bool chunks = bell_chunks_ok<S>(src);
+    const int slog = bell_stream_log2<S>(src);
     for (unsigned int row = blockIdx.x * nw + wave; row < src.rows; row += gridDim.x * nw) {
         const unsigned int rb = row / bs;
         unsigned int count = 0u;
-        if (chunks) {
-            const unsigned int nc = src.ell_cols / (16u / (unsigned int) sizeof(S));
-            for (unsigned int c = lane_id(); c < nc; c += 64u) {
-                bell_chunk_vals<S> g;
-                unsigned int c0;
-                count += (unsigned int) __popc(bell_chunk_mask<S>(src, width, row, c, &g, &c0));
-            }
+        if (slog >= 0) {
+            constexpr unsigned int PER = 16u / sizeof(S);
+            const unsigned int nc = src.ell_cols / PER;
+            const unsigned int cmask = (1u << slog) - 1u;
+            bell_stream_row<S, BELL_STREAM_U>(src, width, (unsigned int) slog, row, [&](unsigned int c, const u32x4 &q, unsigned int bc) {
+                const unsigned int col0 = bc * bs + (c & cmask) * PER;
+                if (c < nc) count += (unsigned int) __popc(chunk_mask<S>(q, bc, col0, src.cols));
+            });
             count = wave_sum_u32(count);
             if (lane_id() == 0u) row_nnz[row] = count;
             continue;
@@ -456,7 +535,7 @@ __global__ void sell_emit_bell_kernel(csamd_blocked_ell_view src, csamd_sliced_e
     const unsigned int bs = src.block_size;
     const unsigned int width_blocks = bs != 0u ? src.ell_cols / bs : 0u;
     const S *sval = (const S *) src.val;
-    const bool chunks = bell_chunks_ok<S>(src);
+    const int slog = bell_stream_log2<S>(src);
     for (unsigned int row = blockIdx.x * nw + wave; row < src.rows; row += gridDim.x * nw) {
         const unsigned int slice = sell_slice_of(dst, row);
         if (slice >= dst.slice_count) continue;
@@ -466,15 +545,15 @@ __global__ void sell_emit_bell_kernel(csamd_blocked_ell_view src, csamd_sliced_e
         const unsigned int rb = row / bs;
         const unsigned int total = width_blocks * bs;
         unsigned int written = 0u;
-        if (chunks) {
+        if (slog >= 0) {
             constexpr unsigned int PER = 16u / sizeof(S);
             const unsigned int nc = src.ell_cols / PER;
-            for (unsigned int c0i = 0u; c0i < nc; c0i += 64u) {
-                const unsigned int c = c0i + lane_id();
-                bell_chunk_vals<S> g;
-                g.q = make_uint4(0u, 0u, 0u, 0u);
-                unsigned int col0 = 0u;
-                const unsigned int m = c < nc ? bell_chunk_mask<S>(src, width_blocks, row, c, &g, &col0) : 0u;
+            const unsigned int cmask = (1u << slog) - 1u;
+            CSAMD_GLOBAL unsigned int *ocol = gmem(dst.col_idx) + base;
+            CSAMD_GLOBAL S *oval = gmem(dval) + base;
+            bell_stream_row<S, BELL_STREAM_U>(src, width_blocks, (unsigned int) slog, row, [&](unsigned int c, const u32x4 &q, unsigned int bc) {
+                const unsigned int col0 = bc * bs + (c & cmask) * PER;
+                const unsigned int m = c < nc ? chunk_mask<S>(q, bc, col0, src.cols) : 0u;
                 const unsigned int cnt = (unsigned int) __popc(m);
                 const unsigned int incl = wave_inclusive_scan_u32(cnt);
                 unsigned int pos = written + incl - cnt;
@@ -482,14 +561,14 @@ __global__ void sell_emit_bell_kernel(csamd_blocked_ell_view src, csamd_sliced_e
                 for (unsigned int e = 0; e < PER; ++e) {
                     if ((m >> e) & 1u) {
                         if (pos < width) {
-                            dst.col_idx[base + pos] = col0 + e;
-                            dval[base + pos] = g.v[e];
+                            ocol[pos] = col0 + e;
+                            oval[pos] = chunk_val<S>(q, e);
                         }
                         ++pos;
                     }
                 }
-                written += __shfl(incl, 63);
-            }
+                written += (unsigned int) __builtin_amdgcn_readlane((int) incl, 63);
+            });
         } else {
         for (unsigned int e0 = 0u; e0 < total; e0 += 64u) {
             const unsigned int e = e0 + lane_id();
