@@ -346,17 +346,6 @@ __device__ __forceinline__ unsigned int bell_chunk_mask(const csamd_blocked_ell_
     return m;
 }
 
-// Value e of a 16-byte chunk.
-template<typename S> __device__ __forceinline__ S chunk_val(const u32x4 &q, unsigned int e);
-template<> __device__ __forceinline__ uint16_t chunk_val<uint16_t>(const u32x4 &q, unsigned int e) {
-    const unsigned int w = lane_of(q, e >> 1);
-    return (uint16_t) ((e & 1u) ? (w >> 16) : (w & 0xffffu));
-}
-template<> __device__ __forceinline__ float chunk_val<float>(const u32x4 &q, unsigned int e) { return __uint_as_float(lane_of(q, e)); }
-template<> __device__ __forceinline__ uint8_t chunk_val<uint8_t>(const u32x4 &q, unsigned int e) {
-    return (uint8_t) (lane_of(q, e >> 2) >> ((e & 3u) * 8u));
-}
-
 // Bit e set when value e of the chunk is nonzero at a valid column < cols
 // (block column bc, first column col0).
 template<typename S>

@@ -27,6 +27,7 @@
 // keep bits, so the column map is read only for surviving values.  Rows that are not whole 16-byte groups take a
 // one-wave-per-row path.
 #include "common.hh"
+#include "stream.hh"
 
 #include <cstdlib>
 #include <cstring>
@@ -412,12 +413,120 @@ __global__ void __launch_bounds__(FL_THREADS) fl_gather_kernel(csamd_blocked_ell
     }
 }
 
+// Streaming gather pass (block-keep bits available, a power-of-two number
+// of 16-byte groups per block, the keep bits and the row block's block
+// columns staged in LDS): like fl_gather_kernel, but each thread takes 16
+// consecutive values per step (16 / PER consecutive groups), the next
+// step's loads are in flight while the current one is consumed, one block
+// scan per step orders the survivors, and nothing in the loop reads global
+// memory besides the value stream (LDS reads do not wait for the loads in
+// flight).  Survivors are written with their source columns; the copy pass
+// maps them through col_map.
+template<typename S>
+__global__ void __launch_bounds__(FL_THREADS) fl_gather_stream_kernel(csamd_blocked_ell_view src, const S *__restrict__ val, fl_masks m,
+                                                                      unsigned int cpb_log2, unsigned int row0, unsigned int cap,
+                                                                      unsigned int *__restrict__ counts, unsigned int *__restrict__ claim,
+                                                                      unsigned int *__restrict__ dup, unsigned int *__restrict__ src_row,
+                                                                      unsigned int *__restrict__ overflow, unsigned int *__restrict__ t_idx,
+                                                                      S *__restrict__ t_val) {
+    constexpr unsigned int PER = 16u / sizeof(S);
+    constexpr unsigned int UC = 16u / PER;           // groups per thread and step (16 values)
+    constexpr unsigned int STEP = FL_THREADS * UC;   // groups per step
+    const unsigned int bs = src.block_size;
+    const unsigned int width = src.ell_cols / bs;
+    const unsigned int ng = src.ell_cols / PER;
+    const unsigned int cpb_mask = (1u << cpb_log2) - 1u;
+    __shared__ unsigned int smem[16];
+    extern __shared__ unsigned int lds_dyn[];  // [keep_blocks] keep bits, then [width] block columns
+    typedef const __attribute__((address_space(3))) unsigned int *lds_ptr;
+    unsigned int *lkeep_w = lds_dyn;
+    unsigned int *lslots_w = lds_dyn + ((m.keep_blocks + 3u) & ~3u);
+    const lds_ptr lkeep = (lds_ptr) lkeep_w;
+    const lds_ptr lslots = (lds_ptr) lslots_w;
+    for (unsigned int k = threadIdx.x; k < m.keep_blocks; k += FL_THREADS) lkeep_w[k] = gmem(m.block_keep)[k];
+    CSAMD_GLOBAL unsigned int *gi = gmem(t_idx);
+    CSAMD_GLOBAL S *gv = gmem(t_val);
+    unsigned int cur_rb = ~0u;
+    for (unsigned int r = blockIdx.x; r < src.rows; r += gridDim.x) {
+        const unsigned int o = fl_out_row(m, r);
+        if (o >= m.output_rows) continue;
+        const unsigned int rb = r / bs;
+        if (rb != cur_rb) {
+            __syncthreads();
+            const CSAMD_GLOBAL unsigned int *g = gmem((const unsigned int *) src.blockColIdx) + (unsigned long long) rb * width;
+            for (unsigned int k = threadIdx.x; k < width; k += FL_THREADS) lslots_w[k] = g[k];
+            cur_rb = rb;
+            __syncthreads();
+        }
+        const CSAMD_GLOBAL u32x4 *rq = (const CSAMD_GLOBAL u32x4 *) gmem(val + (unsigned long long) r * src.ell_cols);
+        const unsigned long long base = (unsigned long long) (row0 + r) * cap;
+        unsigned int w = 0u;
+        auto load = [&](unsigned int g0, u32x4 (&buf)[UC]) {
+#pragma unroll
+            for (unsigned int u = 0; u < UC; ++u) buf[u] = rq[min(g0 + threadIdx.x * UC + u, ng - 1u)];
+        };
+        auto consume = [&](unsigned int g0, const u32x4 (&buf)[UC]) {
+            unsigned int mask[UC], col0[UC];
+            unsigned int cnt = 0u;
+#pragma unroll
+            for (unsigned int u = 0; u < UC; ++u) {
+                const unsigned int gidx = g0 + threadIdx.x * UC + u;
+                unsigned int nz = 0u;
+#pragma unroll
+                for (unsigned int k = 0; k < PER; ++k)
+                    if (fl_nonzero<S>(chunk_val<S>(buf[u], k))) nz |= 1u << k;
+                const unsigned int bc = lslots[min(gidx, ng - 1u) >> cpb_log2];
+                const unsigned int j0 = (gidx & cpb_mask) * PER;
+                const unsigned int kb = bc < m.keep_blocks ? lkeep[bc] : 0u;  // CSAMD_INVALID_COL included
+                mask[u] = gidx < ng ? nz & (kb >> j0) : 0u;
+                col0[u] = bc * bs + j0;
+                cnt += (unsigned int) __popc(mask[u]);
+            }
+            unsigned int total = 0u;
+            unsigned int k = w + block_exclusive_scan_u32(cnt, smem, &total);
+#pragma unroll
+            for (unsigned int u = 0; u < UC; ++u) {
+#pragma unroll
+                for (unsigned int j = 0; j < PER; ++j) {
+                    if ((mask[u] >> j) & 1u) {
+                        if (k < cap) {
+                            gi[base + k] = col0[u] + j;
+                            gv[base + k] = chunk_val<S>(buf[u], j);
+                        }
+                        ++k;
+                    }
+                }
+            }
+            w += total;
+        };
+        if (ng != 0u) {
+            u32x4 A[UC], B[UC];
+            load(0u, A);
+            for (unsigned int g0 = 0u; g0 < ng; g0 += 2u * STEP) {
+                load(g0 + STEP, B);
+                consume(g0, A);
+                if (g0 + STEP >= ng) break;
+                load(g0 + 2u * STEP, A);
+                consume(g0 + STEP, B);
+            }
+        }
+        if (threadIdx.x == 0u) {
+            if (atomicExch(&claim[o], 1u) != 0u) *dup = o + 1u;
+            counts[o] = w;
+            src_row[o] = row0 + r;
+            if (w > cap) *overflow = 1u;
+        }
+    }
+}
+
 // Copy pass after the gather: output row o's entries from its source row's
-// region, one wave per row.
+// region, one wave per row; col_map (non-null after the streaming gather)
+// takes the stored source columns to output columns.
 template<typename S>
 __global__ void fl_copy_kernel(unsigned int output_rows, unsigned int cap, const unsigned int *__restrict__ ptr,
                                const unsigned int *__restrict__ src_row, const unsigned int *__restrict__ t_idx,
-                               const S *__restrict__ t_val, unsigned int *__restrict__ out_idx, S *__restrict__ out_val) {
+                               const S *__restrict__ t_val, const unsigned int *__restrict__ col_map, unsigned int *__restrict__ out_idx,
+                               S *__restrict__ out_val) {
     const unsigned int lane = lane_id();
     const unsigned int nw = blockDim.x >> 6;
     for (unsigned int o = blockIdx.x * nw + (threadIdx.x >> 6); o < output_rows; o += gridDim.x * nw) {
@@ -425,7 +534,8 @@ __global__ void fl_copy_kernel(unsigned int output_rows, unsigned int cap, const
         if (n == 0u) continue;
         const unsigned long long base = (unsigned long long) src_row[o] * cap;
         for (unsigned int k = lane; k < n; k += 64u) {
-            gmem(out_idx)[b + k] = gmem(t_idx)[base + k];
+            const unsigned int c = gmem(t_idx)[base + k];
+            gmem(out_idx)[b + k] = col_map != nullptr ? gmem(col_map)[c] : c;  // streaming gather: source columns
             gmem(out_val)[b + k] = gmem(t_val)[base + k];
         }
     }
@@ -507,9 +617,17 @@ template<typename S> struct fl_launch {
 template<typename S> struct fl_gather_launch {
     static int run(const csamd_blocked_ell_view *src, const fl_masks &m, unsigned int row0, unsigned int cap, unsigned int *counts,
                    unsigned int *claim, unsigned int *dup, unsigned int *src_row, unsigned int *overflow, unsigned int *t_idx,
-                   void *t_val, hipStream_t s) {
+                   void *t_val, bool stream, hipStream_t s) {
         const unsigned int grid = grid_for(src->rows, 1u, (unsigned int) device_cu_count() * 16u);
         const unsigned int width = src->ell_cols / src->block_size;
+        if (stream) {
+            const unsigned int cpb = src->block_size / (16u / (unsigned int) sizeof(S));
+            const unsigned int cpb_log2 = (unsigned int) __builtin_ctz(cpb);
+            const size_t lds = ((size_t) ((m.keep_blocks + 3u) & ~3u) + width) * 4u;
+            fl_gather_stream_kernel<S><<<grid, FL_THREADS, lds, s>>>(*src, (const S *) src->val, m, cpb_log2, row0, cap, counts, claim,
+                                                                    dup, src_row, overflow, t_idx, (S *) t_val);
+            return hip_ok(hipGetLastError(), "fl_gather_stream_kernel");
+        }
         const unsigned int slot_cap = width <= FL_LDS_SLOTS ? width : 0u;
         fl_gather_kernel<S, 4u><<<grid, FL_THREADS, (size_t) slot_cap * 4u, s>>>(*src, (const S *) src->val, m, slot_cap, row0, cap,
                                                                                 counts, claim, dup, src_row, overflow, t_idx, (S *) t_val);
@@ -518,9 +636,10 @@ template<typename S> struct fl_gather_launch {
 };
 template<typename S> struct fl_copy_launch {
     static int run(unsigned int output_rows, unsigned int cap, const unsigned int *ptr, const unsigned int *src_row,
-                   const unsigned int *t_idx, const void *t_val, unsigned int *out_idx, void *out_val, hipStream_t s) {
+                   const unsigned int *t_idx, const void *t_val, const unsigned int *col_map, unsigned int *out_idx, void *out_val,
+                   hipStream_t s) {
         fl_copy_kernel<S><<<grid_for(output_rows, FL_WAVES, (unsigned int) device_cu_count() * 16u), FL_THREADS, 0, s>>>(
-            output_rows, cap, ptr, src_row, t_idx, (const S *) t_val, out_idx, (S *) out_val);
+            output_rows, cap, ptr, src_row, t_idx, (const S *) t_val, col_map, out_idx, (S *) out_val);
         return hip_ok(hipGetLastError(), "fl_copy_kernel");
     }
 };
@@ -640,6 +759,20 @@ int filter_parts(const csamd_blocked_ell_view *parts, unsigned int part_count, u
             t_idx = (unsigned int *) st->tmp_block;
             t_val = (char *) st->tmp_block + off_tv;
         }
+        // The streaming gather applies when every non-empty part has block
+        // keep bits, a power-of-two number of 16-byte groups per block and
+        // LDS room for its keep bits and a row block's block columns; it
+        // stores source columns, which the copy pass maps.
+        bool stream = gather;
+        for (unsigned int p = 0; p < part_count && stream; ++p) {
+            const csamd_blocked_ell_view &v = parts[p];
+            if (v.rows == 0u || v.ell_cols == 0u) continue;
+            const bool keep_bits = cols != 0u && v.cols == cols && v.block_size != 0u && v.block_size <= 32u &&
+                                   v.block_size % (16u / (unsigned int) vb) == 0u;
+            const unsigned int cpb = keep_bits ? v.block_size / (16u / (unsigned int) vb) : 0u;
+            const unsigned long long kb = (cols + v.block_size - 1u) / (v.block_size != 0u ? v.block_size : 1u);
+            stream = keep_bits && (cpb & (cpb - 1u)) == 0u && (kb + 4ull + v.ell_cols / v.block_size) * 4ull <= 48ull * 1024ull;
+        }
         // mode 0: count, 1: emit, 2: gather
         auto pass = [&](int mode) -> int {
             unsigned int r0 = 0u;
@@ -659,7 +792,7 @@ int filter_parts(const csamd_blocked_ell_view *parts, unsigned int part_count, u
                                  keep_bits ? (unsigned int) ((cols + v.block_size - 1u) / v.block_size) : 0u};
                 if (v.rows != 0u && v.ell_cols != 0u) {
                     const int ok = mode == 2 ? fl_dispatch<fl_gather_launch>(value_type, &v, m, r0, (unsigned int) cap, counts, claim, dup,
-                                                                             src_row, overflow, t_idx, t_val, s)
+                                                                             src_row, overflow, t_idx, t_val, stream, s)
                                              : fl_dispatch<fl_launch>(value_type, &v, m, counts, claim, dup, (const unsigned int *) ptr,
                                                                       out_idx, out_val, mode == 1, s);
                     if (!ok) return 0;
@@ -687,6 +820,7 @@ int filter_parts(const csamd_blocked_ell_view *parts, unsigned int part_count, u
                 CSAMD_REQUIRE(live <= max_nnz, "%s: %u surviving entries exceed the source nnz %llu", who, live, max_nnz);
                 if (!fl_dispatch<fl_copy_launch>(value_type, output_rows, (unsigned int) cap, (const unsigned int *) ptr,
                                                  (const unsigned int *) src_row, (const unsigned int *) t_idx, (const void *) t_val,
+                                                 stream ? (const unsigned int *) col_map : nullptr,
                                                  out_idx, out_val, s))
                     return 0;
                 done = true;
