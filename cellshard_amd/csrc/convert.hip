@@ -358,49 +358,6 @@ __device__ __forceinline__ unsigned int chunk_mask(const u32x4 &q, unsigned int 
     return bc != CSAMD_INVALID_COL ? m : 0u;
 }
 
-template<typename S, unsigned int U> struct bell_row_buf {
-    u32x4 q[U];
-    unsigned int bc[U];
-};
-
-// Walk the 16-byte chunks of one Blocked-ELL value row with the whole wave,
-// U chunks per lane per step (chunk c0 + 64 u + lane), through two register
-// buffers in turn: the next step's value and block-column loads are in
-// flight while visit(c, q, bc) runs for the current step, u = 0, 1, ... in
-// order.  Loads past the row's last chunk re-read it (visit sees c >= nc
-// for them), so every load is unconditional.  Chunks per block must be a
-// power of two (2^cpb_log2).
-template<typename S, unsigned int U, typename Visit>
-__device__ __forceinline__ void bell_stream_row(const csamd_blocked_ell_view &src, unsigned int width, unsigned int cpb_log2,
-                                                unsigned int row, Visit visit) {
-    constexpr unsigned int PER = 16u / sizeof(S);
-    constexpr unsigned int STEP = 64u * U;
-    const unsigned int nc = src.ell_cols / PER;
-    if (nc == 0u) return;
-    const unsigned int lane = lane_id();
-    const CSAMD_GLOBAL u32x4 *vrow = (const CSAMD_GLOBAL u32x4 *) gmem((const S *) src.val + (unsigned long long) row * src.ell_cols);
-    const CSAMD_GLOBAL unsigned int *bcrow = gmem(src.blockColIdx + (unsigned long long) (row / src.block_size) * width);
-    auto load = [&](unsigned int c0, bell_row_buf<S, U> &B) {
-#pragma unroll
-        for (unsigned int u = 0; u < U; ++u) {
-            const unsigned int c = min(c0 + 64u * u + lane, nc - 1u);
-            B.q[u] = vrow[c];
-            B.bc[u] = bcrow[c >> cpb_log2];
-        }
-    };
-    bell_row_buf<S, U> A, B;
-    load(0u, A);
-    for (unsigned int c0 = 0u; c0 < nc; c0 += 2u * STEP) {
-        load(c0 + STEP, B);
-#pragma unroll
-        for (unsigned int u = 0; u < U; ++u) visit(c0 + 64u * u + lane, A.q[u], A.bc[u]);
-        if (c0 + STEP >= nc) break;
-        load(c0 + 2u * STEP, A);
-#pragma unroll
-        for (unsigned int u = 0; u < U; ++u) visit(c0 + STEP + 64u * u + lane, B.q[u], B.bc[u]);
-    }
-}
-
 // log2 of the 16-byte chunks per block when the streaming walk applies
 // (chunked layout, a power-of-two number of chunks per block), else -1.
 template<typename S>

@@ -9,6 +9,7 @@
 // streams, per-wave group bins in LDS, wave64 butterfly reductions.
 #include "common.hh"
 #include "group_reduce.hh"
+#include "stream.hh"
 
 #include <cstdlib>
 
@@ -820,19 +821,83 @@ __global__ void __launch_bounds__(MG_THREADS) reopt_gather_kernel(csamd_blocked_
     }
 }
 
+// Streaming form of the gather (keep bits available, block size 8, 16 or
+// 32): a wave streams its row 4 chunks per lane ahead through two register
+// buffers (value chunk and block column together), the keep bits come from
+// an LDS copy, and survivors are written with their source columns (the
+// copy pass maps them through col_map), so nothing inside the loop waits on
+// a dependent global load.
+__global__ void __launch_bounds__(MG_THREADS) reopt_gather_stream_kernel(csamd_blocked_ell_view src, csamd_row_feature_mask_view masks,
+                                                                         unsigned int output_rows, unsigned int cap, unsigned int cpb_log2,
+                                                                         unsigned int *__restrict__ major_ptr,
+                                                                         unsigned int *__restrict__ src_row, unsigned int *__restrict__ overflow,
+                                                                         unsigned int *__restrict__ t_idx, uint16_t *__restrict__ t_val,
+                                                                         const unsigned int *__restrict__ block_keep, unsigned int keep_blocks) {
+    extern __shared__ unsigned int lkeep_w[];  // [keep_blocks]
+    const __attribute__((address_space(3))) unsigned int *lkeep = (const __attribute__((address_space(3))) unsigned int *) lkeep_w;
+    for (unsigned int k = threadIdx.x; k < keep_blocks; k += blockDim.x) lkeep_w[k] = gmem(block_keep)[k];
+    __syncthreads();
+    const unsigned int wave = threadIdx.x >> 6;
+    const unsigned int bs = src.block_size;
+    const unsigned int width = src.ell_cols / bs;
+    const unsigned int nc = src.ell_cols / 8u;
+    const unsigned int cmask = (1u << cpb_log2) - 1u;
+    CSAMD_GLOBAL unsigned int *gi = gmem(t_idx);
+    CSAMD_GLOBAL uint16_t *gv = gmem(t_val);
+    for (unsigned int row = blockIdx.x * MG_WAVES + wave; row < src.rows; row += gridDim.x * MG_WAVES) {
+        if (!row_participates(masks, row)) continue;
+        const unsigned int out_row = masks.row_remap != nullptr ? masks.row_remap[row] : row;
+        if (out_row >= output_rows) continue;
+        const unsigned long long base = (unsigned long long) row * cap;
+        unsigned int w = 0u;
+        bell_stream_row<uint16_t, 4u>(src, width, cpb_log2, row, [&](unsigned int c, const u32x4 &q, unsigned int bc) {
+            unsigned int nz = 0u;
+#pragma unroll
+            for (unsigned int e = 0; e < 8u; ++e)
+                if ((chunk_val<uint16_t>(q, e) & 0x7fffu) != 0u) nz |= 1u << e;
+            const unsigned int j0 = (c & cmask) * 8u;
+            const unsigned int kb = bc < keep_blocks ? lkeep[bc] : 0u;  // CSAMD_INVALID_COL included
+            const unsigned int m = c < nc ? nz & (kb >> j0) : 0u;
+            const unsigned int cnt = (unsigned int) __popc(m);
+            const unsigned int incl = wave_inclusive_scan_u32(cnt);
+            unsigned int k = w + incl - cnt;
+            w += (unsigned int) __builtin_amdgcn_readlane((int) incl, 63);
+            const unsigned int col0 = bc * bs + j0;
+#pragma unroll
+            for (unsigned int e = 0; e < 8u; ++e) {
+                if ((m >> e) & 1u) {
+                    if (k < cap) {
+                        gi[base + k] = col0 + e;
+                        gv[base + k] = chunk_val<uint16_t>(q, e);
+                    }
+                    ++k;
+                }
+            }
+        });
+        if (lane_id() == 0u) {
+            major_ptr[out_row + 1u] = w;
+            src_row[out_row] = row;
+            if (w > cap) *overflow = 1u;
+        }
+    }
+}
+
 // Copy pass after the gather: output row o's entries from its source row's
-// region, one wave per row.
+// region, one wave per row; col_map (non-null after the streaming gather)
+// takes the stored source columns to output columns.
 __global__ void __launch_bounds__(MG_THREADS) reopt_copy_kernel(unsigned int output_rows, unsigned int cap,
                                                                 const unsigned int *__restrict__ major_ptr,
                                                                 const unsigned int *__restrict__ src_row,
                                                                 const unsigned int *__restrict__ t_idx, const uint16_t *__restrict__ t_val,
+                                                                const unsigned int *__restrict__ col_map,
                                                                 unsigned int *__restrict__ minor_idx, uint16_t *__restrict__ val) {
     for (unsigned int o = blockIdx.x * MG_WAVES + (threadIdx.x >> 6); o < output_rows; o += gridDim.x * MG_WAVES) {
         const unsigned int b = major_ptr[o], n = major_ptr[o + 1u] - b;
         if (n == 0u) continue;
         const unsigned long long base = (unsigned long long) src_row[o] * cap;
         for (unsigned int k = lane_id(); k < n; k += 64u) {
-            gmem(minor_idx)[b + k] = gmem(t_idx)[base + k];
+            const unsigned int c = gmem(t_idx)[base + k];
+            gmem(minor_idx)[b + k] = col_map != nullptr ? gmem(col_map)[c] : c;  // streaming gather: source columns
             gmem(val)[b + k] = gmem(t_val)[base + k];
         }
     }
@@ -1365,9 +1430,20 @@ int csamd_manual_reoptimize_masked_sparse(const csamd_blocked_ell_view *src, con
         }
         unsigned int *t_idx = (unsigned int *) st->gather_block;
         uint16_t *t_val = (uint16_t *) ((char *) st->gather_block + align_up((size_t) (src->rows * cap) * 4u, 256u));
-        reopt_gather_kernel<<<grid, MG_THREADS, 0, s>>>(*src, m, output_rows, (unsigned int) cap, c.majorPtr, src_row, overflow, t_idx,
-                                                        t_val, col_map, block_keep, keep_blocks);
-        CSAMD_HIP(hipGetLastError(), "reopt_gather_kernel");
+        // streaming form: keep bits, a power-of-two number of 16-byte chunks
+        // per block (block size 8, 16 or 32) and the keep bits in LDS
+        const unsigned int cpb = src->block_size / 8u;
+        const bool stream = block_keep != nullptr && (cpb & (cpb - 1u)) == 0u && (size_t) keep_blocks * 4u <= 32768u;
+        if (stream) {
+            reopt_gather_stream_kernel<<<grid, MG_THREADS, (size_t) keep_blocks * 4u, s>>>(
+                *src, m, output_rows, (unsigned int) cap, (unsigned int) __builtin_ctz(cpb), c.majorPtr, src_row, overflow, t_idx, t_val,
+                block_keep, keep_blocks);
+            CSAMD_HIP(hipGetLastError(), "reopt_gather_stream_kernel");
+        } else {
+            reopt_gather_kernel<<<grid, MG_THREADS, 0, s>>>(*src, m, output_rows, (unsigned int) cap, c.majorPtr, src_row, overflow,
+                                                            t_idx, t_val, col_map, block_keep, keep_blocks);
+            CSAMD_HIP(hipGetLastError(), "reopt_gather_kernel");
+        }
         reopt_scan_kernel<<<1, 1024, 0, s>>>(output_rows, c.majorPtr);
         CSAMD_HIP(hipGetLastError(), "reopt_scan_kernel");
         unsigned int flag = 0u;
@@ -1377,7 +1453,8 @@ int csamd_manual_reoptimize_masked_sparse(const csamd_blocked_ell_view *src, con
         CSAMD_HIP(hipStreamSynchronize(s), "hipStreamSynchronize masked reoptimize");
         if (flag == 0u) {
             reopt_copy_kernel<<<grid_for(output_rows, MG_WAVES, (unsigned int) device_cu_count() * 16u), MG_THREADS, 0, s>>>(
-                output_rows, (unsigned int) cap, c.majorPtr, src_row, t_idx, t_val, c.minorIdx, (uint16_t *) c.val);
+                output_rows, (unsigned int) cap, c.majorPtr, src_row, t_idx, t_val, stream ? col_map : nullptr, c.minorIdx,
+                (uint16_t *) c.val);
             CSAMD_HIP(hipGetLastError(), "reopt_copy_kernel");
             CSAMD_HIP(hipStreamSynchronize(s), "hipStreamSynchronize masked reoptimize");
             done = true;
