@@ -25,7 +25,6 @@
 // Sources with unsorted or repeated minor indices take a fallback scatter
 // (K4u: LDS cursors, rows in order, entries of one row in any order).
 #include "common.hh"
-#include "stream.hh"
 
 #include <cstdlib>
 
@@ -171,18 +170,14 @@ __global__ void __launch_bounds__(TR_COUNT_THREADS) tr_count_kernel(unsigned int
 // tr_count_kernel.
 //
 // The tile's entries are walked flat, not row by row, in groups of 4
-// consecutive entries (16-byte loads): pass p of a wave covers 64 U groups,
-// lane l taking U consecutive ones (4 U consecutive entries), and the loads
-// of the wave's next pass are in flight while the current one is counted.
-// A lane finds the row of its first entry once per pass (binary search over
-// the row starts in LDS); a lane's run crosses at most a row boundary or
-// two, so the per-entry work is the count, the band test and the order
-// test, and the row advance sits behind a wave-uniform branch that is
-// rarely taken.  Only the tile's first and last passes test entries
-// against the tile's bounds: the group holding the tile's first entry may
-// start in the previous tile and the one holding its last may end in the
-// next (those entries are skipped); the array's final partial group, if
-// any, is read one entry at a time.
+// consecutive entries (one 16-byte load per lane and group): pass p covers
+// 64 U groups from the tile's first group + 64 U p, wave w takes passes w,
+// w + 16, ..., and the loads of a wave's next pass are in flight while the
+// current one is counted.  Each lane finds its entries' row from the row
+// starts in LDS (binary search once per pass, then a forward walk).  The
+// group holding the tile's first entry may start in the previous tile and
+// the one holding its last may end in the next (those entries are skipped);
+// the array's final partial group, if any, is read one entry at a time.
 template<unsigned int U>
 __global__ void __launch_bounds__(TR_COUNT_THREADS) tr_count_packed_kernel(unsigned int cDim, unsigned int uDim, unsigned int nnz,
                                                                            const unsigned int *__restrict__ ptr,
@@ -191,7 +186,6 @@ __global__ void __launch_bounds__(TR_COUNT_THREADS) tr_count_packed_kernel(unsig
                                                                            unsigned int *__restrict__ flags) {
     constexpr unsigned int PG = 64u * U;  // groups per pass
     constexpr unsigned int NW = TR_COUNT_THREADS / 64u;
-    static_assert(TR_BAND == 256u, "band = column >> 8");
     extern __shared__ unsigned int lds_words[];
     const unsigned int hw = (uDim + 1u) >> 1;
     constexpr unsigned short NONE = 0xffffu;                          // band without entries in the row (so far)
@@ -211,24 +205,31 @@ __global__ void __launch_bounds__(TR_COUNT_THREADS) tr_count_packed_kernel(unsig
     const unsigned int g0 = E0 >> 2, g1 = (E1 + 3u) >> 2;  // groups [g0, g1) hold the tile
     const unsigned int ng = E1 > E0 ? g1 - g0 : 0u;
     const unsigned int gfull = nnz >> 2;                     // groups wholly inside the array
-    const unsigned int Ev = E1 < gfull * 4u ? E1 : gfull * 4u;  // entries read by the group loads: [E0, Ev)
-    const CSAMD_GLOBAL u32x4 *idx4 = (const CSAMD_GLOBAL u32x4 *) gmem(idx);
-    const CSAMD_GLOBAL unsigned int *idxg = gmem(idx);
+    const uint4 *idx4 = (const uint4 *) idx;
 
-    // lane's groups of pass p: [g0 + p PG + lane U, + U), clamped into the
-    // tile's groups and the array's whole groups; `before` is the column of
-    // the entry before the lane's first (lane 0: loaded; others: the
-    // previous lane's last, moved over once the loads land)
-    auto load = [&](unsigned int p, u32x4 (&c)[U], unsigned int &before) {
-        const unsigned int gp = g0 + p * PG + lane * U;
+    // 16-byte loads only (groups past the array's last whole group read that
+    // group; their entries are skipped here and visited one by one at the end)
+    auto load = [&](unsigned int p, uint4 (&c)[U], unsigned int &before) {
+        const unsigned int gp = g0 + p * PG;
 #pragma unroll
         for (unsigned int u = 0; u < U; ++u) {
-            const unsigned int g = gp + u;
+            const unsigned int g = gp + 64u * u + lane;
             const unsigned int gc = g < g1 ? g : g1 - 1u;
             c[u] = idx4[gc < gfull ? gc : gfull - 1u];
         }
-        const unsigned int pb = g0 + p * PG;
-        before = idxg[pb > 0u ? pb * 4u - 1u : 0u];
+        before = idx[gp > 0u ? gp * 4u - 1u : 0u];
+    };
+    // one entry: i in [E0, E1), its column cc, the previous entry's column pc
+    // (used only when i is not its row's first), its row's start rs
+    auto visit = [&](unsigned int i, unsigned int cc, unsigned int pc, unsigned int row, unsigned int rs, bool &bad,
+                     bool &unsorted) {
+        bad |= cc >= uDim;
+        const unsigned int cs = cc < uDim ? cc : 0u;  // out of range: reported, not counted right
+        atomicAdd(&hist[cs >> 1], 1u << ((cs & 1u) << 4));
+        const bool inner = i > rs;
+        unsorted |= inner && pc >= cc;
+        const unsigned int band = cs / TR_BAND;
+        if (!inner || band != (pc < uDim ? pc : 0u) / TR_BAND) off[band * TR_ROWS + row] = (unsigned short) (i - rs);
     };
     // last row with rp[row] <= i (i in [E0, E1))
     auto row_of = [&](unsigned int i) -> unsigned int {
@@ -240,84 +241,40 @@ __global__ void __launch_bounds__(TR_COUNT_THREADS) tr_count_packed_kernel(unsig
         }
         return lo;
     };
-    unsigned int cmax = 0u;   // largest column seen (range check at the end)
-    bool unsorted = false;
-    // unused band-start slot of this lane (stores of entries that open no band)
-    unsigned short *dummy = off + (size_t) bands * TR_ROWS + 2u * tid;
-    auto count = [&](unsigned int p, const u32x4 (&c)[U], unsigned int before) {
-        const unsigned int ib = (g0 + p * PG + lane * U) * 4u;  // lane's first entry
-        const unsigned int up = (unsigned int) __shfl_up((int) c[U - 1u].w, 1);
-        unsigned int pc = lane == 0u ? before : up;  // column of the entry before this one
-        unsigned int row = row_of(ib < E0 ? E0 : (ib < E1 ? ib : E1 - 1u));
+    const unsigned int Ev = E1 < gfull * 4u ? E1 : gfull * 4u;  // entries read by the group loads: [E0, Ev)
+    auto count = [&](unsigned int p, const uint4 (&c)[U], unsigned int before) {
+        const unsigned int ib = (g0 + p * PG) * 4u;  // first entry of the pass
+        unsigned int i = ib + 4u * lane;
+        unsigned int row = row_of(i < E0 ? E0 : (i < E1 ? i : E1 - 1u));
         unsigned int rs = rp[row], re = rp[row + 1u];
-        // wave-uniform: does the pass reach outside [E0, Ev)?
-        const bool edge = (g0 + p * PG) * 4u < E0 || (g0 + (p + 1u) * PG) * 4u > Ev;
-        // Fast form: the lane's run lies in its row and at most one more row
-        // b (the row of entry re); an entry then takes row / rs, or b / re
-        // once it reaches re, with no branch.
-        unsigned int rowb = row, rsb = rs;
-        bool one_more = true;
-        if (!edge && re < ib + 4u * U) {
-            rowb = row_of(re);
-            rsb = rp[rowb];
-            one_more = rp[rowb + 1u] >= ib + 4u * U;
-        }
-        if (!edge && __builtin_amdgcn_ballot_w64(!one_more) == 0ull) {
-            bool uns = false;
-#pragma unroll
-            for (unsigned int u = 0; u < U; ++u) {
-                const unsigned int col[4] = {c[u].x, c[u].y, c[u].z, c[u].w};
-#pragma unroll
-                for (unsigned int e = 0; e < 4u; ++e) {
-                    const unsigned int i = ib + 4u * u + e;
-                    const unsigned int cc = col[e];
-                    const bool nb = i >= re;
-                    const unsigned int r = nb ? rowb : row;
-                    const unsigned int rsi = nb ? rsb : rs;
-                    cmax = cc > cmax ? cc : cmax;
-                    const unsigned int cs = cc < uDim ? cc : 0u;  // out of range: reported, not counted right
-                    atomicAdd(&hist[cs >> 1], 1u << ((cs & 1u) << 4));
-                    const bool inner = i != rsi;
-                    uns |= inner && pc >= cc;
-                    const bool open = !inner || (cs >> 8) != (pc >> 8);
-                    unsigned short *dst = open ? off + (cs >> 8) * TR_ROWS + r : dummy;
-                    *dst = (unsigned short) (i - rsi);
-                    pc = cc;
-                }
-            }
-            unsorted |= uns;
-            return;
-        }
+        unsigned int carry = before;  // column of the entry before the group row u
+        bool bad = false, unsorted = false;
 #pragma unroll
         for (unsigned int u = 0; u < U; ++u) {
+            const unsigned int up = (unsigned int) __shfl_up((int) c[u].w, 1);
+            const unsigned int first_prev = lane == 0u ? carry : up;
+            carry = (unsigned int) __shfl((int) c[u].w, 63);
             const unsigned int col[4] = {c[u].x, c[u].y, c[u].z, c[u].w};
+            const unsigned int base = ib + 256u * u + 4u * lane;
 #pragma unroll
             for (unsigned int e = 0; e < 4u; ++e) {
-                const unsigned int i = ib + 4u * u + e;
-                const unsigned int cc = col[e];
-                while (i >= re && i < E1) {  // forward to the entry's row (empty rows skipped)
+                i = base + e;
+                if (i < E0 || i >= Ev) continue;
+                while (i >= re) {  // forward to the entry's row (empty rows skipped)
                     ++row;
                     rs = re;
                     re = rp[row + 1u];
                 }
-                const bool valid = !edge || (i >= E0 && i < Ev);
-                if (valid) {
-                    cmax = cc > cmax ? cc : cmax;
-                    const unsigned int cs = cc < uDim ? cc : 0u;
-                    atomicAdd(&hist[cs >> 1], 1u << ((cs & 1u) << 4));
-                    const bool inner = i != rs;
-                    unsorted |= inner && pc >= cc;
-                    const unsigned int band = cs >> 8;
-                    if (!inner || band != (pc >> 8)) off[band * TR_ROWS + row] = (unsigned short) (i - rs);
-                }
-                pc = cc;
+                visit(i, col[e], e == 0u ? first_prev : col[e == 0u ? 0u : e - 1u], row, rs, bad, unsorted);
             }
         }
+        if (bad) flags[1] = 1u;
+        if (unsorted) flags[0] = 1u;
     };
     // two register sets in turn: the next pass loads while this one counts
     unsigned int p = wave;
     if (p * PG < ng) {
-        u32x4 ca[U], cb[U];
+        uint4 ca[U], cb[U];
         unsigned int ba, bb;
         load(p, ca, ba);
         for (;;) {
@@ -335,25 +292,15 @@ __global__ void __launch_bounds__(TR_COUNT_THREADS) tr_count_packed_kernel(unsig
             p = pn;
         }
     }
-    if (cmax >= uDim) flags[1] = 1u;
-    if (unsorted) flags[0] = 1u;
     // the array's final partial group (at most 3 entries, last tile only)
     if (tid == 0u && Ev < E1) {
-        bool bad = false, uns = false;
+        bool bad = false, unsorted = false;
         for (unsigned int i = Ev > E0 ? Ev : E0; i < E1; ++i) {
             const unsigned int row = row_of(i);
-            const unsigned int rs = rp[row];
-            const unsigned int cc = idx[i], pc = idx[i > 0u ? i - 1u : 0u];
-            bad |= cc >= uDim;
-            const unsigned int cs = cc < uDim ? cc : 0u;
-            atomicAdd(&hist[cs >> 1], 1u << ((cs & 1u) << 4));
-            const bool inner = i > rs;
-            uns |= inner && pc >= cc;
-            const unsigned int band = cs >> 8;
-            if (!inner || band != (pc < uDim ? pc : 0u) >> 8) off[band * TR_ROWS + row] = (unsigned short) (i - rs);
+            visit(i, idx[i], idx[i > 0u ? i - 1u : 0u], row, rp[row], bad, unsorted);
         }
         if (bad) flags[1] = 1u;
-        if (uns) flags[0] = 1u;
+        if (unsorted) flags[0] = 1u;
     }
     __syncthreads();
     unsigned int *tile_counts = counts + (size_t) t * uDim;
@@ -374,9 +321,7 @@ __global__ void __launch_bounds__(TR_COUNT_THREADS) tr_count_packed_kernel(unsig
 }
 
 size_t count_packed_lds_bytes(unsigned int uDim, unsigned int bands) {
-    // counters, row starts, band-start table, one unused slot per thread
-    return (size_t) ((uDim + 1u) >> 1) * 4u + (size_t) (TR_ROWS + 4u) * 4u + (size_t) bands * TR_ROWS * 2u +
-           (size_t) TR_COUNT_THREADS * 4u;
+    return (size_t) ((uDim + 1u) >> 1) * 4u + (size_t) (TR_ROWS + 4u) * 4u + (size_t) bands * TR_ROWS * 2u;
 }
 
 // K2: counts[t][c] <- sum_{t' < t} counts[t'][c] (in place), coltot[c] = total.
