@@ -68,14 +68,33 @@ __global__ void reorder_segments_kernel(const unsigned int *__restrict__ order, 
                                         const unsigned int *__restrict__ src_idx, const S *__restrict__ src_val, unsigned int n,
                                         const unsigned int *__restrict__ dst_ptr, unsigned int *__restrict__ dst_idx,
                                         S *__restrict__ dst_val) {
-    const unsigned int wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    constexpr unsigned int U = 8u;  // entries per lane in flight: all loads of a step before its stores
+    const unsigned int wave = threadIdx.x >> 6, nw = blockDim.x >> 6, lane = lane_id();
+    const CSAMD_GLOBAL unsigned int *si = gmem(src_idx);
+    const CSAMD_GLOBAL S *sv = gmem(src_val);
+    CSAMD_GLOBAL unsigned int *di = gmem(dst_idx);
+    CSAMD_GLOBAL S *dv = gmem(dst_val);
     for (unsigned int d = blockIdx.x * nw + wave; d < n; d += gridDim.x * nw) {
         const unsigned int s = order[d];
         const unsigned int b = src_ptr[s], len = src_ptr[s + 1u] - b;
         const unsigned int o = dst_ptr[d];
-        for (unsigned int k = lane_id(); k < len; k += 64u) {
-            dst_idx[o + k] = src_idx[b + k];
-            dst_val[o + k] = src_val[b + k];
+        for (unsigned int k0 = 0u; k0 < len; k0 += 64u * U) {
+            unsigned int ix[U];
+            S vv[U];
+#pragma unroll
+            for (unsigned int u = 0; u < U; ++u) {
+                const unsigned int k = min(k0 + 64u * u + lane, len - 1u);  // clamped: every lane loads
+                ix[u] = si[b + k];
+                vv[u] = sv[b + k];
+            }
+#pragma unroll
+            for (unsigned int u = 0; u < U; ++u) {
+                const unsigned int k = k0 + 64u * u + lane;
+                if (k < len) {
+                    di[o + k] = ix[u];
+                    dv[o + k] = vv[u];
+                }
+            }
         }
     }
 }

@@ -65,18 +65,40 @@ __device__ __forceinline__ void sell_write_row(const CSAMD_GLOBAL unsigned int *
     const unsigned int h = head < width ? head : width;
     if (lane < h) one(lane);
     const unsigned int nq = (width - h) >> 2;
-    for (unsigned int q = lane; q < nq; q += 64u) {
-        const unsigned int j = h + 4u * q;  // base + j is a multiple of 4
-        u32x4 c;
-        S v[4];
+    // QU quads per lane per step, every load of a step issued before its
+    // stores (entry indices clamped to the row's last entry; entries past
+    // n are replaced by padding when stored)
+    constexpr unsigned int QU = 4u;
+    const unsigned int last = n != 0u ? n - 1u : 0u;
+    for (unsigned int q0 = 0u; q0 < nq; q0 += 64u * QU) {
+        unsigned int ci[QU][4];
+        S vi[QU][4];
+        if (n != 0u) {
 #pragma unroll
-        for (unsigned int t = 0; t < 4u; ++t) {
-            const bool in = j + t < n;
-            c[t] = in ? idx[b + j + t] : CSAMD_INVALID_COL;
-            v[t] = in ? val[b + j + t] : S(0);
+            for (unsigned int u = 0; u < QU; ++u)
+#pragma unroll
+                for (unsigned int t = 0; t < 4u; ++t) {
+                    const unsigned int jj = min(h + 4u * (q0 + 64u * u + lane) + t, last);
+                    ci[u][t] = idx[b + jj];
+                    vi[u][t] = val[b + jj];
+                }
         }
-        *(CSAMD_GLOBAL u32x4 *) (oc + base + j) = c;
-        sell_quad<S>::store(ov + base + j, v);
+#pragma unroll
+        for (unsigned int u = 0; u < QU; ++u) {
+            const unsigned int q = q0 + 64u * u + lane;
+            if (q >= nq) continue;
+            const unsigned int j = h + 4u * q;  // base + j is a multiple of 4
+            u32x4 c;
+            S v[4];
+#pragma unroll
+            for (unsigned int t = 0; t < 4u; ++t) {
+                const bool in = j + t < n;
+                c[t] = in ? ci[u][t] : CSAMD_INVALID_COL;
+                v[t] = in ? vi[u][t] : S(0);
+            }
+            *(CSAMD_GLOBAL u32x4 *) (oc + base + j) = c;
+            sell_quad<S>::store(ov + base + j, v);
+        }
     }
     const unsigned int t0 = h + 4u * nq;
     if (lane < width - t0) one(t0 + lane);
