@@ -224,12 +224,12 @@ __global__ void __launch_bounds__(TR_COUNT_THREADS) tr_count_packed_kernel(unsig
     auto visit = [&](unsigned int i, unsigned int cc, unsigned int pc, unsigned int row, unsigned int rs, bool &bad,
                      bool &unsorted) {
         bad |= cc >= uDim;
-        const unsigned int cs = cc < uDim ? cc : 0u;  // out of range: reported, not counted right
+        const unsigned int cs = min(cc, uDim - 1u);  // out of range: reported, not counted right
         atomicAdd(&hist[cs >> 1], 1u << ((cs & 1u) << 4));
         const bool inner = i > rs;
         unsorted |= inner && pc >= cc;
         const unsigned int band = cs / TR_BAND;
-        if (!inner || band != (pc < uDim ? pc : 0u) / TR_BAND) off[band * TR_ROWS + row] = (unsigned short) (i - rs);
+        if (!inner || band != min(pc, uDim - 1u) / TR_BAND) off[band * TR_ROWS + row] = (unsigned short) (i - rs);
     };
     // last row with rp[row] <= i (i in [E0, E1))
     auto row_of = [&](unsigned int i) -> unsigned int {
@@ -242,6 +242,10 @@ __global__ void __launch_bounds__(TR_COUNT_THREADS) tr_count_packed_kernel(unsig
         return lo;
     };
     const unsigned int Ev = E1 < gfull * 4u ? E1 : gfull * 4u;  // entries read by the group loads: [E0, Ev)
+    const unsigned int cmax = uDim - 1u;                        // uDim > 0 here
+    // one lane-private u16 slot per lane (4 bytes apart) takes the band-start
+    // stores of entries that open no band, so every entry stores without a branch
+    const unsigned int junk = bands * TR_ROWS + 2u * lane;
     auto count = [&](unsigned int p, const uint4 (&c)[U], unsigned int before) {
         const unsigned int ib = (g0 + p * PG) * 4u;  // first entry of the pass
         unsigned int i = ib + 4u * lane;
@@ -249,6 +253,7 @@ __global__ void __launch_bounds__(TR_COUNT_THREADS) tr_count_packed_kernel(unsig
         unsigned int rs = rp[row], re = rp[row + 1u];
         unsigned int carry = before;  // column of the entry before the group row u
         bool bad = false, unsorted = false;
+        unsigned int top = 0u;  // largest column seen on the group path
 #pragma unroll
         for (unsigned int u = 0; u < U; ++u) {
             const unsigned int up = (unsigned int) __shfl_up((int) c[u].w, 1);
@@ -256,6 +261,33 @@ __global__ void __launch_bounds__(TR_COUNT_THREADS) tr_count_packed_kernel(unsig
             carry = (unsigned int) __shfl((int) c[u].w, 63);
             const unsigned int col[4] = {c[u].x, c[u].y, c[u].z, c[u].w};
             const unsigned int base = ib + 256u * u + 4u * lane;
+            // forward to the row of the group's first entry (empty rows skipped)
+            const unsigned int bi = base < E0 ? E0 : (base < E1 ? base : E1 - 1u);
+            while (bi >= re) {
+                ++row;
+                rs = re;
+                re = rp[row + 1u];
+            }
+            const bool whole = base >= E0 && base < Ev && Ev - base >= 4u && re - base >= 4u;
+            if (__all(whole)) {
+                // every lane's four entries lie inside the read range and one row
+                const unsigned int d = base - rs;
+                unsigned int cs[4], bd[4];
+#pragma unroll
+                for (unsigned int e = 0; e < 4u; ++e) {
+                    cs[e] = min(col[e], cmax);
+                    bd[e] = cs[e] / TR_BAND;
+                    atomicAdd(&hist[cs[e] >> 1], 1u << ((cs[e] & 1u) << 4));
+                }
+                top = max(top, max(max(col[0], col[1]), max(col[2], col[3])));
+                unsorted |= (d != 0u && first_prev >= col[0]) | (col[0] >= col[1]) | (col[1] >= col[2]) | (col[2] >= col[3]);
+                const bool open0 = d == 0u || bd[0] != min(first_prev, cmax) / TR_BAND;
+                off[open0 ? bd[0] * TR_ROWS + row : junk] = (unsigned short) d;
+#pragma unroll
+                for (unsigned int e = 1; e < 4u; ++e)
+                    off[bd[e] != bd[e - 1u] ? bd[e] * TR_ROWS + row : junk] = (unsigned short) (d + e);
+                continue;
+            }
 #pragma unroll
             for (unsigned int e = 0; e < 4u; ++e) {
                 i = base + e;
@@ -268,6 +300,7 @@ __global__ void __launch_bounds__(TR_COUNT_THREADS) tr_count_packed_kernel(unsig
                 visit(i, col[e], e == 0u ? first_prev : col[e == 0u ? 0u : e - 1u], row, rs, bad, unsorted);
             }
         }
+        bad |= top > cmax;
         if (bad) flags[1] = 1u;
         if (unsorted) flags[0] = 1u;
     };
@@ -321,7 +354,7 @@ __global__ void __launch_bounds__(TR_COUNT_THREADS) tr_count_packed_kernel(unsig
 }
 
 size_t count_packed_lds_bytes(unsigned int uDim, unsigned int bands) {
-    return (size_t) ((uDim + 1u) >> 1) * 4u + (size_t) (TR_ROWS + 4u) * 4u + (size_t) bands * TR_ROWS * 2u;
+    return (size_t) ((uDim + 1u) >> 1) * 4u + (size_t) (TR_ROWS + 4u) * 4u + (size_t) bands * TR_ROWS * 2u + 64u * 4u;
 }
 
 // K2: counts[t][c] <- sum_{t' < t} counts[t'][c] (in place), coltot[c] = total.
@@ -678,7 +711,7 @@ int transpose_impl(unsigned int cDim, unsigned int uDim, unsigned int nnz, const
     const char *uenv = std::getenv("CSAMD_TR_U");
     const unsigned int uw = uenv != nullptr ? (unsigned int) std::atoi(uenv) : 8u;
     const size_t packed_lds = count_packed_lds_bytes(uDim, L.bands);
-    const bool packed = L.band_table && uDim < 65536u && nnz >= 4u && nnz < 0xffff0000u && packed_lds <= (size_t) max_lds &&
+    const bool packed = L.band_table && uDim > 0u && uDim < 65536u && nnz >= 4u && nnz < 0xffff0000u && packed_lds <= (size_t) max_lds &&
                         ((uintptr_t) d_idx & 15u) == 0u;  // 16-byte group loads
     // counts (and band starts) -> per-tile running offsets -> out_ptr
     auto count_pass = [&](bool use_packed, bool with_rbt) -> int {
