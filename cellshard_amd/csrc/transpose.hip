@@ -597,13 +597,32 @@ __global__ void __launch_bounds__(TR_SORT_THREADS) tr_band_kernel(unsigned int c
         }
         __syncthreads();
         // D: runs to their final places (consecutive staging slots of one run
-        // are consecutive output positions)
-        for (unsigned int k = tid; k < n; k += T) {
-            const RT e = s_out[k];
-            const unsigned int key = R::key(e);
-            const unsigned int pos = delta[(key >> 8) & 255u] + k;
-            out_idx[pos] = r0 + rb0 + (key & 255u);
-            out_val[pos] = R::val(e);
+        // are consecutive output positions).  DU slots per thread per step:
+        // all their staged records are read, then all their segments'
+        // deltas, then the stores, so a step waits for two LDS round trips
+        // rather than two per slot (slots past n read slot n - 1, unused).
+        constexpr unsigned int DU = 8u;  // the asm below names eight
+        for (unsigned int k0 = 0u; k0 * T < n; k0 += DU) {  // uniform
+            RT e[DU];
+            unsigned int d[DU];
+#pragma unroll
+            for (unsigned int u = 0; u < DU; ++u) {
+                const unsigned int k = tid + (k0 + u) * T;
+                e[u] = s_out[k < n ? k : n - 1u];
+            }
+#pragma unroll
+            for (unsigned int u = 0; u < DU; ++u) d[u] = delta[(R::key(e[u]) >> 8) & 255u];
+            // all eight read here, not sunk one by one into the guarded stores
+            asm volatile("" : "+v"(d[0]), "+v"(d[1]), "+v"(d[2]), "+v"(d[3]), "+v"(d[4]), "+v"(d[5]), "+v"(d[6]), "+v"(d[7]));
+#pragma unroll
+            for (unsigned int u = 0; u < DU; ++u) {
+                const unsigned int k = tid + (k0 + u) * T;
+                if (k >= n) continue;
+                const unsigned int key = R::key(e[u]);
+                const unsigned int pos = d[u] + k;
+                out_idx[pos] = r0 + rb0 + (key & 255u);
+                out_val[pos] = R::val(e[u]);
+            }
         }
         // E: advance the cursors, clear the masks.  No barrier before: D reads
         // neither (gcur was folded into delta in B, the masks were last read
