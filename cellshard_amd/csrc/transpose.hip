@@ -363,12 +363,15 @@ __global__ void tr_tile_offsets_kernel(unsigned int *__restrict__ counts, unsign
     for (unsigned int c = blockIdx.x * blockDim.x + threadIdx.x; c < uDim; c += gridDim.x * blockDim.x) {
         unsigned int run = 0u;
         unsigned int t = 0u;
-        for (; t + 8u <= tiles; t += 8u) {
-            unsigned int n[8];
+        // one thread per segment walks every tile: 32 loads in flight per
+        // step (the grid is only uDim threads, so each needs many)
+        constexpr unsigned int TU = 32u;
+        for (; t + TU <= tiles; t += TU) {
+            unsigned int n[TU];
 #pragma unroll
-            for (unsigned int k = 0; k < 8u; ++k) n[k] = counts[(size_t) (t + k) * uDim + c];
+            for (unsigned int k = 0; k < TU; ++k) n[k] = counts[(size_t) (t + k) * uDim + c];
 #pragma unroll
-            for (unsigned int k = 0; k < 8u; ++k) {
+            for (unsigned int k = 0; k < TU; ++k) {
                 counts[(size_t) (t + k) * uDim + c] = run;
                 run += n[k];
             }
