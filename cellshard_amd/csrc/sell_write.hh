@@ -70,7 +70,11 @@ __device__ __forceinline__ void sell_write_row(const CSAMD_GLOBAL unsigned int *
     // n are replaced by padding when stored)
     constexpr unsigned int QU = 4u;
     const unsigned int last = n != 0u ? n - 1u : 0u;
-    for (unsigned int q0 = 0u; q0 < nq; q0 += 64u * QU) {
+    // quads [0, qe) hold at least one entry; steps wholly past them store
+    // padding without loading (the second loop)
+    const unsigned int qe = n > h ? min(nq, (n - h + 3u) >> 2) : 0u;
+    unsigned int q0 = 0u;
+    for (; q0 < qe; q0 += 64u * QU) {
         unsigned int ci[QU][4];
         S vi[QU][4];
         if (n != 0u) {
@@ -96,6 +100,18 @@ __device__ __forceinline__ void sell_write_row(const CSAMD_GLOBAL unsigned int *
                 c[t] = in ? ci[u][t] : CSAMD_INVALID_COL;
                 v[t] = in ? vi[u][t] : S(0);
             }
+            *(CSAMD_GLOBAL u32x4 *) (oc + base + j) = c;
+            sell_quad<S>::store(ov + base + j, v);
+        }
+    }
+    for (; q0 < nq; q0 += 64u * QU) {
+        const u32x4 c = {CSAMD_INVALID_COL, CSAMD_INVALID_COL, CSAMD_INVALID_COL, CSAMD_INVALID_COL};
+        const S v[4] = {S(0), S(0), S(0), S(0)};
+#pragma unroll
+        for (unsigned int u = 0; u < QU; ++u) {
+            const unsigned int q = q0 + 64u * u + lane;
+            if (q >= nq) continue;
+            const unsigned int j = h + 4u * q;
             *(CSAMD_GLOBAL u32x4 *) (oc + base + j) = c;
             sell_quad<S>::store(ov + base + j, v);
         }
