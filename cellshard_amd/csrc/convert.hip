@@ -539,10 +539,7 @@ __global__ void sell_emit_bell_kernel(csamd_blocked_ell_view src, csamd_sliced_e
             written += (unsigned int) __popcll(bal);
         }
         }
-        for (unsigned int k = written + lane_id(); k < width; k += 64u) {
-            dst.col_idx[base + k] = CSAMD_INVALID_COL;
-            dval[base + k] = (S) 0;
-        }
+        sell_pad_row<S>(gmem(dst.col_idx), gmem(dval), base, written, width, sell_quads_ok<S>(dst.col_idx, dval));
     }
 }
 

@@ -120,4 +120,36 @@ __device__ __forceinline__ void sell_write_row(const CSAMD_GLOBAL unsigned int *
     if (lane < width - t0) one(t0 + lane);
 }
 
+// One wave pads slots [base + from, base + width) of a Sliced-ELL row
+// (CSAMD_INVALID_COL / 0): quads from the first 4-aligned slot when the
+// arrays allow them (`quads`, as sell_quads_ok), single slots otherwise
+// and at both ends.
+template<typename S>
+__device__ __forceinline__ void sell_pad_row(CSAMD_GLOBAL unsigned int *oc, CSAMD_GLOBAL S *ov, unsigned long long base,
+                                             unsigned int from, unsigned int width, bool quads) {
+    const unsigned int lane = lane_id();
+    if (from >= width) return;
+    auto one = [&](unsigned int j) {
+        oc[base + j] = CSAMD_INVALID_COL;
+        ov[base + j] = S(0);
+    };
+    if (!quads) {
+        for (unsigned int j = from + lane; j < width; j += 64u) one(j);
+        return;
+    }
+    const unsigned int head = (unsigned int) ((4ull - ((base + from) & 3ull)) & 3ull);
+    const unsigned int h = head < width - from ? head : width - from;
+    if (lane < h) one(from + lane);
+    const unsigned int j0 = from + h;  // base + j0 is a multiple of 4
+    const unsigned int nq = (width - j0) >> 2;
+    const u32x4 c = {CSAMD_INVALID_COL, CSAMD_INVALID_COL, CSAMD_INVALID_COL, CSAMD_INVALID_COL};
+    const S v[4] = {S(0), S(0), S(0), S(0)};
+    for (unsigned int q = lane; q < nq; q += 64u) {
+        *(CSAMD_GLOBAL u32x4 *) (oc + base + j0 + 4u * q) = c;
+        sell_quad<S>::store(ov + base + j0 + 4u * q, v);
+    }
+    const unsigned int t0 = j0 + 4u * nq;
+    if (lane < width - t0) one(t0 + lane);
+}
+
 } // namespace csamd
