@@ -913,11 +913,14 @@ __global__ void __launch_bounds__(256) label_finish_kernel(
             }
             unsigned int r = l;  // one past the last w with bounds[w] < ge
             while (r < W && bounds[r] < ge) ++r;
-            for (unsigned int w0 = l; w0 < r; w0 += 4u) {
-                double ps[4], pq[4];
-                unsigned int pc[4];
+            // FU workgroups' partials loaded per step (a group spans about W / G
+            // workgroups, so most groups take one or two steps), then summed in w order
+            constexpr unsigned int FU = 16u;
+            for (unsigned int w0 = l; w0 < r; w0 += FU) {
+                double ps[FU], pq[FU];
+                unsigned int pc[FU];
 #pragma unroll
-                for (unsigned int k = 0; k < 4u; ++k) {  // loads first, then sums in w order
+                for (unsigned int k = 0; k < FU; ++k) {  // loads first, then sums in w order
                     const unsigned int w = w0 + k < r ? w0 + k : l;  // clamped: never reads past the slots
                     const bool used = w0 + k < r && bounds[w] < bounds[w + 1u];
                     const unsigned long long slot = fbase + (unsigned long long) (w + g) * ftile;
@@ -926,7 +929,7 @@ __global__ void __launch_bounds__(256) label_finish_kernel(
                     pc[k] = used ? pcnt[slot] : 0u;
                 }
 #pragma unroll
-                for (unsigned int k = 0; k < 4u; ++k) {
+                for (unsigned int k = 0; k < FU; ++k) {
                     sm += ps[k];
                     q += pq[k];
                     n += pc[k];
